@@ -1,0 +1,178 @@
+/*
+ * voxmi.h -- C ABI of the MI355X voxelizer + GigaVoxels brick writer.
+ *
+ * Drop-in for the reference class cuda::Voxelizer
+ * (apecoraro/VoxVizThesis VoxViz/Voxelizer/Voxelizer/Voxelizer.h 22-244),
+ * which main.cu Voxelize() (3225-3341) drives as:
+ *
+ *   ctor -> initCuda -> allocateVoxelMemory -> setVoxelizationParams ->
+ *   addTriangleGroup* -> allocateTriangleMemory ->
+ *   computeEdgesFaceNormalsAndBounds -> generateVoxelsAndOctTree ->
+ *   deallocateTriangleMemory  (resetCuda on failure)
+ *
+ * Each entry point below names the reference member it replaces.  Plain
+ * pointers and sizes only; one handle per voxelizer; no global state.
+ * Boolean members of the reference return 1 (true) / 0 (false) here;
+ * vox_generate keeps the reference's 1 / 0 / -1 convention.  Error text
+ * accumulates in the handle and is read with vox_get_error, as
+ * Voxelizer::getErrorMessage (Voxelizer.cu 3176-3187) does.
+ *
+ * Semantics: the reference run with one chunk covering the whole voxel space
+ * S = N + 4*((N/B)>>1) per axis.  A chunk smaller than S is rejected (the
+ * multi-chunk seam behaviour is not reproduced).
+ */
+#ifndef VOXMI_H
+#define VOXMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vox_ctx vox_ctx;
+
+/* Texture address modes, numbered as cudaTextureAddressMode
+ * (main.cu OpenGLModeToCudaMode 3207-3223). */
+enum { VOX_ADDR_WRAP = 0, VOX_ADDR_CLAMP = 1, VOX_ADDR_MIRROR = 2, VOX_ADDR_BORDER = 3 };
+
+/* Options (vox_set_options). */
+enum {
+    /* Build mip level d from level d-1.  Default (flag clear) reproduces the
+     * reference's computeMipMaps recursion, where level d >= 2 reads level
+     * d-2 (Voxelizer.cu 513-517). */
+    VOX_OPT_MIP_CHAIN = 1u << 0,
+    /* Reserved: every mip level's sparse tiles are kept until the next
+     * vox_generate, so vox_copy_level always works. */
+    VOX_OPT_KEEP_LEVELS = 1u << 1,
+    /* Skip the dense S^3 occupancy bitmap (it is produced by default). */
+    VOX_OPT_NO_OCCUPANCY = 1u << 2,
+    /* Record per-kernel HIP-event timings for vox_get_stats. */
+    VOX_OPT_TIMING = 1u << 3
+};
+
+/* One triangle group: the arguments of cuda::Voxelizer::addTriangleGroup
+ * (Voxelizer.h 128-162).  Triangle soup, 3 vertices per triangle.
+ * vnormals, uvs, rgba8 may be NULL.  A group is textured when both uvs and
+ * rgba8 are given (ComputeColorsAndNormals); otherwise it is coloured by
+ * its normals (ComputeColorsAsNormals). */
+typedef struct vox_group {
+    const float*   verts;      /* 9*ntri floats                               */
+    const float*   vnormals;   /* 9*ntri floats or NULL                        */
+    const float*   uvs;        /* 6*ntri floats or NULL                        */
+    uint64_t       ntri;
+    float          bbox_min[3];/* group bounds, world units                    */
+    float          bbox_max[3];
+    int32_t        is_terrain; /* fill up to 19 voxels below textured hits     */
+    const uint8_t* rgba8;      /* width*height*4 bytes or NULL                 */
+    int32_t        width, height;
+    int32_t        addr0, addr1;
+} vox_group;
+
+/* Device-resident result of the last vox_generate.  All pointers are HIP
+ * device pointers owned by the handle; valid until the next vox_generate,
+ * vox_reset or vox_destroy. */
+#define VOX_MAX_LEVELS 16
+typedef struct vox_result {
+    int32_t  levels;                 /* octree depth count L                   */
+    uint32_t S[3];                   /* voxel space per axis                   */
+    uint32_t brick_dim[3];           /* B                                      */
+    const uint32_t* occupancy;       /* S^3 bits, bit i = voxel (z*Sy+y)*Sx+x  */
+    uint64_t occupancy_words;
+    /* per tree depth d (0 = root): nodes = 8^d, x fastest, then y, z */
+    uint64_t        nodes[VOX_MAX_LEVELS];
+    const uint8_t*  types[VOX_MAX_LEVELS];       /* 0 CONST,1 NON-CONST,2 LEAF-CONST */
+    const uint32_t* const_colors[VOX_MAX_LEVELS];/* RGBA8, R in the low byte    */
+    const uint32_t* brick_index[VOX_MAX_LEVELS]; /* 0xffffffff when no brick    */
+    uint64_t        bricks[VOX_MAX_LEVELS];      /* NON-CONST nodes at depth d  */
+    const uint32_t* brick_colors[VOX_MAX_LEVELS];/* bricks*(B+2)^3 RGBA8        */
+    const uint32_t* brick_normals[VOX_MAX_LEVELS];/* bricks*(B+2)^3 RGB8 (+0)   */
+    const uint32_t* brick_node[VOX_MAX_LEVELS];  /* node index of each brick    */
+} vox_result;
+
+/* Work counts and timings of the last vox_generate. */
+typedef struct vox_stats {
+    uint64_t triangles;               /* T                                      */
+    uint64_t pairs;                   /* (triangle, tile) bin entries           */
+    uint64_t sat_tests;               /* (triangle, voxel) overlap tests        */
+    uint64_t active_tiles[VOX_MAX_LEVELS]; /* 8^3 voxel tiles per mip level    */
+    uint64_t occupied_voxels;
+    uint64_t bricks_total;
+    double   ms_total;                /* HIP-event time of the device pipeline  */
+    int32_t  nkernels;
+    char     kernel_name[64][32];
+    double   kernel_ms[64];           /* summed per kernel name                */
+    uint32_t kernel_launches[64];
+} vox_stats;
+
+/* cuda::Voxelizer::Voxelizer(voxDim, brickDim, voxChunkDim)
+ * (Voxelizer.h 122-124, Voxelizer.cu 125-153).  Always returns a handle
+ * (NULL only when out of host memory); invalid dimensions (N/B not a power
+ * of two >= 2, unequal ratios, chunk < S) are reported by vox_get_error and
+ * make every later call fail.  chunk_dim may be NULL: one chunk covering the
+ * voxel space (the reference's default of 256 only covers it for N <= 128
+ * with B = 8, or similar). */
+vox_ctx* vox_create(const uint32_t vox_dim[3], const uint32_t brick_dim[3],
+                    const uint32_t chunk_dim[3]);
+/* cuda::Voxelizer::~Voxelizer (Voxelizer.cu 272-277). */
+void vox_destroy(vox_ctx* h);
+
+/* cuda::Voxelizer::setVoxelizationParams (Voxelizer.h 164, Voxelizer.cu 155-161). */
+int vox_set_params(vox_ctx* h, const float p[3], const float delta[3]);
+/* Options bit set (VOX_OPT_*); not in the reference. */
+int vox_set_options(vox_ctx* h, uint32_t options);
+
+/* cuda::Voxelizer::addTriangleGroup (Voxelizer.h 128-162).  Host pointers;
+ * like the reference, they are only read by vox_allocate_triangle_memory. */
+int vox_add_group(vox_ctx* h, const vox_group* g);
+/* Same, but every pointer in *g is a device pointer already resident in HBM
+ * (no copy is made; the caller keeps them alive until vox_generate returns). */
+int vox_add_group_device(vox_ctx* h, const vox_group* g);
+
+/* cuda::Voxelizer::initCuda (Voxelizer.cu 297-322): device 0, own stream. */
+int vox_init(vox_ctx* h);
+/* cuda::Voxelizer::resetCuda (Voxelizer.cu 324-344): frees everything,
+ * resets the device, reallocates voxel memory. */
+int vox_reset(vox_ctx* h);
+/* cuda::Voxelizer::allocateVoxelMemory / deallocateVoxelMemory
+ * (Voxelizer.cu 163-236). */
+int  vox_allocate_voxel_memory(vox_ctx* h);
+void vox_deallocate_voxel_memory(vox_ctx* h);
+/* cuda::Voxelizer::allocateTriangleMemory / deallocateTriangleMemory
+ * (Voxelizer.cu 346-457, 238-242): H2D of host groups. */
+int  vox_allocate_triangle_memory(vox_ctx* h);
+void vox_deallocate_triangle_memory(vox_ctx* h);
+/* cuda::Voxelizer::computeEdgesFaceNormalsAndBounds (Voxelizer.cu 459-489):
+ * per-triangle setup and binning of triangles to 8^3 voxel tiles. */
+int vox_compute_edges_face_normals_and_bounds(vox_ctx* h);
+/* cuda::Voxelizer::generateVoxelsAndOctTree (Voxelizer.cu 2180-2814).
+ * Returns 1 ok, 0 error, -1 no group overlaps the grid.  out_dir may be NULL
+ * (results stay in HBM, see vox_get_result); otherwise tree.gvx and
+ * voxels_<depth>.gvb are written there.  compressed must be 0 (the DXT path
+ * relied on the OpenGL driver, VoxelBrickWriter.cu 590-723). */
+int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed);
+/* cuda::Voxelizer::getErrorMessage (Voxelizer.cu 3176-3187). */
+const char* vox_get_error(vox_ctx* h);
+
+/* In-memory result of the last vox_generate (device pointers). */
+int vox_get_result(vox_ctx* h, vox_result* out);
+/* Work counts and timings of the last vox_generate. */
+int vox_get_stats(vox_ctx* h, vox_stats* out);
+/* Copy n bytes from a device pointer of the result to host memory. */
+int vox_copy_to_host(vox_ctx* h, void* dst, const void* src_device, uint64_t nbytes);
+/* Materialise mip level k (0 = full resolution) densely on the host:
+ * colors (S_k^3 RGBA8) and normals (3*S_k^3 floats), x fastest.
+ * dims receives S_k. */
+int vox_copy_level(vox_ctx* h, int32_t level, uint32_t* colors, float* normals,
+                   uint32_t dims[3]);
+/* Write tree.gvx and voxels_<depth>.gvb of the last vox_generate into
+ * out_dir (host-side writer, Voxelizer.cu 2816-3061, VoxelBrickWriter.cu
+ * 725-818). */
+int vox_write_files(vox_ctx* h, const char* out_dir);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,60 @@
+"""The C-ABI library loads and exports every symbol include/voxmi.h declares
+(CPU only: no compute call is made)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from voxvizthesis_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "voxmi.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(vox_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_matches_binding_list():
+    assert declared_symbols() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (vox_[a-z0-9_]+)", out))
+    assert set(declared_symbols()) <= exported
+
+
+def test_library_carries_gfx950_code_object():
+    # the offload bundle id names the target of the embedded code object
+    blob = open(_lib.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_create_validates_dimensions_without_gpu():
+    lib = _lib.load()
+    u3 = ctypes.c_uint32 * 3
+    h = lib.vox_create(u3(48, 48, 48), u3(8, 8, 8), None)   # 48/8 = 6, not a power of 2
+    try:
+        assert b"power of two" in lib.vox_get_error(h)
+        assert lib.vox_set_params(h, (ctypes.c_float * 3)(0, 0, 0),
+                                  (ctypes.c_float * 3)(1, 1, 1)) == 0
+    finally:
+        lib.vox_destroy(h)
+    h = lib.vox_create(u3(1024, 1024, 1024), u3(16, 16, 16), u3(256, 256, 256))
+    try:
+        assert b"chunk dimension" in lib.vox_get_error(h)
+    finally:
+        lib.vox_destroy(h)
+    h = lib.vox_create(u3(128, 128, 128), u3(8, 8, 8), u3(256, 256, 256))
+    try:
+        assert lib.vox_get_error(h) == b""
+    finally:
+        lib.vox_destroy(h)

@@ -1,0 +1,53 @@
+"""HIP path against the oracle, bit-exact (GPU)."""
+import numpy as np
+import pytest
+
+import oracle
+from tests._compare import compare_results
+from voxvizthesis_amd import scenes
+from voxvizthesis_amd._lib import VOX_OPT_MIP_CHAIN
+from voxvizthesis_amd.voxelizer import voxelize_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(scene, chain=False, levels=True):
+    print(f"[parity] {scene.name} chain={chain} T={scene.ntri}", flush=True)
+    ref = oracle.run(scene, flags=oracle.REF_MIP_CHAIN if chain else 0, keep_mips=levels)
+    vz = voxelize_scene(scene, options=VOX_OPT_MIP_CHAIN if chain else 0)
+    try:
+        assert vz.status == ref.status
+        res = vz.result()
+        gl = [vz.level(k) for k in range(ref.levels)] if levels else None
+        errs = compare_results(res, ref, ref_mips=ref.mips, gpu_levels=gl)
+        assert not errs, "\n".join(errs[:20])
+    finally:
+        vz.close()
+    return ref
+
+
+def test_two_triangles():
+    ref = _check(scenes.scene_two_triangles())
+    assert int(np.unpackbits(ref.occupancy.view(np.uint8)).sum()) > 0
+
+
+def test_small_terrain():
+    _check(scenes.scene_small_terrain())
+
+
+def test_small_terrain_mip_chain():
+    _check(scenes.scene_small_terrain(), chain=True)
+
+
+def test_mixed_groups():
+    _check(scenes.scene_mixed())
+
+
+def test_mixed_groups_b4():
+    _check(scenes.scene_mixed(seed=9, N=64, B=4))
+
+
+def test_type_x_split_miss():
+    # N=256, B=4: the leaf typing launch split leaves x = N + extra (the +1
+    # border of the last node column) unexamined.
+    _check(scenes.scene_terrain_n(128, 256, 4), levels=False)

@@ -1,0 +1,103 @@
+"""ctypes binding of libvoxmi.so (include/voxmi.h).
+
+The HIP library is the product path: importing this module on a machine
+without the built library raises immediately, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvoxmi.so")
+
+VOX_MAX_LEVELS = 16
+VOX_OPT_MIP_CHAIN = 1 << 0
+VOX_OPT_KEEP_LEVELS = 1 << 1
+VOX_OPT_NO_OCCUPANCY = 1 << 2
+VOX_OPT_TIMING = 1 << 3
+
+# Every symbol include/voxmi.h declares.
+EXPORTS = [
+    "vox_create", "vox_destroy", "vox_set_params", "vox_set_options", "vox_add_group",
+    "vox_add_group_device", "vox_init", "vox_reset", "vox_allocate_voxel_memory",
+    "vox_deallocate_voxel_memory", "vox_allocate_triangle_memory",
+    "vox_deallocate_triangle_memory", "vox_compute_edges_face_normals_and_bounds",
+    "vox_generate", "vox_get_error", "vox_get_result", "vox_get_stats", "vox_copy_to_host",
+    "vox_copy_level", "vox_write_files",
+]
+
+
+class VoxGroup(C.Structure):
+    _fields_ = [
+        ("verts", C.c_void_p), ("vnormals", C.c_void_p), ("uvs", C.c_void_p),
+        ("ntri", C.c_uint64), ("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3),
+        ("is_terrain", C.c_int32), ("rgba8", C.c_void_p), ("width", C.c_int32),
+        ("height", C.c_int32), ("addr0", C.c_int32), ("addr1", C.c_int32),
+    ]
+
+
+class VoxResult(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int32), ("S", C.c_uint32 * 3), ("brick_dim", C.c_uint32 * 3),
+        ("occupancy", C.c_void_p), ("occupancy_words", C.c_uint64),
+        ("nodes", C.c_uint64 * VOX_MAX_LEVELS),
+        ("types", C.c_void_p * VOX_MAX_LEVELS),
+        ("const_colors", C.c_void_p * VOX_MAX_LEVELS),
+        ("brick_index", C.c_void_p * VOX_MAX_LEVELS),
+        ("bricks", C.c_uint64 * VOX_MAX_LEVELS),
+        ("brick_colors", C.c_void_p * VOX_MAX_LEVELS),
+        ("brick_normals", C.c_void_p * VOX_MAX_LEVELS),
+        ("brick_node", C.c_void_p * VOX_MAX_LEVELS),
+    ]
+
+
+class VoxStats(C.Structure):
+    _fields_ = [
+        ("triangles", C.c_uint64), ("pairs", C.c_uint64), ("sat_tests", C.c_uint64),
+        ("active_tiles", C.c_uint64 * VOX_MAX_LEVELS), ("occupied_voxels", C.c_uint64),
+        ("bricks_total", C.c_uint64), ("ms_total", C.c_double), ("nkernels", C.c_int32),
+        ("kernel_name", (C.c_char * 32) * 64), ("kernel_ms", C.c_double * 64),
+        ("kernel_launches", C.c_uint32 * 64),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Load libvoxmi.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+            f"g.build()'` (hipcc, gfx950). There is no CPU fallback.")
+    L = C.CDLL(LIB_PATH)
+    u3 = C.POINTER(C.c_uint32)
+    f3 = C.POINTER(C.c_float)
+    vp = C.c_void_p
+    L.vox_create.restype = vp
+    L.vox_create.argtypes = [u3, u3, u3]
+    L.vox_destroy.argtypes = [vp]
+    L.vox_set_params.argtypes = [vp, f3, f3]
+    L.vox_set_options.argtypes = [vp, C.c_uint32]
+    L.vox_add_group.argtypes = [vp, C.POINTER(VoxGroup)]
+    L.vox_add_group_device.argtypes = [vp, C.POINTER(VoxGroup)]
+    for n in ("vox_init", "vox_reset", "vox_allocate_voxel_memory",
+              "vox_allocate_triangle_memory", "vox_compute_edges_face_normals_and_bounds"):
+        getattr(L, n).argtypes = [vp]
+    for n in ("vox_deallocate_voxel_memory", "vox_deallocate_triangle_memory"):
+        getattr(L, n).argtypes = [vp]
+        getattr(L, n).restype = None
+    L.vox_generate.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
+    L.vox_get_error.restype = C.c_char_p
+    L.vox_get_error.argtypes = [vp]
+    L.vox_get_result.argtypes = [vp, C.POINTER(VoxResult)]
+    L.vox_get_stats.argtypes = [vp, C.POINTER(VoxStats)]
+    L.vox_copy_to_host.argtypes = [vp, vp, vp, C.c_uint64]
+    L.vox_copy_level.argtypes = [vp, C.c_int32, vp, vp, u3]
+    L.vox_write_files.argtypes = [vp, C.c_char_p]
+    _lib = L
+    return L

@@ -1,0 +1,146 @@
+// gvformat.cpp -- GigaVoxels file writers: tree.gvx and voxels_<depth>.gvb.
+//
+// tree.gvx follows Voxelizer::writeOctTree / WriteOctTreeNode
+// (Voxelizer.cu 2816-3061): depth-first XML from the root, children in z, y,
+// x loops, LEAF-CONST nodes without children, default iostream formatting.
+// voxels_<depth>.gvb follows VoxelBrickWriter::startBricksFile and
+// WriteBinaryBrick / WriteUByteBinaryBrick (VoxelBrickWriter.cu 71-124,
+// 725-818, 509-588): int32 compressed flag, then per brick in index order
+// u64 dims, u32 borders, int32 0, u32 size, RGBA8, int32 0, u32 size, RGB8.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "../../include/voxmi.h"
+
+namespace {
+
+struct HostTree {
+    int L = 0;
+    uint32_t B[3];
+    std::vector<std::vector<uint8_t>> types;
+    std::vector<std::vector<uint32_t>> cc, bidx;
+};
+
+void write_node(std::ofstream& f, const HostTree& t, int d, uint32_t x, uint32_t y, uint32_t z,
+                uint64_t xo, uint64_t yo, uint64_t zo) {
+    const uint32_t Rd = 1u << d;
+    const uint64_t i = ((uint64_t)z * Rd + y) * Rd + x;
+    const uint8_t type = t.types[d][i];
+    for (int k = 0; k <= d; ++k) f << "    ";
+    if (type == 1) {
+        f << "<Node " << "MipMapX=\"" << xo << "\" " << "MipMapY=\"" << yo << "\" "
+          << "MipMapZ=\"" << zo << "\" " << "Type=\"NON-CONST\" " << "Brick=\"" << t.bidx[d][i]
+          << "\" " << "Depth=\"" << d << "\"";
+    } else {
+        const uint32_t c = t.cc[d][i];
+        f << "<Node " << "MipMapX=\"" << xo << "\" " << "MipMapY=\"" << yo << "\" "
+          << "MipMapZ=\"" << zo << "\" " << "Type=\"" << (type == 0 ? "CONST" : "LEAF-CONST")
+          << "\" " << "ColorR=\"" << static_cast<float>(c & 255u) / 255.0 << "\" "
+          << "ColorG=\"" << static_cast<float>((c >> 8) & 255u) / 255.0 << "\" "
+          << "ColorB=\"" << static_cast<float>((c >> 16) & 255u) / 255.0 << "\" "
+          << "ColorA=\"" << static_cast<float>(c >> 24) / 255.0 << "\" " << "Depth=\"" << d
+          << "\"";
+    }
+    if (d + 1 < t.L && type != 2) {
+        f << ">" << std::endl;
+        for (uint32_t cz = 0; cz < 2; ++cz)
+            for (uint32_t cy = 0; cy < 2; ++cy)
+                for (uint32_t cx = 0; cx < 2; ++cx)
+                    write_node(f, t, d + 1, 2 * x + cx, 2 * y + cy, 2 * z + cz,
+                               (xo << 1) + cx * t.B[0], (yo << 1) + cy * t.B[1],
+                               (zo << 1) + cz * t.B[2]);
+        for (int k = 0; k <= d; ++k) f << "    ";
+        f << "</Node>" << std::endl;
+    } else {
+        f << " />" << std::endl;
+    }
+}
+
+}  // namespace
+
+// Declared in voxmi.h; defined here with access to the result through the
+// public C ABI only.
+extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3],
+                                    const float delta[3], const uint32_t N[3]);
+
+extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3],
+                                    const float delta[3], const uint32_t N[3]) {
+    vox_result r;
+    if (!vox_get_result(h, &r)) return 0;
+    mkdir(out_dir, 0755);
+    HostTree t;
+    t.L = r.levels;
+    for (int a = 0; a < 3; ++a) t.B[a] = r.brick_dim[a];
+    t.types.resize(t.L);
+    t.cc.resize(t.L);
+    t.bidx.resize(t.L);
+    const uint64_t bvox = (uint64_t)(t.B[0] + 2) * (t.B[1] + 2) * (t.B[2] + 2);
+    for (int d = 0; d < t.L; ++d) {
+        const uint64_t n = r.nodes[d];
+        t.types[d].resize(n);
+        t.cc[d].resize(n);
+        t.bidx[d].resize(n);
+        if (!vox_copy_to_host(h, t.types[d].data(), r.types[d], n) ||
+            !vox_copy_to_host(h, t.cc[d].data(), r.const_colors[d], n * 4) ||
+            !vox_copy_to_host(h, t.bidx[d].data(), r.brick_index[d], n * 4))
+            return 0;
+        // voxels_<depth>.gvb
+        std::string fn = std::string(out_dir) + "/voxels_" + std::to_string(d) + ".gvb";
+        std::ofstream f(fn, std::ios_base::out | std::ios_base::binary);
+        if (!f.is_open()) return 0;
+        int32_t compressed = 0;
+        f.write((const char*)&compressed, 4);
+        const uint64_t nb = r.bricks[d];
+        const uint64_t chunk = 4096;
+        std::vector<uint32_t> col, nrm;
+        std::vector<uint8_t> rgb(bvox * 3);
+        for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+            const uint64_t m = std::min<uint64_t>(chunk, nb - b0);
+            col.resize(m * bvox);
+            nrm.resize(m * bvox);
+            if (!vox_copy_to_host(h, col.data(), r.brick_colors[d] + b0 * bvox, m * bvox * 4) ||
+                !vox_copy_to_host(h, nrm.data(), r.brick_normals[d] + b0 * bvox, m * bvox * 4))
+                return 0;
+            for (uint64_t j = 0; j < m; ++j) {
+                const uint64_t dims[3] = {t.B[0] + 2, t.B[1] + 2, t.B[2] + 2};
+                const uint32_t border[3] = {1, 1, 1};
+                f.write((const char*)dims, sizeof(dims));
+                f.write((const char*)border, sizeof(border));
+                int32_t zero = 0;
+                uint32_t sz = (uint32_t)(bvox * 4);
+                f.write((const char*)&zero, 4);
+                f.write((const char*)&sz, 4);
+                f.write((const char*)&col[j * bvox], bvox * 4);
+                for (uint64_t v = 0; v < bvox; ++v) {
+                    const uint32_t q = nrm[j * bvox + v];
+                    rgb[3 * v] = (uint8_t)(q & 255u);
+                    rgb[3 * v + 1] = (uint8_t)((q >> 8) & 255u);
+                    rgb[3 * v + 2] = (uint8_t)((q >> 16) & 255u);
+                }
+                sz = (uint32_t)(bvox * 3);
+                f.write((const char*)&zero, 4);
+                f.write((const char*)&sz, 4);
+                f.write((const char*)rgb.data(), bvox * 3);
+            }
+        }
+        if (f.fail()) return 0;
+    }
+    std::ofstream tf(std::string(out_dir) + "/tree.gvx", std::fstream::out);
+    if (!tf.is_open()) return 0;
+    tf << "<GigaVoxelsOctTree " << "MaxDepth=\"" << t.L << "\" " << "X=\"" << p[0] << "\" "
+       << "Y=\"" << p[1] << "\" " << "Z=\"" << p[2] << "\" " << "DeltaX=\"" << delta[0] << "\" "
+       << "DeltaY=\"" << delta[1] << "\" " << "DeltaZ=\"" << delta[2] << "\" "
+       << "VolumeXSize=\"" << N[0] << "\" " << "VolumeYSize=\"" << N[1] << "\" "
+       << "VolumeZSize=\"" << N[2] << "\" " << "BrickXSize=\"" << t.B[0] << "\" "
+       << "BrickYSize=\"" << t.B[1] << "\" " << "BrickZSize=\"" << t.B[2] << "\" "
+       << "Binary=\"YES\" " << "Compressed=\"NO\" >" << std::endl;
+    write_node(tf, t, 0, 0, 0, 0, 0, 0, 0);
+    tf << "</GigaVoxelsOctTree>" << std::endl;
+    return tf.fail() ? 0 : 1;
+}

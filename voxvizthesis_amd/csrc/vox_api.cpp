@@ -1,0 +1,825 @@
+// vox_api.cpp -- host orchestration and C ABI (include/voxmi.h).
+//
+// Mirrors cuda::Voxelizer (Voxelizer.h 22-244, Voxelizer.cu): the same
+// call sequence and return conventions, but the whole pipeline runs on one
+// HIP stream with everything resident in HBM.  Host round trips are limited
+// to three small counter reads per vox_generate (bin entries, active tiles
+// per level, bricks per level) that size the next allocations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/voxmi.h"
+#include "vox_internal.h"
+
+using namespace voxmi;
+
+namespace {
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t bytes, std::stringstream& err, const char* what) {
+        if (bytes <= cap && p) return true;
+        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            err << "hipMalloc failed for " << what << " (" << b << " bytes): "
+                << hipGetErrorString(e) << std::endl;
+            p = nullptr;
+            return false;
+        }
+        cap = b;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct HostGroup {
+    vox_group desc;
+    bool device = false;
+};
+
+struct LevelBufs {
+    LevelMap map{};
+    uint64_t active = 0;
+    DevBuf mark, excl, tileIdx, list, col, nrm;
+};
+
+struct DepthBufs {
+    uint32_t Rd = 0;
+    uint64_t nodes = 0, bricks = 0;
+    DevBuf types, cc, flag, excl, bidx, brickNode, bcol, bnrm;
+};
+
+struct TimedLaunch {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct vox_ctx {
+    uint32_t N[3] = {0, 0, 0}, B[3] = {0, 0, 0}, chunk[3] = {0, 0, 0};
+    uint32_t extra[3] = {0, 0, 0}, S[3] = {0, 0, 0};
+    int L = 0;
+    bool valid = false;
+    bool inited = false;
+    bool voxMem = false;
+    bool triMem = false;
+    bool prepared = false;
+    bool haveResult = false;
+    uint32_t options = 0;
+    float p[3] = {0, 0, 0}, dP[3] = {1, 1, 1}, offP[3] = {0, 0, 0};
+    std::stringstream err;
+    std::string errCache;
+    std::vector<HostGroup> groups;
+    hipStream_t st = nullptr;
+
+    // triangle memory
+    uint64_t T = 0;
+    DevBuf verts, fnorm, bbox, vn, uv, img, grpFill;
+    std::vector<GroupArgs> gargs;
+    // binning
+    uint64_t numBins = 0, pairs = 0;
+    DevBuf binCount, binStart, binTri, scanTmp;
+    // levels
+    VoxParams P{};
+    std::vector<LevelBufs> lv;
+    DevBuf occ0, hit0, sat;
+    std::vector<DepthBufs> tree;
+    DevBuf bitmap;
+    DevBuf counters;  // small host-read counters
+    vox_stats stats{};
+    std::vector<TimedLaunch> timed;
+    std::vector<hipEvent_t> eventPool;
+    size_t eventNext = 0;
+    hipEvent_t evStart = nullptr, evStop = nullptr;
+
+    bool fail(const std::string& m) {
+        err << m << std::endl;
+        return false;
+    }
+    bool check(hipError_t e, const char* what) {
+        if (e == hipSuccess) return true;
+        err << what << " failed: " << hipGetErrorString(e) << std::endl;
+        return false;
+    }
+    hipEvent_t ev() {
+        if (eventNext == eventPool.size()) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            eventPool.push_back(e);
+        }
+        return eventPool[eventNext++];
+    }
+    // Record a named span around launches when timing is on.
+    template <class F> void run(const char* name, F&& f) {
+        if (options & VOX_OPT_TIMING) {
+            hipEvent_t a = ev(), b = ev();
+            (void)hipEventRecord(a, st);
+            f();
+            (void)hipEventRecord(b, st);
+            timed.push_back({name, a, b});
+        } else {
+            f();
+        }
+    }
+};
+
+namespace {
+
+inline float gclampf(float x, float lo, float hi) {
+    float m = x < hi ? x : hi;
+    return m > lo ? m : lo;
+}
+inline uint64_t trunc_u(float f) {
+    if (!(f == f) || f <= 0.0f) return 0;
+    if (f >= 1.8e19f) return UINT64_MAX;
+    return (uint64_t)f;
+}
+
+bool scan(vox_ctx* h, const uint32_t* in, uint32_t* out, uint64_t n) {
+    if (!h->scanTmp.ensure(scan_tmp_words(n) * 4, h->err, "scan scratch")) return false;
+    launch_scan(h->st, in, out, n, h->scanTmp.as<uint32_t>());
+    return true;
+}
+
+// Voxels of the x axis examined by ComputeOctTreeNodeType for one level: the
+// launch is split in two along x when the extent exceeds 128 voxels and the
+// halved block count is only rounded up when odd (Voxelizer.cu 1037-1120),
+// which can leave the last x block uncovered.
+uint32_t type_x_limit(uint32_t Ck, uint32_t border, uint32_t fullVoxDim, const uint32_t Ckyz[2],
+                      const uint32_t borderyz[2], const uint32_t fullyz[2]) {
+    uint32_t tpb[3];
+    const uint32_t C3[3] = {Ck, Ckyz[0], Ckyz[1]};
+    const uint32_t b3[3] = {border, borderyz[0], borderyz[1]};
+    const uint32_t f3[3] = {fullVoxDim, fullyz[0], fullyz[1]};
+    for (int a = 0; a < 3; ++a) {
+        tpb[a] = C3[a];
+        uint32_t fvo = tpb[a] + 0u - b3[a];
+        if (fvo > f3[a]) tpb[a] = f3[a] - (0u - b3[a]) + 2u;
+    }
+    uint32_t th = tpb[0], nb = 1;
+    if (tpb[0] > 8 || tpb[1] > 8 || tpb[2] > 8) {
+        th = std::min(tpb[0], 8u);
+        nb = (uint32_t)std::ceil((float)tpb[0] / (float)th);
+    }
+    uint32_t nexec = 1;
+    if (tpb[0] > 128u) {
+        nb >>= 1;
+        if ((nb & 1u) != 0u) nb += 1;
+        nexec = 2;
+    }
+    return std::min(nexec * nb * th, Ck);
+}
+
+void collect_timings(vox_ctx* h) {
+    vox_stats& s = h->stats;
+    s.nkernels = 0;
+    std::map<std::string, int> slot;
+    for (auto& t : h->timed) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
+        auto it = slot.find(t.name);
+        int i;
+        if (it == slot.end()) {
+            if (s.nkernels >= 64) continue;
+            i = s.nkernels++;
+            slot[t.name] = i;
+            std::snprintf(s.kernel_name[i], sizeof(s.kernel_name[i]), "%s", t.name.c_str());
+            s.kernel_ms[i] = 0.0;
+            s.kernel_launches[i] = 0;
+        } else {
+            i = it->second;
+        }
+        s.kernel_ms[i] += ms;
+        s.kernel_launches[i] += 1;
+    }
+    h->timed.clear();
+    h->eventNext = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+vox_ctx* vox_create(const uint32_t vox_dim[3], const uint32_t brick_dim[3],
+                    const uint32_t chunk_dim[3]) {
+    vox_ctx* h = new (std::nothrow) vox_ctx();
+    if (!h) return nullptr;
+    int L = 0;
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+        h->N[a] = vox_dim[a];
+        h->B[a] = brick_dim[a];
+        if (brick_dim[a] < 2 || vox_dim[a] % brick_dim[a] != 0) {
+            h->err << "voxel dimension must be a multiple of a brick dimension >= 2" << std::endl;
+            ok = false;
+            break;
+        }
+        uint32_t r = vox_dim[a] / brick_dim[a];
+        if (r < 2 || (r & (r - 1)) != 0) {
+            // The reference loops forever here (Voxelizer.cu 146-152).
+            h->err << "voxel / brick ratio must be a power of two >= 2" << std::endl;
+            ok = false;
+            break;
+        }
+        int l = 1;
+        while ((brick_dim[a] << (l - 1)) != vox_dim[a]) ++l;
+        if (a > 0 && l != L) {
+            h->err << "voxel / brick ratio must be equal on all axes" << std::endl;
+            ok = false;
+            break;
+        }
+        L = l;
+        h->extra[a] = r >> 1;
+        h->S[a] = vox_dim[a] + 4 * h->extra[a];
+        h->chunk[a] = chunk_dim ? chunk_dim[a] : h->S[a];
+        if (h->S[a] > 65535u) {
+            h->err << "voxel space above 65535 per axis is not supported" << std::endl;
+            ok = false;
+            break;
+        }
+        if (L > VOX_MAX_LEVELS) {
+            h->err << "too many octree levels" << std::endl;
+            ok = false;
+            break;
+        }
+        if (h->chunk[a] < h->S[a]) {
+            h->err << "chunk dimension " << h->chunk[a] << " is smaller than the voxel space "
+                   << h->S[a] << " (N + 4*((N/B)>>1)); multi-chunk seam semantics are not "
+                   << "reproduced, pass a chunk that covers the voxel space" << std::endl;
+            ok = false;
+            break;
+        }
+    }
+    h->L = L;
+    h->valid = ok;
+    return h;
+}
+
+void vox_deallocate_triangle_memory(vox_ctx* h);
+void vox_deallocate_voxel_memory(vox_ctx* h);
+
+void vox_destroy(vox_ctx* h) {
+    if (!h) return;
+    vox_deallocate_triangle_memory(h);
+    vox_deallocate_voxel_memory(h);
+    for (auto e : h->eventPool) (void)hipEventDestroy(e);
+    if (h->evStart) (void)hipEventDestroy(h->evStart);
+    if (h->evStop) (void)hipEventDestroy(h->evStop);
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+}
+
+int vox_set_params(vox_ctx* h, const float p[3], const float delta[3]) {
+    if (!h || !h->valid) return 0;
+    for (int a = 0; a < 3; ++a) {
+        h->p[a] = p[a];
+        h->dP[a] = delta[a];
+        // offsetP = p - extra * delta (Voxelizer.cu 160)
+        h->offP[a] = p[a] - ((float)h->extra[a] * delta[a]);
+    }
+    return 1;
+}
+
+int vox_set_options(vox_ctx* h, uint32_t options) {
+    if (!h) return 0;
+    h->options = options;
+    return 1;
+}
+
+static int add_group(vox_ctx* h, const vox_group* g, bool device) {
+    if (!h || !h->valid || !g) return 0;
+    if (!g->verts && g->ntri > 0) return h->fail("group has no vertices"), 0;
+    if (g->rgba8 && (g->width <= 0 || g->height <= 0)) return h->fail("group image has no size"), 0;
+    if (h->T + g->ntri > 0x7fffffffull) return h->fail("more than 2^31 triangles"), 0;
+    HostGroup hg;
+    hg.desc = *g;
+    hg.device = device;
+    h->groups.push_back(hg);
+    h->T += g->ntri;
+    h->triMem = false;
+    h->prepared = false;
+    return 1;
+}
+int vox_add_group(vox_ctx* h, const vox_group* g) { return add_group(h, g, false); }
+int vox_add_group_device(vox_ctx* h, const vox_group* g) { return add_group(h, g, true); }
+
+int vox_init(vox_ctx* h) {
+    if (!h || !h->valid) return 0;
+    if (h->inited) return 1;
+    if (!h->check(hipSetDevice(0), "hipSetDevice")) return 0;
+    if (!h->check(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking), "hipStreamCreate")) return 0;
+    (void)hipEventCreate(&h->evStart);
+    (void)hipEventCreate(&h->evStop);
+    h->inited = true;
+    return 1;
+}
+
+int vox_allocate_voxel_memory(vox_ctx* h) {
+    if (!h || !h->valid) return 0;
+    if (!h->inited && !vox_init(h)) return 0;
+    // Tile maps and per-depth node arrays are sized here; tile pools grow on
+    // demand in vox_generate.
+    VoxParams& P = h->P;
+    for (int a = 0; a < 3; ++a) {
+        P.S[a] = h->S[a];
+        P.TG[a] = (h->S[a] + kTile - 1) / kTile;
+    }
+    P.numTiles = (uint64_t)P.TG[0] * P.TG[1] * P.TG[2];
+    h->lv.assign(h->L, LevelBufs());
+    for (int k = 0; k < h->L; ++k) {
+        LevelBufs& l = h->lv[k];
+        for (int a = 0; a < 3; ++a) {
+            l.map.dims[a] = h->S[a] >> k;
+            l.map.TG[a] = (l.map.dims[a] + kTile - 1) / kTile;
+        }
+        l.map.numTiles = (uint64_t)l.map.TG[0] * l.map.TG[1] * l.map.TG[2];
+        uint64_t nt = l.map.numTiles;
+        if (!l.mark.ensure(nt * 4, h->err, "tile marks") ||
+            !l.excl.ensure((nt + 1) * 4, h->err, "tile scan") ||
+            !l.tileIdx.ensure(nt * 4, h->err, "tile map") ||
+            !l.list.ensure(nt * 4, h->err, "tile list"))
+            return 0;
+        l.map.tileIdx = l.tileIdx.as<int32_t>();
+    }
+    h->tree.assign(h->L, DepthBufs());
+    for (int d = 0; d < h->L; ++d) {
+        DepthBufs& t = h->tree[d];
+        t.Rd = 1u << d;
+        t.nodes = (uint64_t)t.Rd * t.Rd * t.Rd;
+        if (!t.types.ensure(t.nodes, h->err, "node types") ||
+            !t.cc.ensure(t.nodes * 4, h->err, "node colours") ||
+            !t.flag.ensure(t.nodes * 4, h->err, "node flags") ||
+            !t.excl.ensure((t.nodes + 1) * 4, h->err, "node scan") ||
+            !t.bidx.ensure(t.nodes * 4, h->err, "brick index"))
+            return 0;
+    }
+    if (!h->counters.ensure(4096, h->err, "counters")) return 0;
+    if (!h->sat.ensure(8, h->err, "sat counter")) return 0;
+    h->voxMem = true;
+    return 1;
+}
+
+void vox_deallocate_voxel_memory(vox_ctx* h) {
+    if (!h) return;
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    for (auto& l : h->lv) {
+        l.mark.release(); l.excl.release(); l.tileIdx.release(); l.list.release();
+        l.col.release(); l.nrm.release();
+    }
+    h->lv.clear();
+    for (auto& t : h->tree) {
+        t.types.release(); t.cc.release(); t.flag.release(); t.excl.release(); t.bidx.release();
+        t.brickNode.release(); t.bcol.release(); t.bnrm.release();
+    }
+    h->tree.clear();
+    h->occ0.release(); h->hit0.release(); h->sat.release(); h->bitmap.release();
+    h->counters.release(); h->binCount.release(); h->binStart.release(); h->binTri.release();
+    h->scanTmp.release();
+    h->voxMem = false;
+    h->haveResult = false;
+}
+
+int vox_allocate_triangle_memory(vox_ctx* h) {
+    if (!h || !h->valid) return 0;
+    if (!h->inited && !vox_init(h)) return 0;
+    uint64_t T = h->T, nvn = 0, nuv = 0, nimg = 0;
+    for (auto& g : h->groups) {
+        if (g.desc.vnormals) nvn += g.desc.ntri;
+        if (g.desc.uvs && g.desc.rgba8) {
+            nuv += g.desc.ntri;
+            nimg += (uint64_t)g.desc.width * g.desc.height * 4;
+        }
+    }
+    if (!h->verts.ensure(T * 36, h->err, "vertex buffer") ||
+        !h->fnorm.ensure(T * 12, h->err, "face normals") ||
+        !h->bbox.ensure(T * 12, h->err, "bounds") ||
+        !h->vn.ensure(nvn * 36, h->err, "vertex normals") ||
+        !h->uv.ensure(nuv * 24, h->err, "uvs") ||
+        !h->img.ensure(nimg, h->err, "images") ||
+        !h->grpFill.ensure(h->groups.size() * 4 + 4, h->err, "group flags"))
+        return 0;
+    // Concatenate groups (allocateTriangleMemory, Voxelizer.cu 346-457).
+    uint64_t to = 0, vo = 0, uo = 0, io = 0;
+    h->gargs.clear();
+    for (auto& g : h->groups) {
+        const vox_group& d = g.desc;
+        hipMemcpyKind kind = g.device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        GroupArgs ga{};
+        ga.triOff = (uint32_t)to;
+        ga.ntri = (uint32_t)d.ntri;
+        if (d.ntri && !h->check(hipMemcpyAsync(h->verts.as<float>() + 9 * to, d.verts, d.ntri * 36,
+                                               kind, h->st), "vertex copy"))
+            return 0;
+        if (d.vnormals) {
+            if (d.ntri && !h->check(hipMemcpyAsync(h->vn.as<float>() + 9 * vo, d.vnormals,
+                                                   d.ntri * 36, kind, h->st), "normal copy"))
+                return 0;
+            ga.vn = h->vn.as<float>() + 9 * vo;
+            ga.hasVN = 1;
+            vo += d.ntri;
+        }
+        if (d.uvs && d.rgba8) {
+            if (d.ntri && !h->check(hipMemcpyAsync(h->uv.as<float>() + 6 * uo, d.uvs, d.ntri * 24,
+                                                   kind, h->st), "uv copy"))
+                return 0;
+            uint64_t ib = (uint64_t)d.width * d.height * 4;
+            if (!h->check(hipMemcpyAsync(h->img.as<uint8_t>() + io, d.rgba8, ib, kind, h->st),
+                          "image copy"))
+                return 0;
+            ga.uv = h->uv.as<float>() + 6 * uo;
+            ga.img = h->img.as<uint8_t>() + io;
+            ga.w = d.width;
+            ga.h = d.height;
+            ga.addr0 = d.addr0;
+            ga.addr1 = d.addr1;
+            ga.textured = 1;
+            uo += d.ntri;
+            io += ib;
+        }
+        ga.terrain = d.is_terrain ? 1 : 0;
+        h->gargs.push_back(ga);
+        to += d.ntri;
+    }
+    if (!h->check(hipStreamSynchronize(h->st), "triangle upload")) return 0;
+    h->triMem = true;
+    h->prepared = false;
+    return 1;
+}
+
+void vox_deallocate_triangle_memory(vox_ctx* h) {
+    if (!h) return;
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    h->verts.release(); h->fnorm.release(); h->bbox.release(); h->vn.release();
+    h->uv.release(); h->img.release(); h->grpFill.release();
+    h->groups.clear();
+    h->gargs.clear();
+    h->T = 0;
+    h->triMem = false;
+    h->prepared = false;
+}
+
+int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
+    if (!h || !h->valid) return 0;
+    if (!h->voxMem && !vox_allocate_voxel_memory(h)) return 0;
+    if (!h->triMem && !vox_allocate_triangle_memory(h)) return 0;
+    VoxParams& P = h->P;
+    for (int a = 0; a < 3; ++a) {
+        P.offP[a] = h->offP[a];
+        P.dP[a] = h->dP[a];
+    }
+    const int G = (int)h->groups.size();
+    // Group voxel bounds (addTriangleGroup, Voxelizer.h 137-144) and chunk
+    // overlap (Voxelizer.cu 2341-2353).
+    std::vector<int32_t> fill(G + 1, 0);
+    for (int g = 0; g < G; ++g) {
+        const vox_group& d = h->groups[g].desc;
+        GroupArgs& ga = h->gargs[g];
+        uint32_t mn[3], mx[3];
+        for (int a = 0; a < 3; ++a) {
+            float lo = gclampf((d.bbox_min[a] - h->offP[a]) / h->dP[a], 0.0f, (float)h->S[a]);
+            float hi = gclampf((d.bbox_max[a] - h->offP[a]) / h->dP[a], 0.0f, (float)h->S[a]);
+            mn[a] = (uint32_t)trunc_u(lo);
+            mx[a] = (uint32_t)trunc_u(hi);
+        }
+        ga.overlaps = (mn[0] <= h->S[0] - 1 && mn[1] <= h->S[1] - 1 && mn[2] <= h->S[2] - 1) ? 1 : 0;
+        ga.zStart = mn[2];
+        ga.zEnd = std::min(mx[2], h->S[2] - 1);
+        fill[g] = (ga.overlaps && ga.textured && ga.terrain) ? 1 : 0;
+    }
+    if (!h->check(hipMemcpyAsync(h->grpFill.p, fill.data(), 4 * (G + 1), hipMemcpyHostToDevice, h->st),
+                  "group flags"))
+        return 0;
+    h->numBins = (uint64_t)std::max(G, 1) * P.numTiles;
+    if (!h->binCount.ensure((h->numBins + 1) * 4, h->err, "bin counts") ||
+        !h->binStart.ensure((h->numBins + 1) * 4, h->err, "bin offsets"))
+        return 0;
+    uint32_t* cnt = h->binCount.as<uint32_t>();
+    uint32_t* start = h->binStart.as<uint32_t>();
+    h->run("fill", [&] { launch_fill_u32(h->st, cnt, h->numBins, 0u); });
+    h->run("setup", [&] {
+        for (int g = 0; g < G; ++g)
+            launch_setup(h->st, P, h->gargs[g], g, h->verts.as<float>(), h->fnorm.as<float>(),
+                         h->bbox.as<uint32_t>(), cnt);
+    });
+    h->run("scan", [&] { scan(h, cnt, start, h->numBins); });
+    uint32_t total = 0;
+    if (!h->check(hipMemcpyAsync(&total, start + h->numBins, 4, hipMemcpyDeviceToHost, h->st),
+                  "bin total") ||
+        !h->check(hipStreamSynchronize(h->st), "setup"))
+        return 0;
+    h->pairs = total;
+    if (!h->binTri.ensure((uint64_t)total * 4, h->err, "bin entries")) return 0;
+    h->run("fill", [&] { launch_fill_u32(h->st, cnt, h->numBins, 0u); });
+    h->run("scatter", [&] {
+        for (int g = 0; g < G; ++g)
+            launch_scatter(h->st, P, h->gargs[g], g, h->bbox.as<uint32_t>(), start, cnt,
+                           h->binTri.as<uint32_t>());
+    });
+    h->prepared = true;
+    return 1;
+}
+
+int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
+    if (!h || !h->valid) return 0;
+    if (compressed) {
+        h->fail("compressed (DXT) bricks are not supported: the reference delegated the encoding "
+                "to the OpenGL driver");
+        return 0;
+    }
+    if (!h->prepared && !vox_compute_edges_face_normals_and_bounds(h)) return 0;
+    const int G = (int)h->groups.size();
+    const int L = h->L;
+    VoxParams& P = h->P;
+    hipStream_t st = h->st;
+    std::memset(&h->stats, 0, sizeof(h->stats));
+    (void)hipEventRecord(h->evStart, st);
+
+    int firstOverlap = -1;
+    for (int g = 0; g < G; ++g)
+        if (h->gargs[g].overlaps) { firstOverlap = g; break; }
+
+    // Active tiles: level 0 from the bins, level k from level src(k).
+    LevelBufs& l0 = h->lv[0];
+    h->run("activity", [&] {
+        launch_activity0(st, P, G, h->grpFill.as<int32_t>(), h->binStart.as<uint32_t>(),
+                         l0.mark.as<uint32_t>());
+    });
+    const bool chain = (h->options & VOX_OPT_MIP_CHAIN) != 0;
+    auto srcOf = [&](int k) { return chain ? k - 1 : (k == 1 ? 0 : k - 2); };
+    uint32_t* ctr = h->counters.as<uint32_t>();
+    // Each level is compacted before the next one marks its parents; the
+    // dense tile list is sized for every tile of the level.
+    for (int k = 0; k < L; ++k) {
+        LevelBufs& l = h->lv[k];
+        if (k > 0) {
+            LevelBufs& s = h->lv[srcOf(k)];
+            h->run("fill", [&] { launch_fill_u32(st, l.mark.as<uint32_t>(), l.map.numTiles, 0u); });
+            h->run("mark", [&] { launch_mark_parent(st, s.map, l.map, l.mark.as<uint32_t>()); });
+        }
+        h->run("scan", [&] { scan(h, l.mark.as<uint32_t>(), l.excl.as<uint32_t>(), l.map.numTiles); });
+        h->run("compact", [&] {
+            launch_compact(st, l.map.numTiles, l.mark.as<uint32_t>(), l.excl.as<uint32_t>(),
+                           l.tileIdx.as<int32_t>(), l.list.as<int32_t>());
+        });
+        if (!h->check(hipMemcpyAsync(ctr + k, l.excl.as<uint32_t>() + l.map.numTiles, 4,
+                                     hipMemcpyDeviceToDevice, st), "tile count"))
+            return 0;
+    }
+    uint32_t act[VOX_MAX_LEVELS] = {0};
+    if (!h->check(hipMemcpyAsync(act, ctr, 4 * L, hipMemcpyDeviceToHost, st), "tile counts") ||
+        !h->check(hipStreamSynchronize(st), "tile maps"))
+        return 0;
+    for (int k = 0; k < L; ++k) {
+        LevelBufs& l = h->lv[k];
+        l.active = act[k];
+        h->stats.active_tiles[k] = act[k];
+        if (!l.col.ensure((uint64_t)act[k] * kTileVox * 4, h->err, "tile colours") ||
+            !l.nrm.ensure((uint64_t)act[k] * kTileVox * 12, h->err, "tile normals"))
+            return 0;
+    }
+    if (!h->occ0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "occupancy words") ||
+        !h->hit0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "hit words"))
+        return 0;
+
+    // Voxelize, colour and fill, group by group (Voxelizer.cu 2335-2553).
+    unsigned long long* satc = h->sat.as<unsigned long long>();
+    h->run("fill", [&] { launch_fill_u32(st, reinterpret_cast<uint32_t*>(satc), 2, 0u); });
+    for (int g = 0; g < G; ++g) {
+        const GroupArgs& ga = h->gargs[g];
+        if (!ga.overlaps) continue;
+        h->run("voxelize", [&] {
+            launch_voxelize(st, P, ga, g, g == firstOverlap ? 1 : 0, l0.active, l0.list.as<int32_t>(),
+                            h->verts.as<float>(), h->fnorm.as<float>(), h->bbox.as<uint32_t>(),
+                            h->binStart.as<uint32_t>(), h->binTri.as<uint32_t>(),
+                            l0.col.as<uint32_t>(), l0.nrm.as<float>(), h->occ0.as<uint32_t>(),
+                            h->hit0.as<uint32_t>(), satc);
+        });
+        if (ga.textured && ga.terrain) {
+            h->run("fill_terrain", [&] {
+                launch_fill(st, P, l0.active, l0.list.as<int32_t>(), l0.tileIdx.as<int32_t>(),
+                            l0.col.as<uint32_t>(), l0.nrm.as<float>(), h->hit0.as<uint32_t>());
+            });
+        }
+    }
+    if (firstOverlap < 0 && l0.active > 0) {
+        // no group overlaps: tiles cannot be active; keep pools defined anyway
+        h->run("fill", [&] { launch_fill_u32(st, l0.col.as<uint32_t>(), l0.active * kTileVox, 0u); });
+    }
+    // Mip chain (computeMipMaps, Voxelizer.cu 491-594).
+    for (int k = 1; k < L; ++k) {
+        LevelBufs& s = h->lv[srcOf(k)];
+        LevelBufs& l = h->lv[k];
+        h->run("mip", [&] {
+            launch_mip(st, s.map, l.map, l.active, l.list.as<int32_t>(), s.col.as<uint32_t>(),
+                       s.nrm.as<float>(), l.col.as<uint32_t>(), l.nrm.as<float>());
+        });
+    }
+    // Node typing per depth (computeOctTreeNodes, Voxelizer.cu 931-1290).
+    uint32_t border[3] = {h->extra[0], h->extra[1], h->extra[2]};
+    std::vector<std::array<uint32_t, 3>> borders(L);
+    for (int d = L - 1; d >= 0; --d) {
+        int k = L - 1 - d;
+        LevelBufs& l = h->lv[k];
+        DepthBufs& t = h->tree[d];
+        borders[d] = {border[0], border[1], border[2]};
+        const uint32_t Ckyz[2] = {l.map.dims[1], l.map.dims[2]};
+        const uint32_t byz[2] = {border[1], border[2]};
+        const uint32_t fyz[2] = {t.Rd * h->B[1], t.Rd * h->B[2]};
+        uint32_t xLimit = type_x_limit(l.map.dims[0], border[0], t.Rd * h->B[0], Ckyz, byz, fyz);
+        h->run("node_type", [&] {
+            launch_node_type(st, l.map, l.col.as<uint32_t>(), t.Rd, h->B, border, xLimit,
+                             t.types.as<uint8_t>(), t.cc.as<uint32_t>(), t.flag.as<uint32_t>());
+        });
+        h->run("scan", [&] { scan(h, t.flag.as<uint32_t>(), t.excl.as<uint32_t>(), t.nodes); });
+        if (!h->check(hipMemcpyAsync(ctr + 32 + d, t.excl.as<uint32_t>() + t.nodes, 4,
+                                     hipMemcpyDeviceToDevice, st), "brick count"))
+            return 0;
+        for (int a = 0; a < 3; ++a) border[a] = std::max(border[a] >> 1, 1u);
+    }
+    uint32_t nb[VOX_MAX_LEVELS] = {0};
+    if (!h->check(hipMemcpyAsync(nb, ctr + 32, 4 * L, hipMemcpyDeviceToHost, st), "brick counts") ||
+        !h->check(hipStreamSynchronize(st), "node typing"))
+        return 0;
+    const uint64_t bvox = (uint64_t)(h->B[0] + 2) * (h->B[1] + 2) * (h->B[2] + 2);
+    for (int d = L - 1; d >= 0; --d) {
+        int k = L - 1 - d;
+        LevelBufs& l = h->lv[k];
+        DepthBufs& t = h->tree[d];
+        t.bricks = nb[d];
+        h->stats.bricks_total += nb[d];
+        if (!t.brickNode.ensure((uint64_t)nb[d] * 4, h->err, "brick nodes") ||
+            !t.bcol.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick colours") ||
+            !t.bnrm.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick normals"))
+            return 0;
+        h->run("brick_assign", [&] {
+            launch_brick_assign(st, t.nodes, t.flag.as<uint32_t>(), t.excl.as<uint32_t>(),
+                                t.bidx.as<uint32_t>(), t.brickNode.as<uint32_t>());
+        });
+        uint32_t bd[3] = {borders[d][0], borders[d][1], borders[d][2]};
+        h->run("brick_gather", [&] {
+            launch_brick_gather(st, l.map, l.col.as<uint32_t>(), l.nrm.as<float>(), nb[d],
+                                t.brickNode.as<uint32_t>(), t.Rd, h->B, bd, t.bcol.as<uint32_t>(),
+                                t.bnrm.as<uint32_t>());
+        });
+    }
+    // CollapseConstantBranches (Voxelizer.cu 701-774).
+    for (int d = L - 2; d >= 0; --d) {
+        h->run("collapse", [&] {
+            launch_collapse(st, h->tree[d].Rd, d + 1 == L - 1 ? 1 : 0, h->tree[d].types.as<uint8_t>(),
+                            h->tree[d + 1].types.as<uint8_t>());
+        });
+    }
+    // Dense occupancy bitmap.
+    const uint64_t nvox = (uint64_t)h->S[0] * h->S[1] * h->S[2];
+    const uint64_t words = (nvox + 31) / 32;
+    if (!(h->options & VOX_OPT_NO_OCCUPANCY)) {
+        if (!h->bitmap.ensure(words * 4, h->err, "occupancy bitmap")) return 0;
+        h->run("fill", [&] { launch_fill_u32(st, h->bitmap.as<uint32_t>(), words, 0u); });
+        h->run("occupancy", [&] {
+            launch_occupancy(st, P, l0.active, l0.list.as<int32_t>(), h->occ0.as<uint32_t>(),
+                             h->bitmap.as<uint32_t>());
+        });
+    }
+    (void)hipEventRecord(h->evStop, st);
+    unsigned long long sat = 0;
+    if (!h->check(hipMemcpyAsync(&sat, satc, 8, hipMemcpyDeviceToHost, st), "counters") ||
+        !h->check(hipStreamSynchronize(st), "generate"))
+        return 0;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, h->evStart, h->evStop);
+    h->stats.ms_total = ms;
+    h->stats.triangles = h->T;
+    h->stats.pairs = h->pairs;
+    h->stats.sat_tests = sat;
+    if (h->options & VOX_OPT_TIMING) collect_timings(h);
+    h->haveResult = true;
+    h->prepared = false;  // bins are consumed; a new generate re-runs setup
+    if (out_dir) {
+        if (!binary) {
+            h->fail("text (.gvt) brick output is not supported; pass binary=1");
+            return 0;
+        }
+        if (!vox_write_files(h, out_dir)) return 0;
+    }
+    return firstOverlap >= 0 ? 1 : -1;
+}
+
+const char* vox_get_error(vox_ctx* h) {
+    if (!h) return "null handle";
+    h->errCache = h->err.str();
+    return h->errCache.c_str();
+}
+
+int vox_reset(vox_ctx* h) {
+    if (!h) return 0;
+    // resetCuda (Voxelizer.cu 324-344): free, reset the device, reallocate.
+    vox_deallocate_triangle_memory(h);
+    vox_deallocate_voxel_memory(h);
+    for (auto e : h->eventPool) (void)hipEventDestroy(e);
+    h->eventPool.clear();
+    h->eventNext = 0;
+    if (h->evStart) (void)hipEventDestroy(h->evStart);
+    if (h->evStop) (void)hipEventDestroy(h->evStop);
+    h->evStart = h->evStop = nullptr;
+    if (h->st) (void)hipStreamDestroy(h->st);
+    h->st = nullptr;
+    h->inited = false;
+    if (!h->check(hipDeviceReset(), "hipDeviceReset")) return 0;
+    if (!vox_init(h)) return 0;
+    return vox_allocate_voxel_memory(h);
+}
+
+int vox_get_result(vox_ctx* h, vox_result* out) {
+    if (!h || !h->valid || !out) return 0;
+    if (!h->haveResult) return h->fail("no result: call vox_generate first"), 0;
+    std::memset(out, 0, sizeof(*out));
+    out->levels = h->L;
+    for (int a = 0; a < 3; ++a) {
+        out->S[a] = h->S[a];
+        out->brick_dim[a] = h->B[a];
+    }
+    const uint64_t nvox = (uint64_t)h->S[0] * h->S[1] * h->S[2];
+    if (!(h->options & VOX_OPT_NO_OCCUPANCY)) {
+        out->occupancy = h->bitmap.as<uint32_t>();
+        out->occupancy_words = (nvox + 31) / 32;
+    }
+    for (int d = 0; d < h->L; ++d) {
+        DepthBufs& t = h->tree[d];
+        out->nodes[d] = t.nodes;
+        out->types[d] = t.types.as<uint8_t>();
+        out->const_colors[d] = t.cc.as<uint32_t>();
+        out->brick_index[d] = t.bidx.as<uint32_t>();
+        out->bricks[d] = t.bricks;
+        out->brick_colors[d] = t.bcol.as<uint32_t>();
+        out->brick_normals[d] = t.bnrm.as<uint32_t>();
+        out->brick_node[d] = t.brickNode.as<uint32_t>();
+    }
+    return 1;
+}
+
+int vox_get_stats(vox_ctx* h, vox_stats* out) {
+    if (!h || !out) return 0;
+    *out = h->stats;
+    return 1;
+}
+
+int vox_copy_to_host(vox_ctx* h, void* dst, const void* src_device, uint64_t nbytes) {
+    if (!h || !h->st) return 0;
+    if (nbytes == 0) return 1;
+    if (!h->check(hipMemcpyAsync(dst, src_device, nbytes, hipMemcpyDeviceToHost, h->st), "copy") ||
+        !h->check(hipStreamSynchronize(h->st), "copy"))
+        return 0;
+    return 1;
+}
+
+int vox_copy_level(vox_ctx* h, int32_t level, uint32_t* colors, float* normals, uint32_t dims[3]) {
+    if (!h || !h->valid || !h->haveResult) return 0;
+    if (level < 0 || level >= h->L) return h->fail("level out of range"), 0;
+    LevelBufs& l = h->lv[level];
+    for (int a = 0; a < 3; ++a) dims[a] = l.map.dims[a];
+    const uint64_t n = (uint64_t)l.map.dims[0] * l.map.dims[1] * l.map.dims[2];
+    DevBuf c, nr;
+    if (!c.ensure(n * 4, h->err, "dense colours") || !nr.ensure(n * 12, h->err, "dense normals"))
+        return 0;
+    launch_dense_level(h->st, l.map, l.col.as<uint32_t>(), l.nrm.as<float>(), c.as<uint32_t>(),
+                       nr.as<float>());
+    bool ok = h->check(hipMemcpyAsync(colors, c.p, n * 4, hipMemcpyDeviceToHost, h->st), "copy") &&
+              h->check(hipMemcpyAsync(normals, nr.p, n * 12, hipMemcpyDeviceToHost, h->st), "copy") &&
+              h->check(hipStreamSynchronize(h->st), "copy level");
+    c.release();
+    nr.release();
+    return ok ? 1 : 0;
+}
+
+int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3], const float delta[3],
+                         const uint32_t N[3]);
+
+int vox_write_files(vox_ctx* h, const char* out_dir) {
+    if (!h || !h->valid || !out_dir) return 0;
+    if (!vox_write_files_impl(h, out_dir, h->p, h->dP, h->N))
+        return h->fail(std::string("failed to write GigaVoxels files to ") + out_dir), 0;
+    return 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// vox_internal.h -- shared declarations between the HIP kernels
+// (vox_kernels.hip) and the host orchestration (vox_api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace voxmi {
+
+constexpr int kTile = 8;            // voxel tile edge (8^3 = 512 voxels)
+constexpr int kTileVox = 512;
+constexpr int kTileWords = 16;      // 512 bits per tile
+constexpr int kBlock = 256;         // threads per workgroup for tile kernels
+constexpr int kFillDepth = 19;      // terrain fill reach (VoxelizerKernels.cu 888)
+constexpr int kFillTiles = 3;       // ceil(19 / 8)
+
+struct VoxParams {
+    uint32_t S[3];        // voxel space per axis (N + 4*extra)
+    uint32_t TG[3];       // level-0 tile grid, ceil(S/8)
+    uint64_t numTiles;    // TG[0]*TG[1]*TG[2]
+    float offP[3];        // p - extra*delta
+    float dP[3];          // delta
+};
+
+struct GroupArgs {
+    uint32_t triOff;      // first global triangle index of the group
+    uint32_t ntri;
+    uint32_t zStart, zEnd;
+    int32_t overlaps;     // group bbox overlaps the chunk
+    int32_t textured;
+    int32_t terrain;
+    int32_t hasVN;
+    const float* vn;      // group-local vertex normals (9 per triangle) or null
+    const float* uv;      // group-local uvs (6 per triangle) or null
+    const uint8_t* img;   // RGBA8 image or null
+    int32_t w, h, addr0, addr1;
+};
+
+struct LevelMap {         // sparse tiles of one mip level
+    uint32_t dims[3];     // S_k
+    uint32_t TG[3];       // tile grid
+    uint64_t numTiles;
+    const int32_t* tileIdx;   // dense map tile -> compact index or -1
+};
+
+// ---- launchers (all asynchronous on `st`) ----
+void launch_setup(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
+                  const float* verts, float* fnorm, uint32_t* bbox, uint32_t* binCount);
+void launch_scatter(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
+                    const uint32_t* bbox, const uint32_t* binStart, uint32_t* binFill,
+                    uint32_t* binTri);
+void launch_activity0(hipStream_t st, const VoxParams& P, int ngroups, const int32_t* grpFill,
+                      const uint32_t* binStart, uint32_t* mark);
+void launch_compact(hipStream_t st, uint64_t n, const uint32_t* mark, const uint32_t* excl,
+                    int32_t* tileIdx, int32_t* list);
+void launch_mark_parent(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                        uint32_t* markDst);
+void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g, int init,
+                     uint64_t nActive, const int32_t* list, const float* verts,
+                     const float* fnorm, const uint32_t* bbox, const uint32_t* binStart,
+                     const uint32_t* binTri, uint32_t* col, float* nrm, uint32_t* occ,
+                     uint32_t* hit, unsigned long long* satCounter);
+void launch_fill(hipStream_t st, const VoxParams& P, uint64_t nActive, const int32_t* list,
+                 const int32_t* tileIdx, uint32_t* col, float* nrm, const uint32_t* hit);
+void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
+                const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
+                uint32_t* colDst, float* nrmDst);
+void launch_node_type(hipStream_t st, const LevelMap& lm, const uint32_t* col, uint32_t Rd,
+                      const uint32_t B[3], const uint32_t border[3], uint32_t xLimit,
+                      uint8_t* types, uint32_t* cc, uint32_t* flag);
+void launch_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
+                         const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode);
+void launch_brick_gather(hipStream_t st, const LevelMap& lm, const uint32_t* col,
+                         const float* nrm, uint64_t nbricks, const uint32_t* brickNode,
+                         uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
+                         uint32_t* bcol, uint32_t* bnrm);
+void launch_collapse(hipStream_t st, uint32_t Rd, int childIsLeaf, uint8_t* types,
+                     const uint8_t* childTypes);
+void launch_occupancy(hipStream_t st, const VoxParams& P, uint64_t nActive, const int32_t* list,
+                      const uint32_t* occ, uint32_t* bitmap);
+void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
+                        const float* nrm, uint32_t* outCol, float* outNrm);
+void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
+// exclusive scan of n u32 values; out has n+1 entries, out[n] = total.
+// tmp must hold scan_tmp_words(n) words.
+uint64_t scan_tmp_words(uint64_t n);
+void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp);
+
+}  // namespace voxmi

@@ -1,0 +1,1083 @@
+// vox_kernels.hip -- HIP kernels of the MI355X voxelizer (gfx950).
+//
+// Pipeline (one stream, everything resident in HBM):
+//   setup      per triangle: face normal, voxel bounds, count of 8^3 tiles
+//   scatter    (group, tile) bins of triangle indices (counting sort)
+//   activity   tiles with triangles, plus the terrain-fill reach below them
+//   voxelize   one workgroup per active tile and group: the tile's triangles
+//              are staged in LDS with their plane / edge-test constants, the
+//              (triangle, voxel) pairs of their clipped bounds are spread over
+//              the workgroup, hits are reduced per voxel with LDS atomics
+//              (count, three lowest and the highest triangle index), then each
+//              hit voxel averages colour and normal
+//   fill       terrain fill below textured hits
+//   mip        2x2x2 box filter, tile to tile
+//   node_type  per octree node: brick + 1-voxel border equal to its sample?
+//   assign / gather / collapse: brick addresses by prefix sum, brick payload,
+//              constant-branch collapse
+//
+// All fp32 arithmetic follows the reference's operation order (GLM 0.9.3
+// dot / cross / normalize) and is compiled with -ffp-contract=off; division
+// and sqrt are correctly rounded (hipcc default).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vox_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace voxmi {
+namespace {
+
+struct f2 { float x, y; };
+struct f3 { float x, y, z; };
+struct f4 { float x, y, z, w; };
+
+__device__ __forceinline__ float cget(const f3& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 muls3(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 x, f3 y) {
+    return {x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
+}
+__device__ __forceinline__ f3 normalize3(f3 v) {
+    float sqr = v.x * v.x + v.y * v.y + v.z * v.z;
+    float inv = 1.0f / sqrtf(sqr);
+    return muls3(v, inv);
+}
+__device__ __forceinline__ float gmin(float x, float y) { return x < y ? x : y; }
+__device__ __forceinline__ float gmax(float x, float y) { return x > y ? x : y; }
+__device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmax(gmin(x, hi), lo); }
+
+__device__ __forceinline__ int64_t trunc_i(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 9.0e18f) return INT64_C(9000000000000000000);
+    if (f <= -9.0e18f) return -INT64_C(9000000000000000000);
+    return (int64_t)f;
+}
+__device__ __forceinline__ uint32_t quant_u8(float f) {
+    if (!(f == f)) return 0u;
+    if (f <= 0.0f) return 0u;
+    if (f >= 255.0f) return 255u;
+    return (uint32_t)(int)f;
+}
+__device__ __forceinline__ uint32_t pack4(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+    return r | (g << 8) | (b << 16) | (a << 24);
+}
+__device__ __forceinline__ f4 unpack4(uint32_t c) {
+    return {(float)(c & 255u) / 255.0f, (float)((c >> 8) & 255u) / 255.0f,
+            (float)((c >> 16) & 255u) / 255.0f, (float)(c >> 24) / 255.0f};
+}
+__device__ __forceinline__ uint32_t to_color(f4 c) {
+    return pack4(quant_u8(c.x * 255.0f), quant_u8(c.y * 255.0f), quant_u8(c.z * 255.0f),
+                 quant_u8(c.w * 255.0f));
+}
+
+__device__ __forceinline__ f3 load3(const float* p) { return {p[0], p[1], p[2]}; }
+
+// Inclusive scan across a 64-wide wavefront.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of one value per thread over a 256-thread workgroup.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) {
+        uint32_t s = s_w[i];
+        pre += (i < w) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - v;
+}
+
+// ---------------------------------------------------------------------------
+// setup: ComputeEdgesFaceNormalsAndBounds (VoxelizerKernels.cu 230-347) plus
+// the count of 8^3 tiles each triangle's bounds touch, per (group, tile) bin.
+// Bounds are packed as min | max << 16 per axis.
+__global__ void k_setup(VoxParams P, GroupArgs G, int g, const float* __restrict__ verts,
+                        float* __restrict__ fnorm, uint32_t* __restrict__ bbox,
+                        uint32_t* __restrict__ binCount) {
+    uint64_t lt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lt >= G.ntri) return;
+    uint64_t t = G.triOff + lt;
+    const float* q = verts + 9 * t;
+    f3 v[3] = {load3(q), load3(q + 3), load3(q + 6)};
+    f3 e0 = sub3(v[1], v[0]);
+    f3 p0p2 = sub3(v[2], v[0]);
+    f3 n = normalize3(cross3(e0, p0p2));
+    fnorm[3 * t] = n.x;
+    fnorm[3 * t + 1] = n.y;
+    fnorm[3 * t + 2] = n.z;
+    uint32_t mn[3], mx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        f3 diff = {(v[k].x - P.offP[0]) / P.dP[0], (v[k].y - P.offP[1]) / P.dP[1],
+                   (v[k].z - P.offP[2]) / P.dP[2]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int64_t iv = trunc_i(cget(diff, a));
+            int64_t hi = (int64_t)P.S[a] - 1;
+            uint32_t u = (uint32_t)(iv < 0 ? 0 : (iv > hi ? hi : iv));
+            if (k == 0) { mn[a] = u; mx[a] = u; }
+            else { mn[a] = u < mn[a] ? u : mn[a]; mx[a] = u > mx[a] ? u : mx[a]; }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) bbox[3 * t + a] = mn[a] | (mx[a] << 16);
+    if (!G.overlaps) return;
+    uint32_t z0 = mn[2] > G.zStart ? mn[2] : G.zStart;
+    uint32_t z1 = mx[2] < G.zEnd ? mx[2] : G.zEnd;
+    if (z0 > z1) return;
+    const uint64_t base = (uint64_t)g * P.numTiles;
+    for (uint32_t tz = z0 >> 3; tz <= (z1 >> 3); ++tz)
+        for (uint32_t ty = mn[1] >> 3; ty <= (mx[1] >> 3); ++ty)
+            for (uint32_t tx = mn[0] >> 3; tx <= (mx[0] >> 3); ++tx) {
+                uint64_t lin = ((uint64_t)tz * P.TG[1] + ty) * P.TG[0] + tx;
+                atomicAdd(&binCount[base + lin], 1u);
+            }
+}
+
+__global__ void k_scatter(VoxParams P, GroupArgs G, int g, const uint32_t* __restrict__ bbox,
+                          const uint32_t* __restrict__ binStart, uint32_t* __restrict__ binFill,
+                          uint32_t* __restrict__ binTri) {
+    uint64_t lt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lt >= G.ntri || !G.overlaps) return;
+    uint64_t t = G.triOff + lt;
+    uint32_t bx = bbox[3 * t], by = bbox[3 * t + 1], bz = bbox[3 * t + 2];
+    uint32_t z0 = (bz & 0xffffu) > G.zStart ? (bz & 0xffffu) : G.zStart;
+    uint32_t z1 = (bz >> 16) < G.zEnd ? (bz >> 16) : G.zEnd;
+    if (z0 > z1) return;
+    const uint64_t base = (uint64_t)g * P.numTiles;
+    for (uint32_t tz = z0 >> 3; tz <= (z1 >> 3); ++tz)
+        for (uint32_t ty = (by & 0xffffu) >> 3; ty <= ((by >> 16) >> 3); ++ty)
+            for (uint32_t tx = (bx & 0xffffu) >> 3; tx <= ((bx >> 16) >> 3); ++tx) {
+                uint64_t key = base + ((uint64_t)tz * P.TG[1] + ty) * P.TG[0] + tx;
+                uint32_t pos = binStart[key] + atomicAdd(&binFill[key], 1u);
+                binTri[pos] = (uint32_t)t;
+            }
+}
+
+// A level-0 tile is active when any group has triangles binned to it, or when
+// a terrain group with a texture has triangles up to three tiles above it
+// (the fill reaches 19 voxels down, VoxelizerKernels.cu 888-917).
+__global__ void k_activity0(VoxParams P, int ngroups, const int32_t* __restrict__ grpFill,
+                            const uint32_t* __restrict__ binStart, uint32_t* __restrict__ mark) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.numTiles) return;
+    uint32_t tz = (uint32_t)(t / ((uint64_t)P.TG[0] * P.TG[1]));
+    uint32_t on = 0;
+    for (int g = 0; g < ngroups && !on; ++g) {
+        uint64_t key = (uint64_t)g * P.numTiles + t;
+        if (binStart[key + 1] > binStart[key]) on = 1;
+        if (!on && grpFill[g]) {
+            for (int q = 1; q <= kFillTiles; ++q) {
+                if (tz + q >= P.TG[2]) break;
+                uint64_t k2 = key + (uint64_t)q * P.TG[0] * P.TG[1];
+                if (binStart[k2 + 1] > binStart[k2]) { on = 1; break; }
+            }
+        }
+    }
+    mark[t] = on;
+}
+
+__global__ void k_compact(uint64_t n, const uint32_t* __restrict__ mark,
+                          const uint32_t* __restrict__ excl, int32_t* __restrict__ tileIdx,
+                          int32_t* __restrict__ list) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (mark[t]) {
+        uint32_t e = excl[t];
+        tileIdx[t] = (int32_t)e;
+        list[e] = (int32_t)t;
+    } else {
+        tileIdx[t] = -1;
+    }
+}
+
+// Level-k tile (a,b,c) reads source voxels [16a, 16a+16), i.e. source tiles
+// 2a and 2a+1 on each axis.
+__global__ void k_mark_parent(LevelMap src, LevelMap dst, uint32_t* __restrict__ markDst) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= src.numTiles) return;
+    if (src.tileIdx[t] < 0) return;
+    uint32_t tx = (uint32_t)(t % src.TG[0]);
+    uint32_t ty = (uint32_t)((t / src.TG[0]) % src.TG[1]);
+    uint32_t tz = (uint32_t)(t / ((uint64_t)src.TG[0] * src.TG[1]));
+    uint32_t px = tx >> 1, py = ty >> 1, pz = tz >> 1;
+    if (px >= dst.TG[0] || py >= dst.TG[1] || pz >= dst.TG[2]) return;
+    markDst[((uint64_t)pz * dst.TG[1] + py) * dst.TG[0] + px] = 1u;
+}
+
+// ---------------------------------------------------------------------------
+// voxelize: ComputeVoxelization (VoxelizerKernels.cu 349-530) for one group
+// on one tile, then ComputeColorsAndNormals / ComputeColorsAsNormals
+// (VoxelizerKernels.cu 689-1083) for the tile's hit voxels.
+constexpr int kStage = 256;     // triangles staged per round
+constexpr int kTriStride = 33;  // floats per staged triangle (padded)
+
+// Bilinear tex2D with normalized coordinates (VoxelizerKernels.cu 586-600):
+// fractional weights held in 8 fractional bits; NaN coordinates read as 0.
+__device__ __forceinline__ int wrap_index(int i, int n, int mode, bool& border) {
+    if (mode == 0) { int m = i % n; return m < 0 ? m + n : m; }
+    if (mode == 2) {
+        int period = 2 * n;
+        int m = i % period;
+        if (m < 0) m += period;
+        return m < n ? m : period - 1 - m;
+    }
+    if (mode == 3) {
+        if (i < 0 || i >= n) { border = true; return 0; }
+        return i;
+    }
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+__device__ __forceinline__ f4 texel(const GroupArgs& G, int i, int j) {
+    bool border = false;
+    int x = wrap_index(i, G.w, G.addr0, border);
+    int y = wrap_index(j, G.h, G.addr1, border);
+    if (border) return {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t c = *reinterpret_cast<const uint32_t*>(G.img + 4 * ((size_t)y * (size_t)G.w + (size_t)x));
+    return unpack4(c);
+}
+__device__ f4 tex2d(const GroupArgs& G, float u, float v) {
+    if (!(u == u)) u = 0.0f;
+    if (!(v == v)) v = 0.0f;
+    float xb = u * (float)G.w - 0.5f;
+    float yb = v * (float)G.h - 0.5f;
+    float fx = floorf(xb), fy = floorf(yb);
+    float ax = rintf((xb - fx) * 256.0f) / 256.0f;
+    float ay = rintf((yb - fy) * 256.0f) / 256.0f;
+    int i = (int)fmaxf(fminf(fx, 1.0e9f), -1.0e9f);
+    int j = (int)fmaxf(fminf(fy, 1.0e9f), -1.0e9f);
+    f4 t00 = texel(G, i, j), t10 = texel(G, i + 1, j);
+    f4 t01 = texel(G, i, j + 1), t11 = texel(G, i + 1, j + 1);
+    float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay);
+    float w01 = (1.0f - ax) * ay, w11 = ax * ay;
+    f4 r;
+    r.x = ((w00 * t00.x + w10 * t10.x) + w01 * t01.x) + w11 * t11.x;
+    r.y = ((w00 * t00.y + w10 * t10.y) + w01 * t01.y) + w11 * t11.y;
+    r.z = ((w00 * t00.z + w10 * t10.z) + w01 * t01.z) + w11 * t11.z;
+    r.w = ((w00 * t00.w + w10 * t10.w) + w01 * t01.w) + w11 * t11.w;
+    return r;
+}
+
+// ComputeVoxelBarycentricCoords (VoxelizerKernels.cu 602-624).
+__device__ __forceinline__ f2 bary(f2 P, f2 A, f2 B, f2 C) {
+    f2 v0 = {C.x - A.x, C.y - A.y};
+    f2 v1 = {B.x - A.x, B.y - A.y};
+    f2 v2 = {P.x - A.x, P.y - A.y};
+    float dot00 = v0.x * v0.x + v0.y * v0.y;
+    float dot01 = v0.x * v1.x + v0.y * v1.y;
+    float dot02 = v0.x * v2.x + v0.y * v2.y;
+    float dot11 = v1.x * v1.x + v1.y * v1.y;
+    float dot12 = v1.x * v2.x + v1.y * v2.y;
+    float invDenom = 1.0f / (dot00 * dot11 - dot01 * dot01);
+    return {gclamp((dot11 * dot02 - dot01 * dot12) * invDenom, 0.0f, 1.0f),
+            gclamp((dot00 * dot12 - dot01 * dot02) * invDenom, 0.0f, 1.0f)};
+}
+__device__ __forceinline__ f3 interp_n(f2 uv, f3 a, f3 b, f3 c) {
+    f3 V = muls3(sub3(b, a), uv.y);
+    f3 U = muls3(sub3(c, a), uv.x);
+    return add3(add3(a, U), V);
+}
+__device__ __forceinline__ f2 bary_axis(int axis, f3 c, const f3* v) {
+    if (axis == 0) return bary({c.z, c.y}, {v[0].z, v[0].y}, {v[1].z, v[1].y}, {v[2].z, v[2].y});
+    if (axis == 1) return bary({c.x, c.z}, {v[0].x, v[0].z}, {v[1].x, v[1].z}, {v[2].x, v[2].z});
+    return bary({c.x, c.y}, {v[0].x, v[0].y}, {v[1].x, v[1].y}, {v[2].x, v[2].y});
+}
+
+// Colour / normal update of one voxel with new hits from group G.
+// slots: up to 4 global triangle indices in reference slot order.
+__device__ void colour_update(const GroupArgs& G, const float* __restrict__ verts,
+                              const float* __restrict__ fnorm, f3 voxCenter, uint32_t count,
+                              const int32_t* slots, int nslots, uint32_t& col, f3& nrm) {
+    f4 color = unpack4(col);
+    f3 normal = nrm;
+    if (G.textured) {
+        uint32_t triCount = count < 4u ? count : 4u;
+        for (int k = 0; k < nslots; ++k) {
+            uint32_t ti = (uint32_t)slots[k];
+            const float* q = verts + 9 * (size_t)ti;
+            f3 v[3] = {load3(q), load3(q + 3), load3(q + 6)};
+            f3 fn = load3(fnorm + 3 * (size_t)ti);
+            size_t lt = ti - G.triOff;
+            const float* uvp = G.uv + 6 * lt;
+            float ax = fabsf(fn.x), ay = fabsf(fn.y), az = fabsf(fn.z);
+            int axis = (ax > ay && ax > az) ? 0 : (ay > az ? 1 : 2);
+            f2 bc = bary_axis(axis, voxCenter, v);
+            f3 addn;
+            if (G.hasVN) {
+                const float* nq = G.vn + 9 * lt;
+                addn = interp_n(bc, load3(nq), load3(nq + 3), load3(nq + 6));
+            } else {
+                addn = fn;
+            }
+            normal = add3(normal, addn);
+            f2 uv1 = {uvp[0], uvp[1]}, uv2 = {uvp[2], uvp[3]}, uv3 = {uvp[4], uvp[5]};
+            f2 Vv = {(uv2.x - uv1.x) * bc.y, (uv2.y - uv1.y) * bc.y};
+            f2 Uu = {(uv3.x - uv1.x) * bc.x, (uv3.y - uv1.y) * bc.x};
+            f2 tc = {uv1.x + Uu.x + Vv.x, uv1.y + Uu.y + Vv.y};
+            f4 tx = tex2d(G, tc.x, tc.y);
+            if ((double)tx.w <= 0.005) {
+                --triCount;
+            } else {
+                tx.x *= tx.w;
+                tx.y *= tx.w;
+                tx.z *= tx.w;
+                color.x += tx.x;
+                color.y += tx.y;
+                color.z += tx.z;
+                color.w += tx.w;
+            }
+        }
+        if (triCount > 0u) {
+            float fc = (float)triCount;
+            color.x /= fc;
+            color.y /= fc;
+            color.z /= fc;
+            color.w = gmin(color.w, 1.0f);
+            normal.x /= fc;
+            normal.y /= fc;
+            normal.z /= fc;
+            normal = normalize3(normal);
+        }
+    } else {
+        uint32_t triCount = count;
+        for (int k = 0; k < nslots; ++k) {
+            uint32_t ti = (uint32_t)slots[k];
+            f3 fn = load3(fnorm + 3 * (size_t)ti);
+            if (G.hasVN) {
+                const float* q = verts + 9 * (size_t)ti;
+                f3 v[3] = {load3(q), load3(q + 3), load3(q + 6)};
+                size_t lt = ti - G.triOff;
+                const float* nq = G.vn + 9 * lt;
+                int axis = (fn.x > fn.y && fn.x > fn.z) ? 0 : (fn.y > fn.z ? 1 : 2);
+                f2 bc = bary_axis(axis, voxCenter, v);
+                f3 ni = interp_n(bc, load3(nq), load3(nq + 3), load3(nq + 6));
+                color.x += ni.x;
+                color.y += ni.y;
+                color.z += ni.z;
+                color.w += 1.0f;
+            } else {
+                color.x += fn.x;
+                color.y += fn.y;
+                color.z += fn.z;
+                color.w += 1.0f;
+            }
+        }
+        float fc = (float)triCount;
+        color.x /= fc;
+        color.y /= fc;
+        color.z /= fc;
+        color.w = gmin(color.w, 1.0f);
+        normal = normalize3(f3{color.x, color.y, color.z});
+        if (color.x < 0.0f) color.x *= -1.0f;
+        if (color.y < 0.0f) color.y *= -1.0f;
+        if (color.z < 0.0f) color.z *= -1.0f;
+    }
+    col = to_color(color);
+    nrm = normal;
+}
+
+__global__ __launch_bounds__(kBlock) void k_voxelize(
+    VoxParams P, GroupArgs G, int g, int init, const int32_t* __restrict__ list,
+    const float* __restrict__ verts, const float* __restrict__ fnorm,
+    const uint32_t* __restrict__ bbox, const uint32_t* __restrict__ binStart,
+    const uint32_t* __restrict__ binTri, uint32_t* __restrict__ col, float* __restrict__ nrm,
+    uint32_t* __restrict__ occ, uint32_t* __restrict__ hit,
+    unsigned long long* __restrict__ satCounter) {
+    __shared__ uint32_t s_cnt[kTileVox];
+    __shared__ int32_t s_lo0[kTileVox], s_lo1[kTileVox], s_lo2[kTileVox], s_hi[kTileVox];
+    __shared__ float s_tri[kStage * kTriStride];
+    __shared__ uint32_t s_box[kStage];
+    __shared__ uint32_t s_tid[kStage];
+    __shared__ uint32_t s_off[kStage + 1];
+    __shared__ uint32_t s_w[kBlock / 64];
+
+    const int tid = threadIdx.x;
+    const uint64_t a = blockIdx.x;
+    const uint64_t tlin = (uint64_t)list[a];
+    const uint32_t tx = (uint32_t)(tlin % P.TG[0]);
+    const uint32_t ty = (uint32_t)((tlin / P.TG[0]) % P.TG[1]);
+    const uint32_t tz = (uint32_t)(tlin / ((uint64_t)P.TG[0] * P.TG[1]));
+    const uint32_t ox = tx * kTile, oy = ty * kTile, oz = tz * kTile;
+
+    for (int v = tid; v < kTileVox; v += kBlock) {
+        s_cnt[v] = 0u;
+        s_lo0[v] = s_lo1[v] = s_lo2[v] = 0x7fffffff;
+        s_hi[v] = -1;
+    }
+    const uint64_t key = (uint64_t)g * P.numTiles + tlin;
+    const uint32_t bStart = binStart[key];
+    const uint32_t nb = binStart[key + 1] - bStart;
+    uint32_t localTests = 0;
+    __syncthreads();
+
+    for (uint32_t c0 = 0; c0 < nb; c0 += kStage) {
+        const uint32_t m = (nb - c0) < (uint32_t)kStage ? (nb - c0) : (uint32_t)kStage;
+        uint32_t w = 0;
+        if ((uint32_t)tid < m) {
+            const uint32_t t = binTri[bStart + c0 + tid];
+            const uint32_t bx = bbox[3 * (size_t)t], by = bbox[3 * (size_t)t + 1],
+                           bz = bbox[3 * (size_t)t + 2];
+            int32_t x0 = (int32_t)(bx & 0xffffu) - (int32_t)ox, x1 = (int32_t)(bx >> 16) - (int32_t)ox;
+            int32_t y0 = (int32_t)(by & 0xffffu) - (int32_t)oy, y1 = (int32_t)(by >> 16) - (int32_t)oy;
+            uint32_t zlo = (bz & 0xffffu) > G.zStart ? (bz & 0xffffu) : G.zStart;
+            uint32_t zhi = (bz >> 16) < G.zEnd ? (bz >> 16) : G.zEnd;
+            int32_t z0 = (int32_t)zlo - (int32_t)oz, z1 = (int32_t)zhi - (int32_t)oz;
+            x0 = x0 < 0 ? 0 : x0; y0 = y0 < 0 ? 0 : y0; z0 = z0 < 0 ? 0 : z0;
+            x1 = x1 > 7 ? 7 : x1; y1 = y1 > 7 ? 7 : y1; z1 = z1 > 7 ? 7 : z1;
+            if (x0 <= x1 && y0 <= y1 && z0 <= z1) {
+                w = (uint32_t)((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1));
+                s_box[tid] = (uint32_t)x0 | ((uint32_t)y0 << 3) | ((uint32_t)z0 << 6) |
+                             ((uint32_t)(x1 - x0) << 9) | ((uint32_t)(y1 - y0) << 12) |
+                             ((uint32_t)(z1 - z0) << 15);
+            } else {
+                s_box[tid] = 0u;
+            }
+            s_tid[tid] = t;
+            // Plane test constants (TrianglePlaneOverlapsVoxel, VK:17-53).
+            const float* q = verts + 9 * (size_t)t;
+            f3 v[3] = {load3(q), load3(q + 3), load3(q + 6)};
+            f3 n = load3(fnorm + 3 * (size_t)t);
+            float* st = &s_tri[tid * kTriStride];
+            st[0] = n.x; st[1] = n.y; st[2] = n.z;
+            st[3] = dot3(f3{-n.x, -n.y, -n.z}, v[0]);
+            // Edge-test constants (TriangleOverlapsVoxelOnAxis, VK:93-144),
+            // pairs XY, ZX, YZ.
+            f3 e[3] = {sub3(v[1], v[0]), sub3(v[2], v[1]), sub3(v[0], v[2])};
+            const int A1[3] = {0, 2, 1}, A2[3] = {1, 0, 2}, OT[3] = {2, 1, 0};
+            const f3 pp = {P.offP[0], P.offP[1], P.offP[2]};
+            const f3 dd = {P.dP[0], P.dP[1], P.dP[2]};
+#pragma unroll
+            for (int pr = 0; pr < 3; ++pr) {
+                const int a1 = A1[pr], a2 = A2[pr], other = OT[pr];
+                float ps = cget(n, other) >= 0.0f ? 1.0f : -1.0f;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    float enx = (-cget(e[i], a2)) * ps, eny = cget(e[i], a1) * ps;
+                    float vpx = enx * cget(dd, a1), vpy = eny * cget(dd, a2);
+                    float dpp = ((-enx) * cget(v[i], a1) + (-eny) * cget(v[i], a2)) +
+                                (enx * cget(pp, a1) + eny * cget(pp, a2)) +
+                                gmax(0.0f, cget(dd, a1) * enx) + gmax(0.0f, cget(dd, a2) * eny);
+                    st[4 + (pr * 3 + i) * 3 + 0] = dpp;
+                    st[4 + (pr * 3 + i) * 3 + 1] = vpx;
+                    st[4 + (pr * 3 + i) * 3 + 2] = vpy;
+                }
+            }
+        }
+        uint32_t total;
+        uint32_t off = block_excl_scan(w, s_w, &total);
+        if ((uint32_t)tid < m) s_off[tid] = off;
+        if (tid == 0) s_off[m] = total;
+        __syncthreads();
+        localTests += 0;
+        for (uint32_t item = tid; item < total; item += kBlock) {
+            // triangle j with s_off[j] <= item < s_off[j+1]
+            uint32_t lo = 0, hi = m;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (s_off[mid] <= item) lo = mid; else hi = mid;
+            }
+            const uint32_t j = lo;
+            const uint32_t r = item - s_off[j];
+            const uint32_t bx = s_box[j];
+            const uint32_t ex = ((bx >> 9) & 7u) + 1u, ey = ((bx >> 12) & 7u) + 1u;
+            const uint32_t lx = (bx & 7u) + r % ex;
+            const uint32_t ly = ((bx >> 3) & 7u) + (r / ex) % ey;
+            const uint32_t lz = ((bx >> 6) & 7u) + r / (ex * ey);
+            const float* st = &s_tri[j * kTriStride];
+            const float uvw[3] = {(float)(ox + lx), (float)(oy + ly), (float)(oz + lz)};
+            ++localTests;
+            // TrianglePlaneOverlapsVoxel (VK:17-53).
+            const f3 n = {st[0], st[1], st[2]};
+            const float D = st[3];
+            const f3 p0 = {P.offP[0] + uvw[0] * P.dP[0], P.offP[1] + uvw[1] * P.dP[1],
+                           P.offP[2] + uvw[2] * P.dP[2]};
+            const float dist1 = dot3(n, p0) + D;
+            bool planeHit = false;
+            {
+                const float dx = P.dP[0], dy = P.dP[1], dz = P.dP[2];
+                const f3 cd[7] = {{dx, dy, dz}, {dx, dy, 0.0f}, {dx, 0.0f, 0.0f}, {0.0f, 0.0f, dz},
+                                  {0.0f, dy, dz}, {dx, 0.0f, dz}, {0.0f, dy, 0.0f}};
+#pragma unroll
+                for (int c = 0; c < 7; ++c) {
+                    f3 p1 = add3(p0, cd[c]);
+                    float dist2 = dot3(n, p1) + D;
+                    planeHit = planeHit || (dist1 * dist2 <= 0.0f);
+                }
+            }
+            if (!planeHit) continue;
+            // TriangleOverlapsVoxel (VK:146-170): XY, ZX, YZ.
+            bool edgeHit = true;
+            const int A1[3] = {0, 2, 1}, A2[3] = {1, 0, 2};
+#pragma unroll
+            for (int pr = 0; pr < 3; ++pr) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float* c = &st[4 + (pr * 3 + i) * 3];
+                    float ot = c[0] + uvw[A2[pr]] * c[2] + uvw[A1[pr]] * c[1];
+                    edgeHit = edgeHit && (ot >= 0.0f);
+                }
+            }
+            if (!edgeHit) continue;
+            const uint32_t vi = (lz * 8u + ly) * 8u + lx;
+            const int32_t t = (int32_t)s_tid[j];
+            atomicAdd(&s_cnt[vi], 1u);
+            // Three lowest indices by an atomic-min cascade: each level keeps
+            // min(old, x) and passes max(old, x) on, so level i ends holding
+            // the i-th smallest index whatever the interleaving.  Highest by
+            // atomic max.
+            int32_t x = t;
+            int32_t old = atomicMin(&s_lo0[vi], x);
+            x = old > x ? old : x;
+            if (x != 0x7fffffff) {
+                old = atomicMin(&s_lo1[vi], x);
+                x = old > x ? old : x;
+                if (x != 0x7fffffff) atomicMin(&s_lo2[vi], x);
+            }
+            atomicMax(&s_hi[vi], t);
+        }
+        __syncthreads();
+    }
+    if (satCounter) {
+        // per-workgroup total of (triangle, voxel) tests for the work counters
+        uint32_t tot;
+        block_excl_scan(localTests, s_w, &tot);
+        if (tid == 0) atomicAdd(satCounter, (unsigned long long)tot);
+    }
+
+    // Finalise: colour / normal of voxels with new hits.
+    const uint64_t poolBase = a * kTileVox;
+    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+        const uint32_t v = (uint32_t)(pass * kBlock + tid);
+        const uint32_t cnt = s_cnt[v];
+        const uint32_t wIdx = v >> 5;
+        const uint32_t prevOcc = init ? 0u : occ[a * kTileWords + wIdx];
+        const bool wasHit = (prevOcc >> (v & 31u)) & 1u;
+        const bool isHit = cnt > 0u;
+        if (isHit) {
+            int32_t slots[4];
+            int ns = 0;
+            const int32_t lo[3] = {s_lo0[v], s_lo1[v], s_lo2[v]};
+            const uint32_t k3 = cnt < 3u ? cnt : 3u;
+            for (uint32_t k = 0; k < k3; ++k) slots[ns++] = lo[k];
+            if (cnt >= 4u) slots[ns++] = s_hi[v];
+            uint32_t c = init ? 0u : col[poolBase + v];
+            f3 nv = init ? f3{0.0f, 0.0f, 0.0f}
+                         : f3{nrm[(a * 3 + 0) * kTileVox + v], nrm[(a * 3 + 1) * kTileVox + v],
+                              nrm[(a * 3 + 2) * kTileVox + v]};
+            const uint32_t lx = v & 7u, ly = (v >> 3) & 7u, lz = v >> 6;
+            f3 cpos = {P.offP[0] + (float)(ox + lx) * P.dP[0], P.offP[1] + (float)(oy + ly) * P.dP[1],
+                       P.offP[2] + (float)(oz + lz) * P.dP[2]};
+            f3 voxCenter = add3(cpos, muls3(f3{P.dP[0], P.dP[1], P.dP[2]}, 0.5f));
+            colour_update(G, verts, fnorm, voxCenter, (wasHit ? 1u : 0u) + cnt, slots, ns, c, nv);
+            col[poolBase + v] = c;
+            nrm[(a * 3 + 0) * kTileVox + v] = nv.x;
+            nrm[(a * 3 + 1) * kTileVox + v] = nv.y;
+            nrm[(a * 3 + 2) * kTileVox + v] = nv.z;
+        } else if (init) {
+            col[poolBase + v] = 0u;
+            nrm[(a * 3 + 0) * kTileVox + v] = 0.0f;
+            nrm[(a * 3 + 1) * kTileVox + v] = 0.0f;
+            nrm[(a * 3 + 2) * kTileVox + v] = 0.0f;
+        }
+        // wave ballot -> 64 hit bits of voxels [pass*256 + 64*wave, +64)
+        const uint64_t mask = __ballot(isHit);
+        if ((tid & 63) == 0) {
+            const uint32_t w0 = v >> 5;  // even word index
+            const uint32_t lo32 = (uint32_t)mask, hi32 = (uint32_t)(mask >> 32);
+            hit[a * kTileWords + w0] = lo32;
+            hit[a * kTileWords + w0 + 1] = hi32;
+            if (init) {
+                occ[a * kTileWords + w0] = lo32;
+                occ[a * kTileWords + w0 + 1] = hi32;
+            } else {
+                occ[a * kTileWords + w0] |= lo32;
+                occ[a * kTileWords + w0 + 1] |= hi32;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fill: terrain fill (VoxelizerKernels.cu 886-917).  A voxel without new hits
+// takes colour and normal of the nearest voxel above (at most 19) that has
+// new hits in this group.
+__global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, const int32_t* __restrict__ list,
+                                                 const int32_t* __restrict__ tileIdx,
+                                                 uint32_t* __restrict__ col,
+                                                 float* __restrict__ nrm,
+                                                 const uint32_t* __restrict__ hit) {
+    __shared__ uint32_t s_hit[kFillTiles + 1][kTileWords];
+    __shared__ int32_t s_idx[kFillTiles + 1];
+    const int tid = threadIdx.x;
+    const uint64_t a = blockIdx.x;
+    const uint64_t tlin = (uint64_t)list[a];
+    const uint32_t tz = (uint32_t)(tlin / ((uint64_t)P.TG[0] * P.TG[1]));
+    const uint64_t slab = (uint64_t)P.TG[0] * P.TG[1];
+    if (tid <= kFillTiles) {
+        int32_t idx = -1;
+        if (tid == 0) idx = (int32_t)a;
+        else if (tz + tid < P.TG[2]) idx = tileIdx[tlin + (uint64_t)tid * slab];
+        s_idx[tid] = idx;
+    }
+    __syncthreads();
+    if (tid < (kFillTiles + 1) * kTileWords) {
+        const int q = tid / kTileWords, wd = tid % kTileWords;
+        const int32_t idx = s_idx[q];
+        s_hit[q][wd] = idx >= 0 ? hit[(uint64_t)idx * kTileWords + wd] : 0u;
+    }
+    __syncthreads();
+    const uint32_t oz = tz * kTile;
+    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+        const uint32_t v = (uint32_t)(pass * kBlock + tid);
+        if ((s_hit[0][v >> 5] >> (v & 31u)) & 1u) continue;
+        const uint32_t lxy = v & 63u, lz = v >> 6;
+        for (uint32_t k = 1; k <= (uint32_t)kFillDepth; ++k) {
+            const uint32_t zz = lz + k;
+            if (oz + zz >= P.S[2]) break;
+            const uint32_t q = zz >> 3, v2 = ((zz & 7u) << 6) | lxy;
+            if ((s_hit[q][v2 >> 5] >> (v2 & 31u)) & 1u) {
+                const uint64_t sa = (uint64_t)s_idx[q];
+                col[a * kTileVox + v] = col[sa * kTileVox + v2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    nrm[(a * 3 + c) * kTileVox + v] = nrm[(sa * 3 + c) * kTileVox + v2];
+                break;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mip: ComputeVoxelMipMap8 (VoxelizerKernels.cu 1520-1663).
+__global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
+                                                const int32_t* __restrict__ listDst,
+                                                const uint32_t* __restrict__ colSrc,
+                                                const float* __restrict__ nrmSrc,
+                                                uint32_t* __restrict__ colDst,
+                                                float* __restrict__ nrmDst) {
+    const int tid = threadIdx.x;
+    const uint64_t b = blockIdx.x;
+    const uint64_t tlin = (uint64_t)listDst[b];
+    const uint32_t tx = (uint32_t)(tlin % dst.TG[0]);
+    const uint32_t ty = (uint32_t)((tlin / dst.TG[0]) % dst.TG[1]);
+    const uint32_t tz = (uint32_t)(tlin / ((uint64_t)dst.TG[0] * dst.TG[1]));
+    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+        const uint32_t v = (uint32_t)(pass * kBlock + tid);
+        const uint32_t X = tx * kTile + (v & 7u), Y = ty * kTile + ((v >> 3) & 7u),
+                       Z = tz * kTile + (v >> 6);
+        uint32_t outC = 0u;
+        f3 outN = {0.0f, 0.0f, 0.0f};
+        if (X < dst.dims[0] && Y < dst.dims[1] && Z < dst.dims[2]) {
+            const uint32_t bx = 2 * X, by = 2 * Y, bz = 2 * Z;
+            const uint32_t x1 = min(bx + 1, src.dims[0] - 1), y1 = min(by + 1, src.dims[1] - 1),
+                           z1 = min(bz + 1, src.dims[2] - 1);
+            const uint64_t st = ((uint64_t)(bz >> 3) * src.TG[1] + (by >> 3)) * src.TG[0] + (bx >> 3);
+            const int32_t sa = src.tileIdx[st];
+            if (sa >= 0) {
+                const uint32_t xs[2] = {bx & 7u, x1 & 7u}, ys[2] = {by & 7u, y1 & 7u},
+                               zs[2] = {bz & 7u, z1 & 7u};
+                uint32_t vid[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    vid[i] = (zs[(i >> 2) & 1] * 8u + ys[(i >> 1) & 1]) * 8u + xs[i & 1];
+                const uint64_t cb = (uint64_t)sa * kTileVox;
+                const uint64_t nb0 = (uint64_t)sa * 3 * kTileVox;
+                f4 avg = unpack4(colSrc[cb + vid[0]]);
+                f3 an = {nrmSrc[nb0 + vid[0]], nrmSrc[nb0 + kTileVox + vid[0]],
+                         nrmSrc[nb0 + 2 * kTileVox + vid[0]]};
+#pragma unroll
+                for (int i = 1; i < 8; ++i) {
+                    f4 cur = unpack4(colSrc[cb + vid[i]]);
+                    avg.x += cur.x; avg.y += cur.y; avg.z += cur.z; avg.w += cur.w;
+                    an = add3(an, f3{nrmSrc[nb0 + vid[i]], nrmSrc[nb0 + kTileVox + vid[i]],
+                                     nrmSrc[nb0 + 2 * kTileVox + vid[i]]});
+                }
+                const float oneOverEight = 1.0f / 8.0f;
+                avg.w *= oneOverEight;
+                avg.x *= oneOverEight;
+                avg.y *= oneOverEight;
+                avg.z *= oneOverEight;
+                an = muls3(an, oneOverEight);
+                float len2 = dot3(an, an);
+                if (len2 > 0.0f) {
+                    float l = sqrtf(len2);
+                    an = {an.x / l, an.y / l, an.z / l};
+                } else {
+                    an = {0.0f, 0.0f, 0.0f};
+                }
+                outC = to_color(avg);
+                outN = an;
+            }
+        }
+        colDst[b * kTileVox + v] = outC;
+        nrmDst[(b * 3 + 0) * kTileVox + v] = outN.x;
+        nrmDst[(b * 3 + 1) * kTileVox + v] = outN.y;
+        nrmDst[(b * 3 + 2) * kTileVox + v] = outN.z;
+    }
+}
+
+__device__ __forceinline__ int64_t tile_of(const LevelMap& lm, uint32_t X, uint32_t Y, uint32_t Z) {
+    const uint64_t t = ((uint64_t)(Z >> 3) * lm.TG[1] + (Y >> 3)) * lm.TG[0] + (X >> 3);
+    return lm.tileIdx[t];
+}
+__device__ __forceinline__ uint32_t vox_of(uint32_t X, uint32_t Y, uint32_t Z) {
+    return ((Z & 7u) * 8u + (Y & 7u)) * 8u + (X & 7u);
+}
+
+// ---------------------------------------------------------------------------
+// node_type: ComputeOctTreeNodeConstColor + ComputeOctTreeNodeType with the
+// border handlers (VoxelizerKernels.cu 1680-2380).  A node is NON-CONST when
+// any voxel of its stored brick (interior + 1-voxel border) differs from the
+// voxel at the brick's first interior corner; voxels at x >= xLimit are not
+// examined (the reference's two-launch x split, Voxelizer.cu 1081-1120).
+__global__ __launch_bounds__(kBlock) void k_node_type(LevelMap lm, const uint32_t* __restrict__ col,
+                                                      uint32_t Rd, uint32_t Bx, uint32_t By,
+                                                      uint32_t Bz, uint32_t bx, uint32_t by,
+                                                      uint32_t bz, uint32_t xLimit,
+                                                      uint8_t* __restrict__ types,
+                                                      uint32_t* __restrict__ cc,
+                                                      uint32_t* __restrict__ flag) {
+    __shared__ uint32_t s_any;
+    const int tid = threadIdx.x;
+    const uint64_t n = blockIdx.x;
+    const uint32_t nx = (uint32_t)(n % Rd), ny = (uint32_t)((n / Rd) % Rd),
+                   nz = (uint32_t)(n / ((uint64_t)Rd * Rd));
+    const uint32_t r0x = nx * Bx + bx - 1, r0y = ny * By + by - 1, r0z = nz * Bz + bz - 1;
+    const uint32_t ex = Bx + 2, ey = By + 2, ez = Bz + 2;
+    const uint32_t t0x = r0x >> 3, t0y = r0y >> 3, t0z = r0z >> 3;
+    const uint32_t ntx = ((r0x + ex - 1) >> 3) - t0x + 1, nty = ((r0y + ey - 1) >> 3) - t0y + 1,
+                   ntz = ((r0z + ez - 1) >> 3) - t0z + 1;
+    if (tid == 0) s_any = 0u;
+    __syncthreads();
+    for (uint32_t i = tid; i < ntx * nty * ntz; i += kBlock) {
+        const uint32_t ix = t0x + i % ntx, iy = t0y + (i / ntx) % nty, iz = t0z + i / (ntx * nty);
+        if (ix < lm.TG[0] && iy < lm.TG[1] && iz < lm.TG[2]) {
+            if (lm.tileIdx[((uint64_t)iz * lm.TG[1] + iy) * lm.TG[0] + ix] >= 0) s_any = 1u;
+        }
+    }
+    __syncthreads();
+    if (!s_any) {
+        if (tid == 0) { types[n] = 0; cc[n] = 0u; flag[n] = 0u; }
+        return;
+    }
+    const uint32_t sx = r0x + 1, sy = r0y + 1, sz = r0z + 1;
+    const int64_t sa = tile_of(lm, sx, sy, sz);
+    const uint32_t sample = sa >= 0 ? col[(uint64_t)sa * kTileVox + vox_of(sx, sy, sz)] : 0u;
+    uint32_t diff = 0u;
+    const uint32_t nvox = ex * ey * ez;
+    for (uint32_t i = tid; i < nvox && !diff; i += kBlock) {
+        const uint32_t X = r0x + i % ex, Y = r0y + (i / ex) % ey, Z = r0z + i / (ex * ey);
+        if (X >= xLimit) continue;
+        const int64_t ta = tile_of(lm, X, Y, Z);
+        const uint32_t c = ta >= 0 ? col[(uint64_t)ta * kTileVox + vox_of(X, Y, Z)] : 0u;
+        diff = c != sample ? 1u : 0u;
+    }
+    __syncthreads();
+    if (tid == 0) s_any = 0u;
+    __syncthreads();
+    if (diff) s_any = 1u;
+    __syncthreads();
+    if (tid == 0) {
+        types[n] = (uint8_t)s_any;
+        cc[n] = sample;
+        flag[n] = s_any;
+    }
+}
+
+__global__ void k_brick_assign(uint64_t nodes, const uint32_t* __restrict__ flag,
+                               const uint32_t* __restrict__ excl, uint32_t* __restrict__ bidx,
+                               uint32_t* __restrict__ brickNode) {
+    uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nodes) return;
+    if (flag[n]) {
+        bidx[n] = excl[n];
+        brickNode[excl[n]] = (uint32_t)n;
+    } else {
+        bidx[n] = 0xffffffffu;
+    }
+}
+
+// brick gather: WriteBinaryBrick / WriteUByteBinaryBrick middle-brick case
+// (VoxelBrickWriter.cu 725-818, 509-588): (B+2)^3 voxels, RGBA8 colour and
+// RGB8 normal ((n+1)*0.5*255, truncated).
+__global__ __launch_bounds__(kBlock) void k_brick_gather(
+    LevelMap lm, const uint32_t* __restrict__ col, const float* __restrict__ nrm,
+    const uint32_t* __restrict__ brickNode, uint32_t Rd, uint32_t Bx, uint32_t By, uint32_t Bz,
+    uint32_t bx, uint32_t by, uint32_t bz, uint32_t* __restrict__ bcol,
+    uint32_t* __restrict__ bnrm) {
+    const uint64_t j = blockIdx.x;
+    const uint64_t n = brickNode[j];
+    const uint32_t nx = (uint32_t)(n % Rd), ny = (uint32_t)((n / Rd) % Rd),
+                   nz = (uint32_t)(n / ((uint64_t)Rd * Rd));
+    const uint32_t r0x = nx * Bx + bx - 1, r0y = ny * By + by - 1, r0z = nz * Bz + bz - 1;
+    const uint32_t ex = Bx + 2, ey = By + 2, ez = Bz + 2;
+    const uint32_t nvox = ex * ey * ez;
+    for (uint32_t i = threadIdx.x; i < nvox; i += kBlock) {
+        const uint32_t X = r0x + i % ex, Y = r0y + (i / ex) % ey, Z = r0z + i / (ex * ey);
+        const int64_t ta = tile_of(lm, X, Y, Z);
+        uint32_t c = 0u;
+        f3 nv = {0.0f, 0.0f, 0.0f};
+        if (ta >= 0) {
+            const uint32_t v = vox_of(X, Y, Z);
+            c = col[(uint64_t)ta * kTileVox + v];
+            nv = {nrm[((uint64_t)ta * 3 + 0) * kTileVox + v], nrm[((uint64_t)ta * 3 + 1) * kTileVox + v],
+                  nrm[((uint64_t)ta * 3 + 2) * kTileVox + v]};
+        }
+        float qx = ((nv.x + 1.0f) * 0.5f), qy = ((nv.y + 1.0f) * 0.5f), qz = ((nv.z + 1.0f) * 0.5f);
+        bcol[j * nvox + i] = c;
+        bnrm[j * nvox + i] = pack4(quant_u8(qx * 255.0f), quant_u8(qy * 255.0f), quant_u8(qz * 255.0f), 0u);
+    }
+}
+
+// CollapseConstantBranches (Voxelizer.cu 701-774), one level at a time,
+// leaves first.
+__global__ void k_collapse(uint32_t Rd, int childIsLeaf, uint8_t* __restrict__ types,
+                           const uint8_t* __restrict__ childTypes) {
+    uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nn = (uint64_t)Rd * Rd * Rd;
+    if (n >= nn) return;
+    if (types[n] != 0) return;
+    const uint32_t x = (uint32_t)(n % Rd), y = (uint32_t)((n / Rd) % Rd),
+                   z = (uint32_t)(n / ((uint64_t)Rd * Rd));
+    const uint32_t Rc = Rd * 2;
+    bool all = true;
+    for (uint32_t c = 0; c < 8; ++c) {
+        const uint32_t cx = 2 * x + (c & 1u), cy = 2 * y + ((c >> 1) & 1u), cz = 2 * z + (c >> 2);
+        const uint8_t t = childTypes[((uint64_t)cz * Rc + cy) * Rc + cx];
+        all = all && (childIsLeaf ? (t == 0) : (t == 2));
+    }
+    if (all) types[n] = 2;
+}
+
+__global__ __launch_bounds__(kBlock) void k_occupancy(VoxParams P, const int32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ occ,
+                                                      uint32_t* __restrict__ bitmap) {
+    const uint64_t a = blockIdx.x;
+    const uint64_t tlin = (uint64_t)list[a];
+    const uint32_t tx = (uint32_t)(tlin % P.TG[0]);
+    const uint32_t ty = (uint32_t)((tlin / P.TG[0]) % P.TG[1]);
+    const uint32_t tz = (uint32_t)(tlin / ((uint64_t)P.TG[0] * P.TG[1]));
+    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+        const uint32_t v = (uint32_t)(pass * kBlock + threadIdx.x);
+        if (!((occ[a * kTileWords + (v >> 5)] >> (v & 31u)) & 1u)) continue;
+        const uint64_t X = tx * kTile + (v & 7u), Y = ty * kTile + ((v >> 3) & 7u),
+                       Z = tz * kTile + (v >> 6);
+        const uint64_t i = (Z * P.S[1] + Y) * P.S[0] + X;
+        atomicOr(&bitmap[i >> 5], 1u << (i & 31u));
+    }
+}
+
+__global__ void k_dense_level(LevelMap lm, const uint32_t* __restrict__ col,
+                              const float* __restrict__ nrm, uint32_t* __restrict__ outCol,
+                              float* __restrict__ outNrm) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = (uint64_t)lm.dims[0] * lm.dims[1] * lm.dims[2];
+    if (i >= n) return;
+    const uint32_t X = (uint32_t)(i % lm.dims[0]), Y = (uint32_t)((i / lm.dims[0]) % lm.dims[1]),
+                   Z = (uint32_t)(i / ((uint64_t)lm.dims[0] * lm.dims[1]));
+    const int64_t ta = tile_of(lm, X, Y, Z);
+    uint32_t c = 0u;
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (ta >= 0) {
+        const uint32_t v = vox_of(X, Y, Z);
+        c = col[(uint64_t)ta * kTileVox + v];
+        nx = nrm[((uint64_t)ta * 3 + 0) * kTileVox + v];
+        ny = nrm[((uint64_t)ta * 3 + 1) * kTileVox + v];
+        nz = nrm[((uint64_t)ta * 3 + 2) * kTileVox + v];
+    }
+    outCol[i] = c;
+    outNrm[3 * i] = nx;
+    outNrm[3 * i + 1] = ny;
+    outNrm[3 * i + 2] = nz;
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan (reduce, scan of block sums, down-sweep).
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in,
+                                                        uint64_t n, uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_w[kBlock / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t tot;
+    block_excl_scan(s, s_w, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_sums(uint32_t* __restrict__ sums, uint64_t nb,
+                                                      uint32_t* __restrict__ out, uint64_t n) {
+    __shared__ uint32_t s_w[kBlock / 64];
+    uint32_t carry = 0;
+    for (uint64_t c0 = 0; c0 < nb; c0 += kBlock) {
+        const uint64_t i = c0 + threadIdx.x;
+        const uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t tot;
+        const uint32_t e = block_excl_scan(v, s_w, &tot);
+        if (i < nb) sums[i] = carry + e;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict__ in, uint64_t n,
+                                                      const uint32_t* __restrict__ sums,
+                                                      uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_w[kBlock / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t vals[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k;
+        vals[k] = i < n ? in[i] : 0u;
+        s += vals[k];
+    }
+    uint32_t tot;
+    uint32_t e = block_excl_scan(s, s_w, &tot) + sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k;
+        if (i < n) out[i] = e;
+        e += vals[k];
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+inline unsigned grid_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// Fill kernel instead of hipMemsetAsync so the pipeline stays capturable.
+void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
+    if (n == 0) return;
+    uint64_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    k_fill_u32<<<(unsigned)blocks, kBlock, 0, st>>>(p, n, v);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+void launch_setup(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
+                  const float* verts, float* fnorm, uint32_t* bbox, uint32_t* binCount) {
+    if (G.ntri == 0) return;
+    k_setup<<<grid_for(G.ntri, kBlock), kBlock, 0, st>>>(P, G, g, verts, fnorm, bbox, binCount);
+}
+void launch_scatter(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
+                    const uint32_t* bbox, const uint32_t* binStart, uint32_t* binFill,
+                    uint32_t* binTri) {
+    if (G.ntri == 0 || !G.overlaps) return;
+    k_scatter<<<grid_for(G.ntri, kBlock), kBlock, 0, st>>>(P, G, g, bbox, binStart, binFill, binTri);
+}
+void launch_activity0(hipStream_t st, const VoxParams& P, int ngroups, const int32_t* grpFill,
+                      const uint32_t* binStart, uint32_t* mark) {
+    k_activity0<<<grid_for(P.numTiles, kBlock), kBlock, 0, st>>>(P, ngroups, grpFill, binStart, mark);
+}
+void launch_compact(hipStream_t st, uint64_t n, const uint32_t* mark, const uint32_t* excl,
+                    int32_t* tileIdx, int32_t* list) {
+    if (n == 0) return;
+    k_compact<<<grid_for(n, kBlock), kBlock, 0, st>>>(n, mark, excl, tileIdx, list);
+}
+void launch_mark_parent(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint32_t* markDst) {
+    if (src.numTiles == 0) return;
+    k_mark_parent<<<grid_for(src.numTiles, kBlock), kBlock, 0, st>>>(src, dst, markDst);
+}
+void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g, int init,
+                     uint64_t nActive, const int32_t* list, const float* verts,
+                     const float* fnorm, const uint32_t* bbox, const uint32_t* binStart,
+                     const uint32_t* binTri, uint32_t* col, float* nrm, uint32_t* occ,
+                     uint32_t* hit, unsigned long long* satCounter) {
+    if (nActive == 0) return;
+    k_voxelize<<<(unsigned)nActive, kBlock, 0, st>>>(P, G, g, init, list, verts, fnorm, bbox,
+                                                     binStart, binTri, col, nrm, occ, hit,
+                                                     satCounter);
+}
+void launch_fill(hipStream_t st, const VoxParams& P, uint64_t nActive, const int32_t* list,
+                 const int32_t* tileIdx, uint32_t* col, float* nrm, const uint32_t* hit) {
+    if (nActive == 0) return;
+    k_fill<<<(unsigned)nActive, kBlock, 0, st>>>(P, list, tileIdx, col, nrm, hit);
+}
+void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
+                const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
+                uint32_t* colDst, float* nrmDst) {
+    if (nActive == 0) return;
+    k_mip<<<(unsigned)nActive, kBlock, 0, st>>>(src, dst, listDst, colSrc, nrmSrc, colDst, nrmDst);
+}
+void launch_node_type(hipStream_t st, const LevelMap& lm, const uint32_t* col, uint32_t Rd,
+                      const uint32_t B[3], const uint32_t border[3], uint32_t xLimit,
+                      uint8_t* types, uint32_t* cc, uint32_t* flag) {
+    const uint64_t nodes = (uint64_t)Rd * Rd * Rd;
+    k_node_type<<<(unsigned)nodes, kBlock, 0, st>>>(lm, col, Rd, B[0], B[1], B[2], border[0],
+                                                    border[1], border[2], xLimit, types, cc, flag);
+}
+void launch_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
+                         const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode) {
+    k_brick_assign<<<grid_for(nodes, kBlock), kBlock, 0, st>>>(nodes, flag, excl, bidx, brickNode);
+}
+void launch_brick_gather(hipStream_t st, const LevelMap& lm, const uint32_t* col,
+                         const float* nrm, uint64_t nbricks, const uint32_t* brickNode,
+                         uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
+                         uint32_t* bcol, uint32_t* bnrm) {
+    if (nbricks == 0) return;
+    k_brick_gather<<<(unsigned)nbricks, kBlock, 0, st>>>(lm, col, nrm, brickNode, Rd, B[0], B[1],
+                                                         B[2], border[0], border[1], border[2],
+                                                         bcol, bnrm);
+}
+void launch_collapse(hipStream_t st, uint32_t Rd, int childIsLeaf, uint8_t* types,
+                     const uint8_t* childTypes) {
+    const uint64_t nn = (uint64_t)Rd * Rd * Rd;
+    k_collapse<<<grid_for(nn, kBlock), kBlock, 0, st>>>(Rd, childIsLeaf, types, childTypes);
+}
+void launch_occupancy(hipStream_t st, const VoxParams& P, uint64_t nActive, const int32_t* list,
+                      const uint32_t* occ, uint32_t* bitmap) {
+    if (nActive == 0) return;
+    k_occupancy<<<(unsigned)nActive, kBlock, 0, st>>>(P, list, occ, bitmap);
+}
+void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
+                        const float* nrm, uint32_t* outCol, float* outNrm) {
+    const uint64_t n = (uint64_t)lm.dims[0] * lm.dims[1] * lm.dims[2];
+    if (n == 0) return;
+    k_dense_level<<<grid_for(n, kBlock), kBlock, 0, st>>>(lm, col, nrm, outCol, outNrm);
+}
+uint64_t scan_tmp_words(uint64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp) {
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) {
+        launch_fill_u32(st, out, 1, 0u);
+        return;
+    }
+    k_scan_reduce<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp);
+    k_scan_sums<<<1, kBlock, 0, st>>>(tmp, nb, out, n);
+    k_scan_down<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp, out);
+}
+
+}  // namespace voxmi
