@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Voxelizer benchmark: one JSON line on stdout.
+
+Workload (one GPU): the full-octree terrain scene -- 1025x1025 fBm heightfield
+(2,097,152 textured triangles) plus 2,000 icosphere rocks (160,000 untextured
+triangles) voxelized into a 1024^3 grid of 16^3 bricks (18^3 stored), octree
+depths 0-6, constant bricks culled.  A step is one pass of the hot path with
+the triangles already resident in HBM: triangle setup and binning
+(computeEdgesFaceNormalsAndBounds) followed by voxelization, colour, fill,
+mip chain, node typing, brick gather and collapse (generateVoxelsAndOctTree),
+ending with the bricks, node arrays and occupancy bitmap in HBM.
+
+value = N^3 voxels classified per second (Gvoxels/s).  The roofline object
+describes the kernel with the largest share of device time, timed with HIP
+events on the library's stream; its algorithmic bytes per launch are given by
+kernel_bytes() below.  cpu_baseline times the oracle (a scalar C++ port of
+the reference logic) on the same scene at 512^3 on one host core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gvoxels/s + bricks/s voxelized (HBM GB/s % peak); p99 brick-request latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(scene, st, res_bricks, B):
+    """Compulsory HBM bytes of one step (the pipeline formula):
+    96*T + 16*P + 8*leaf_nodes + N^3/8 + sum_k bricks_k*(B+2)^3*8
+    + sum_{k>=1} bricks_{k-1}*(B+2)^3*8 + 8*nodes."""
+    T = st.triangles
+    P = st.pairs
+    N = scene.N[0]
+    L = len(res_bricks)
+    bvox = (B + 2) ** 3
+    leaf_nodes = 8 ** (L - 1)
+    nodes = sum(8 ** d for d in range(L))
+    bricks = sum(res_bricks)
+    child_reads = sum(res_bricks[d + 1] for d in range(L - 1))
+    return (96 * T + 16 * P + 8 * leaf_nodes + N ** 3 // 8 + bricks * bvox * 8
+            + child_reads * bvox * 8 + 8 * nodes)
+
+
+def kernel_bytes(name, scene, st, bricks, B, launches):
+    """Algorithmic bytes per launch of the named kernel."""
+    bvox = (B + 2) ** 3
+    if name == "brick_gather":
+        # read (B+2)^3 colours + float normals, write RGBA8 colour + RGB8(+pad) normal
+        return sum(bricks) * bvox * (16 + 8) / launches
+    if name == "voxelize":
+        # triangle attributes read once (vertices 36 B, vertex normals 36 B,
+        # UVs 24 B where the group has them) + (triangle, tile) bin entries
+        # written and read (16 B each, the pipeline formula's 16*P)
+        tri_bytes = 0
+        for m in scene.meshes:
+            tri_bytes += m.ntri * (36 + (36 if m.vnormals is not None else 0)
+                                   + (24 if m.uvs is not None else 0))
+        return (tri_bytes + 16 * st.pairs) / launches
+    if name == "mip":
+        # each active level-k tile reads 8 source voxels and writes one, 16 B each
+        tiles = sum(st.active_tiles[k] for k in range(1, len(bricks)))
+        return tiles * 512 * 16 * 9 / launches
+    if name == "node_type":
+        return 0.0
+    return 0.0
+
+
+def cpu_baseline(sample_n: int):
+    import oracle
+    from voxvizthesis_amd import scenes
+    sc = scenes.scene_full_octree(cells=sample_n, N=sample_n, B=16)
+    oracle.build()
+    t0 = time.perf_counter()
+    r = oracle.run(sc, keep_mips=False)
+    dt = time.perf_counter() - t0
+    if r.status != 1:
+        raise RuntimeError("oracle sample failed")
+    return {
+        "value": round(sample_n ** 3 / dt / 1e9, 6),
+        "unit": "Gvoxels/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle (scalar C++ port of the reference logic) on the same scene "
+                   f"(seeds 2/3, {sample_n + 1}^2 terrain + 2,000 rocks) at {sample_n}^3, "
+                   f"16^3 bricks, {dt:.1f} s on one host core"),
+        "seconds": round(dt, 3),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=512)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the voxelizer "
+              f"shards by tile across GPUs by design but this build measures one GPU.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from voxvizthesis_amd import scenes
+    from voxvizthesis_amd._lib import VOX_OPT_TIMING
+    from voxvizthesis_amd.voxelizer import Voxelizer
+
+    t0 = time.perf_counter()
+    scene = scenes.scene_full_octree()
+    log(f"[bench] scene {scene.name}: {scene.ntri} triangles, generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    B = scene.B[0]
+    vz = Voxelizer(scene.N, scene.B, None)
+    ok = vz.initCuda() and vz.allocateVoxelMemory() and vz.setVoxelizationParams(scene.p,
+                                                                              scene.delta)
+    for m in scene.meshes:
+        ok = ok and vz.addMesh(m)
+    t_up = time.perf_counter()
+    ok = ok and vz.allocateTriangleMemory()           # H2D once: inputs resident in HBM
+    t_up = time.perf_counter() - t_up
+    if not ok:
+        raise RuntimeError(vz.getErrorMessage())
+
+    def step():
+        if not vz.computeEdgesFaceNormalsAndBounds():
+            raise RuntimeError(vz.getErrorMessage())
+        if vz.generateVoxelsAndOctTree(None) != 1:
+            raise RuntimeError(vz.getErrorMessage())
+
+    for i in range(args.warmup):
+        step()
+    log(f"[bench] warmup {args.warmup} done")
+    # generate() ends with a stream synchronize, so the timed region is
+    # bracketed by device synchronisation on both sides.
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    dt = time.perf_counter() - t0
+    ms = dt / args.steps * 1e3
+    st = vz.stats()
+    res = vz.result(with_bricks=False)
+    bricks = [t["nb"] for t in res["tree"]]
+    log(f"[bench] {args.steps} steps: {ms:.3f} ms/step, bricks {bricks}")
+
+    # one more step with per-kernel HIP events for the roofline
+    vz._L.vox_set_options(vz._h, VOX_OPT_TIMING)
+    step()
+    ts = vz.stats()
+    kern = {}
+    for i in range(ts.nkernels):
+        name = ts.kernel_name[i].value.decode()
+        base = name.split("_g")[0] if name.startswith("voxelize_g") else name
+        ms0, n0 = kern.get(base, (0.0, 0))
+        kern[base] = (ms0 + ts.kernel_ms[i], n0 + ts.kernel_launches[i])
+    kern_detail = {ts.kernel_name[i].value.decode(): round(ts.kernel_ms[i], 4)
+                   for i in range(ts.nkernels)}
+    vz._L.vox_set_options(vz._h, 0)
+    dom = max(kern.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dlaunch) = dom
+    per_launch_ms = dms / dlaunch
+    kb = kernel_bytes(dname, scene, st, bricks, B, dlaunch)
+    achieved = kb / (per_launch_ms * 1e-3) / 1e9
+    N = scene.N[0]
+    total_bytes = algorithmic_bytes(scene, st, bricks, B)
+    # Measured HBM bytes per launch of the kernel (rocprofv3 FETCH_SIZE +
+    # WRITE_SIZE passes over this workload, scaled by the WRITE_SIZE reading of
+    # the occupancy-bitmap fill of known size), written by
+    # `python -m voxvizthesis_amd.profiling traffic ... --json profiles/traffic.json`.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        kk = "k_" + dname
+        if kk in tj.get("kernels", {}):
+            traffic = int(tj["kernels"][kk]["bytes_per_launch"])
+    out = {
+        "metric": METRIC,
+        "value": round(N ** 3 / (ms * 1e-3) / 1e9, 3),
+        "unit": "Gvoxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm terrain + icosphere rocks)",
+        "config": {
+            "workload": "full octree: 1025^2 fBm terrain + 2,000 rocks, 1024^3 grid, 16^3 "
+                        "bricks, depths 0-6",
+            "triangles": int(st.triangles),
+            "grid": N,
+            "brick": B,
+            "levels": len(bricks),
+        },
+        "bricks_per_s": round(sum(bricks) / (ms * 1e-3), 1),
+        "bricks": bricks,
+        "pipeline_hbm_gbs": round(total_bytes / (ms * 1e-3) / 1e9, 1),
+        "pipeline_hbm_frac": round(total_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "pipeline_algorithmic_bytes": int(total_bytes),
+        "pcie_upload_s": round(t_up, 4),
+        "kernels_ms": kern_detail,
+        "roofline": {
+            "kernel": dname,
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "per_launch_ms": round(per_launch_ms, 4),
+            "algorithmic_bytes_per_launch": int(kb),
+        },
+    }
+    if not args.no_cpu_baseline:
+        log(f"[bench] cpu baseline on a {args.cpu_sample}^3 sample ...")
+        cb = cpu_baseline(args.cpu_sample)
+        out["cpu_baseline"] = cb
+        out["gpu_vs_cpu"] = round(out["value"] / cb["value"], 1)
+    vz.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

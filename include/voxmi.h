@@ -133,7 +133,9 @@ int vox_add_group_device(vox_ctx* h, const vox_group* g);
 /* cuda::Voxelizer::initCuda (Voxelizer.cu 297-322): device 0, own stream. */
 int vox_init(vox_ctx* h);
 /* cuda::Voxelizer::resetCuda (Voxelizer.cu 324-344): frees everything,
- * resets the device, reallocates voxel memory. */
+ * recreates the handle's stream and events and reallocates voxel memory.
+ * Unlike the reference it does not reset the device, which other users of
+ * the process may share. */
 int vox_reset(vox_ctx* h);
 /* cuda::Voxelizer::allocateVoxelMemory / deallocateVoxelMemory
  * (Voxelizer.cu 163-236). */

@@ -58,3 +58,16 @@ def test_create_validates_dimensions_without_gpu():
         assert lib.vox_get_error(h) == b""
     finally:
         lib.vox_destroy(h)
+
+
+def test_cpp_mirror_header_compiles_and_reports(tmp_path):
+    """include/voxmi_voxelizer.hpp builds with main.cu-style argument types
+    and reports construction errors like cuda::Voxelizer::getErrorMessage."""
+    exe = tmp_path / "hpp_mirror_check"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "hpp_mirror_check.cpp"), "-L", libdir,
+                    "-lvoxmi", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "extra 8 8 8" in r.stdout

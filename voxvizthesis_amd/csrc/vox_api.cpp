@@ -60,7 +60,21 @@ struct HostGroup {
 struct LevelBufs {
     LevelMap map{};
     uint64_t active = 0;
-    DevBuf mark, excl, tileIdx, list, col, nrm;
+    int aliasOf = -1;     // level whose tiles hold this level's voxels
+    DevBuf mark, excl, tileIdx, list, col, qn, nrm;
+};
+
+// A level as the kernels see it.  In the reference's mip recursion level
+// 2j (j >= 1) is built from level 2j-2 with the same 2x2x2 addressing that
+// built level 2j-1 from the same source, so it equals level 2j-1 on its
+// smaller extent; such a level reads level 2j-1's tiles with its own dims.
+struct LevelView {
+    LevelMap map;
+    uint64_t active;
+    int32_t* list;
+    uint32_t* col;
+    uint32_t* qn;
+    float* nrm;
 };
 
 struct DepthBufs {
@@ -103,7 +117,8 @@ struct vox_ctx {
     // levels
     VoxParams P{};
     std::vector<LevelBufs> lv;
-    DevBuf occ0, hit0, sat;
+    DevBuf occ0, hit0, sat, work, workCount, rec, recCount;
+    unsigned persistentBlocks = 2048;
     std::vector<DepthBufs> tree;
     DevBuf bitmap;
     DevBuf counters;  // small host-read counters
@@ -131,7 +146,7 @@ struct vox_ctx {
         return eventPool[eventNext++];
     }
     // Record a named span around launches when timing is on.
-    template <class F> void run(const char* name, F&& f) {
+    template <class F> void run(const std::string& name, F&& f) {
         if (options & VOX_OPT_TIMING) {
             hipEvent_t a = ev(), b = ev();
             (void)hipEventRecord(a, st);
@@ -145,6 +160,20 @@ struct vox_ctx {
 };
 
 namespace {
+
+LevelView view(vox_ctx* h, int k) {
+    LevelBufs& l = h->lv[k];
+    LevelBufs& b = h->lv[l.aliasOf >= 0 ? l.aliasOf : k];
+    LevelView v;
+    v.map = b.map;
+    for (int a = 0; a < 3; ++a) v.map.dims[a] = l.map.dims[a];
+    v.active = b.active;
+    v.list = b.list.as<int32_t>();
+    v.col = b.col.as<uint32_t>();
+    v.qn = b.qn.as<uint32_t>();
+    v.nrm = b.nrm.as<float>();
+    return v;
+}
 
 inline float gclampf(float x, float lo, float hi) {
     float m = x < hi ? x : hi;
@@ -331,6 +360,9 @@ int vox_init(vox_ctx* h) {
     if (!h->check(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking), "hipStreamCreate")) return 0;
     (void)hipEventCreate(&h->evStart);
     (void)hipEventCreate(&h->evStop);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) == hipSuccess && cus > 0)
+        h->persistentBlocks = (unsigned)cus * 8u;
     h->inited = true;
     return 1;
 }
@@ -385,7 +417,7 @@ void vox_deallocate_voxel_memory(vox_ctx* h) {
     if (h->st) (void)hipStreamSynchronize(h->st);
     for (auto& l : h->lv) {
         l.mark.release(); l.excl.release(); l.tileIdx.release(); l.list.release();
-        l.col.release(); l.nrm.release();
+        l.col.release(); l.qn.release(); l.nrm.release();
     }
     h->lv.clear();
     for (auto& t : h->tree) {
@@ -394,6 +426,7 @@ void vox_deallocate_voxel_memory(vox_ctx* h) {
     }
     h->tree.clear();
     h->occ0.release(); h->hit0.release(); h->sat.release(); h->bitmap.release();
+    h->work.release(); h->workCount.release(); h->rec.release(); h->recCount.release();
     h->counters.release(); h->binCount.release(); h->binStart.release(); h->binTri.release();
     h->scanTmp.release();
     h->voxMem = false;
@@ -572,8 +605,10 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     // dense tile list is sized for every tile of the level.
     for (int k = 0; k < L; ++k) {
         LevelBufs& l = h->lv[k];
+        l.aliasOf = (!chain && k >= 2 && (k % 2) == 0) ? k - 1 : -1;
+        if (l.aliasOf >= 0) continue;
         if (k > 0) {
-            LevelBufs& s = h->lv[srcOf(k)];
+            LevelView s = view(h, srcOf(k));
             h->run("fill", [&] { launch_fill_u32(st, l.mark.as<uint32_t>(), l.map.numTiles, 0u); });
             h->run("mark", [&] { launch_mark_parent(st, s.map, l.map, l.mark.as<uint32_t>()); });
         }
@@ -592,15 +627,38 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         return 0;
     for (int k = 0; k < L; ++k) {
         LevelBufs& l = h->lv[k];
+        if (l.aliasOf >= 0) {
+            l.active = 0;
+            h->stats.active_tiles[k] = h->lv[l.aliasOf].active;
+            continue;
+        }
         l.active = act[k];
         h->stats.active_tiles[k] = act[k];
         if (!l.col.ensure((uint64_t)act[k] * kTileVox * 4, h->err, "tile colours") ||
+            !l.qn.ensure((uint64_t)act[k] * kTileVox * 4, h->err, "tile brick normals") ||
             !l.nrm.ensure((uint64_t)act[k] * kTileVox * 12, h->err, "tile normals"))
             return 0;
     }
     if (!h->occ0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "occupancy words") ||
-        !h->hit0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "hit words"))
+        !h->hit0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "hit words") ||
+        !h->work.ensure((uint64_t)l0.active * 4, h->err, "tile work list") ||
+        !h->workCount.ensure(4, h->err, "tile work count") ||
+        !h->rec.ensure((uint64_t)l0.active * kTileVox * 32, h->err, "hit records") ||
+        !h->recCount.ensure(4, h->err, "hit record count"))
         return 0;
+    h->run("fill", [&] { launch_fill_u32(st, h->occ0.as<uint32_t>(), l0.active * kTileWords, 0u); });
+    if (firstOverlap >= 0) {
+        const GroupArgs& g0 = h->gargs[firstOverlap];
+        if (!(g0.textured && g0.terrain)) {
+            // tiles without triangles of the first group start as zero voxels
+            h->run("fill", [&] {
+                launch_fill_u32(st, l0.col.as<uint32_t>(), l0.active * kTileVox, 0u);
+                launch_fill_u32(st, l0.qn.as<uint32_t>(), l0.active * kTileVox, 0x007f7f7fu);
+                launch_fill_u32(st, reinterpret_cast<uint32_t*>(l0.nrm.as<float>()),
+                                l0.active * kTileVox * 3, 0u);
+            });
+        }
+    }
 
     // Voxelize, colour and fill, group by group (Voxelizer.cu 2335-2553).
     unsigned long long* satc = h->sat.as<unsigned long long>();
@@ -608,31 +666,42 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     for (int g = 0; g < G; ++g) {
         const GroupArgs& ga = h->gargs[g];
         if (!ga.overlaps) continue;
-        h->run("voxelize", [&] {
+        h->run("fill", [&] { launch_fill_u32(st, h->hit0.as<uint32_t>(), l0.active * kTileWords, 0u); });
+        h->run("group_tiles", [&] {
+            launch_group_tiles(st, P, g, l0.active, l0.list.as<int32_t>(), h->binStart.as<uint32_t>(),
+                               h->work.as<int32_t>(), h->workCount.as<uint32_t>(),
+                               h->recCount.as<uint32_t>());
+        });
+        h->run("voxelize_g" + std::to_string(g), [&] {
             launch_voxelize(st, P, ga, g, g == firstOverlap ? 1 : 0, l0.active, l0.list.as<int32_t>(),
                             h->verts.as<float>(), h->fnorm.as<float>(), h->bbox.as<uint32_t>(),
                             h->binStart.as<uint32_t>(), h->binTri.as<uint32_t>(),
-                            l0.col.as<uint32_t>(), l0.nrm.as<float>(), h->occ0.as<uint32_t>(),
-                            h->hit0.as<uint32_t>(), satc);
+                            h->occ0.as<uint32_t>(), h->hit0.as<uint32_t>(), satc,
+                            h->work.as<int32_t>(), h->workCount.as<uint32_t>(), h->rec.as<uint4>(),
+                            h->recCount.as<uint32_t>(), h->persistentBlocks);
+        });
+        h->run("colour_g" + std::to_string(g), [&] {
+            launch_colour(st, P, ga, g == firstOverlap ? 1 : 0, h->rec.as<uint4>(),
+                          h->recCount.as<uint32_t>(), l0.list.as<int32_t>(), h->verts.as<float>(),
+                          h->fnorm.as<float>(), l0.col.as<uint32_t>(), l0.qn.as<uint32_t>(),
+                          l0.nrm.as<float>(), h->persistentBlocks);
         });
         if (ga.textured && ga.terrain) {
             h->run("fill_terrain", [&] {
-                launch_fill(st, P, l0.active, l0.list.as<int32_t>(), l0.tileIdx.as<int32_t>(),
-                            l0.col.as<uint32_t>(), l0.nrm.as<float>(), h->hit0.as<uint32_t>());
+                launch_fill(st, P, g, g == firstOverlap ? 1 : 0, l0.active, l0.list.as<int32_t>(),
+                            l0.tileIdx.as<int32_t>(), h->binStart.as<uint32_t>(),
+                            l0.col.as<uint32_t>(), l0.qn.as<uint32_t>(), l0.nrm.as<float>(),
+                            h->hit0.as<uint32_t>());
             });
         }
     }
-    if (firstOverlap < 0 && l0.active > 0) {
-        // no group overlaps: tiles cannot be active; keep pools defined anyway
-        h->run("fill", [&] { launch_fill_u32(st, l0.col.as<uint32_t>(), l0.active * kTileVox, 0u); });
-    }
     // Mip chain (computeMipMaps, Voxelizer.cu 491-594).
     for (int k = 1; k < L; ++k) {
-        LevelBufs& s = h->lv[srcOf(k)];
-        LevelBufs& l = h->lv[k];
+        if (h->lv[k].aliasOf >= 0) continue;
+        LevelView s = view(h, srcOf(k));
+        LevelView l = view(h, k);
         h->run("mip", [&] {
-            launch_mip(st, s.map, l.map, l.active, l.list.as<int32_t>(), s.col.as<uint32_t>(),
-                       s.nrm.as<float>(), l.col.as<uint32_t>(), l.nrm.as<float>());
+            launch_mip(st, s.map, l.map, l.active, l.list, s.col, s.nrm, l.col, l.qn, l.nrm);
         });
     }
     // Node typing per depth (computeOctTreeNodes, Voxelizer.cu 931-1290).
@@ -640,7 +709,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     std::vector<std::array<uint32_t, 3>> borders(L);
     for (int d = L - 1; d >= 0; --d) {
         int k = L - 1 - d;
-        LevelBufs& l = h->lv[k];
+        LevelView l = view(h, k);
         DepthBufs& t = h->tree[d];
         borders[d] = {border[0], border[1], border[2]};
         const uint32_t Ckyz[2] = {l.map.dims[1], l.map.dims[2]};
@@ -648,8 +717,8 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         const uint32_t fyz[2] = {t.Rd * h->B[1], t.Rd * h->B[2]};
         uint32_t xLimit = type_x_limit(l.map.dims[0], border[0], t.Rd * h->B[0], Ckyz, byz, fyz);
         h->run("node_type", [&] {
-            launch_node_type(st, l.map, l.col.as<uint32_t>(), t.Rd, h->B, border, xLimit,
-                             t.types.as<uint8_t>(), t.cc.as<uint32_t>(), t.flag.as<uint32_t>());
+            launch_node_type(st, l.map, l.active, l.list, l.col, t.Rd, h->B, border, xLimit,
+                             t.cc.as<uint32_t>(), t.flag.as<uint32_t>());
         });
         h->run("scan", [&] { scan(h, t.flag.as<uint32_t>(), t.excl.as<uint32_t>(), t.nodes); });
         if (!h->check(hipMemcpyAsync(ctr + 32 + d, t.excl.as<uint32_t>() + t.nodes, 4,
@@ -664,7 +733,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     const uint64_t bvox = (uint64_t)(h->B[0] + 2) * (h->B[1] + 2) * (h->B[2] + 2);
     for (int d = L - 1; d >= 0; --d) {
         int k = L - 1 - d;
-        LevelBufs& l = h->lv[k];
+        LevelView l = view(h, k);
         DepthBufs& t = h->tree[d];
         t.bricks = nb[d];
         h->stats.bricks_total += nb[d];
@@ -674,11 +743,12 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
             return 0;
         h->run("brick_assign", [&] {
             launch_brick_assign(st, t.nodes, t.flag.as<uint32_t>(), t.excl.as<uint32_t>(),
-                                t.bidx.as<uint32_t>(), t.brickNode.as<uint32_t>());
+                                t.bidx.as<uint32_t>(), t.brickNode.as<uint32_t>(),
+                                t.types.as<uint8_t>());
         });
         uint32_t bd[3] = {borders[d][0], borders[d][1], borders[d][2]};
         h->run("brick_gather", [&] {
-            launch_brick_gather(st, l.map, l.col.as<uint32_t>(), l.nrm.as<float>(), nb[d],
+            launch_brick_gather(st, l.map, l.col, l.qn, nb[d],
                                 t.brickNode.as<uint32_t>(), t.Rd, h->B, bd, t.bcol.as<uint32_t>(),
                                 t.bnrm.as<uint32_t>());
         });
@@ -733,7 +803,10 @@ const char* vox_get_error(vox_ctx* h) {
 
 int vox_reset(vox_ctx* h) {
     if (!h) return 0;
-    // resetCuda (Voxelizer.cu 324-344): free, reset the device, reallocate.
+    // resetCuda (Voxelizer.cu 324-344) frees, resets the device and
+    // reallocates.  The device is not reset here: other users of the process
+    // (torch, other handles) own memory on it.  Freeing everything and
+    // recreating the stream and events leaves the handle in the same state.
     vox_deallocate_triangle_memory(h);
     vox_deallocate_voxel_memory(h);
     for (auto e : h->eventPool) (void)hipEventDestroy(e);
@@ -745,7 +818,6 @@ int vox_reset(vox_ctx* h) {
     if (h->st) (void)hipStreamDestroy(h->st);
     h->st = nullptr;
     h->inited = false;
-    if (!h->check(hipDeviceReset(), "hipDeviceReset")) return 0;
     if (!vox_init(h)) return 0;
     return vox_allocate_voxel_memory(h);
 }
@@ -796,14 +868,13 @@ int vox_copy_to_host(vox_ctx* h, void* dst, const void* src_device, uint64_t nby
 int vox_copy_level(vox_ctx* h, int32_t level, uint32_t* colors, float* normals, uint32_t dims[3]) {
     if (!h || !h->valid || !h->haveResult) return 0;
     if (level < 0 || level >= h->L) return h->fail("level out of range"), 0;
-    LevelBufs& l = h->lv[level];
+    LevelView l = view(h, level);
     for (int a = 0; a < 3; ++a) dims[a] = l.map.dims[a];
     const uint64_t n = (uint64_t)l.map.dims[0] * l.map.dims[1] * l.map.dims[2];
     DevBuf c, nr;
     if (!c.ensure(n * 4, h->err, "dense colours") || !nr.ensure(n * 12, h->err, "dense normals"))
         return 0;
-    launch_dense_level(h->st, l.map, l.col.as<uint32_t>(), l.nrm.as<float>(), c.as<uint32_t>(),
-                       nr.as<float>());
+    launch_dense_level(h->st, l.map, l.col, l.nrm, c.as<uint32_t>(), nr.as<float>());
     bool ok = h->check(hipMemcpyAsync(colors, c.p, n * 4, hipMemcpyDeviceToHost, h->st), "copy") &&
               h->check(hipMemcpyAsync(normals, nr.p, n * 12, hipMemcpyDeviceToHost, h->st), "copy") &&
               h->check(hipStreamSynchronize(h->st), "copy level");

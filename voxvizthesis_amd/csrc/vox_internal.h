@@ -58,20 +58,30 @@ void launch_mark_parent(hipStream_t st, const LevelMap& src, const LevelMap& dst
 void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g, int init,
                      uint64_t nActive, const int32_t* list, const float* verts,
                      const float* fnorm, const uint32_t* bbox, const uint32_t* binStart,
-                     const uint32_t* binTri, uint32_t* col, float* nrm, uint32_t* occ,
-                     uint32_t* hit, unsigned long long* satCounter);
-void launch_fill(hipStream_t st, const VoxParams& P, uint64_t nActive, const int32_t* list,
-                 const int32_t* tileIdx, uint32_t* col, float* nrm, const uint32_t* hit);
+                     const uint32_t* binTri, uint32_t* occ, uint32_t* hit,
+                     unsigned long long* satCounter, int32_t* work, uint32_t* workCount,
+                     uint4* rec, uint32_t* recCount, unsigned persistentBlocks);
+void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nActive,
+                        const int32_t* list, const uint32_t* binStart, int32_t* work,
+                        uint32_t* workCount, uint32_t* recCount);
+void launch_colour(hipStream_t st, const VoxParams& P, const GroupArgs& G, int init,
+                   const uint4* rec, const uint32_t* recCount, const int32_t* list,
+                   const float* verts, const float* fnorm, uint32_t* col, uint32_t* qn, float* nrm,
+                   unsigned persistentBlocks);
+void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t nActive,
+                 const int32_t* list, const int32_t* tileIdx, const uint32_t* binStart,
+                 uint32_t* col, uint32_t* qn, float* nrm, const uint32_t* hit);
 void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
                 const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
-                uint32_t* colDst, float* nrmDst);
-void launch_node_type(hipStream_t st, const LevelMap& lm, const uint32_t* col, uint32_t Rd,
-                      const uint32_t B[3], const uint32_t border[3], uint32_t xLimit,
-                      uint8_t* types, uint32_t* cc, uint32_t* flag);
+                uint32_t* colDst, uint32_t* qnDst, float* nrmDst);
+void launch_node_type(hipStream_t st, const LevelMap& lm, uint64_t nActive, const int32_t* list,
+                      const uint32_t* col, uint32_t Rd, const uint32_t B[3],
+                      const uint32_t border[3], uint32_t xLimit, uint32_t* cc, uint32_t* flag);
 void launch_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
-                         const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode);
+                         const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode,
+                         uint8_t* types);
 void launch_brick_gather(hipStream_t st, const LevelMap& lm, const uint32_t* col,
-                         const float* nrm, uint64_t nbricks, const uint32_t* brickNode,
+                         const uint32_t* qn, uint64_t nbricks, const uint32_t* brickNode,
                          uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
                          uint32_t* bcol, uint32_t* bnrm);
 void launch_collapse(hipStream_t st, uint32_t Rd, int childIsLeaf, uint8_t* types,

@@ -65,9 +65,34 @@ __device__ __forceinline__ uint32_t quant_u8(float f) {
 __device__ __forceinline__ uint32_t pack4(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
     return r | (g << 8) | (b << 16) | (a << 24);
 }
-__device__ __forceinline__ f4 unpack4(uint32_t c) {
-    return {(float)(c & 255u) / 255.0f, (float)((c >> 8) & 255u) / 255.0f,
-            (float)((c >> 16) & 255u) / 255.0f, (float)(c >> 24) / 255.0f};
+// VoxColorToVec4 (VoxelizerKernels.cu 657-663): c / 255.0f per channel,
+// read from a 256-entry table of the correctly rounded quotients (filled by
+// fill_inv255 at kernel start; a multiply by 1/255 differs in 126 entries).
+__device__ __forceinline__ f4 unpack4(uint32_t c, const float* __restrict__ inv255) {
+    return {inv255[c & 255u], inv255[(c >> 8) & 255u], inv255[(c >> 16) & 255u],
+            inv255[c >> 24]};
+}
+__device__ __forceinline__ void fill_inv255(float* s_inv255) {
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_inv255[i] = (float)i / 255.0f;
+}
+
+// Linear tile index -> tile coordinates (32-bit arithmetic; tile grids hold
+// fewer than 2^31 tiles).
+__device__ __forceinline__ void tile_xyz(uint64_t tlin64, const uint32_t* TG, uint32_t& x,
+                                         uint32_t& y, uint32_t& z) {
+    const uint32_t t = (uint32_t)tlin64;
+    const uint32_t q = t / TG[0];
+    x = t - q * TG[0];
+    z = q / TG[1];
+    y = q - z * TG[1];
+}
+// Node index at a depth with Rd = 2^ld nodes per axis.
+__device__ __forceinline__ void node_xyz(uint64_t n, uint32_t ld, uint32_t& x, uint32_t& y,
+                                         uint32_t& z) {
+    const uint64_t m = (1ull << ld) - 1ull;
+    x = (uint32_t)(n & m);
+    y = (uint32_t)((n >> ld) & m);
+    z = (uint32_t)(n >> (2 * ld));
 }
 __device__ __forceinline__ uint32_t to_color(f4 c) {
     return pack4(quant_u8(c.x * 255.0f), quant_u8(c.y * 255.0f), quant_u8(c.z * 255.0f),
@@ -75,6 +100,14 @@ __device__ __forceinline__ uint32_t to_color(f4 c) {
 }
 
 __device__ __forceinline__ f3 load3(const float* p) { return {p[0], p[1], p[2]}; }
+
+// Brick normal encoding of WriteUByteBinaryBrick (VoxelBrickWriter.cu
+// 557-568): ((n + 1) * 0.5) * 255 truncated, RGB8 in the low three bytes.
+__device__ __forceinline__ uint32_t quant_normal(f3 n) {
+    float qx = ((n.x + 1.0f) * 0.5f), qy = ((n.y + 1.0f) * 0.5f), qz = ((n.z + 1.0f) * 0.5f);
+    return pack4(quant_u8(qx * 255.0f), quant_u8(qy * 255.0f), quant_u8(qz * 255.0f), 0u);
+}
+constexpr uint32_t kZeroNormalQ = 0x007f7f7fu;   // quant_normal of (0, 0, 0)
 
 // Inclusive scan across a 64-wide wavefront.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
@@ -179,7 +212,8 @@ __global__ void k_activity0(VoxParams P, int ngroups, const int32_t* __restrict_
                             const uint32_t* __restrict__ binStart, uint32_t* __restrict__ mark) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.numTiles) return;
-    uint32_t tz = (uint32_t)(t / ((uint64_t)P.TG[0] * P.TG[1]));
+    uint32_t tx_, ty_, tz;
+    tile_xyz(t, P.TG, tx_, ty_, tz);
     uint32_t on = 0;
     for (int g = 0; g < ngroups && !on; ++g) {
         uint64_t key = (uint64_t)g * P.numTiles + t;
@@ -215,9 +249,8 @@ __global__ void k_mark_parent(LevelMap src, LevelMap dst, uint32_t* __restrict__
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= src.numTiles) return;
     if (src.tileIdx[t] < 0) return;
-    uint32_t tx = (uint32_t)(t % src.TG[0]);
-    uint32_t ty = (uint32_t)((t / src.TG[0]) % src.TG[1]);
-    uint32_t tz = (uint32_t)(t / ((uint64_t)src.TG[0] * src.TG[1]));
+    uint32_t tx, ty, tz;
+    tile_xyz(t, src.TG, tx, ty, tz);
     uint32_t px = tx >> 1, py = ty >> 1, pz = tz >> 1;
     if (px >= dst.TG[0] || py >= dst.TG[1] || pz >= dst.TG[2]) return;
     markDst[((uint64_t)pz * dst.TG[1] + py) * dst.TG[0] + px] = 1u;
@@ -227,7 +260,7 @@ __global__ void k_mark_parent(LevelMap src, LevelMap dst, uint32_t* __restrict__
 // voxelize: ComputeVoxelization (VoxelizerKernels.cu 349-530) for one group
 // on one tile, then ComputeColorsAndNormals / ComputeColorsAsNormals
 // (VoxelizerKernels.cu 689-1083) for the tile's hit voxels.
-constexpr int kStage = 256;     // triangles staged per round
+constexpr int kStage = 128;     // triangles staged per round
 constexpr int kTriStride = 33;  // floats per staged triangle (padded)
 
 // Bilinear tex2D with normalized coordinates (VoxelizerKernels.cu 586-600):
@@ -246,15 +279,16 @@ __device__ __forceinline__ int wrap_index(int i, int n, int mode, bool& border) 
     }
     return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
-__device__ __forceinline__ f4 texel(const GroupArgs& G, int i, int j) {
+__device__ __forceinline__ f4 texel(const GroupArgs& G, int i, int j,
+                                    const float* __restrict__ inv255) {
     bool border = false;
     int x = wrap_index(i, G.w, G.addr0, border);
     int y = wrap_index(j, G.h, G.addr1, border);
     if (border) return {0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t c = *reinterpret_cast<const uint32_t*>(G.img + 4 * ((size_t)y * (size_t)G.w + (size_t)x));
-    return unpack4(c);
+    return unpack4(c, inv255);
 }
-__device__ f4 tex2d(const GroupArgs& G, float u, float v) {
+__device__ f4 tex2d(const GroupArgs& G, float u, float v, const float* __restrict__ inv255) {
     if (!(u == u)) u = 0.0f;
     if (!(v == v)) v = 0.0f;
     float xb = u * (float)G.w - 0.5f;
@@ -264,8 +298,8 @@ __device__ f4 tex2d(const GroupArgs& G, float u, float v) {
     float ay = rintf((yb - fy) * 256.0f) / 256.0f;
     int i = (int)fmaxf(fminf(fx, 1.0e9f), -1.0e9f);
     int j = (int)fmaxf(fminf(fy, 1.0e9f), -1.0e9f);
-    f4 t00 = texel(G, i, j), t10 = texel(G, i + 1, j);
-    f4 t01 = texel(G, i, j + 1), t11 = texel(G, i + 1, j + 1);
+    f4 t00 = texel(G, i, j, inv255), t10 = texel(G, i + 1, j, inv255);
+    f4 t01 = texel(G, i, j + 1, inv255), t11 = texel(G, i + 1, j + 1, inv255);
     float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay);
     float w01 = (1.0f - ax) * ay, w11 = ax * ay;
     f4 r;
@@ -305,8 +339,9 @@ __device__ __forceinline__ f2 bary_axis(int axis, f3 c, const f3* v) {
 // slots: up to 4 global triangle indices in reference slot order.
 __device__ void colour_update(const GroupArgs& G, const float* __restrict__ verts,
                               const float* __restrict__ fnorm, f3 voxCenter, uint32_t count,
-                              const int32_t* slots, int nslots, uint32_t& col, f3& nrm) {
-    f4 color = unpack4(col);
+                              const int32_t* slots, int nslots, uint32_t& col, f3& nrm,
+                              const float* __restrict__ inv255) {
+    f4 color = unpack4(col, inv255);
     f3 normal = nrm;
     if (G.textured) {
         uint32_t triCount = count < 4u ? count : 4u;
@@ -332,7 +367,7 @@ __device__ void colour_update(const GroupArgs& G, const float* __restrict__ vert
             f2 Vv = {(uv2.x - uv1.x) * bc.y, (uv2.y - uv1.y) * bc.y};
             f2 Uu = {(uv3.x - uv1.x) * bc.x, (uv3.y - uv1.y) * bc.x};
             f2 tc = {uv1.x + Uu.x + Vv.x, uv1.y + Uu.y + Vv.y};
-            f4 tx = tex2d(G, tc.x, tc.y);
+            f4 tx = tex2d(G, tc.x, tc.y, inv255);
             if ((double)tx.w <= 0.005) {
                 --triCount;
             } else {
@@ -394,12 +429,35 @@ __device__ void colour_update(const GroupArgs& G, const float* __restrict__ vert
     nrm = normal;
 }
 
+// Edge-test constants of TriangleOverlapsVoxelOnAxis (VK:93-144) for one
+// axis pair with compile-time axes: per edge i, the per-voxel test is
+// dp + uvw[A2] * vpy + uvw[A1] * vpx >= 0.
+template <int A1, int A2, int OT>
+__device__ __forceinline__ void stage_pair(const float (&v)[3][3], const float (&n)[3],
+                                           const float* pp, const float* dd, float* st) {
+    const float ps = n[OT] >= 0.0f ? 1.0f : -1.0f;
+    const float e[3][3] = {{v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]},
+                           {v[2][0] - v[1][0], v[2][1] - v[1][1], v[2][2] - v[1][2]},
+                           {v[0][0] - v[2][0], v[0][1] - v[2][1], v[0][2] - v[2][2]}};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float enx = (-e[i][A2]) * ps, eny = e[i][A1] * ps;
+        const float vpx = enx * dd[A1], vpy = eny * dd[A2];
+        const float dpp = ((-enx) * v[i][A1] + (-eny) * v[i][A2]) + (enx * pp[A1] + eny * pp[A2]) +
+                          gmax(0.0f, dd[A1] * enx) + gmax(0.0f, dd[A2] * eny);
+        st[i * 3 + 0] = dpp;
+        st[i * 3 + 1] = vpx;
+        st[i * 3 + 2] = vpy;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_voxelize(
-    VoxParams P, GroupArgs G, int g, int init, const int32_t* __restrict__ list,
+    VoxParams P, GroupArgs G, int g, int init, const int32_t* __restrict__ work,
+    const uint32_t* __restrict__ workCount, const int32_t* __restrict__ list,
     const float* __restrict__ verts, const float* __restrict__ fnorm,
     const uint32_t* __restrict__ bbox, const uint32_t* __restrict__ binStart,
-    const uint32_t* __restrict__ binTri, uint32_t* __restrict__ col, float* __restrict__ nrm,
-    uint32_t* __restrict__ occ, uint32_t* __restrict__ hit,
+    const uint32_t* __restrict__ binTri, uint32_t* __restrict__ occ, uint32_t* __restrict__ hit,
+    uint4* __restrict__ rec, uint32_t* __restrict__ recCount,
     unsigned long long* __restrict__ satCounter) {
     __shared__ uint32_t s_cnt[kTileVox];
     __shared__ int32_t s_lo0[kTileVox], s_lo1[kTileVox], s_lo2[kTileVox], s_hi[kTileVox];
@@ -409,23 +467,29 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
     __shared__ uint32_t s_off[kStage + 1];
     __shared__ uint32_t s_w[kBlock / 64];
 
+    __shared__ uint32_t s_magic[65];   // ceil(65536 / d) for d = 1..64
+    __shared__ uint32_t s_hitw[kTileWords], s_hitoff[kTileWords], s_occprev[kTileWords];
+    __shared__ uint32_t s_recBase;
     const int tid = threadIdx.x;
-    const uint64_t a = blockIdx.x;
+    if (tid >= 1 && tid <= 64) s_magic[tid] = (65536u + (uint32_t)tid - 1u) / (uint32_t)tid;
+    const uint32_t nwork = *workCount;
+    uint32_t localTests = 0;
+    // Persistent grid over the tiles that hold triangles of this group.
+    for (uint32_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const uint64_t a = (uint64_t)work[wi];
     const uint64_t tlin = (uint64_t)list[a];
-    const uint32_t tx = (uint32_t)(tlin % P.TG[0]);
-    const uint32_t ty = (uint32_t)((tlin / P.TG[0]) % P.TG[1]);
-    const uint32_t tz = (uint32_t)(tlin / ((uint64_t)P.TG[0] * P.TG[1]));
+    uint32_t tx, ty, tz;
+    tile_xyz(tlin, P.TG, tx, ty, tz);
     const uint32_t ox = tx * kTile, oy = ty * kTile, oz = tz * kTile;
-
+    const uint64_t key = (uint64_t)g * P.numTiles + tlin;
+    const uint32_t bStart = binStart[key];
+    const uint32_t nb = binStart[key + 1] - bStart;
     for (int v = tid; v < kTileVox; v += kBlock) {
         s_cnt[v] = 0u;
         s_lo0[v] = s_lo1[v] = s_lo2[v] = 0x7fffffff;
         s_hi[v] = -1;
     }
-    const uint64_t key = (uint64_t)g * P.numTiles + tlin;
-    const uint32_t bStart = binStart[key];
-    const uint32_t nb = binStart[key + 1] - bStart;
-    uint32_t localTests = 0;
+    if (tid < kTileWords) s_occprev[tid] = init ? 0u : occ[a * kTileWords + tid];
     __syncthreads();
 
     for (uint32_t c0 = 0; c0 < nb; c0 += kStage) {
@@ -451,56 +515,75 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
                 s_box[tid] = 0u;
             }
             s_tid[tid] = t;
-            // Plane test constants (TrianglePlaneOverlapsVoxel, VK:17-53).
-            const float* q = verts + 9 * (size_t)t;
-            f3 v[3] = {load3(q), load3(q + 3), load3(q + 6)};
-            f3 n = load3(fnorm + 3 * (size_t)t);
-            float* st = &s_tri[tid * kTriStride];
-            st[0] = n.x; st[1] = n.y; st[2] = n.z;
-            st[3] = dot3(f3{-n.x, -n.y, -n.z}, v[0]);
-            // Edge-test constants (TriangleOverlapsVoxelOnAxis, VK:93-144),
-            // pairs XY, ZX, YZ.
-            f3 e[3] = {sub3(v[1], v[0]), sub3(v[2], v[1]), sub3(v[0], v[2])};
-            const int A1[3] = {0, 2, 1}, A2[3] = {1, 0, 2}, OT[3] = {2, 1, 0};
-            const f3 pp = {P.offP[0], P.offP[1], P.offP[2]};
-            const f3 dd = {P.dP[0], P.dP[1], P.dP[2]};
-#pragma unroll
-            for (int pr = 0; pr < 3; ++pr) {
-                const int a1 = A1[pr], a2 = A2[pr], other = OT[pr];
-                float ps = cget(n, other) >= 0.0f ? 1.0f : -1.0f;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    float enx = (-cget(e[i], a2)) * ps, eny = cget(e[i], a1) * ps;
-                    float vpx = enx * cget(dd, a1), vpy = eny * cget(dd, a2);
-                    float dpp = ((-enx) * cget(v[i], a1) + (-eny) * cget(v[i], a2)) +
-                                (enx * cget(pp, a1) + eny * cget(pp, a2)) +
-                                gmax(0.0f, cget(dd, a1) * enx) + gmax(0.0f, cget(dd, a2) * eny);
-                    st[4 + (pr * 3 + i) * 3 + 0] = dpp;
-                    st[4 + (pr * 3 + i) * 3 + 1] = vpx;
-                    st[4 + (pr * 3 + i) * 3 + 2] = vpy;
-                }
-            }
         }
         uint32_t total;
-        uint32_t off = block_excl_scan(w, s_w, &total);
+        uint32_t off = block_excl_scan(w, s_w, &total);   // barrier: s_tid visible
         if ((uint32_t)tid < m) s_off[tid] = off;
         if (tid == 0) s_off[m] = total;
+        // Per (triangle, axis pair): the edge-test constants of
+        // TriangleOverlapsVoxelOnAxis (VK:93-144), pairs XY, ZX, YZ; pair 0
+        // also holds the plane-test constants of TrianglePlaneOverlapsVoxel
+        // (VK:17-53).  The per-voxel work then needs 2 mul + 2 add per edge.
+        // Items ordered pair-major so that a wavefront mostly shares one pair.
+        for (uint32_t it = tid; it < 3u * m; it += kBlock) {
+            const uint32_t pr = (it >= m ? 1u : 0u) + (it >= 2u * m ? 1u : 0u);
+            const uint32_t j = it - pr * m;
+            const uint32_t t = s_tid[j];
+            const float* q = verts + 9 * (size_t)t;
+            const float v[3][3] = {{q[0], q[1], q[2]}, {q[3], q[4], q[5]}, {q[6], q[7], q[8]}};
+            const float* fq = fnorm + 3 * (size_t)t;
+            const float n[3] = {fq[0], fq[1], fq[2]};
+            float* st = &s_tri[j * kTriStride];
+            if (pr == 0u) {
+                st[0] = n[0]; st[1] = n[1]; st[2] = n[2];
+                st[3] = dot3(f3{-n[0], -n[1], -n[2]}, f3{v[0][0], v[0][1], v[0][2]});
+                stage_pair<0, 1, 2>(v, n, P.offP, P.dP, st + 4);
+            } else if (pr == 1u) {
+                stage_pair<2, 0, 1>(v, n, P.offP, P.dP, st + 13);
+            } else {
+                stage_pair<1, 2, 0>(v, n, P.offP, P.dP, st + 22);
+            }
+        }
         __syncthreads();
-        localTests += 0;
-        for (uint32_t item = tid; item < total; item += kBlock) {
-            // triangle j with s_off[j] <= item < s_off[j+1]
+        // Each thread takes a contiguous range of (triangle, voxel) items: one
+        // search for its first triangle, then an incremental walk over the
+        // clipped bounds (x fastest), moving to the next triangle at its end.
+        const uint32_t per = (total + kBlock - 1) / kBlock;
+        uint32_t item = (uint32_t)tid * per;
+        const uint32_t itemEnd = item + per < total ? item + per : total;
+        uint32_t j = 0, bx = 0, ex = 1, ey = 1, cx = 0, cy = 0, cz = 0, jEnd = 0;
+        if (item < itemEnd) {
             uint32_t lo = 0, hi = m;
             while (hi - lo > 1) {
-                uint32_t mid = (lo + hi) >> 1;
+                const uint32_t mid = (lo + hi) >> 1;
                 if (s_off[mid] <= item) lo = mid; else hi = mid;
             }
-            const uint32_t j = lo;
+            j = lo;
             const uint32_t r = item - s_off[j];
-            const uint32_t bx = s_box[j];
-            const uint32_t ex = ((bx >> 9) & 7u) + 1u, ey = ((bx >> 12) & 7u) + 1u;
-            const uint32_t lx = (bx & 7u) + r % ex;
-            const uint32_t ly = ((bx >> 3) & 7u) + (r / ex) % ey;
-            const uint32_t lz = ((bx >> 6) & 7u) + r / (ex * ey);
+            bx = s_box[j];
+            ex = ((bx >> 9) & 7u) + 1u;
+            ey = ((bx >> 12) & 7u) + 1u;
+            // r < 512: (r * ceil(65536 / d)) >> 16 == r / d for d <= 64
+            const uint32_t exy = ex * ey;
+            cz = (r * s_magic[exy]) >> 16;
+            const uint32_t rxy = r - cz * exy;
+            cy = (rxy * s_magic[ex]) >> 16;
+            cx = rxy - cy * ex;
+            jEnd = s_off[j + 1];
+        }
+        for (; item < itemEnd; ++item) {
+            if (item == jEnd) {
+                // next triangle with a non-empty clipped box
+                do { ++j; jEnd = s_off[j + 1]; } while (jEnd == item);
+                bx = s_box[j];
+                ex = ((bx >> 9) & 7u) + 1u;
+                ey = ((bx >> 12) & 7u) + 1u;
+                cx = cy = cz = 0;
+            }
+            const uint32_t lx = (bx & 7u) + cx;
+            const uint32_t ly = ((bx >> 3) & 7u) + cy;
+            const uint32_t lz = ((bx >> 6) & 7u) + cz;
+            if (++cx == ex) { cx = 0; if (++cy == ey) { cy = 0; ++cz; } }
             const float* st = &s_tri[j * kTriStride];
             const float uvw[3] = {(float)(ox + lx), (float)(oy + ly), (float)(oz + lz)};
             ++localTests;
@@ -555,82 +638,135 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
         }
         __syncthreads();
     }
+
+    // Hit words by wave ballot, then one record per hit voxel (tile, voxel,
+    // count, slots in reference order) for the colour pass.
+    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+        const uint32_t v = (uint32_t)(pass * kBlock + tid);
+        const uint64_t mask = __ballot(s_cnt[v] > 0u);
+        if ((tid & 63) == 0) {
+            const uint32_t w0 = v >> 5;  // even word index
+            s_hitw[w0] = (uint32_t)mask;
+            s_hitw[w0 + 1] = (uint32_t)(mask >> 32);
+        }
+    }
+    __syncthreads();
+    if (tid < kTileWords) {
+        const uint32_t w = s_hitw[tid];
+        s_hitoff[tid] = (uint32_t)__popc(w);
+        hit[a * kTileWords + tid] = w;
+        occ[a * kTileWords + tid] |= w;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < kTileWords; ++w) { const uint32_t c = s_hitoff[w]; s_hitoff[w] = acc; acc += c; }
+        s_recBase = acc ? atomicAdd(recCount, acc) : 0u;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+        const uint32_t v = (uint32_t)(pass * kBlock + tid);
+        const uint32_t w = s_hitw[v >> 5];
+        if ((w >> (v & 31u)) & 1u) {
+            const uint32_t below = w & ((1u << (v & 31u)) - 1u);
+            const uint32_t h = s_hitoff[v >> 5] + (uint32_t)__popc(below);
+            const uint32_t cnt = s_cnt[v];
+            const uint32_t wasHit = (s_occprev[v >> 5] >> (v & 31u)) & 1u;
+            int32_t sl[4] = {-1, -1, -1, -1};
+            const uint32_t k3 = cnt < 3u ? cnt : 3u;
+            if (k3 > 0u) sl[0] = s_lo0[v];
+            if (k3 > 1u) sl[1] = s_lo1[v];
+            if (k3 > 2u) sl[2] = s_lo2[v];
+            const uint32_t ns = k3 + (cnt >= 4u ? 1u : 0u);
+            if (cnt >= 4u) sl[3] = s_hi[v];
+            const size_t r = (size_t)(s_recBase + h) * 2;
+            const uint32_t X = ox + (v & 7u), Y = oy + ((v >> 3) & 7u), Z = oz + (v >> 6);
+            rec[r] = make_uint4((uint32_t)a, v | (ns << 9) | (wasHit << 12) | (Z << 16), cnt,
+                                (uint32_t)sl[0]);
+            rec[r + 1] = make_uint4((uint32_t)sl[1], (uint32_t)sl[2], (uint32_t)sl[3], X | (Y << 16));
+        }
+    }
+    __syncthreads();   // LDS is reused by the next tile
+    }
     if (satCounter) {
-        // per-workgroup total of (triangle, voxel) tests for the work counters
         uint32_t tot;
         block_excl_scan(localTests, s_w, &tot);
         if (tid == 0) atomicAdd(satCounter, (unsigned long long)tot);
     }
+}
 
-    // Finalise: colour / normal of voxels with new hits.
-    const uint64_t poolBase = a * kTileVox;
-    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
-        const uint32_t v = (uint32_t)(pass * kBlock + tid);
-        const uint32_t cnt = s_cnt[v];
-        const uint32_t wIdx = v >> 5;
-        const uint32_t prevOcc = init ? 0u : occ[a * kTileWords + wIdx];
-        const bool wasHit = (prevOcc >> (v & 31u)) & 1u;
-        const bool isHit = cnt > 0u;
-        if (isHit) {
-            int32_t slots[4];
-            int ns = 0;
-            const int32_t lo[3] = {s_lo0[v], s_lo1[v], s_lo2[v]};
-            const uint32_t k3 = cnt < 3u ? cnt : 3u;
-            for (uint32_t k = 0; k < k3; ++k) slots[ns++] = lo[k];
-            if (cnt >= 4u) slots[ns++] = s_hi[v];
-            uint32_t c = init ? 0u : col[poolBase + v];
-            f3 nv = init ? f3{0.0f, 0.0f, 0.0f}
-                         : f3{nrm[(a * 3 + 0) * kTileVox + v], nrm[(a * 3 + 1) * kTileVox + v],
-                              nrm[(a * 3 + 2) * kTileVox + v]};
-            const uint32_t lx = v & 7u, ly = (v >> 3) & 7u, lz = v >> 6;
-            f3 cpos = {P.offP[0] + (float)(ox + lx) * P.dP[0], P.offP[1] + (float)(oy + ly) * P.dP[1],
-                       P.offP[2] + (float)(oz + lz) * P.dP[2]};
-            f3 voxCenter = add3(cpos, muls3(f3{P.dP[0], P.dP[1], P.dP[2]}, 0.5f));
-            colour_update(G, verts, fnorm, voxCenter, (wasHit ? 1u : 0u) + cnt, slots, ns, c, nv);
-            col[poolBase + v] = c;
-            nrm[(a * 3 + 0) * kTileVox + v] = nv.x;
-            nrm[(a * 3 + 1) * kTileVox + v] = nv.y;
-            nrm[(a * 3 + 2) * kTileVox + v] = nv.z;
-        } else if (init) {
-            col[poolBase + v] = 0u;
-            nrm[(a * 3 + 0) * kTileVox + v] = 0.0f;
-            nrm[(a * 3 + 1) * kTileVox + v] = 0.0f;
-            nrm[(a * 3 + 2) * kTileVox + v] = 0.0f;
-        }
-        // wave ballot -> 64 hit bits of voxels [pass*256 + 64*wave, +64)
-        const uint64_t mask = __ballot(isHit);
-        if ((tid & 63) == 0) {
-            const uint32_t w0 = v >> 5;  // even word index
-            const uint32_t lo32 = (uint32_t)mask, hi32 = (uint32_t)(mask >> 32);
-            hit[a * kTileWords + w0] = lo32;
-            hit[a * kTileWords + w0 + 1] = hi32;
-            if (init) {
-                occ[a * kTileWords + w0] = lo32;
-                occ[a * kTileWords + w0 + 1] = hi32;
-            } else {
-                occ[a * kTileWords + w0] |= lo32;
-                occ[a * kTileWords + w0 + 1] |= hi32;
-            }
-        }
+// Colour pass over the hit records of one group: ComputeColorsAndNormals /
+// ComputeColorsAsNormals (VoxelizerKernels.cu 689-1083) for each voxel with
+// new hits, one thread per voxel (grid-stride).
+__global__ __launch_bounds__(kBlock) void k_colour(
+    VoxParams P, GroupArgs G, int init, const uint4* __restrict__ rec,
+    const uint32_t* __restrict__ recCount, const int32_t* __restrict__ list,
+    const float* __restrict__ verts, const float* __restrict__ fnorm, uint32_t* __restrict__ col,
+    uint32_t* __restrict__ qn, float* __restrict__ nrm) {
+    __shared__ float s_inv255[256];
+    fill_inv255(s_inv255);
+    __syncthreads();
+    const uint32_t n = *recCount;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint4 r0 = rec[2 * (size_t)i], r1 = rec[2 * (size_t)i + 1];
+        const uint64_t a = r0.x;
+        const uint32_t v = r0.y & 511u, ns = (r0.y >> 9) & 7u, wasHit = (r0.y >> 12) & 1u;
+        const uint32_t cnt = r0.z;
+        const int32_t slots[4] = {(int32_t)r0.w, (int32_t)r1.x, (int32_t)r1.y, (int32_t)r1.z};
+        const uint64_t o = a * kTileVox + v;
+        uint32_t c = init ? 0u : col[o];
+        f3 nv = init ? f3{0.0f, 0.0f, 0.0f}
+                     : f3{nrm[(a * 3 + 0) * kTileVox + v], nrm[(a * 3 + 1) * kTileVox + v],
+                          nrm[(a * 3 + 2) * kTileVox + v]};
+        const uint32_t X = r1.w & 0xffffu, Y = r1.w >> 16, Z = r0.y >> 16;
+        const f3 cpos = {P.offP[0] + (float)X * P.dP[0], P.offP[1] + (float)Y * P.dP[1],
+                         P.offP[2] + (float)Z * P.dP[2]};
+        const f3 voxCenter = add3(cpos, muls3(f3{P.dP[0], P.dP[1], P.dP[2]}, 0.5f));
+        colour_update(G, verts, fnorm, voxCenter, wasHit + cnt, slots, (int)ns, c, nv, s_inv255);
+        col[o] = c;
+        qn[o] = quant_normal(nv);
+        nrm[(a * 3 + 0) * kTileVox + v] = nv.x;
+        nrm[(a * 3 + 1) * kTileVox + v] = nv.y;
+        nrm[(a * 3 + 2) * kTileVox + v] = nv.z;
     }
+}
+
+// Tiles holding triangles of group g, appended in any order (the tiles are
+// independent, so the order does not change any result).
+__global__ void k_group_tiles(VoxParams P, int g, uint64_t nActive, const int32_t* __restrict__ list,
+                              const uint32_t* __restrict__ binStart, int32_t* __restrict__ work,
+                              uint32_t* __restrict__ workCount) {
+    const uint64_t a = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nActive) return;
+    const uint64_t key = (uint64_t)g * P.numTiles + (uint64_t)list[a];
+    if (binStart[key + 1] > binStart[key]) work[atomicAdd(workCount, 1u)] = (int32_t)a;
 }
 
 // ---------------------------------------------------------------------------
 // fill: terrain fill (VoxelizerKernels.cu 886-917).  A voxel without new hits
 // takes colour and normal of the nearest voxel above (at most 19) that has
-// new hits in this group.
-__global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, const int32_t* __restrict__ list,
+// new hits in this group.  Per column of the tile a 32-bit mask of hits over
+// this tile and the three above gives the nearest source by one bit scan.
+// For the first group the voxelize / colour passes write hit voxels only, so
+// here every other voxel gets its fill value or zero.
+__global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init,
+                                                 const int32_t* __restrict__ list,
                                                  const int32_t* __restrict__ tileIdx,
+                                                 const uint32_t* __restrict__ binStart,
                                                  uint32_t* __restrict__ col,
+                                                 uint32_t* __restrict__ qn,
                                                  float* __restrict__ nrm,
                                                  const uint32_t* __restrict__ hit) {
     __shared__ uint32_t s_hit[kFillTiles + 1][kTileWords];
     __shared__ int32_t s_idx[kFillTiles + 1];
+    __shared__ uint32_t s_colmask[64];
     const int tid = threadIdx.x;
     const uint64_t a = blockIdx.x;
     const uint64_t tlin = (uint64_t)list[a];
-    const uint32_t tz = (uint32_t)(tlin / ((uint64_t)P.TG[0] * P.TG[1]));
+    uint32_t tx_, ty_, tz;
+    tile_xyz(tlin, P.TG, tx_, ty_, tz);
     const uint64_t slab = (uint64_t)P.TG[0] * P.TG[1];
+    const bool undefinedTile = init != 0;
     if (tid <= kFillTiles) {
         int32_t idx = -1;
         if (tid == 0) idx = (int32_t)a;
@@ -644,23 +780,38 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, const int32_t* __r
         s_hit[q][wd] = idx >= 0 ? hit[(uint64_t)idx * kTileWords + wd] : 0u;
     }
     __syncthreads();
-    const uint32_t oz = tz * kTile;
+    if (tid < 64) {
+        uint32_t m = 0u;
+#pragma unroll
+        for (int q = 0; q <= kFillTiles; ++q)
+#pragma unroll
+            for (int lz = 0; lz < kTile; ++lz) {
+                const uint32_t v2 = ((uint32_t)lz << 6) | (uint32_t)tid;
+                m |= ((s_hit[q][v2 >> 5] >> (v2 & 31u)) & 1u) << (q * kTile + lz);
+            }
+        s_colmask[tid] = m;
+    }
+    __syncthreads();
     for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
         const uint32_t v = (uint32_t)(pass * kBlock + tid);
-        if ((s_hit[0][v >> 5] >> (v & 31u)) & 1u) continue;
-        const uint32_t lxy = v & 63u, lz = v >> 6;
-        for (uint32_t k = 1; k <= (uint32_t)kFillDepth; ++k) {
-            const uint32_t zz = lz + k;
-            if (oz + zz >= P.S[2]) break;
-            const uint32_t q = zz >> 3, v2 = ((zz & 7u) << 6) | lxy;
-            if ((s_hit[q][v2 >> 5] >> (v2 & 31u)) & 1u) {
-                const uint64_t sa = (uint64_t)s_idx[q];
-                col[a * kTileVox + v] = col[sa * kTileVox + v2];
+        const uint32_t c = v & 63u, lz = v >> 6;
+        const uint32_t m = s_colmask[c];
+        if ((m >> lz) & 1u) continue;
+        const uint32_t above = (m >> (lz + 1u)) & ((1u << kFillDepth) - 1u);
+        if (above) {
+            const uint32_t zz = lz + (uint32_t)__ffs(above);
+            const uint32_t q = zz >> 3, v2 = ((zz & 7u) << 6) | c;
+            const uint64_t sa = (uint64_t)s_idx[q];
+            col[a * kTileVox + v] = col[sa * kTileVox + v2];
+            qn[a * kTileVox + v] = qn[sa * kTileVox + v2];
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    nrm[(a * 3 + c) * kTileVox + v] = nrm[(sa * 3 + c) * kTileVox + v2];
-                break;
-            }
+            for (int k = 0; k < 3; ++k)
+                nrm[(a * 3 + k) * kTileVox + v] = nrm[(sa * 3 + k) * kTileVox + v2];
+        } else if (undefinedTile) {
+            col[a * kTileVox + v] = 0u;
+            qn[a * kTileVox + v] = kZeroNormalQ;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) nrm[(a * 3 + k) * kTileVox + v] = 0.0f;
         }
     }
 }
@@ -672,13 +823,16 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
                                                 const uint32_t* __restrict__ colSrc,
                                                 const float* __restrict__ nrmSrc,
                                                 uint32_t* __restrict__ colDst,
+                                                uint32_t* __restrict__ qnDst,
                                                 float* __restrict__ nrmDst) {
     const int tid = threadIdx.x;
     const uint64_t b = blockIdx.x;
     const uint64_t tlin = (uint64_t)listDst[b];
-    const uint32_t tx = (uint32_t)(tlin % dst.TG[0]);
-    const uint32_t ty = (uint32_t)((tlin / dst.TG[0]) % dst.TG[1]);
-    const uint32_t tz = (uint32_t)(tlin / ((uint64_t)dst.TG[0] * dst.TG[1]));
+    uint32_t tx, ty, tz;
+    tile_xyz(tlin, dst.TG, tx, ty, tz);
+    __shared__ float s_inv255[256];
+    fill_inv255(s_inv255);
+    __syncthreads();
     for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
         const uint32_t v = (uint32_t)(pass * kBlock + tid);
         const uint32_t X = tx * kTile + (v & 7u), Y = ty * kTile + ((v >> 3) & 7u),
@@ -700,12 +854,12 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
                     vid[i] = (zs[(i >> 2) & 1] * 8u + ys[(i >> 1) & 1]) * 8u + xs[i & 1];
                 const uint64_t cb = (uint64_t)sa * kTileVox;
                 const uint64_t nb0 = (uint64_t)sa * 3 * kTileVox;
-                f4 avg = unpack4(colSrc[cb + vid[0]]);
+                f4 avg = unpack4(colSrc[cb + vid[0]], s_inv255);
                 f3 an = {nrmSrc[nb0 + vid[0]], nrmSrc[nb0 + kTileVox + vid[0]],
                          nrmSrc[nb0 + 2 * kTileVox + vid[0]]};
 #pragma unroll
                 for (int i = 1; i < 8; ++i) {
-                    f4 cur = unpack4(colSrc[cb + vid[i]]);
+                    f4 cur = unpack4(colSrc[cb + vid[i]], s_inv255);
                     avg.x += cur.x; avg.y += cur.y; avg.z += cur.z; avg.w += cur.w;
                     an = add3(an, f3{nrmSrc[nb0 + vid[i]], nrmSrc[nb0 + kTileVox + vid[i]],
                                      nrmSrc[nb0 + 2 * kTileVox + vid[i]]});
@@ -728,6 +882,7 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
             }
         }
         colDst[b * kTileVox + v] = outC;
+        qnDst[b * kTileVox + v] = quant_normal(outN);
         nrmDst[(b * 3 + 0) * kTileVox + v] = outN.x;
         nrmDst[(b * 3 + 1) * kTileVox + v] = outN.y;
         nrmDst[(b * 3 + 2) * kTileVox + v] = outN.z;
@@ -743,70 +898,126 @@ __device__ __forceinline__ uint32_t vox_of(uint32_t X, uint32_t Y, uint32_t Z) {
 }
 
 // ---------------------------------------------------------------------------
-// node_type: ComputeOctTreeNodeConstColor + ComputeOctTreeNodeType with the
+// Node typing: ComputeOctTreeNodeConstColor + ComputeOctTreeNodeType with the
 // border handlers (VoxelizerKernels.cu 1680-2380).  A node is NON-CONST when
-// any voxel of its stored brick (interior + 1-voxel border) differs from the
-// voxel at the brick's first interior corner; voxels at x >= xLimit are not
-// examined (the reference's two-launch x split, Voxelizer.cu 1081-1120).
-__global__ __launch_bounds__(kBlock) void k_node_type(LevelMap lm, const uint32_t* __restrict__ col,
-                                                      uint32_t Rd, uint32_t Bx, uint32_t By,
-                                                      uint32_t Bz, uint32_t bx, uint32_t by,
-                                                      uint32_t bz, uint32_t xLimit,
-                                                      uint8_t* __restrict__ types,
-                                                      uint32_t* __restrict__ cc,
-                                                      uint32_t* __restrict__ flag) {
-    __shared__ uint32_t s_any;
-    const int tid = threadIdx.x;
-    const uint64_t n = blockIdx.x;
-    const uint32_t nx = (uint32_t)(n % Rd), ny = (uint32_t)((n / Rd) % Rd),
-                   nz = (uint32_t)(n / ((uint64_t)Rd * Rd));
-    const uint32_t r0x = nx * Bx + bx - 1, r0y = ny * By + by - 1, r0z = nz * Bz + bz - 1;
-    const uint32_t ex = Bx + 2, ey = By + 2, ez = Bz + 2;
-    const uint32_t t0x = r0x >> 3, t0y = r0y >> 3, t0z = r0z >> 3;
-    const uint32_t ntx = ((r0x + ex - 1) >> 3) - t0x + 1, nty = ((r0y + ey - 1) >> 3) - t0y + 1,
-                   ntz = ((r0z + ez - 1) >> 3) - t0z + 1;
-    if (tid == 0) s_any = 0u;
-    __syncthreads();
-    for (uint32_t i = tid; i < ntx * nty * ntz; i += kBlock) {
-        const uint32_t ix = t0x + i % ntx, iy = t0y + (i / ntx) % nty, iz = t0z + i / (ntx * nty);
-        if (ix < lm.TG[0] && iy < lm.TG[1] && iz < lm.TG[2]) {
-            if (lm.tileIdx[((uint64_t)iz * lm.TG[1] + iy) * lm.TG[0] + ix] >= 0) s_any = 1u;
-        }
-    }
-    __syncthreads();
-    if (!s_any) {
-        if (tid == 0) { types[n] = 0; cc[n] = 0u; flag[n] = 0u; }
-        return;
-    }
-    const uint32_t sx = r0x + 1, sy = r0y + 1, sz = r0z + 1;
+// any voxel of its stored brick (interior + 1-voxel border, x < xLimit: the
+// reference's two-launch x split, Voxelizer.cu 1081-1120, can leave the last
+// x block unexamined) differs from the node's sample, the voxel at the
+// brick's first interior corner.  Split in two passes:
+//   node_sample  per node: sample colour, and NON-CONST already when the
+//                sample is non-zero and the brick reaches an inactive tile
+//                (whose voxels are all zero);
+//   node_compare per active tile: every voxel is read once and compared with
+//                the samples of the (at most 8) nodes whose bricks hold it.
+__global__ void k_node_sample(LevelMap lm, const uint32_t* __restrict__ col, uint32_t Rd,
+                              uint32_t Bx, uint32_t By, uint32_t Bz, uint32_t bx, uint32_t by,
+                              uint32_t bz, uint32_t xLimit, uint32_t* __restrict__ cc,
+                              uint32_t* __restrict__ flag) {
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (uint64_t)Rd * Rd * Rd) return;
+    uint32_t nx, ny, nz;
+    node_xyz(n, (uint32_t)(__ffs(Rd) - 1), nx, ny, nz);
+    const uint32_t sx = nx * Bx + bx, sy = ny * By + by, sz = nz * Bz + bz;
     const int64_t sa = tile_of(lm, sx, sy, sz);
     const uint32_t sample = sa >= 0 ? col[(uint64_t)sa * kTileVox + vox_of(sx, sy, sz)] : 0u;
-    uint32_t diff = 0u;
-    const uint32_t nvox = ex * ey * ez;
-    for (uint32_t i = tid; i < nvox && !diff; i += kBlock) {
-        const uint32_t X = r0x + i % ex, Y = r0y + (i / ex) % ey, Z = r0z + i / (ex * ey);
-        if (X >= xLimit) continue;
-        const int64_t ta = tile_of(lm, X, Y, Z);
-        const uint32_t c = ta >= 0 ? col[(uint64_t)ta * kTileVox + vox_of(X, Y, Z)] : 0u;
-        diff = c != sample ? 1u : 0u;
+    uint32_t f = 0u;
+    if (sample != 0u) {
+        const uint32_t r0x = sx - 1, r0y = sy - 1, r0z = sz - 1;
+        uint32_t r1x = sx + Bx;
+        if (r1x > xLimit - 1) r1x = xLimit - 1;
+        const uint32_t r1y = sy + By, r1z = sz + Bz;
+        for (uint32_t tz = r0z >> 3; tz <= (r1z >> 3) && !f; ++tz)
+            for (uint32_t ty = r0y >> 3; ty <= (r1y >> 3) && !f; ++ty)
+                for (uint32_t tx = r0x >> 3; tx <= (r1x >> 3) && r0x <= r1x; ++tx) {
+                    const uint64_t t = ((uint64_t)tz * lm.TG[1] + ty) * lm.TG[0] + tx;
+                    if (lm.tileIdx[t] < 0) { f = 1u; break; }
+                }
+    }
+    cc[n] = sample;
+    flag[n] = f;
+}
+
+// nodes n whose stored brick [n*B + b - 1, n*B + b + B] holds coordinate X
+__device__ __forceinline__ void node_span(uint32_t X, uint32_t B, uint32_t b, uint32_t Rd,
+                                          int32_t& lo, int32_t& hi) {
+    const int32_t x = (int32_t)X - (int32_t)b;   // voxel relative to interior start
+    // n*B - 1 <= x <= n*B + B  <=>  (x - B)/B <= n <= (x + 1)/B
+    int32_t l = x - (int32_t)B;
+    lo = l >= 0 ? (l + (int32_t)B - 1) / (int32_t)B : -((-l) / (int32_t)B);
+    const int32_t h1 = x + 1;
+    hi = h1 >= 0 ? h1 / (int32_t)B : -((-h1 + (int32_t)B - 1) / (int32_t)B);
+    if (lo < 0) lo = 0;
+    if (hi > (int32_t)Rd - 1) hi = (int32_t)Rd - 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_node_compare(
+    LevelMap lm, const int32_t* __restrict__ list, const uint32_t* __restrict__ col, uint32_t Rd,
+    uint32_t Bx, uint32_t By, uint32_t Bz, uint32_t bx, uint32_t by, uint32_t bz, uint32_t xLimit,
+    const uint32_t* __restrict__ cc, uint32_t* __restrict__ flag) {
+    // at most ((8 + B) / B + 1)^3 nodes touch a tile: 216 for B = 2
+    __shared__ uint32_t s_flag[216];
+    const int tid = threadIdx.x;
+    const uint64_t a = blockIdx.x;
+    const uint64_t tlin = (uint64_t)list[a];
+    uint32_t tx, ty, tz;
+    tile_xyz(tlin, lm.TG, tx, ty, tz);
+    const uint32_t B3[3] = {Bx, By, Bz}, b3[3] = {bx, by, bz};
+    const uint32_t o3[3] = {tx * kTile, ty * kTile, tz * kTile};
+    // nodes whose bricks hold each tile coordinate, per axis
+    __shared__ int32_t s_slo[3][kTile], s_shi[3][kTile];
+    if (tid < 3 * kTile) {
+        const int ax = tid / kTile, l = tid % kTile;
+        int32_t lo, hi;
+        node_span(o3[ax] + (uint32_t)l, B3[ax], b3[ax], Rd, lo, hi);
+        s_slo[ax][l] = lo;
+        s_shi[ax][l] = hi;
+    }
+    if (tid < 216) s_flag[tid] = 0u;
+    __syncthreads();
+    int32_t blo[3], bhi[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        blo[ax] = s_slo[ax][0];
+        bhi[ax] = s_shi[ax][kTile - 1];
+    }
+    const int32_t ex = bhi[0] - blo[0] + 1, ey = bhi[1] - blo[1] + 1;
+    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+        const uint32_t v = (uint32_t)(pass * kBlock + tid);
+        const uint32_t lx = v & 7u, ly = (v >> 3) & 7u, lz = v >> 6;
+        const uint32_t X = o3[0] + lx, Y = o3[1] + ly, Z = o3[2] + lz;
+        if (X >= xLimit || X >= lm.dims[0] || Y >= lm.dims[1] || Z >= lm.dims[2]) continue;
+        const int32_t xl = s_slo[0][lx], xh = s_shi[0][lx];
+        const int32_t yl = s_slo[1][ly], yh = s_shi[1][ly];
+        const int32_t zl = s_slo[2][lz], zh = s_shi[2][lz];
+        if (xl > xh || yl > yh || zl > zh) continue;
+        const uint32_t c = col[a * kTileVox + v];
+        for (int32_t nz = zl; nz <= zh; ++nz)
+            for (int32_t ny = yl; ny <= yh; ++ny)
+                for (int32_t nx = xl; nx <= xh; ++nx) {
+                    const uint64_t n = ((uint64_t)nz * Rd + ny) * Rd + nx;
+                    if (c != cc[n]) {
+                        const int32_t li = ((nz - blo[2]) * ey + (ny - blo[1])) * ex + (nx - blo[0]);
+                        s_flag[li] = 1u;
+                    }
+                }
     }
     __syncthreads();
-    if (tid == 0) s_any = 0u;
-    __syncthreads();
-    if (diff) s_any = 1u;
-    __syncthreads();
-    if (tid == 0) {
-        types[n] = (uint8_t)s_any;
-        cc[n] = sample;
-        flag[n] = s_any;
+    if (tid < 216) {
+        const int32_t ez = bhi[2] - blo[2] + 1;
+        if (tid < ex * ey * ez && s_flag[tid]) {
+            const int32_t nx = blo[0] + tid % ex, ny = blo[1] + (tid / ex) % ey,
+                          nz = blo[2] + tid / (ex * ey);
+            flag[((uint64_t)nz * Rd + ny) * Rd + nx] = 1u;
+        }
     }
 }
 
 __global__ void k_brick_assign(uint64_t nodes, const uint32_t* __restrict__ flag,
                                const uint32_t* __restrict__ excl, uint32_t* __restrict__ bidx,
-                               uint32_t* __restrict__ brickNode) {
+                               uint32_t* __restrict__ brickNode, uint8_t* __restrict__ types) {
     uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nodes) return;
+    types[n] = flag[n] ? 1 : 0;
     if (flag[n]) {
         bidx[n] = excl[n];
         brickNode[excl[n]] = (uint32_t)n;
@@ -818,32 +1029,54 @@ __global__ void k_brick_assign(uint64_t nodes, const uint32_t* __restrict__ flag
 // brick gather: WriteBinaryBrick / WriteUByteBinaryBrick middle-brick case
 // (VoxelBrickWriter.cu 725-818, 509-588): (B+2)^3 voxels, RGBA8 colour and
 // RGB8 normal ((n+1)*0.5*255, truncated).
+// i / d for i < 2^22 and a small divisor d, given inv = 1.0f / d: the float
+// quotient is within one of the integer quotient; one correction step.
+__device__ __forceinline__ uint32_t udiv_small(uint32_t i, uint32_t d, float inv) {
+    uint32_t q = (uint32_t)((float)i * inv);
+    const int32_t r = (int32_t)(i - q * d);
+    if (r < 0) --q;
+    else if (r >= (int32_t)d) ++q;
+    return q;
+}
+
 __global__ __launch_bounds__(kBlock) void k_brick_gather(
-    LevelMap lm, const uint32_t* __restrict__ col, const float* __restrict__ nrm,
+    LevelMap lm, const uint32_t* __restrict__ col, const uint32_t* __restrict__ qn,
     const uint32_t* __restrict__ brickNode, uint32_t Rd, uint32_t Bx, uint32_t By, uint32_t Bz,
     uint32_t bx, uint32_t by, uint32_t bz, uint32_t* __restrict__ bcol,
     uint32_t* __restrict__ bnrm) {
+    // tiles under the brick: at most 6 per axis for B <= 38
+    __shared__ int32_t s_tile[216];
     const uint64_t j = blockIdx.x;
     const uint64_t n = brickNode[j];
-    const uint32_t nx = (uint32_t)(n % Rd), ny = (uint32_t)((n / Rd) % Rd),
-                   nz = (uint32_t)(n / ((uint64_t)Rd * Rd));
+    uint32_t nx, ny, nz;
+    node_xyz(n, (uint32_t)(__ffs(Rd) - 1), nx, ny, nz);
     const uint32_t r0x = nx * Bx + bx - 1, r0y = ny * By + by - 1, r0z = nz * Bz + bz - 1;
     const uint32_t ex = Bx + 2, ey = By + 2, ez = Bz + 2;
-    const uint32_t nvox = ex * ey * ez;
+    const uint32_t t0x = r0x >> 3, t0y = r0y >> 3, t0z = r0z >> 3;
+    const uint32_t ntx = ((r0x + ex - 1) >> 3) - t0x + 1, nty = ((r0y + ey - 1) >> 3) - t0y + 1,
+                   ntz = ((r0z + ez - 1) >> 3) - t0z + 1;
+    for (uint32_t i = threadIdx.x; i < ntx * nty * ntz; i += kBlock) {
+        const uint32_t ix = t0x + i % ntx, iy = t0y + (i / ntx) % nty, iz = t0z + i / (ntx * nty);
+        s_tile[i] = lm.tileIdx[((uint64_t)iz * lm.TG[1] + iy) * lm.TG[0] + ix];
+    }
+    __syncthreads();
+    const uint32_t nvox = ex * ey * ez, exy = ex * ey;
+    const float invx = 1.0f / (float)ex, invxy = 1.0f / (float)exy;
     for (uint32_t i = threadIdx.x; i < nvox; i += kBlock) {
-        const uint32_t X = r0x + i % ex, Y = r0y + (i / ex) % ey, Z = r0z + i / (ex * ey);
-        const int64_t ta = tile_of(lm, X, Y, Z);
-        uint32_t c = 0u;
-        f3 nv = {0.0f, 0.0f, 0.0f};
+        const uint32_t z = udiv_small(i, exy, invxy);
+        const uint32_t rxy = i - z * exy;
+        const uint32_t y = udiv_small(rxy, ex, invx);
+        const uint32_t x = rxy - y * ex;
+        const uint32_t X = r0x + x, Y = r0y + y, Z = r0z + z;
+        const int32_t ta = s_tile[(((Z >> 3) - t0z) * nty + ((Y >> 3) - t0y)) * ntx + ((X >> 3) - t0x)];
+        uint32_t c = 0u, q = kZeroNormalQ;
         if (ta >= 0) {
-            const uint32_t v = vox_of(X, Y, Z);
-            c = col[(uint64_t)ta * kTileVox + v];
-            nv = {nrm[((uint64_t)ta * 3 + 0) * kTileVox + v], nrm[((uint64_t)ta * 3 + 1) * kTileVox + v],
-                  nrm[((uint64_t)ta * 3 + 2) * kTileVox + v]};
+            const uint64_t o = (uint64_t)ta * kTileVox + vox_of(X, Y, Z);
+            c = col[o];
+            q = qn[o];
         }
-        float qx = ((nv.x + 1.0f) * 0.5f), qy = ((nv.y + 1.0f) * 0.5f), qz = ((nv.z + 1.0f) * 0.5f);
         bcol[j * nvox + i] = c;
-        bnrm[j * nvox + i] = pack4(quant_u8(qx * 255.0f), quant_u8(qy * 255.0f), quant_u8(qz * 255.0f), 0u);
+        bnrm[j * nvox + i] = q;
     }
 }
 
@@ -855,8 +1088,8 @@ __global__ void k_collapse(uint32_t Rd, int childIsLeaf, uint8_t* __restrict__ t
     const uint64_t nn = (uint64_t)Rd * Rd * Rd;
     if (n >= nn) return;
     if (types[n] != 0) return;
-    const uint32_t x = (uint32_t)(n % Rd), y = (uint32_t)((n / Rd) % Rd),
-                   z = (uint32_t)(n / ((uint64_t)Rd * Rd));
+    uint32_t x, y, z;
+    node_xyz(n, (uint32_t)(__ffs(Rd) - 1), x, y, z);
     const uint32_t Rc = Rd * 2;
     bool all = true;
     for (uint32_t c = 0; c < 8; ++c) {
@@ -872,9 +1105,21 @@ __global__ __launch_bounds__(kBlock) void k_occupancy(VoxParams P, const int32_t
                                                       uint32_t* __restrict__ bitmap) {
     const uint64_t a = blockIdx.x;
     const uint64_t tlin = (uint64_t)list[a];
-    const uint32_t tx = (uint32_t)(tlin % P.TG[0]);
-    const uint32_t ty = (uint32_t)((tlin / P.TG[0]) % P.TG[1]);
-    const uint32_t tz = (uint32_t)(tlin / ((uint64_t)P.TG[0] * P.TG[1]));
+    uint32_t tx, ty, tz;
+    tile_xyz(tlin, P.TG, tx, ty, tz);
+    if ((P.S[0] & 7u) == 0u) {
+        // a tile row of 8 voxels is one byte of the bitmap
+        const uint32_t r = threadIdx.x;
+        if (r < 64u) {
+            const uint32_t byte = (occ[a * kTileWords + (r >> 2)] >> ((r & 3u) * 8u)) & 255u;
+            if (byte) {
+                const uint64_t Y = ty * kTile + (r & 7u), Z = tz * kTile + (r >> 3);
+                const uint64_t i = (Z * P.S[1] + Y) * P.S[0] + (uint64_t)tx * kTile;
+                reinterpret_cast<uint8_t*>(bitmap)[i >> 3] = (uint8_t)byte;
+            }
+        }
+        return;
+    }
     for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
         const uint32_t v = (uint32_t)(pass * kBlock + threadIdx.x);
         if (!((occ[a * kTileWords + (v >> 5)] >> (v & 31u)) & 1u)) continue;
@@ -1014,41 +1259,68 @@ void launch_mark_parent(hipStream_t st, const LevelMap& src, const LevelMap& dst
 void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g, int init,
                      uint64_t nActive, const int32_t* list, const float* verts,
                      const float* fnorm, const uint32_t* bbox, const uint32_t* binStart,
-                     const uint32_t* binTri, uint32_t* col, float* nrm, uint32_t* occ,
-                     uint32_t* hit, unsigned long long* satCounter) {
+                     const uint32_t* binTri, uint32_t* occ, uint32_t* hit,
+                     unsigned long long* satCounter, int32_t* work, uint32_t* workCount,
+                     uint4* rec, uint32_t* recCount, unsigned persistentBlocks) {
     if (nActive == 0) return;
-    k_voxelize<<<(unsigned)nActive, kBlock, 0, st>>>(P, G, g, init, list, verts, fnorm, bbox,
-                                                     binStart, binTri, col, nrm, occ, hit,
-                                                     satCounter);
+    unsigned blocks = (unsigned)(nActive < persistentBlocks ? nActive : persistentBlocks);
+    k_voxelize<<<blocks, kBlock, 0, st>>>(P, G, g, init, work, workCount, list, verts, fnorm, bbox,
+                                          binStart, binTri, occ, hit, rec, recCount, satCounter);
 }
-void launch_fill(hipStream_t st, const VoxParams& P, uint64_t nActive, const int32_t* list,
-                 const int32_t* tileIdx, uint32_t* col, float* nrm, const uint32_t* hit) {
+void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nActive,
+                        const int32_t* list, const uint32_t* binStart, int32_t* work,
+                        uint32_t* workCount, uint32_t* recCount) {
+    launch_fill_u32(st, workCount, 1, 0u);
+    launch_fill_u32(st, recCount, 1, 0u);
     if (nActive == 0) return;
-    k_fill<<<(unsigned)nActive, kBlock, 0, st>>>(P, list, tileIdx, col, nrm, hit);
+    k_group_tiles<<<grid_for(nActive, kBlock), kBlock, 0, st>>>(P, g, nActive, list, binStart,
+                                                                 work, workCount);
+}
+void launch_colour(hipStream_t st, const VoxParams& P, const GroupArgs& G, int init,
+                   const uint4* rec, const uint32_t* recCount, const int32_t* list,
+                   const float* verts, const float* fnorm, uint32_t* col, uint32_t* qn, float* nrm,
+                   unsigned persistentBlocks) {
+    k_colour<<<persistentBlocks, kBlock, 0, st>>>(P, G, init, rec, recCount, list, verts, fnorm,
+                                                  col, qn, nrm);
+}
+void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t nActive,
+                 const int32_t* list, const int32_t* tileIdx, const uint32_t* binStart,
+                 uint32_t* col, uint32_t* qn, float* nrm, const uint32_t* hit) {
+    if (nActive == 0) return;
+    k_fill<<<(unsigned)nActive, kBlock, 0, st>>>(P, g, init, list, tileIdx, binStart, col, qn, nrm,
+                                                 hit);
 }
 void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
                 const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
-                uint32_t* colDst, float* nrmDst) {
+                uint32_t* colDst, uint32_t* qnDst, float* nrmDst) {
     if (nActive == 0) return;
-    k_mip<<<(unsigned)nActive, kBlock, 0, st>>>(src, dst, listDst, colSrc, nrmSrc, colDst, nrmDst);
+    k_mip<<<(unsigned)nActive, kBlock, 0, st>>>(src, dst, listDst, colSrc, nrmSrc, colDst, qnDst,
+                                                nrmDst);
 }
-void launch_node_type(hipStream_t st, const LevelMap& lm, const uint32_t* col, uint32_t Rd,
-                      const uint32_t B[3], const uint32_t border[3], uint32_t xLimit,
-                      uint8_t* types, uint32_t* cc, uint32_t* flag) {
+void launch_node_type(hipStream_t st, const LevelMap& lm, uint64_t nActive, const int32_t* list,
+                      const uint32_t* col, uint32_t Rd, const uint32_t B[3],
+                      const uint32_t border[3], uint32_t xLimit, uint32_t* cc, uint32_t* flag) {
     const uint64_t nodes = (uint64_t)Rd * Rd * Rd;
-    k_node_type<<<(unsigned)nodes, kBlock, 0, st>>>(lm, col, Rd, B[0], B[1], B[2], border[0],
-                                                    border[1], border[2], xLimit, types, cc, flag);
+    k_node_sample<<<grid_for(nodes, kBlock), kBlock, 0, st>>>(lm, col, Rd, B[0], B[1], B[2],
+                                                             border[0], border[1], border[2],
+                                                             xLimit, cc, flag);
+    if (nActive == 0) return;
+    k_node_compare<<<(unsigned)nActive, kBlock, 0, st>>>(lm, list, col, Rd, B[0], B[1], B[2],
+                                                         border[0], border[1], border[2], xLimit,
+                                                         cc, flag);
 }
 void launch_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
-                         const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode) {
-    k_brick_assign<<<grid_for(nodes, kBlock), kBlock, 0, st>>>(nodes, flag, excl, bidx, brickNode);
+                         const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode,
+                         uint8_t* types) {
+    k_brick_assign<<<grid_for(nodes, kBlock), kBlock, 0, st>>>(nodes, flag, excl, bidx, brickNode,
+                                                               types);
 }
 void launch_brick_gather(hipStream_t st, const LevelMap& lm, const uint32_t* col,
-                         const float* nrm, uint64_t nbricks, const uint32_t* brickNode,
+                         const uint32_t* qn, uint64_t nbricks, const uint32_t* brickNode,
                          uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
                          uint32_t* bcol, uint32_t* bnrm) {
     if (nbricks == 0) return;
-    k_brick_gather<<<(unsigned)nbricks, kBlock, 0, st>>>(lm, col, nrm, brickNode, Rd, B[0], B[1],
+    k_brick_gather<<<(unsigned)nbricks, kBlock, 0, st>>>(lm, col, qn, brickNode, Rd, B[0], B[1],
                                                          B[2], border[0], border[1], border[2],
                                                          bcol, bnrm);
 }

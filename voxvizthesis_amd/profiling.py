@@ -1,0 +1,91 @@
+"""Summaries of rocprofv3 output for the voxelizer workloads.
+
+    python -m voxvizthesis_amd.profiling stats <kt_kernel_stats.csv>
+    python -m voxvizthesis_amd.profiling traffic <fetch.csv> <write.csv> \
+        --known-fetch BYTES --known-write BYTES [--json out.json]
+
+`traffic` turns the FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch) into
+HBM bytes per launch.  The WRITE_SIZE scale is the ratio the counter shows on
+a kernel of known size in the same run: the largest k_fill_u32 dispatch writes
+exactly the S^3 occupancy bitmap.  That ratio also applies to FETCH_SIZE (same
+counter units) unless --known-fetch names the bytes the largest
+k_scan_reduce dispatch reads (only meaningful when its input is not cached).
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+
+
+def kname(s: str) -> str:
+    return s.replace("voxmi::(anonymous namespace)::", "").split("(")[0]
+
+
+def load_counter(path: str, counter: str):
+    per = collections.defaultdict(float)
+    names = {}
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        d = int(r["Dispatch_Id"])
+        per[d] += float(r["Counter_Value"])
+        names[d] = kname(r["Kernel_Name"])
+    by_kernel = collections.defaultdict(list)
+    for d, v in sorted(per.items()):
+        by_kernel[names[d]].append(v * 1024.0)      # KiB -> bytes
+    return by_kernel
+
+
+def stats(path: str) -> str:
+    rows = list(csv.DictReader(open(path)))
+    out = ["| kernel | calls | avg us | total % |", "|---|---|---|---|"]
+    for r in rows:
+        out.append(f"| {kname(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['Percentage']):.1f} |")
+    return "\n".join(out)
+
+
+def traffic(fetch_csv, write_csv, known_fetch, known_write, steps=1):
+    f = load_counter(fetch_csv, "FETCH_SIZE")
+    w = load_counter(write_csv, "WRITE_SIZE")
+    fw = known_write / max(w["k_fill_u32"])
+    ff = known_fetch / max(f["k_scan_reduce"]) if known_fetch else fw
+    res = {"fetch_scale": ff, "write_scale": fw, "kernels": {}}
+    for k in sorted(set(f) | set(w)):
+        fs = f.get(k, [])
+        ws = w.get(k, [])
+        tot = sum(fs) * ff + sum(ws) * fw
+        n = max(len(fs), len(ws), 1)
+        res["kernels"][k] = {"launches": n, "fetch_bytes": sum(fs) * ff,
+                             "write_bytes": sum(ws) * fw, "bytes_per_launch": tot / n}
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("csv")
+    b = sub.add_parser("traffic")
+    b.add_argument("fetch")
+    b.add_argument("write")
+    b.add_argument("--known-fetch", type=float, default=0.0)
+    b.add_argument("--known-write", type=float, required=True)
+    b.add_argument("--json")
+    args = ap.parse_args()
+    if args.cmd == "stats":
+        print(stats(args.csv))
+    else:
+        res = traffic(args.fetch, args.write, args.known_fetch, args.known_write)
+        if args.json:
+            with open(args.json, "w") as fh:
+                json.dump(res, fh, indent=1)
+        for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["bytes_per_launch"]):
+            print(f"{k:20s} launches={v['launches']:3d} bytes/launch={v['bytes_per_launch']:.4g}")
+        print(f"scales: fetch {res['fetch_scale']:.4f} write {res['write_scale']:.4f}")
+
+
+if __name__ == "__main__":
+    main()
