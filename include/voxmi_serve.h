@@ -1,0 +1,88 @@
+/*
+ * voxmi_serve.h -- C ABI of the sparse-octree ray cast that produces brick
+ * requests, and of the device-resident brick server that admits them into
+ * the brick pool with LRU eviction.
+ *
+ * Ray cast: the front-to-back traversal of VoxViz/Shaders/GigaVoxels.frag
+ * (main 544-805, TraverseConstant 219-266, TraverseBrick 311-375,
+ * AddToNodeUsageList 508-542) as a HIP kernel, one thread per pixel, over the
+ * node texture and brick pool of voxmi_pool.h.  Each pixel records up to 12
+ * node ids ((x+1)<<20 | (y+1)<<10 | (z+1) of the node texel, 0 = none,
+ * GigaVoxels.frag 508-525) in the 2x2-pixel interleave of the shader
+ * (571-577).  The ids are then deduplicated on the device (wave-level first,
+ * then one flag per node texel) into the frame's request list, replacing the
+ * selection-mask / histopyramid / compaction passes and the host-side
+ * std::map of the reference (GigaVoxelsRenderer.cpp 2048-2110,
+ * GigaVoxelsOctTree.cpp 430-652).
+ *
+ * Brick server: per frame, the nodes of the request list that hold a brick
+ * in the pool are marked used; NON-CONST nodes without a resident brick are
+ * admitted into the least recently used slots (BrickPool::update,
+ * GigaVoxelsBrickPool.cpp 561-682), their bricks copied from the voxelizer's
+ * result in HBM and their node texels updated.  Requests that find no
+ * evictable slot stay queued and are admitted in a later frame.
+ */
+#ifndef VOXMI_SERVE_H
+#define VOXMI_SERVE_H
+
+#include <stdint.h>
+
+#include "voxmi_pool.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vox_raycaster vox_raycaster;
+
+/* A pinhole camera in world units.  look and up are unit vectors and
+ * orthogonal; left = up x look (the reference's Camera::getLeft). */
+typedef struct vox_camera {
+    float    position[3];
+    float    look[3];
+    float    up[3];
+    float    fov_y_deg;         /* vertical field of view                    */
+    int32_t  width, height;     /* viewport in pixels                        */
+    uint32_t frame;             /* frame counter (Camera::getFrameCount)     */
+} vox_camera;
+
+typedef struct vox_raycast_params {
+    float    vol_min[3];        /* world box of the octree                   */
+    float    vol_max[3];
+    float    ray_step;          /* RayStepSize in unit-volume coordinates;
+                                   0: half a leaf voxel, as the renderer sets
+                                   it (GigaVoxelsRenderer.cpp 1705)          */
+    float    lod_scalar;        /* LodScalar; 0: 10 (GigaVoxelsRenderer.cpp 151) */
+    uint32_t jitter_seed;       /* 32x32 jitter table (JitterTexSize = 32)   */
+    uint32_t max_traversals;    /* bound on root restarts per ray; 0: 4096.
+                                   Not in the reference; every ray ends.     */
+} vox_raycast_params;
+
+typedef struct vox_frame_view {
+    int32_t   width, height;
+    uint32_t* rgba;             /* device, width*height RGBA8 (R low byte)   */
+    uint32_t* usage;            /* device, width*height*12 node ids          */
+    uint32_t* requests;         /* device, the frame's distinct node ids     */
+    uint32_t  nrequests;        /* capped at max_requests                    */
+    uint32_t  nrequests_total;  /* distinct ids before the cap               */
+    double    ms_raycast;       /* HIP-event time of the ray-cast kernel     */
+    double    ms_requests;      /* HIP-event time of the deduplication       */
+} vox_frame_view;
+
+/* Ray caster over one octree (its node texture and brick pool).  The frame
+ * buffers are sized for max_width x max_height; max_requests caps the
+ * request list (the reference caps the compacted list at 32768,
+ * GigaVoxelsRenderer.cpp 2063-2064). */
+vox_raycaster* vox_raycaster_create(vox_octree* tree, int32_t max_width, int32_t max_height,
+                                    uint32_t max_requests);
+void vox_raycaster_destroy(vox_raycaster* rc);
+const char* vox_raycaster_get_error(vox_raycaster* rc);
+/* One frame: ray cast, then request deduplication.  Synchronous. */
+int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycast_params* prm);
+int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,4 +1,4 @@
-"""ctypes wrapper around the CPU oracle (oracle/ref_cpu.cpp).
+"""ctypes wrapper around the CPU oracle (oracle/ref_cpu.cpp, oracle/ref_pool.cpp).
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, always as the checker or the timed CPU baseline,
@@ -26,6 +26,18 @@ class RefGroup(C.Structure):
         ("is_terrain", C.c_int32), ("rgba8", C.c_void_p), ("width", C.c_int32),
         ("height", C.c_int32), ("addr0", C.c_int32), ("addr1", C.c_int32),
     ]
+
+
+class RefRayCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("look", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_y_deg", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
+                ("frame", C.c_uint32)]
+
+
+class RefRayParams(C.Structure):
+    _fields_ = [("vol_min", C.c_float * 3), ("vol_max", C.c_float * 3), ("ray_step", C.c_float),
+                ("lod_scalar", C.c_float), ("jitter_seed", C.c_uint32),
+                ("max_traversals", C.c_uint32)]
 
 
 def build(force: bool = False) -> str:
@@ -75,6 +87,31 @@ def lib():
         L.ref_free.argtypes = [C.c_void_p]
         L.ref_normal_error_fp64.restype = C.c_double
         L.ref_normal_error_fp64.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        # node pool / brick pool (ref_pool.h)
+        L.ref_pool_load.restype = C.c_void_p
+        L.ref_pool_load.argtypes = [C.c_char_p, C.POINTER(C.c_uint32), C.c_int32]
+        L.ref_pool_error.restype = C.c_char_p
+        L.ref_pool_node_dims.argtypes = [C.c_void_p, C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint64)]
+        L.ref_pool_brick_dims.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+        for name in ("ref_pool_nodes", "ref_pool_brick_colors", "ref_pool_brick_normals"):
+            f = getattr(L, name)
+            f.restype = C.POINTER(C.c_uint32)
+            f.argtypes = [C.c_void_p]
+        L.ref_pool_depths.restype = C.c_int32
+        L.ref_pool_depths.argtypes = [C.c_void_p]
+        L.ref_pool_file_bricks.restype = C.c_uint64
+        L.ref_pool_file_bricks.argtypes = [C.c_void_p, C.c_int32]
+        for name in ("ref_pool_file_brick_colors", "ref_pool_file_brick_normals"):
+            f = getattr(L, name)
+            f.restype = C.POINTER(C.c_uint32)
+            f.argtypes = [C.c_void_p, C.c_int32]
+        L.ref_pool_free.argtypes = [C.c_void_p]
+        # ray cast (ref_raycast.h)
+        u3p = C.POINTER(C.c_uint32)
+        L.ref_raycast.restype = C.c_int32
+        L.ref_raycast.argtypes = [C.POINTER(RefRayCamera), C.POINTER(RefRayParams), C.c_int32,
+                                  u3p, C.c_void_p, u3p, C.c_void_p, u3p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -165,3 +202,87 @@ def run(scene, flags: int = 0, keep_mips: bool = True) -> RefResult:
         return RefResult(int(st.value), levels, tuple(S3), occ, tree, mips, nerr)
     finally:
         L.ref_free(h)
+
+
+REF_POOL_STORE_CONST = 1
+
+
+class RefPool:
+    """Node texture and brick pool as the runtime builds them from files."""
+
+    def __init__(self, node_dims, blocks, nodes, brick_dims, bcol, bnrm, file_bricks):
+        self.node_dims = node_dims        # (x, y, z)
+        self.blocks = blocks
+        self.nodes = nodes                # (z, y, x, 2) uint32
+        self.brick_dims = brick_dims
+        self.brick_colors = bcol          # (z, y, x) uint32
+        self.brick_normals = bnrm
+        self.file_bricks = file_bricks    # per depth: (colors (nb, bvox), normals (nb, bvox))
+
+
+def pool_load(directory: str, slots=(8, 8, 8), store_const: bool = True) -> RefPool:
+    """Load dir/tree.gvx + voxels_<d>.gvb the way GigaVoxelsReader::Load,
+    BrickPool::initBricks(root, 1) and OctTreeNodePool::create3DTexture do."""
+    L = lib()
+    s3 = (C.c_uint32 * 3)(*slots)
+    h = L.ref_pool_load(directory.encode(), s3, REF_POOL_STORE_CONST if store_const else 0)
+    if not h:
+        raise RuntimeError("pool oracle failed: " + L.ref_pool_error().decode())
+    try:
+        nd = (C.c_uint32 * 3)()
+        blocks = C.c_uint64()
+        L.ref_pool_node_dims(h, nd, C.byref(blocks))
+        ntex = nd[0] * nd[1] * nd[2]
+        nodes = _copy(L.ref_pool_nodes(h), 2 * ntex, np.uint32).reshape(nd[2], nd[1], nd[0], 2)
+        bd = (C.c_uint32 * 3)()
+        L.ref_pool_brick_dims(h, bd)
+        nb = bd[0] * bd[1] * bd[2]
+        bcol = _copy(L.ref_pool_brick_colors(h), nb, np.uint32).reshape(bd[2], bd[1], bd[0])
+        bnrm = _copy(L.ref_pool_brick_normals(h), nb, np.uint32).reshape(bd[2], bd[1], bd[0])
+        e = [bd[a] // slots[a] for a in range(3)]     # B + 2 per axis
+        bvox = e[0] * e[1] * e[2]
+        fb = []
+        for d in range(L.ref_pool_depths(h)):
+            n = int(L.ref_pool_file_bricks(h, d))
+            if n == 0:
+                fb.append((np.zeros((0, bvox), np.uint32), np.zeros((0, bvox), np.uint32)))
+                continue
+            fc = _copy(L.ref_pool_file_brick_colors(h, d), n * bvox, np.uint32).reshape(n, bvox)
+            fn = _copy(L.ref_pool_file_brick_normals(h, d), n * bvox, np.uint32).reshape(n, bvox)
+            fb.append((fc, fn))
+        return RefPool(tuple(nd), int(blocks.value), nodes, tuple(bd), bcol, bnrm, fb)
+    finally:
+        L.ref_pool_free(h)
+
+
+def raycast(camera: dict, params: dict, levels: int, B, nodes: np.ndarray, pool: np.ndarray):
+    """Per-pixel node usage (H, W, 12) and RGBA8 (H, W) of one frame.
+    nodes: (z, y, x, 2) node texture; pool: (z, y, x) brick pool colours."""
+    L = lib()
+    cam = RefRayCamera()
+    cam.position[:] = [float(v) for v in camera["position"]]
+    cam.look[:] = [float(v) for v in camera["look"]]
+    cam.up[:] = [float(v) for v in camera["up"]]
+    cam.fov_y_deg = float(camera["fov_y_deg"])
+    cam.width, cam.height = int(camera["width"]), int(camera["height"])
+    cam.frame = int(camera.get("frame", 0))
+    prm = RefRayParams()
+    prm.vol_min[:] = [float(v) for v in params["vol_min"]]
+    prm.vol_max[:] = [float(v) for v in params["vol_max"]]
+    prm.ray_step = float(params.get("ray_step", 0.0))
+    prm.lod_scalar = float(params.get("lod_scalar", 0.0))
+    prm.jitter_seed = int(params.get("jitter_seed", 0))
+    prm.max_traversals = int(params.get("max_traversals", 0))
+    nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+    pool = np.ascontiguousarray(pool, dtype=np.uint32)
+    nd = (C.c_uint32 * 3)(nodes.shape[2], nodes.shape[1], nodes.shape[0])
+    pd = (C.c_uint32 * 3)(pool.shape[2], pool.shape[1], pool.shape[0])
+    b3 = (C.c_uint32 * 3)(*B)
+    W, H = cam.width, cam.height
+    usage = np.zeros((H, W, 12), np.uint32)
+    rgba = np.zeros((H, W), np.uint32)
+    ok = L.ref_raycast(C.byref(cam), C.byref(prm), levels, b3, nodes.ctypes.data, nd,
+                       pool.ctypes.data, pd, usage.ctypes.data, rgba.ctypes.data)
+    if not ok:
+        raise RuntimeError("ray cast oracle: empty volume box")
+    return usage, rgba

@@ -12,14 +12,20 @@ from voxvizthesis_amd import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    txt = open(os.path.join(ROOT, "include", "voxmi.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(vox_[a-z0-9_]+)\s*\(", txt)))
+def declared_symbols(header=None):
+    names = [header] if header else ["voxmi.h", "voxmi_pool.h", "voxmi_serve.h"]
+    out = set()
+    for h in names:
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        out |= set(re.findall(r"\b(vox_[a-z0-9_]+)\s*\(", txt))
+    return sorted(out)
 
 
 def test_header_matches_binding_list():
-    assert declared_symbols() == sorted(_lib.EXPORTS)
+    assert declared_symbols("voxmi.h") == sorted(_lib.EXPORTS)
+    assert declared_symbols("voxmi_pool.h") == sorted(_lib.POOL_EXPORTS)
+    assert declared_symbols("voxmi_serve.h") == sorted(_lib.SERVE_EXPORTS)
 
 
 def test_library_exports_every_declared_symbol():

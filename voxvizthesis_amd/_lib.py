@@ -1,4 +1,4 @@
-"""ctypes binding of libvoxmi.so (include/voxmi.h).
+"""ctypes binding of libvoxmi.so (include/voxmi.h, voxmi_pool.h, voxmi_serve.h).
 
 The HIP library is the product path: importing this module on a machine
 without the built library raises immediately, there is no CPU fallback.
@@ -26,6 +26,19 @@ EXPORTS = [
     "vox_generate", "vox_get_error", "vox_get_result", "vox_get_stats", "vox_copy_to_host",
     "vox_copy_level", "vox_write_files",
 ]
+# Every symbol include/voxmi_pool.h declares.
+POOL_EXPORTS = [
+    "vox_brick_pool_create", "vox_brick_pool_destroy", "vox_brick_pool_get_error",
+    "vox_brick_pool_get_view", "vox_octree_create", "vox_octree_destroy", "vox_octree_get_error",
+    "vox_octree_get_view", "vox_pool_copy_to_host",
+]
+# Every symbol include/voxmi_serve.h declares.
+SERVE_EXPORTS = [
+    "vox_raycaster_create", "vox_raycaster_destroy", "vox_raycaster_get_error",
+    "vox_raycast_frame", "vox_raycaster_get_frame",
+]
+VOX_TREE_REFERENCE_CONST = 1
+VOX_TREE_EXACT_CONST = 2
 
 
 class VoxGroup(C.Structure):
@@ -60,6 +73,39 @@ class VoxStats(C.Structure):
         ("kernel_name", (C.c_char * 32) * 64), ("kernel_ms", C.c_double * 64),
         ("kernel_launches", C.c_uint32 * 64),
     ]
+
+
+class VoxBrickPoolView(C.Structure):
+    _fields_ = [
+        ("brick_dim", C.c_uint32 * 3), ("slots", C.c_uint32 * 3), ("dims", C.c_uint32 * 3),
+        ("colors", C.c_void_p), ("normals", C.c_void_p), ("resident", C.c_uint32),
+    ]
+
+
+class VoxOctreeView(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int32), ("node_dim", C.c_uint32 * 3), ("node_blocks", C.c_uint64),
+        ("nodes", C.c_void_p), ("node_texel", C.c_void_p * VOX_MAX_LEVELS),
+    ]
+
+
+class VoxCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("look", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_y_deg", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
+                ("frame", C.c_uint32)]
+
+
+class VoxRaycastParams(C.Structure):
+    _fields_ = [("vol_min", C.c_float * 3), ("vol_max", C.c_float * 3), ("ray_step", C.c_float),
+                ("lod_scalar", C.c_float), ("jitter_seed", C.c_uint32),
+                ("max_traversals", C.c_uint32)]
+
+
+class VoxFrameView(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgba", C.c_void_p),
+                ("usage", C.c_void_p), ("requests", C.c_void_p), ("nrequests", C.c_uint32),
+                ("nrequests_total", C.c_uint32), ("ms_raycast", C.c_double),
+                ("ms_requests", C.c_double)]
 
 
 _lib = None
@@ -99,5 +145,30 @@ def load():
     L.vox_copy_to_host.argtypes = [vp, vp, vp, C.c_uint64]
     L.vox_copy_level.argtypes = [vp, C.c_int32, vp, vp, u3]
     L.vox_write_files.argtypes = [vp, C.c_char_p]
+    # node pool / brick pool
+    L.vox_brick_pool_create.restype = vp
+    L.vox_brick_pool_create.argtypes = [u3, u3]
+    L.vox_brick_pool_destroy.argtypes = [vp]
+    L.vox_brick_pool_destroy.restype = None
+    L.vox_brick_pool_get_error.restype = C.c_char_p
+    L.vox_brick_pool_get_error.argtypes = [vp]
+    L.vox_brick_pool_get_view.argtypes = [vp, C.POINTER(VoxBrickPoolView)]
+    L.vox_octree_create.restype = vp
+    L.vox_octree_create.argtypes = [vp, C.POINTER(VoxResult), C.c_uint32]
+    L.vox_octree_destroy.argtypes = [vp]
+    L.vox_octree_destroy.restype = None
+    L.vox_octree_get_error.restype = C.c_char_p
+    L.vox_octree_get_error.argtypes = [vp]
+    L.vox_octree_get_view.argtypes = [vp, C.POINTER(VoxOctreeView)]
+    L.vox_pool_copy_to_host.argtypes = [vp, vp, C.c_uint64]
+    # ray cast
+    L.vox_raycaster_create.restype = vp
+    L.vox_raycaster_create.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint32]
+    L.vox_raycaster_destroy.argtypes = [vp]
+    L.vox_raycaster_destroy.restype = None
+    L.vox_raycaster_get_error.restype = C.c_char_p
+    L.vox_raycaster_get_error.argtypes = [vp]
+    L.vox_raycast_frame.argtypes = [vp, C.POINTER(VoxCamera), C.POINTER(VoxRaycastParams)]
+    L.vox_raycaster_get_frame.argtypes = [vp, C.POINTER(VoxFrameView)]
     _lib = L
     return L

@@ -26,32 +26,6 @@ using namespace voxmi;
 
 namespace {
 
-// Grow-only device buffer.
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    bool ensure(size_t bytes, std::stringstream& err, const char* what) {
-        if (bytes <= cap && p) return true;
-        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
-        size_t b = std::max<size_t>(bytes, 256);
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) {
-            err << "hipMalloc failed for " << what << " (" << b << " bytes): "
-                << hipGetErrorString(e) << std::endl;
-            p = nullptr;
-            return false;
-        }
-        cap = b;
-        return true;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
-};
-
 struct HostGroup {
     vox_group desc;
     bool device = false;

@@ -5,7 +5,38 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <sstream>
+
+#include "../../include/voxmi_pool.h"
+
 namespace voxmi {
+
+// Grow-only device buffer; allocation failures are reported into err.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t bytes, std::stringstream& err, const char* what) {
+        if (bytes <= cap && p) return true;
+        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            err << "hipMalloc failed for " << what << " (" << b << " bytes): "
+                << hipGetErrorString(e) << std::endl;
+            p = nullptr;
+            return false;
+        }
+        cap = b;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
 
 constexpr int kTile = 8;            // voxel tile edge (8^3 = 512 voxels)
 constexpr int kTileVox = 512;
@@ -42,6 +73,27 @@ struct LevelMap {         // sparse tiles of one mip level
     uint64_t numTiles;
     const int32_t* tileIdx;   // dense map tile -> compact index or -1
 };
+
+// Brick arrays of one octree, per depth (the voxelizer result's bricks).
+constexpr int kMaxLevels = 16;
+struct BrickSrc {
+    const uint32_t* col[kMaxLevels];
+    const uint32_t* nrm[kMaxLevels];
+};
+
+// What the ray cast and the brick server need of an octree and its pool.
+struct OctreeRefs {
+    uint32_t ndim[3];       // node texture texels per axis
+    uint32_t* tex;          // node texture (2 words per texel)
+    int32_t levels;
+    uint32_t B[3];          // brick interior
+    uint32_t poolDims[3];   // brick pool texels per axis
+    uint32_t poolSlots[3];
+    uint32_t* poolCol;
+    uint32_t* poolNrm;
+    hipStream_t st;         // the pool's stream
+};
+bool octree_refs(vox_octree* t, OctreeRefs& r);
 
 // ---- launchers (all asynchronous on `st`) ----
 void launch_setup(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
@@ -91,6 +143,12 @@ void launch_occupancy(hipStream_t st, const VoxParams& P, uint64_t nActive, cons
 void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
                         const float* nrm, uint32_t* outCol, float* outNrm);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
+// Upload requests (depth, brick, slot, node texel or 0xffffffff) into brick
+// pool slots; *nreq requests, at most maxReq.
+void launch_brick_upload(hipStream_t st, const BrickSrc& src, const uint4* req,
+                         const uint32_t* nreq, uint32_t maxReq, const uint32_t e[3],
+                         const uint32_t slots[3], const uint32_t dims[3], uint32_t* col,
+                         uint32_t* nrm, uint32_t* tex);
 // exclusive scan of n u32 values; out has n+1 entries, out[n] = total.
 // tmp must hold scan_tmp_words(n) words.
 uint64_t scan_tmp_words(uint64_t n);

@@ -1,0 +1,626 @@
+// vox_serve.hip -- sparse-octree ray cast producing brick requests, and the
+// request deduplication (C ABI in include/voxmi_serve.h).
+//
+// The ray cast restates VoxViz/Shaders/GigaVoxels.frag main (544-805) per
+// pixel: descend from the root while the node is constant or has a resident
+// brick, its projected voxel is larger than a pixel and it has children;
+// at the stopping node composite its constant colour or the most recently
+// descended resident brick up to the node's exit, record the node in the
+// pixel's usage list, restart from the root.  fp32, no contraction, the
+// shader's operation order; GLSL built-ins are defined as follows (the same
+// definitions hold in the oracle):
+//   normalize(v) = v * (1 / sqrt(dot(v, v))), dot = (x*x + y*y) + z*z
+//   pow(x, 2^n)  = x squared n times (all exponents in the shader are 2^n)
+//   texture()    = trilinear, clamp to edge, weights rounded to 8 fractional
+//                  bits, unorm8 texels as c / 255
+//   uvec3(v)     = truncation, negative components 0
+// Rays start on the box of the octree (the rasterized front faces of the
+// reference), at the camera when it is inside the box.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/voxmi_serve.h"
+#include "vox_internal.h"
+
+using namespace voxmi;
+
+namespace {
+
+constexpr int kMaxUsage = 12;  // 3 lists x 4 components per pixel
+constexpr uint32_t kConstBit = 0x40000000u;
+constexpr uint32_t kResidentBit = 0x80000000u;
+
+struct RayUniforms {
+    float camPos[3], camUp[3], camLeft[3];
+    float volMin[3], volMax[3], volScale[3];
+    float look[3], right[3], upv[3];  // world ray basis: look, right*tanX, up*tanY
+    float rayStep, rootVoxelHalf, pixelSize, lodScalar;
+    float brickStep[3], brickDim[3], poolDim[3];
+    int32_t depthMinusOne, maxPush, rootConst, W, H, maxTraversals;
+    uint32_t ndim[3], pdim[3];
+};
+
+struct v3 {
+    float x, y, z;
+};
+__device__ __forceinline__ v3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ v3 add(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ v3 sub(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 mul(v3 a, v3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ v3 divv(v3 a, v3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ v3 muls(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ v3 normalize(v3 v) { return muls(v, 1.0f / sqrtf(dot(v, v))); }
+__device__ __forceinline__ float signf(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+__device__ __forceinline__ v3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+
+__device__ __forceinline__ float inside_count(v3 p, v3 mn, v3 mx) {
+    const v3 t1 = mk(signf(p.x - mn.x), signf(p.y - mn.y), signf(p.z - mn.z));
+    const v3 t2 = mk(signf(mx.x - p.x), signf(mx.y - p.y), signf(mx.z - p.z));
+    return dot(t1, t2);
+}
+
+// x^(2^n) by squaring.
+__device__ __forceinline__ float pow2n(float x, int n) {
+    for (int i = 0; i < n; ++i) x = x * x;
+    return x;
+}
+
+// ComputeMinRayBoxIntersection (GigaVoxels.frag 155-180): the smallest
+// positive-direction exit time over the six faces, 4 if none.
+__device__ __forceinline__ float exit_time(v3 p, v3 d, v3 mn, v3 mx) {
+    float tMin = 4.0f;
+    // faces with outward normals -x, -y, -z through mn; +x, +y, +z through mx
+    const float dd[6] = {-d.x, -d.y, -d.z, d.x, d.y, d.z};
+    const float nv[6] = {-(p.x - mn.x), -(p.y - mn.y), -(p.z - mn.z),
+                         p.x - mx.x, p.y - mx.y, p.z - mx.z};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        if (dd[i] > 0.0f) {
+            const float t = -nv[i] / dd[i];
+            if (t < tMin && fabsf(t) > 0.0f) tMin = t;
+        }
+    }
+    return tMin;
+}
+
+__device__ __forceinline__ void unpack_rgba(uint32_t c, float o[4]) {
+    o[0] = (float)(c & 255u) / 255.0f;
+    o[1] = (float)((c >> 8) & 255u) / 255.0f;
+    o[2] = (float)((c >> 16) & 255u) / 255.0f;
+    o[3] = (float)(c >> 24) / 255.0f;
+}
+
+// texture(BrickSampler, s): trilinear, clamp to edge.
+__device__ void sample_brick(const uint32_t* __restrict__ col, const RayUniforms& U, v3 s,
+                             float r[4]) {
+    float c[3] = {s.x, s.y, s.z};
+    int i0[3];
+    float w[3];
+    for (int a = 0; a < 3; ++a) {
+        float u = c[a];
+        if (!(u == u)) u = 0.0f;
+        const float b = u * U.poolDim[a] - 0.5f;
+        const float f = floorf(b);
+        w[a] = rintf((b - f) * 256.0f) / 256.0f;
+        i0[a] = (int)fmaxf(fminf(f, 1.0e9f), -1.0e9f);
+    }
+    r[0] = r[1] = r[2] = r[3] = 0.0f;
+    for (int k = 0; k < 8; ++k) {
+        const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+        int ix = i0[0] + dx, iy = i0[1] + dy, iz = i0[2] + dz;
+        ix = ix < 0 ? 0 : (ix >= (int)U.pdim[0] ? (int)U.pdim[0] - 1 : ix);
+        iy = iy < 0 ? 0 : (iy >= (int)U.pdim[1] ? (int)U.pdim[1] - 1 : iy);
+        iz = iz < 0 ? 0 : (iz >= (int)U.pdim[2] ? (int)U.pdim[2] - 1 : iz);
+        const float wx = dx ? w[0] : 1.0f - w[0];
+        const float wy = dy ? w[1] : 1.0f - w[1];
+        const float wz = dz ? w[2] : 1.0f - w[2];
+        const float wk = (wx * wy) * wz;
+        float t[4];
+        unpack_rgba(col[((uint64_t)iz * U.pdim[1] + iy) * U.pdim[0] + ix], t);
+        r[0] = r[0] + wk * t[0];
+        r[1] = r[1] + wk * t[1];
+        r[2] = r[2] + wk * t[2];
+        r[3] = r[3] + wk * t[3];
+    }
+}
+
+__device__ __forceinline__ void blend(float dst[4], const float src[4]) {
+    const float k = 1.0f - dst[3];
+    dst[0] = k * src[0] + dst[0];
+    dst[1] = k * src[1] + dst[1];
+    dst[2] = k * src[2] + dst[2];
+    dst[3] = k * src[3] + dst[3];
+}
+
+__device__ __forceinline__ uint2 fetch_node(const uint32_t* __restrict__ tex, const RayUniforms& U,
+                                            uint32_t x, uint32_t y, uint32_t z) {
+    // texelFetch outside the texture reads 0 (never reached with valid pointers)
+    if (x >= U.ndim[0] || y >= U.ndim[1] || z >= U.ndim[2]) return make_uint2(0u, 0u);
+    const uint64_t t = ((uint64_t)z * U.ndim[1] + y) * U.ndim[0] + x;
+    return make_uint2(tex[2 * t], tex[2 * t + 1]);
+}
+
+__global__ void __launch_bounds__(256)
+k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __restrict__ pool,
+          const float* __restrict__ jitter, uint32_t* __restrict__ rgbaOut,
+          uint32_t* __restrict__ usageOut) {
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= U.W || j >= U.H) return;
+    uint32_t usage[kMaxUsage];
+#pragma unroll
+    for (int k = 0; k < kMaxUsage; ++k) usage[k] = 0u;
+    float dst[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+
+    // the pixel's ray, world -> unit volume
+    const float sx = (2.0f * ((float)i + 0.5f)) / (float)U.W - 1.0f;
+    const float sy = (2.0f * ((float)j + 0.5f)) / (float)U.H - 1.0f;
+    const v3 dw = add(add(ld3(U.look), muls(ld3(U.right), sx)), muls(ld3(U.upv), sy));
+    const v3 dn = divv(dw, ld3(U.volScale));
+    const v3 cam = ld3(U.camPos), vmin = ld3(U.volMin), vmax = ld3(U.volMax);
+    float tEnter = -INFINITY, tExit = INFINITY;
+    bool hit = true;
+    const float dc[3] = {dn.x, dn.y, dn.z}, cc[3] = {cam.x, cam.y, cam.z};
+    for (int a = 0; a < 3; ++a) {
+        if (dc[a] == 0.0f) {
+            if (cc[a] < U.volMin[a] || cc[a] > U.volMax[a]) hit = false;
+            continue;
+        }
+        const float t0 = (U.volMin[a] - cc[a]) / dc[a];
+        const float t1 = (U.volMax[a] - cc[a]) / dc[a];
+        tEnter = fmaxf(tEnter, fminf(t0, t1));
+        tExit = fminf(tExit, fmaxf(t0, t1));
+    }
+    const float tStart = fmaxf(tEnter, 0.0f);
+    if (!(tExit >= tStart)) hit = false;
+
+    if (hit) {
+        v3 rayPos = add(cam, muls(dn, tStart));
+        const v3 rayDir = normalize(dn);
+        const float jit = jitter[(j & 31) * 32 + (i & 31)];
+        rayPos = add(rayPos, muls(muls(rayDir, U.rayStep), jit));
+        float inside = inside_count(rayPos, vmin, vmax);
+        if (inside >= 3.0f) {
+            int curList = 0;  // list * 4 + component
+            const int startPush = (i % 2) * kMaxUsage + (j % 2) * kMaxUsage * 2;
+            const int stopPush = startPush + kMaxUsage;
+            const int maxPerBlock = kMaxUsage * 4;
+            const v3 brickRayStep = mul(rayDir, ld3(U.brickStep));
+            const v3 camUp = ld3(U.camUp), camLeft = ld3(U.camLeft);
+            const uint2 root = fetch_node(tex, U, 0, 0, 0);
+            uint32_t px = 0, py = 0, pz = 0;
+            uint2 node = root;
+            v3 extMin = vmin, extMax = vmax;
+            float nodeSize = 1.0f, vh = U.rootVoxelHalf;
+            v3 rayBrickPos = rayPos, curBrickRayPos = rayPos;
+            uint2 curBrickPtr = root;
+            bool loaded = false, curConst = U.rootConst != 0;
+            int curDepth = 0, curBrickDepth = 0;
+            uint32_t cx = 0, cy = 0, cz = 0;
+            int nlt = 0, pushed = 0, traversals = 0;
+            while (inside >= 3.0f && traversals < U.maxTraversals) {
+                curConst = (node.x & kConstBit) != 0u;
+                loaded = (node.x & kResidentBit) != 0u;
+                if (loaded) {
+                    curBrickPtr = node;
+                    curBrickRayPos = rayBrickPos;
+                    curBrickDepth = curDepth;
+                }
+                // ComputeVoxelSize (GigaVoxels.frag 483-490) * LodScalar
+                const v3 ll = normalize(sub(add(sub(rayPos, muls(camUp, vh)), muls(camLeft, vh)), cam));
+                const v3 ur = normalize(sub(sub(add(rayPos, muls(camUp, vh)), muls(camLeft, vh)), cam));
+                const float proj = (1.0f - dot(ll, ur)) * U.lodScalar;
+                bool stop = (!curConst && !loaded) || U.pixelSize > proj;
+                if (!stop) {
+                    cx = (node.x >> 20) & 1023u;
+                    cy = (node.x >> 10) & 1023u;
+                    cz = node.x & 1023u;
+                    // the depth bound never binds for a tree built here (the
+                    // last depth has no child pointers); it keeps a damaged
+                    // texture from descending forever
+                    stop = (cx | cy | cz) == 0u || curDepth >= U.depthMinusOne;
+                }
+                if (stop) {
+                    const uint32_t id = ((px + 1u) << 20) | ((py + 1u) << 10) | (pz + 1u);
+                    if (curConst) {
+                        const int n = U.depthMinusOne - curDepth;
+                        const float ratio = ldexpf(1.0f, n);
+                        const float step = U.rayStep * ratio;
+                        float cv[4];
+                        unpack_rgba(node.y, cv);
+                        const float tExitNode = exit_time(rayPos, rayDir, extMin, extMax);
+                        if (cv[3] < 0.005f) {
+                            const int ns = (int)(tExitNode / step) + 1;  // AdvanceRay
+                            rayPos = add(rayPos, muls(rayDir, (float)ns * step));
+                        } else {
+                            int ns = (int)(tExitNode / step) + 1;
+                            const v3 ws = muls(rayDir, step);
+                            float src[4];
+                            src[3] = 1.0f - pow2n(1.0f - cv[3], n);
+                            src[0] = cv[0] * ratio;
+                            src[1] = cv[1] * ratio;
+                            src[2] = cv[2] * ratio;
+                            for (int s = 0; s < ns; ++s) {
+                                rayPos = add(rayPos, ws);
+                                if (dst[3] > 0.995f) {
+                                    const int rest = ns - (s + 1);
+                                    rayPos = add(rayPos, muls(rayDir, (float)rest * step));
+                                    break;
+                                }
+                                blend(dst, src);
+                            }
+                        }
+                    } else {
+                        const v3 bp = mk((float)((curBrickPtr.y >> 20) & 1023u) / U.poolDim[0],
+                                         (float)((curBrickPtr.y >> 10) & 1023u) / U.poolDim[1],
+                                         (float)(curBrickPtr.y & 1023u) / U.poolDim[2]);
+                        v3 cbr = add(bp, divv(mul(curBrickRayPos, ld3(U.brickDim)),
+                                              ld3(U.poolDim)));
+                        const float ratio = ldexpf(1.0f, U.depthMinusOne - curBrickDepth);
+                        const v3 ws = muls(muls(rayDir, U.rayStep), ratio);
+                        const float tExitNode = exit_time(rayPos, rayDir, extMin, extMax);
+                        const float step = U.rayStep * ratio;
+                        int ns = (int)(tExitNode / step) + 1;
+                        for (int s = 0; s < ns; ++s) {
+                            rayPos = add(rayPos, ws);
+                            if (dst[3] > 0.995f) {
+                                const int rest = ns - (s + 1);
+                                rayPos = add(rayPos, muls(rayDir, (float)rest * step));
+                                break;
+                            }
+                            float src[4];
+                            sample_brick(pool, U, cbr, src);
+                            src[3] = 1.0f - (1.0f - src[3]);  // pow(1 - a, 1.0)
+                            blend(dst, src);
+                            cbr = add(cbr, brickRayStep);
+                        }
+                    }
+                    if (nlt >= startPush && nlt < stopPush && pushed < U.maxPush) {
+                        ++pushed;
+#pragma unroll
+                        for (int k = 0; k < kMaxUsage; ++k)
+                            if (k == curList) usage[k] = id;
+                        curList = curList + 1 == kMaxUsage ? 0 : curList + 1;
+                    }
+                    if (dst[3] > 0.995f && (curConst || loaded)) break;
+                    // octTreeNodePtr is not reset (GigaVoxels.frag 718-742): a
+                    // traversal that stops at the root records the previous
+                    // stopping node's texel.
+                    node = root;
+                    extMin = vmin;
+                    extMax = vmax;
+                    nodeSize = 1.0f;
+                    vh = U.rootVoxelHalf;
+                    rayBrickPos = rayPos;
+                    curBrickRayPos = rayPos;
+                    curBrickPtr = node;
+                    loaded = false;
+                    curConst = U.rootConst != 0;
+                    curDepth = 0;
+                    curBrickDepth = 0;
+                    cx = cy = cz = 0;
+                    inside = inside_count(rayPos, vmin, vmax);
+                    ++traversals;
+                    ++nlt;
+                    if (nlt >= maxPerBlock) nlt = 0;
+                } else {
+                    const v3 bo = muls(rayBrickPos, 2.0f);
+                    const uint32_t ox = bo.x >= 1.0f ? 1u : 0u;
+                    const uint32_t oy = bo.y >= 1.0f ? 1u : 0u;
+                    const uint32_t oz = bo.z >= 1.0f ? 1u : 0u;
+                    rayBrickPos = sub(bo, mk((float)ox, (float)oy, (float)oz));
+                    px = cx + ox;
+                    py = cy + oy;
+                    pz = cz + oz;
+                    ++curDepth;
+                    nodeSize *= 0.5f;
+                    extMin = add(extMin, muls(mk((float)ox, (float)oy, (float)oz), nodeSize));
+                    extMax = sub(extMax, muls(mk(1.0f - (float)ox, 1.0f - (float)oy,
+                                                 1.0f - (float)oz), nodeSize));
+                    vh *= 0.5f;
+                    node = fetch_node(tex, U, px, py, pz);
+                }
+            }
+        }
+    }
+    const uint64_t pix = (uint64_t)j * U.W + i;
+    if (rgbaOut) {
+        uint32_t o = 0;
+        for (int c = 0; c < 4; ++c) {
+            float v = dst[c] * 255.0f;
+            v = v > 255.0f ? 255.0f : (v > 0.0f ? v : 0.0f);
+            o |= (uint32_t)v << (8 * c);
+        }
+        rgbaOut[pix] = o;
+    }
+    uint4* u4 = reinterpret_cast<uint4*>(usageOut + pix * kMaxUsage);
+    u4[0] = make_uint4(usage[0], usage[1], usage[2], usage[3]);
+    u4[1] = make_uint4(usage[4], usage[5], usage[6], usage[7]);
+    u4[2] = make_uint4(usage[8], usage[9], usage[10], usage[11]);
+}
+
+// Distinct node ids of the frame: lanes of a wave holding the same id elect
+// one leader (ballot loop), leaders claim the node texel's frame stamp, the
+// first claimant appends the id.  counts[0] = appended (capped), counts[1] =
+// distinct ids.
+__global__ void __launch_bounds__(256)
+k_requests(const uint32_t* __restrict__ usage, uint64_t n, uint32_t tag, uint32_t ndx,
+           uint32_t ndy, uint32_t ndz, uint32_t* __restrict__ stamp, uint32_t* __restrict__ list,
+           uint32_t cap, uint32_t* __restrict__ counts) {
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n;
+         base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = base + threadIdx.x;
+        uint32_t id = e < n ? usage[e] : 0u;
+        // wave-level deduplication
+        uint64_t pending = __ballot(id != 0u);
+        bool leader = false;
+        while (pending) {
+            const int l = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t lid = __shfl(id, l);
+            const uint64_t same = __ballot(id == lid);
+            if ((int)(threadIdx.x & 63) == l) leader = true;
+            pending &= ~same;
+        }
+        if (leader) {
+            const uint32_t x = ((id >> 20) & 1023u) - 1u;
+            const uint32_t y = ((id >> 10) & 1023u) - 1u;
+            const uint32_t z = (id & 1023u) - 1u;
+            const uint64_t t = ((uint64_t)z * ndy + y) * ndx + x;
+            if (x < ndx && y < ndy && z < ndz && atomicExch(&stamp[t], tag) != tag) {
+                const uint32_t k = atomicAdd(&counts[1], 1u);
+                if (k < cap) {
+                    list[k] = id;
+                    atomicAdd(&counts[0], 1u);
+                }
+            }
+        }
+    }
+}
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+struct vox_raycaster {
+    vox_octree* tree = nullptr;
+    bool valid = false;
+    int32_t maxW = 0, maxH = 0, W = 0, H = 0;
+    uint32_t maxReq = 0, nreq = 0, nreqTotal = 0, tag = 0;
+    uint32_t ndim[3] = {0, 0, 0};
+    hipStream_t st = nullptr;
+    DevBuf rgba, usage, req, counts, stamp, jitter;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    double msRay = 0, msReq = 0;
+    std::stringstream err;
+    std::string errCache;
+    bool fail(const std::string& m) {
+        err << m << std::endl;
+        return false;
+    }
+    bool check(hipError_t e, const char* what) {
+        if (e == hipSuccess) return true;
+        err << what << " failed: " << hipGetErrorString(e) << std::endl;
+        return false;
+    }
+};
+
+namespace {
+
+// Uniforms of one frame (GigaVoxelsRenderer.cpp 1650-1816), in float like
+// the renderer's QVector3D arithmetic.
+bool ray_uniforms(const vox_camera& cam, const vox_raycast_params& prm, int levels,
+                  const uint32_t B[3], const uint32_t poolDims[3], const uint32_t ndim[3],
+                  RayUniforms& U) {
+    std::memset(&U, 0, sizeof(U));
+    float T[3], S[3];
+    for (int a = 0; a < 3; ++a) {
+        T[a] = (prm.vol_min[a] + prm.vol_max[a]) * 0.5f;
+        S[a] = prm.vol_max[a] - prm.vol_min[a];
+        if (!(S[a] > 0.0f)) return false;
+        U.volScale[a] = S[a];
+        U.volMin[a] = (prm.vol_min[a] - T[a]) / S[a] + 0.5f;
+        U.volMax[a] = (prm.vol_max[a] - T[a]) / S[a] + 0.5f;
+        U.camPos[a] = (cam.position[a] - T[a]) / S[a] + 0.5f;
+        U.camUp[a] = cam.up[a];
+    }
+    // left = up x look
+    U.camLeft[0] = cam.up[1] * cam.look[2] - cam.up[2] * cam.look[1];
+    U.camLeft[1] = cam.up[2] * cam.look[0] - cam.up[0] * cam.look[2];
+    U.camLeft[2] = cam.up[0] * cam.look[1] - cam.up[1] * cam.look[0];
+    const float tanY = std::tan(cam.fov_y_deg * 0.5f * 3.14159265358979f / 180.0f);
+    const float tanX = tanY * (float)cam.width / (float)cam.height;
+    for (int a = 0; a < 3; ++a) {
+        U.look[a] = cam.look[a];
+        U.right[a] = -U.camLeft[a] * tanX;
+        U.upv[a] = cam.up[a] * tanY;
+    }
+    // RayStepSize: half the smallest voxel edge of the unit volume
+    const uint32_t Nmax = std::max(B[0], std::max(B[1], B[2])) << (levels - 1);
+    U.rayStep = prm.ray_step > 0.0f ? prm.ray_step : (1.0f / (float)Nmax) * 0.5f;
+    if (!(U.rayStep >= 1.0e-5f)) return false;  // bounds the samples per node
+    U.lodScalar = prm.lod_scalar > 0.0f ? prm.lod_scalar : 10.0f;
+    // BrickStepVector = (1 / pool) * (rayStep * 2^(L-1) * S) / (S / B)
+    const float pow2 = std::ldexp(1.0f, levels - 1);
+    for (int a = 0; a < 3; ++a) {
+        const float rayStepRoot = (S[a] * U.rayStep) * pow2;
+        const float rootVoxel = S[a] / (float)B[a];
+        const float ratio = rayStepRoot / rootVoxel;
+        U.brickStep[a] = (1.0f / (float)poolDims[a]) * ratio;
+        U.brickDim[a] = (float)B[a];
+        U.poolDim[a] = (float)poolDims[a];
+        U.pdim[a] = poolDims[a];
+        U.ndim[a] = ndim[a];
+    }
+    const float bx = (float)B[0], by = (float)B[1], bz = (float)B[2];
+    const float minB = bx < ((by < bz) ? by : bz) ? bx : bz;  // renderer's ternary
+    U.rootVoxelHalf = (1.0f / minB) * 0.5f;
+    // PixelSize: 1 - cos of the angle between the rays through the centre
+    // pixel corner and its diagonal neighbour (Camera::getPixelSize).
+    {
+        const double W = cam.width, H = cam.height;
+        double r0[3], r1[3];
+        for (int a = 0; a < 3; ++a) {
+            const double sx0 = 2.0 * (W * 0.5) / W - 1.0, sy0 = 2.0 * (H * 0.5) / H - 1.0;
+            const double sx1 = 2.0 * (W * 0.5 + 1.0) / W - 1.0, sy1 = 2.0 * (H * 0.5 + 1.0) / H - 1.0;
+            r0[a] = cam.look[a] + (double)U.right[a] * sx0 + (double)U.upv[a] * sy0;
+            r1[a] = cam.look[a] + (double)U.right[a] * sx1 + (double)U.upv[a] * sy1;
+        }
+        const double n0 = std::sqrt(r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]);
+        const double n1 = std::sqrt(r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
+        const double c = (r0[0] * r1[0] + r0[1] * r1[1] + r0[2] * r1[2]) / (n0 * n1);
+        U.pixelSize = (float)(1.0 - c);
+    }
+    U.depthMinusOne = levels - 1;
+    U.maxPush = kMaxUsage * (int)((cam.frame % 2) + 1);
+    U.W = cam.width;
+    U.H = cam.height;
+    U.maxTraversals = prm.max_traversals ? (int32_t)prm.max_traversals : 4096;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+vox_raycaster* vox_raycaster_create(vox_octree* tree, int32_t max_width, int32_t max_height,
+                                    uint32_t max_requests) {
+    vox_raycaster* rc = new (std::nothrow) vox_raycaster;
+    if (!rc) return nullptr;
+    rc->tree = tree;
+    OctreeRefs o;
+    if (!tree || !octree_refs(tree, o)) {
+        rc->fail("raycaster: invalid octree");
+        return rc;
+    }
+    for (int a = 0; a < 3; ++a) rc->ndim[a] = o.ndim[a];
+    rc->st = o.st;
+    if (max_width <= 0 || max_height <= 0 || max_requests == 0) {
+        rc->fail("raycaster: empty frame or request capacity");
+        return rc;
+    }
+    rc->maxW = max_width;
+    rc->maxH = max_height;
+    rc->maxReq = max_requests;
+    const uint64_t npix = (uint64_t)max_width * max_height;
+    const uint64_t ntex = (uint64_t)rc->ndim[0] * rc->ndim[1] * rc->ndim[2];
+    if (!rc->rgba.ensure(4 * npix, rc->err, "rgba") ||
+        !rc->usage.ensure(4 * npix * kMaxUsage, rc->err, "usage") ||
+        !rc->req.ensure(4ull * max_requests, rc->err, "requests") ||
+        !rc->counts.ensure(8, rc->err, "counts") || !rc->stamp.ensure(4 * ntex, rc->err, "stamp") ||
+        !rc->jitter.ensure(4 * 1024, rc->err, "jitter"))
+        return rc;
+    launch_fill_u32(rc->st, rc->stamp.as<uint32_t>(), ntex, 0u);
+    for (auto& e : rc->ev)
+        if (!rc->check(hipEventCreate(&e), "hipEventCreate")) return rc;
+    if (!rc->check(hipStreamSynchronize(rc->st), "synchronize")) return rc;
+    rc->valid = true;
+    return rc;
+}
+
+void vox_raycaster_destroy(vox_raycaster* rc) {
+    if (!rc) return;
+    if (rc->st) (void)hipStreamSynchronize(rc->st);
+    for (auto e : rc->ev)
+        if (e) (void)hipEventDestroy(e);
+    rc->rgba.release();
+    rc->usage.release();
+    rc->req.release();
+    rc->counts.release();
+    rc->stamp.release();
+    rc->jitter.release();
+    delete rc;
+}
+
+const char* vox_raycaster_get_error(vox_raycaster* rc) {
+    if (!rc) return "null raycaster";
+    rc->errCache = rc->err.str();
+    return rc->errCache.c_str();
+}
+
+int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycast_params* prm) {
+    if (!rc || !rc->valid || !cam || !prm) return 0;
+    if (cam->width <= 0 || cam->height <= 0 || cam->width > rc->maxW || cam->height > rc->maxH)
+        return rc->fail("raycast: viewport exceeds the raycaster's frame size"), 0;
+    OctreeRefs o;
+    if (!octree_refs(rc->tree, o)) return rc->fail("raycast: octree unavailable"), 0;
+    const uint32_t* nd = o.ndim;
+    uint32_t* tex = o.tex;
+    hipStream_t st = o.st;
+    RayUniforms U;
+    if (!ray_uniforms(*cam, *prm, o.levels, o.B, o.poolDims, o.ndim, U))
+        return rc->fail("raycast: empty volume box or ray step below 1e-5"), 0;
+    // root type for RootNodeIsConstant
+    uint32_t rootW0 = 0;
+    if (!rc->check(hipMemcpyAsync(&rootW0, tex, 4, hipMemcpyDeviceToHost, st), "read root") ||
+        !rc->check(hipStreamSynchronize(st), "synchronize"))
+        return 0;
+    U.rootConst = (rootW0 & kConstBit) ? 1 : 0;
+    // jitter table: 32 x 32 values in [0, 1) from the seed
+    std::vector<float> jit(1024);
+    uint64_t s = prm->jitter_seed;
+    for (int k = 0; k < 1024; ++k) jit[k] = (float)(splitmix64(s) >> 40) * (1.0f / 16777216.0f);
+    if (!rc->check(hipMemcpyAsync(rc->jitter.p, jit.data(), 4096, hipMemcpyHostToDevice, st),
+                   "jitter upload"))
+        return 0;
+    rc->W = cam->width;
+    rc->H = cam->height;
+    const uint64_t npix = (uint64_t)rc->W * rc->H;
+    if (++rc->tag == 0) {  // stamp wrap: clear
+        launch_fill_u32(st, rc->stamp.as<uint32_t>(), (uint64_t)nd[0] * nd[1] * nd[2], 0u);
+        rc->tag = 1;
+    }
+    launch_fill_u32(st, rc->counts.as<uint32_t>(), 2, 0u);
+    (void)hipEventRecord(rc->ev[0], st);
+    dim3 grid((rc->W + 15) / 16, (rc->H + 15) / 16);
+    k_raycast<<<grid, 256, 0, st>>>(U, tex, o.poolCol, rc->jitter.as<float>(), rc->rgba.as<uint32_t>(),
+                                    rc->usage.as<uint32_t>());
+    (void)hipEventRecord(rc->ev[1], st);
+    const uint64_t nent = npix * kMaxUsage;
+    const unsigned rgrid = (unsigned)std::min<uint64_t>((nent + 255) / 256, 8192);
+    k_requests<<<rgrid, 256, 0, st>>>(rc->usage.as<uint32_t>(), nent, rc->tag, nd[0], nd[1],
+                                      nd[2], rc->stamp.as<uint32_t>(), rc->req.as<uint32_t>(), rc->maxReq,
+                                      rc->counts.as<uint32_t>());
+    (void)hipEventRecord(rc->ev[2], st);
+    uint32_t cnt[2] = {0, 0};
+    if (!rc->check(hipGetLastError(), "raycast kernels") ||
+        !rc->check(hipMemcpyAsync(cnt, rc->counts.p, 8, hipMemcpyDeviceToHost, st), "counts") ||
+        !rc->check(hipStreamSynchronize(st), "synchronize"))
+        return 0;
+    rc->nreq = cnt[0];
+    rc->nreqTotal = cnt[1];
+    float a = 0, b = 0;
+    (void)hipEventElapsedTime(&a, rc->ev[0], rc->ev[1]);
+    (void)hipEventElapsedTime(&b, rc->ev[1], rc->ev[2]);
+    rc->msRay = a;
+    rc->msReq = b;
+    return 1;
+}
+
+int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out) {
+    if (!rc || !rc->valid || !out) return 0;
+    out->width = rc->W;
+    out->height = rc->H;
+    out->rgba = rc->rgba.as<uint32_t>();
+    out->usage = rc->usage.as<uint32_t>();
+    out->requests = rc->req.as<uint32_t>();
+    out->nrequests = rc->nreq;
+    out->nrequests_total = rc->nreqTotal;
+    out->ms_raycast = rc->msRay;
+    out->ms_requests = rc->msReq;
+    return 1;
+}
+
+}  // extern "C"
