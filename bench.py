@@ -77,6 +77,59 @@ def kernel_bytes(name, scene, st, bricks, B, launches):
     return 0.0
 
 
+def serving_bench(vz, scene, frames: int, W: int, H: int, slots, max_up: int):
+    """Paged flythrough over the voxelized octree: per frame a W x H sparse
+    octree ray cast emits brick requests and the brick server admits them into
+    the pool (LRU).  Returns throughput and request latency figures."""
+    import numpy as np
+    from voxvizthesis_amd import scenes
+    from voxvizthesis_amd.gvpool import BrickPool, OctTree
+    from voxvizthesis_amd.gvserve import BrickServer, RayCaster, camera, raycast_params
+
+    bp = BrickPool(scene.B, slots)
+    t = OctTree(bp, vz)
+    rc = RayCaster(t, W, H)
+    srv = BrickServer(t, max_up)
+    lo = np.asarray(scene.p, np.float32)
+    hi = lo + np.asarray(scene.N, np.float32) * np.asarray(scene.delta, np.float32)
+    prm = raycast_params(lo, hi, jitter_seed=5)
+    path = scenes.flythrough(seed=5, frames=frames)
+    ray_ms, dedup_ms, upd_ms, req = [], [], [], 0
+    t0 = time.perf_counter()
+    for f, (pos, look, up) in enumerate(path):
+        v = rc.frame(camera(pos, look, up, 60.0, W, H, f), prm)
+        ray_ms.append(v.ms_raycast)
+        dedup_ms.append(v.ms_requests)
+        req += v.nrequests
+        st = srv.frame(rc)
+        upd_ms.append(st.ms_update_last)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    lat = srv.latencies_us() / 1e3
+    out = {
+        "workload": f"{frames}-frame seeded Catmull-Rom flythrough (12 points, altitude "
+                    f"0.05-0.2 extent) over the full octree, {W}x{H} ray cast",
+        "frames": frames,
+        "pool_slots": int(np.prod(slots)),
+        "max_uploads_per_frame": max_up,
+        "requests": int(req),
+        "uploads": int(st.uploads),
+        "evictions": int(st.evictions),
+        "pending_at_end": int(st.pending),
+        "bricks_per_s": round(st.uploads / wall, 1),
+        "frame_ms": round(wall / frames * 1e3, 3),
+        "raycast_ms_mean": round(float(np.mean(ray_ms)), 4),
+        "dedup_ms_mean": round(float(np.mean(dedup_ms)), 4),
+        "update_ms_mean": round(float(np.mean(upd_ms)), 4),
+        "latency_p50_ms": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
+        "latency_p99_ms": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
+        "latency_max_ms": round(float(lat.max()), 4) if len(lat) else None,
+    }
+    for o in (srv, rc, t, bp):
+        o.close()
+    return out
+
+
 def cpu_baseline(sample_n: int):
     import oracle
     from voxvizthesis_amd import scenes
@@ -106,6 +159,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--serve-frames", type=int, default=300)
+    ap.add_argument("--serve-width", type=int, default=1920)
+    ap.add_argument("--serve-height", type=int, default=1080)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the voxelizer "
@@ -226,6 +283,13 @@ def main():
             "algorithmic_bytes_per_launch": int(kb),
         },
     }
+    if not args.no_serve:
+        log(f"[bench] serving {args.serve_frames} frames at {args.serve_width}x"
+            f"{args.serve_height} ...")
+        sv = serving_bench(vz, scene, args.serve_frames, args.serve_width, args.serve_height,
+                           (56, 56, 56), 4096)
+        out["serving"] = sv
+        log(f"[bench] serving: {sv['frame_ms']} ms/frame, p99 {sv['latency_p99_ms']} ms")
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline on a {args.cpu_sample}^3 sample ...")
         cb = cpu_baseline(args.cpu_sample)

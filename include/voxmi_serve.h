@@ -81,6 +81,53 @@ const char* vox_raycaster_get_error(vox_raycaster* rc);
 int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycast_params* prm);
 int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out);
 
+/* ---- brick server ---------------------------------------------------------
+ * Per frame (vox_server_frame), on the device:
+ *  1. every requested node with a resident brick marks its slot used this
+ *     frame (BrickPool::notifyUsed, GigaVoxelsBrickPool.cpp 406-427); every
+ *     requested NON-CONST node without one becomes pending, and so does its
+ *     parent when that is NON-CONST without a brick (BrickPool::update uploads
+ *     the parent too, GigaVoxelsBrickPool.cpp 631-676);
+ *  2. the frame's new pending nodes join a FIFO in ascending texel order;
+ *  3. up to max_uploads_per_frame FIFO entries are admitted into slots not
+ *     used this frame: empty slots first, then least recently used, ties by
+ *     slot index; an evicted node loses its resident bit;
+ *  4. the admitted bricks are copied from the voxelizer result into their
+ *     slots and their node texels get the resident bit and brick pointer.
+ * Entries that find no slot stay in the FIFO for later frames.  The latency
+ * of an admitted request is the device clock from the ray cast appending it
+ * to the end of its brick copy. */
+typedef struct vox_server vox_server;
+
+typedef struct vox_server_stats {
+    uint64_t frames;
+    uint64_t requests;          /* distinct request ids seen, over frames    */
+    uint64_t uploads;           /* bricks copied into the pool               */
+    uint64_t evictions;
+    uint32_t pending;           /* FIFO entries waiting for a slot           */
+    uint32_t resident;          /* slots holding a brick                     */
+    uint32_t last_uploads;      /* admitted in the last frame                */
+    double   ms_update_last;    /* HIP-event time of the last update         */
+    double   ms_update_total;
+    double   clock_khz;         /* device clock the latencies are counted in */
+} vox_server_stats;
+
+/* Server over one octree and its brick pool.  max_uploads_per_frame bounds
+ * the admissions per frame (0: the pool's slot count). */
+vox_server* vox_server_create(vox_octree* tree, uint32_t max_uploads_per_frame);
+void vox_server_destroy(vox_server* s);
+const char* vox_server_get_error(vox_server* s);
+/* Serve the request list of the ray caster's last frame.  Synchronous. */
+int vox_server_frame(vox_server* s, vox_raycaster* rc);
+int vox_server_get_stats(vox_server* s, vox_server_stats* out);
+/* Latencies (device clock ticks, see clock_khz) of every admitted request
+ * since creation, in admission order; copies min(n, cap) values, *n = total. */
+int vox_server_latencies(vox_server* s, uint64_t* out, uint64_t cap, uint64_t* n);
+/* Slot of each node texel (0xffffffff: none) and node texel of each slot,
+ * device pointers. */
+int vox_server_state(vox_server* s, const uint32_t** node_slot, const uint32_t** slot_node,
+                     uint64_t* ntexels, uint32_t* nslots);
+
 #ifdef __cplusplus
 }
 #endif

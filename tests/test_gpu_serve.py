@@ -1,0 +1,165 @@
+"""Serving loop on the GPU against the oracles, frame by frame: the ray cast
+(bit-exact per pixel against the ray-cast oracle on the current pool), then
+the brick server (node texture, slot tables and every resident brick against
+the server oracle and the voxelizer's bricks).  A second test serves two
+camera streams through one server and checks that each stream's bricks come
+out the same as when that stream is served alone."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle.server import NONE, RefServer
+from voxvizthesis_amd import scenes
+from voxvizthesis_amd._lib import VOX_OPT_MIP_CHAIN
+from voxvizthesis_amd.gvpool import BrickPool, OctTree
+from voxvizthesis_amd.gvserve import BrickServer, RayCaster, camera, raycast_params
+from voxvizthesis_amd.voxelizer import voxelize_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def look_at(pos, target, up=(0.0, 0.0, 1.0)):
+    pos, target, up = (np.asarray(v, np.float64) for v in (pos, target, up))
+    look = target - pos
+    look /= np.linalg.norm(look)
+    up = up - up.dot(look) * look
+    up /= np.linalg.norm(up)
+    return look.astype(np.float32), up.astype(np.float32)
+
+
+def path(n, W, H, start, end, target):
+    out = []
+    for f in range(n):
+        a = f / max(n - 1, 1)
+        pos = tuple((1 - a) * s + a * e for s, e in zip(start, end))
+        look, up = look_at(pos, target)
+        out.append(dict(position=pos, look=look, up=up, fov_y_deg=50.0, width=W, height=H,
+                        frame=f))
+    return out
+
+
+class Rig:
+    def __init__(self, scene, slots, max_up, W, H):
+        self.scene = scene
+        self.vz = voxelize_scene(scene, options=VOX_OPT_MIP_CHAIN)
+        self.bp = BrickPool(scene.B, slots)
+        self.t = OctTree(self.bp, self.vz)
+        self.rc = [RayCaster(self.t, W, H)]
+        self.srv = BrickServer(self.t, max_up)
+        self.res = self.vz.result()
+        self.L = self.res["levels"]
+        lo = np.asarray(scene.p, np.float32)
+        hi = lo + np.asarray(scene.N, np.float32) * np.asarray(scene.delta, np.float32)
+        self.prm = dict(vol_min=lo, vol_max=hi, jitter_seed=11)
+        self.texel = [self.t.node_texel(d) for d in range(self.L)]
+        self.info = {}
+        for d, tx in enumerate(self.texel):
+            for n in np.nonzero(tx != NONE)[0]:
+                self.info[int(tx[n])] = (d, int(n))
+
+    def ref_server(self, slots, max_up):
+        types = [tr["types"] for tr in self.res["tree"]]
+        bidx = [tr["bidx"] for tr in self.res["tree"]]
+        return RefServer(self.t.nodes(), types, bidx, self.texel, slots, self.scene.B, max_up)
+
+    def brick_of(self, texel):
+        d, n = self.info[texel]
+        tr = self.res["tree"][d]
+        b = int(tr["bidx"][n])
+        return tr["bcol"][b], tr["bnrm"][b]
+
+    def check_pool(self):
+        """Every slot holding a node holds that node's brick."""
+        col, nrm = self.bp.textures()
+        _, slot_node = self.srv.state()
+        sl = self.bp.view().slots
+        e = [b + 2 for b in self.scene.B]
+        for s, t in enumerate(slot_node):
+            if t == NONE:
+                continue
+            ox, oy, oz = (s % sl[0]) * e[0], ((s // sl[0]) % sl[1]) * e[1], \
+                (s // (sl[0] * sl[1])) * e[2]
+            bc, bn = self.brick_of(int(t))
+            assert (col[oz:oz + e[2], oy:oy + e[1], ox:ox + e[0]].ravel() == bc).all()
+            assert (nrm[oz:oz + e[2], oy:oy + e[1], ox:ox + e[0]].ravel() == bn).all()
+        return {int(t): s for s, t in enumerate(slot_node) if t != NONE}
+
+    def close(self):
+        for o in [self.srv] + self.rc + [self.t, self.bp, self.vz]:
+            o.close()
+
+
+def test_serving_flythrough_matches_oracles():
+    W, H, slots, max_up = 64, 48, (4, 4, 4), 16
+    rig = Rig(scenes.scene_small_terrain(), slots, max_up, W, H)
+    try:
+        ref = rig.ref_server(slots, max_up)
+        frames = path(6, W, H, (0.5, -0.8, 1.4), (0.45, 0.2, 0.75), (0.5, 0.55, 0.5))
+        total = 0
+        for cam in frames:
+            print(f"[serve] frame {cam['frame']} pos {cam['position']}", flush=True)
+            nodes = rig.t.nodes()
+            assert (nodes == ref.texture()).all()
+            rig.rc[0].frame(camera(**cam), raycast_params(**rig.prm))
+            ru, rcol = oracle.raycast(cam, rig.prm, rig.L, rig.scene.B, nodes,
+                                      rig.bp.textures()[0])
+            assert (rig.rc[0].usage() == ru).all() and (rig.rc[0].rgba() == rcol).all()
+            st = rig.srv.frame(rig.rc[0])
+            n = ref.step(sorted(set(ru[ru != 0].tolist())))
+            assert st.last_uploads == n
+            total += n
+            assert (rig.t.nodes() == ref.texture()).all()
+            node_slot, _ = rig.srv.state()
+            got = {int(t): int(s) for t, s in enumerate(node_slot) if s != NONE}
+            assert got == ref.node_slot
+            assert st.pending == len(ref.fifo)
+            rig.check_pool()
+        st = rig.srv.stats()
+        assert st.uploads == total and total > 0
+        assert st.evictions > 0 or total <= 64
+        lat = rig.srv.latencies_us()
+        assert len(lat) == total and (lat > 0).all()
+    finally:
+        rig.close()
+
+
+def test_two_streams_same_alone_and_mixed():
+    """Stream A served alone and stream A mixed with stream B through one
+    server: every node A's run made resident and the mixed run made resident
+    too reads, through its node texel's brick pointer, the same brick -- the
+    voxelizer's brick for that node -- in both runs."""
+    W, H = 48, 32
+    A = path(4, W, H, (0.5, -0.8, 1.4), (0.5, 0.0, 0.9), (0.5, 0.55, 0.5))
+    B = path(4, W, H, (1.6, 1.3, 1.2), (1.1, 1.0, 0.8), (0.45, 0.5, 0.5))
+    sc = scenes.scene_small_terrain()
+
+    def serve(streams):
+        rig = Rig(sc, (16, 16, 16), 0, W, H)
+        try:
+            while len(rig.rc) < len(streams):
+                rig.rc.append(RayCaster(rig.t, W, H))
+            for f in range(len(streams[0])):
+                for rc, cams in zip(rig.rc, streams):
+                    rc.frame(camera(**cams[f]), raycast_params(**rig.prm))
+                    rig.srv.frame(rc)
+            rig.check_pool()
+            nodes = rig.t.nodes().reshape(-1, 2)
+            col, _ = rig.bp.textures()
+            e = [b + 2 for b in sc.B]
+            out = {}
+            for t in np.nonzero(nodes[:, 0] & 0x80000000)[0]:
+                w1 = int(nodes[t, 1])
+                x, y, z = ((w1 >> 20) & 1023) - 1, ((w1 >> 10) & 1023) - 1, (w1 & 1023) - 1
+                out[int(t)] = col[z:z + e[2], y:y + e[1], x:x + e[0]].tobytes()
+                assert out[int(t)] == rig.brick_of(int(t))[0].tobytes()
+            return out
+        finally:
+            rig.close()
+
+    alone = serve([A])
+    mixed = serve([A, B])
+    common = set(alone) & set(mixed)
+    assert len(alone) > 8 and len(common) >= len(alone) // 2
+    assert len(mixed) > len(alone)
+    for t in common:
+        assert alone[t] == mixed[t]

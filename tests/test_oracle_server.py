@@ -1,0 +1,75 @@
+"""Pins the brick-server oracle (oracle/server.py) on a nine-node tree with a
+two-slot pool, frame by frame: FIFO order within a frame, eviction of the
+least recently used slot (ties by slot index), protection of slots used in
+the current frame, the parent rule, deferral when no slot is free, and the
+node texel bits after each admission."""
+import numpy as np
+
+from oracle.server import NONE, RB, RefServer
+
+CB = 0x40000000
+
+
+def pk(x, y, z):
+    return (x << 20) | (y << 10) | z
+
+
+def tree():
+    tex = np.zeros((2, 2, 4, 2), np.uint32)            # 16 texels: dims (4, 2, 2)
+    tex[0, 0, 0] = (RB | pk(2, 0, 0), pk(1, 1, 1))     # root: brick in slot 0
+    types1 = np.array([1, 1, 1, 0, 1, 2, 1, 1], np.uint8)
+    texel1 = np.zeros(8, np.uint32)
+    for i in range(8):
+        x, y, z = 2 + (i & 1), (i >> 1) & 1, i >> 2
+        tex[z, y, x, 0] = CB if types1[i] != 1 else 0
+        texel1[i] = (z * 2 + y) * 4 + x
+    types = [np.array([1], np.uint8), types1]
+    bidx = [np.array([0], np.uint32), np.cumsum(types1 == 1).astype(np.uint32) - 1]
+    texel = [np.array([0], np.uint32), texel1]
+    return tex, types, bidx, texel
+
+
+def nid(t, nd=(4, 2, 2)):
+    x, y, z = t % nd[0], (t // nd[0]) % nd[1], t // (nd[0] * nd[1])
+    return ((x + 1) << 20) | ((y + 1) << 10) | (z + 1)
+
+
+def test_lru_fifo_parent_and_deferral():
+    tex, types, bidx, texel = tree()
+    s = RefServer(tex, types, bidx, texel, slots=(2, 1, 1), B=(2, 2, 2), max_up=2)
+    c = [int(t) for t in texel[1]]
+    assert s.node_slot == {0: 0}
+    # frame 1: two children; slot 1 is empty (key 0), slot 0 holds the root
+    # (unused this frame): both taken, in slot order, FIFO in texel order
+    assert s.step([nid(c[1]), nid(c[0]), nid(c[3])]) == 2       # c3 is CONST: ignored
+    assert s.admitted[-1] == [(c[0], 0), (c[1], 1)]
+    t = s.texture().reshape(-1, 2)
+    assert not (t[0, 0] & RB)                                    # root evicted
+    assert t[c[0], 0] & RB and t[c[0], 1] == pk(1, 1, 1)
+    assert t[c[1], 0] & RB and t[c[1], 1] == pk(5, 1, 1)         # slot 1 origin x = 4
+    # frame 2: c0 used again (slot 0 protected); c2 requested: its parent,
+    # the root, is NON-CONST without a brick and is queued before it
+    assert s.step([nid(c[0]), nid(c[2])]) == 1
+    assert s.admitted[-1] == [(0, 1)]                            # root into slot 1
+    assert list(s.fifo) == [c[2]]                                # deferred
+    t = s.texture().reshape(-1, 2)
+    assert not (t[c[1], 0] & RB) and t[0, 0] & RB
+    # frame 3: nothing requested; both slots age 1, the lower index goes
+    assert s.step([]) == 1
+    assert s.admitted[-1] == [(c[2], 0)]
+    assert s.node_slot == {0: 1, c[2]: 0}
+    assert not s.fifo and not s.pending
+
+
+def test_requests_outside_tree_and_constants_are_ignored():
+    tex, types, bidx, texel = tree()
+    s = RefServer(tex, types, bidx, texel, slots=(2, 1, 1), B=(2, 2, 2), max_up=2)
+    c = [int(t) for t in texel[1]]
+    bad = nid(1)                     # a sibling texel of the root: no node there
+    assert 1 not in [int(t) for t in texel[1]] + [0]
+    assert s.step([bad, nid(c[3]), nid(c[5])]) == 0    # CONST, LEAF-CONST
+    assert not s.fifo
+    # the resident root touched: its slot is protected this frame
+    assert s.step([nid(0), nid(c[0]), nid(c[1])]) == 1
+    assert s.admitted[-1] == [(c[0], 1)]
+    assert list(s.fifo) == [c[1]]

@@ -36,6 +36,8 @@ POOL_EXPORTS = [
 SERVE_EXPORTS = [
     "vox_raycaster_create", "vox_raycaster_destroy", "vox_raycaster_get_error",
     "vox_raycast_frame", "vox_raycaster_get_frame",
+    "vox_server_create", "vox_server_destroy", "vox_server_get_error", "vox_server_frame",
+    "vox_server_get_stats", "vox_server_latencies", "vox_server_state",
 ]
 VOX_TREE_REFERENCE_CONST = 1
 VOX_TREE_EXACT_CONST = 2
@@ -108,6 +110,13 @@ class VoxFrameView(C.Structure):
                 ("ms_requests", C.c_double)]
 
 
+class VoxServerStats(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("requests", C.c_uint64), ("uploads", C.c_uint64),
+                ("evictions", C.c_uint64), ("pending", C.c_uint32), ("resident", C.c_uint32),
+                ("last_uploads", C.c_uint32), ("ms_update_last", C.c_double),
+                ("ms_update_total", C.c_double), ("clock_khz", C.c_double)]
+
+
 _lib = None
 
 
@@ -170,5 +179,17 @@ def load():
     L.vox_raycaster_get_error.argtypes = [vp]
     L.vox_raycast_frame.argtypes = [vp, C.POINTER(VoxCamera), C.POINTER(VoxRaycastParams)]
     L.vox_raycaster_get_frame.argtypes = [vp, C.POINTER(VoxFrameView)]
+    # brick server
+    L.vox_server_create.restype = vp
+    L.vox_server_create.argtypes = [vp, C.c_uint32]
+    L.vox_server_destroy.argtypes = [vp]
+    L.vox_server_destroy.restype = None
+    L.vox_server_get_error.restype = C.c_char_p
+    L.vox_server_get_error.argtypes = [vp]
+    L.vox_server_frame.argtypes = [vp, vp]
+    L.vox_server_get_stats.argtypes = [vp, C.POINTER(VoxServerStats)]
+    L.vox_server_latencies.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+    L.vox_server_state.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
     _lib = L
     return L

@@ -9,6 +9,7 @@
 #include <sstream>
 
 #include "../../include/voxmi_pool.h"
+#include "../../include/voxmi_serve.h"
 
 namespace voxmi {
 
@@ -92,8 +93,18 @@ struct OctreeRefs {
     uint32_t* poolCol;
     uint32_t* poolNrm;
     hipStream_t st;         // the pool's stream
+    // per depth: node types, brick index, node texel (0xffffffff: not in tree)
+    const uint8_t* types[kMaxLevels];
+    const uint32_t* bidx[kMaxLevels];
+    const uint32_t* texel[kMaxLevels];
+    BrickSrc src;           // the voxelizer result's bricks
 };
 bool octree_refs(vox_octree* t, OctreeRefs& r);
+
+// The request list of a ray caster's last frame: distinct node ids, the
+// device clock when each was appended, and the count (host-side).
+bool raycaster_requests(vox_raycaster* rc, const uint32_t** ids, const uint64_t** times,
+                        uint32_t* n);
 
 // ---- launchers (all asynchronous on `st`) ----
 void launch_setup(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,

@@ -138,6 +138,36 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, u
     return pre + inc - v;
 }
 
+// Walk over the (group, tile) bins of a box of tiles, x fastest.
+struct TileWalk {
+    uint32_t x0, x1, y0, y1, z1, x, y, z;
+    uint32_t TGx, TGy;
+    uint64_t base, key;
+    __device__ TileWalk(const VoxParams& P, int g, uint32_t ax0, uint32_t ax1, uint32_t ay0,
+                        uint32_t ay1, uint32_t az0, uint32_t az1)
+        : x0(ax0), x1(ax1), y0(ay0), y1(ay1), z1(az1), x(ax0), y(ay0), z(az0), TGx(P.TG[0]),
+          TGy(P.TG[1]), base((uint64_t)g * P.numTiles) {
+        key = base + ((uint64_t)z * TGy + y) * TGx + x;
+    }
+    // advance; false past the last bin
+    __device__ bool next() {
+        if (++x > x1) {
+            x = x0;
+            if (++y > y1) {
+                y = y0;
+                if (++z > z1) return false;
+            }
+        }
+        key = base + ((uint64_t)z * TGy + y) * TGx + x;
+        return true;
+    }
+};
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int l) {
+    const uint32_t lo = __shfl((uint32_t)v, l), hi = __shfl((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // ---------------------------------------------------------------------------
 // setup: ComputeEdgesFaceNormalsAndBounds (VoxelizerKernels.cu 230-347) plus
 // the count of 8^3 tiles each triangle's bounds touch, per (group, tile) bin.
@@ -176,13 +206,21 @@ __global__ void k_setup(VoxParams P, GroupArgs G, int g, const float* __restrict
     uint32_t z0 = mn[2] > G.zStart ? mn[2] : G.zStart;
     uint32_t z1 = mx[2] < G.zEnd ? mx[2] : G.zEnd;
     if (z0 > z1) return;
-    const uint64_t base = (uint64_t)g * P.numTiles;
-    for (uint32_t tz = z0 >> 3; tz <= (z1 >> 3); ++tz)
-        for (uint32_t ty = mn[1] >> 3; ty <= (mx[1] >> 3); ++ty)
-            for (uint32_t tx = mn[0] >> 3; tx <= (mx[0] >> 3); ++tx) {
-                uint64_t lin = ((uint64_t)tz * P.TG[1] + ty) * P.TG[0] + tx;
-                atomicAdd(&binCount[base + lin], 1u);
-            }
+    TileWalk w(P, g, mn[0] >> 3, mx[0] >> 3, mn[1] >> 3, mx[1] >> 3, z0 >> 3, z1 >> 3);
+    // Neighbouring triangles mostly share tiles: lanes holding the same bin
+    // add once, with the count of the group.
+    bool act = true;
+    while (__ballot(act)) {
+        uint64_t pend = __ballot(act);
+        while (pend) {
+            const int l = __ffsll((unsigned long long)pend) - 1;
+            const uint64_t lk = shfl_u64(w.key, l);
+            const uint64_t same = __ballot(act && w.key == lk);
+            if ((int)(threadIdx.x & 63u) == l) atomicAdd(&binCount[lk], (uint32_t)__popcll(same));
+            pend &= ~same;
+        }
+        if (act) act = w.next();
+    }
 }
 
 __global__ void k_scatter(VoxParams P, GroupArgs G, int g, const uint32_t* __restrict__ bbox,
@@ -195,14 +233,28 @@ __global__ void k_scatter(VoxParams P, GroupArgs G, int g, const uint32_t* __res
     uint32_t z0 = (bz & 0xffffu) > G.zStart ? (bz & 0xffffu) : G.zStart;
     uint32_t z1 = (bz >> 16) < G.zEnd ? (bz >> 16) : G.zEnd;
     if (z0 > z1) return;
-    const uint64_t base = (uint64_t)g * P.numTiles;
-    for (uint32_t tz = z0 >> 3; tz <= (z1 >> 3); ++tz)
-        for (uint32_t ty = (by & 0xffffu) >> 3; ty <= ((by >> 16) >> 3); ++ty)
-            for (uint32_t tx = (bx & 0xffffu) >> 3; tx <= ((bx >> 16) >> 3); ++tx) {
-                uint64_t key = base + ((uint64_t)tz * P.TG[1] + ty) * P.TG[0] + tx;
-                uint32_t pos = binStart[key] + atomicAdd(&binFill[key], 1u);
-                binTri[pos] = (uint32_t)t;
-            }
+    TileWalk w(P, g, (bx & 0xffffu) >> 3, (bx >> 16) >> 3, (by & 0xffffu) >> 3, (by >> 16) >> 3,
+               z0 >> 3, z1 >> 3);
+    // One atomic per group of lanes holding the same bin; lanes take
+    // consecutive places after the group's base.
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint64_t below = (1ull << lane) - 1ull;
+    bool act = true;
+    while (__ballot(act)) {
+        uint64_t pend = __ballot(act);
+        while (pend) {
+            const int l = __ffsll((unsigned long long)pend) - 1;
+            const uint64_t lk = shfl_u64(w.key, l);
+            const uint64_t same = __ballot(act && w.key == lk);
+            uint32_t base = 0;
+            if (lane == l) base = atomicAdd(&binFill[lk], (uint32_t)__popcll(same));
+            base = __shfl(base, l);
+            if (act && w.key == lk)
+                binTri[binStart[lk] + base + (uint32_t)__popcll(same & below)] = (uint32_t)t;
+            pend &= ~same;
+        }
+        if (act) act = w.next();
+    }
 }
 
 // A level-0 tile is active when any group has triangles binned to it, or when
@@ -261,7 +313,7 @@ __global__ void k_mark_parent(LevelMap src, LevelMap dst, uint32_t* __restrict__
 // on one tile, then ComputeColorsAndNormals / ComputeColorsAsNormals
 // (VoxelizerKernels.cu 689-1083) for the tile's hit voxels.
 constexpr int kStage = 128;     // triangles staged per round
-constexpr int kTriStride = 33;  // floats per staged triangle (padded)
+constexpr int kTriStride = 35;  // floats per staged triangle (padded, odd)
 
 // Bilinear tex2D with normalized coordinates (VoxelizerKernels.cu 586-600):
 // fractional weights held in 8 fractional bits; NaN coordinates read as 0.
@@ -536,8 +588,24 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
             float* st = &s_tri[j * kTriStride];
             if (pr == 0u) {
                 st[0] = n[0]; st[1] = n[1]; st[2] = n[2];
-                st[3] = dot3(f3{-n[0], -n[1], -n[2]}, f3{v[0][0], v[0][1], v[0][2]});
+                const float D = dot3(f3{-n[0], -n[1], -n[2]}, f3{v[0][0], v[0][1], v[0][2]});
+                st[3] = D;
                 stage_pair<0, 1, 2>(v, n, P.offP, P.dP, st + 4);
+                // Plane-test bounds: the 8 corner distances are dist1 plus
+                // the sums of the negative / positive parts of n_i * dP_i;
+                // thr bounds the fp32 error of any corner distance (256 unit
+                // roundoffs of the largest |n . p| + |D| over the grid).
+                float neg = 0.0f, pos = 0.0f, M = fabsf(D);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float o = n[k] * P.dP[k];
+                    neg += fminf(o, 0.0f);
+                    pos += fmaxf(o, 0.0f);
+                    M += fabsf(n[k]) * (fabsf(P.offP[k]) + (float)(P.S[k] + 1u) * fabsf(P.dP[k]));
+                }
+                st[31] = neg;
+                st[32] = pos;
+                st[33] = M * (1.0f / 65536.0f);
             } else if (pr == 1u) {
                 stage_pair<2, 0, 1>(v, n, P.offP, P.dP, st + 13);
             } else {
@@ -593,8 +661,15 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
             const f3 p0 = {P.offP[0] + uvw[0] * P.dP[0], P.offP[1] + uvw[1] * P.dP[1],
                            P.offP[2] + uvw[2] * P.dP[2]};
             const float dist1 = dot3(n, p0) + D;
+            // Certain outcomes first: every corner on one side (no hit) or
+            // corners on both sides beyond the error bound (hit).  Only the
+            // band where a corner may round to the other side runs the
+            // reference's 8-corner products.
+            const float lo = dist1 + st[31], hi = dist1 + st[32], thr = st[33];
             bool planeHit = false;
-            {
+            if (lo < -thr && hi > thr) {
+                planeHit = true;
+            } else if (!(lo > thr || hi < -thr)) {
                 const float dx = P.dP[0], dy = P.dP[1], dz = P.dP[2];
                 const f3 cd[7] = {{dx, dy, dz}, {dx, dy, 0.0f}, {dx, 0.0f, 0.0f}, {0.0f, 0.0f, dz},
                                   {0.0f, dy, dz}, {dx, 0.0f, dz}, {0.0f, dy, 0.0f}};

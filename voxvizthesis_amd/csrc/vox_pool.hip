@@ -180,7 +180,7 @@ __global__ void k_brick_upload(BrickSrc src, const uint4* __restrict__ req,
                                const uint32_t* __restrict__ nreq, uint32_t ex, uint32_t ey,
                                uint32_t ez, uint32_t sx, uint32_t sy, uint32_t dx, uint32_t dy,
                                uint32_t* __restrict__ col, uint32_t* __restrict__ nrm,
-                               uint32_t* __restrict__ tex) {
+                               uint32_t* __restrict__ tex, unsigned long long* __restrict__ done) {
     const uint32_t count = *nreq;
     const uint32_t nv = ex * ey * ez;
     for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
@@ -203,6 +203,10 @@ __global__ void k_brick_upload(BrickSrc src, const uint4* __restrict__ req,
             tex[2 * (uint64_t)q.w] |= kResidentBit;
             tex[2 * (uint64_t)q.w + 1] = pack10(ox + 1, oy + 1, oz + 1);
         }
+        if (done) {
+            __syncthreads();  // the whole brick is written
+            if (threadIdx.x == 0) done[r] = (unsigned long long)wall_clock64();
+        }
     }
 }
 
@@ -213,7 +217,19 @@ void launch_brick_upload(hipStream_t st, const BrickSrc& src, const uint4* req,
     if (maxReq == 0) return;
     const unsigned grid = maxReq < 4096u ? maxReq : 4096u;
     k_brick_upload<<<grid, kPoolBlock, 0, st>>>(src, req, nreq, e[0], e[1], e[2], slots[0],
-                                                slots[1], dims[0], dims[1], col, nrm, tex);
+                                                slots[1], dims[0], dims[1], col, nrm, tex,
+                                                nullptr);
+}
+
+// Same, recording the device clock when each request's copy has finished.
+void launch_brick_upload_timed(hipStream_t st, const BrickSrc& src, const uint4* req,
+                               const uint32_t* nreq, uint32_t maxReq, const uint32_t e[3],
+                               const uint32_t slots[3], const uint32_t dims[3], uint32_t* col,
+                               uint32_t* nrm, uint32_t* tex, unsigned long long* done) {
+    if (maxReq == 0) return;
+    const unsigned grid = maxReq < 4096u ? maxReq : 4096u;
+    k_brick_upload<<<grid, kPoolBlock, 0, st>>>(src, req, nreq, e[0], e[1], e[2], slots[0],
+                                                slots[1], dims[0], dims[1], col, nrm, tex, done);
 }
 
 }  // namespace voxmi
@@ -431,6 +447,13 @@ bool octree_refs(vox_octree* t, OctreeRefs& r) {
     r.poolCol = t->bp->col.as<uint32_t>();
     r.poolNrm = t->bp->nrm.as<uint32_t>();
     r.st = t->bp->st;
+    for (int d = 0; d < kMaxLevels; ++d) {
+        const bool in = d < t->L;
+        r.types[d] = in ? t->types[d].as<uint8_t>() : nullptr;
+        r.bidx[d] = in ? t->bidx[d].as<uint32_t>() : nullptr;
+        r.texel[d] = in ? t->texel[d].as<uint32_t>() : nullptr;
+    }
+    r.src = t->src;
     return true;
 }
 

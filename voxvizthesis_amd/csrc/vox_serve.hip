@@ -354,7 +354,8 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
 __global__ void __launch_bounds__(256)
 k_requests(const uint32_t* __restrict__ usage, uint64_t n, uint32_t tag, uint32_t ndx,
            uint32_t ndy, uint32_t ndz, uint32_t* __restrict__ stamp, uint32_t* __restrict__ list,
-           uint32_t cap, uint32_t* __restrict__ counts) {
+           unsigned long long* __restrict__ times, uint32_t cap,
+           uint32_t* __restrict__ counts) {
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n;
          base += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t e = base + threadIdx.x;
@@ -378,6 +379,7 @@ k_requests(const uint32_t* __restrict__ usage, uint64_t n, uint32_t tag, uint32_
                 const uint32_t k = atomicAdd(&counts[1], 1u);
                 if (k < cap) {
                     list[k] = id;
+                    times[k] = (unsigned long long)wall_clock64();  // emitted
                     atomicAdd(&counts[0], 1u);
                 }
             }
@@ -401,7 +403,7 @@ struct vox_raycaster {
     uint32_t maxReq = 0, nreq = 0, nreqTotal = 0, tag = 0;
     uint32_t ndim[3] = {0, 0, 0};
     hipStream_t st = nullptr;
-    DevBuf rgba, usage, req, counts, stamp, jitter;
+    DevBuf rgba, usage, req, reqTime, counts, stamp, jitter;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     double msRay = 0, msReq = 0;
     std::stringstream err;
@@ -416,6 +418,19 @@ struct vox_raycaster {
         return false;
     }
 };
+
+namespace voxmi {
+
+bool raycaster_requests(vox_raycaster* rc, const uint32_t** ids, const uint64_t** times,
+                        uint32_t* n) {
+    if (!rc || !rc->valid) return false;
+    *ids = rc->req.as<uint32_t>();
+    *times = rc->reqTime.as<uint64_t>();
+    *n = rc->nreq;
+    return true;
+}
+
+}  // namespace voxmi
 
 namespace {
 
@@ -519,6 +534,7 @@ vox_raycaster* vox_raycaster_create(vox_octree* tree, int32_t max_width, int32_t
     if (!rc->rgba.ensure(4 * npix, rc->err, "rgba") ||
         !rc->usage.ensure(4 * npix * kMaxUsage, rc->err, "usage") ||
         !rc->req.ensure(4ull * max_requests, rc->err, "requests") ||
+        !rc->reqTime.ensure(8ull * max_requests, rc->err, "request times") ||
         !rc->counts.ensure(8, rc->err, "counts") || !rc->stamp.ensure(4 * ntex, rc->err, "stamp") ||
         !rc->jitter.ensure(4 * 1024, rc->err, "jitter"))
         return rc;
@@ -538,6 +554,7 @@ void vox_raycaster_destroy(vox_raycaster* rc) {
     rc->rgba.release();
     rc->usage.release();
     rc->req.release();
+    rc->reqTime.release();
     rc->counts.release();
     rc->stamp.release();
     rc->jitter.release();
@@ -591,7 +608,8 @@ int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycas
     const uint64_t nent = npix * kMaxUsage;
     const unsigned rgrid = (unsigned)std::min<uint64_t>((nent + 255) / 256, 8192);
     k_requests<<<rgrid, 256, 0, st>>>(rc->usage.as<uint32_t>(), nent, rc->tag, nd[0], nd[1],
-                                      nd[2], rc->stamp.as<uint32_t>(), rc->req.as<uint32_t>(), rc->maxReq,
+                                      nd[2], rc->stamp.as<uint32_t>(), rc->req.as<uint32_t>(),
+                                      rc->reqTime.as<unsigned long long>(), rc->maxReq,
                                       rc->counts.as<uint32_t>());
     (void)hipEventRecord(rc->ev[2], st);
     uint32_t cnt[2] = {0, 0};

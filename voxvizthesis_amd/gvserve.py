@@ -1,8 +1,12 @@
-"""Sparse-octree ray cast producing brick requests (include/voxmi_serve.h).
+"""Sparse-octree ray cast producing brick requests, and the brick server
+that admits them into the pool (include/voxmi_serve.h).
 
 `RayCaster` runs the front-to-back traversal of VoxViz/Shaders/GigaVoxels.frag
 over an `OctTree`'s node texture and brick pool on the device and returns the
 frame's per-pixel node usage and its distinct requested node ids.
+`BrickServer` takes a ray caster's request list each frame and uploads the
+missing bricks into least recently used slots (BrickPool::update,
+GigaVoxelsBrickPool.cpp 561-682), all on the device.
 """
 from __future__ import annotations
 
@@ -80,6 +84,64 @@ class RayCaster:
     def close(self):
         if getattr(self, "_h", None):
             self._L.vox_raycaster_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BrickServer:
+    def __init__(self, octree, max_uploads_per_frame=0):
+        self._L = _lib.load()
+        self.tree = octree
+        self._h = self._L.vox_server_create(octree._h, int(max_uploads_per_frame))
+        if not self._h:
+            raise MemoryError("vox_server_create")
+        err = self.getErrorMessage()
+        if err:
+            self.close()
+            raise RuntimeError(err)
+
+    def getErrorMessage(self) -> str:
+        return self._L.vox_server_get_error(self._h).decode()
+
+    def frame(self, raycaster: RayCaster):
+        if not self._L.vox_server_frame(self._h, raycaster._h):
+            raise RuntimeError(self.getErrorMessage())
+        return self.stats()
+
+    def stats(self) -> _lib.VoxServerStats:
+        s = _lib.VoxServerStats()
+        if not self._L.vox_server_get_stats(self._h, C.byref(s)):
+            raise RuntimeError(self.getErrorMessage())
+        return s
+
+    def latencies_us(self) -> np.ndarray:
+        n = C.c_uint64()
+        self._L.vox_server_latencies(self._h, None, 0, C.byref(n))
+        out = np.empty(n.value, np.uint64)
+        if n.value:
+            self._L.vox_server_latencies(self._h, out.ctypes.data, n.value, C.byref(n))
+        khz = self.stats().clock_khz
+        return out.astype(np.float64) * (1e3 / khz)
+
+    def state(self):
+        """(slot of each node texel, node texel of each slot) as host arrays."""
+        ns, sn = C.c_void_p(), C.c_void_p()
+        nt, nsl = C.c_uint64(), C.c_uint32()
+        self._L.vox_server_state(self._h, C.byref(ns), C.byref(sn), C.byref(nt), C.byref(nsl))
+        a = np.empty(nt.value, np.uint32)
+        b = np.empty(nsl.value, np.uint32)
+        self._L.vox_pool_copy_to_host(a.ctypes.data, ns, a.nbytes)
+        self._L.vox_pool_copy_to_host(b.ctypes.data, sn, b.nbytes)
+        return a, b
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.vox_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

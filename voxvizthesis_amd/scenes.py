@@ -318,3 +318,45 @@ def scene_terrain_n(cells: int, N: int, B: int, seed: int = 11) -> Scene:
     """Heightfield of (cells+1)^2 vertices on an N^3 grid of B^3 bricks."""
     return Scene(meshes=[heightfield_terrain(cells, seed)], N=(N, N, N), B=(B, B, B),
                  name=f"terrain{cells + 1}_N{N}_B{B}")
+
+
+def flythrough(seed: int = 5, frames: int = 300, n_points: int = 12, extent: float = 1.0,
+               terrain_seed: int = 2, alt=(0.05, 0.2)):
+    """Seeded Catmull-Rom camera path over the terrain: n_points control
+    points uniform in the inner 80% of the xy extent at altitude alt above the
+    fBm surface; the camera looks along the path's horizontal heading, tilted
+    down by atan(0.1 / 0.25).  Returns a list of (position, look, up)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    h = terrain_height_fn(terrain_seed, extent)
+    xy = rng.uniform(0.1 * extent, 0.9 * extent, size=(n_points, 2))
+    z = h(xy[:, 0], xy[:, 1]) + rng.uniform(alt[0] * extent, alt[1] * extent, size=n_points)
+    P = np.concatenate([xy, z[:, None]], axis=1)
+    P = np.concatenate([P[:1], P, P[-1:]], axis=0)   # end tangents
+
+    def cr(t):
+        seg = min(int(t), n_points - 2)
+        u = t - seg
+        p0, p1, p2, p3 = P[seg], P[seg + 1], P[seg + 2], P[seg + 3]
+        return 0.5 * ((2 * p1) + (-p0 + p2) * u + (2 * p0 - 5 * p1 + 4 * p2 - p3) * u * u
+                      + (-p0 + 3 * p1 - 3 * p2 + p3) * u ** 3)
+
+    out = []
+    span = n_points - 1
+    for f in range(frames):
+        t = span * f / max(frames - 1, 1)
+        pos = cr(t)
+        if t + 0.25 <= span:
+            ahead = cr(t + 0.25)
+        else:                                    # past the last point: keep heading
+            ahead = pos + (pos - cr(max(t - 0.25, 0.0)))
+        fwd = ahead - pos
+        fwd[2] = 0.0
+        if np.linalg.norm(fwd) < 1e-9:
+            fwd = np.array([1.0, 0.0, 0.0])
+        fwd = 0.25 * extent * fwd / np.linalg.norm(fwd)
+        look = fwd + np.array([0.0, 0.0, -0.1 * extent])
+        look = look / np.linalg.norm(look)
+        up = np.array([0.0, 0.0, 1.0]) - look[2] * look
+        up /= np.linalg.norm(up)
+        out.append((pos.astype(np.float32), look.astype(np.float32), up.astype(np.float32)))
+    return out
