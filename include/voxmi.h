@@ -151,8 +151,11 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h);
 /* cuda::Voxelizer::generateVoxelsAndOctTree (Voxelizer.cu 2180-2814).
  * Returns 1 ok, 0 error, -1 no group overlaps the grid.  out_dir may be NULL
  * (results stay in HBM, see vox_get_result); otherwise tree.gvx and
- * voxels_<depth>.gvb are written there.  compressed must be 0 (the DXT path
- * relied on the OpenGL driver, VoxelBrickWriter.cu 590-723). */
+ * voxels_<depth>.gvb are written there.  compressed = 1 also produces
+ * block-compressed bricks (BC3 colours, BC1 normals, see vox_get_compressed)
+ * and writes them to the .gvb files, as WriteCompressedBinaryBrick does
+ * (VoxelBrickWriter.cu 590-723); the reference had the OpenGL driver encode
+ * them, here the project's deterministic encoder does. */
 int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed);
 /* cuda::Voxelizer::getErrorMessage (Voxelizer.cu 3176-3187). */
 const char* vox_get_error(vox_ctx* h);
@@ -168,6 +171,21 @@ int vox_copy_to_host(vox_ctx* h, void* dst, const void* src_device, uint64_t nby
  * dims receives S_k. */
 int vox_copy_level(vox_ctx* h, int32_t level, uint32_t* colors, float* normals,
                    uint32_t dims[3]);
+/* Block-compressed bricks of the last vox_generate(..., compressed = 1):
+ * per depth, bricks * blocks_per_brick blocks, blocks_per_brick =
+ * ceil((B+2)/4)^2 * (B+2) (each slice of a brick in 4x4 blocks, row-major,
+ * slices in z).  Colours: 16-byte BC3 blocks; normals: 8-byte BC1 blocks.
+ * Returns 0 when the last generate was not compressed.  Not in the
+ * reference (which kept compressed bricks only in files). */
+typedef struct vox_compressed {
+    int32_t        levels;
+    uint64_t       blocks_per_brick;
+    uint64_t       bricks[VOX_MAX_LEVELS];
+    const uint8_t* colors[VOX_MAX_LEVELS];
+    const uint8_t* normals[VOX_MAX_LEVELS];
+} vox_compressed;
+int vox_get_compressed(vox_ctx* h, vox_compressed* out);
+
 /* Write tree.gvx and voxels_<depth>.gvb of the last vox_generate into
  * out_dir (host-side writer, Voxelizer.cu 2816-3061, VoxelBrickWriter.cu
  * 725-818). */

@@ -107,6 +107,21 @@ def lib():
             f.restype = C.POINTER(C.c_uint32)
             f.argtypes = [C.c_void_p, C.c_int32]
         L.ref_pool_free.argtypes = [C.c_void_p]
+        L.ref_pool_compressed.restype = C.c_int32
+        L.ref_pool_compressed.argtypes = [C.c_void_p]
+        L.ref_pool_file_compressed.restype = C.POINTER(C.c_uint8)
+        L.ref_pool_file_compressed.argtypes = [C.c_void_p, C.c_int32, C.c_int32,
+                                               C.POINTER(C.c_uint64)]
+        # block compression (ref_bc.h)
+        L.ref_bc_blocks_per_brick.restype = C.c_uint64
+        L.ref_bc_blocks_per_brick.argtypes = [C.POINTER(C.c_uint32)]
+        L.ref_bc_encode_bricks.restype = None
+        L.ref_bc_encode_bricks.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64,
+                                           C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p]
+        L.ref_bc1_decode.restype = None
+        L.ref_bc1_decode.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.ref_bc3_decode.restype = None
+        L.ref_bc3_decode.argtypes = [C.c_void_p, C.c_void_p]
         # ray cast (ref_raycast.h)
         u3p = C.POINTER(C.c_uint32)
         L.ref_raycast.restype = C.c_int32
@@ -250,7 +265,17 @@ def pool_load(directory: str, slots=(8, 8, 8), store_const: bool = True) -> RefP
             fc = _copy(L.ref_pool_file_brick_colors(h, d), n * bvox, np.uint32).reshape(n, bvox)
             fn = _copy(L.ref_pool_file_brick_normals(h, d), n * bvox, np.uint32).reshape(n, bvox)
             fb.append((fc, fn))
-        return RefPool(tuple(nd), int(blocks.value), nodes, tuple(bd), bcol, bnrm, fb)
+        out = RefPool(tuple(nd), int(blocks.value), nodes, tuple(bd), bcol, bnrm, fb)
+        out.compressed = bool(L.ref_pool_compressed(h))
+        out.file_compressed = []
+        for d in range(L.ref_pool_depths(h)):
+            pair = []
+            for which in (0, 1):
+                nbytes = C.c_uint64()
+                ptr = L.ref_pool_file_compressed(h, d, which, C.byref(nbytes))
+                pair.append(_copy(ptr, nbytes.value, np.uint8))
+            out.file_compressed.append(tuple(pair))
+        return out
     finally:
         L.ref_pool_free(h)
 
@@ -286,3 +311,34 @@ def raycast(camera: dict, params: dict, levels: int, B, nodes: np.ndarray, pool:
     if not ok:
         raise RuntimeError("ray cast oracle: empty volume box")
     return usage, rgba
+
+
+def bc_encode_bricks(col: np.ndarray, nrm: np.ndarray, e):
+    """Block-compress bricks (nbricks, e^3 texels): (BC3 colour bytes, BC1
+    normal bytes), blocks_per_brick * 16 / * 8 bytes per brick."""
+    L = lib()
+    e3 = (C.c_uint32 * 3)(*e)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    nrm = np.ascontiguousarray(nrm, dtype=np.uint32)
+    nb = col.shape[0]
+    blocks = int(L.ref_bc_blocks_per_brick(e3))
+    oc = np.zeros(nb * blocks * 16, np.uint8)
+    on = np.zeros(nb * blocks * 8, np.uint8)
+    if nb:
+        L.ref_bc_encode_bricks(col.ctypes.data, nrm.ctypes.data, nb, e3, oc.ctypes.data,
+                               on.ctypes.data)
+    return oc, on
+
+
+def bc1_decode(block: bytes, four_always: bool = False) -> np.ndarray:
+    out = np.zeros(16, np.uint32)
+    b = np.frombuffer(bytes(block), np.uint8).copy()
+    lib().ref_bc1_decode(b.ctypes.data, 1 if four_always else 0, out.ctypes.data)
+    return out
+
+
+def bc3_decode(block: bytes) -> np.ndarray:
+    out = np.zeros(16, np.uint32)
+    b = np.frombuffer(bytes(block), np.uint8).copy()
+    lib().ref_bc3_decode(b.ctypes.data, out.ctypes.data)
+    return out

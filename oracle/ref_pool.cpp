@@ -93,7 +93,9 @@ struct ref_pool {
     std::vector<uint32_t> bcol, bnrm;  // brick pool textures
     int depths = 0;
     std::vector<std::vector<uint32_t>> fcol, fnrm;  // bricks read from .gvb
+    std::vector<std::vector<uint8_t>> fccomp, fncomp;  // compressed payloads
     std::vector<uint64_t> fcount;
+    int compressed = 0;
     int32_t flags = 0;
 };
 
@@ -186,6 +188,7 @@ bool load_bricks(ref_pool* p, const std::string& path, int depth) {
     }
     int32_t compressed = 0;
     f.read((char*)&compressed, 4);
+    if (compressed) p->compressed = 1;
     auto& nc = p->nonConst[depth];
     uint64_t count = 0;
     for (size_t i = 0; i < nc.size(); ++i) {
@@ -229,6 +232,17 @@ bool load_bricks(ref_pool* p, const std::string& path, int depth) {
             return false;
         }
         const uint64_t nv = dims[0] * dims[1] * dims[2];
+        if (compressed) {
+            // compressed payloads are kept as bytes (the reader only sizes them)
+            p->fccomp[depth].insert(p->fccomp[depth].end(), col.begin(), col.end());
+            p->fncomp[depth].insert(p->fncomp[depth].end(), nrm.begin(), nrm.end());
+            Node& n = p->nodes[nc[i]];
+            n.brickTex[0] = (int)border[0];
+            n.brickTex[1] = (int)border[1];
+            n.brickTex[2] = (int)border[2];
+            ++count;
+            continue;
+        }
         if (col.size() != nv * 4 || nrm.size() != nv * 3) {
             g_err = "brick size does not match its dimensions";
             return false;
@@ -259,7 +273,8 @@ bool load_bricks(ref_pool* p, const std::string& path, int depth) {
 void init_root_brick(ref_pool* p) {
     Node& root = p->nodes[0];
     if (root.w[0] & 0x40000000u) return;
-    if (p->slots[0] * p->slots[1] * p->slots[2] == 0 || p->fcount.empty() || p->fcount[0] == 0)
+    if (p->slots[0] * p->slots[1] * p->slots[2] == 0 || p->fcount.empty() || p->fcount[0] == 0 ||
+        p->compressed)
         return;
     const uint32_t e[3] = {p->B[0] + 2, p->B[1] + 2, p->B[2] + 2};
     for (uint32_t z = 0; z < e[2]; ++z)
@@ -372,6 +387,8 @@ extern "C" ref_pool* ref_pool_load(const char* dir, const uint32_t slots[3], int
     }
     p->fcol.assign(p->depths, {});
     p->fnrm.assign(p->depths, {});
+    p->fccomp.assign(p->depths, {});
+    p->fncomp.assign(p->depths, {});
     p->fcount.assign(p->depths, 0);
     for (int i = 0; i < p->depths; ++i) {
         if (p->nonConst[i].empty()) continue;
@@ -413,5 +430,12 @@ extern "C" const uint32_t* ref_pool_file_brick_colors(const ref_pool* p, int32_t
 }
 extern "C" const uint32_t* ref_pool_file_brick_normals(const ref_pool* p, int32_t d) {
     return p->fnrm[d].data();
+}
+extern "C" int32_t ref_pool_compressed(const ref_pool* p) { return p->compressed; }
+extern "C" const uint8_t* ref_pool_file_compressed(const ref_pool* p, int32_t d, int32_t normals,
+                                                   uint64_t* nbytes) {
+    const auto& v = normals ? p->fncomp[d] : p->fccomp[d];
+    *nbytes = v.size();
+    return v.data();
 }
 extern "C" void ref_pool_free(ref_pool* p) { delete p; }

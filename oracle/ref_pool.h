@@ -55,6 +55,12 @@ int32_t         ref_pool_depths(const ref_pool* p);
 uint64_t        ref_pool_file_bricks(const ref_pool* p, int32_t depth);
 const uint32_t* ref_pool_file_brick_colors(const ref_pool* p, int32_t depth);
 const uint32_t* ref_pool_file_brick_normals(const ref_pool* p, int32_t depth);
+/* Compressed .gvb files (flag 1): the payloads of depth d as read, colours
+ * (normals = 0) or normals (normals = 1); the root's brick is not uploaded
+ * (the pool here holds uncompressed texels). */
+int32_t         ref_pool_compressed(const ref_pool* p);
+const uint8_t*  ref_pool_file_compressed(const ref_pool* p, int32_t depth, int32_t normals,
+                                         uint64_t* nbytes);
 void            ref_pool_free(ref_pool* p);
 
 #ifdef __cplusplus

@@ -24,7 +24,7 @@ EXPORTS = [
     "vox_deallocate_voxel_memory", "vox_allocate_triangle_memory",
     "vox_deallocate_triangle_memory", "vox_compute_edges_face_normals_and_bounds",
     "vox_generate", "vox_get_error", "vox_get_result", "vox_get_stats", "vox_copy_to_host",
-    "vox_copy_level", "vox_write_files",
+    "vox_copy_level", "vox_write_files", "vox_get_compressed",
 ]
 # Every symbol include/voxmi_pool.h declares.
 POOL_EXPORTS = [
@@ -64,6 +64,15 @@ class VoxResult(C.Structure):
         ("brick_colors", C.c_void_p * VOX_MAX_LEVELS),
         ("brick_normals", C.c_void_p * VOX_MAX_LEVELS),
         ("brick_node", C.c_void_p * VOX_MAX_LEVELS),
+    ]
+
+
+class VoxCompressed(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int32), ("blocks_per_brick", C.c_uint64),
+        ("bricks", C.c_uint64 * VOX_MAX_LEVELS),
+        ("colors", C.c_void_p * VOX_MAX_LEVELS),
+        ("normals", C.c_void_p * VOX_MAX_LEVELS),
     ]
 
 
@@ -154,6 +163,7 @@ def load():
     L.vox_copy_to_host.argtypes = [vp, vp, vp, C.c_uint64]
     L.vox_copy_level.argtypes = [vp, C.c_int32, vp, vp, u3]
     L.vox_write_files.argtypes = [vp, C.c_char_p]
+    L.vox_get_compressed.argtypes = [vp, C.POINTER(VoxCompressed)]
     # node pool / brick pool
     L.vox_brick_pool_create.restype = vp
     L.vox_brick_pool_create.argtypes = [u3, u3]

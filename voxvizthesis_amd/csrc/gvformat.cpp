@@ -6,7 +6,9 @@
 // voxels_<depth>.gvb follows VoxelBrickWriter::startBricksFile and
 // WriteBinaryBrick / WriteUByteBinaryBrick (VoxelBrickWriter.cu 71-124,
 // 725-818, 509-588): int32 compressed flag, then per brick in index order
-// u64 dims, u32 borders, int32 0, u32 size, RGBA8, int32 0, u32 size, RGB8.
+// u64 dims, u32 borders, int32 0, u32 size, RGBA8, int32 0, u32 size, RGB8;
+// compressed (WriteCompressedBinaryBrick, 590-723): the same with flags 1
+// and BC3 colour / BC1 normal blocks.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -73,6 +75,8 @@ extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float
                                     const float delta[3], const uint32_t N[3]) {
     vox_result r;
     if (!vox_get_result(h, &r)) return 0;
+    vox_compressed cz;
+    const bool compressed = vox_get_compressed(h, &cz) == 1;
     mkdir(out_dir, 0755);
     HostTree t;
     t.L = r.levels;
@@ -94,10 +98,39 @@ extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float
         std::string fn = std::string(out_dir) + "/voxels_" + std::to_string(d) + ".gvb";
         std::ofstream f(fn, std::ios_base::out | std::ios_base::binary);
         if (!f.is_open()) return 0;
-        int32_t compressed = 0;
-        f.write((const char*)&compressed, 4);
+        const int32_t cflag = compressed ? 1 : 0;
+        f.write((const char*)&cflag, 4);
         const uint64_t nb = r.bricks[d];
         const uint64_t chunk = 4096;
+        if (compressed) {
+            // WriteCompressedBinaryBrick: BC3 colours, BC1 normals per brick
+            const uint64_t cb = cz.blocks_per_brick * 16, nbb = cz.blocks_per_brick * 8;
+            std::vector<uint8_t> cc, cn;
+            for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+                const uint64_t m = std::min<uint64_t>(chunk, nb - b0);
+                cc.resize(m * cb);
+                cn.resize(m * nbb);
+                if (!vox_copy_to_host(h, cc.data(), cz.colors[d] + b0 * cb, m * cb) ||
+                    !vox_copy_to_host(h, cn.data(), cz.normals[d] + b0 * nbb, m * nbb))
+                    return 0;
+                for (uint64_t j = 0; j < m; ++j) {
+                    const uint64_t dims[3] = {t.B[0] + 2, t.B[1] + 2, t.B[2] + 2};
+                    const uint32_t border[3] = {1, 1, 1};
+                    f.write((const char*)dims, sizeof(dims));
+                    f.write((const char*)border, sizeof(border));
+                    uint32_t sz = (uint32_t)cb;
+                    f.write((const char*)&cflag, 4);
+                    f.write((const char*)&sz, 4);
+                    f.write((const char*)&cc[j * cb], cb);
+                    sz = (uint32_t)nbb;
+                    f.write((const char*)&cflag, 4);
+                    f.write((const char*)&sz, 4);
+                    f.write((const char*)&cn[j * nbb], nbb);
+                }
+            }
+            if (f.fail()) return 0;
+            continue;
+        }
         std::vector<uint32_t> col, nrm;
         std::vector<uint8_t> rgb(bvox * 3);
         for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
@@ -139,7 +172,8 @@ extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float
        << "VolumeXSize=\"" << N[0] << "\" " << "VolumeYSize=\"" << N[1] << "\" "
        << "VolumeZSize=\"" << N[2] << "\" " << "BrickXSize=\"" << t.B[0] << "\" "
        << "BrickYSize=\"" << t.B[1] << "\" " << "BrickZSize=\"" << t.B[2] << "\" "
-       << "Binary=\"YES\" " << "Compressed=\"NO\" >" << std::endl;
+       << "Binary=\"YES\" " << "Compressed=\"" << (compressed ? "YES" : "NO") << "\" >"
+       << std::endl;
     write_node(tf, t, 0, 0, 0, 0, 0, 0, 0);
     tf << "</GigaVoxelsOctTree>" << std::endl;
     return tf.fail() ? 0 : 1;

@@ -55,6 +55,7 @@ struct DepthBufs {
     uint32_t Rd = 0;
     uint64_t nodes = 0, bricks = 0;
     DevBuf types, cc, flag, excl, bidx, brickNode, bcol, bnrm;
+    DevBuf ccol, cnrm;  // block-compressed bricks (compressed output)
 };
 
 struct TimedLaunch {
@@ -74,6 +75,7 @@ struct vox_ctx {
     bool triMem = false;
     bool prepared = false;
     bool haveResult = false;
+    bool compressed = false;  // the last generate produced compressed bricks
     uint32_t options = 0;
     float p[3] = {0, 0, 0}, dP[3] = {1, 1, 1}, offP[3] = {0, 0, 0};
     std::stringstream err;
@@ -549,11 +551,7 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
 
 int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     if (!h || !h->valid) return 0;
-    if (compressed) {
-        h->fail("compressed (DXT) bricks are not supported: the reference delegated the encoding "
-                "to the OpenGL driver");
-        return 0;
-    }
+    h->compressed = compressed != 0;
     if (!h->prepared && !vox_compute_edges_face_normals_and_bounds(h)) return 0;
     const int G = (int)h->groups.size();
     const int L = h->L;
@@ -727,6 +725,22 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
                                 t.bnrm.as<uint32_t>());
         });
     }
+    // Compressed output: BC3 colours, BC1 normals (WriteCompressedBinaryBrick,
+    // VoxelBrickWriter.cu 590-723, with the project's encoder).
+    if (h->compressed) {
+        const uint32_t e[3] = {h->B[0] + 2, h->B[1] + 2, h->B[2] + 2};
+        const uint64_t blocks = bc_blocks_per_brick(e);
+        for (int d = 0; d < L; ++d) {
+            DepthBufs& t = h->tree[d];
+            if (!t.ccol.ensure(t.bricks * blocks * 16, h->err, "compressed colours") ||
+                !t.cnrm.ensure(t.bricks * blocks * 8, h->err, "compressed normals"))
+                return 0;
+            h->run("compress", [&] {
+                launch_bc_bricks(st, t.bcol.as<uint32_t>(), t.bnrm.as<uint32_t>(), t.bricks, e,
+                                 t.ccol.as<uint32_t>(), t.cnrm.as<uint32_t>());
+            });
+        }
+    }
     // CollapseConstantBranches (Voxelizer.cu 701-774).
     for (int d = L - 2; d >= 0; --d) {
         h->run("collapse", [&] {
@@ -859,6 +873,21 @@ int vox_copy_level(vox_ctx* h, int32_t level, uint32_t* colors, float* normals, 
 
 int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3], const float delta[3],
                          const uint32_t N[3]);
+
+int vox_get_compressed(vox_ctx* h, vox_compressed* out) {
+    if (!h || !h->valid || !out) return 0;
+    if (!h->haveResult || !h->compressed) return 0;
+    std::memset(out, 0, sizeof(*out));
+    const uint32_t e[3] = {h->B[0] + 2, h->B[1] + 2, h->B[2] + 2};
+    out->levels = h->L;
+    out->blocks_per_brick = bc_blocks_per_brick(e);
+    for (int d = 0; d < h->L; ++d) {
+        out->bricks[d] = h->tree[d].bricks;
+        out->colors[d] = h->tree[d].ccol.as<uint8_t>();
+        out->normals[d] = h->tree[d].cnrm.as<uint8_t>();
+    }
+    return 1;
+}
 
 int vox_write_files(vox_ctx* h, const char* out_dir) {
     if (!h || !h->valid || !out_dir) return 0;

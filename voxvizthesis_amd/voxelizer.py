@@ -149,6 +149,19 @@ class Voxelizer:
         return (col.reshape(dims[2], dims[1], dims[0]),
                 nrm.reshape(dims[2], dims[1], dims[0], 3))
 
+    def compressed(self):
+        """Per depth (BC3 colour bytes, BC1 normal bytes) of the last
+        generateVoxelsAndOctTree(..., outputCompressed=True), or None."""
+        z = _lib.VoxCompressed()
+        if not self._L.vox_get_compressed(self._h, C.byref(z)):
+            return None
+        out = []
+        for d in range(z.levels):
+            nb = z.bricks[d] * z.blocks_per_brick
+            out.append((self._d2h(z.colors[d], nb * 16, np.uint8),
+                        self._d2h(z.normals[d], nb * 8, np.uint8)))
+        return out
+
     def writeFiles(self, outputDir) -> bool:
         return bool(self._L.vox_write_files(self._h, str(outputDir).encode()))
 
@@ -164,7 +177,8 @@ class Voxelizer:
             pass
 
 
-def voxelize_scene(scene, options: int = 0, chunk=None) -> Voxelizer:
+def voxelize_scene(scene, options: int = 0, chunk=None, out_dir=None,
+                   compressed: bool = False) -> Voxelizer:
     """Run the reference call sequence (main.cu Voxelize 3225-3341) on a
     synthetic Scene and return the voxelizer holding the device result."""
     vz = Voxelizer(scene.N, scene.B, chunk, options=options)
@@ -175,7 +189,7 @@ def voxelize_scene(scene, options: int = 0, chunk=None) -> Voxelizer:
     ok = ok and vz.allocateTriangleMemory() and vz.computeEdgesFaceNormalsAndBounds()
     if not ok:
         raise RuntimeError(vz.getErrorMessage())
-    st = vz.generateVoxelsAndOctTree(None)
+    st = vz.generateVoxelsAndOctTree(out_dir, True, compressed)
     if st == 0:
         raise RuntimeError(vz.getErrorMessage())
     vz.status = st
