@@ -130,6 +130,72 @@ def serving_bench(vz, scene, frames: int, W: int, H: int, slots, max_up: int):
     return out
 
 
+def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 5):
+    """Batched voxelization: n_jobs independent meshes (half fBm heightfield
+    tiles, half random soups, triangle counts log-uniform 2K..tmax, grids from
+    `grids`) captured into one hipGraph.  The autotuner picks each job's brick
+    edge (8/16/32) and persistent launch shape by timing eager runs."""
+    import numpy as np
+    from voxvizthesis_amd import scenes
+    from voxvizthesis_amd.batch import Batch, autotune, make_job
+
+    jobs_sc = scenes.batch_scenes(seed=4, n_jobs=n_jobs, tmax=tmax, grids=grids)
+    choices, jobs = [], []
+    t0 = time.perf_counter()
+    for sc in jobs_sc:
+        if do_autotune:
+            B, pb, _ = autotune(sc, reps=1)
+        else:
+            B, pb = 8, 0
+        choices.append((B, pb))
+        jobs.append(make_job(sc, B, pb))
+    t_tune = time.perf_counter() - t0
+    # eager: every job's setup + generate back to back (host round trips)
+    eager = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for v in jobs:
+            v.computeEdgesFaceNormalsAndBounds()
+            v.generateVoxelsAndOctTree(None)
+        eager.append(time.perf_counter() - t0)
+    b = Batch(jobs)
+    b.capture()
+    graph, check = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        if not b.launch():
+            raise RuntimeError(b.getErrorMessage())
+        graph.append(time.perf_counter() - t0)
+        st = b.stats()
+        check.append(st.ms_check_last)
+    st = b.stats()
+    T = sum(sc.ntri for sc in jobs_sc)
+    vox = sum(sc.N[0] ** 3 for sc in jobs_sc)
+    g = float(np.median(graph))
+    out = {
+        "workload": f"{n_jobs} meshes (fBm tiles + random soups), triangles log-uniform "
+                    f"2K-{int(tmax)}, grids {list(grids)}",
+        "jobs": n_jobs,
+        "triangles": int(T),
+        "voxels": int(vox),
+        "autotuned": bool(do_autotune),
+        "brick_edges": {str(k): sum(1 for c in choices if c[0] == k) for k in (8, 16, 32)},
+        "autotune_s": round(t_tune, 2),
+        "eager_ms": round(float(np.median(eager)) * 1e3, 3),
+        "graph_ms": round(g * 1e3, 3),
+        "graph_device_ms": round(st.ms_graph_last, 3),
+        "checksum_ms": round(float(np.median(check)), 3),
+        "graph_nodes": int(st.graph_nodes),
+        "meshes_per_s": round(n_jobs / g, 1),
+        "gvoxels_per_s": round(vox / g / 1e9, 3),
+        "mtriangles_per_s": round(T / g / 1e6, 2),
+    }
+    b.close()
+    for v in jobs:
+        v.close()
+    return out
+
+
 def cpu_baseline(sample_n: int):
     import oracle
     from voxvizthesis_amd import scenes
@@ -163,6 +229,11 @@ def main():
     ap.add_argument("--serve-width", type=int, default=1920)
     ap.add_argument("--serve-height", type=int, default=1080)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=32)
+    ap.add_argument("--batch-tmax", type=float, default=2.6e5)
+    ap.add_argument("--batch-grids", default="64,128,256")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the voxelizer "
@@ -290,6 +361,13 @@ def main():
                            (56, 56, 56), 4096)
         out["serving"] = sv
         log(f"[bench] serving: {sv['frame_ms']} ms/frame, p99 {sv['latency_p99_ms']} ms")
+    if not args.no_batch:
+        log(f"[bench] batch of {args.batch_jobs} meshes ...")
+        bb = batch_bench(args.batch_jobs, args.batch_tmax,
+                         tuple(int(x) for x in args.batch_grids.split(",")),
+                         not args.no_autotune)
+        out["batched"] = bb
+        log(f"[bench] batch: graph {bb['graph_ms']} ms vs eager {bb['eager_ms']} ms")
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline on a {args.cpu_sample}^3 sample ...")
         cb = cpu_baseline(args.cpu_sample)

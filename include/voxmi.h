@@ -122,6 +122,10 @@ void vox_destroy(vox_ctx* h);
 int vox_set_params(vox_ctx* h, const float p[3], const float delta[3]);
 /* Options bit set (VOX_OPT_*); not in the reference. */
 int vox_set_options(vox_ctx* h, uint32_t options);
+/* Launch shape: workgroups of the persistent voxelize and colour kernels
+ * (default 8 per compute unit); the autotuner's knob.  Not in the
+ * reference. */
+int vox_set_launch(vox_ctx* h, uint32_t persistent_blocks);
 
 /* cuda::Voxelizer::addTriangleGroup (Voxelizer.h 128-162).  Host pointers;
  * like the reference, they are only read by vox_allocate_triangle_memory. */

@@ -1,4 +1,5 @@
-"""ctypes binding of libvoxmi.so (include/voxmi.h, voxmi_pool.h, voxmi_serve.h).
+"""ctypes binding of libvoxmi.so (include/voxmi.h, voxmi_pool.h, voxmi_serve.h,
+voxmi_batch.h).
 
 The HIP library is the product path: importing this module on a machine
 without the built library raises immediately, there is no CPU fallback.
@@ -24,7 +25,12 @@ EXPORTS = [
     "vox_deallocate_voxel_memory", "vox_allocate_triangle_memory",
     "vox_deallocate_triangle_memory", "vox_compute_edges_face_normals_and_bounds",
     "vox_generate", "vox_get_error", "vox_get_result", "vox_get_stats", "vox_copy_to_host",
-    "vox_copy_level", "vox_write_files", "vox_get_compressed",
+    "vox_copy_level", "vox_write_files", "vox_get_compressed", "vox_set_launch",
+]
+# Every symbol include/voxmi_batch.h declares.
+BATCH_EXPORTS = [
+    "vox_batch_create", "vox_batch_destroy", "vox_batch_get_error", "vox_batch_capture",
+    "vox_batch_launch", "vox_batch_get_stats",
 ]
 # Every symbol include/voxmi_pool.h declares.
 POOL_EXPORTS = [
@@ -74,6 +80,12 @@ class VoxCompressed(C.Structure):
         ("colors", C.c_void_p * VOX_MAX_LEVELS),
         ("normals", C.c_void_p * VOX_MAX_LEVELS),
     ]
+
+
+class VoxBatchStats(C.Structure):
+    _fields_ = [("jobs", C.c_int32), ("launches", C.c_uint64), ("ms_graph_last", C.c_double),
+                ("ms_check_last", C.c_double), ("ms_capture", C.c_double),
+                ("graph_nodes", C.c_uint64)]
 
 
 class VoxStats(C.Structure):
@@ -164,6 +176,17 @@ def load():
     L.vox_copy_level.argtypes = [vp, C.c_int32, vp, vp, u3]
     L.vox_write_files.argtypes = [vp, C.c_char_p]
     L.vox_get_compressed.argtypes = [vp, C.POINTER(VoxCompressed)]
+    L.vox_set_launch.argtypes = [vp, C.c_uint32]
+    # batch
+    L.vox_batch_create.restype = vp
+    L.vox_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32]
+    L.vox_batch_destroy.argtypes = [vp]
+    L.vox_batch_destroy.restype = None
+    L.vox_batch_get_error.restype = C.c_char_p
+    L.vox_batch_get_error.argtypes = [vp]
+    L.vox_batch_capture.argtypes = [vp]
+    L.vox_batch_launch.argtypes = [vp]
+    L.vox_batch_get_stats.argtypes = [vp, C.POINTER(VoxBatchStats)]
     # node pool / brick pool
     L.vox_brick_pool_create.restype = vp
     L.vox_brick_pool_create.argtypes = [u3, u3]

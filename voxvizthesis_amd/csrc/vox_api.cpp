@@ -9,6 +9,8 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
+#include <new>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -20,6 +22,7 @@
 #include <vector>
 
 #include "../../include/voxmi.h"
+#include "../../include/voxmi_batch.h"
 #include "vox_internal.h"
 
 using namespace voxmi;
@@ -76,6 +79,13 @@ struct vox_ctx {
     bool prepared = false;
     bool haveResult = false;
     bool compressed = false;  // the last generate produced compressed bricks
+    // Replay (graph capture) pass: no host round trips; the sizes are the
+    // ones the last eager pass measured.
+    bool replay = false;
+    bool planned = false;
+    uint64_t planPairs = 0;
+    uint32_t planAct[VOX_MAX_LEVELS] = {};
+    uint32_t planNb[VOX_MAX_LEVELS] = {};
     uint32_t options = 0;
     float p[3] = {0, 0, 0}, dP[3] = {1, 1, 1}, offP[3] = {0, 0, 0};
     std::stringstream err;
@@ -123,7 +133,7 @@ struct vox_ctx {
     }
     // Record a named span around launches when timing is on.
     template <class F> void run(const std::string& name, F&& f) {
-        if (options & VOX_OPT_TIMING) {
+        if ((options & VOX_OPT_TIMING) && !replay) {
             hipEvent_t a = ev(), b = ev();
             (void)hipEventRecord(a, st);
             f();
@@ -516,7 +526,9 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
         ga.zEnd = std::min(mx[2], h->S[2] - 1);
         fill[g] = (ga.overlaps && ga.textured && ga.terrain) ? 1 : 0;
     }
-    if (!h->check(hipMemcpyAsync(h->grpFill.p, fill.data(), 4 * (G + 1), hipMemcpyHostToDevice, h->st),
+    // (replay: the flags are already on the device from the eager pass)
+    if (!h->replay &&
+        !h->check(hipMemcpyAsync(h->grpFill.p, fill.data(), 4 * (G + 1), hipMemcpyHostToDevice, h->st),
                   "group flags"))
         return 0;
     h->numBins = (uint64_t)std::max(G, 1) * P.numTiles;
@@ -533,10 +545,15 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
     });
     h->run("scan", [&] { scan(h, cnt, start, h->numBins); });
     uint32_t total = 0;
-    if (!h->check(hipMemcpyAsync(&total, start + h->numBins, 4, hipMemcpyDeviceToHost, h->st),
-                  "bin total") ||
-        !h->check(hipStreamSynchronize(h->st), "setup"))
-        return 0;
+    if (h->replay) {
+        total = (uint32_t)h->planPairs;
+    } else {
+        if (!h->check(hipMemcpyAsync(&total, start + h->numBins, 4, hipMemcpyDeviceToHost, h->st),
+                      "bin total") ||
+            !h->check(hipStreamSynchronize(h->st), "setup"))
+            return 0;
+        h->planPairs = total;
+    }
     h->pairs = total;
     if (!h->binTri.ensure((uint64_t)total * 4, h->err, "bin entries")) return 0;
     h->run("fill", [&] { launch_fill_u32(h->st, cnt, h->numBins, 0u); });
@@ -552,13 +569,16 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
 int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     if (!h || !h->valid) return 0;
     h->compressed = compressed != 0;
+    if (h->replay && out_dir) return h->fail("a replayed generate writes no files"), 0;
     if (!h->prepared && !vox_compute_edges_face_normals_and_bounds(h)) return 0;
     const int G = (int)h->groups.size();
     const int L = h->L;
     VoxParams& P = h->P;
     hipStream_t st = h->st;
-    std::memset(&h->stats, 0, sizeof(h->stats));
-    (void)hipEventRecord(h->evStart, st);
+    if (!h->replay) {
+        std::memset(&h->stats, 0, sizeof(h->stats));
+        (void)hipEventRecord(h->evStart, st);
+    }
 
     int firstOverlap = -1;
     for (int g = 0; g < G; ++g)
@@ -594,9 +614,14 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
             return 0;
     }
     uint32_t act[VOX_MAX_LEVELS] = {0};
-    if (!h->check(hipMemcpyAsync(act, ctr, 4 * L, hipMemcpyDeviceToHost, st), "tile counts") ||
-        !h->check(hipStreamSynchronize(st), "tile maps"))
-        return 0;
+    if (h->replay) {
+        std::memcpy(act, h->planAct, sizeof(act));
+    } else {
+        if (!h->check(hipMemcpyAsync(act, ctr, 4 * L, hipMemcpyDeviceToHost, st), "tile counts") ||
+            !h->check(hipStreamSynchronize(st), "tile maps"))
+            return 0;
+        std::memcpy(h->planAct, act, sizeof(act));
+    }
     for (int k = 0; k < L; ++k) {
         LevelBufs& l = h->lv[k];
         if (l.aliasOf >= 0) {
@@ -699,9 +724,14 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         for (int a = 0; a < 3; ++a) border[a] = std::max(border[a] >> 1, 1u);
     }
     uint32_t nb[VOX_MAX_LEVELS] = {0};
-    if (!h->check(hipMemcpyAsync(nb, ctr + 32, 4 * L, hipMemcpyDeviceToHost, st), "brick counts") ||
-        !h->check(hipStreamSynchronize(st), "node typing"))
-        return 0;
+    if (h->replay) {
+        std::memcpy(nb, h->planNb, sizeof(nb));
+    } else {
+        if (!h->check(hipMemcpyAsync(nb, ctr + 32, 4 * L, hipMemcpyDeviceToHost, st), "brick counts") ||
+            !h->check(hipStreamSynchronize(st), "node typing"))
+            return 0;
+        std::memcpy(h->planNb, nb, sizeof(nb));
+    }
     const uint64_t bvox = (uint64_t)(h->B[0] + 2) * (h->B[1] + 2) * (h->B[2] + 2);
     for (int d = L - 1; d >= 0; --d) {
         int k = L - 1 - d;
@@ -759,11 +789,17 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
                              h->bitmap.as<uint32_t>());
         });
     }
+    if (h->replay) {
+        h->haveResult = true;
+        h->prepared = false;
+        return firstOverlap >= 0 ? 1 : -1;
+    }
     (void)hipEventRecord(h->evStop, st);
     unsigned long long sat = 0;
     if (!h->check(hipMemcpyAsync(&sat, satc, 8, hipMemcpyDeviceToHost, st), "counters") ||
         !h->check(hipStreamSynchronize(st), "generate"))
         return 0;
+    h->planned = true;
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, h->evStart, h->evStop);
     h->stats.ms_total = ms;
@@ -895,5 +931,269 @@ int vox_write_files(vox_ctx* h, const char* out_dir) {
         return h->fail(std::string("failed to write GigaVoxels files to ") + out_dir), 0;
     return 1;
 }
+
+
+// ---------------------------------------------------------------------------
+// batch: many handles captured into one hipGraph (include/voxmi_batch.h)
+
+namespace {
+
+void buffer_list(vox_ctx* h, std::vector<void*>& out) {
+    out.clear();
+    DevBuf* top[] = {&h->verts, &h->fnorm, &h->bbox, &h->vn, &h->uv, &h->img, &h->grpFill,
+                     &h->binCount, &h->binStart, &h->binTri, &h->scanTmp, &h->occ0, &h->hit0,
+                     &h->sat, &h->work, &h->workCount, &h->rec, &h->recCount, &h->bitmap,
+                     &h->counters};
+    for (DevBuf* b : top) out.push_back(b->p);
+    for (auto& l : h->lv) {
+        DevBuf* lb[] = {&l.mark, &l.excl, &l.tileIdx, &l.list, &l.col, &l.qn, &l.nrm};
+        for (DevBuf* b : lb) out.push_back(b->p);
+    }
+    for (auto& t : h->tree) {
+        DevBuf* tb[] = {&t.types, &t.cc, &t.flag, &t.excl, &t.bidx, &t.brickNode,
+                        &t.bcol,  &t.bnrm, &t.ccol, &t.cnrm};
+        for (DevBuf* b : tb) out.push_back(b->p);
+    }
+}
+
+// Host-side part of a job's signature: everything that sizes the pipeline
+// besides the triangle data.
+uint64_t host_signature(vox_ctx* h) {
+    uint64_t z = 1469598103934665603ull;
+    auto mixin = [&](const void* p, size_t n) {
+        const uint8_t* c = static_cast<const uint8_t*>(p);
+        for (size_t i = 0; i < n; ++i) z = (z ^ c[i]) * 1099511628211ull;
+    };
+    mixin(h->N, sizeof(h->N));
+    mixin(h->B, sizeof(h->B));
+    mixin(h->p, sizeof(h->p));
+    mixin(h->dP, sizeof(h->dP));
+    mixin(&h->options, sizeof(h->options));
+    mixin(&h->compressed, sizeof(h->compressed));
+    mixin(&h->persistentBlocks, sizeof(h->persistentBlocks));
+    for (auto& g : h->groups) {
+        const vox_group& d = g.desc;
+        mixin(&d.ntri, sizeof(d.ntri));
+        mixin(d.bbox_min, sizeof(d.bbox_min));
+        mixin(d.bbox_max, sizeof(d.bbox_max));
+        mixin(&d.is_terrain, sizeof(d.is_terrain));
+        const int32_t flags[6] = {d.vnormals != nullptr, d.uvs != nullptr, d.rgba8 != nullptr,
+                                  d.width, d.height, d.addr0 * 4 + d.addr1};
+        mixin(flags, sizeof(flags));
+    }
+    return z;
+}
+
+// Device checksum of a job's uploaded triangle data into *out.
+void launch_input_hash(vox_ctx* h, hipStream_t st, unsigned long long* out) {
+    uint64_t nvn = 0, nuv = 0, nimg = 0;
+    for (auto& g : h->groups) {
+        if (g.desc.vnormals) nvn += g.desc.ntri;
+        if (g.desc.uvs && g.desc.rgba8) {
+            nuv += g.desc.ntri;
+            nimg += (uint64_t)g.desc.width * g.desc.height * 4;
+        }
+    }
+    launch_hash_u32(st, h->verts.as<uint32_t>(), h->T * 9, 1, out);
+    launch_hash_u32(st, h->vn.as<uint32_t>(), nvn * 9, 2, out);
+    launch_hash_u32(st, h->uv.as<uint32_t>(), nuv * 6, 3, out);
+    launch_hash_u32(st, h->img.as<uint32_t>(), nimg / 4, 4, out);
+}
+
+}  // namespace
+
+struct vox_batch {
+    std::vector<vox_ctx*> jobs;
+    hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool valid = false, captured = false;
+    std::vector<std::vector<void*>> bufs;
+    std::vector<uint64_t> hostSig, devSig;
+    DevBuf sums;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    vox_batch_stats stats{};
+    std::stringstream err;
+    std::string errCache;
+    bool fail(const std::string& m) {
+        err << m << std::endl;
+        return false;
+    }
+    bool check(hipError_t e, const char* what) {
+        if (e == hipSuccess) return true;
+        err << what << " failed: " << hipGetErrorString(e) << std::endl;
+        return false;
+    }
+    bool checksums(std::vector<uint64_t>& out) {
+        launch_fill_u32(st, sums.as<uint32_t>(), 2 * jobs.size(), 0u);
+        for (size_t j = 0; j < jobs.size(); ++j)
+            launch_input_hash(jobs[j], st, sums.as<unsigned long long>() + j);
+        out.assign(jobs.size(), 0);
+        return check(hipMemcpyAsync(out.data(), sums.p, 8 * jobs.size(), hipMemcpyDeviceToHost, st),
+                     "checksum") &&
+               check(hipStreamSynchronize(st), "checksum");
+    }
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        captured = false;
+    }
+};
+
+extern "C" {
+
+vox_batch* vox_batch_create(vox_ctx* const* jobs, int32_t njobs) {
+    vox_batch* b = new (std::nothrow) vox_batch;
+    if (!b) return nullptr;
+    if (!jobs || njobs <= 0) {
+        b->fail("batch: no jobs");
+        return b;
+    }
+    for (int32_t j = 0; j < njobs; ++j) {
+        vox_ctx* h = jobs[j];
+        if (!h || !h->valid || !h->planned || !h->triMem) {
+            b->fail("batch: job " + std::to_string(j) +
+                    " has no eager vox_generate with uploaded triangles to size it");
+            return b;
+        }
+        for (int32_t i = 0; i < j; ++i)
+            if (jobs[i] == h) {
+                b->fail("batch: a handle appears twice");
+                return b;
+            }
+        b->jobs.push_back(h);
+    }
+    if (!b->check(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking), "hipStreamCreate") ||
+        !b->sums.ensure(8 * b->jobs.size(), b->err, "checksums"))
+        return b;
+    for (auto& e : b->ev)
+        if (!b->check(hipEventCreate(&e), "hipEventCreate")) return b;
+    b->stats.jobs = njobs;
+    b->valid = true;
+    return b;
+}
+
+void vox_batch_destroy(vox_batch* b) {
+    if (!b) return;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    b->drop_graph();
+    for (auto e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    b->sums.release();
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+}
+
+const char* vox_batch_get_error(vox_batch* b) {
+    if (!b) return "null batch";
+    b->errCache = b->err.str();
+    return b->errCache.c_str();
+}
+
+int vox_batch_capture(vox_batch* b) {
+    if (!b || !b->valid) return 0;
+    b->drop_graph();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (vox_ctx* h : b->jobs)
+        if (!h->check(hipStreamSynchronize(h->st), "synchronize")) return b->fail(h->err.str()), 0;
+    b->bufs.assign(b->jobs.size(), {});
+    for (size_t j = 0; j < b->jobs.size(); ++j) buffer_list(b->jobs[j], b->bufs[j]);
+    if (!b->check(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal), "begin capture"))
+        return 0;
+    bool ok = true;
+    for (size_t j = 0; j < b->jobs.size() && ok; ++j) {
+        vox_ctx* h = b->jobs[j];
+        hipStream_t own = h->st;
+        h->st = b->st;
+        h->replay = true;
+        h->prepared = false;
+        ok = vox_compute_edges_face_normals_and_bounds(h) == 1 &&
+             vox_generate(h, nullptr, 1, h->compressed ? 1 : 0) != 0;
+        h->replay = false;
+        h->st = own;
+        if (!ok) b->fail("batch: capture of job " + std::to_string(j) + " failed: " + h->err.str());
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->st, &g);
+    if (!ok || !b->check(e, "end capture")) {
+        if (g) (void)hipGraphDestroy(g);
+        return 0;
+    }
+    b->graph = g;
+    // the capture must not have moved any buffer (sizes are the planned ones)
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        std::vector<void*> now;
+        buffer_list(b->jobs[j], now);
+        if (now != b->bufs[j]) {
+            b->drop_graph();
+            return b->fail("batch: a buffer was reallocated during capture"), 0;
+        }
+    }
+    size_t nn = 0;
+    (void)hipGraphGetNodes(b->graph, nullptr, &nn);
+    b->stats.graph_nodes = nn;
+    if (!b->check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "instantiate")) {
+        b->drop_graph();
+        return 0;
+    }
+    b->hostSig.resize(b->jobs.size());
+    for (size_t j = 0; j < b->jobs.size(); ++j) b->hostSig[j] = host_signature(b->jobs[j]);
+    if (!b->checksums(b->devSig)) return 0;
+    b->captured = true;
+    b->stats.ms_capture =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 1;
+}
+
+int vox_batch_launch(vox_batch* b) {
+    if (!b || !b->valid) return 0;
+    if (!b->captured) return b->fail("batch: capture first"), 0;
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        vox_ctx* h = b->jobs[j];
+        std::vector<void*> now;
+        buffer_list(h, now);
+        if (now != b->bufs[j] || host_signature(h) != b->hostSig[j])
+            return b->fail("batch: job " + std::to_string(j) +
+                           " changed since the capture (buffers or parameters); capture again"), 0;
+        if (!h->check(hipStreamSynchronize(h->st), "synchronize")) return 0;
+    }
+    (void)hipEventRecord(b->ev[0], b->st);
+    std::vector<uint64_t> sig;
+    if (!b->checksums(sig)) return 0;
+    for (size_t j = 0; j < b->jobs.size(); ++j)
+        if (sig[j] != b->devSig[j])
+            return b->fail("batch: the triangles of job " + std::to_string(j) +
+                           " changed since the capture; capture again"), 0;
+    (void)hipEventRecord(b->ev[1], b->st);
+    if (!b->check(hipGraphLaunch(b->exec, b->st), "graph launch")) return 0;
+    (void)hipEventRecord(b->ev[2], b->st);
+    if (!b->check(hipStreamSynchronize(b->st), "graph")) return 0;
+    float a = 0, c = 0;
+    (void)hipEventElapsedTime(&a, b->ev[0], b->ev[1]);
+    (void)hipEventElapsedTime(&c, b->ev[1], b->ev[2]);
+    b->stats.ms_check_last = a;
+    b->stats.ms_graph_last = c;
+    ++b->stats.launches;
+    for (vox_ctx* h : b->jobs) h->haveResult = true;
+    return 1;
+}
+
+int vox_batch_get_stats(vox_batch* b, vox_batch_stats* out) {
+    if (!b || !b->valid || !out) return 0;
+    *out = b->stats;
+    return 1;
+}
+
+int vox_set_launch(vox_ctx* h, uint32_t persistent_blocks) {
+    if (!h || !h->valid) return 0;
+    if (persistent_blocks == 0 || persistent_blocks > (1u << 20))
+        return h->fail("persistent_blocks out of range"), 0;
+    h->persistentBlocks = persistent_blocks;
+    return 1;
+}
+
+}  // extern "C" (batch)
 
 }  // extern "C"

@@ -154,6 +154,9 @@ void launch_occupancy(hipStream_t st, const VoxParams& P, uint64_t nActive, cons
 void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
                         const float* nrm, uint32_t* outCol, float* outNrm);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
+// *out += order-independent checksum of n words (salted).
+void launch_hash_u32(hipStream_t st, const uint32_t* p, uint64_t n, uint64_t salt,
+                     unsigned long long* out);
 // Block compression of bricks: BC3 colours (16 B per block) and BC1 normals
 // (8 B per block), ceil(e/4)^2 * e blocks per brick of e^3 texels.
 uint64_t bc_blocks_per_brick(const uint32_t e[3]);

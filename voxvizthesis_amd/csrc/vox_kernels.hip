@@ -1287,6 +1287,24 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict
     }
 }
 
+// Order-independent 64-bit checksum of n words: the sum of a mixed
+// (word, index) per word.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__global__ void k_hash_u32(const uint32_t* __restrict__ p, uint64_t n, uint64_t salt,
+                           unsigned long long* __restrict__ out) {
+    unsigned long long acc = 0ull;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        acc += mix64(((unsigned long long)p[i] << 32 | (unsigned long long)(i & 0xffffffffu)) ^
+                     (salt * 0x9E3779B97F4A7C15ull + (i >> 32)));
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+    if ((threadIdx.x & 63u) == 0u) atomicAdd(out, acc);
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -1298,6 +1316,13 @@ inline unsigned grid_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 1)
 }  // namespace
 
 // Fill kernel instead of hipMemsetAsync so the pipeline stays capturable.
+void launch_hash_u32(hipStream_t st, const uint32_t* p, uint64_t n, uint64_t salt,
+                     unsigned long long* out) {
+    if (n == 0) return;
+    const uint64_t g = (n + kBlock - 1) / kBlock;
+    k_hash_u32<<<(unsigned)(g < 4096 ? g : 4096), kBlock, 0, st>>>(p, n, salt, out);
+}
+
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
     if (n == 0) return;
     uint64_t blocks = (n + kBlock - 1) / kBlock;

@@ -360,3 +360,28 @@ def flythrough(seed: int = 5, frames: int = 300, n_points: int = 12, extent: flo
         up /= np.linalg.norm(up)
         out.append((pos.astype(np.float32), look.astype(np.float32), up.astype(np.float32)))
     return out
+
+
+def batch_scenes(seed: int = 4, n_jobs: int = 256, tmin: float = 2e3, tmax: float = 2e6,
+                 grids=(64, 128, 256, 512, 1024)):
+    """Independent meshes for batched voxelization: triangle counts
+    log-uniform in [tmin, tmax], half fBm heightfield tiles (2*cells^2
+    triangles, cells = sqrt(T/2) rounded to a power of two between 16 and
+    1024) and half uniform random soups with edges of 0.5-4 voxels; each on
+    a grid drawn from `grids`, at most 8x the mesh's resolution."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for j in range(n_jobs):
+        T = float(np.exp(rng.uniform(np.log(tmin), np.log(tmax))))
+        N = int(rng.choice(grids))
+        if j % 2 == 0:
+            cells = int(2 ** np.clip(np.round(np.log2(np.sqrt(T / 2.0))), 4, 10))
+            mesh = heightfield_terrain(cells, seed=seed * 1000 + j)
+            N = min(N, max(64, 8 * cells))
+            out.append(Scene(meshes=[mesh], N=(N, N, N), B=(8, 8, 8),
+                             name=f"job{j}_terrain{cells + 1}_N{N}"))
+        else:
+            mesh = random_soup(int(T), seed * 1000 + j, N)
+            out.append(Scene(meshes=[mesh], N=(N, N, N), B=(8, 8, 8),
+                             name=f"job{j}_soup{int(T)}_N{N}"))
+    return out
