@@ -1,5 +1,5 @@
 """ctypes binding of libvoxmi.so (include/voxmi.h, voxmi_pool.h, voxmi_serve.h,
-voxmi_batch.h).
+voxmi_batch.h, voxmi_tiler.h).
 
 The HIP library is the product path: importing this module on a machine
 without the built library raises immediately, there is no CPU fallback.
@@ -45,6 +45,11 @@ SERVE_EXPORTS = [
     "vox_server_create", "vox_server_destroy", "vox_server_get_error", "vox_server_frame",
     "vox_server_get_stats", "vox_server_latencies", "vox_server_state",
 ]
+# Every symbol include/voxmi_tiler.h declares.
+TILER_EXPORTS = [
+    "vox_tiler_create", "vox_tiler_destroy", "vox_tiler_get_error", "vox_tiler_assign",
+    "vox_tiler_get",
+]
 VOX_TREE_REFERENCE_CONST = 1
 VOX_TREE_EXACT_CONST = 2
 
@@ -80,6 +85,16 @@ class VoxCompressed(C.Structure):
         ("colors", C.c_void_p * VOX_MAX_LEVELS),
         ("normals", C.c_void_p * VOX_MAX_LEVELS),
     ]
+
+
+class VoxTileGrid(C.Structure):
+    _fields_ = [("origin", C.c_float * 2), ("size", C.c_float * 2), ("border", C.c_float * 2),
+                ("dim", C.c_uint32 * 2)]
+
+
+class VoxTileLists(C.Structure):
+    _fields_ = [("ntiles", C.c_uint64), ("pairs", C.c_uint64), ("tile_start", C.c_void_p),
+                ("tile_tris", C.c_void_p), ("tile_zmin", C.c_void_p), ("tile_zmax", C.c_void_p)]
 
 
 class VoxBatchStats(C.Structure):
@@ -187,6 +202,15 @@ def load():
     L.vox_batch_capture.argtypes = [vp]
     L.vox_batch_launch.argtypes = [vp]
     L.vox_batch_get_stats.argtypes = [vp, C.POINTER(VoxBatchStats)]
+    # tiler
+    L.vox_tiler_create.restype = vp
+    L.vox_tiler_create.argtypes = []
+    L.vox_tiler_destroy.argtypes = [vp]
+    L.vox_tiler_destroy.restype = None
+    L.vox_tiler_get_error.restype = C.c_char_p
+    L.vox_tiler_get_error.argtypes = [vp]
+    L.vox_tiler_assign.argtypes = [vp, vp, C.c_uint64, C.c_int32, C.POINTER(VoxTileGrid)]
+    L.vox_tiler_get.argtypes = [vp, C.POINTER(VoxTileLists)]
     # node pool / brick pool
     L.vox_brick_pool_create.restype = vp
     L.vox_brick_pool_create.argtypes = [u3, u3]
