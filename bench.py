@@ -77,10 +77,13 @@ def kernel_bytes(name, scene, st, bricks, B, launches):
     return 0.0
 
 
-def serving_bench(vz, scene, frames: int, W: int, H: int, slots, max_up: int):
-    """Paged flythrough over the voxelized octree: per frame a W x H sparse
-    octree ray cast emits brick requests and the brick server admits them into
-    the pool (LRU).  Returns throughput and request latency figures."""
+def serving_bench(vz, scene, frames: int, W: int, H: int, slots, max_up: int, streams: int = 1):
+    """Paged flythrough over the voxelized octree: per frame each camera
+    stream's W x H sparse-octree ray cast emits brick requests and one brick
+    server admits them all into the shared pool (LRU).  Frames are queued
+    without host waits: the ray casts of a frame overlap on the device and
+    the server frame waits for them.  Returns throughput and request latency
+    figures."""
     import numpy as np
     from voxvizthesis_amd import scenes
     from voxvizthesis_amd.gvpool import BrickPool, OctTree
@@ -88,28 +91,33 @@ def serving_bench(vz, scene, frames: int, W: int, H: int, slots, max_up: int):
 
     bp = BrickPool(scene.B, slots)
     t = OctTree(bp, vz)
-    rc = RayCaster(t, W, H)
+    rcs = [RayCaster(t, W, H) for _ in range(streams)]
     srv = BrickServer(t, max_up)
     lo = np.asarray(scene.p, np.float32)
     hi = lo + np.asarray(scene.N, np.float32) * np.asarray(scene.delta, np.float32)
     prm = raycast_params(lo, hi, jitter_seed=5)
-    path = scenes.flythrough(seed=5, frames=frames)
-    ray_ms, dedup_ms, upd_ms, req = [], [], [], 0
+    paths = [scenes.flythrough(seed=5 + k, frames=frames) for k in range(streams)]
+    ray_ms, dedup_ms, req = [], [], 0
     t0 = time.perf_counter()
-    for f, (pos, look, up) in enumerate(path):
-        v = rc.frame(camera(pos, look, up, 60.0, W, H, f), prm)
-        ray_ms.append(v.ms_raycast)
-        dedup_ms.append(v.ms_requests)
-        req += v.nrequests
-        st = srv.frame(rc)
-        upd_ms.append(st.ms_update_last)
-    wall = time.perf_counter() - t0
+    for f in range(frames):
+        for rc, path in zip(rcs, paths):
+            pos, look, up = path[f]
+            rc.frame(camera(pos, look, up, 60.0, W, H, f), prm, wait=False)
+        srv.frame(rcs, wait=False)
+        for rc in rcs:  # waits for the ray casts only
+            v = rc.view()
+            ray_ms.append(v.ms_raycast)
+            dedup_ms.append(v.ms_requests)
+            req += v.nrequests
     st = srv.stats()
+    wall = time.perf_counter() - t0
     lat = srv.latencies_us() / 1e3
     out = {
         "workload": f"{frames}-frame seeded Catmull-Rom flythrough (12 points, altitude "
-                    f"0.05-0.2 extent) over the full octree, {W}x{H} ray cast",
+                    f"0.05-0.2 extent) over the full octree, {streams} camera stream(s) of "
+                    f"{W}x{H} ray cast",
         "frames": frames,
+        "streams": streams,
         "pool_slots": int(np.prod(slots)),
         "max_uploads_per_frame": max_up,
         "requests": int(req),
@@ -120,14 +128,69 @@ def serving_bench(vz, scene, frames: int, W: int, H: int, slots, max_up: int):
         "frame_ms": round(wall / frames * 1e3, 3),
         "raycast_ms_mean": round(float(np.mean(ray_ms)), 4),
         "dedup_ms_mean": round(float(np.mean(dedup_ms)), 4),
-        "update_ms_mean": round(float(np.mean(upd_ms)), 4),
+        "update_ms_mean": round(st.ms_update_total / max(st.frames, 1), 4),
         "latency_p50_ms": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
         "latency_p99_ms": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
         "latency_max_ms": round(float(lat.max()), 4) if len(lat) else None,
     }
-    for o in (srv, rc, t, bp):
+    for o in [srv] + rcs + [t, bp]:
         o.close()
     return out
+
+
+def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, slots,
+                max_trees: int, max_loading: int):
+    """Served paging: a (cells+1)^2 fBm terrain in a cells^3 virtual grid,
+    tiled into tile^3 octrees voxelized on demand on the device as the
+    seeded flythrough first sees them (the pager), served from one shared
+    brick pool with LRU eviction."""
+    import numpy as np
+    from voxvizthesis_amd import scenes
+    from voxvizthesis_amd.pager import TilePager
+
+    t0 = time.perf_counter()
+    terrain = scenes.heightfield_terrain(cells, 5)
+    t_gen = time.perf_counter() - t0
+    log(f"[bench] paged terrain {cells + 1}^2 generated in {t_gen:.1f} s")
+    t0 = time.perf_counter()
+    pg = TilePager([terrain], 1.0 / cells, tile, B, slots, max_trees=max_trees,
+                   max_loading=max_loading, streams=1, max_width=W, max_height=H,
+                   near=1e-3, far=10.0)
+    t_setup = time.perf_counter() - t0
+    path = scenes.flythrough(seed=5, frames=frames, terrain_seed=5)
+    wall = []
+    try:
+        for f, (pos, look, up) in enumerate(path):
+            cam = dict(position=pos, look=look, up=up, fov_y_deg=60.0, width=W, height=H)
+            a = time.perf_counter()
+            pg.frame([cam])
+            wall.append(time.perf_counter() - a)
+        st = pg.server.stats()
+        lat = pg.server.latencies_us() / 1e3
+        loads = np.asarray(pg.load_ms) if pg.load_ms else np.zeros(1)
+        total = sum(wall)
+        return {
+            "workload": f"{cells + 1}^2 fBm terrain (seed 5) in a {cells}^3 virtual grid, "
+                        f"{tile}^3 tiles of {B}^3 bricks voxelized on demand, {frames}-frame "
+                        f"seeded flythrough, {W}x{H} ray cast",
+            "tiles": len(pg.trees),
+            "tiles_voxelized": int(pg.tiles_loaded),
+            "pool_slots": int(np.prod(slots)),
+            "pool_gb": round(float(np.prod(slots)) * (B + 2) ** 3 * 8 / 1e9, 2),
+            "scene_setup_s": round(t_setup, 2),
+            "frames": frames,
+            "frame_ms": round(total / frames * 1e3, 3),
+            "frame_ms_p99": round(float(np.percentile(np.asarray(wall) * 1e3, 99)), 3),
+            "tile_load_ms_p50": round(float(np.percentile(loads, 50)), 2),
+            "tile_load_ms_max": round(float(loads.max()), 2),
+            "uploads": int(st.uploads),
+            "evictions": int(st.evictions),
+            "bricks_per_s": round(st.uploads / total, 1),
+            "latency_p50_ms": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
+            "latency_p99_ms": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
+        }
+    finally:
+        pg.close()
 
 
 def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 5):
@@ -228,7 +291,12 @@ def main():
     ap.add_argument("--serve-frames", type=int, default=300)
     ap.add_argument("--serve-width", type=int, default=1920)
     ap.add_argument("--serve-height", type=int, default=1080)
+    ap.add_argument("--serve2-width", type=int, default=2560)
+    ap.add_argument("--serve2-height", type=int, default=1440)
+    ap.add_argument("--serve-max-uploads", type=int, default=65536)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--paged-cells", type=int, default=4096)
+    ap.add_argument("--no-paged", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=32)
     ap.add_argument("--batch-tmax", type=float, default=2.6e5)
     ap.add_argument("--batch-grids", default="64,128,256")
@@ -358,9 +426,21 @@ def main():
         log(f"[bench] serving {args.serve_frames} frames at {args.serve_width}x"
             f"{args.serve_height} ...")
         sv = serving_bench(vz, scene, args.serve_frames, args.serve_width, args.serve_height,
-                           (56, 56, 56), 4096)
+                           (56, 56, 56), args.serve_max_uploads)
         out["serving"] = sv
         log(f"[bench] serving: {sv['frame_ms']} ms/frame, p99 {sv['latency_p99_ms']} ms")
+        log(f"[bench] serving two {args.serve2_width}x{args.serve2_height} streams ...")
+        sv2 = serving_bench(vz, scene, args.serve_frames, args.serve2_width, args.serve2_height,
+                            (56, 56, 56), args.serve_max_uploads, streams=2)
+        out["serving_two_streams"] = sv2
+        log(f"[bench] two streams: {sv2['frame_ms']} ms/frame, p99 {sv2['latency_p99_ms']} ms")
+    if not args.no_paged:
+        log(f"[bench] paged serving over a {args.paged_cells}^3 virtual grid ...")
+        pb = paged_bench(args.paged_cells, args.serve_frames, args.serve_width,
+                         args.serve_height, 1024, 16, (56, 56, 56), 48, 4)
+        out["paged"] = pb
+        log(f"[bench] paged: {pb['frame_ms']} ms/frame, {pb['tiles_voxelized']} tiles, "
+            f"p99 {pb['latency_p99_ms']} ms")
     if not args.no_batch:
         log(f"[bench] batch of {args.batch_jobs} meshes ...")
         bb = batch_bench(args.batch_jobs, args.batch_tmax,

@@ -133,6 +133,17 @@ int vox_add_group(vox_ctx* h, const vox_group* g);
 /* Same, but every pointer in *g is a device pointer already resident in HBM
  * (no copy is made; the caller keeps them alive until vox_generate returns). */
 int vox_add_group_device(vox_ctx* h, const vox_group* g);
+/* A group of n triangles picked from device-resident arrays: triangle k is
+ * row idx[k] - base of g's verts / vnormals / uvs (device pointers of the
+ * whole group, g->ntri rows; idx is a device array whose values lie in
+ * [base, base + g->ntri)).  The rows are gathered on the device by
+ * vox_allocate_triangle_memory, which also sets the group's bounds to those
+ * of the picked triangles, as the reference's tiler does for the triangles
+ * of a tile (main.cu 3848-3880); g->bbox is ignored.  For tiles voxelized
+ * from a scene resident in HBM (with the tiler's per-tile lists).  Not in
+ * the reference. */
+int vox_add_group_gather(vox_ctx* h, const vox_group* g, const uint32_t* idx, uint64_t n,
+                         uint32_t base);
 
 /* cuda::Voxelizer::initCuda (Voxelizer.cu 297-322): device 0, own stream. */
 int vox_init(vox_ctx* h);
@@ -189,6 +200,27 @@ typedef struct vox_compressed {
     const uint8_t* normals[VOX_MAX_LEVELS];
 } vox_compressed;
 int vox_get_compressed(vox_ctx* h, vox_compressed* out);
+
+/* Asynchronous generate, not in the reference (whose pager thread loads
+ * finished tiles from disk while frames go on, GigaVoxelsDatabasePager.cpp
+ * 232-338): vox_generate(h, NULL, 1, compressed) -- with the setup pass if
+ * it has not run -- on a host worker thread, so the caller can go on issuing
+ * work for other handles (ray casts, other tiles) while this one's device
+ * work and host round trips proceed on its own stream.  Until vox_wait has
+ * returned its result every other call on h fails (vox_get_error says so),
+ * except vox_destroy, which waits for the worker first.  Returns 1 when the
+ * worker started. */
+int vox_generate_async(vox_ctx* h, int32_t compressed);
+/* Result of the running or finished vox_generate_async: 1 ok, 0 error,
+ * -1 no group overlaps the grid, 2 still running (only when block = 0). */
+int vox_wait(vox_ctx* h, int32_t block);
+/* Free what only the next vox_generate needs (triangles, bins, level tiles
+ * and their work lists), keeping the result (vox_get_result and
+ * vox_get_compressed stay valid; vox_copy_level does not).  For callers
+ * that keep many results resident, such as a pager of tiles.  The next
+ * vox_generate needs its groups added and uploaded again.  Not in the
+ * reference. */
+int vox_release_work_memory(vox_ctx* h);
 
 /* Write tree.gvx and voxels_<depth>.gvb of the last vox_generate into
  * out_dir (host-side writer, Voxelizer.cu 2816-3061, VoxelBrickWriter.cu

@@ -49,6 +49,11 @@ typedef struct vox_octree vox_octree;
  *   VOX_TREE_EXACT_CONST      the voxelizer's colour itself. */
 #define VOX_TREE_REFERENCE_CONST 1u
 #define VOX_TREE_EXACT_CONST 2u
+/* The root's brick is not uploaded at creation; a brick server the tree is
+ * attached to admits it through its LRU like any other brick, as the
+ * reference's pager does for the trees it loads (GigaVoxelsDatabasePager.cpp
+ * 300-310). */
+#define VOX_TREE_NO_ROOT_BRICK 4u
 
 typedef struct vox_brick_pool_view {
     uint32_t  brick_dim[3];     /* B (stored bricks are B+2 per axis)         */
@@ -91,7 +96,8 @@ void vox_octree_destroy(vox_octree* t);
 const char* vox_octree_get_error(vox_octree* t);
 int vox_octree_get_view(vox_octree* t, vox_octree_view* out);
 
-/* Copy n bytes from a device pointer of a view to host memory. */
+/* Copy n bytes from a device pointer of a view to host memory, after all
+ * device work queued so far (every stream) has finished. */
 int vox_pool_copy_to_host(void* dst, const void* src_device, uint64_t nbytes);
 
 #ifdef __cplusplus

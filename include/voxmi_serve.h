@@ -21,6 +21,18 @@
  * GigaVoxelsBrickPool.cpp 561-682), their bricks copied from the voxelizer's
  * result in HBM and their node texels updated.  Requests that find no
  * evictable slot stay queued and are admitted in a later frame.
+ *
+ * Ordering.  Every ray caster and server runs on its own HIP stream and
+ * returns without waiting for the device.  A ray-cast frame starts after the
+ * last write of its octree's node texture and brick pool (the octree build or
+ * a server frame); a server frame starts after the request lists it is given.
+ * Ray casters of one frame therefore overlap on the device with each other
+ * and with unrelated work, and a host loop of "ray cast every stream, then
+ * one server frame" runs without host round trips.  All ray casts reading a
+ * pool must be given to the server frame that follows them (or have
+ * finished): the server overwrites slots.  Reading results
+ * (vox_raycaster_get_frame, vox_server_get_stats, vox_server_state,
+ * vox_pool_copy_to_host) waits for the work they read.
  */
 #ifndef VOXMI_SERVE_H
 #define VOXMI_SERVE_H
@@ -56,6 +68,12 @@ typedef struct vox_raycast_params {
     uint32_t jitter_seed;       /* 32x32 jitter table (JitterTexSize = 32)   */
     uint32_t max_traversals;    /* bound on root restarts per ray; 0: 4096.
                                    Not in the reference; every ray ends.     */
+    int32_t  rect[4];           /* pixels x0, y0, x1, y1 (exclusive) cast this
+                                   frame; all zero: the whole viewport.  The
+                                   reference shades only the pixels the
+                                   tree's box covers; a rectangle holding the
+                                   box's projection gives the same result
+                                   there.  Pixels outside are not written.   */
 } vox_raycast_params;
 
 typedef struct vox_frame_view {
@@ -77,8 +95,10 @@ vox_raycaster* vox_raycaster_create(vox_octree* tree, int32_t max_width, int32_t
                                     uint32_t max_requests);
 void vox_raycaster_destroy(vox_raycaster* rc);
 const char* vox_raycaster_get_error(vox_raycaster* rc);
-/* One frame: ray cast, then request deduplication.  Synchronous. */
+/* One frame: ray cast, then request deduplication, queued on the ray
+ * caster's stream. */
 int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycast_params* prm);
+/* The last frame's buffers, counts and timings (waits for the frame). */
 int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out);
 
 /* ---- brick server ---------------------------------------------------------
@@ -112,19 +132,48 @@ typedef struct vox_server_stats {
     double   clock_khz;         /* device clock the latencies are counted in */
 } vox_server_stats;
 
-/* Server over one octree and its brick pool.  max_uploads_per_frame bounds
- * the admissions per frame (0: the pool's slot count). */
+/* Server over one octree and its brick pool, adopting the root brick the
+ * octree build uploaded.  max_uploads_per_frame bounds the admissions per
+ * frame (0: the pool's slot count). */
 vox_server* vox_server_create(vox_octree* tree, uint32_t max_uploads_per_frame);
+/* Server over a brick pool shared by up to max_trees octrees of at most
+ * max_texels_per_tree node texels each (the reference's one BrickPool with
+ * one LRU list for every tree the pager has loaded, GigaVoxelsBrickPool.cpp
+ * 406-449).  Nodes are named by global texel: tree slot * max_texels_per_tree
+ * + texel (vox_server_state, FIFO order).  Not in the reference as an
+ * object of its own. */
+vox_server* vox_server_create_shared(vox_brick_pool* bp, uint32_t max_trees,
+                                     uint64_t max_texels_per_tree, uint32_t max_uploads_per_frame);
+/* Attach an octree built on the server's pool; *id receives its tree slot.
+ * A NON-CONST root without a resident brick becomes pending at once, as the
+ * pager queues the root of every tree it loads (GigaVoxelsDatabasePager.cpp
+ * 300-310); build such octrees with VOX_TREE_NO_ROOT_BRICK.  Bricks already
+ * resident in the octree's texture are adopted (an error if their slot is
+ * taken).  Waits for the device. */
+int vox_server_attach(vox_server* s, vox_octree* tree, uint32_t* id);
+/* Detach an octree (the pager unloading a tree): its slots become empty and
+ * are taken first, as BrickPool::notifyDeleted moves them to the front of
+ * the LRU list (GigaVoxelsBrickPool.cpp 426-449), and its nodes lose their
+ * resident bits; its queued requests are voided (an admission that meets
+ * one leaves the slot as it was).  Waits for the device; the octree may then
+ * be destroyed (its ray casters first) or attached again. */
+int vox_server_detach(vox_server* s, vox_octree* tree);
 void vox_server_destroy(vox_server* s);
 const char* vox_server_get_error(vox_server* s);
-/* Serve the request list of the ray caster's last frame.  Synchronous. */
+/* Serve the request list of the ray caster's last frame (queued). */
 int vox_server_frame(vox_server* s, vox_raycaster* rc);
+/* Serve the request lists of n ray casters' last frames -- camera streams,
+ * or the trees of one view -- in one admission step: the lists are touched
+ * in the order given, then new pending nodes join the FIFO in ascending
+ * global texel order.  Each ray caster's octree must be attached. */
+int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n);
 int vox_server_get_stats(vox_server* s, vox_server_stats* out);
-/* Latencies (device clock ticks, see clock_khz) of every admitted request
- * since creation, in admission order; copies min(n, cap) values, *n = total. */
+/* Latencies (device clock ticks, see clock_khz) of the admitted requests
+ * since creation, in admission order; copies min(n, cap) values, *n = total
+ * (at most 2^22 are kept). */
 int vox_server_latencies(vox_server* s, uint64_t* out, uint64_t cap, uint64_t* n);
-/* Slot of each node texel (0xffffffff: none) and node texel of each slot,
- * device pointers. */
+/* Slot of each global node texel (0xffffffff: none) and global node texel
+ * of each slot, device pointers (waits for the server's queued work). */
 int vox_server_state(vox_server* s, const uint32_t** node_slot, const uint32_t** slot_node,
                      uint64_t* ntexels, uint32_t* nslots);
 

@@ -44,11 +44,19 @@ void vox_tiler_destroy(vox_tiler* t);
 const char* vox_tiler_get_error(vox_tiler* t);
 /* verts: 9 floats per triangle, a device pointer (device = 1) or host
  * memory copied in first (device = 0).  Triangle indices within a tile come
- * out in no particular order; the reference keeps input order, so callers
- * sort each tile's list before building voxelizer groups from it. */
+ * out in ascending (input) order, the order the reference appends them in
+ * (the voxelizer's per-voxel triangle slots depend on it). */
 int vox_tiler_assign(vox_tiler* t, const float* verts, uint64_t ntri, int32_t device,
                      const vox_tile_grid* grid);
 int vox_tiler_get(vox_tiler* t, vox_tile_lists* out);
+/* Where each input group starts in every tile's list: first holds nbounds
+ * ascending triangle indices (the first triangle of each group, then the
+ * total); out (host, ntiles * nbounds) receives, per tile, the position in
+ * tile_tris of the tile's first triangle with index >= first[k].  Group g of
+ * tile t is then tile_tris[out[t*nb + g] .. out[t*nb + g + 1]) (with
+ * vox_add_group_gather, base = first[g]).  Not in the reference, whose tiler
+ * keeps one list per texture (main.cu 3848-3880). */
+int vox_tiler_group_bounds(vox_tiler* t, const uint32_t* first, uint32_t nbounds, uint32_t* out);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,13 @@ public:
         _err = vox_get_error(_h);
         return _err;
     }
+    // Not in the reference: generate (no files) on a worker thread; wait()
+    // returns its 1 / 0 / -1, or 2 while it runs when block is false.
+    bool generateAsync(bool outputCompressed = false) {
+        return vox_generate_async(_h, outputCompressed ? 1 : 0) != 0;
+    }
+    int wait(bool block = true) { return vox_wait(_h, block ? 1 : 0); }
+    bool releaseWorkMemory() { return vox_release_work_memory(_h) != 0; }
     // Border voxels added on each side of the grid, per axis (Voxelizer.h 179);
     // the tiler sizes its triangle overlap from it (main.cu 4171-4175).
     const uvec3& getExtraVoxChunk() const { return _extra; }

@@ -257,16 +257,17 @@ def pool_load(directory: str, slots=(8, 8, 8), store_const: bool = True) -> RefP
         e = [bd[a] // slots[a] for a in range(3)]     # B + 2 per axis
         bvox = e[0] * e[1] * e[2]
         fb = []
+        compressed = bool(L.ref_pool_compressed(h))
         for d in range(L.ref_pool_depths(h)):
             n = int(L.ref_pool_file_bricks(h, d))
-            if n == 0:
+            if n == 0 or compressed:  # compressed files keep only the block bytes
                 fb.append((np.zeros((0, bvox), np.uint32), np.zeros((0, bvox), np.uint32)))
                 continue
             fc = _copy(L.ref_pool_file_brick_colors(h, d), n * bvox, np.uint32).reshape(n, bvox)
             fn = _copy(L.ref_pool_file_brick_normals(h, d), n * bvox, np.uint32).reshape(n, bvox)
             fb.append((fc, fn))
         out = RefPool(tuple(nd), int(blocks.value), nodes, tuple(bd), bcol, bnrm, fb)
-        out.compressed = bool(L.ref_pool_compressed(h))
+        out.compressed = compressed
         out.file_compressed = []
         for d in range(L.ref_pool_depths(h)):
             pair = []

@@ -33,6 +33,9 @@ int main() {
         std::printf("chunk error '%s'\n", vz.getErrorMessage().c_str());
         bad |= vz.getErrorMessage().find("chunk") == std::string::npos;
         bad |= vz.initCuda();      // must refuse after a construction error
+        bad |= vz.generateAsync();  // and start no worker
+        std::printf("wait without a worker %d\n", vz.wait(false));
+        bad |= vz.wait(false) != 0;
     }
     std::printf(bad ? "FAIL\n" : "OK\n");
     return bad;

@@ -1,0 +1,126 @@
+"""Tile pager on the GPU: tiles voxelized on demand from the scene resident
+in HBM, several at once on worker threads while frames are ray cast, give
+every tile the octree it gets alone (voxelized from host arrays with the
+tile's triangles, in input order), bit-exactly; each tree's ray cast over
+its screen rectangle equals the ray-cast oracle's full frame there; every
+slot of the shared pool holds the brick of the node it serves."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle.server import NONE
+from voxvizthesis_amd import scenes
+from voxvizthesis_amd.gvserve import box_rect
+from voxvizthesis_amd.pager import TilePager
+from voxvizthesis_amd.tiler import PagedOctTreeGenerator, TileAssignment
+from voxvizthesis_amd.voxelizer import Voxelizer
+
+pytestmark = pytest.mark.gpu
+
+
+def look_at(pos, target, up=(0.0, 0.0, 1.0)):
+    pos, target, up = (np.asarray(v, np.float64) for v in (pos, target, up))
+    look = target - pos
+    look /= np.linalg.norm(look)
+    up = up - up.dot(look) * look
+    up /= np.linalg.norm(up)
+    return look.astype(np.float32), up.astype(np.float32)
+
+
+def snapshot(vz):
+    r = vz.result()
+    out = [r["occupancy"]]
+    for t in r["tree"]:
+        out += [t["types"], t["const"], t["bidx"], t["bcol"], t["bnrm"]]
+    return out
+
+
+def alone(gen, assign, tr):
+    vz = Voxelizer((gen.N,) * 3, (gen.B,) * 3)
+    try:
+        assert vz.initCuda() and vz.allocateVoxelMemory()
+        st = gen._voxelize(vz, assign.tile(tr.tile), tr.origin, None)
+        return st, (snapshot(vz) if st == 1 else None)
+    finally:
+        vz.close()
+
+
+def test_pager_tiles_match_alone_and_pool_is_consistent():
+    sc = scenes.scene_full_octree(cells=64, N=64, B=8, n_rocks=60)
+    W, H = 64, 48
+    pg = TilePager(sc.meshes, 1.0 / 128, 64, 8, (8, 8, 8), max_trees=8, max_loading=3,
+                   streams=1, max_width=W, max_height=H, near=1e-3, far=100.0)
+    try:
+        print(f"[pager] {len(pg.trees)} paged trees, grid {pg.gen.dim}", flush=True)
+        assert len(pg.trees) >= 4
+        path = [((0.5 + 0.05 * f, -0.6, 1.1 - 0.02 * f), (0.5, 0.5, 0.45)) for f in range(10)]
+        checked = 0
+        for f, (pos, target) in enumerate(path):
+            look, up = look_at(pos, target)
+            cam = dict(position=pos, look=look, up=up, fov_y_deg=60.0, width=W, height=H)
+            drawn = pg.frame([cam], serve=False)
+            if f in (6, 9):
+                pool = pg.pool.textures()[0]
+                for tr, rc in drawn[0]:
+                    rect = box_rect(cam, tr.box_min, tr.box_max)
+                    nodes = tr.octree.nodes()
+                    prm = dict(vol_min=tr.box_min, vol_max=tr.box_max)
+                    ru, rcol = oracle.raycast(dict(cam, frame=f), prm, tr.octree.view().levels,
+                                              (8, 8, 8), nodes, pool)
+                    u, c = rc.usage(), rc.rgba()
+                    if rect is None:
+                        x0, y0, x1, y1 = 0, 0, W, H
+                    else:
+                        x0, y0, x1, y1 = rect
+                    assert (u[y0:y1, x0:x1] == ru[y0:y1, x0:x1]).all()
+                    assert (c[y0:y1, x0:x1] == rcol[y0:y1, x0:x1]).all()
+                    outside = ru.copy()
+                    outside[y0:y1, x0:x1] = 0
+                    assert not outside.any()
+                    checked += 1
+            pg.serve()
+        pg.drain()
+        assert checked > 0
+        loaded = [t for t in pg.trees if t.state == "loaded"]
+        print(f"[pager] loaded {len(loaded)} trees, load ms {np.round(pg.load_ms, 1)}",
+              flush=True)
+        assert len(loaded) >= 2
+        # every loaded tile equals the same tile voxelized alone
+        gen = PagedOctTreeGenerator(sc.meshes, 1.0 / 128, 64, 8)
+        g = gen.grid()
+        assign = TileAssignment(gen.verts, g)
+        for tr in loaded:
+            st, ref = alone(gen, assign, tr)
+            assert st == 1
+            got = snapshot(tr.vz)
+            assert len(got) == len(ref) and all((a == b).all() for a, b in zip(got, ref)), \
+                f"tile {tr.x},{tr.y},{tr.z}"
+        # every slot holds the brick of the node it serves
+        node_slot, slot_node = pg.server.state()
+        col = pg.pool.textures()[0]
+        sl = pg.pool.view().slots
+        e = 10
+        by_slot = {}
+        for k, t in enumerate(pg.trees):
+            if t.state == "loaded":
+                by_slot[pg.server_slot(t)] = t
+        n_res = 0
+        for s, gidx in enumerate(slot_node):
+            if gidx == NONE:
+                continue
+            k, x = divmod(int(gidx), pg.cap)
+            tr = by_slot[k]
+            res = tr.vz.result()
+            info = {}
+            for d in range(res["levels"]):
+                tx = tr.octree.node_texel(d)
+                for nn in np.nonzero(tx != NONE)[0]:
+                    info[int(tx[nn])] = (d, int(nn))
+            d, nn = info[x]
+            b = int(res["tree"][d]["bidx"][nn])
+            ox, oy, oz = (s % sl[0]) * e, ((s // sl[0]) % sl[1]) * e, (s // (sl[0] * sl[1])) * e
+            assert (col[oz:oz + e, oy:oy + e, ox:ox + e].ravel() == res["tree"][d]["bcol"][b]).all()
+            n_res += 1
+        assert n_res > 0
+    finally:
+        pg.close()

@@ -73,3 +73,76 @@ def test_requests_outside_tree_and_constants_are_ignored():
     assert s.step([nid(0), nid(c[0]), nid(c[1])]) == 1
     assert s.admitted[-1] == [(c[0], 1)]
     assert list(s.fifo) == [c[1]]
+
+
+def _run_both(requests):
+    """The single-tree oracle and the shared-pool oracle with that one tree
+    attached: the same admissions, textures and slot tables frame by frame."""
+    from oracle.server import RefSharedServer
+    tex, types, bidx, texel = tree()
+    a = RefServer(tex, types, bidx, texel, slots=(2, 1, 1), B=(2, 2, 2), max_up=2)
+    b = RefSharedServer(slots=(2, 1, 1), B=(2, 2, 2), max_trees=3, cap=16, max_up=2)
+    assert b.attach(tex, types, bidx, texel) == 0
+    for ids in requests:
+        assert a.step(ids) == b.step([(0, ids)])
+        assert a.admitted[-1] == b.admitted[-1]
+        assert (a.texture() == b.texture(0)).all()
+        assert a.node_slot == b.node_slot
+        assert list(a.fifo) == list(b.fifo)
+
+
+def test_shared_oracle_with_one_tree_is_the_single_tree_oracle():
+    tex, types, bidx, texel = tree()
+    c = [int(t) for t in texel[1]]
+    _run_both([[nid(c[1]), nid(c[0]), nid(c[3])], [nid(c[0]), nid(c[2])], [],
+               [nid(1), nid(c[5]), nid(c[6])], [nid(0), nid(c[7]), nid(c[4])], []])
+
+
+def _no_root_tree():
+    tex, types, bidx, texel = tree()
+    tex = tex.copy()
+    tex[0, 0, 0] = (pk(2, 0, 0), 0)      # root brick not uploaded
+    return tex, types, bidx, texel
+
+
+def test_shared_attach_seeds_root_and_detach_frees_slots():
+    from oracle.server import RefSharedServer
+    s = RefSharedServer(slots=(2, 1, 1), B=(2, 2, 2), max_trees=2, cap=16, max_up=2)
+    A = tree()
+    Bt = _no_root_tree()
+    c = [int(t) for t in A[3][1]]
+    assert s.attach(*A) == 0 and s.node_slot == {0: 0}          # root adopted
+    assert s.attach(*Bt) == 1 and s.pending == {16}             # root pending
+    assert s.step([]) == 1 and s.admitted[-1] == [(16, 1)]      # empty slot first
+    # A's c0 (parent resident) and B's c1 (parent resident): both slots age
+    # out, the older (A's root, key 254) first
+    assert s.step([(0, [nid(c[0])]), (1, [nid(c[1])])]) == 2
+    assert s.admitted[-1] == [(c[0], 0), (16 + c[1], 1)]
+    assert not (s.texture(0).reshape(-1, 2)[0, 0] & RB)         # A root evicted
+    assert not (s.texture(1).reshape(-1, 2)[0, 0] & RB)         # B root evicted
+    s.detach(0)
+    assert s.slot_node[0] is None and s.node_slot == {16 + c[1]: 1}
+    # B's c2: its parent (B's root) lost its brick, both queued, root first
+    assert s.step([(1, [nid(c[2])])]) == 2
+    assert s.admitted[-1] == [(16, 0), (16 + c[2], 1)]
+    t1 = s.texture(1).reshape(-1, 2)
+    assert t1[0, 0] & RB and t1[0, 1] == pk(1, 1, 1)
+    assert not (t1[c[1], 0] & RB) and t1[c[2], 0] & RB and t1[c[2], 1] == pk(5, 1, 1)
+
+
+def test_shared_detach_drops_queued_entries():
+    from oracle.server import RefSharedServer
+    s = RefSharedServer(slots=(2, 1, 1), B=(2, 2, 2), max_trees=2, cap=16, max_up=1)
+    A = tree()
+    c = [int(t) for t in A[3][1]]
+    s.attach(*A)
+    s.attach(*_no_root_tree())
+    assert s.step([]) == 1 and s.admitted[-1] == [(16, 1)]
+    assert s.step([(0, [nid(c[0]), nid(c[1])])]) == 1
+    assert s.admitted[-1] == [(c[0], 0)] and list(s.fifo) == [c[1]]
+    s.detach(0)
+    assert not s.fifo and not s.pending and s.slot_node == [None, 16]
+    assert s.step([]) == 0 and s.admitted[-1] == []
+    # B's c0 and c1 queued in texel order; one admission, into the empty slot
+    assert s.step([(1, [nid(c[1]), nid(c[0])])]) == 1
+    assert s.admitted[-1] == [(16 + c[0], 0)] and list(s.fifo) == [16 + c[1]]

@@ -1,0 +1,64 @@
+"""Host-side logic of the tile pager (CPU): the frustum test of
+vox::Frustum (plane extraction, normalisation, the near-plane distance),
+the pager's load order, and the screen rectangle of a box, each against
+closed forms."""
+import math
+
+import numpy as np
+
+from voxvizthesis_amd.gvserve import box_rect
+from voxvizthesis_amd.pager import (frustum_planes, load_order, look_at_matrix, perspective,
+                                    sphere_in_frustum)
+
+
+def planes_x(fov=90.0, near=0.1, far=100.0):
+    """Camera at the origin looking along +x, up +z, square viewport."""
+    return frustum_planes(perspective(fov, 1.0, near, far),
+                          look_at_matrix((0, 0, 0), (1, 0, 0), (0, 0, 1)))
+
+
+def test_plane_normals_are_unit_and_point_inward():
+    p = planes_x()
+    assert np.allclose(np.linalg.norm(p[:, :3], axis=1), 1.0)
+    # a point straight ahead is on the inner side of all six planes
+    assert (p[:, :3] @ np.array([10.0, 0, 0]) + p[:, 3] > 0).all()
+    # near plane (last): x = 0.1, far plane: x = 100
+    assert np.allclose(p[5], [1, 0, 0, -0.1])
+    assert np.allclose(p[4], [-1, 0, 0, 100.0])
+    # the side planes of a 90-degree frustum are the diagonals x = +-y, x = +-z
+    s = 1 / math.sqrt(2)
+    sides = {tuple(np.round(q, 6)) for q in p[:4]}
+    for q in ([s, s, 0, 0], [s, -s, 0, 0], [s, 0, s, 0], [s, 0, -s, 0]):
+        assert tuple(np.round(q, 6)) in sides
+
+
+def test_sphere_in_frustum_cases():
+    p = planes_x()
+    ok, d = sphere_in_frustum(p, (10.0, 0.0, 0.0), 1.0)
+    assert ok and abs(d - 9.9) < 1e-5                  # distance to the near plane
+    assert not sphere_in_frustum(p, (-10.0, 0.0, 0.0), 1.0)[0]   # behind
+    assert not sphere_in_frustum(p, (10.0, 10.5, 0.0), 0.1)[0]   # right of x = y
+    # its distance to that plane is 0.5 / sqrt(2): a radius above it reaches in
+    assert sphere_in_frustum(p, (10.0, 10.5, 0.0), 0.36)[0]
+    assert not sphere_in_frustum(p, (10.0, 10.5, 0.0), 0.35)[0]
+    assert not sphere_in_frustum(p, (200.0, 0.0, 0.0), 1.0)[0]   # beyond far
+
+
+def test_load_order_recent_then_near():
+    reqs = [("a", 5, 3.0), ("b", 7, 9.0), ("c", 7, 1.0), ("d", 6, 0.5)]
+    assert [r[0] for r in load_order(reqs)] == ["c", "b", "d", "a"]
+
+
+def test_box_rect_centred_and_behind():
+    cam = dict(position=(0.5, -2.0, 0.5), look=(0.0, 1.0, 0.0), up=(0.0, 0.0, 1.0),
+               fov_y_deg=60.0, width=200, height=100)
+    x0, y0, x1, y1 = box_rect(cam, np.zeros(3), np.ones(3))
+    # symmetric about the viewport centre (to the one-pixel margin)
+    assert abs((x0 + x1) - 200) <= 2 and abs((y0 + y1) - 100) <= 2
+    # the near face (y = 0, 2 away) spans +-0.5 / (2 tan(30 deg)) of the half height
+    half = 0.5 / (2.0 * math.tan(math.radians(30.0)))
+    assert abs((y1 - y0) - (2 * half * 50 + 2)) <= 2
+    # a corner behind the camera: the whole viewport
+    assert box_rect(dict(cam, position=(0.5, 0.5, 0.5)), np.zeros(3), np.ones(3)) is None
+    # entirely off to the side: nothing
+    assert box_rect(dict(cam, position=(40.0, -2.0, 0.5)), np.zeros(3), np.ones(3)) == ()

@@ -26,6 +26,7 @@ EXPORTS = [
     "vox_deallocate_triangle_memory", "vox_compute_edges_face_normals_and_bounds",
     "vox_generate", "vox_get_error", "vox_get_result", "vox_get_stats", "vox_copy_to_host",
     "vox_copy_level", "vox_write_files", "vox_get_compressed", "vox_set_launch",
+    "vox_generate_async", "vox_wait", "vox_release_work_memory", "vox_add_group_gather",
 ]
 # Every symbol include/voxmi_batch.h declares.
 BATCH_EXPORTS = [
@@ -44,14 +45,17 @@ SERVE_EXPORTS = [
     "vox_raycast_frame", "vox_raycaster_get_frame",
     "vox_server_create", "vox_server_destroy", "vox_server_get_error", "vox_server_frame",
     "vox_server_get_stats", "vox_server_latencies", "vox_server_state",
+    "vox_server_create_shared", "vox_server_attach", "vox_server_detach",
+    "vox_server_frame_multi",
 ]
 # Every symbol include/voxmi_tiler.h declares.
 TILER_EXPORTS = [
     "vox_tiler_create", "vox_tiler_destroy", "vox_tiler_get_error", "vox_tiler_assign",
-    "vox_tiler_get",
+    "vox_tiler_get", "vox_tiler_group_bounds",
 ]
 VOX_TREE_REFERENCE_CONST = 1
 VOX_TREE_EXACT_CONST = 2
+VOX_TREE_NO_ROOT_BRICK = 4
 
 
 class VoxGroup(C.Structure):
@@ -136,7 +140,7 @@ class VoxCamera(C.Structure):
 class VoxRaycastParams(C.Structure):
     _fields_ = [("vol_min", C.c_float * 3), ("vol_max", C.c_float * 3), ("ray_step", C.c_float),
                 ("lod_scalar", C.c_float), ("jitter_seed", C.c_uint32),
-                ("max_traversals", C.c_uint32)]
+                ("max_traversals", C.c_uint32), ("rect", C.c_int32 * 4)]
 
 
 class VoxFrameView(C.Structure):
@@ -192,6 +196,10 @@ def load():
     L.vox_write_files.argtypes = [vp, C.c_char_p]
     L.vox_get_compressed.argtypes = [vp, C.POINTER(VoxCompressed)]
     L.vox_set_launch.argtypes = [vp, C.c_uint32]
+    L.vox_generate_async.argtypes = [vp, C.c_int32]
+    L.vox_wait.argtypes = [vp, C.c_int32]
+    L.vox_release_work_memory.argtypes = [vp]
+    L.vox_add_group_gather.argtypes = [vp, C.POINTER(VoxGroup), vp, C.c_uint64, C.c_uint32]
     # batch
     L.vox_batch_create.restype = vp
     L.vox_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32]
@@ -211,6 +219,7 @@ def load():
     L.vox_tiler_get_error.argtypes = [vp]
     L.vox_tiler_assign.argtypes = [vp, vp, C.c_uint64, C.c_int32, C.POINTER(VoxTileGrid)]
     L.vox_tiler_get.argtypes = [vp, C.POINTER(VoxTileLists)]
+    L.vox_tiler_group_bounds.argtypes = [vp, vp, C.c_uint32, vp]
     # node pool / brick pool
     L.vox_brick_pool_create.restype = vp
     L.vox_brick_pool_create.argtypes = [u3, u3]
@@ -244,6 +253,11 @@ def load():
     L.vox_server_get_error.restype = C.c_char_p
     L.vox_server_get_error.argtypes = [vp]
     L.vox_server_frame.argtypes = [vp, vp]
+    L.vox_server_create_shared.restype = vp
+    L.vox_server_create_shared.argtypes = [vp, C.c_uint32, C.c_uint64, C.c_uint32]
+    L.vox_server_attach.argtypes = [vp, vp, C.POINTER(C.c_uint32)]
+    L.vox_server_detach.argtypes = [vp, vp]
+    L.vox_server_frame_multi.argtypes = [vp, C.POINTER(C.c_void_p), C.c_uint32]
     L.vox_server_get_stats.argtypes = [vp, C.POINTER(VoxServerStats)]
     L.vox_server_latencies.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
     L.vox_server_state.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),

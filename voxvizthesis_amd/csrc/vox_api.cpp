@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <new>
 #include <cmath>
@@ -19,6 +20,7 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/voxmi.h"
@@ -32,6 +34,9 @@ namespace {
 struct HostGroup {
     vox_group desc;
     bool device = false;
+    // vox_add_group_gather: desc.ntri rows idx[k] - base of the device arrays
+    const uint32_t* idx = nullptr;
+    uint32_t base = 0;
 };
 
 struct LevelBufs {
@@ -87,6 +92,11 @@ struct vox_ctx {
     uint32_t planAct[VOX_MAX_LEVELS] = {};
     uint32_t planNb[VOX_MAX_LEVELS] = {};
     uint32_t options = 0;
+    // vox_generate_async: the worker thread owns the handle while running.
+    std::thread worker;
+    std::atomic<int> async{0};  // 0 none, 1 running, 2 finished (not waited)
+    int asyncRet = 0;
+    bool workReleased = false;  // vox_release_work_memory freed the level tiles
     float p[3] = {0, 0, 0}, dP[3] = {1, 1, 1}, offP[3] = {0, 0, 0};
     std::stringstream err;
     std::string errCache;
@@ -95,7 +105,7 @@ struct vox_ctx {
 
     // triangle memory
     uint64_t T = 0;
-    DevBuf verts, fnorm, bbox, vn, uv, img, grpFill;
+    DevBuf verts, fnorm, bbox, vn, uv, img, grpFill, gbounds;
     std::vector<GroupArgs> gargs;
     // binning
     uint64_t numBins = 0, pairs = 0;
@@ -146,6 +156,13 @@ struct vox_ctx {
 };
 
 namespace {
+
+// The handle whose asynchronous generate this thread is running.
+thread_local vox_ctx* t_worker = nullptr;
+
+// False while a vox_generate_async of h runs on another thread: every entry
+// point but vox_wait and vox_destroy then refuses the handle.
+bool idle(vox_ctx* h) { return h->async.load(std::memory_order_acquire) != 1 || t_worker == h; }
 
 LevelView view(vox_ctx* h, int k) {
     LevelBufs& l = h->lv[k];
@@ -296,6 +313,8 @@ void vox_deallocate_voxel_memory(vox_ctx* h);
 
 void vox_destroy(vox_ctx* h) {
     if (!h) return;
+    if (h->worker.joinable()) h->worker.join();
+    h->async.store(0);
     vox_deallocate_triangle_memory(h);
     vox_deallocate_voxel_memory(h);
     for (auto e : h->eventPool) (void)hipEventDestroy(e);
@@ -306,7 +325,7 @@ void vox_destroy(vox_ctx* h) {
 }
 
 int vox_set_params(vox_ctx* h, const float p[3], const float delta[3]) {
-    if (!h || !h->valid) return 0;
+    if (!h || !h->valid || !idle(h)) return 0;
     for (int a = 0; a < 3; ++a) {
         h->p[a] = p[a];
         h->dP[a] = delta[a];
@@ -317,13 +336,13 @@ int vox_set_params(vox_ctx* h, const float p[3], const float delta[3]) {
 }
 
 int vox_set_options(vox_ctx* h, uint32_t options) {
-    if (!h) return 0;
+    if (!h || !idle(h)) return 0;
     h->options = options;
     return 1;
 }
 
 static int add_group(vox_ctx* h, const vox_group* g, bool device) {
-    if (!h || !h->valid || !g) return 0;
+    if (!h || !h->valid || !g || !idle(h)) return 0;
     if (!g->verts && g->ntri > 0) return h->fail("group has no vertices"), 0;
     if (g->rgba8 && (g->width <= 0 || g->height <= 0)) return h->fail("group image has no size"), 0;
     if (h->T + g->ntri > 0x7fffffffull) return h->fail("more than 2^31 triangles"), 0;
@@ -339,8 +358,21 @@ static int add_group(vox_ctx* h, const vox_group* g, bool device) {
 int vox_add_group(vox_ctx* h, const vox_group* g) { return add_group(h, g, false); }
 int vox_add_group_device(vox_ctx* h, const vox_group* g) { return add_group(h, g, true); }
 
+int vox_add_group_gather(vox_ctx* h, const vox_group* g, const uint32_t* idx, uint64_t n,
+                         uint32_t base) {
+    if (!h || !h->valid || !g || !idle(h)) return 0;
+    if (n && !idx) return h->fail("gathered group has no index list"), 0;
+    if (n > g->ntri) return h->fail("gathered group selects more triangles than it has"), 0;
+    vox_group d = *g;
+    d.ntri = n;
+    if (!add_group(h, &d, true)) return 0;
+    h->groups.back().idx = idx;
+    h->groups.back().base = base;
+    return 1;
+}
+
 int vox_init(vox_ctx* h) {
-    if (!h || !h->valid) return 0;
+    if (!h || !h->valid || !idle(h)) return 0;
     if (h->inited) return 1;
     if (!h->check(hipSetDevice(0), "hipSetDevice")) return 0;
     if (!h->check(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking), "hipStreamCreate")) return 0;
@@ -354,7 +386,7 @@ int vox_init(vox_ctx* h) {
 }
 
 int vox_allocate_voxel_memory(vox_ctx* h) {
-    if (!h || !h->valid) return 0;
+    if (!h || !h->valid || !idle(h)) return 0;
     if (!h->inited && !vox_init(h)) return 0;
     // Tile maps and per-depth node arrays are sized here; tile pools grow on
     // demand in vox_generate.
@@ -399,7 +431,7 @@ int vox_allocate_voxel_memory(vox_ctx* h) {
 }
 
 void vox_deallocate_voxel_memory(vox_ctx* h) {
-    if (!h) return;
+    if (!h || !idle(h)) return;
     if (h->st) (void)hipStreamSynchronize(h->st);
     for (auto& l : h->lv) {
         l.mark.release(); l.excl.release(); l.tileIdx.release(); l.list.release();
@@ -420,7 +452,7 @@ void vox_deallocate_voxel_memory(vox_ctx* h) {
 }
 
 int vox_allocate_triangle_memory(vox_ctx* h) {
-    if (!h || !h->valid) return 0;
+    if (!h || !h->valid || !idle(h)) return 0;
     if (!h->inited && !vox_init(h)) return 0;
     uint64_t T = h->T, nvn = 0, nuv = 0, nimg = 0;
     for (auto& g : h->groups) {
@@ -430,7 +462,8 @@ int vox_allocate_triangle_memory(vox_ctx* h) {
             nimg += (uint64_t)g.desc.width * g.desc.height * 4;
         }
     }
-    if (!h->verts.ensure(T * 36, h->err, "vertex buffer") ||
+    if (!h->gbounds.ensure(h->groups.size() * 24 + 24, h->err, "group bounds") ||
+        !h->verts.ensure(T * 36, h->err, "vertex buffer") ||
         !h->fnorm.ensure(T * 12, h->err, "face normals") ||
         !h->bbox.ensure(T * 12, h->err, "bounds") ||
         !h->vn.ensure(nvn * 36, h->err, "vertex normals") ||
@@ -447,20 +480,35 @@ int vox_allocate_triangle_memory(vox_ctx* h) {
         GroupArgs ga{};
         ga.triOff = (uint32_t)to;
         ga.ntri = (uint32_t)d.ntri;
-        if (d.ntri && !h->check(hipMemcpyAsync(h->verts.as<float>() + 9 * to, d.verts, d.ntri * 36,
-                                               kind, h->st), "vertex copy"))
+        const bool gather = g.idx != nullptr;
+        if (gather) {
+            // rows picked by index, and the group's bounds from them (the
+            // tiler's per-tile triBBox, main.cu 3868-3870)
+            int32_t* gb = h->gbounds.as<int32_t>() + 6 * (&g - h->groups.data());
+            launch_fill_u32(h->st, (uint32_t*)gb, 3, 0x7fffffffu);
+            launch_fill_u32(h->st, (uint32_t*)gb + 3, 3, 0x80000000u);
+            launch_gather_rows(h->st, d.verts, g.idx, d.ntri, g.base, 9, h->verts.as<float>() + 9 * to);
+            launch_tri_bounds(h->st, h->verts.as<float>() + 9 * to, d.ntri, gb);
+        } else if (d.ntri && !h->check(hipMemcpyAsync(h->verts.as<float>() + 9 * to, d.verts,
+                                                      d.ntri * 36, kind, h->st), "vertex copy")) {
             return 0;
+        }
         if (d.vnormals) {
-            if (d.ntri && !h->check(hipMemcpyAsync(h->vn.as<float>() + 9 * vo, d.vnormals,
-                                                   d.ntri * 36, kind, h->st), "normal copy"))
+            if (gather)
+                launch_gather_rows(h->st, d.vnormals, g.idx, d.ntri, g.base, 9,
+                                   h->vn.as<float>() + 9 * vo);
+            else if (d.ntri && !h->check(hipMemcpyAsync(h->vn.as<float>() + 9 * vo, d.vnormals,
+                                                        d.ntri * 36, kind, h->st), "normal copy"))
                 return 0;
             ga.vn = h->vn.as<float>() + 9 * vo;
             ga.hasVN = 1;
             vo += d.ntri;
         }
         if (d.uvs && d.rgba8) {
-            if (d.ntri && !h->check(hipMemcpyAsync(h->uv.as<float>() + 6 * uo, d.uvs, d.ntri * 24,
-                                                   kind, h->st), "uv copy"))
+            if (gather)
+                launch_gather_rows(h->st, d.uvs, g.idx, d.ntri, g.base, 6, h->uv.as<float>() + 6 * uo);
+            else if (d.ntri && !h->check(hipMemcpyAsync(h->uv.as<float>() + 6 * uo, d.uvs,
+                                                        d.ntri * 24, kind, h->st), "uv copy"))
                 return 0;
             uint64_t ib = (uint64_t)d.width * d.height * 4;
             if (!h->check(hipMemcpyAsync(h->img.as<uint8_t>() + io, d.rgba8, ib, kind, h->st),
@@ -480,16 +528,37 @@ int vox_allocate_triangle_memory(vox_ctx* h) {
         h->gargs.push_back(ga);
         to += d.ntri;
     }
-    if (!h->check(hipStreamSynchronize(h->st), "triangle upload")) return 0;
+    if (!h->check(hipGetLastError(), "triangle gather") ||
+        !h->check(hipStreamSynchronize(h->st), "triangle upload"))
+        return 0;
+    bool anyGather = false;
+    for (auto& g : h->groups) anyGather |= g.idx != nullptr;
+    if (anyGather) {
+        std::vector<int32_t> gb(6 * h->groups.size());
+        if (!h->check(hipMemcpy(gb.data(), h->gbounds.p, 4 * gb.size(), hipMemcpyDeviceToHost),
+                      "group bounds"))
+            return 0;
+        for (size_t k = 0; k < h->groups.size(); ++k) {
+            HostGroup& g = h->groups[k];
+            if (!g.idx || g.desc.ntri == 0) continue;
+            for (int a = 0; a < 6; ++a) {
+                const int32_t o = gb[6 * k + a];
+                const int32_t bits = o >= 0 ? o : (int32_t)(0x80000000u - (uint32_t)(o + 1));
+                float f;
+                std::memcpy(&f, &bits, 4);
+                (a < 3 ? g.desc.bbox_min[a] : g.desc.bbox_max[a - 3]) = f;
+            }
+        }
+    }
     h->triMem = true;
     h->prepared = false;
     return 1;
 }
 
 void vox_deallocate_triangle_memory(vox_ctx* h) {
-    if (!h) return;
+    if (!h || !idle(h)) return;
     if (h->st) (void)hipStreamSynchronize(h->st);
-    h->verts.release(); h->fnorm.release(); h->bbox.release(); h->vn.release();
+    h->verts.release(); h->fnorm.release(); h->bbox.release(); h->vn.release(); h->gbounds.release();
     h->uv.release(); h->img.release(); h->grpFill.release();
     h->groups.clear();
     h->gargs.clear();
@@ -499,7 +568,7 @@ void vox_deallocate_triangle_memory(vox_ctx* h) {
 }
 
 int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
-    if (!h || !h->valid) return 0;
+    if (!h || !h->valid || !idle(h)) return 0;
     if (!h->voxMem && !vox_allocate_voxel_memory(h)) return 0;
     if (!h->triMem && !vox_allocate_triangle_memory(h)) return 0;
     VoxParams& P = h->P;
@@ -567,7 +636,7 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
 }
 
 int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
-    if (!h || !h->valid) return 0;
+    if (!h || !h->valid || !idle(h)) return 0;
     h->compressed = compressed != 0;
     if (h->replay && out_dir) return h->fail("a replayed generate writes no files"), 0;
     if (!h->prepared && !vox_compute_edges_face_normals_and_bounds(h)) return 0;
@@ -789,6 +858,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
                              h->bitmap.as<uint32_t>());
         });
     }
+    h->workReleased = false;
     if (h->replay) {
         h->haveResult = true;
         h->prepared = false;
@@ -821,12 +891,13 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
 
 const char* vox_get_error(vox_ctx* h) {
     if (!h) return "null handle";
+    if (!idle(h)) return "the handle is running vox_generate_async; call vox_wait first";
     h->errCache = h->err.str();
     return h->errCache.c_str();
 }
 
 int vox_reset(vox_ctx* h) {
-    if (!h) return 0;
+    if (!h || !idle(h)) return 0;
     // resetCuda (Voxelizer.cu 324-344) frees, resets the device and
     // reallocates.  The device is not reset here: other users of the process
     // (torch, other handles) own memory on it.  Freeing everything and
@@ -847,7 +918,7 @@ int vox_reset(vox_ctx* h) {
 }
 
 int vox_get_result(vox_ctx* h, vox_result* out) {
-    if (!h || !h->valid || !out) return 0;
+    if (!h || !h->valid || !out || !idle(h)) return 0;
     if (!h->haveResult) return h->fail("no result: call vox_generate first"), 0;
     std::memset(out, 0, sizeof(*out));
     out->levels = h->L;
@@ -875,13 +946,13 @@ int vox_get_result(vox_ctx* h, vox_result* out) {
 }
 
 int vox_get_stats(vox_ctx* h, vox_stats* out) {
-    if (!h || !out) return 0;
+    if (!h || !out || !idle(h)) return 0;
     *out = h->stats;
     return 1;
 }
 
 int vox_copy_to_host(vox_ctx* h, void* dst, const void* src_device, uint64_t nbytes) {
-    if (!h || !h->st) return 0;
+    if (!h || !h->st || !idle(h)) return 0;
     if (nbytes == 0) return 1;
     if (!h->check(hipMemcpyAsync(dst, src_device, nbytes, hipMemcpyDeviceToHost, h->st), "copy") ||
         !h->check(hipStreamSynchronize(h->st), "copy"))
@@ -890,7 +961,8 @@ int vox_copy_to_host(vox_ctx* h, void* dst, const void* src_device, uint64_t nby
 }
 
 int vox_copy_level(vox_ctx* h, int32_t level, uint32_t* colors, float* normals, uint32_t dims[3]) {
-    if (!h || !h->valid || !h->haveResult) return 0;
+    if (!h || !h->valid || !h->haveResult || !idle(h)) return 0;
+    if (h->workReleased) return h->fail("vox_copy_level: the level tiles were released"), 0;
     if (level < 0 || level >= h->L) return h->fail("level out of range"), 0;
     LevelView l = view(h, level);
     for (int a = 0; a < 3; ++a) dims[a] = l.map.dims[a];
@@ -911,7 +983,7 @@ int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3], cons
                          const uint32_t N[3]);
 
 int vox_get_compressed(vox_ctx* h, vox_compressed* out) {
-    if (!h || !h->valid || !out) return 0;
+    if (!h || !h->valid || !out || !idle(h)) return 0;
     if (!h->haveResult || !h->compressed) return 0;
     std::memset(out, 0, sizeof(*out));
     const uint32_t e[3] = {h->B[0] + 2, h->B[1] + 2, h->B[2] + 2};
@@ -926,7 +998,7 @@ int vox_get_compressed(vox_ctx* h, vox_compressed* out) {
 }
 
 int vox_write_files(vox_ctx* h, const char* out_dir) {
-    if (!h || !h->valid || !out_dir) return 0;
+    if (!h || !h->valid || !out_dir || !idle(h)) return 0;
     if (!vox_write_files_impl(h, out_dir, h->p, h->dP, h->N))
         return h->fail(std::string("failed to write GigaVoxels files to ") + out_dir), 0;
     return 1;
@@ -984,8 +1056,9 @@ uint64_t host_signature(vox_ctx* h) {
     return z;
 }
 
-// Device checksum of a job's uploaded triangle data into *out.
-void launch_input_hash(vox_ctx* h, hipStream_t st, unsigned long long* out) {
+// The job's input buffers as hash segments (vertices, vertex normals, uvs,
+// images), each block of the hash launch covering at least 2048 words.
+void input_segments(vox_ctx* h, uint32_t job, std::vector<HashSeg>& segs, uint32_t& blocks) {
     uint64_t nvn = 0, nuv = 0, nimg = 0;
     for (auto& g : h->groups) {
         if (g.desc.vnormals) nvn += g.desc.ntri;
@@ -994,10 +1067,20 @@ void launch_input_hash(vox_ctx* h, hipStream_t st, unsigned long long* out) {
             nimg += (uint64_t)g.desc.width * g.desc.height * 4;
         }
     }
-    launch_hash_u32(st, h->verts.as<uint32_t>(), h->T * 9, 1, out);
-    launch_hash_u32(st, h->vn.as<uint32_t>(), nvn * 9, 2, out);
-    launch_hash_u32(st, h->uv.as<uint32_t>(), nuv * 6, 3, out);
-    launch_hash_u32(st, h->img.as<uint32_t>(), nimg / 4, 4, out);
+    const struct {
+        const uint32_t* p;
+        uint64_t n;
+    } bufs[4] = {{h->verts.as<uint32_t>(), h->T * 9},
+                 {h->vn.as<uint32_t>(), nvn * 9},
+                 {h->uv.as<uint32_t>(), nuv * 6},
+                 {h->img.as<uint32_t>(), nimg / 4}};
+    for (uint64_t k = 0; k < 4; ++k) {
+        if (bufs[k].n == 0 || !bufs[k].p) continue;
+        uint64_t nb = (bufs[k].n + 2047) / 2048;
+        if (nb > 1024) nb = 1024;
+        segs.push_back(HashSeg{bufs[k].p, bufs[k].n, k + 1, job, blocks});
+        blocks += (uint32_t)nb;
+    }
 }
 
 }  // namespace
@@ -1010,7 +1093,12 @@ struct vox_batch {
     bool valid = false, captured = false;
     std::vector<std::vector<void*>> bufs;
     std::vector<uint64_t> hostSig, devSig;
-    DevBuf sums;
+    DevBuf sums, segs;
+    uint32_t nseg = 0, hashBlocks = 0;
+    // one captured branch per job: its stream and the event joining it back
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> joined;
+    hipEvent_t fork = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     vox_batch_stats stats{};
     std::stringstream err;
@@ -1024,10 +1112,22 @@ struct vox_batch {
         err << what << " failed: " << hipGetErrorString(e) << std::endl;
         return false;
     }
+    // The segment table of every job's inputs (their buffers are fixed
+    // between a capture and its launches).
+    bool build_segments() {
+        std::vector<HashSeg> v;
+        uint32_t blocks = 0;
+        for (size_t j = 0; j < jobs.size(); ++j) input_segments(jobs[j], (uint32_t)j, v, blocks);
+        nseg = (uint32_t)v.size();
+        hashBlocks = blocks;
+        if (nseg == 0) return true;
+        return segs.ensure(sizeof(HashSeg) * v.size(), err, "hash segments") &&
+               check(hipMemcpy(segs.p, v.data(), sizeof(HashSeg) * v.size(), hipMemcpyHostToDevice),
+                     "hash segments");
+    }
     bool checksums(std::vector<uint64_t>& out) {
         launch_fill_u32(st, sums.as<uint32_t>(), 2 * jobs.size(), 0u);
-        for (size_t j = 0; j < jobs.size(); ++j)
-            launch_input_hash(jobs[j], st, sums.as<unsigned long long>() + j);
+        launch_hash_segs(st, segs.as<HashSeg>(), nseg, hashBlocks, sums.as<unsigned long long>());
         out.assign(jobs.size(), 0);
         return check(hipMemcpyAsync(out.data(), sums.p, 8 * jobs.size(), hipMemcpyDeviceToHost, st),
                      "checksum") &&
@@ -1053,7 +1153,7 @@ vox_batch* vox_batch_create(vox_ctx* const* jobs, int32_t njobs) {
     }
     for (int32_t j = 0; j < njobs; ++j) {
         vox_ctx* h = jobs[j];
-        if (!h || !h->valid || !h->planned || !h->triMem) {
+        if (!h || !h->valid || !idle(h) || !h->planned || !h->triMem) {
             b->fail("batch: job " + std::to_string(j) +
                     " has no eager vox_generate with uploaded triangles to size it");
             return b;
@@ -1070,6 +1170,16 @@ vox_batch* vox_batch_create(vox_ctx* const* jobs, int32_t njobs) {
         return b;
     for (auto& e : b->ev)
         if (!b->check(hipEventCreate(&e), "hipEventCreate")) return b;
+    if (!b->check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate"))
+        return b;
+    b->branch.assign(b->jobs.size(), nullptr);
+    b->joined.assign(b->jobs.size(), nullptr);
+    for (size_t j = 0; j < b->jobs.size(); ++j)
+        if (!b->check(hipStreamCreateWithFlags(&b->branch[j], hipStreamNonBlocking),
+                      "hipStreamCreate") ||
+            !b->check(hipEventCreateWithFlags(&b->joined[j], hipEventDisableTiming),
+                      "hipEventCreate"))
+            return b;
     b->stats.jobs = njobs;
     b->valid = true;
     return b;
@@ -1081,7 +1191,13 @@ void vox_batch_destroy(vox_batch* b) {
     b->drop_graph();
     for (auto e : b->ev)
         if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (auto e : b->joined)
+        if (e) (void)hipEventDestroy(e);
+    for (auto q : b->branch)
+        if (q) (void)hipStreamDestroy(q);
     b->sums.release();
+    b->segs.release();
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
 }
@@ -1097,23 +1213,35 @@ int vox_batch_capture(vox_batch* b) {
     b->drop_graph();
     const auto t0 = std::chrono::steady_clock::now();
     for (vox_ctx* h : b->jobs)
+        if (!idle(h)) return b->fail("batch: a job is running vox_generate_async"), 0;
+    for (vox_ctx* h : b->jobs)
         if (!h->check(hipStreamSynchronize(h->st), "synchronize")) return b->fail(h->err.str()), 0;
     b->bufs.assign(b->jobs.size(), {});
     for (size_t j = 0; j < b->jobs.size(); ++j) buffer_list(b->jobs[j], b->bufs[j]);
+    // Fork-join capture: every job is a branch of its own, forked from and
+    // joined back into the batch stream, so the graph holds independent
+    // chains the device can run side by side.
     if (!b->check(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal), "begin capture"))
         return 0;
-    bool ok = true;
+    bool ok = b->check(hipEventRecord(b->fork, b->st), "fork");
     for (size_t j = 0; j < b->jobs.size() && ok; ++j) {
         vox_ctx* h = b->jobs[j];
         hipStream_t own = h->st;
-        h->st = b->st;
+        ok = b->check(hipStreamWaitEvent(b->branch[j], b->fork, 0), "fork");
+        if (!ok) break;
+        h->st = b->branch[j];
         h->replay = true;
         h->prepared = false;
         ok = vox_compute_edges_face_normals_and_bounds(h) == 1 &&
              vox_generate(h, nullptr, 1, h->compressed ? 1 : 0) != 0;
         h->replay = false;
         h->st = own;
-        if (!ok) b->fail("batch: capture of job " + std::to_string(j) + " failed: " + h->err.str());
+        if (!ok) {
+            b->fail("batch: capture of job " + std::to_string(j) + " failed: " + h->err.str());
+            break;
+        }
+        ok = b->check(hipEventRecord(b->joined[j], b->branch[j]), "join") &&
+             b->check(hipStreamWaitEvent(b->st, b->joined[j], 0), "join");
     }
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->st, &g);
@@ -1140,7 +1268,7 @@ int vox_batch_capture(vox_batch* b) {
     }
     b->hostSig.resize(b->jobs.size());
     for (size_t j = 0; j < b->jobs.size(); ++j) b->hostSig[j] = host_signature(b->jobs[j]);
-    if (!b->checksums(b->devSig)) return 0;
+    if (!b->build_segments() || !b->checksums(b->devSig)) return 0;
     b->captured = true;
     b->stats.ms_capture =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1150,6 +1278,8 @@ int vox_batch_capture(vox_batch* b) {
 int vox_batch_launch(vox_batch* b) {
     if (!b || !b->valid) return 0;
     if (!b->captured) return b->fail("batch: capture first"), 0;
+    for (vox_ctx* h : b->jobs)
+        if (!idle(h)) return b->fail("batch: a job is running vox_generate_async"), 0;
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         vox_ctx* h = b->jobs[j];
         std::vector<void*> now;
@@ -1187,10 +1317,57 @@ int vox_batch_get_stats(vox_batch* b, vox_batch_stats* out) {
 }
 
 int vox_set_launch(vox_ctx* h, uint32_t persistent_blocks) {
-    if (!h || !h->valid) return 0;
+    if (!h || !h->valid || !idle(h)) return 0;
     if (persistent_blocks == 0 || persistent_blocks > (1u << 20))
         return h->fail("persistent_blocks out of range"), 0;
     h->persistentBlocks = persistent_blocks;
+    return 1;
+}
+
+int vox_generate_async(vox_ctx* h, int32_t compressed) {
+    if (!h || !h->valid || !idle(h)) return 0;
+    if (!h->inited && !vox_init(h)) return 0;
+    if (h->worker.joinable()) h->worker.join();  // finished, never waited for
+    h->async.store(1, std::memory_order_release);
+    try {
+        h->worker = std::thread([h, compressed] {
+            t_worker = h;
+            (void)hipSetDevice(0);
+            const int r = vox_generate(h, nullptr, 1, compressed);
+            h->asyncRet = r;
+            t_worker = nullptr;
+            h->async.store(2, std::memory_order_release);
+        });
+    } catch (...) {
+        h->async.store(0);
+        return h->fail("vox_generate_async: no worker thread"), 0;
+    }
+    return 1;
+}
+
+int vox_wait(vox_ctx* h, int32_t block) {
+    if (!h) return 0;
+    const int s = h->async.load(std::memory_order_acquire);
+    if (s == 0) return h->fail("vox_wait: no vox_generate_async was started"), 0;
+    if (s == 1 && !block) return 2;
+    h->worker.join();
+    h->async.store(0);
+    return h->asyncRet;
+}
+
+int vox_release_work_memory(vox_ctx* h) {
+    if (!h || !h->valid || !idle(h)) return 0;
+    if (h->st && !h->check(hipStreamSynchronize(h->st), "synchronize")) return 0;
+    for (auto& l : h->lv) {
+        l.col.release(); l.qn.release(); l.nrm.release();
+        l.active = 0;
+    }
+    h->occ0.release(); h->hit0.release(); h->work.release(); h->workCount.release();
+    h->rec.release(); h->recCount.release(); h->binTri.release(); h->binCount.release();
+    h->binStart.release(); h->scanTmp.release();
+    vox_deallocate_triangle_memory(h);
+    h->planned = false;
+    h->workReleased = true;
     return 1;
 }
 

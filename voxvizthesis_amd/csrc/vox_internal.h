@@ -93,6 +93,8 @@ struct OctreeRefs {
     uint32_t* poolCol;
     uint32_t* poolNrm;
     hipStream_t st;         // the pool's stream
+    hipEvent_t ready;       // recorded after the last write of the node texture
+    vox_brick_pool* pool;
     // per depth: node types, brick index, node texel (0xffffffff: not in tree)
     const uint8_t* types[kMaxLevels];
     const uint32_t* bidx[kMaxLevels];
@@ -101,10 +103,31 @@ struct OctreeRefs {
 };
 bool octree_refs(vox_octree* t, OctreeRefs& r);
 
-// The request list of a ray caster's last frame: distinct node ids, the
-// device clock when each was appended, and the count (host-side).
-bool raycaster_requests(vox_raycaster* rc, const uint32_t** ids, const uint64_t** times,
-                        uint32_t* n);
+// The request list of a ray caster's last frame, all on the device: distinct
+// node ids, the device clock when each was appended, the count (the first
+// of the ray caster's counters, at most maxIds), the event recorded when
+// the list was complete, and the tree it was cast against.
+struct RequestRefs {
+    const uint32_t* ids;
+    const uint64_t* times;
+    const uint32_t* count;
+    uint32_t maxIds;
+    hipEvent_t done;
+    vox_octree* tree;
+};
+bool raycaster_requests(vox_raycaster* rc, RequestRefs& r);
+
+// A brick pool's storage and its stream.
+struct PoolRefs {
+    uint32_t B[3], dims[3], slots[3];
+    uint32_t* col;
+    uint32_t* nrm;
+    hipStream_t st;
+};
+bool pool_refs(vox_brick_pool* bp, PoolRefs& r);
+// Record the octree's ready event on `st` after work there wrote its node
+// texture.
+bool octree_record_written(vox_octree* t, hipStream_t st);
 
 // ---- launchers (all asynchronous on `st`) ----
 void launch_setup(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
@@ -154,9 +177,24 @@ void launch_occupancy(hipStream_t st, const VoxParams& P, uint64_t nActive, cons
 void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
                         const float* nrm, uint32_t* outCol, float* outNrm);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
-// *out += order-independent checksum of n words (salted).
-void launch_hash_u32(hipStream_t st, const uint32_t* p, uint64_t n, uint64_t salt,
-                     unsigned long long* out);
+// dst row k = src row idx[k] - base, w floats per row.
+void launch_gather_rows(hipStream_t st, const float* src, const uint32_t* idx, uint64_t n,
+                        uint32_t base, uint32_t w, float* dst);
+// Vertex bounds of n triangles into out[6] (ordered ints: min xyz, max xyz;
+// out must hold INT_MAX x3, INT_MIN x3 first).
+void launch_tri_bounds(hipStream_t st, const float* v, uint64_t n, int32_t* out);
+// One buffer of a multi-buffer hash: n words at p hashed with salt into
+// outs[out]; block0 = the first block of the launch that works on it
+// (segments in ascending block0, none empty).
+struct HashSeg {
+    const uint32_t* p;
+    uint64_t n;
+    uint64_t salt;
+    uint32_t out;
+    uint32_t block0;
+};
+void launch_hash_segs(hipStream_t st, const HashSeg* segs, uint32_t nseg, uint32_t blocks,
+                      unsigned long long* outs);
 // Block compression of bricks: BC3 colours (16 B per block) and BC1 normals
 // (8 B per block), ceil(e/4)^2 * e blocks per brick of e^3 texels.
 uint64_t bc_blocks_per_brick(const uint32_t e[3]);

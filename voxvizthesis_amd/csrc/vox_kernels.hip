@@ -1294,15 +1294,78 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
-__global__ void k_hash_u32(const uint32_t* __restrict__ p, uint64_t n, uint64_t salt,
-                           unsigned long long* __restrict__ out) {
+// Several buffers hashed by one launch: segment s owns the blocks from
+// segs[s].block0 up to the next segment's; each block adds its partial sum
+// to the segment's output word.
+__global__ void k_hash_segs(const HashSeg* __restrict__ segs, uint32_t nseg,
+                            unsigned long long* __restrict__ outs) {
+    __shared__ unsigned long long part[kBlock / 64];
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (segs[mid].block0 <= b) lo = mid;
+        else hi = mid;
+    }
+    const HashSeg sg = segs[lo];
+    const uint32_t nb = (lo + 1 < nseg ? segs[lo + 1].block0 : gridDim.x) - sg.block0;
     unsigned long long acc = 0ull;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        acc += mix64(((unsigned long long)p[i] << 32 | (unsigned long long)(i & 0xffffffffu)) ^
-                     (salt * 0x9E3779B97F4A7C15ull + (i >> 32)));
+    for (uint64_t i = (uint64_t)(b - sg.block0) * blockDim.x + threadIdx.x; i < sg.n;
+         i += (uint64_t)nb * blockDim.x)
+        acc += mix64(((unsigned long long)sg.p[i] << 32 | (unsigned long long)(i & 0xffffffffu)) ^
+                     (sg.salt * 0x9E3779B97F4A7C15ull + (i >> 32)));
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
-    if ((threadIdx.x & 63u) == 0u) atomicAdd(out, acc);
+    if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0ull;
+        for (uint32_t w = 0; w < blockDim.x / 64u; ++w) t += part[w];
+        atomicAdd(&outs[sg.out], t);
+    }
+}
+
+// Rows idx[k] - base of src (w floats each) into row k of dst.
+__global__ void k_gather_rows(const float* __restrict__ src, const uint32_t* __restrict__ idx,
+                              uint64_t n, uint32_t base, uint32_t w, float* __restrict__ dst) {
+    const uint64_t total = n * w;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = e / w;
+        const uint32_t c = (uint32_t)(e - k * w);
+        dst[e] = src[(uint64_t)(idx[k] - base) * w + c];
+    }
+}
+
+// Order-preserving int image of a float, for atomic min / max.
+__device__ __forceinline__ int32_t ord_f32(float f) {
+    const int32_t i = __float_as_int(f);
+    return i >= 0 ? i : (int32_t)(0x80000000u - (uint32_t)i) - 1;
+}
+
+// Bounds of n triangles' vertices (9 floats each): out[0..2] min, [3..5] max
+// as ordered ints (out starts at INT_MAX / INT_MIN).
+__global__ void k_tri_bounds(const float* __restrict__ v, uint64_t n, int32_t* __restrict__ out) {
+    int32_t mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+    int32_t mx[3] = {(int32_t)0x80000000, (int32_t)0x80000000, (int32_t)0x80000000};
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x)
+        for (int k = 0; k < 9; ++k) {
+            const int32_t o = ord_f32(v[9 * t + k]);
+            const int a = k % 3;
+            mn[a] = o < mn[a] ? o : mn[a];
+            mx[a] = o > mx[a] ? o : mx[a];
+        }
+    for (int a = 0; a < 3; ++a)
+        for (int o = 32; o > 0; o >>= 1) {
+            const int32_t m = __shfl_down(mn[a], o), M = __shfl_down(mx[a], o);
+            mn[a] = m < mn[a] ? m : mn[a];
+            mx[a] = M > mx[a] ? M : mx[a];
+        }
+    if ((threadIdx.x & 63u) == 0u)
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&out[a], mn[a]);
+            atomicMax(&out[3 + a], mx[a]);
+        }
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
@@ -1316,11 +1379,25 @@ inline unsigned grid_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 1)
 }  // namespace
 
 // Fill kernel instead of hipMemsetAsync so the pipeline stays capturable.
-void launch_hash_u32(hipStream_t st, const uint32_t* p, uint64_t n, uint64_t salt,
-                     unsigned long long* out) {
+void launch_hash_segs(hipStream_t st, const HashSeg* segs, uint32_t nseg, uint32_t blocks,
+                      unsigned long long* outs) {
+    if (nseg == 0 || blocks == 0) return;
+    k_hash_segs<<<blocks, kBlock, 0, st>>>(segs, nseg, outs);
+}
+
+void launch_gather_rows(hipStream_t st, const float* src, const uint32_t* idx, uint64_t n,
+                        uint32_t base, uint32_t w, float* dst) {
     if (n == 0) return;
-    const uint64_t g = (n + kBlock - 1) / kBlock;
-    k_hash_u32<<<(unsigned)(g < 4096 ? g : 4096), kBlock, 0, st>>>(p, n, salt, out);
+    uint64_t blocks = (n * w + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    k_gather_rows<<<(unsigned)blocks, kBlock, 0, st>>>(src, idx, n, base, w, dst);
+}
+
+void launch_tri_bounds(hipStream_t st, const float* v, uint64_t n, int32_t* out) {
+    if (n == 0) return;
+    uint64_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 4096) blocks = 4096;
+    k_tri_bounds<<<(unsigned)blocks, kBlock, 0, st>>>(v, n, out);
 }
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {

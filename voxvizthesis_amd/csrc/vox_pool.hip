@@ -180,7 +180,7 @@ __global__ void k_brick_upload(BrickSrc src, const uint4* __restrict__ req,
                                const uint32_t* __restrict__ nreq, uint32_t ex, uint32_t ey,
                                uint32_t ez, uint32_t sx, uint32_t sy, uint32_t dx, uint32_t dy,
                                uint32_t* __restrict__ col, uint32_t* __restrict__ nrm,
-                               uint32_t* __restrict__ tex, unsigned long long* __restrict__ done) {
+                               uint32_t* __restrict__ tex) {
     const uint32_t count = *nreq;
     const uint32_t nv = ex * ey * ez;
     for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
@@ -203,10 +203,6 @@ __global__ void k_brick_upload(BrickSrc src, const uint4* __restrict__ req,
             tex[2 * (uint64_t)q.w] |= kResidentBit;
             tex[2 * (uint64_t)q.w + 1] = pack10(ox + 1, oy + 1, oz + 1);
         }
-        if (done) {
-            __syncthreads();  // the whole brick is written
-            if (threadIdx.x == 0) done[r] = (unsigned long long)wall_clock64();
-        }
     }
 }
 
@@ -217,19 +213,7 @@ void launch_brick_upload(hipStream_t st, const BrickSrc& src, const uint4* req,
     if (maxReq == 0) return;
     const unsigned grid = maxReq < 4096u ? maxReq : 4096u;
     k_brick_upload<<<grid, kPoolBlock, 0, st>>>(src, req, nreq, e[0], e[1], e[2], slots[0],
-                                                slots[1], dims[0], dims[1], col, nrm, tex,
-                                                nullptr);
-}
-
-// Same, recording the device clock when each request's copy has finished.
-void launch_brick_upload_timed(hipStream_t st, const BrickSrc& src, const uint4* req,
-                               const uint32_t* nreq, uint32_t maxReq, const uint32_t e[3],
-                               const uint32_t slots[3], const uint32_t dims[3], uint32_t* col,
-                               uint32_t* nrm, uint32_t* tex, unsigned long long* done) {
-    if (maxReq == 0) return;
-    const unsigned grid = maxReq < 4096u ? maxReq : 4096u;
-    k_brick_upload<<<grid, kPoolBlock, 0, st>>>(src, req, nreq, e[0], e[1], e[2], slots[0],
-                                                slots[1], dims[0], dims[1], col, nrm, tex, done);
+                                                slots[1], dims[0], dims[1], col, nrm, tex);
 }
 
 }  // namespace voxmi
@@ -267,6 +251,10 @@ struct vox_octree {
     DevBuf state[VOX_MAX_LEVELS], sub[VOX_MAX_LEVELS], pre[VOX_MAX_LEVELS];
     DevBuf texel[VOX_MAX_LEVELS];
     DevBuf tex, req, nreq;
+    // Recorded after every write of the node texture (the build here, a
+    // brick server's frame): readers of the texture on other streams wait
+    // on it.
+    hipEvent_t ready = nullptr;
     BrickSrc src{};
     uint64_t bricks[VOX_MAX_LEVELS] = {};
     std::stringstream err;
@@ -413,7 +401,8 @@ bool build_tree(vox_octree* t, const vox_result* r) {
     // host staging of the one request; both stay alive until the final sync
     const uint4 q = make_uint4(0u, 0u, bp->resident, 0u);  // root texel is 0
     const uint32_t one = 1;
-    if (rootType == 1 && t->bricks[0] > 0 && bp->resident < bp->nslots) {
+    if (rootType == 1 && t->bricks[0] > 0 && bp->resident < bp->nslots &&
+        !(t->flags & VOX_TREE_NO_ROOT_BRICK)) {
         if (!t->req.ensure(sizeof(uint4), t->err, "request") ||
             !t->nreq.ensure(4, t->err, "request count"))
             return false;
@@ -427,6 +416,7 @@ bool build_tree(vox_octree* t, const vox_result* r) {
         ++bp->resident;
     }
     return t->check(hipGetLastError(), "brick upload") &&
+           t->check(hipEventRecord(t->ready, st), "event record") &&
            t->check(hipStreamSynchronize(st), "synchronize");
 }
 
@@ -447,6 +437,8 @@ bool octree_refs(vox_octree* t, OctreeRefs& r) {
     r.poolCol = t->bp->col.as<uint32_t>();
     r.poolNrm = t->bp->nrm.as<uint32_t>();
     r.st = t->bp->st;
+    r.ready = t->ready;
+    r.pool = t->bp;
     for (int d = 0; d < kMaxLevels; ++d) {
         const bool in = d < t->L;
         r.types[d] = in ? t->types[d].as<uint8_t>() : nullptr;
@@ -455,6 +447,23 @@ bool octree_refs(vox_octree* t, OctreeRefs& r) {
     }
     r.src = t->src;
     return true;
+}
+
+bool pool_refs(vox_brick_pool* bp, PoolRefs& r) {
+    if (!bp || !bp->valid) return false;
+    for (int a = 0; a < 3; ++a) {
+        r.B[a] = bp->B[a];
+        r.dims[a] = bp->dims[a];
+        r.slots[a] = bp->slots[a];
+    }
+    r.col = bp->col.as<uint32_t>();
+    r.nrm = bp->nrm.as<uint32_t>();
+    r.st = bp->st;
+    return true;
+}
+
+bool octree_record_written(vox_octree* t, hipStream_t st) {
+    return t && t->ready && hipEventRecord(t->ready, st) == hipSuccess;
 }
 
 }  // namespace voxmi
@@ -539,6 +548,12 @@ vox_octree* vox_octree_create(vox_brick_pool* bp, const vox_result* tree, uint32
         t->fail("octree: tree is NULL");
         return t;
     }
+    if (flags & ~(VOX_TREE_REFERENCE_CONST | VOX_TREE_EXACT_CONST | VOX_TREE_NO_ROOT_BRICK)) {
+        t->fail("octree: unknown flags");
+        return t;
+    }
+    if (!t->check(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming), "hipEventCreate"))
+        return t;
     t->valid = build_tree(t, tree);
     return t;
 }
@@ -558,6 +573,7 @@ void vox_octree_destroy(vox_octree* t) {
     t->tex.release();
     t->req.release();
     t->nreq.release();
+    if (t->ready) (void)hipEventDestroy(t->ready);
     delete t;
 }
 
@@ -580,6 +596,8 @@ int vox_octree_get_view(vox_octree* t, vox_octree_view* out) {
 
 int vox_pool_copy_to_host(void* dst, const void* src_device, uint64_t nbytes) {
     if (!dst || !src_device) return 0;
+    // the library's streams are non-blocking: wait for all of them
+    if (hipDeviceSynchronize() != hipSuccess) return 0;
     return hipMemcpy(dst, src_device, nbytes, hipMemcpyDeviceToHost) == hipSuccess ? 1 : 0;
 }
 

@@ -43,7 +43,8 @@ struct RayUniforms {
     float look[3], right[3], upv[3];  // world ray basis: look, right*tanX, up*tanY
     float rayStep, rootVoxelHalf, pixelSize, lodScalar;
     float brickStep[3], brickDim[3], poolDim[3];
-    int32_t depthMinusOne, maxPush, rootConst, W, H, maxTraversals;
+    int32_t depthMinusOne, maxPush, W, H, maxTraversals;
+    int32_t rx0, ry0, rx1, ry1;  // pixels cast: [rx0, rx1) x [ry0, ry1)
     uint32_t ndim[3], pdim[3];
 };
 
@@ -98,9 +99,11 @@ __device__ __forceinline__ void unpack_rgba(uint32_t c, float o[4]) {
     o[3] = (float)(c >> 24) / 255.0f;
 }
 
-// texture(BrickSampler, s): trilinear, clamp to edge.
-__device__ void sample_brick(const uint32_t* __restrict__ col, const RayUniforms& U, v3 s,
-                             float r[4]) {
+// texture(BrickSampler, s): trilinear, clamp to edge.  unorm8 channel c
+// reads u8[c] = c / 255 (a table of the correctly rounded quotients); the
+// pool has at most 2^30 texels, so texel indices fit 32 bits.
+__device__ void sample_brick(const uint32_t* __restrict__ col, const RayUniforms& U,
+                             const float* __restrict__ u8, v3 s, float r[4]) {
     float c[3] = {s.x, s.y, s.z};
     int i0[3];
     float w[3];
@@ -123,12 +126,11 @@ __device__ void sample_brick(const uint32_t* __restrict__ col, const RayUniforms
         const float wy = dy ? w[1] : 1.0f - w[1];
         const float wz = dz ? w[2] : 1.0f - w[2];
         const float wk = (wx * wy) * wz;
-        float t[4];
-        unpack_rgba(col[((uint64_t)iz * U.pdim[1] + iy) * U.pdim[0] + ix], t);
-        r[0] = r[0] + wk * t[0];
-        r[1] = r[1] + wk * t[1];
-        r[2] = r[2] + wk * t[2];
-        r[3] = r[3] + wk * t[3];
+        const uint32_t c = col[((uint32_t)iz * U.pdim[1] + (uint32_t)iy) * U.pdim[0] + (uint32_t)ix];
+        r[0] = r[0] + wk * u8[c & 255u];
+        r[1] = r[1] + wk * u8[(c >> 8) & 255u];
+        r[2] = r[2] + wk * u8[(c >> 16) & 255u];
+        r[3] = r[3] + wk * u8[c >> 24];
     }
 }
 
@@ -152,9 +154,12 @@ __global__ void __launch_bounds__(256)
 k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __restrict__ pool,
           const float* __restrict__ jitter, uint32_t* __restrict__ rgbaOut,
           uint32_t* __restrict__ usageOut) {
-    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (i >= U.W || j >= U.H) return;
+    __shared__ float u8[256];
+    u8[threadIdx.x] = (float)threadIdx.x / 255.0f;  // blockDim.x == 256
+    __syncthreads();
+    const int i = U.rx0 + (int)blockIdx.x * 16 + (int)(threadIdx.x & 15);
+    const int j = U.ry0 + (int)blockIdx.y * 16 + (int)(threadIdx.x >> 4);
+    if (i >= U.rx1 || j >= U.ry1) return;
     uint32_t usage[kMaxUsage];
 #pragma unroll
     for (int k = 0; k < kMaxUsage; ++k) usage[k] = 0u;
@@ -202,7 +207,8 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
             float nodeSize = 1.0f, vh = U.rootVoxelHalf;
             v3 rayBrickPos = rayPos, curBrickRayPos = rayPos;
             uint2 curBrickPtr = root;
-            bool loaded = false, curConst = U.rootConst != 0;
+            const bool rootConst = (root.x & kConstBit) != 0u;  // RootNodeIsConstant
+            bool loaded = false, curConst = rootConst;
             int curDepth = 0, curBrickDepth = 0;
             uint32_t cx = 0, cy = 0, cz = 0;
             int nlt = 0, pushed = 0, traversals = 0;
@@ -277,7 +283,7 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
                                 break;
                             }
                             float src[4];
-                            sample_brick(pool, U, cbr, src);
+                            sample_brick(pool, U, u8, cbr, src);
                             src[3] = 1.0f - (1.0f - src[3]);  // pow(1 - a, 1.0)
                             blend(dst, src);
                             cbr = add(cbr, brickRayStep);
@@ -303,7 +309,7 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
                     curBrickRayPos = rayPos;
                     curBrickPtr = node;
                     loaded = false;
-                    curConst = U.rootConst != 0;
+                    curConst = rootConst;
                     curDepth = 0;
                     curBrickDepth = 0;
                     cx = cy = cz = 0;
@@ -347,19 +353,27 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
     u4[2] = make_uint4(usage[8], usage[9], usage[10], usage[11]);
 }
 
-// Distinct node ids of the frame: lanes of a wave holding the same id elect
-// one leader (ballot loop), leaders claim the node texel's frame stamp, the
-// first claimant appends the id.  counts[0] = appended (capped), counts[1] =
-// distinct ids.
+// Distinct node ids of the frame's pixel rectangle: lanes of a wave
+// holding the same id elect one leader (ballot loop), leaders claim the node
+// texel's frame stamp, the first claimant appends the id.  counts[0] =
+// appended (capped), counts[1] = distinct ids.  Entry e of the rectangle is
+// list slot e % 12 of its pixel e / 12 (row by row).
 __global__ void __launch_bounds__(256)
-k_requests(const uint32_t* __restrict__ usage, uint64_t n, uint32_t tag, uint32_t ndx,
-           uint32_t ndy, uint32_t ndz, uint32_t* __restrict__ stamp, uint32_t* __restrict__ list,
+k_requests(const uint32_t* __restrict__ usage, uint32_t W, int32_t rx0, int32_t ry0, uint32_t rw,
+           uint64_t n, uint32_t tag, uint32_t ndx, uint32_t ndy, uint32_t ndz,
+           uint32_t* __restrict__ stamp, uint32_t* __restrict__ list,
            unsigned long long* __restrict__ times, uint32_t cap,
            uint32_t* __restrict__ counts) {
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n;
          base += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t e = base + threadIdx.x;
-        uint32_t id = e < n ? usage[e] : 0u;
+        uint32_t id = 0u;
+        if (e < n) {
+            const uint64_t p = e / kMaxUsage;
+            const uint32_t k = (uint32_t)(e - p * kMaxUsage);
+            const uint64_t px = (uint64_t)rx0 + p % rw, py = (uint64_t)ry0 + p / rw;
+            id = usage[(py * W + px) * kMaxUsage + k];
+        }
         // wave-level deduplication
         uint64_t pending = __ballot(id != 0u);
         bool leader = false;
@@ -375,7 +389,10 @@ k_requests(const uint32_t* __restrict__ usage, uint64_t n, uint32_t tag, uint32_
             const uint32_t y = ((id >> 10) & 1023u) - 1u;
             const uint32_t z = (id & 1023u) - 1u;
             const uint64_t t = ((uint64_t)z * ndy + y) * ndx + x;
-            if (x < ndx && y < ndy && z < ndz && atomicExch(&stamp[t], tag) != tag) {
+            // a plain read first: most leaders meet a node another wave has
+            // claimed already, and skip the atomic on its (hot) stamp word
+            if (x < ndx && y < ndy && z < ndz && stamp[t] != tag &&
+                atomicExch(&stamp[t], tag) != tag) {
                 const uint32_t k = atomicAdd(&counts[1], 1u);
                 if (k < cap) {
                     list[k] = id;
@@ -387,11 +404,17 @@ k_requests(const uint32_t* __restrict__ usage, uint64_t n, uint32_t tag, uint32_
     }
 }
 
-uint64_t splitmix64(uint64_t& s) {
-    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+// Jitter table: 32 x 32 values in [0, 1), the k-th from the k+1-th output of
+// splitmix64 seeded with `seed` (the state advances by the golden-ratio
+// increment, so output k+1 is the finaliser of seed + (k+1) * increment).
+__global__ void k_jitter(uint64_t seed, float* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 1024) return;
+    uint64_t z = seed + (uint64_t)(k + 1) * 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
+    z ^= z >> 31;
+    out[k] = (float)(z >> 40) * (1.0f / 16777216.0f);
 }
 
 }  // namespace
@@ -402,9 +425,12 @@ struct vox_raycaster {
     int32_t maxW = 0, maxH = 0, W = 0, H = 0;
     uint32_t maxReq = 0, nreq = 0, nreqTotal = 0, tag = 0;
     uint32_t ndim[3] = {0, 0, 0};
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;  // the ray caster's own stream
     DevBuf rgba, usage, req, reqTime, counts, stamp, jitter;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t done = nullptr;  // the last frame's request list is complete
+    bool timingPending = false, countsPending = false, haveJitter = false;
+    uint64_t jitterSeed = 0;
     double msRay = 0, msReq = 0;
     std::stringstream err;
     std::string errCache;
@@ -421,12 +447,14 @@ struct vox_raycaster {
 
 namespace voxmi {
 
-bool raycaster_requests(vox_raycaster* rc, const uint32_t** ids, const uint64_t** times,
-                        uint32_t* n) {
+bool raycaster_requests(vox_raycaster* rc, RequestRefs& r) {
     if (!rc || !rc->valid) return false;
-    *ids = rc->req.as<uint32_t>();
-    *times = rc->reqTime.as<uint64_t>();
-    *n = rc->nreq;
+    r.ids = rc->req.as<uint32_t>();
+    r.times = rc->reqTime.as<uint64_t>();
+    r.count = rc->counts.as<uint32_t>();
+    r.maxIds = rc->maxReq;
+    r.done = rc->done;
+    r.tree = rc->tree;
     return true;
 }
 
@@ -502,6 +530,20 @@ bool ray_uniforms(const vox_camera& cam, const vox_raycast_params& prm, int leve
     U.maxPush = kMaxUsage * (int)((cam.frame % 2) + 1);
     U.W = cam.width;
     U.H = cam.height;
+    const int32_t* r = prm.rect;
+    if (r[0] == 0 && r[1] == 0 && r[2] == 0 && r[3] == 0) {
+        U.rx0 = U.ry0 = 0;
+        U.rx1 = cam.width;
+        U.ry1 = cam.height;
+    } else {
+        if (r[0] < 0 || r[1] < 0 || r[2] > cam.width || r[3] > cam.height || r[2] <= r[0] ||
+            r[3] <= r[1])
+            return false;
+        U.rx0 = r[0];
+        U.ry0 = r[1];
+        U.rx1 = r[2];
+        U.ry1 = r[3];
+    }
     U.maxTraversals = prm.max_traversals ? (int32_t)prm.max_traversals : 4096;
     return true;
 }
@@ -521,11 +563,12 @@ vox_raycaster* vox_raycaster_create(vox_octree* tree, int32_t max_width, int32_t
         return rc;
     }
     for (int a = 0; a < 3; ++a) rc->ndim[a] = o.ndim[a];
-    rc->st = o.st;
     if (max_width <= 0 || max_height <= 0 || max_requests == 0) {
         rc->fail("raycaster: empty frame or request capacity");
         return rc;
     }
+    if (!rc->check(hipStreamCreateWithFlags(&rc->st, hipStreamNonBlocking), "hipStreamCreate"))
+        return rc;
     rc->maxW = max_width;
     rc->maxH = max_height;
     rc->maxReq = max_requests;
@@ -539,9 +582,13 @@ vox_raycaster* vox_raycaster_create(vox_octree* tree, int32_t max_width, int32_t
         !rc->jitter.ensure(4 * 1024, rc->err, "jitter"))
         return rc;
     launch_fill_u32(rc->st, rc->stamp.as<uint32_t>(), ntex, 0u);
+    launch_fill_u32(rc->st, rc->counts.as<uint32_t>(), 2, 0u);
     for (auto& e : rc->ev)
         if (!rc->check(hipEventCreate(&e), "hipEventCreate")) return rc;
-    if (!rc->check(hipStreamSynchronize(rc->st), "synchronize")) return rc;
+    if (!rc->check(hipEventCreateWithFlags(&rc->done, hipEventDisableTiming), "hipEventCreate") ||
+        !rc->check(hipEventRecord(rc->done, rc->st), "event record") ||
+        !rc->check(hipStreamSynchronize(rc->st), "synchronize"))
+        return rc;
     rc->valid = true;
     return rc;
 }
@@ -551,6 +598,7 @@ void vox_raycaster_destroy(vox_raycaster* rc) {
     if (rc->st) (void)hipStreamSynchronize(rc->st);
     for (auto e : rc->ev)
         if (e) (void)hipEventDestroy(e);
+    if (rc->done) (void)hipEventDestroy(rc->done);
     rc->rgba.release();
     rc->usage.release();
     rc->req.release();
@@ -558,6 +606,7 @@ void vox_raycaster_destroy(vox_raycaster* rc) {
     rc->counts.release();
     rc->stamp.release();
     rc->jitter.release();
+    if (rc->st) (void)hipStreamDestroy(rc->st);
     delete rc;
 }
 
@@ -567,6 +616,22 @@ const char* vox_raycaster_get_error(vox_raycaster* rc) {
     return rc->errCache.c_str();
 }
 
+namespace {
+
+// Timings of the frame whose events were recorded last (waits for it).
+void settle_timing(vox_raycaster* rc) {
+    if (!rc->timingPending) return;
+    (void)hipEventSynchronize(rc->ev[2]);
+    float a = 0, b = 0;
+    (void)hipEventElapsedTime(&a, rc->ev[0], rc->ev[1]);
+    (void)hipEventElapsedTime(&b, rc->ev[1], rc->ev[2]);
+    rc->msRay = a;
+    rc->msReq = b;
+    rc->timingPending = false;
+}
+
+}  // namespace
+
 int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycast_params* prm) {
     if (!rc || !rc->valid || !cam || !prm) return 0;
     if (cam->width <= 0 || cam->height <= 0 || cam->width > rc->maxW || cam->height > rc->maxH)
@@ -575,60 +640,63 @@ int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycas
     if (!octree_refs(rc->tree, o)) return rc->fail("raycast: octree unavailable"), 0;
     const uint32_t* nd = o.ndim;
     uint32_t* tex = o.tex;
-    hipStream_t st = o.st;
+    hipStream_t st = rc->st;
     RayUniforms U;
     if (!ray_uniforms(*cam, *prm, o.levels, o.B, o.poolDims, o.ndim, U))
-        return rc->fail("raycast: empty volume box or ray step below 1e-5"), 0;
-    // root type for RootNodeIsConstant
-    uint32_t rootW0 = 0;
-    if (!rc->check(hipMemcpyAsync(&rootW0, tex, 4, hipMemcpyDeviceToHost, st), "read root") ||
-        !rc->check(hipStreamSynchronize(st), "synchronize"))
-        return 0;
-    U.rootConst = (rootW0 & kConstBit) ? 1 : 0;
-    // jitter table: 32 x 32 values in [0, 1) from the seed
-    std::vector<float> jit(1024);
-    uint64_t s = prm->jitter_seed;
-    for (int k = 0; k < 1024; ++k) jit[k] = (float)(splitmix64(s) >> 40) * (1.0f / 16777216.0f);
-    if (!rc->check(hipMemcpyAsync(rc->jitter.p, jit.data(), 4096, hipMemcpyHostToDevice, st),
-                   "jitter upload"))
-        return 0;
+        return rc->fail("raycast: empty volume box, ray step below 1e-5 or a pixel rectangle "
+                        "outside the viewport"), 0;
+    settle_timing(rc);  // the previous frame's events are reused below
+    // the node texture and brick pool as the last writer (octree build or
+    // brick server frame) left them
+    if (!rc->check(hipStreamWaitEvent(st, o.ready, 0), "wait for the node texture")) return 0;
+    if (!rc->haveJitter || rc->jitterSeed != prm->jitter_seed) {
+        k_jitter<<<4, 256, 0, st>>>(prm->jitter_seed, rc->jitter.as<float>());
+        rc->jitterSeed = prm->jitter_seed;
+        rc->haveJitter = true;
+    }
     rc->W = cam->width;
     rc->H = cam->height;
-    const uint64_t npix = (uint64_t)rc->W * rc->H;
+    const uint32_t rw = (uint32_t)(U.rx1 - U.rx0), rh = (uint32_t)(U.ry1 - U.ry0);
+    const uint64_t npix = (uint64_t)rw * rh;
     if (++rc->tag == 0) {  // stamp wrap: clear
         launch_fill_u32(st, rc->stamp.as<uint32_t>(), (uint64_t)nd[0] * nd[1] * nd[2], 0u);
         rc->tag = 1;
     }
     launch_fill_u32(st, rc->counts.as<uint32_t>(), 2, 0u);
     (void)hipEventRecord(rc->ev[0], st);
-    dim3 grid((rc->W + 15) / 16, (rc->H + 15) / 16);
+    dim3 grid((rw + 15) / 16, (rh + 15) / 16);
     k_raycast<<<grid, 256, 0, st>>>(U, tex, o.poolCol, rc->jitter.as<float>(), rc->rgba.as<uint32_t>(),
                                     rc->usage.as<uint32_t>());
     (void)hipEventRecord(rc->ev[1], st);
     const uint64_t nent = npix * kMaxUsage;
     const unsigned rgrid = (unsigned)std::min<uint64_t>((nent + 255) / 256, 8192);
-    k_requests<<<rgrid, 256, 0, st>>>(rc->usage.as<uint32_t>(), nent, rc->tag, nd[0], nd[1],
+    k_requests<<<rgrid, 256, 0, st>>>(rc->usage.as<uint32_t>(), (uint32_t)rc->W, U.rx0, U.ry0, rw,
+                                      nent, rc->tag, nd[0], nd[1],
                                       nd[2], rc->stamp.as<uint32_t>(), rc->req.as<uint32_t>(),
                                       rc->reqTime.as<unsigned long long>(), rc->maxReq,
                                       rc->counts.as<uint32_t>());
     (void)hipEventRecord(rc->ev[2], st);
-    uint32_t cnt[2] = {0, 0};
     if (!rc->check(hipGetLastError(), "raycast kernels") ||
-        !rc->check(hipMemcpyAsync(cnt, rc->counts.p, 8, hipMemcpyDeviceToHost, st), "counts") ||
-        !rc->check(hipStreamSynchronize(st), "synchronize"))
+        !rc->check(hipEventRecord(rc->done, st), "event record"))
         return 0;
-    rc->nreq = cnt[0];
-    rc->nreqTotal = cnt[1];
-    float a = 0, b = 0;
-    (void)hipEventElapsedTime(&a, rc->ev[0], rc->ev[1]);
-    (void)hipEventElapsedTime(&b, rc->ev[1], rc->ev[2]);
-    rc->msRay = a;
-    rc->msReq = b;
+    rc->timingPending = true;
+    rc->countsPending = true;
     return 1;
 }
 
 int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out) {
     if (!rc || !rc->valid || !out) return 0;
+    if (rc->countsPending) {
+        uint32_t cnt[2] = {0, 0};
+        if (!rc->check(hipMemcpyAsync(cnt, rc->counts.p, 8, hipMemcpyDeviceToHost, rc->st),
+                       "counts") ||
+            !rc->check(hipStreamSynchronize(rc->st), "synchronize"))
+            return 0;
+        rc->nreq = cnt[0];
+        rc->nreqTotal = cnt[1];
+        rc->countsPending = false;
+    }
+    settle_timing(rc);
     out->width = rc->W;
     out->height = rc->H;
     out->rgba = rc->rgba.as<uint32_t>();

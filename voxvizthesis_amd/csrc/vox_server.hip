@@ -10,6 +10,12 @@
 // pending nodes join a FIFO in ascending texel order, and victims are the
 // slots with the oldest use (empty first, ties by slot index), found with a
 // 257-bucket age histogram instead of a linked list.
+//
+// One server can serve several octrees that share its brick pool, as the
+// reference's BrickPool singleton serves every loaded tree of the pager
+// (GigaVoxelsRenderer.cpp 1520-1582): a node is then named by its global
+// texel, tree slot * texels per tree + texel, and the device table of
+// attached trees gives each tree's node arrays, node texture and bricks.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -31,10 +37,16 @@ constexpr uint32_t kResidentBit = 0x80000000u;
 constexpr int kB = 256;
 constexpr uint32_t kKeys = 257;  // 0 empty, 1..255 by age (older first), 256 used now
 
-struct DepthTabs {
+// One attached tree as the kernels see it.
+struct TreeTab {
     const uint8_t* types[kMaxLevels];
     const uint32_t* bidx[kMaxLevels];
     const uint32_t* texel[kMaxLevels];
+    const uint32_t* col[kMaxLevels];   // the voxelizer's bricks
+    const uint32_t* nrm[kMaxLevels];
+    uint32_t* tex;                     // node texture, 2 words per texel
+    uint32_t ndim[3];
+    uint32_t live;
 };
 
 unsigned grid_for(uint64_t n) {
@@ -42,8 +54,16 @@ unsigned grid_for(uint64_t n) {
     return (unsigned)(g < 65536 ? (g ? g : 1) : 65536);
 }
 
+__device__ __forceinline__ uint32_t pack10(uint32_t x, uint32_t y, uint32_t z) {
+    return (x << 20) | (y << 10) | z;
+}
+
+__global__ void k_srv_set_tab(TreeTab* __restrict__ tabs, uint32_t i, TreeTab v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) tabs[i] = v;
+}
+
 // texel -> (depth, node) of every node in the tree.
-__global__ void k_srv_info(uint64_t n, uint32_t sh, uint32_t d, const uint32_t* __restrict__ texel,
+__global__ void k_srv_info(uint64_t n, uint32_t d, const uint32_t* __restrict__ texel,
                            uint32_t* __restrict__ info) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
@@ -52,11 +72,13 @@ __global__ void k_srv_info(uint64_t n, uint32_t sh, uint32_t d, const uint32_t* 
     }
 }
 
-// Slots already holding a brick (the root's, uploaded when the octree was
-// built): recovered from the resident bit and brick pointer of the texels.
+// Slots already holding a brick (a root uploaded when the octree was built):
+// recovered from the resident bit and brick pointer of the texels.  A slot
+// the server already gave to another node is a conflict.
 __global__ void k_srv_adopt(uint64_t ntex, const uint32_t* __restrict__ tex, uint32_t ex,
                             uint32_t ey, uint32_t ez, uint32_t sx, uint32_t sy, uint32_t nslots,
-                            uint32_t* __restrict__ nodeSlot, uint32_t* __restrict__ slotNode) {
+                            uint32_t base, uint32_t* __restrict__ nodeSlot,
+                            uint32_t* __restrict__ slotNode, uint32_t* __restrict__ conflict) {
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ntex;
          t += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t w0 = tex[2 * t], w1 = tex[2 * t + 1];
@@ -65,14 +87,16 @@ __global__ void k_srv_adopt(uint64_t ntex, const uint32_t* __restrict__ tex, uin
         const uint32_t y = (((w1 >> 10) & 1023u) - 1u) / ey;
         const uint32_t z = ((w1 & 1023u) - 1u) / ez;
         const uint32_t s = (z * sy + y) * sx + x;
-        if (s < nslots) {
-            nodeSlot[t] = s;
-            slotNode[s] = (uint32_t)t;
+        if (s >= nslots) continue;
+        if (atomicCAS(&slotNode[s], kNone, base + (uint32_t)t) != kNone) {
+            atomicOr(conflict, 1u);
+            continue;
         }
+        nodeSlot[base + t] = s;
     }
 }
 
-__device__ __forceinline__ uint32_t parent_texel(uint32_t info, const DepthTabs& D) {
+__device__ __forceinline__ uint32_t parent_texel(uint32_t info, const TreeTab* T) {
     const uint32_t d = info >> 27;
     if (d == 0) return kNone;
     const uint32_t n = info & 0x07ffffffu;
@@ -80,45 +104,68 @@ __device__ __forceinline__ uint32_t parent_texel(uint32_t info, const DepthTabs&
     const uint32_t m = (1u << sh) - 1u;
     const uint32_t x = n & m, y = (n >> sh) & m, z = n >> (2 * sh);
     const uint32_t p = (((z >> 1) << (2 * (sh - 1))) | ((y >> 1) << (sh - 1)) | (x >> 1));
-    return D.texel[d - 1][p];
+    return T->texel[d - 1][p];
 }
 
-__device__ __forceinline__ bool non_const(uint32_t info, const DepthTabs& D) {
-    return D.types[info >> 27][info & 0x07ffffffu] == 1;
+__device__ __forceinline__ bool non_const(uint32_t info, const TreeTab* T) {
+    return T->types[info >> 27][info & 0x07ffffffu] == 1;
 }
 
-__device__ __forceinline__ void mark_pending(uint32_t t, unsigned long long emit,
+__device__ __forceinline__ void mark_pending(uint32_t g, unsigned long long emit,
                                              uint32_t* __restrict__ pend,
                                              unsigned long long* __restrict__ pendEmit) {
-    if (atomicCAS(&pend[t], 0u, 2u) == 0u) pendEmit[t] = emit;
+    if (atomicCAS(&pend[g], 0u, 2u) == 0u) pendEmit[g] = emit;
+}
+
+// A NON-CONST root without a brick becomes pending when its tree is
+// attached (the pager queues a loaded tree's root, GigaVoxelsDatabasePager.cpp
+// 300-310).
+__global__ void k_srv_seed_root(const TreeTab* __restrict__ tabs, uint32_t tree, uint32_t base,
+                                const uint32_t* __restrict__ nodeSlot, uint32_t* __restrict__ pend,
+                                unsigned long long* __restrict__ pendEmit) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const TreeTab* T = tabs + tree;
+    const uint32_t t = T->texel[0][0];
+    if (t == kNone || T->types[0][0] != 1) return;
+    if (nodeSlot[base + t] != kNone) return;
+    mark_pending(base + t, (unsigned long long)wall_clock64(), pend, pendEmit);
 }
 
 // 1. touch used slots, mark new pending nodes (and their parents).
+// tot: [0] uploads, [1] evictions, [2] request ids seen
 __global__ void k_srv_touch(const uint32_t* __restrict__ ids, const uint64_t* __restrict__ times,
-                            uint32_t nids, uint32_t ndx, uint32_t ndy, uint32_t ndz, uint32_t frame,
-                            DepthTabs D, const uint32_t* __restrict__ info,
-                            const uint32_t* __restrict__ nodeSlot, uint32_t* __restrict__ slotLast,
-                            uint32_t* __restrict__ pend,
-                            unsigned long long* __restrict__ pendEmit) {
+                            const uint32_t* __restrict__ count, uint32_t maxIds, uint32_t tree,
+                            uint32_t base, uint32_t frame, const TreeTab* __restrict__ tabs,
+                            const uint32_t* __restrict__ info, const uint32_t* __restrict__ nodeSlot,
+                            uint32_t* __restrict__ slotLast, uint32_t* __restrict__ pend,
+                            unsigned long long* __restrict__ pendEmit,
+                            unsigned long long* __restrict__ tot) {
+    const TreeTab* T = tabs + tree;
+    if (!T->live) return;
+    const uint32_t c = *count;
+    const uint32_t nids = c < maxIds ? c : maxIds;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&tot[2], (unsigned long long)nids);
+    const uint32_t ndx = T->ndim[0], ndy = T->ndim[1], ndz = T->ndim[2];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nids; k += gridDim.x * blockDim.x) {
         const uint32_t id = ids[k];
         const uint32_t x = ((id >> 20) & 1023u) - 1u, y = ((id >> 10) & 1023u) - 1u,
                        z = (id & 1023u) - 1u;
         if (x >= ndx || y >= ndy || z >= ndz) continue;
-        const uint32_t t = (z * ndy + y) * ndx + x;
-        const uint32_t inf = info[t];
+        const uint32_t g = base + (z * ndy + y) * ndx + x;
+        const uint32_t inf = info[g];
         if (inf == kNone) continue;  // stale id outside the tree
-        const uint32_t s = nodeSlot[t];
+        const uint32_t s = nodeSlot[g];
         if (s != kNone) {
             slotLast[s] = frame;
             continue;
         }
-        if (!non_const(inf, D)) continue;
+        if (!non_const(inf, T)) continue;
         const unsigned long long emit = (unsigned long long)times[k];
-        mark_pending(t, emit, pend, pendEmit);
-        const uint32_t p = parent_texel(inf, D);
-        if (p != kNone && nodeSlot[p] == kNone && non_const(info[p], D))
-            mark_pending(p, emit, pend, pendEmit);
+        mark_pending(g, emit, pend, pendEmit);
+        const uint32_t p = parent_texel(inf, T);
+        if (p == kNone) continue;
+        const uint32_t gp = base + p;
+        if (nodeSlot[gp] == kNone && non_const(info[gp], T)) mark_pending(gp, emit, pend, pendEmit);
     }
 }
 
@@ -130,7 +177,7 @@ __global__ void k_srv_newflags(uint64_t ntex, const uint32_t* __restrict__ pend,
 }
 
 // 2. append the new pending texels to the FIFO in ascending order.
-// ctr: [0] head, [1] tail (monotonic), [2] admitted this frame, [3] evictions
+// ctr: [0] head, [1] tail (monotonic), [2] admitted this frame
 __global__ void k_srv_enqueue(uint64_t ntex, const uint32_t* __restrict__ flag,
                               const uint32_t* __restrict__ excl, uint32_t* __restrict__ pend,
                               const unsigned long long* __restrict__ pendEmit,
@@ -140,7 +187,7 @@ __global__ void k_srv_enqueue(uint64_t ntex, const uint32_t* __restrict__ flag,
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ntex;
          t += (uint64_t)gridDim.x * blockDim.x) {
         if (!flag[t]) continue;
-        const uint32_t pos = (tail + excl[t]) % cap;
+        const uint32_t pos = (uint32_t)(((uint64_t)tail + excl[t]) % cap);
         ring[pos] = (uint32_t)t;
         ringEmit[pos] = pendEmit[t];
         pend[t] = 1u;
@@ -214,15 +261,17 @@ __global__ void k_srv_selflags(uint32_t nslots, const uint32_t* __restrict__ key
 }
 
 // 4. admission: FIFO entry i -> i-th selected slot (ascending slot index).
+// req[i] = (tree << 8 | depth, brick, slot, texel).
 __global__ void k_srv_admit(uint32_t nslots, const uint32_t* __restrict__ flag,
                             const uint32_t* __restrict__ excl, const uint32_t* __restrict__ sel,
-                            uint32_t frame, DepthTabs D, const uint32_t* __restrict__ info,
-                            uint32_t* __restrict__ ring, const unsigned long long* __restrict__ ringEmit,
-                            uint32_t cap, uint32_t* __restrict__ ctr, uint32_t* __restrict__ nodeSlot,
+                            uint32_t frame, const TreeTab* __restrict__ tabs, uint32_t cap,
+                            const uint32_t* __restrict__ info, const uint32_t* __restrict__ ring,
+                            const unsigned long long* __restrict__ ringEmit, uint32_t ringCap,
+                            const uint32_t* __restrict__ ctr, uint32_t* __restrict__ nodeSlot,
                             uint32_t* __restrict__ slotNode, uint32_t* __restrict__ slotLast,
-                            uint32_t* __restrict__ pend, uint32_t* __restrict__ tex,
-                            uint4* __restrict__ req, unsigned long long* __restrict__ reqEmit,
-                            uint32_t* __restrict__ nreq) {
+                            uint32_t* __restrict__ pend, uint4* __restrict__ req,
+                            unsigned long long* __restrict__ reqEmit, uint32_t* __restrict__ nreq,
+                            unsigned long long* __restrict__ tot) {
     const uint32_t n = sel[0];
     const uint32_t head = ctr[0];
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nslots;
@@ -230,21 +279,24 @@ __global__ void k_srv_admit(uint32_t nslots, const uint32_t* __restrict__ flag,
         if (!flag[s]) continue;
         const uint32_t i = excl[s];
         if (i >= n) continue;
-        const uint32_t pos = (head + i) % cap;
-        const uint32_t t = ring[pos];
+        const uint32_t pos = (uint32_t)(((uint64_t)head + i) % ringCap);
+        const uint32_t g = ring[pos];
         const uint32_t old = slotNode[s];
         if (old != kNone) {
             nodeSlot[old] = kNone;
-            tex[2 * (uint64_t)old] &= ~kResidentBit;
-            atomicAdd(&ctr[3], 1u);
+            const uint32_t to = old / cap;
+            const TreeTab* To = tabs + to;
+            if (To->live) To->tex[2 * (uint64_t)(old - to * cap)] &= ~kResidentBit;
+            atomicAdd(&tot[1], 1ull);
         }
-        slotNode[s] = t;
+        slotNode[s] = g;
         slotLast[s] = frame;
-        nodeSlot[t] = s;
-        pend[t] = 0u;
-        const uint32_t inf = info[t];
+        nodeSlot[g] = s;
+        pend[g] = 0u;
+        const uint32_t tr = g / cap;
+        const uint32_t inf = info[g];
         const uint32_t d = inf >> 27, node = inf & 0x07ffffffu;
-        req[i] = make_uint4(d, D.bidx[d][node], s, t);
+        req[i] = make_uint4((tr << 8) | d, tabs[tr].bidx[d][node], s, g - tr * cap);
         reqEmit[i] = ringEmit[pos];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *nreq = n;
@@ -257,14 +309,53 @@ __global__ void k_srv_advance(const uint32_t* __restrict__ sel, uint32_t* __rest
     }
 }
 
-// Latency of each admitted request: copy done - emitted.
+// 5. copy each admitted brick from its tree's voxelizer result into its
+// slot, set the node texel's resident bit and brick pointer, and record the
+// device clock once the whole brick is written.
+__global__ void k_srv_upload(const TreeTab* __restrict__ tabs, const uint4* __restrict__ req,
+                             const uint32_t* __restrict__ nreq, uint32_t ex, uint32_t ey,
+                             uint32_t ez, uint32_t sx, uint32_t sy, uint32_t dx, uint32_t dy,
+                             uint32_t* __restrict__ col, uint32_t* __restrict__ nrm,
+                             unsigned long long* __restrict__ done) {
+    const uint32_t count = *nreq;
+    const uint32_t nv = ex * ey * ez;
+    for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
+        const uint4 q = req[r];
+        const TreeTab* T = tabs + (q.x >> 8);
+        const uint32_t d = q.x & 255u;
+        const uint32_t* sc = T->col[d] + (uint64_t)q.y * nv;
+        const uint32_t* sn = T->nrm[d] + (uint64_t)q.y * nv;
+        const uint32_t ox = (q.z % sx) * ex;
+        const uint32_t oy = ((q.z / sx) % sy) * ey;
+        const uint32_t oz = (q.z / (sx * sy)) * ez;
+        for (uint32_t v = threadIdx.x; v < nv; v += blockDim.x) {
+            const uint32_t x = v % ex;
+            const uint32_t yz = v / ex;
+            const uint32_t y = yz % ey;
+            const uint32_t z = yz / ey;
+            const uint64_t t = ((uint64_t)(oz + z) * dy + (oy + y)) * dx + (ox + x);
+            col[t] = sc[v];
+            nrm[t] = sn[v];
+        }
+        if (threadIdx.x == 0) {
+            T->tex[2 * (uint64_t)q.w] |= kResidentBit;
+            T->tex[2 * (uint64_t)q.w + 1] = pack10(ox + 1, oy + 1, oz + 1);
+        }
+        __syncthreads();  // the whole brick is written
+        if (threadIdx.x == 0) done[r] = (unsigned long long)wall_clock64();
+    }
+}
+
+// Latency of each admitted request: copy done - emitted; counts the uploads.
 __global__ void k_srv_latency(const uint32_t* __restrict__ nreq,
                               const unsigned long long* __restrict__ reqEmit,
                               const unsigned long long* __restrict__ done,
                               unsigned long long* __restrict__ log, uint64_t logCap,
-                              unsigned long long* __restrict__ logCount) {
+                              const unsigned long long* __restrict__ logCount,
+                              unsigned long long* __restrict__ tot) {
     const uint32_t n = *nreq;
     const unsigned long long base = *logCount;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&tot[0], (unsigned long long)n);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const unsigned long long k = base + i;
         if (k < logCap) log[k] = done[i] - reqEmit[i];
@@ -276,17 +367,77 @@ __global__ void k_srv_logcount(const uint32_t* __restrict__ nreq,
     if (threadIdx.x == 0 && blockIdx.x == 0) *logCount += *nreq;
 }
 
+// Detach: the tree's slots become empty (taken first) and its nodes lose
+// their resident bits ...
+__global__ void k_srv_detach_slots(uint32_t nslots, uint32_t lo, uint32_t hi,
+                                   const TreeTab* __restrict__ T, uint32_t* __restrict__ slotNode,
+                                   uint32_t* __restrict__ slotLast) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nslots;
+         s += gridDim.x * blockDim.x) {
+        const uint32_t g = slotNode[s];
+        if (g != kNone && g >= lo && g < hi) {
+            T->tex[2 * (uint64_t)(g - lo)] &= ~kResidentBit;
+            slotNode[s] = kNone;
+            slotLast[s] = 0u;
+        }
+    }
+}
+
+// ... and its FIFO entries leave the queue, the others keeping their order:
+// flag[i] for the i-th entry from the head that stays ...
+__global__ void k_srv_ring_keep(const uint32_t* __restrict__ ring, uint32_t ringCap,
+                                const uint32_t* __restrict__ ctr, uint32_t lo, uint32_t hi,
+                                uint32_t* __restrict__ flag) {
+    const uint32_t head = ctr[0], len = ctr[1] - ctr[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ringCap;
+         i += gridDim.x * blockDim.x) {
+        bool keep = false;
+        if (i < len) {
+            const uint32_t g = ring[(uint32_t)(((uint64_t)head + i) % ringCap)];
+            keep = g < lo || g >= hi;
+        }
+        flag[i] = keep ? 1u : 0u;
+    }
+}
+
+// ... moved to the front of the second ring, which then becomes the queue
+// (head 0, tail = kept count).
+__global__ void k_srv_ring_move(const uint32_t* __restrict__ ring,
+                                const unsigned long long* __restrict__ ringEmit, uint32_t ringCap,
+                                const uint32_t* __restrict__ ctr, const uint32_t* __restrict__ flag,
+                                const uint32_t* __restrict__ excl, uint32_t* __restrict__ ring2,
+                                unsigned long long* __restrict__ ringEmit2) {
+    const uint32_t head = ctr[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ringCap;
+         i += gridDim.x * blockDim.x) {
+        if (!flag[i]) continue;
+        const uint32_t pos = (uint32_t)(((uint64_t)head + i) % ringCap);
+        ring2[excl[i]] = ring[pos];
+        ringEmit2[excl[i]] = ringEmit[pos];
+    }
+}
+
+__global__ void k_srv_ring_reset(const uint32_t* __restrict__ excl, uint32_t ringCap,
+                                 uint32_t* __restrict__ ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctr[0] = 0u;
+        ctr[1] = excl[ringCap];
+    }
+}
+
 }  // namespace
 
 struct vox_server {
-    vox_octree* tree = nullptr;
+    vox_brick_pool* bp = nullptr;
+    PoolRefs pool{};
     bool valid = false;
-    uint32_t maxUp = 0, nslots = 0, frame = 0;
-    uint64_t ntex = 0;
-    uint32_t ndim[3] = {0, 0, 0};
+    bool timingPending = false;
+    uint32_t maxUp = 0, nslots = 0, frame = 0, maxTrees = 0;
+    uint64_t cap = 0, ntex = 0;  // texels per tree, all trees
     hipStream_t st = nullptr;
-    DepthTabs D{};
-    DevBuf info, nodeSlot, slotNode, slotLast, pend, pendEmit, ring, ringEmit, ctr;
+    std::vector<vox_octree*> trees;
+    DevBuf tabs, info, nodeSlot, slotNode, slotLast, pend, pendEmit, ring, ringEmit, ctr, tot, conflict;
+    DevBuf ring2, ringEmit2;  // the queue after a detach's compaction
     DevBuf flagT, exclT, scanTmpT, key, hist, sel, flagS, eqExclS, exclS, scanTmpS;
     DevBuf req, reqEmit, nreq, done, log, logCount;
     uint64_t logCap = 1u << 22;
@@ -303,47 +454,118 @@ struct vox_server {
         err << what << " failed: " << hipGetErrorString(e) << std::endl;
         return false;
     }
+    int slot_of(vox_octree* t) const {
+        for (uint32_t i = 0; i < maxTrees; ++i)
+            if (trees[i] == t) return (int)i;
+        return -1;
+    }
+    // Time of the frame whose events were recorded last (waits for it).
+    void settle_timing() {
+        if (!timingPending) return;
+        (void)hipEventSynchronize(ev[1]);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        stats.ms_update_last = ms;
+        stats.ms_update_total += ms;
+        timingPending = false;
+    }
 };
 
-namespace voxmi {
-void launch_brick_upload_timed(hipStream_t st, const BrickSrc& src, const uint4* req,
-                               const uint32_t* nreq, uint32_t maxReq, const uint32_t e[3],
-                               const uint32_t slots[3], const uint32_t dims[3], uint32_t* col,
-                               uint32_t* nrm, uint32_t* tex, unsigned long long* done);
+namespace {
+
+bool attach(vox_server* s, vox_octree* tree, uint32_t* id) {
+    OctreeRefs o;
+    if (!tree || !octree_refs(tree, o)) return s->fail("server: invalid octree");
+    if (o.pool != s->bp) return s->fail("server: the octree uses another brick pool");
+    if (o.levels > 9) return s->fail("server: more than 9 octree depths do not fit the texel table");
+    const uint64_t ntex = (uint64_t)o.ndim[0] * o.ndim[1] * o.ndim[2];
+    if (ntex > s->cap) return s->fail("server: the octree has more node texels than a tree slot");
+    if (s->slot_of(tree) >= 0) return s->fail("server: the octree is attached already");
+    const int i = s->slot_of(nullptr);
+    if (i < 0) return s->fail("server: no free tree slot");
+    hipStream_t st = s->st;
+    if (!s->check(hipStreamWaitEvent(st, o.ready, 0), "wait for the octree")) return false;
+    TreeTab v{};
+    for (int d = 0; d < kMaxLevels; ++d) {
+        v.types[d] = o.types[d];
+        v.bidx[d] = o.bidx[d];
+        v.texel[d] = o.texel[d];
+        v.col[d] = o.src.col[d];
+        v.nrm[d] = o.src.nrm[d];
+    }
+    v.tex = o.tex;
+    for (int a = 0; a < 3; ++a) v.ndim[a] = o.ndim[a];
+    v.live = 1;
+    const uint32_t base = (uint32_t)(i * s->cap);
+    launch_fill_u32(st, s->conflict.as<uint32_t>(), 1, 0u);
+    k_srv_set_tab<<<1, 64, 0, st>>>(s->tabs.as<TreeTab>(), (uint32_t)i, v);
+    for (int d = 0; d < o.levels; ++d) {
+        const uint64_t n = 1ull << (3 * d);
+        k_srv_info<<<grid_for(n), kB, 0, st>>>(n, (uint32_t)d, o.texel[d],
+                                               s->info.as<uint32_t>() + base);
+    }
+    k_srv_adopt<<<grid_for(ntex), kB, 0, st>>>(
+        ntex, o.tex, o.B[0] + 2, o.B[1] + 2, o.B[2] + 2, o.poolSlots[0], o.poolSlots[1], s->nslots,
+        base, s->nodeSlot.as<uint32_t>(), s->slotNode.as<uint32_t>(), s->conflict.as<uint32_t>());
+    k_srv_seed_root<<<1, 64, 0, st>>>(s->tabs.as<TreeTab>(), (uint32_t)i, base,
+                                      s->nodeSlot.as<uint32_t>(), s->pend.as<uint32_t>(),
+                                      s->pendEmit.as<unsigned long long>());
+    uint32_t conflict = 0;
+    if (!s->check(hipGetLastError(), "attach kernels") ||
+        !s->check(hipMemcpyAsync(&conflict, s->conflict.p, 4, hipMemcpyDeviceToHost, st), "flag") ||
+        !s->check(hipStreamSynchronize(st), "synchronize"))
+        return false;
+    s->trees[i] = tree;
+    if (conflict) {
+        vox_server_detach(s, tree);
+        return s->fail("server: a brick of the octree sits in a slot the server has given to "
+                       "another node; build octrees for a shared pool with "
+                       "VOX_TREE_NO_ROOT_BRICK");
+    }
+    if (id) *id = (uint32_t)i;
+    return true;
 }
+
+}  // namespace
 
 extern "C" {
 
-vox_server* vox_server_create(vox_octree* tree, uint32_t max_uploads_per_frame) {
+vox_server* vox_server_create_shared(vox_brick_pool* bp, uint32_t max_trees,
+                                     uint64_t max_texels_per_tree, uint32_t max_uploads_per_frame) {
     vox_server* s = new (std::nothrow) vox_server;
     if (!s) return nullptr;
-    s->tree = tree;
-    OctreeRefs o;
-    if (!tree || !octree_refs(tree, o)) {
-        s->fail("server: invalid octree");
+    s->bp = bp;
+    if (!bp || !pool_refs(bp, s->pool)) {
+        s->fail("server: invalid brick pool");
         return s;
     }
-    if (o.levels > 9) {
-        s->fail("server: more than 9 octree depths do not fit the texel table");
+    if (max_trees == 0 || max_texels_per_tree == 0 ||
+        (uint64_t)max_trees * max_texels_per_tree >= 0xffffffffull) {
+        s->fail("server: tree slots times texels per tree must be between 1 and 2^32 - 2");
         return s;
     }
-    s->st = o.st;
-    for (int a = 0; a < 3; ++a) s->ndim[a] = o.ndim[a];
-    s->ntex = (uint64_t)o.ndim[0] * o.ndim[1] * o.ndim[2];
-    s->nslots = o.poolSlots[0] * o.poolSlots[1] * o.poolSlots[2];
+    if (max_trees > (1u << 23)) {
+        s->fail("server: too many tree slots");
+        return s;
+    }
+    s->maxTrees = max_trees;
+    s->cap = max_texels_per_tree;
+    s->ntex = (uint64_t)max_trees * max_texels_per_tree;
+    s->trees.assign(max_trees, nullptr);
+    s->nslots = s->pool.slots[0] * s->pool.slots[1] * s->pool.slots[2];
     s->maxUp = max_uploads_per_frame ? max_uploads_per_frame : s->nslots;
     if (s->maxUp > s->nslots) s->maxUp = s->nslots;
-    for (int d = 0; d < kMaxLevels; ++d) {
-        s->D.types[d] = o.types[d];
-        s->D.bidx[d] = o.bidx[d];
-        s->D.texel[d] = o.texel[d];
-    }
+    if (!s->check(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking), "hipStreamCreate"))
+        return s;
     const uint64_t nt = s->ntex, ns = s->nslots;
-    if (!s->info.ensure(4 * nt, s->err, "info") || !s->nodeSlot.ensure(4 * nt, s->err, "nodeSlot") ||
+    if (!s->tabs.ensure(sizeof(TreeTab) * max_trees, s->err, "tree table") ||
+        !s->info.ensure(4 * nt, s->err, "info") || !s->nodeSlot.ensure(4 * nt, s->err, "nodeSlot") ||
         !s->slotNode.ensure(4 * ns, s->err, "slotNode") ||
         !s->slotLast.ensure(4 * ns, s->err, "slotLast") || !s->pend.ensure(4 * nt, s->err, "pend") ||
         !s->pendEmit.ensure(8 * nt, s->err, "pendEmit") || !s->ring.ensure(4 * nt, s->err, "ring") ||
         !s->ringEmit.ensure(8 * nt, s->err, "ringEmit") || !s->ctr.ensure(16, s->err, "ctr") ||
+        !s->ring2.ensure(4 * nt, s->err, "ring") || !s->ringEmit2.ensure(8 * nt, s->err, "ringEmit") ||
+        !s->tot.ensure(32, s->err, "totals") || !s->conflict.ensure(4, s->err, "conflict") ||
         !s->flagT.ensure(4 * nt, s->err, "flagT") || !s->exclT.ensure(4 * (nt + 1), s->err, "exclT") ||
         !s->scanTmpT.ensure(4 * scan_tmp_words(nt), s->err, "scanTmpT") ||
         !s->key.ensure(4 * ns, s->err, "key") || !s->hist.ensure(4 * kKeys, s->err, "hist") ||
@@ -358,21 +580,17 @@ vox_server* vox_server_create(vox_octree* tree, uint32_t max_uploads_per_frame) 
         !s->logCount.ensure(8, s->err, "latency count"))
         return s;
     hipStream_t st = s->st;
+    launch_fill_u32(st, s->tabs.as<uint32_t>(), sizeof(TreeTab) / 4 * max_trees, 0u);
     launch_fill_u32(st, s->info.as<uint32_t>(), nt, kNone);
-    for (int d = 0; d < o.levels; ++d) {
-        const uint64_t n = 1ull << (3 * d);
-        k_srv_info<<<grid_for(n), kB, 0, st>>>(n, (uint32_t)d, (uint32_t)d, o.texel[d],
-                                               s->info.as<uint32_t>());
-    }
     launch_fill_u32(st, s->nodeSlot.as<uint32_t>(), nt, kNone);
     launch_fill_u32(st, s->slotNode.as<uint32_t>(), ns, kNone);
     launch_fill_u32(st, s->slotLast.as<uint32_t>(), ns, 0u);
     launch_fill_u32(st, s->pend.as<uint32_t>(), nt, 0u);
     launch_fill_u32(st, s->ctr.as<uint32_t>(), 4, 0u);
+    launch_fill_u32(st, s->tot.as<uint32_t>(), 8, 0u);
     launch_fill_u32(st, s->logCount.as<uint32_t>(), 2, 0u);
-    k_srv_adopt<<<grid_for(nt), kB, 0, st>>>(nt, o.tex, o.B[0] + 2, o.B[1] + 2, o.B[2] + 2,
-                                             o.poolSlots[0], o.poolSlots[1], s->nslots,
-                                             s->nodeSlot.as<uint32_t>(), s->slotNode.as<uint32_t>());
+    // the pool's own stream may still be clearing it
+    (void)hipStreamSynchronize(s->pool.st);
     for (auto& e : s->ev)
         if (!s->check(hipEventCreate(&e), "hipEventCreate")) return s;
     int khz = 0;
@@ -389,17 +607,32 @@ vox_server* vox_server_create(vox_octree* tree, uint32_t max_uploads_per_frame) 
     return s;
 }
 
+vox_server* vox_server_create(vox_octree* tree, uint32_t max_uploads_per_frame) {
+    OctreeRefs o;
+    if (!tree || !octree_refs(tree, o)) {
+        vox_server* s = new (std::nothrow) vox_server;
+        if (s) s->fail("server: invalid octree");
+        return s;
+    }
+    const uint64_t ntex = (uint64_t)o.ndim[0] * o.ndim[1] * o.ndim[2];
+    vox_server* s = vox_server_create_shared(o.pool, 1, ntex, max_uploads_per_frame);
+    if (s && s->valid && !attach(s, tree, nullptr)) s->valid = false;
+    return s;
+}
+
 void vox_server_destroy(vox_server* s) {
     if (!s) return;
     if (s->st) (void)hipStreamSynchronize(s->st);
     for (auto e : s->ev)
         if (e) (void)hipEventDestroy(e);
-    DevBuf* bufs[] = {&s->info, &s->nodeSlot, &s->slotNode, &s->slotLast, &s->pend, &s->pendEmit,
-                      &s->ring, &s->ringEmit, &s->ctr, &s->flagT, &s->exclT, &s->scanTmpT,
-                      &s->key, &s->hist, &s->sel, &s->flagS, &s->eqExclS, &s->exclS,
-                      &s->scanTmpS, &s->req, &s->reqEmit, &s->nreq, &s->done, &s->log,
-                      &s->logCount};
+    DevBuf* bufs[] = {&s->tabs, &s->info, &s->nodeSlot, &s->slotNode, &s->slotLast, &s->pend,
+                      &s->pendEmit, &s->ring, &s->ringEmit, &s->ctr, &s->tot, &s->conflict,
+                      &s->ring2, &s->ringEmit2,
+                      &s->flagT, &s->exclT, &s->scanTmpT, &s->key, &s->hist, &s->sel, &s->flagS,
+                      &s->eqExclS, &s->exclS, &s->scanTmpS, &s->req, &s->reqEmit, &s->nreq,
+                      &s->done, &s->log, &s->logCount};
     for (DevBuf* b : bufs) b->release();
+    if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
 
@@ -409,24 +642,70 @@ const char* vox_server_get_error(vox_server* s) {
     return s->errCache.c_str();
 }
 
-int vox_server_frame(vox_server* s, vox_raycaster* rc) {
+int vox_server_attach(vox_server* s, vox_octree* tree, uint32_t* id) {
     if (!s || !s->valid) return 0;
-    const uint32_t* ids = nullptr;
-    const uint64_t* times = nullptr;
-    uint32_t nids = 0;
-    if (!raycaster_requests(rc, &ids, &times, &nids))
-        return s->fail("server: invalid ray caster"), 0;
-    OctreeRefs o;
-    if (!octree_refs(s->tree, o)) return s->fail("server: octree unavailable"), 0;
+    return attach(s, tree, id) ? 1 : 0;
+}
+
+int vox_server_detach(vox_server* s, vox_octree* tree) {
+    if (!s || !s->valid) return 0;
+    const int i = s->slot_of(tree);
+    if (!tree || i < 0) return s->fail("server: the octree is not attached"), 0;
     hipStream_t st = s->st;
+    const uint32_t lo = (uint32_t)(i * s->cap), hi = (uint32_t)(lo + s->cap);
+    k_srv_detach_slots<<<grid_for(s->nslots), kB, 0, st>>>(s->nslots, lo, hi,
+                                                           s->tabs.as<TreeTab>() + i,
+                                                           s->slotNode.as<uint32_t>(),
+                                                           s->slotLast.as<uint32_t>());
+    const uint32_t rc = (uint32_t)s->ntex;
+    k_srv_ring_keep<<<grid_for(rc), kB, 0, st>>>(s->ring.as<uint32_t>(), rc, s->ctr.as<uint32_t>(),
+                                                 lo, hi, s->flagT.as<uint32_t>());
+    launch_scan(st, s->flagT.as<uint32_t>(), s->exclT.as<uint32_t>(), rc, s->scanTmpT.as<uint32_t>());
+    k_srv_ring_move<<<grid_for(rc), kB, 0, st>>>(
+        s->ring.as<uint32_t>(), s->ringEmit.as<unsigned long long>(), rc, s->ctr.as<uint32_t>(),
+        s->flagT.as<uint32_t>(), s->exclT.as<uint32_t>(), s->ring2.as<uint32_t>(),
+        s->ringEmit2.as<unsigned long long>());
+    k_srv_ring_reset<<<1, 64, 0, st>>>(s->exclT.as<uint32_t>(), rc, s->ctr.as<uint32_t>());
+    std::swap(s->ring, s->ring2);
+    std::swap(s->ringEmit, s->ringEmit2);
+    launch_fill_u32(st, s->nodeSlot.as<uint32_t>() + lo, s->cap, kNone);
+    launch_fill_u32(st, s->pend.as<uint32_t>() + lo, s->cap, 0u);
+    launch_fill_u32(st, s->info.as<uint32_t>() + lo, s->cap, kNone);
+    k_srv_set_tab<<<1, 64, 0, st>>>(s->tabs.as<TreeTab>(), (uint32_t)i, TreeTab{});
+    if (!octree_record_written(tree, st)) return s->fail("server: event record failed"), 0;
+    s->trees[i] = nullptr;
+    return s->check(hipGetLastError(), "detach kernels") &&
+                   s->check(hipStreamSynchronize(st), "synchronize")
+               ? 1
+               : 0;
+}
+
+int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n) {
+    if (!s || !s->valid) return 0;
+    if (n && !rcs) return s->fail("server: no ray casters"), 0;
+    std::vector<RequestRefs> lists(n);
+    std::vector<uint32_t> slot(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!raycaster_requests(rcs[k], lists[k])) return s->fail("server: invalid ray caster"), 0;
+        const int i = s->slot_of(lists[k].tree);
+        if (i < 0) return s->fail("server: a ray caster's octree is not attached"), 0;
+        slot[k] = (uint32_t)i;
+    }
+    hipStream_t st = s->st;
+    s->settle_timing();  // the events are reused below
     const uint32_t frame = ++s->frame;
     const uint64_t nt = s->ntex, ns = s->nslots;
+    TreeTab* tabs = s->tabs.as<TreeTab>();
+    for (uint32_t k = 0; k < n; ++k)
+        if (!s->check(hipStreamWaitEvent(st, lists[k].done, 0), "wait for a request list"))
+            return 0;
     (void)hipEventRecord(s->ev[0], st);
-    if (nids)
-        k_srv_touch<<<grid_for(nids), kB, 0, st>>>(
-            ids, times, nids, s->ndim[0], s->ndim[1], s->ndim[2], frame, s->D,
-            s->info.as<uint32_t>(), s->nodeSlot.as<uint32_t>(), s->slotLast.as<uint32_t>(),
-            s->pend.as<uint32_t>(), s->pendEmit.as<unsigned long long>());
+    for (uint32_t k = 0; k < n; ++k)
+        k_srv_touch<<<grid_for(lists[k].maxIds), kB, 0, st>>>(
+            lists[k].ids, lists[k].times, lists[k].count, lists[k].maxIds, slot[k],
+            (uint32_t)(slot[k] * s->cap), frame, tabs, s->info.as<uint32_t>(),
+            s->nodeSlot.as<uint32_t>(), s->slotLast.as<uint32_t>(), s->pend.as<uint32_t>(),
+            s->pendEmit.as<unsigned long long>(), s->tot.as<unsigned long long>());
     k_srv_newflags<<<grid_for(nt), kB, 0, st>>>(nt, s->pend.as<uint32_t>(), s->flagT.as<uint32_t>());
     launch_scan(st, s->flagT.as<uint32_t>(), s->exclT.as<uint32_t>(), nt, s->scanTmpT.as<uint32_t>());
     k_srv_enqueue<<<grid_for(nt), kB, 0, st>>>(
@@ -451,62 +730,71 @@ int vox_server_frame(vox_server* s, vox_raycaster* rc) {
                 s->scanTmpS.as<uint32_t>());
     k_srv_admit<<<grid_for(ns), kB, 0, st>>>(
         (uint32_t)ns, s->flagS.as<uint32_t>(), s->exclS.as<uint32_t>(), s->sel.as<uint32_t>(),
-        frame, s->D, s->info.as<uint32_t>(), s->ring.as<uint32_t>(),
+        frame, tabs, (uint32_t)s->cap, s->info.as<uint32_t>(), s->ring.as<uint32_t>(),
         s->ringEmit.as<unsigned long long>(), (uint32_t)nt, s->ctr.as<uint32_t>(),
         s->nodeSlot.as<uint32_t>(), s->slotNode.as<uint32_t>(), s->slotLast.as<uint32_t>(),
-        s->pend.as<uint32_t>(), o.tex, s->req.as<uint4>(), s->reqEmit.as<unsigned long long>(),
-        s->nreq.as<uint32_t>());
+        s->pend.as<uint32_t>(), s->req.as<uint4>(), s->reqEmit.as<unsigned long long>(),
+        s->nreq.as<uint32_t>(), s->tot.as<unsigned long long>());
     k_srv_advance<<<1, 64, 0, st>>>(s->sel.as<uint32_t>(), s->ctr.as<uint32_t>());
-    const uint32_t e[3] = {o.B[0] + 2, o.B[1] + 2, o.B[2] + 2};
-    launch_brick_upload_timed(st, o.src, s->req.as<uint4>(), s->nreq.as<uint32_t>(), s->maxUp, e,
-                              o.poolSlots, o.poolDims, o.poolCol, o.poolNrm, o.tex,
-                              s->done.as<unsigned long long>());
+    const PoolRefs& P = s->pool;
+    const unsigned ugrid = s->maxUp < 4096u ? s->maxUp : 4096u;
+    k_srv_upload<<<ugrid, kB, 0, st>>>(tabs, s->req.as<uint4>(), s->nreq.as<uint32_t>(),
+                                       P.B[0] + 2, P.B[1] + 2, P.B[2] + 2, P.slots[0], P.slots[1],
+                                       P.dims[0], P.dims[1], P.col, P.nrm,
+                                       s->done.as<unsigned long long>());
     k_srv_latency<<<grid_for(s->maxUp), kB, 0, st>>>(
         s->nreq.as<uint32_t>(), s->reqEmit.as<unsigned long long>(),
         s->done.as<unsigned long long>(), s->log.as<unsigned long long>(), s->logCap,
-        s->logCount.as<unsigned long long>());
+        s->logCount.as<unsigned long long>(), s->tot.as<unsigned long long>());
     k_srv_logcount<<<1, 64, 0, st>>>(s->nreq.as<uint32_t>(), s->logCount.as<unsigned long long>());
     (void)hipEventRecord(s->ev[1], st);
-    uint32_t ctr[4] = {0, 0, 0, 0};
-    if (!s->check(hipGetLastError(), "server kernels") ||
-        !s->check(hipMemcpyAsync(ctr, s->ctr.p, 16, hipMemcpyDeviceToHost, st), "counters") ||
-        !s->check(hipStreamSynchronize(st), "synchronize"))
-        return 0;
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, s->ev[0], s->ev[1]);
+    if (!s->check(hipGetLastError(), "server kernels")) return 0;
+    // ray casts of the next frame wait for the node textures and pool
+    for (vox_octree* t : s->trees)
+        if (t && !octree_record_written(t, st)) return s->fail("server: event record failed"), 0;
+    s->timingPending = true;
     s->stats.frames = frame;
-    s->stats.requests += nids;
-    s->stats.uploads += ctr[2];
-    s->stats.evictions = ctr[3];
-    s->stats.pending = ctr[1] - ctr[0];
-    s->stats.last_uploads = ctr[2];
-    s->stats.ms_update_last = ms;
-    s->stats.ms_update_total += ms;
     return 1;
+}
+
+int vox_server_frame(vox_server* s, vox_raycaster* rc) {
+    return vox_server_frame_multi(s, &rc, 1);
 }
 
 int vox_server_get_stats(vox_server* s, vox_server_stats* out) {
     if (!s || !s->valid || !out) return 0;
-    *out = s->stats;
-    // resident slots
+    uint32_t ctr[4] = {0, 0, 0, 0};
+    unsigned long long tot[4] = {0, 0, 0, 0};
     std::vector<uint32_t> sn(s->nslots);
-    if (hipMemcpy(sn.data(), s->slotNode.p, 4ull * s->nslots, hipMemcpyDeviceToHost) != hipSuccess)
+    if (!s->check(hipMemcpyAsync(ctr, s->ctr.p, 16, hipMemcpyDeviceToHost, s->st), "counters") ||
+        !s->check(hipMemcpyAsync(tot, s->tot.p, 32, hipMemcpyDeviceToHost, s->st), "totals") ||
+        !s->check(hipMemcpyAsync(sn.data(), s->slotNode.p, 4ull * s->nslots, hipMemcpyDeviceToHost,
+                                 s->st), "slots") ||
+        !s->check(hipStreamSynchronize(s->st), "synchronize"))
         return 0;
+    s->settle_timing();
+    s->stats.uploads = tot[0];
+    s->stats.evictions = tot[1];
+    s->stats.requests = tot[2];
+    s->stats.pending = ctr[1] - ctr[0];
+    s->stats.last_uploads = ctr[2];
     uint32_t r = 0;
     for (uint32_t v : sn) r += v != kNone;
-    out->resident = r;
+    s->stats.resident = r;
+    *out = s->stats;
     return 1;
 }
 
 int vox_server_latencies(vox_server* s, uint64_t* out, uint64_t cap, uint64_t* n) {
     if (!s || !s->valid || !n) return 0;
     unsigned long long cnt = 0;
-    if (hipMemcpy(&cnt, s->logCount.p, 8, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    if (!s->check(hipMemcpyAsync(&cnt, s->logCount.p, 8, hipMemcpyDeviceToHost, s->st), "count") ||
+        !s->check(hipStreamSynchronize(s->st), "synchronize"))
+        return 0;
     *n = cnt;
     uint64_t m = cnt < s->logCap ? cnt : s->logCap;
     if (m > cap) m = cap;
-    if (m && out &&
-        hipMemcpy(out, s->log.p, 8 * m, hipMemcpyDeviceToHost) != hipSuccess)
+    if (m && out && !s->check(hipMemcpy(out, s->log.p, 8 * m, hipMemcpyDeviceToHost), "log"))
         return 0;
     return 1;
 }
@@ -514,6 +802,7 @@ int vox_server_latencies(vox_server* s, uint64_t* out, uint64_t cap, uint64_t* n
 int vox_server_state(vox_server* s, const uint32_t** node_slot, const uint32_t** slot_node,
                      uint64_t* ntexels, uint32_t* nslots) {
     if (!s || !s->valid) return 0;
+    if (!s->check(hipStreamSynchronize(s->st), "synchronize")) return 0;
     if (node_slot) *node_slot = s->nodeSlot.as<uint32_t>();
     if (slot_node) *slot_node = s->slotNode.as<uint32_t>();
     if (ntexels) *ntexels = s->ntex;

@@ -6,10 +6,13 @@
 // reach, where a vertex within the voxel border of a cell edge also reaches
 // the neighbouring cell (computeMinMaxGridCell, main.cu 4078-4107) and a
 // vertex left of / below the grid origin reaches cell 0.  Here one thread per
-// triangle computes that range, the per-tile counts and placements use
-// wave-aggregated atomics over a prefix sum, and each tile also gets the z
-// extent of its triangles' vertices (GridTree::add, main.cu 3896-3915).
+// triangle computes that range; per-tile counts and z extents (GridTree::add,
+// main.cu 3896-3915) use wave-aggregated atomics; every triangle then writes
+// its (tile, triangle) pairs at its own offset, so the pairs lie in triangle
+// order, and a stable radix sort by tile leaves each tile's triangles in
+// input order, the order the reference appends them in.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <cstdint>
 #include <cstring>
@@ -84,24 +87,19 @@ __device__ __forceinline__ int32_t ord(float f) {
     return i >= 0 ? i : (int32_t)(0x80000000u - (uint32_t)i) - 1;
 }
 
-// pass 0: counts and z extents; pass 1: placement.
-template <int PASS>
-__global__ void k_tile_assign(const float* __restrict__ verts, uint64_t ntri, Grid g,
-                              uint32_t* __restrict__ cnt, const uint32_t* __restrict__ start,
-                              uint32_t* __restrict__ tris, int32_t* __restrict__ zmin,
-                              int32_t* __restrict__ zmax) {
+// Counts per tile and per triangle, and the tiles' z extents.
+__global__ void k_tile_count(const float* __restrict__ verts, uint64_t ntri, Grid g,
+                             uint32_t* __restrict__ cnt, uint32_t* __restrict__ triCnt,
+                             int32_t* __restrict__ zmin, int32_t* __restrict__ zmax) {
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (t >= ntri) return;
     const float* v = verts + 9 * t;
     uint32_t x0, x1, y0, y1;
     tri_range(v, g, x0, x1, y0, y1);
-    int32_t zlo = 0, zhi = 0;
-    if (PASS == 0) {
-        zlo = ord(fminf(fminf(v[2], v[5]), v[8]));
-        zhi = ord(fmaxf(fmaxf(v[2], v[5]), v[8]));
-    }
+    triCnt[t] = (x1 - x0 + 1u) * (y1 - y0 + 1u);
+    const int32_t zlo = ord(fminf(fminf(v[2], v[5]), v[8]));
+    const int32_t zhi = ord(fmaxf(fmaxf(v[2], v[5]), v[8]));
     const int lane = (int)(threadIdx.x & 63u);
-    const uint64_t below = (1ull << lane) - 1ull;
     uint32_t x = x0, y = y0;
     bool act = true;
     while (__ballot(act)) {
@@ -111,18 +109,10 @@ __global__ void k_tile_assign(const float* __restrict__ verts, uint64_t ntri, Gr
             const int l = __ffsll((unsigned long long)pend) - 1;
             const uint64_t lk = shfl64(key, l);
             const uint64_t same = __ballot(act && key == lk);
-            if (PASS == 0) {
-                if (lane == l) atomicAdd(&cnt[lk], (uint32_t)__popcll(same));
-                if (act && key == lk) {
-                    atomicMin(&zmin[lk], zlo);
-                    atomicMax(&zmax[lk], zhi);
-                }
-            } else {
-                uint32_t base = 0;
-                if (lane == l) base = atomicAdd(&cnt[lk], (uint32_t)__popcll(same));
-                base = __shfl(base, l);
-                if (act && key == lk)
-                    tris[start[lk] + base + (uint32_t)__popcll(same & below)] = (uint32_t)t;
+            if (lane == l) atomicAdd(&cnt[lk], (uint32_t)__popcll(same));
+            if (act && key == lk) {
+                atomicMin(&zmin[lk], zlo);
+                atomicMax(&zmax[lk], zhi);
             }
             pend &= ~same;
         }
@@ -133,6 +123,24 @@ __global__ void k_tile_assign(const float* __restrict__ verts, uint64_t ntri, Gr
             }
         }
     }
+}
+
+// The (tile, triangle) pairs of triangle t at its offset: pairs in
+// triangle order, tiles row by row within a triangle.
+__global__ void k_tile_pairs(const float* __restrict__ verts, uint64_t ntri, Grid g,
+                             const uint32_t* __restrict__ triOff, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    uint32_t x0, x1, y0, y1;
+    tri_range(verts + 9 * t, g, x0, x1, y0, y1);
+    uint32_t o = triOff[t];
+    for (uint32_t y = y0; y <= y1; ++y)
+        for (uint32_t x = x0; x <= x1; ++x) {
+            keys[o] = y * g.dim[0] + x;
+            vals[o] = (uint32_t)t;
+            ++o;
+        }
 }
 
 __global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t v) {
@@ -150,6 +158,26 @@ __global__ void k_unord(const int32_t* __restrict__ in, uint64_t n, float* __res
     }
 }
 
+// For every tile and group boundary k: the position in the tile's ordered
+// list of its first triangle with index >= first[k].
+__global__ void k_group_bounds(const uint32_t* __restrict__ start, const uint32_t* __restrict__ tris,
+                               uint64_t ntiles, const uint32_t* __restrict__ first, uint32_t nb,
+                               uint32_t* __restrict__ out) {
+    const uint64_t total = ntiles * nb;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t tile = e / nb;
+        const uint32_t v = first[e - tile * nb];
+        uint32_t lo = start[tile], hi = start[tile + 1];
+        while (lo < hi) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (tris[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        out[e] = lo;
+    }
+}
+
 unsigned grid_for(uint64_t n) {
     const uint64_t g = (n + kB - 1) / kB;
     return (unsigned)(g < 65536 ? (g ? g : 1) : 65536);
@@ -162,6 +190,7 @@ struct vox_tiler {
     hipStream_t st = nullptr;
     uint64_t ntiles = 0, pairs = 0;
     DevBuf cnt, start, tris, zmin, zmax, zminf, zmaxf, scanTmp, hostVerts;
+    DevBuf triCnt, triOff, triScan, keys, keys2, vals, sortTmp;
     std::stringstream err;
     std::string errCache;
     bool fail(const std::string& m) {
@@ -190,7 +219,8 @@ void vox_tiler_destroy(vox_tiler* t) {
     if (!t) return;
     if (t->st) (void)hipStreamSynchronize(t->st);
     DevBuf* b[] = {&t->cnt, &t->start, &t->tris, &t->zmin, &t->zmax, &t->zminf, &t->zmaxf,
-                   &t->scanTmp, &t->hostVerts};
+                   &t->scanTmp, &t->hostVerts, &t->triCnt, &t->triOff, &t->triScan, &t->keys,
+                   &t->keys2, &t->vals, &t->sortTmp};
     for (DevBuf* x : b) x->release();
     if (t->st) (void)hipStreamDestroy(t->st);
     delete t;
@@ -223,21 +253,29 @@ int vox_tiler_assign(vox_tiler* t, const float* verts, uint64_t ntri, int32_t de
             return t->fail("tiler: bad grid"), 0;
     }
     const uint64_t n = (uint64_t)g.dim[0] * g.dim[1];
+    if (n >= 0xffffffffull) return t->fail("tiler: too many tiles"), 0;
     t->ntiles = n;
     hipStream_t st = t->st;
+    const uint64_t nt = ntri ? ntri : 1;
     if (!t->cnt.ensure(4 * (n + 1), t->err, "tile counts") ||
         !t->start.ensure(4 * (n + 1), t->err, "tile offsets") ||
         !t->zmin.ensure(4 * n, t->err, "z min") || !t->zmax.ensure(4 * n, t->err, "z max") ||
         !t->zminf.ensure(4 * n, t->err, "z min") || !t->zmaxf.ensure(4 * n, t->err, "z max") ||
-        !t->scanTmp.ensure(4 * scan_tmp_words(n), t->err, "scan"))
+        !t->scanTmp.ensure(4 * scan_tmp_words(n), t->err, "scan") ||
+        !t->triCnt.ensure(4 * (nt + 1), t->err, "triangle counts") ||
+        !t->triOff.ensure(4 * (nt + 1), t->err, "triangle offsets") ||
+        !t->triScan.ensure(4 * scan_tmp_words(nt), t->err, "scan"))
         return 0;
     launch_fill_u32(st, t->cnt.as<uint32_t>(), n, 0u);
     k_fill_i32<<<grid_for(n), kB, 0, st>>>(t->zmin.as<int32_t>(), n, 0x7fffffff);
     k_fill_i32<<<grid_for(n), kB, 0, st>>>(t->zmax.as<int32_t>(), n, (int32_t)0x80000000);
-    if (ntri)
-        k_tile_assign<0><<<(unsigned)((ntri + kB - 1) / kB), kB, 0, st>>>(
-            verts, ntri, g, t->cnt.as<uint32_t>(), nullptr, nullptr, t->zmin.as<int32_t>(),
-            t->zmax.as<int32_t>());
+    if (ntri) {
+        k_tile_count<<<(unsigned)((ntri + kB - 1) / kB), kB, 0, st>>>(
+            verts, ntri, g, t->cnt.as<uint32_t>(), t->triCnt.as<uint32_t>(),
+            t->zmin.as<int32_t>(), t->zmax.as<int32_t>());
+        launch_scan(st, t->triCnt.as<uint32_t>(), t->triOff.as<uint32_t>(), ntri,
+                    t->triScan.as<uint32_t>());
+    }
     launch_scan(st, t->cnt.as<uint32_t>(), t->start.as<uint32_t>(), n, t->scanTmp.as<uint32_t>());
     uint32_t total = 0;
     if (!t->check(hipMemcpyAsync(&total, t->start.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, st),
@@ -246,15 +284,57 @@ int vox_tiler_assign(vox_tiler* t, const float* verts, uint64_t ntri, int32_t de
         return 0;
     t->pairs = total;
     if (!t->tris.ensure(4ull * total + 4, t->err, "tile triangles")) return 0;
-    launch_fill_u32(st, t->cnt.as<uint32_t>(), n, 0u);
-    if (ntri)
-        k_tile_assign<1><<<(unsigned)((ntri + kB - 1) / kB), kB, 0, st>>>(
-            verts, ntri, g, t->cnt.as<uint32_t>(), t->start.as<uint32_t>(), t->tris.as<uint32_t>(),
-            nullptr, nullptr);
+    if (total) {
+        if (!t->keys.ensure(4ull * total, t->err, "pair keys") ||
+            !t->keys2.ensure(4ull * total, t->err, "pair keys") ||
+            !t->vals.ensure(4ull * total, t->err, "pair triangles"))
+            return 0;
+        k_tile_pairs<<<(unsigned)((ntri + kB - 1) / kB), kB, 0, st>>>(
+            verts, ntri, g, t->triOff.as<uint32_t>(), t->keys.as<uint32_t>(),
+            t->vals.as<uint32_t>());
+        unsigned bits = 1;
+        while (bits < 32 && (1ull << bits) < n) ++bits;
+        size_t tmpBytes = 0;
+        if (!t->check(rocprim::radix_sort_pairs(nullptr, tmpBytes, t->keys.as<uint32_t>(),
+                                                t->keys2.as<uint32_t>(), t->vals.as<uint32_t>(),
+                                                t->tris.as<uint32_t>(), (size_t)total, 0u, bits,
+                                                st),
+                      "sort size") ||
+            !t->sortTmp.ensure(tmpBytes, t->err, "sort storage") ||
+            !t->check(rocprim::radix_sort_pairs(t->sortTmp.p, tmpBytes, t->keys.as<uint32_t>(),
+                                                t->keys2.as<uint32_t>(), t->vals.as<uint32_t>(),
+                                                t->tris.as<uint32_t>(), (size_t)total, 0u, bits,
+                                                st),
+                      "sort"))
+            return 0;
+    }
     k_unord<<<grid_for(n), kB, 0, st>>>(t->zmin.as<int32_t>(), n, t->zminf.as<float>());
     k_unord<<<grid_for(n), kB, 0, st>>>(t->zmax.as<int32_t>(), n, t->zmaxf.as<float>());
     return t->check(hipGetLastError(), "tiler kernels") &&
            t->check(hipStreamSynchronize(st), "synchronize");
+}
+
+int vox_tiler_group_bounds(vox_tiler* t, const uint32_t* first, uint32_t nbounds, uint32_t* out) {
+    if (!t || !t->valid || !first || !out || nbounds == 0) return 0;
+    if (t->ntiles == 0) return t->fail("tiler: assign first"), 0;
+    const uint64_t total = t->ntiles * nbounds;
+    DevBuf f, o;
+    bool ok = f.ensure(4ull * nbounds, t->err, "bounds") && o.ensure(4 * total, t->err, "bounds") &&
+              t->check(hipMemcpyAsync(f.p, first, 4ull * nbounds, hipMemcpyHostToDevice, t->st),
+                       "bounds upload");
+    if (ok) {
+        k_group_bounds<<<grid_for(total), kB, 0, t->st>>>(t->start.as<uint32_t>(),
+                                                          t->tris.as<uint32_t>(), t->ntiles,
+                                                          f.as<uint32_t>(), nbounds,
+                                                          o.as<uint32_t>());
+        ok = t->check(hipGetLastError(), "bounds kernel") &&
+             t->check(hipMemcpyAsync(out, o.p, 4 * total, hipMemcpyDeviceToHost, t->st),
+                      "bounds") &&
+             t->check(hipStreamSynchronize(t->st), "synchronize");
+    }
+    f.release();
+    o.release();
+    return ok ? 1 : 0;
 }
 
 int vox_tiler_get(vox_tiler* t, vox_tile_lists* out) {

@@ -83,11 +83,16 @@ class OctTree:
     CONST_MODES = {"text": 0, "reference": _lib.VOX_TREE_REFERENCE_CONST,
                    "exact": _lib.VOX_TREE_EXACT_CONST}
 
-    def __init__(self, brickPool: BrickPool, voxelizer, const_mode: str = "text"):
+    def __init__(self, brickPool: BrickPool, voxelizer, const_mode: str = "text",
+                 root_brick: bool = True):
+        """root_brick=False leaves the root's brick to a shared brick server
+        (VOX_TREE_NO_ROOT_BRICK), as for trees a pager loads."""
         self._L = _lib.load()
         self.pool = brickPool
         self._res = voxelizer.raw_result()
         flags = self.CONST_MODES[const_mode]
+        if not root_brick:
+            flags |= _lib.VOX_TREE_NO_ROOT_BRICK
         self._h = self._L.vox_octree_create(brickPool._h, C.byref(self._res), flags)
         if not self._h:
             raise MemoryError("vox_octree_create")

@@ -6,7 +6,11 @@ over an `OctTree`'s node texture and brick pool on the device and returns the
 frame's per-pixel node usage and its distinct requested node ids.
 `BrickServer` takes a ray caster's request list each frame and uploads the
 missing bricks into least recently used slots (BrickPool::update,
-GigaVoxelsBrickPool.cpp 561-682), all on the device.
+GigaVoxelsBrickPool.cpp 561-682), all on the device.  `BrickServer.shared`
+serves several octrees on one brick pool (one LRU for every tree, as the
+reference's BrickPool singleton), with trees attached and detached as a
+pager loads and unloads them.  Frames are queued without host waits; the
+accessors that read results wait for the work they read.
 """
 from __future__ import annotations
 
@@ -28,13 +32,48 @@ def camera(position, look, up, fov_y_deg, width, height, frame=0) -> _lib.VoxCam
 
 
 def raycast_params(vol_min, vol_max, ray_step=0.0, lod_scalar=0.0, jitter_seed=0,
-                   max_traversals=0) -> _lib.VoxRaycastParams:
+                   max_traversals=0, rect=None) -> _lib.VoxRaycastParams:
+    """rect: (x0, y0, x1, y1) pixels to cast (x1, y1 exclusive), or None for
+    the whole viewport."""
     p = _lib.VoxRaycastParams()
     p.vol_min[:] = [float(v) for v in vol_min]
     p.vol_max[:] = [float(v) for v in vol_max]
     p.ray_step, p.lod_scalar = float(ray_step), float(lod_scalar)
     p.jitter_seed, p.max_traversals = int(jitter_seed), int(max_traversals)
+    p.rect[:] = [0, 0, 0, 0] if rect is None else [int(v) for v in rect]
     return p
+
+
+def box_rect(cam: dict, box_min, box_max):
+    """Pixel rectangle holding the projection of a world box for a camera
+    dict (position, look, up, fov_y_deg, width, height), with a one-pixel
+    margin; None when a corner lies at or behind the camera plane (then the
+    whole viewport is cast), () when the projection misses the viewport."""
+    pos = np.asarray(cam["position"], np.float64)
+    look = np.asarray(cam["look"], np.float64)
+    up = np.asarray(cam["up"], np.float64)
+    left = np.cross(up, look)
+    W, H = int(cam["width"]), int(cam["height"])
+    tan_y = np.tan(np.radians(float(cam["fov_y_deg"])) * 0.5)
+    tan_x = tan_y * W / H
+    xs, ys = [], []
+    for c in range(8):
+        p = np.array([(box_max if c & 1 else box_min)[0], (box_max if c & 2 else box_min)[1],
+                      (box_max if c & 4 else box_min)[2]], np.float64) - pos
+        zc = p.dot(look)
+        if zc <= 1e-9:
+            return None
+        sx = (-p.dot(left) / zc) / tan_x   # right = -left
+        sy = (p.dot(up) / zc) / tan_y
+        xs.append((sx + 1.0) * 0.5 * W)
+        ys.append((sy + 1.0) * 0.5 * H)
+    x0 = max(int(np.floor(min(xs))) - 1, 0)
+    y0 = max(int(np.floor(min(ys))) - 1, 0)
+    x1 = min(int(np.ceil(max(xs))) + 1, W)
+    y1 = min(int(np.ceil(max(ys))) + 1, H)
+    if x1 <= x0 or y1 <= y0:
+        return ()
+    return (x0, y0, x1, y1)
 
 
 class RayCaster:
@@ -53,11 +92,16 @@ class RayCaster:
     def getErrorMessage(self) -> str:
         return self._L.vox_raycaster_get_error(self._h).decode()
 
-    def frame(self, cam: _lib.VoxCamera, prm: _lib.VoxRaycastParams) -> _lib.VoxFrameView:
+    def frame(self, cam: _lib.VoxCamera, prm: _lib.VoxRaycastParams, wait: bool = True):
+        """Queue one frame; with wait, return its view (counts, timings)."""
         if not self._L.vox_raycast_frame(self._h, C.byref(cam), C.byref(prm)):
             raise RuntimeError(self.getErrorMessage())
+        return self.view() if wait else None
+
+    def view(self) -> _lib.VoxFrameView:
         v = _lib.VoxFrameView()
-        self._L.vox_raycaster_get_frame(self._h, C.byref(v))
+        if not self._L.vox_raycaster_get_frame(self._h, C.byref(v)):
+            raise RuntimeError(self.getErrorMessage())
         return v
 
     def _copy(self, ptr, n):
@@ -94,10 +138,11 @@ class RayCaster:
 
 
 class BrickServer:
-    def __init__(self, octree, max_uploads_per_frame=0):
+    def __init__(self, octree, max_uploads_per_frame=0, _handle=None):
         self._L = _lib.load()
         self.tree = octree
-        self._h = self._L.vox_server_create(octree._h, int(max_uploads_per_frame))
+        self._h = _handle if _handle is not None else self._L.vox_server_create(
+            octree._h, int(max_uploads_per_frame))
         if not self._h:
             raise MemoryError("vox_server_create")
         err = self.getErrorMessage()
@@ -105,13 +150,35 @@ class BrickServer:
             self.close()
             raise RuntimeError(err)
 
+    @classmethod
+    def shared(cls, pool, max_trees, max_texels_per_tree, max_uploads_per_frame=0):
+        """Server over `pool` for up to max_trees attached octrees."""
+        L = _lib.load()
+        h = L.vox_server_create_shared(pool._h, int(max_trees), int(max_texels_per_tree),
+                                       int(max_uploads_per_frame))
+        return cls(None, _handle=h)
+
+    def attach(self, octree) -> int:
+        i = C.c_uint32()
+        if not self._L.vox_server_attach(self._h, octree._h, C.byref(i)):
+            raise RuntimeError(self.getErrorMessage())
+        return i.value
+
+    def detach(self, octree):
+        if not self._L.vox_server_detach(self._h, octree._h):
+            raise RuntimeError(self.getErrorMessage())
+
     def getErrorMessage(self) -> str:
         return self._L.vox_server_get_error(self._h).decode()
 
-    def frame(self, raycaster: RayCaster):
-        if not self._L.vox_server_frame(self._h, raycaster._h):
+    def frame(self, raycaster, wait: bool = True):
+        """Serve one ray caster's (or a list of ray casters') last frame;
+        with wait, return the stats after it."""
+        rcs = raycaster if isinstance(raycaster, (list, tuple)) else [raycaster]
+        arr = (C.c_void_p * max(len(rcs), 1))(*[r._h for r in rcs])
+        if not self._L.vox_server_frame_multi(self._h, arr, len(rcs)):
             raise RuntimeError(self.getErrorMessage())
-        return self.stats()
+        return self.stats() if wait else None
 
     def stats(self) -> _lib.VoxServerStats:
         s = _lib.VoxServerStats()

@@ -1,0 +1,395 @@
+"""Paged serving: tiles of a large scene voxelized on demand on the device,
+served from one shared brick pool.
+
+The reference's renderer culls the paged octrees of root.gvp against the
+camera frustum each frame (CullVisitor, GigaVoxelsRenderer.cpp 750-797),
+queues the visible trees that are not loaded, and a pager thread loads them
+from disk, most recently accessed first, then nearest (DatabasePagerThread::run
+and SortByDistAndFrameIndex, GigaVoxelsDatabasePager.cpp 220-338); a loaded
+tree's root brick goes on the brick upload list, loaded trees are drawn far
+to near, and trees not seen for 100 frames are unloaded
+(unloadStalePagedOctTreeNodes, 420-465).
+
+Here the "load" of a tile is its voxelization on the device: the scene's
+triangles stay resident in HBM, the tiler's ordered per-tile lists pick each
+tile's triangles (vox_add_group_gather), and vox_generate_async runs the
+tile on a worker thread and its own stream while frames go on.  A finished
+tile becomes an octree on the shared pool (its root brick admitted through
+the LRU like any other) attached to one brick server; every loaded visible
+tree gets a ray cast per camera stream, and one server frame admits all of
+their requests.  Tiles therefore enter as tree slots and memory free up,
+and their voxelization overlaps with the frames' ray casts on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .gvpool import BrickPool, OctTree
+from .gvserve import BrickServer, RayCaster, box_rect, camera, raycast_params
+from .tiler import PagedOctTreeGenerator
+from .voxelizer import Voxelizer
+
+f32 = np.float32
+
+
+# --- frustum (vox::Frustum, VoxVizCore/Frustum.cpp 14-147) ------------------
+
+def perspective(fovy_deg: float, aspect: float, near: float, far: float) -> np.ndarray:
+    """ComputePerspective / ComputeFrustum (Camera.cpp 60-89): the OpenGL
+    frustum matrix, column-major as QMatrix4x4::data() holds it."""
+    ymax = near * math.tan(fovy_deg * math.pi / 360.0)
+    xmax = ymax * aspect
+    l, r, b, t = -xmax, xmax, -ymax, ymax
+    m = np.zeros(16)
+    m[0] = 2 * near / (r - l)
+    m[5] = 2 * near / (t - b)
+    m[8] = (r + l) / (r - l)
+    m[9] = (t + b) / (t - b)
+    m[10] = -(far + near) / (far - near)
+    m[11] = -1.0
+    m[14] = -2 * far * near / (far - near)
+    return m
+
+
+def look_at_matrix(pos, look, up) -> np.ndarray:
+    """View matrix of a camera at pos looking along look (column-major)."""
+    f = np.asarray(look, np.float64)
+    f = f / np.linalg.norm(f)
+    u = np.asarray(up, np.float64)
+    s = np.cross(f, u)
+    s = s / np.linalg.norm(s)
+    u = np.cross(s, f)
+    p = np.asarray(pos, np.float64)
+    m = np.zeros(16)
+    m[0], m[4], m[8] = s
+    m[1], m[5], m[9] = u
+    m[2], m[6], m[10] = -f
+    m[12] = -s.dot(p)
+    m[13] = -u.dot(p)
+    m[14] = f.dot(p)
+    m[15] = 1.0
+    return m
+
+
+def frustum_planes(proj: np.ndarray, modl: np.ndarray) -> np.ndarray:
+    """Frustum::setFromCamera: clip = modelview x projection, then right,
+    left, bottom, top, far, near planes, each normalised by its normal."""
+    clip = np.zeros(16)
+    for r in range(4):
+        for c in range(4):
+            clip[4 * r + c] = sum(modl[4 * r + k] * proj[4 * k + c] for k in range(4))
+    rows = [(3, 0, -1), (3, 0, 1), (3, 1, 1), (3, 1, -1), (3, 2, -1), (3, 2, 1)]
+    planes = np.zeros((6, 4))
+    for p, (a, b, sgn) in enumerate(rows):
+        for k in range(4):
+            planes[p, k] = clip[4 * k + a] + sgn * clip[4 * k + b]
+        planes[p] /= math.sqrt(planes[p, 0] ** 2 + planes[p, 1] ** 2 + planes[p, 2] ** 2)
+    return planes
+
+
+def sphere_in_frustum(planes: np.ndarray, center, radius: float):
+    """Frustum::sphereInFrustum: (visible, distance to the near plane)."""
+    d = 0.0
+    for p in range(6):
+        d = float(f32(planes[p, 0]) * f32(center[0]) + f32(planes[p, 1]) * f32(center[1])
+                  + f32(planes[p, 2]) * f32(center[2]) + f32(planes[p, 3]))
+        if d <= -radius:
+            return False, d
+    return True, d
+
+
+def load_order(requests):
+    """SortByDistAndFrameIndex (GigaVoxelsDatabasePager.cpp 220-230): most
+    recently accessed first, then nearest.  requests: [(tree, last access,
+    distance)]."""
+    return sorted(requests, key=lambda r: (-r[1], r[2]))
+
+
+# --- the pager ---------------------------------------------------------------
+
+@dataclass
+class PagedTree:
+    x: int
+    y: int
+    z: int
+    tile: int                     # tiler tile index y * dimX + x
+    origin: tuple                 # world position of voxel (0, 0, 0)
+    box_min: np.ndarray
+    box_max: np.ndarray
+    center: np.ndarray
+    radius: float
+    state: str = "unloaded"       # unloaded | loading | loaded | empty
+    slot: int = -1                # the server's tree slot while loaded
+    last_access: int = -1
+    requested_frame: int = -1
+    loaded_frame: int = -1
+    vz: Voxelizer | None = None
+    octree: OctTree | None = None
+    rcs: list = field(default_factory=list)
+
+
+class TilePager:
+    """On-demand tile voxelization + one shared-pool brick server.
+
+    meshes: scene meshes (numpy, as voxvizthesis_amd.scenes builds them);
+    voxel_size, N, B: the tiles' voxel size, grid edge and brick edge (the
+    PagedGigaVoxelOctTreeGenerator parameters, main.cu 4135-4200)."""
+
+    def __init__(self, meshes, voxel_size, N: int, B: int, pool_slots, max_trees: int = 32,
+                 max_uploads: int = 65536, max_loading: int = 4, stale_frames: int = 100,
+                 streams: int = 1, max_width: int = 1920, max_height: int = 1080,
+                 near: float = 1e-3, far: float = 100.0, options: int = 0):
+        import torch  # device memory for the resident scene
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("TilePager needs a GPU")
+        self._L = _lib.load()
+        self.meshes = list(meshes)
+        self.N, self.B = int(N), int(B)
+        self.gen = PagedOctTreeGenerator(self.meshes, voxel_size, self.N, self.B)
+        self.max_loading, self.stale = int(max_loading), int(stale_frames)
+        self.streams, self.W, self.H = int(streams), int(max_width), int(max_height)
+        self.near, self.far, self.options = float(near), float(far), int(options)
+        self.frame_index = 0
+        self._cast = []
+        self.load_ms = []
+        self.tiles_loaded = 0
+        # scene resident in HBM, one set of arrays per mesh
+        dev = torch.device("cuda")
+        self._dev = []
+        for m in self.meshes:
+            d = dict(verts=torch.from_numpy(np.ascontiguousarray(m.verts, np.float32)).to(dev))
+            d["vn"] = None if m.vnormals is None else torch.from_numpy(
+                np.ascontiguousarray(m.vnormals, np.float32)).to(dev)
+            tex = m.uvs is not None and m.image is not None
+            d["uv"] = torch.from_numpy(np.ascontiguousarray(m.uvs, np.float32)).to(dev) \
+                if tex else None
+            d["img"] = torch.from_numpy(np.ascontiguousarray(m.image, np.uint8)).to(dev) \
+                if tex else None
+            self._dev.append(d)
+        all_verts = torch.cat([d["verts"].reshape(-1, 9) for d in self._dev]) if self._dev \
+            else torch.zeros((0, 9), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        # per-tile triangle lists on the device, and each mesh's range in them
+        g = self.gen.grid()
+        self._tiler = self._L.vox_tiler_create()
+        if not self._tiler or not self._L.vox_tiler_assign(self._tiler, all_verts.data_ptr(),
+                                                           all_verts.shape[0], 1, C.byref(g)):
+            raise RuntimeError(self._L.vox_tiler_get_error(self._tiler).decode())
+        del all_verts
+        lst = _lib.VoxTileLists()
+        self._L.vox_tiler_get(self._tiler, C.byref(lst))
+        self._tile_tris = lst.tile_tris
+        nt = int(lst.ntiles)
+        first = np.asarray(self.gen.group_off, np.uint32)
+        self._bounds = np.zeros(nt * first.size, np.uint32)
+        if not self._L.vox_tiler_group_bounds(self._tiler, first.ctypes.data, first.size,
+                                              self._bounds.ctypes.data):
+            raise RuntimeError(self._L.vox_tiler_get_error(self._tiler).decode())
+        self._bounds = self._bounds.reshape(nt, first.size)
+        zmin = np.empty(nt, np.float32)
+        zmax = np.empty(nt, np.float32)
+        self._L.vox_pool_copy_to_host(zmin.ctypes.data, lst.tile_zmin, zmin.nbytes)
+        self._L.vox_pool_copy_to_host(zmax.ctypes.data, lst.tile_zmax, zmax.nbytes)
+        # the paged trees: every non-empty tile, stacked in z (main.cu
+        # 4225-4340), with root.gvp's bounding sphere (4416-4502)
+        self.trees = []
+        size = self.gen.size
+        for y in range(self.gen.dim[1]):
+            for x in range(self.gen.dim[0]):
+                t = y * self.gen.dim[0] + x
+                if self._bounds[t, -1] == self._bounds[t, 0]:
+                    continue
+                for z, (oz, blo, bhi) in enumerate(self.gen.z_stack(zmin[t], zmax[t])):
+                    mn = np.array([self.gen.ox[x], self.gen.oy[y], blo], np.float32)
+                    mx = np.array([f32(self.gen.ox[x] + size[0]), f32(self.gen.oy[y] + size[1]),
+                                   bhi], np.float32)
+                    c = ((mn + mx) * f32(0.5)).astype(np.float32)
+                    dv = (mx - mn).astype(np.float32)
+                    r = float(np.sqrt(f32(f32(f32(dv[0] * dv[0]) + f32(dv[1] * dv[1]))
+                                          + f32(dv[2] * dv[2])) * f32(0.25)))
+                    org = (self.gen.ox[x], self.gen.oy[y], oz)
+                    lo = np.asarray(org, np.float32)
+                    hi = (lo + size).astype(np.float32)
+                    self.trees.append(PagedTree(x, y, z, t, org, lo, hi, c, r))
+        # shared pool and server; a tree's node texture holds at most
+        # 8 * (1 + internal nodes) texels, plus the texture's rounding
+        L = int(round(math.log2(self.N // self.B))) + 1
+        blocks = 1 + sum(8 ** d for d in range(L - 1))
+        side = int(round((8 * blocks) ** (1.0 / 3.0))) + 2
+        self.cap = 8 * blocks + 2 * side * side + 64
+        self.pool = BrickPool((self.B,) * 3, pool_slots)
+        self.server = BrickServer.shared(self.pool, max_trees, self.cap, max_uploads)
+        self.max_trees = int(max_trees)
+
+    # --- loading -----------------------------------------------------------
+    def _start_load(self, tr: PagedTree):
+        import time
+        vz = Voxelizer((self.N,) * 3, (self.B,) * 3, None, options=self.options)
+        ok = vz.initCuda() and vz.allocateVoxelMemory() and \
+            vz.setVoxelizationParams(tr.origin, self.gen.vs)
+        row = self._bounds[tr.tile]
+        for gi, m in enumerate(self.meshes):
+            a, b = int(row[gi]), int(row[gi + 1])
+            if b == a:
+                continue
+            d = self._dev[gi]
+            grp = _lib.VoxGroup()
+            grp.verts = d["verts"].data_ptr()
+            grp.vnormals = None if d["vn"] is None else d["vn"].data_ptr()
+            grp.uvs = None if d["uv"] is None else d["uv"].data_ptr()
+            grp.ntri = m.ntri
+            grp.is_terrain = 1 if m.is_terrain else 0
+            grp.rgba8 = None if d["img"] is None else d["img"].data_ptr()
+            grp.width = 0 if d["img"] is None else int(m.image.shape[1])
+            grp.height = 0 if d["img"] is None else int(m.image.shape[0])
+            grp.addr0, grp.addr1 = int(m.addr[0]), int(m.addr[1])
+            idx = C.c_void_p(self._tile_tris + 4 * a)
+            ok = ok and bool(self._L.vox_add_group_gather(vz._h, C.byref(grp), idx, b - a,
+                                                          int(self.gen.group_off[gi])))
+        ok = ok and vz.allocateTriangleMemory() and vz.generateAsync()
+        if not ok:
+            err = vz.getErrorMessage()
+            vz.close()
+            raise RuntimeError(f"tile {tr.x},{tr.y},{tr.z}: {err}")
+        tr.vz = vz
+        tr.state = "loading"
+        tr._t0 = time.perf_counter()
+
+    def _finish_load(self, tr: PagedTree, st: int):
+        import time
+        if st == 0:
+            raise RuntimeError(f"tile {tr.x},{tr.y},{tr.z}: {tr.vz.getErrorMessage()}")
+        self.load_ms.append((time.perf_counter() - tr._t0) * 1e3)
+        if st == -1:                       # no triangle overlapped the grid
+            tr.vz.close()
+            tr.vz = None
+            tr.state = "empty"
+            return
+        tr.vz.releaseWorkMemory()
+        tr.octree = OctTree(self.pool, tr.vz, root_brick=False)
+        if sum(t.state == "loaded" for t in self.trees) >= self.max_trees:
+            self._unload(min((t for t in self.trees if t.state == "loaded"),
+                             key=lambda t: t.last_access))
+        tr.slot = self.server.attach(tr.octree)
+        tr.rcs = [RayCaster(tr.octree, self.W, self.H) for _ in range(self.streams)]
+        tr.state = "loaded"
+        tr.loaded_frame = self.frame_index
+        self.tiles_loaded += 1
+
+    def server_slot(self, tr: PagedTree) -> int:
+        return tr.slot
+
+    def _unload(self, tr: PagedTree):
+        self.server.detach(tr.octree)
+        tr.slot = -1
+        for rc in tr.rcs:
+            rc.close()
+        tr.rcs = []
+        tr.octree.close()
+        tr.octree = None
+        tr.vz.close()
+        tr.vz = None
+        tr.state = "unloaded"
+
+    # --- one frame -----------------------------------------------------------
+    def frame(self, cams, prm_extra=None, serve: bool = True):
+        """cams: per camera stream a dict of vox_camera fields (position,
+        look, up, fov_y_deg, width, height).  Returns per stream the list of
+        (tree, ray caster) drawn far to near this frame.  serve=False leaves
+        the server frame to serve() (to read the state the casts saw)."""
+        f = self.frame_index
+        # finished voxelizations become trees
+        for tr in self.trees:
+            if tr.state == "loading":
+                st = tr.vz.wait(False)
+                if st != 2:
+                    self._finish_load(tr, st)
+        # cull per stream (CullVisitor)
+        drawn, requests = [], {}
+        for cm in cams:
+            proj = perspective(cm["fov_y_deg"], cm["width"] / cm["height"], self.near, self.far)
+            planes = frustum_planes(proj, look_at_matrix(cm["position"], cm["look"], cm["up"]))
+            vis = []
+            for i, tr in enumerate(self.trees):
+                inside, dist = sphere_in_frustum(planes, tr.center, tr.radius)
+                if not inside:
+                    continue
+                tr.last_access = f
+                if tr.state == "loaded":
+                    vis.append((dist - tr.radius, i))
+                elif tr.state == "unloaded":
+                    requests[i] = min(requests.get(i, dist), dist)
+            vis.sort(reverse=True)                       # far to near
+            drawn.append([self.trees[i] for _, i in vis])
+        # unload trees unseen for stale_frames frames
+        for tr in self.trees:
+            if tr.state == "loaded" and f - tr.last_access > self.stale:
+                self._unload(tr)
+        # start loads, most recent then nearest, as load slots free up
+        loading = sum(t.state == "loading" for t in self.trees)
+        order = load_order([(i, self.trees[i].last_access, d) for i, d in requests.items()])
+        for i, _, _ in order:
+            if loading >= self.max_loading:
+                break
+            tr = self.trees[i]
+            tr.requested_frame = f
+            self._start_load(tr)
+            loading += 1
+        # ray casts of every stream's visible trees, one server frame
+        rcs, out = [], []
+        for k, (cm, trees) in enumerate(zip(cams, drawn)):
+            row = []
+            for tr in trees:
+                rect = box_rect(cm, tr.box_min, tr.box_max)
+                if rect == ():
+                    continue                              # nothing on screen
+                rc = tr.rcs[k]
+                prm = raycast_params(tr.box_min, tr.box_max, rect=rect, **(prm_extra or {}))
+                rc.frame(camera(cm["position"], cm["look"], cm["up"], cm["fov_y_deg"],
+                                cm["width"], cm["height"], f), prm, wait=False)
+                rcs.append(rc)
+                row.append((tr, rc))
+            out.append(row)
+        self._cast = rcs
+        if serve:
+            self.serve()
+        self.frame_index += 1
+        return out
+
+    def serve(self):
+        """The server frame for the ray casts of the last frame."""
+        if self._cast:
+            self.server.frame(self._cast, wait=False)
+        self._cast = []
+
+    def drain(self):
+        """Wait for every running voxelization and finish its load."""
+        for tr in self.trees:
+            if tr.state == "loading":
+                self._finish_load(tr, tr.vz.wait(True))
+
+    def close(self):
+        for tr in self.trees:
+            if tr.state == "loading":
+                tr.vz.wait(True)
+                tr.vz.close()
+                tr.vz = None
+                tr.state = "unloaded"
+            elif tr.state == "loaded":
+                self._unload(tr)
+        if getattr(self, "server", None) is not None:
+            self.server.close()
+            self.server = None
+        if getattr(self, "pool", None) is not None:
+            self.pool.close()
+            self.pool = None
+        if getattr(self, "_tiler", None):
+            self._L.vox_tiler_destroy(self._tiler)
+            self._tiler = None
+        self._dev = []

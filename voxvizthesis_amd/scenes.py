@@ -152,16 +152,42 @@ class Scene:
         return sum(m.ntri for m in self.meshes)
 
 
+def _fbm_heights(g: np.ndarray, seed: int, extent: float) -> np.ndarray:
+    """0.5 E + 0.2 E fbm(x / E, y / E) on the grid g x g ([j, i] -> (x_i,
+    y_j)) as float32, computed in row blocks on a thread pool (the values
+    are elementwise, so the blocking does not change them)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    n1 = g.size
+    out = np.empty((n1, n1), np.float32)
+    rows = max(1, min(256, (1 << 22) // max(n1, 1)))
+
+    def block(j0):
+        j1 = min(j0 + rows, n1)
+        X, Y = np.meshgrid(g, g[j0:j1], indexing="xy")
+        out[j0:j1] = (0.5 * extent + 0.2 * extent * fbm(X / extent, Y / extent, seed)
+                      ).astype(np.float32)
+
+    starts = list(range(0, n1, rows))
+    workers = min(len(starts), os.cpu_count() or 1, 16)
+    if workers <= 1:
+        for j0 in starts:
+            block(j0)
+    else:
+        with ThreadPoolExecutor(workers) as ex:
+            list(ex.map(block, starts))
+    return out
+
+
 def heightfield_terrain(n_cells: int, seed: int, extent: float = 1.0,
                         tex_width: int = 256) -> Mesh:
     """(n_cells+1)^2 fBm heightfield -> 2*n_cells^2 textured terrain triangles."""
     n = n_cells
     g = np.arange(n + 1, dtype=np.float64) * (extent / n)
-    X, Y = np.meshgrid(g, g, indexing="xy")           # [j, i] -> (x_i, y_j)
-    H = 0.5 * extent + 0.2 * extent * fbm(X / extent, Y / extent, seed)
-    Xf = X.astype(np.float32)
-    Yf = Y.astype(np.float32)
-    Hf = H.astype(np.float32)
+    Hf = _fbm_heights(g, seed, extent)
+    X, Y = np.meshgrid(g.astype(np.float32), g.astype(np.float32), indexing="xy")
+    Xf, Yf = X, Y
     P = np.stack([Xf, Yf, Hf], axis=-1)               # (n+1, n+1, 3)
 
     # Vertex normals by central differences in fp32 (one-sided at the rim).

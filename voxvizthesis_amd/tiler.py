@@ -50,8 +50,9 @@ class TileAssignment:
         return a
 
     def tile(self, t: int) -> np.ndarray:
-        """Triangles of tile t in input order (as the reference appends them)."""
-        return np.sort(self.tris[self.start[t]:self.start[t + 1]])
+        """Triangles of tile t in input order (as the reference appends them;
+        the device lists are ordered)."""
+        return self.tris[self.start[t]:self.start[t + 1]]
 
 
 def fmt(v) -> str:

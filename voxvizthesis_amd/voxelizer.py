@@ -97,6 +97,23 @@ class Voxelizer:
     def getErrorMessage(self) -> str:
         return self._L.vox_get_error(self._h).decode()
 
+    # --- asynchronous generate (not in the reference) -------------------------
+    def generateAsync(self, outputCompressed=False) -> bool:
+        """Start generateVoxelsAndOctTree (no files) on a worker thread."""
+        return bool(self._L.vox_generate_async(self._h, 1 if outputCompressed else 0))
+
+    def wait(self, block: bool = True) -> int:
+        """1 / 0 / -1 like generateVoxelsAndOctTree, 2 while still running
+        (block=False)."""
+        return int(self._L.vox_wait(self._h, 1 if block else 0))
+
+    def releaseWorkMemory(self) -> bool:
+        """Free triangles, bins and level tiles, keeping the result."""
+        ok = bool(self._L.vox_release_work_memory(self._h))
+        if ok:
+            self._keep = []
+        return ok
+
     # --- results -------------------------------------------------------------
     def raw_result(self) -> "_lib.VoxResult":
         r = _lib.VoxResult()
