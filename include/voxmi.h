@@ -222,6 +222,15 @@ int vox_wait(vox_ctx* h, int32_t block);
  * reference. */
 int vox_release_work_memory(vox_ctx* h);
 
+/* Device memory for callers with no GPU array library of their own (a scene
+ * kept resident in HBM for vox_add_group_gather or vox_add_group_device):
+ * vox_device_alloc returns NULL on failure; vox_device_upload copies host
+ * bytes to dst + offset and returns when they are on the device; free
+ * waits for the device first.  Not in the reference. */
+void* vox_device_alloc(uint64_t bytes);
+int vox_device_upload(void* dst, uint64_t offset, const void* src, uint64_t bytes);
+int vox_device_free(void* p);
+
 /* Write tree.gvx and voxels_<depth>.gvb of the last vox_generate into
  * out_dir (host-side writer, Voxelizer.cu 2816-3061, VoxelBrickWriter.cu
  * 725-818). */

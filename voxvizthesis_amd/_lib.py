@@ -27,6 +27,7 @@ EXPORTS = [
     "vox_generate", "vox_get_error", "vox_get_result", "vox_get_stats", "vox_copy_to_host",
     "vox_copy_level", "vox_write_files", "vox_get_compressed", "vox_set_launch",
     "vox_generate_async", "vox_wait", "vox_release_work_memory", "vox_add_group_gather",
+    "vox_device_alloc", "vox_device_upload", "vox_device_free",
 ]
 # Every symbol include/voxmi_batch.h declares.
 BATCH_EXPORTS = [
@@ -160,6 +161,38 @@ class VoxServerStats(C.Structure):
 _lib = None
 
 
+class DeviceArray:
+    """A host array copied into device memory of the library's own runtime
+    (vox_device_alloc); .ptr is the device address.  Freed by close()."""
+
+    def __init__(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        self.L = load()
+        self.nbytes = int(a.nbytes)
+        self.ptr = self.L.vox_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise MemoryError(f"vox_device_alloc({self.nbytes})")
+        step = 1 << 30                      # upload in 1 GiB pieces
+        flat = a.reshape(-1).view(np.uint8)
+        for off in range(0, self.nbytes, step):
+            piece = flat[off:off + step]
+            if not self.L.vox_device_upload(self.ptr, off, piece.ctypes.data, piece.nbytes):
+                self.close()
+                raise RuntimeError("vox_device_upload failed")
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.L.vox_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def load():
     """Load libvoxmi.so; raises if it has not been built."""
     global _lib
@@ -200,6 +233,10 @@ def load():
     L.vox_wait.argtypes = [vp, C.c_int32]
     L.vox_release_work_memory.argtypes = [vp]
     L.vox_add_group_gather.argtypes = [vp, C.POINTER(VoxGroup), vp, C.c_uint64, C.c_uint32]
+    L.vox_device_alloc.restype = vp
+    L.vox_device_alloc.argtypes = [C.c_uint64]
+    L.vox_device_upload.argtypes = [vp, C.c_uint64, vp, C.c_uint64]
+    L.vox_device_free.argtypes = [vp]
     # batch
     L.vox_batch_create.restype = vp
     L.vox_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32]

@@ -1355,6 +1355,27 @@ int vox_wait(vox_ctx* h, int32_t block) {
     return h->asyncRet;
 }
 
+void* vox_device_alloc(uint64_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+
+int vox_device_upload(void* dst, uint64_t offset, const void* src, uint64_t bytes) {
+    if (!dst || (bytes && !src)) return 0;
+    if (bytes == 0) return 1;
+    return hipMemcpy(static_cast<uint8_t*>(dst) + offset, src, bytes, hipMemcpyHostToDevice) ==
+                   hipSuccess
+               ? 1
+               : 0;
+}
+
+int vox_device_free(void* p) {
+    if (!p) return 1;
+    if (hipDeviceSynchronize() != hipSuccess) return 0;
+    return hipFree(p) == hipSuccess ? 1 : 0;
+}
+
 int vox_release_work_memory(vox_ctx* h) {
     if (!h || !h->valid || !idle(h)) return 0;
     if (h->st && !h->check(hipStreamSynchronize(h->st), "synchronize")) return 0;

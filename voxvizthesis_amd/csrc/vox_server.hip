@@ -199,22 +199,39 @@ __global__ void k_srv_tail(const uint32_t* __restrict__ excl, uint64_t ntex,
     if (threadIdx.x == 0 && blockIdx.x == 0) ctr[1] += excl[ntex];
 }
 
-// 3a. slot keys and their histogram.
+// 3a. slot keys and their histogram: lanes of a wave with one key add once
+// to the workgroup's histogram in LDS, which adds its non-zero bins to the
+// global one (most slots share a key: empty, or last used together).
 __global__ void k_srv_keys(uint32_t nslots, uint32_t frame, const uint32_t* __restrict__ slotNode,
                            const uint32_t* __restrict__ slotLast, uint32_t* __restrict__ key,
                            uint32_t* __restrict__ hist) {
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nslots;
-         s += gridDim.x * blockDim.x) {
-        uint32_t k;
-        if (slotNode[s] == kNone) {
-            k = 0;
-        } else {
-            const uint32_t age = frame - slotLast[s];
-            k = age == 0 ? 256u : 256u - (age < 255u ? age : 255u);
+    __shared__ uint32_t h[kKeys];
+    for (uint32_t i = threadIdx.x; i < kKeys; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const int lane = (int)(threadIdx.x & 63u);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nslots; base += gridDim.x * blockDim.x) {
+        const uint32_t s = base + threadIdx.x;
+        const bool valid = s < nslots;
+        uint32_t k = 0;
+        if (valid) {
+            if (slotNode[s] != kNone) {
+                const uint32_t age = frame - slotLast[s];
+                k = age == 0 ? 256u : 256u - (age < 255u ? age : 255u);
+            }
+            key[s] = k;
         }
-        key[s] = k;
-        atomicAdd(&hist[k], 1u);
+        uint64_t pend = __ballot(valid);
+        while (pend) {
+            const int l = __ffsll((unsigned long long)pend) - 1;
+            const uint32_t lk = __shfl(k, l);
+            const uint64_t same = __ballot(valid && k == lk);
+            if (lane == l) atomicAdd(&h[lk], (uint32_t)__popcll(same));
+            pend &= ~same;
+        }
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kKeys; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
 // 3b. how many to admit and the key threshold.  sel: [0] n, [1] key T,

@@ -144,10 +144,6 @@ class TilePager:
                  max_uploads: int = 65536, max_loading: int = 4, stale_frames: int = 100,
                  streams: int = 1, max_width: int = 1920, max_height: int = 1080,
                  near: float = 1e-3, far: float = 100.0, options: int = 0):
-        import torch  # device memory for the resident scene
-
-        if not torch.cuda.is_available():
-            raise RuntimeError("TilePager needs a GPU")
         self._L = _lib.load()
         self.meshes = list(meshes)
         self.N, self.B = int(N), int(B)
@@ -160,28 +156,23 @@ class TilePager:
         self.load_ms = []
         self.tiles_loaded = 0
         # scene resident in HBM, one set of arrays per mesh
-        dev = torch.device("cuda")
         self._dev = []
         for m in self.meshes:
-            d = dict(verts=torch.from_numpy(np.ascontiguousarray(m.verts, np.float32)).to(dev))
-            d["vn"] = None if m.vnormals is None else torch.from_numpy(
-                np.ascontiguousarray(m.vnormals, np.float32)).to(dev)
             tex = m.uvs is not None and m.image is not None
-            d["uv"] = torch.from_numpy(np.ascontiguousarray(m.uvs, np.float32)).to(dev) \
-                if tex else None
-            d["img"] = torch.from_numpy(np.ascontiguousarray(m.image, np.uint8)).to(dev) \
-                if tex else None
-            self._dev.append(d)
-        all_verts = torch.cat([d["verts"].reshape(-1, 9) for d in self._dev]) if self._dev \
-            else torch.zeros((0, 9), dtype=torch.float32, device=dev)
-        torch.cuda.synchronize()
+            self._dev.append(dict(
+                verts=_lib.DeviceArray(np.asarray(m.verts, np.float32)),
+                vn=None if m.vnormals is None else _lib.DeviceArray(
+                    np.asarray(m.vnormals, np.float32)),
+                uv=_lib.DeviceArray(np.asarray(m.uvs, np.float32)) if tex else None,
+                img=_lib.DeviceArray(np.asarray(m.image, np.uint8)) if tex else None))
         # per-tile triangle lists on the device, and each mesh's range in them
         g = self.gen.grid()
         self._tiler = self._L.vox_tiler_create()
-        if not self._tiler or not self._L.vox_tiler_assign(self._tiler, all_verts.data_ptr(),
-                                                           all_verts.shape[0], 1, C.byref(g)):
+        if not self._tiler or not self._L.vox_tiler_assign(self._tiler,
+                                                           self.gen.verts.ctypes.data,
+                                                           self.gen.verts.shape[0], 0,
+                                                           C.byref(g)):
             raise RuntimeError(self._L.vox_tiler_get_error(self._tiler).decode())
-        del all_verts
         lst = _lib.VoxTileLists()
         self._L.vox_tiler_get(self._tiler, C.byref(lst))
         self._tile_tris = lst.tile_tris
@@ -240,12 +231,12 @@ class TilePager:
                 continue
             d = self._dev[gi]
             grp = _lib.VoxGroup()
-            grp.verts = d["verts"].data_ptr()
-            grp.vnormals = None if d["vn"] is None else d["vn"].data_ptr()
-            grp.uvs = None if d["uv"] is None else d["uv"].data_ptr()
+            grp.verts = d["verts"].ptr
+            grp.vnormals = None if d["vn"] is None else d["vn"].ptr
+            grp.uvs = None if d["uv"] is None else d["uv"].ptr
             grp.ntri = m.ntri
             grp.is_terrain = 1 if m.is_terrain else 0
-            grp.rgba8 = None if d["img"] is None else d["img"].data_ptr()
+            grp.rgba8 = None if d["img"] is None else d["img"].ptr
             grp.width = 0 if d["img"] is None else int(m.image.shape[1])
             grp.height = 0 if d["img"] is None else int(m.image.shape[0])
             grp.addr0, grp.addr1 = int(m.addr[0]), int(m.addr[1])
@@ -392,4 +383,8 @@ class TilePager:
         if getattr(self, "_tiler", None):
             self._L.vox_tiler_destroy(self._tiler)
             self._tiler = None
+        for d in getattr(self, "_dev", []):
+            for a in d.values():
+                if a is not None:
+                    a.close()
         self._dev = []
