@@ -148,29 +148,46 @@ def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, slot
     from voxvizthesis_amd import scenes
     from voxvizthesis_amd.pager import TilePager
 
+    # In metres with 1 m voxels, as the reference's tiler assumes (it floors
+    # each tree's z origin to a whole unit, main.cu 4258); the camera planes
+    # are the reference's 1 m / 10 km.
+    extent = float(cells)
     t0 = time.perf_counter()
-    terrain = scenes.heightfield_terrain(cells, 5)
+    terrain = scenes.heightfield_terrain(cells, 5, extent=extent)
     t_gen = time.perf_counter() - t0
     log(f"[bench] paged terrain {cells + 1}^2 generated in {t_gen:.1f} s")
     t0 = time.perf_counter()
-    pg = TilePager([terrain], 1.0 / cells, tile, B, slots, max_trees=max_trees,
+    pg = TilePager([terrain], 1.0, tile, B, slots, max_trees=max_trees,
                    max_loading=max_loading, streams=1, max_width=W, max_height=H,
-                   near=1e-3, far=10.0)
+                   near=1.0, far=10000.0)
     t_setup = time.perf_counter() - t0
-    path = scenes.flythrough(seed=5, frames=frames, terrain_seed=5)
-    wall = []
+    path = scenes.flythrough(seed=5, frames=frames, extent=extent, terrain_seed=5)
+    period = 1.0 / 60.0
+    wall, loaded_at = [], []
     try:
+        t_start = time.perf_counter()
         for f, (pos, look, up) in enumerate(path):
+            # frames at 60 Hz: queue the frame, wait for its server step,
+            # then sleep to the next frame time; tile voxelizations run on
+            # their own streams meanwhile
             cam = dict(position=pos, look=look, up=up, fov_y_deg=60.0, width=W, height=H)
             a = time.perf_counter()
             pg.frame([cam])
-            wall.append(time.perf_counter() - a)
+            pg.server.stats()
+            b = time.perf_counter()
+            wall.append(b - a)
+            loaded_at.append(pg.tiles_loaded)
+            rest = t_start + (f + 1) * period - time.perf_counter()
+            if rest > 0:
+                time.sleep(rest)
+        total = time.perf_counter() - t_start
         st = pg.server.stats()
         lat = pg.server.latencies_us() / 1e3
         loads = np.asarray(pg.load_ms) if pg.load_ms else np.zeros(1)
-        total = sum(wall)
+        first = next((f for f, n in enumerate(loaded_at) if n > 0), None)
         return {
-            "workload": f"{cells + 1}^2 fBm terrain (seed 5) in a {cells}^3 virtual grid, "
+            "workload": f"{cells + 1}^2 fBm terrain (seed 5, {cells} m, 1 m voxels) in a "
+                        f"{cells}^3 virtual grid, "
                         f"{tile}^3 tiles of {B}^3 bricks voxelized on demand, {frames}-frame "
                         f"seeded flythrough, {W}x{H} ray cast",
             "tiles": len(pg.trees),
@@ -179,8 +196,11 @@ def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, slot
             "pool_gb": round(float(np.prod(slots)) * (B + 2) ** 3 * 8 / 1e9, 2),
             "scene_setup_s": round(t_setup, 2),
             "frames": frames,
-            "frame_ms": round(total / frames * 1e3, 3),
+            "frame_period_ms": round(period * 1e3, 3),
+            "frame_ms": round(float(np.mean(wall)) * 1e3, 3),
             "frame_ms_p99": round(float(np.percentile(np.asarray(wall) * 1e3, 99)), 3),
+            "frames_over_period": int(sum(w > period for w in wall)),
+            "first_tile_frame": first,
             "tile_load_ms_p50": round(float(np.percentile(loads, 50)), 2),
             "tile_load_ms_max": round(float(loads.max()), 2),
             "uploads": int(st.uploads),
@@ -287,6 +307,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="MiB copied once by a plain copy kernel (profiled runs)")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--serve-frames", type=int, default=300)
     ap.add_argument("--serve-width", type=int, default=1920)
@@ -335,6 +357,18 @@ def main():
         if vz.generateVoxelsAndOctTree(None) != 1:
             raise RuntimeError(vz.getErrorMessage())
 
+    if args.copy_calibration:
+        # one plain copy of known size, the reference point of the traffic
+        # counters of a profiled run (not part of any timed figure)
+        from voxvizthesis_amd import _lib
+        L = _lib.load()
+        nb = args.copy_calibration << 20
+        src, dst = L.vox_device_alloc(nb), L.vox_device_alloc(nb)
+        if not (src and dst and L.vox_device_copy(dst, src, nb)):
+            raise RuntimeError("calibration copy failed")
+        L.vox_device_free(src)
+        L.vox_device_free(dst)
+        log(f"[bench] calibration copy of {nb} bytes done")
     for i in range(args.warmup):
         step()
     log(f"[bench] warmup {args.warmup} done")

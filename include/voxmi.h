@@ -230,6 +230,10 @@ int vox_release_work_memory(vox_ctx* h);
 void* vox_device_alloc(uint64_t bytes);
 int vox_device_upload(void* dst, uint64_t offset, const void* src, uint64_t bytes);
 int vox_device_free(void* p);
+/* Device-to-device copy by a plain copy kernel (16-byte aligned pointers, a
+ * multiple of 16 bytes); returns when done.  Its known traffic is the
+ * reference point that calibrates hardware traffic counters. */
+int vox_device_copy(void* dst, const void* src, uint64_t bytes);
 
 /* Write tree.gvx and voxels_<depth>.gvb of the last vox_generate into
  * out_dir (host-side writer, Voxelizer.cu 2816-3061, VoxelBrickWriter.cu

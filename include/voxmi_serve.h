@@ -85,6 +85,7 @@ typedef struct vox_frame_view {
     uint32_t  nrequests_total;  /* distinct ids before the cap               */
     double    ms_raycast;       /* HIP-event time of the ray-cast kernel     */
     double    ms_requests;      /* HIP-event time of the deduplication       */
+    const float* rgba_f32;      /* device, width*height*4 floats: FragColor  */
 } vox_frame_view;
 
 /* Ray caster over one octree (its node texture and brick pool).  The frame
@@ -100,6 +101,16 @@ const char* vox_raycaster_get_error(vox_raycaster* rc);
 int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycast_params* prm);
 /* The last frame's buffers, counts and timings (waits for the frame). */
 int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out);
+/* Blend the last frames of n ray casters, in the order given (far to near,
+ * as the renderer draws its list), into dst: width*height RGBA32F pixels on
+ * the device, the renderer's colour target (GigaVoxelsRenderer.cpp 511-515).
+ * Each caster's pixel rectangle is blended with glBlendFunc(GL_SRC_ALPHA,
+ * GL_ONE_MINUS_SRC_ALPHA) (GigaVoxelsRenderer.cpp 1019-1022): per channel,
+ * alpha included, out = src * src.a + dst * (1 - src.a) in fp32, src being
+ * the caster's FragColor.  clear = 1 first zeroes dst (glClearBufferfv).
+ * Queued after the casters' frames; their next frames wait for it. */
+int vox_composite(vox_raycaster* const* rcs, uint32_t n, float* dst, int32_t width,
+                  int32_t height, int32_t clear);
 
 /* ---- brick server ---------------------------------------------------------
  * Per frame (vox_server_frame), on the device:

@@ -58,9 +58,22 @@ def test_pager_tiles_match_alone_and_pool_is_consistent():
         for f, (pos, target) in enumerate(path):
             look, up = look_at(pos, target)
             cam = dict(position=pos, look=look, up=up, fov_y_deg=60.0, width=W, height=H)
+            if f in (2, 5):
+                pg.drain()       # loads requested so far become trees now
             drawn = pg.frame([cam], serve=False)
-            if f in (6, 9):
+            if f in (3, 6, 9):
                 pool = pg.pool.textures()[0]
+                # the composite: far to near, glBlendFunc(SRC_ALPHA,
+                # ONE_MINUS_SRC_ALPHA) per channel in fp32, over each rectangle
+                img = np.zeros((H, W, 4), np.float32)
+                for tr, rc in drawn[0]:
+                    src = rc.rgba_f32()
+                    r = box_rect(cam, tr.box_min, tr.box_max)
+                    x0, y0, x1, y1 = (0, 0, W, H) if r is None else r
+                    s_ = src[y0:y1, x0:x1]
+                    a_ = s_[:, :, 3:4]
+                    img[y0:y1, x0:x1] = s_ * a_ + img[y0:y1, x0:x1] * (np.float32(1.0) - a_)
+                assert (pg.image(0) == img).all()
                 for tr, rc in drawn[0]:
                     rect = box_rect(cam, tr.box_min, tr.box_max)
                     nodes = tr.octree.nodes()
@@ -80,6 +93,7 @@ def test_pager_tiles_match_alone_and_pool_is_consistent():
                     checked += 1
             pg.serve()
         pg.drain()
+        print(f"[pager] checked {checked} tree frames", flush=True)
         assert checked > 0
         loaded = [t for t in pg.trees if t.state == "loaded"]
         print(f"[pager] loaded {len(loaded)} trees, load ms {np.round(pg.load_ms, 1)}",

@@ -27,7 +27,7 @@ EXPORTS = [
     "vox_generate", "vox_get_error", "vox_get_result", "vox_get_stats", "vox_copy_to_host",
     "vox_copy_level", "vox_write_files", "vox_get_compressed", "vox_set_launch",
     "vox_generate_async", "vox_wait", "vox_release_work_memory", "vox_add_group_gather",
-    "vox_device_alloc", "vox_device_upload", "vox_device_free",
+    "vox_device_alloc", "vox_device_upload", "vox_device_free", "vox_device_copy",
 ]
 # Every symbol include/voxmi_batch.h declares.
 BATCH_EXPORTS = [
@@ -47,7 +47,7 @@ SERVE_EXPORTS = [
     "vox_server_create", "vox_server_destroy", "vox_server_get_error", "vox_server_frame",
     "vox_server_get_stats", "vox_server_latencies", "vox_server_state",
     "vox_server_create_shared", "vox_server_attach", "vox_server_detach",
-    "vox_server_frame_multi",
+    "vox_server_frame_multi", "vox_composite",
 ]
 # Every symbol include/voxmi_tiler.h declares.
 TILER_EXPORTS = [
@@ -148,7 +148,7 @@ class VoxFrameView(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgba", C.c_void_p),
                 ("usage", C.c_void_p), ("requests", C.c_void_p), ("nrequests", C.c_uint32),
                 ("nrequests_total", C.c_uint32), ("ms_raycast", C.c_double),
-                ("ms_requests", C.c_double)]
+                ("ms_requests", C.c_double), ("rgba_f32", C.c_void_p)]
 
 
 class VoxServerStats(C.Structure):
@@ -237,6 +237,7 @@ def load():
     L.vox_device_alloc.argtypes = [C.c_uint64]
     L.vox_device_upload.argtypes = [vp, C.c_uint64, vp, C.c_uint64]
     L.vox_device_free.argtypes = [vp]
+    L.vox_device_copy.argtypes = [vp, vp, C.c_uint64]
     # batch
     L.vox_batch_create.restype = vp
     L.vox_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32]
@@ -295,6 +296,8 @@ def load():
     L.vox_server_attach.argtypes = [vp, vp, C.POINTER(C.c_uint32)]
     L.vox_server_detach.argtypes = [vp, vp]
     L.vox_server_frame_multi.argtypes = [vp, C.POINTER(C.c_void_p), C.c_uint32]
+    L.vox_composite.argtypes = [C.POINTER(C.c_void_p), C.c_uint32, vp, C.c_int32, C.c_int32,
+                                C.c_int32]
     L.vox_server_get_stats.argtypes = [vp, C.POINTER(VoxServerStats)]
     L.vox_server_latencies.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
     L.vox_server_state.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),

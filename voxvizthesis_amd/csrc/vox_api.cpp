@@ -1370,6 +1370,12 @@ int vox_device_upload(void* dst, uint64_t offset, const void* src, uint64_t byte
                : 0;
 }
 
+int vox_device_copy(void* dst, const void* src, uint64_t bytes) {
+    if (!dst || !src || (bytes & 15u) || ((uintptr_t)dst & 15u) || ((uintptr_t)src & 15u)) return 0;
+    launch_copy_u128(nullptr, src, dst, bytes / 16);
+    return hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess ? 1 : 0;
+}
+
 int vox_device_free(void* p) {
     if (!p) return 1;
     if (hipDeviceSynchronize() != hipSuccess) return 0;

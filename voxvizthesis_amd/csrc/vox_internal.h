@@ -177,6 +177,8 @@ void launch_occupancy(hipStream_t st, const VoxParams& P, uint64_t nActive, cons
 void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
                         const float* nrm, uint32_t* outCol, float* outNrm);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
+// Copy n16 16-byte words.
+void launch_copy_u128(hipStream_t st, const void* src, void* dst, uint64_t n16);
 // dst row k = src row idx[k] - base, w floats per row.
 void launch_gather_rows(hipStream_t st, const float* src, const uint32_t* idx, uint64_t n,
                         uint32_t base, uint32_t w, float* dst);

@@ -1368,6 +1368,13 @@ __global__ void k_tri_bounds(const float* __restrict__ v, uint64_t n, int32_t* _
         }
 }
 
+// Plain copy of n 16-byte words (the reference point of traffic counters).
+__global__ void k_copy_u128(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -1398,6 +1405,12 @@ void launch_tri_bounds(hipStream_t st, const float* v, uint64_t n, int32_t* out)
     uint64_t blocks = (n + kBlock - 1) / kBlock;
     if (blocks > 4096) blocks = 4096;
     k_tri_bounds<<<(unsigned)blocks, kBlock, 0, st>>>(v, n, out);
+}
+
+void launch_copy_u128(hipStream_t st, const void* src, void* dst, uint64_t n16) {
+    if (n16 == 0) return;
+    k_copy_u128<<<4096, kBlock, 0, st>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst),
+                                         n16);
 }
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {

@@ -153,7 +153,7 @@ __device__ __forceinline__ uint2 fetch_node(const uint32_t* __restrict__ tex, co
 __global__ void __launch_bounds__(256)
 k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __restrict__ pool,
           const float* __restrict__ jitter, uint32_t* __restrict__ rgbaOut,
-          uint32_t* __restrict__ usageOut) {
+          float4* __restrict__ rgbaF, uint32_t* __restrict__ usageOut) {
     __shared__ float u8[256];
     u8[threadIdx.x] = (float)threadIdx.x / 255.0f;  // blockDim.x == 256
     __syncthreads();
@@ -347,6 +347,7 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
         }
         rgbaOut[pix] = o;
     }
+    if (rgbaF) rgbaF[pix] = make_float4(dst[0], dst[1], dst[2], dst[3]);  // FragColor
     uint4* u4 = reinterpret_cast<uint4*>(usageOut + pix * kMaxUsage);
     u4[0] = make_uint4(usage[0], usage[1], usage[2], usage[3]);
     u4[1] = make_uint4(usage[4], usage[5], usage[6], usage[7]);
@@ -404,6 +405,21 @@ k_requests(const uint32_t* __restrict__ usage, uint32_t W, int32_t rx0, int32_t 
     }
 }
 
+// One caster's pixel rectangle blended into the frame:
+// glBlendFunc(GL_SRC_ALPHA, GL_ONE_MINUS_SRC_ALPHA) on an RGBA32F target,
+// out = src * src.a + dst * (1 - src.a) for all four channels.
+__global__ void k_blend(const float4* __restrict__ src, int32_t W, int32_t x0, int32_t y0,
+                        int32_t x1, int32_t y1, float4* __restrict__ dst) {
+    const int32_t i = x0 + (int32_t)(blockIdx.x * 16u + (threadIdx.x & 15u));
+    const int32_t j = y0 + (int32_t)(blockIdx.y * 16u + (threadIdx.x >> 4));
+    if (i >= x1 || j >= y1) return;
+    const uint64_t p = (uint64_t)j * W + i;
+    const float4 s = src[p], d = dst[p];
+    const float a = s.w, k = 1.0f - s.w;
+    dst[p] = make_float4(s.x * a + d.x * k, s.y * a + d.y * k, s.z * a + d.z * k,
+                         s.w * a + d.w * k);
+}
+
 // Jitter table: 32 x 32 values in [0, 1), the k-th from the k+1-th output of
 // splitmix64 seeded with `seed` (the state advances by the golden-ratio
 // increment, so output k+1 is the finaliser of seed + (k+1) * increment).
@@ -426,10 +442,12 @@ struct vox_raycaster {
     uint32_t maxReq = 0, nreq = 0, nreqTotal = 0, tag = 0;
     uint32_t ndim[3] = {0, 0, 0};
     hipStream_t st = nullptr;  // the ray caster's own stream
-    DevBuf rgba, usage, req, reqTime, counts, stamp, jitter;
+    DevBuf rgba, rgbaF, usage, req, reqTime, counts, stamp, jitter;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     hipEvent_t done = nullptr;  // the last frame's request list is complete
-    bool timingPending = false, countsPending = false, haveJitter = false;
+    hipEvent_t read = nullptr;  // a composite has read the last frame
+    int32_t rect[4] = {0, 0, 0, 0};  // pixels the last frame wrote
+    bool timingPending = false, countsPending = false, haveJitter = false, readPending = false;
     uint64_t jitterSeed = 0;
     double msRay = 0, msReq = 0;
     std::stringstream err;
@@ -575,6 +593,7 @@ vox_raycaster* vox_raycaster_create(vox_octree* tree, int32_t max_width, int32_t
     const uint64_t npix = (uint64_t)max_width * max_height;
     const uint64_t ntex = (uint64_t)rc->ndim[0] * rc->ndim[1] * rc->ndim[2];
     if (!rc->rgba.ensure(4 * npix, rc->err, "rgba") ||
+        !rc->rgbaF.ensure(16 * npix, rc->err, "rgba f32") ||
         !rc->usage.ensure(4 * npix * kMaxUsage, rc->err, "usage") ||
         !rc->req.ensure(4ull * max_requests, rc->err, "requests") ||
         !rc->reqTime.ensure(8ull * max_requests, rc->err, "request times") ||
@@ -585,7 +604,8 @@ vox_raycaster* vox_raycaster_create(vox_octree* tree, int32_t max_width, int32_t
     launch_fill_u32(rc->st, rc->counts.as<uint32_t>(), 2, 0u);
     for (auto& e : rc->ev)
         if (!rc->check(hipEventCreate(&e), "hipEventCreate")) return rc;
-    if (!rc->check(hipEventCreateWithFlags(&rc->done, hipEventDisableTiming), "hipEventCreate") ||
+    if (!rc->check(hipEventCreateWithFlags(&rc->read, hipEventDisableTiming), "hipEventCreate") ||
+        !rc->check(hipEventCreateWithFlags(&rc->done, hipEventDisableTiming), "hipEventCreate") ||
         !rc->check(hipEventRecord(rc->done, rc->st), "event record") ||
         !rc->check(hipStreamSynchronize(rc->st), "synchronize"))
         return rc;
@@ -599,7 +619,9 @@ void vox_raycaster_destroy(vox_raycaster* rc) {
     for (auto e : rc->ev)
         if (e) (void)hipEventDestroy(e);
     if (rc->done) (void)hipEventDestroy(rc->done);
+    if (rc->read) (void)hipEventDestroy(rc->read);
     rc->rgba.release();
+    rc->rgbaF.release();
     rc->usage.release();
     rc->req.release();
     rc->reqTime.release();
@@ -649,6 +671,10 @@ int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycas
     // the node texture and brick pool as the last writer (octree build or
     // brick server frame) left them
     if (!rc->check(hipStreamWaitEvent(st, o.ready, 0), "wait for the node texture")) return 0;
+    // a composite still reading the last frame
+    if (rc->readPending && !rc->check(hipStreamWaitEvent(st, rc->read, 0), "wait for a composite"))
+        return 0;
+    rc->readPending = false;
     if (!rc->haveJitter || rc->jitterSeed != prm->jitter_seed) {
         k_jitter<<<4, 256, 0, st>>>(prm->jitter_seed, rc->jitter.as<float>());
         rc->jitterSeed = prm->jitter_seed;
@@ -666,7 +692,11 @@ int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycas
     (void)hipEventRecord(rc->ev[0], st);
     dim3 grid((rw + 15) / 16, (rh + 15) / 16);
     k_raycast<<<grid, 256, 0, st>>>(U, tex, o.poolCol, rc->jitter.as<float>(), rc->rgba.as<uint32_t>(),
-                                    rc->usage.as<uint32_t>());
+                                    rc->rgbaF.as<float4>(), rc->usage.as<uint32_t>());
+    rc->rect[0] = U.rx0;
+    rc->rect[1] = U.ry0;
+    rc->rect[2] = U.rx1;
+    rc->rect[3] = U.ry1;
     (void)hipEventRecord(rc->ev[1], st);
     const uint64_t nent = npix * kMaxUsage;
     const unsigned rgrid = (unsigned)std::min<uint64_t>((nent + 255) / 256, 8192);
@@ -681,6 +711,37 @@ int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycas
         return 0;
     rc->timingPending = true;
     rc->countsPending = true;
+    return 1;
+}
+
+int vox_composite(vox_raycaster* const* rcs, uint32_t n, float* dst, int32_t width,
+                  int32_t height, int32_t clear) {
+    if (!dst || width <= 0 || height <= 0 || (n && !rcs)) return 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        vox_raycaster* rc = rcs[k];
+        if (!rc || !rc->valid) return 0;
+        if (rc->W != width || rc->H != height)
+            return rc->fail("composite: the frame size differs from the ray caster's last frame"), 0;
+    }
+    // the blends run in the given order on the first caster's stream (the
+    // null stream when there is none), after every caster's frame
+    hipStream_t st = n ? rcs[0]->st : nullptr;
+    for (uint32_t k = 0; k < n; ++k)
+        if (hipStreamWaitEvent(st, rcs[k]->done, 0) != hipSuccess) return 0;
+    if (clear) launch_fill_u32(st, reinterpret_cast<uint32_t*>(dst), 4ull * width * height, 0u);
+    for (uint32_t k = 0; k < n; ++k) {
+        const int32_t* r = rcs[k]->rect;
+        if (r[2] <= r[0] || r[3] <= r[1]) continue;
+        dim3 grid((unsigned)(r[2] - r[0] + 15) / 16, (unsigned)(r[3] - r[1] + 15) / 16);
+        k_blend<<<grid, 256, 0, st>>>(rcs[k]->rgbaF.as<float4>(), width, r[0], r[1], r[2], r[3],
+                                      reinterpret_cast<float4*>(dst));
+    }
+    if (hipGetLastError() != hipSuccess) return 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (hipEventRecord(rcs[k]->read, st) != hipSuccess) return 0;
+        rcs[k]->readPending = true;
+    }
+    if (!n && hipStreamSynchronize(st) != hipSuccess) return 0;
     return 1;
 }
 
@@ -700,6 +761,7 @@ int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out) {
     out->width = rc->W;
     out->height = rc->H;
     out->rgba = rc->rgba.as<uint32_t>();
+    out->rgba_f32 = rc->rgbaF.as<float>();
     out->usage = rc->usage.as<uint32_t>();
     out->requests = rc->req.as<uint32_t>();
     out->nrequests = rc->nreq;

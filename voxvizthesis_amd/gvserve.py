@@ -120,6 +120,14 @@ class RayCaster:
         self._L.vox_raycaster_get_frame(self._h, C.byref(v))
         return self._copy(v.rgba, v.width * v.height).reshape(v.height, v.width)
 
+    def rgba_f32(self) -> np.ndarray:
+        """The last frame's FragColor, (H, W, 4) float32."""
+        v = self.view()
+        a = np.empty(v.width * v.height * 4, np.float32)
+        if a.size and not self._L.vox_pool_copy_to_host(a.ctypes.data, v.rgba_f32, a.nbytes):
+            raise RuntimeError("copy failed")
+        return a.reshape(v.height, v.width, 4)
+
     def requests(self) -> np.ndarray:
         v = _lib.VoxFrameView()
         self._L.vox_raycaster_get_frame(self._h, C.byref(v))
@@ -135,6 +143,16 @@ class RayCaster:
             self.close()
         except Exception:
             pass
+
+
+def composite(raycasters, dst_ptr: int, width: int, height: int, clear: bool = True):
+    """vox_composite: blend the casters' last frames, in the order given, into
+    the RGBA32F device buffer dst_ptr (width x height x 4 floats)."""
+    L = _lib.load()
+    arr = (C.c_void_p * max(len(raycasters), 1))(*[r._h for r in raycasters])
+    if not L.vox_composite(arr, len(raycasters), dst_ptr, int(width), int(height),
+                           1 if clear else 0):
+        raise RuntimeError("vox_composite failed")
 
 
 class BrickServer:

@@ -12,8 +12,9 @@ to near, and trees not seen for 100 frames are unloaded
 
 Here the "load" of a tile is its voxelization on the device: the scene's
 triangles stay resident in HBM, the tiler's ordered per-tile lists pick each
-tile's triangles (vox_add_group_gather), and vox_generate_async runs the
-tile on a worker thread and its own stream while frames go on.  A finished
+tile's triangles (vox_add_group_gather), and a loader thread runs the tile's
+allocation, gather and generate on the handle's own stream while frames go
+on (the library releases the interpreter lock in every call).  A finished
 tile becomes an octree on the shared pool (its root brick admitted through
 the LRU like any other) attached to one brick server; every loaded visible
 tree gets a ray cast per camera stream, and one server frame admits all of
@@ -30,7 +31,7 @@ import numpy as np
 
 from . import _lib
 from .gvpool import BrickPool, OctTree
-from .gvserve import BrickServer, RayCaster, box_rect, camera, raycast_params
+from .gvserve import BrickServer, RayCaster, box_rect, camera, composite, raycast_params
 from .tiler import PagedOctTreeGenerator
 from .voxelizer import Voxelizer
 
@@ -153,6 +154,8 @@ class TilePager:
         self.near, self.far, self.options = float(near), float(far), int(options)
         self.frame_index = 0
         self._cast = []
+        from concurrent.futures import ThreadPoolExecutor
+        self._loader = ThreadPoolExecutor(max_workers=self.max_loading)
         self.load_ms = []
         self.tiles_loaded = 0
         # scene resident in HBM, one set of arrays per mesh
@@ -165,6 +168,10 @@ class TilePager:
                     np.asarray(m.vnormals, np.float32)),
                 uv=_lib.DeviceArray(np.asarray(m.uvs, np.float32)) if tex else None,
                 img=_lib.DeviceArray(np.asarray(m.image, np.uint8)) if tex else None))
+        # per stream, the RGBA32F frame the trees are blended into
+        self.frames = [_lib.DeviceArray(np.zeros((self.H, self.W, 4), np.float32))
+                       for _ in range(self.streams)]
+        self._frame_dims = [(0, 0)] * self.streams
         # per-tile triangle lists on the device, and each mesh's range in them
         g = self.gen.grid()
         self._tiler = self._L.vox_tiler_create()
@@ -219,8 +226,10 @@ class TilePager:
         self.max_trees = int(max_trees)
 
     # --- loading -----------------------------------------------------------
-    def _start_load(self, tr: PagedTree):
-        import time
+    def _load(self, tr: PagedTree):
+        """A loader thread's work for one tile: voxel memory, the tile's
+        groups gathered from the resident scene, setup and generate on the
+        handle's own stream.  Returns (voxelizer, status)."""
         vz = Voxelizer((self.N,) * 3, (self.B,) * 3, None, options=self.options)
         ok = vz.initCuda() and vz.allocateVoxelMemory() and \
             vz.setVoxelizationParams(tr.origin, self.gen.vs)
@@ -243,26 +252,35 @@ class TilePager:
             idx = C.c_void_p(self._tile_tris + 4 * a)
             ok = ok and bool(self._L.vox_add_group_gather(vz._h, C.byref(grp), idx, b - a,
                                                           int(self.gen.group_off[gi])))
-        ok = ok and vz.allocateTriangleMemory() and vz.generateAsync()
-        if not ok:
+        ok = ok and vz.allocateTriangleMemory() and vz.computeEdgesFaceNormalsAndBounds()
+        st = vz.generateVoxelsAndOctTree(None) if ok else 0
+        if st == 1:
+            vz.releaseWorkMemory()
+        return vz, st
+
+    def _start_load(self, tr: PagedTree):
+        import time
+        tr._t0 = time.perf_counter()
+        tr._job = self._loader.submit(self._load, tr)
+        tr.state = "loading"
+
+    def _finish_load(self, tr: PagedTree):
+        import time
+        vz, st = tr._job.result()
+        tr._job = None
+        tr.vz = vz
+        if st == 0:
             err = vz.getErrorMessage()
             vz.close()
+            tr.vz = None
+            tr.state = "unloaded"
             raise RuntimeError(f"tile {tr.x},{tr.y},{tr.z}: {err}")
-        tr.vz = vz
-        tr.state = "loading"
-        tr._t0 = time.perf_counter()
-
-    def _finish_load(self, tr: PagedTree, st: int):
-        import time
-        if st == 0:
-            raise RuntimeError(f"tile {tr.x},{tr.y},{tr.z}: {tr.vz.getErrorMessage()}")
         self.load_ms.append((time.perf_counter() - tr._t0) * 1e3)
         if st == -1:                       # no triangle overlapped the grid
-            tr.vz.close()
+            vz.close()
             tr.vz = None
             tr.state = "empty"
             return
-        tr.vz.releaseWorkMemory()
         tr.octree = OctTree(self.pool, tr.vz, root_brick=False)
         if sum(t.state == "loaded" for t in self.trees) >= self.max_trees:
             self._unload(min((t for t in self.trees if t.state == "loaded"),
@@ -297,10 +315,8 @@ class TilePager:
         f = self.frame_index
         # finished voxelizations become trees
         for tr in self.trees:
-            if tr.state == "loading":
-                st = tr.vz.wait(False)
-                if st != 2:
-                    self._finish_load(tr, st)
+            if tr.state == "loading" and tr._job.done():
+                self._finish_load(tr)
         # cull per stream (CullVisitor)
         drawn, requests = [], {}
         for cm in cams:
@@ -335,7 +351,7 @@ class TilePager:
         # ray casts of every stream's visible trees, one server frame
         rcs, out = [], []
         for k, (cm, trees) in enumerate(zip(cams, drawn)):
-            row = []
+            row, cast = [], []
             for tr in trees:
                 rect = box_rect(cm, tr.box_min, tr.box_max)
                 if rect == ():
@@ -345,7 +361,11 @@ class TilePager:
                 rc.frame(camera(cm["position"], cm["look"], cm["up"], cm["fov_y_deg"],
                                 cm["width"], cm["height"], f), prm, wait=False)
                 rcs.append(rc)
+                cast.append(rc)
                 row.append((tr, rc))
+            # far to near into the stream's frame (the renderer's blending)
+            composite(cast, self.frames[k].ptr, cm["width"], cm["height"], clear=True)
+            self._frame_dims[k] = (int(cm["width"]), int(cm["height"]))
             out.append(row)
         self._cast = rcs
         if serve:
@@ -359,21 +379,34 @@ class TilePager:
             self.server.frame(self._cast, wait=False)
         self._cast = []
 
+    def image(self, k: int = 0) -> np.ndarray:
+        """Stream k's composited frame, (H, W, 4) float32."""
+        W, H = self._frame_dims[k]
+        a = np.empty((H, W, 4), np.float32)
+        if a.size and not self._L.vox_pool_copy_to_host(a.ctypes.data, self.frames[k].ptr,
+                                                        a.nbytes):
+            raise RuntimeError("copy failed")
+        return a
+
     def drain(self):
         """Wait for every running voxelization and finish its load."""
         for tr in self.trees:
             if tr.state == "loading":
-                self._finish_load(tr, tr.vz.wait(True))
+                tr._job.result()
+                self._finish_load(tr)
 
     def close(self):
-        for tr in self.trees:
+        for tr in getattr(self, "trees", []):
             if tr.state == "loading":
-                tr.vz.wait(True)
-                tr.vz.close()
-                tr.vz = None
+                vz, _ = tr._job.result()
+                vz.close()
+                tr._job = None
                 tr.state = "unloaded"
             elif tr.state == "loaded":
                 self._unload(tr)
+        if getattr(self, "_loader", None) is not None:
+            self._loader.shutdown(wait=True)
+            self._loader = None
         if getattr(self, "server", None) is not None:
             self.server.close()
             self.server = None
@@ -383,6 +416,9 @@ class TilePager:
         if getattr(self, "_tiler", None):
             self._L.vox_tiler_destroy(self._tiler)
             self._tiler = None
+        for fr in getattr(self, "frames", []):
+            fr.close()
+        self.frames = []
         for d in getattr(self, "_dev", []):
             for a in d.values():
                 if a is not None:

@@ -5,11 +5,13 @@
         --known-fetch BYTES --known-write BYTES [--json out.json]
 
 `traffic` turns the FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch) into
-HBM bytes per launch.  The WRITE_SIZE scale is the ratio the counter shows on
-a kernel of known size in the same run: the largest k_fill_u32 dispatch writes
-exactly the S^3 occupancy bitmap.  That ratio also applies to FETCH_SIZE (same
-counter units) unless --known-fetch names the bytes the largest
-k_scan_reduce dispatch reads (only meaningful when its input is not cached).
+HBM bytes per launch.  With --known-copy BYTES (a run of bench.py
+--copy-calibration) each counter is scaled by the ratio it shows on the plain
+copy kernel k_copy_u128, which reads and writes exactly BYTES.  Otherwise
+the WRITE_SIZE scale comes from the largest k_fill_u32 dispatch, which
+writes exactly the S^3 occupancy bitmap (--known-write), and applies to
+FETCH_SIZE too unless --known-fetch names the bytes the largest
+k_scan_reduce dispatch reads.
 """
 from __future__ import annotations
 
@@ -47,11 +49,15 @@ def stats(path: str) -> str:
     return "\n".join(out)
 
 
-def traffic(fetch_csv, write_csv, known_fetch, known_write, steps=1):
+def traffic(fetch_csv, write_csv, known_fetch, known_write, steps=1, known_copy=0.0):
     f = load_counter(fetch_csv, "FETCH_SIZE")
     w = load_counter(write_csv, "WRITE_SIZE")
-    fw = known_write / max(w["k_fill_u32"])
-    ff = known_fetch / max(f["k_scan_reduce"]) if known_fetch else fw
+    if known_copy:
+        fw = known_copy / max(w["k_copy_u128"])
+        ff = known_copy / max(f["k_copy_u128"])
+    else:
+        fw = known_write / max(w["k_fill_u32"])
+        ff = known_fetch / max(f["k_scan_reduce"]) if known_fetch else fw
     res = {"fetch_scale": ff, "write_scale": fw, "kernels": {}}
     for k in sorted(set(f) | set(w)):
         fs = f.get(k, [])
@@ -72,13 +78,15 @@ def main():
     b.add_argument("fetch")
     b.add_argument("write")
     b.add_argument("--known-fetch", type=float, default=0.0)
-    b.add_argument("--known-write", type=float, required=True)
+    b.add_argument("--known-write", type=float, default=0.0)
+    b.add_argument("--known-copy", type=float, default=0.0)
     b.add_argument("--json")
     args = ap.parse_args()
     if args.cmd == "stats":
         print(stats(args.csv))
     else:
-        res = traffic(args.fetch, args.write, args.known_fetch, args.known_write)
+        res = traffic(args.fetch, args.write, args.known_fetch, args.known_write,
+                      known_copy=args.known_copy)
         if args.json:
             with open(args.json, "w") as fh:
                 json.dump(res, fh, indent=1)
