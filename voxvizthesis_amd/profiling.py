@@ -22,7 +22,10 @@ import json
 
 
 def kname(s: str) -> str:
-    return s.replace("voxmi::(anonymous namespace)::", "").split("(")[0]
+    for prefix in ("voxmi::(anonymous namespace)::", "(anonymous namespace)::"):
+        if s.startswith(prefix):
+            s = s[len(prefix):]
+    return s.split("(")[0]
 
 
 def load_counter(path: str, counter: str):
@@ -40,8 +43,10 @@ def load_counter(path: str, counter: str):
     return by_kernel
 
 
-def stats(path: str) -> str:
-    rows = list(csv.DictReader(open(path)))
+def stats(path: str, prefix: str = "") -> str:
+    """Markdown table of a kernel-stats CSV; prefix keeps only the kernels
+    whose short name starts with it (k_srv, k_ray ...)."""
+    rows = [r for r in csv.DictReader(open(path)) if kname(r["Name"]).startswith(prefix)]
     out = ["| kernel | calls | avg us | total % |", "|---|---|---|---|"]
     for r in rows:
         out.append(f"| {kname(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
@@ -74,6 +79,7 @@ def main():
     sub = ap.add_subparsers(dest="cmd", required=True)
     a = sub.add_parser("stats")
     a.add_argument("csv")
+    a.add_argument("--prefix", default="")
     b = sub.add_parser("traffic")
     b.add_argument("fetch")
     b.add_argument("write")
@@ -83,7 +89,7 @@ def main():
     b.add_argument("--json")
     args = ap.parse_args()
     if args.cmd == "stats":
-        print(stats(args.csv))
+        print(stats(args.csv, args.prefix))
     else:
         res = traffic(args.fetch, args.write, args.known_fetch, args.known_write,
                       known_copy=args.known_copy)
