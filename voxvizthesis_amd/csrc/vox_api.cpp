@@ -707,7 +707,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     }
     if (!h->occ0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "occupancy words") ||
         !h->hit0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "hit words") ||
-        !h->work.ensure((uint64_t)l0.active * 4, h->err, "tile work list") ||
+        !h->work.ensure((uint64_t)l0.active * 16, h->err, "tile work list") ||
         !h->workCount.ensure(4, h->err, "tile work count") ||
         !h->rec.ensure((uint64_t)l0.active * kTileVox * 32, h->err, "hit records") ||
         !h->recCount.ensure(4, h->err, "hit record count"))
@@ -735,7 +735,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         h->run("fill", [&] { launch_fill_u32(st, h->hit0.as<uint32_t>(), l0.active * kTileWords, 0u); });
         h->run("group_tiles", [&] {
             launch_group_tiles(st, P, g, l0.active, l0.list.as<int32_t>(), h->binStart.as<uint32_t>(),
-                               h->work.as<int32_t>(), h->workCount.as<uint32_t>(),
+                               h->work.as<uint4>(), h->workCount.as<uint32_t>(),
                                h->recCount.as<uint32_t>());
         });
         h->run("voxelize_g" + std::to_string(g), [&] {
@@ -743,7 +743,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
                             h->verts.as<float>(), h->fnorm.as<float>(), h->bbox.as<uint32_t>(),
                             h->binStart.as<uint32_t>(), h->binTri.as<uint32_t>(),
                             h->occ0.as<uint32_t>(), h->hit0.as<uint32_t>(), satc,
-                            h->work.as<int32_t>(), h->workCount.as<uint32_t>(), h->rec.as<uint4>(),
+                            h->work.as<uint4>(), h->workCount.as<uint32_t>(), h->rec.as<uint4>(),
                             h->recCount.as<uint32_t>(), h->persistentBlocks);
         });
         h->run("colour_g" + std::to_string(g), [&] {

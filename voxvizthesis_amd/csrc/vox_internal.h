@@ -145,10 +145,10 @@ void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int
                      uint64_t nActive, const int32_t* list, const float* verts,
                      const float* fnorm, const uint32_t* bbox, const uint32_t* binStart,
                      const uint32_t* binTri, uint32_t* occ, uint32_t* hit,
-                     unsigned long long* satCounter, int32_t* work, uint32_t* workCount,
+                     unsigned long long* satCounter, uint4* work, uint32_t* workCount,
                      uint4* rec, uint32_t* recCount, unsigned persistentBlocks);
 void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nActive,
-                        const int32_t* list, const uint32_t* binStart, int32_t* work,
+                        const int32_t* list, const uint32_t* binStart, uint4* work,
                         uint32_t* workCount, uint32_t* recCount);
 void launch_colour(hipStream_t st, const VoxParams& P, const GroupArgs& G, int init,
                    const uint4* rec, const uint32_t* recCount, const int32_t* list,

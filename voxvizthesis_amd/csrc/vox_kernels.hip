@@ -504,7 +504,7 @@ __device__ __forceinline__ void stage_pair(const float (&v)[3][3], const float (
 }
 
 __global__ __launch_bounds__(kBlock) void k_voxelize(
-    VoxParams P, GroupArgs G, int g, int init, const int32_t* __restrict__ work,
+    VoxParams P, GroupArgs G, int g, int init, const uint4* __restrict__ work,
     const uint32_t* __restrict__ workCount, const int32_t* __restrict__ list,
     const float* __restrict__ verts, const float* __restrict__ fnorm,
     const uint32_t* __restrict__ bbox, const uint32_t* __restrict__ binStart,
@@ -528,14 +528,15 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
     uint32_t localTests = 0;
     // Persistent grid over the tiles that hold triangles of this group.
     for (uint32_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
-    const uint64_t a = (uint64_t)work[wi];
-    const uint64_t tlin = (uint64_t)list[a];
+    // one load: (active tile, tile index, first bin entry, entries)
+    const uint4 wk = work[wi];
+    const uint64_t a = wk.x;
+    const uint64_t tlin = wk.y;
     uint32_t tx, ty, tz;
     tile_xyz(tlin, P.TG, tx, ty, tz);
     const uint32_t ox = tx * kTile, oy = ty * kTile, oz = tz * kTile;
-    const uint64_t key = (uint64_t)g * P.numTiles + tlin;
-    const uint32_t bStart = binStart[key];
-    const uint32_t nb = binStart[key + 1] - bStart;
+    const uint32_t bStart = wk.z;
+    const uint32_t nb = wk.w;
     for (int v = tid; v < kTileVox; v += kBlock) {
         s_cnt[v] = 0u;
         s_lo0[v] = s_lo1[v] = s_lo2[v] = 0x7fffffff;
@@ -546,11 +547,22 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
 
     for (uint32_t c0 = 0; c0 < nb; c0 += kStage) {
         const uint32_t m = (nb - c0) < (uint32_t)kStage ? (nb - c0) : (uint32_t)kStage;
+        // Thread j < m takes triangle j of the stage: its clipped voxel box,
+        // and its staged constants -- per axis pair XY, ZX, YZ the edge-test
+        // constants of TriangleOverlapsVoxelOnAxis (VK:93-144), with the
+        // plane-test constants of TrianglePlaneOverlapsVoxel (VK:17-53) --
+        // so the per-voxel work needs 2 mul + 2 add per edge.  Bounds,
+        // vertices and normal are loaded together (one dependent round trip
+        // after the bin entry).
         uint32_t w = 0;
         if ((uint32_t)tid < m) {
             const uint32_t t = binTri[bStart + c0 + tid];
             const uint32_t bx = bbox[3 * (size_t)t], by = bbox[3 * (size_t)t + 1],
                            bz = bbox[3 * (size_t)t + 2];
+            const float* q = verts + 9 * (size_t)t;
+            const float v[3][3] = {{q[0], q[1], q[2]}, {q[3], q[4], q[5]}, {q[6], q[7], q[8]}};
+            const float* fq = fnorm + 3 * (size_t)t;
+            const float n[3] = {fq[0], fq[1], fq[2]};
             int32_t x0 = (int32_t)(bx & 0xffffu) - (int32_t)ox, x1 = (int32_t)(bx >> 16) - (int32_t)ox;
             int32_t y0 = (int32_t)(by & 0xffffu) - (int32_t)oy, y1 = (int32_t)(by >> 16) - (int32_t)oy;
             uint32_t zlo = (bz & 0xffffu) > G.zStart ? (bz & 0xffffu) : G.zStart;
@@ -567,51 +579,33 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
                 s_box[tid] = 0u;
             }
             s_tid[tid] = t;
+            float* st = &s_tri[tid * kTriStride];
+            st[0] = n[0]; st[1] = n[1]; st[2] = n[2];
+            const float D = dot3(f3{-n[0], -n[1], -n[2]}, f3{v[0][0], v[0][1], v[0][2]});
+            st[3] = D;
+            stage_pair<0, 1, 2>(v, n, P.offP, P.dP, st + 4);
+            stage_pair<2, 0, 1>(v, n, P.offP, P.dP, st + 13);
+            stage_pair<1, 2, 0>(v, n, P.offP, P.dP, st + 22);
+            // Plane-test bounds: the 8 corner distances are dist1 plus the
+            // sums of the negative / positive parts of n_i * dP_i; thr bounds
+            // the fp32 error of any corner distance (256 unit roundoffs of the
+            // largest |n . p| + |D| over the grid).
+            float neg = 0.0f, pos = 0.0f, M = fabsf(D);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float o = n[k] * P.dP[k];
+                neg += fminf(o, 0.0f);
+                pos += fmaxf(o, 0.0f);
+                M += fabsf(n[k]) * (fabsf(P.offP[k]) + (float)(P.S[k] + 1u) * fabsf(P.dP[k]));
+            }
+            st[31] = neg;
+            st[32] = pos;
+            st[33] = M * (1.0f / 65536.0f);
         }
         uint32_t total;
-        uint32_t off = block_excl_scan(w, s_w, &total);   // barrier: s_tid visible
+        uint32_t off = block_excl_scan(w, s_w, &total);   // barrier
         if ((uint32_t)tid < m) s_off[tid] = off;
         if (tid == 0) s_off[m] = total;
-        // Per (triangle, axis pair): the edge-test constants of
-        // TriangleOverlapsVoxelOnAxis (VK:93-144), pairs XY, ZX, YZ; pair 0
-        // also holds the plane-test constants of TrianglePlaneOverlapsVoxel
-        // (VK:17-53).  The per-voxel work then needs 2 mul + 2 add per edge.
-        // Items ordered pair-major so that a wavefront mostly shares one pair.
-        for (uint32_t it = tid; it < 3u * m; it += kBlock) {
-            const uint32_t pr = (it >= m ? 1u : 0u) + (it >= 2u * m ? 1u : 0u);
-            const uint32_t j = it - pr * m;
-            const uint32_t t = s_tid[j];
-            const float* q = verts + 9 * (size_t)t;
-            const float v[3][3] = {{q[0], q[1], q[2]}, {q[3], q[4], q[5]}, {q[6], q[7], q[8]}};
-            const float* fq = fnorm + 3 * (size_t)t;
-            const float n[3] = {fq[0], fq[1], fq[2]};
-            float* st = &s_tri[j * kTriStride];
-            if (pr == 0u) {
-                st[0] = n[0]; st[1] = n[1]; st[2] = n[2];
-                const float D = dot3(f3{-n[0], -n[1], -n[2]}, f3{v[0][0], v[0][1], v[0][2]});
-                st[3] = D;
-                stage_pair<0, 1, 2>(v, n, P.offP, P.dP, st + 4);
-                // Plane-test bounds: the 8 corner distances are dist1 plus
-                // the sums of the negative / positive parts of n_i * dP_i;
-                // thr bounds the fp32 error of any corner distance (256 unit
-                // roundoffs of the largest |n . p| + |D| over the grid).
-                float neg = 0.0f, pos = 0.0f, M = fabsf(D);
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float o = n[k] * P.dP[k];
-                    neg += fminf(o, 0.0f);
-                    pos += fmaxf(o, 0.0f);
-                    M += fabsf(n[k]) * (fabsf(P.offP[k]) + (float)(P.S[k] + 1u) * fabsf(P.dP[k]));
-                }
-                st[31] = neg;
-                st[32] = pos;
-                st[33] = M * (1.0f / 65536.0f);
-            } else if (pr == 1u) {
-                stage_pair<2, 0, 1>(v, n, P.offP, P.dP, st + 13);
-            } else {
-                stage_pair<1, 2, 0>(v, n, P.offP, P.dP, st + 22);
-            }
-        }
         __syncthreads();
         // Each thread takes a contiguous range of (triangle, voxel) items: one
         // search for its first triangle, then an incremental walk over the
@@ -620,6 +614,18 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
         uint32_t item = (uint32_t)tid * per;
         const uint32_t itemEnd = item + per < total ? item + per : total;
         uint32_t j = 0, bx = 0, ex = 1, ey = 1, cx = 0, cy = 0, cz = 0, jEnd = 0;
+        // The current triangle's staged constants, held in registers while
+        // the walk stays on it (LDS is read again only at a triangle change):
+        // face normal, D, plane-bound offsets and error bound, edge tests.
+        float tn0 = 0.0f, tn1 = 0.0f, tn2 = 0.0f, tD = 0.0f, tNeg = 0.0f, tPos = 0.0f, tThr = 0.0f;
+        float te[27];
+        auto load_tri = [&](uint32_t jj) {
+            const float* st = &s_tri[jj * kTriStride];
+            tn0 = st[0]; tn1 = st[1]; tn2 = st[2]; tD = st[3];
+#pragma unroll
+            for (int k = 0; k < 27; ++k) te[k] = st[4 + k];
+            tNeg = st[31]; tPos = st[32]; tThr = st[33];
+        };
         if (item < itemEnd) {
             uint32_t lo = 0, hi = m;
             while (hi - lo > 1) {
@@ -638,6 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
             cy = (rxy * s_magic[ex]) >> 16;
             cx = rxy - cy * ex;
             jEnd = s_off[j + 1];
+            load_tri(j);
         }
         for (; item < itemEnd; ++item) {
             if (item == jEnd) {
@@ -647,17 +654,17 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
                 ex = ((bx >> 9) & 7u) + 1u;
                 ey = ((bx >> 12) & 7u) + 1u;
                 cx = cy = cz = 0;
+                load_tri(j);
             }
             const uint32_t lx = (bx & 7u) + cx;
             const uint32_t ly = ((bx >> 3) & 7u) + cy;
             const uint32_t lz = ((bx >> 6) & 7u) + cz;
             if (++cx == ex) { cx = 0; if (++cy == ey) { cy = 0; ++cz; } }
-            const float* st = &s_tri[j * kTriStride];
             const float uvw[3] = {(float)(ox + lx), (float)(oy + ly), (float)(oz + lz)};
             ++localTests;
             // TrianglePlaneOverlapsVoxel (VK:17-53).
-            const f3 n = {st[0], st[1], st[2]};
-            const float D = st[3];
+            const f3 n = {tn0, tn1, tn2};
+            const float D = tD;
             const f3 p0 = {P.offP[0] + uvw[0] * P.dP[0], P.offP[1] + uvw[1] * P.dP[1],
                            P.offP[2] + uvw[2] * P.dP[2]};
             const float dist1 = dot3(n, p0) + D;
@@ -665,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
             // corners on both sides beyond the error bound (hit).  Only the
             // band where a corner may round to the other side runs the
             // reference's 8-corner products.
-            const float lo = dist1 + st[31], hi = dist1 + st[32], thr = st[33];
+            const float lo = dist1 + tNeg, hi = dist1 + tPos, thr = tThr;
             bool planeHit = false;
             if (lo < -thr && hi > thr) {
                 planeHit = true;
@@ -688,8 +695,8 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
             for (int pr = 0; pr < 3; ++pr) {
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const float* c = &st[4 + (pr * 3 + i) * 3];
-                    float ot = c[0] + uvw[A2[pr]] * c[2] + uvw[A1[pr]] * c[1];
+                    const int c = (pr * 3 + i) * 3;
+                    float ot = te[c] + uvw[A2[pr]] * te[c + 2] + uvw[A1[pr]] * te[c + 1];
                     edgeHit = edgeHit && (ot >= 0.0f);
                 }
             }
@@ -807,14 +814,18 @@ __global__ __launch_bounds__(kBlock) void k_colour(
 }
 
 // Tiles holding triangles of group g, appended in any order (the tiles are
-// independent, so the order does not change any result).
+// independent, so the order does not change any result), each with what
+// the voxelize pass needs first: (active tile, tile index, first bin entry,
+// entries), so a workgroup starts a tile with one load.
 __global__ void k_group_tiles(VoxParams P, int g, uint64_t nActive, const int32_t* __restrict__ list,
-                              const uint32_t* __restrict__ binStart, int32_t* __restrict__ work,
+                              const uint32_t* __restrict__ binStart, uint4* __restrict__ work,
                               uint32_t* __restrict__ workCount) {
     const uint64_t a = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nActive) return;
-    const uint64_t key = (uint64_t)g * P.numTiles + (uint64_t)list[a];
-    if (binStart[key + 1] > binStart[key]) work[atomicAdd(workCount, 1u)] = (int32_t)a;
+    const uint32_t tlin = (uint32_t)list[a];
+    const uint64_t key = (uint64_t)g * P.numTiles + tlin;
+    const uint32_t b0 = binStart[key], b1 = binStart[key + 1];
+    if (b1 > b0) work[atomicAdd(workCount, 1u)] = make_uint4((uint32_t)a, tlin, b0, b1 - b0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1450,7 +1461,7 @@ void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int
                      uint64_t nActive, const int32_t* list, const float* verts,
                      const float* fnorm, const uint32_t* bbox, const uint32_t* binStart,
                      const uint32_t* binTri, uint32_t* occ, uint32_t* hit,
-                     unsigned long long* satCounter, int32_t* work, uint32_t* workCount,
+                     unsigned long long* satCounter, uint4* work, uint32_t* workCount,
                      uint4* rec, uint32_t* recCount, unsigned persistentBlocks) {
     if (nActive == 0) return;
     unsigned blocks = (unsigned)(nActive < persistentBlocks ? nActive : persistentBlocks);
@@ -1458,7 +1469,7 @@ void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int
                                           binStart, binTri, occ, hit, rec, recCount, satCounter);
 }
 void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nActive,
-                        const int32_t* list, const uint32_t* binStart, int32_t* work,
+                        const int32_t* list, const uint32_t* binStart, uint4* work,
                         uint32_t* workCount, uint32_t* recCount) {
     launch_fill_u32(st, workCount, 1, 0u);
     launch_fill_u32(st, recCount, 1, 0u);
