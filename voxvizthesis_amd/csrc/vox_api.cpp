@@ -705,12 +705,19 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
             !l.nrm.ensure((uint64_t)act[k] * kTileVox * 12, h->err, "tile normals"))
             return 0;
     }
+    if (kRecSegs * rec_seg_cap(l0.active, std::min<uint64_t>(l0.active, h->persistentBlocks)) >=
+        (1ull << 32))
+        return h->fail("more hit records than 32-bit indices address; use fewer persistent "
+                       "workgroups or a smaller grid"), 0;
     if (!h->occ0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "occupancy words") ||
         !h->hit0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "hit words") ||
         !h->work.ensure((uint64_t)l0.active * 16, h->err, "tile work list") ||
         !h->workCount.ensure(4, h->err, "tile work count") ||
-        !h->rec.ensure((uint64_t)l0.active * kTileVox * 32, h->err, "hit records") ||
-        !h->recCount.ensure(4, h->err, "hit record count"))
+        !h->rec.ensure(kRecSegs * rec_seg_cap(l0.active, std::min<uint64_t>(l0.active,
+                                                                          h->persistentBlocks)) *
+                           32,
+                       h->err, "hit records") ||
+        !h->recCount.ensure(4 * kRecSegs, h->err, "hit record counts"))
         return 0;
     h->run("fill", [&] { launch_fill_u32(st, h->occ0.as<uint32_t>(), l0.active * kTileWords, 0u); });
     if (firstOverlap >= 0) {
@@ -747,7 +754,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
                             h->recCount.as<uint32_t>(), h->persistentBlocks);
         });
         h->run("colour_g" + std::to_string(g), [&] {
-            launch_colour(st, P, ga, g == firstOverlap ? 1 : 0, h->rec.as<uint4>(),
+            launch_colour(st, P, ga, g == firstOverlap ? 1 : 0, l0.active, h->rec.as<uint4>(),
                           h->recCount.as<uint32_t>(), l0.list.as<int32_t>(), h->verts.as<float>(),
                           h->fnorm.as<float>(), l0.col.as<uint32_t>(), l0.qn.as<uint32_t>(),
                           l0.nrm.as<float>(), h->persistentBlocks);

@@ -44,6 +44,16 @@ constexpr int kTileVox = 512;
 constexpr int kTileWords = 16;      // 512 bits per tile
 constexpr int kBlock = 256;         // threads per workgroup for tile kernels
 constexpr int kFillDepth = 19;      // terrain fill reach (VoxelizerKernels.cu 888)
+// Hit records of the voxelize pass go to kRecSegs segments, workgroup b
+// appending to segment b % kRecSegs with its own counter (no single hot
+// counter).  Workgroup b takes tiles b, b + grid, ...: at most
+// ceil(nActive / grid) tiles of 512 voxels, and a segment serves at most
+// ceil(grid / kRecSegs) workgroups, so this many records always fit.
+constexpr uint32_t kRecSegs = 64;
+inline uint64_t rec_seg_cap(uint64_t nActive, uint64_t grid) {
+    if (grid == 0) return 0;
+    return ((nActive + grid - 1) / grid) * ((grid + kRecSegs - 1) / kRecSegs) * 512ull;
+}
 constexpr int kFillTiles = 3;       // ceil(19 / 8)
 
 struct VoxParams {
@@ -151,9 +161,9 @@ void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nAct
                         const int32_t* list, const uint32_t* binStart, uint4* work,
                         uint32_t* workCount, uint32_t* recCount);
 void launch_colour(hipStream_t st, const VoxParams& P, const GroupArgs& G, int init,
-                   const uint4* rec, const uint32_t* recCount, const int32_t* list,
-                   const float* verts, const float* fnorm, uint32_t* col, uint32_t* qn, float* nrm,
-                   unsigned persistentBlocks);
+                   uint64_t nActive, const uint4* rec, const uint32_t* recCount,
+                   const int32_t* list, const float* verts, const float* fnorm, uint32_t* col,
+                   uint32_t* qn, float* nrm, unsigned persistentBlocks);
 void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t nActive,
                  const int32_t* list, const int32_t* tileIdx, const uint32_t* binStart,
                  uint32_t* col, uint32_t* qn, float* nrm, const uint32_t* hit);

@@ -509,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
     const float* __restrict__ verts, const float* __restrict__ fnorm,
     const uint32_t* __restrict__ bbox, const uint32_t* __restrict__ binStart,
     const uint32_t* __restrict__ binTri, uint32_t* __restrict__ occ, uint32_t* __restrict__ hit,
-    uint4* __restrict__ rec, uint32_t* __restrict__ recCount,
+    uint4* __restrict__ rec, uint32_t* __restrict__ recCount, uint64_t segCap,
     unsigned long long* __restrict__ satCounter) {
     __shared__ uint32_t s_cnt[kTileVox];
     __shared__ int32_t s_lo0[kTileVox], s_lo1[kTileVox], s_lo2[kTileVox], s_hi[kTileVox];
@@ -743,7 +743,8 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
     if (tid == 0) {
         uint32_t acc = 0;
         for (int w = 0; w < kTileWords; ++w) { const uint32_t c = s_hitoff[w]; s_hitoff[w] = acc; acc += c; }
-        s_recBase = acc ? atomicAdd(recCount, acc) : 0u;
+        const uint32_t seg = blockIdx.x % kRecSegs;
+        s_recBase = acc ? (uint32_t)(seg * segCap) + atomicAdd(&recCount[seg], acc) : 0u;
     }
     __syncthreads();
     for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
@@ -782,15 +783,31 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
 // new hits, one thread per voxel (grid-stride).
 __global__ __launch_bounds__(kBlock) void k_colour(
     VoxParams P, GroupArgs G, int init, const uint4* __restrict__ rec,
-    const uint32_t* __restrict__ recCount, const int32_t* __restrict__ list,
+    const uint32_t* __restrict__ recCount, uint64_t segCap, const int32_t* __restrict__ list,
     const float* __restrict__ verts, const float* __restrict__ fnorm, uint32_t* __restrict__ col,
     uint32_t* __restrict__ qn, float* __restrict__ nrm) {
     __shared__ float s_inv255[256];
+    __shared__ uint32_t s_pre[kRecSegs + 1];   // record segments: prefix of their counts
     fill_inv255(s_inv255);
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < kRecSegs; ++k) {
+            s_pre[k] = acc;
+            acc += recCount[k];
+        }
+        s_pre[kRecSegs] = acc;
+    }
     __syncthreads();
-    const uint32_t n = *recCount;
+    const uint32_t n = s_pre[kRecSegs];
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const uint4 r0 = rec[2 * (size_t)i], r1 = rec[2 * (size_t)i + 1];
+        uint32_t lo = 0, hi = kRecSegs;   // segment of the i-th record
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_pre[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t ri = lo * segCap + (i - s_pre[lo]);
+        const uint4 r0 = rec[2 * ri], r1 = rec[2 * ri + 1];
         const uint64_t a = r0.x;
         const uint32_t v = r0.y & 511u, ns = (r0.y >> 9) & 7u, wasHit = (r0.y >> 12) & 1u;
         const uint32_t cnt = r0.z;
@@ -821,11 +838,25 @@ __global__ void k_group_tiles(VoxParams P, int g, uint64_t nActive, const int32_
                               const uint32_t* __restrict__ binStart, uint4* __restrict__ work,
                               uint32_t* __restrict__ workCount) {
     const uint64_t a = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nActive) return;
-    const uint32_t tlin = (uint32_t)list[a];
-    const uint64_t key = (uint64_t)g * P.numTiles + tlin;
-    const uint32_t b0 = binStart[key], b1 = binStart[key + 1];
-    if (b1 > b0) work[atomicAdd(workCount, 1u)] = make_uint4((uint32_t)a, tlin, b0, b1 - b0);
+    uint32_t tlin = 0, b0 = 0, b1 = 0;
+    if (a < nActive) {
+        tlin = (uint32_t)list[a];
+        const uint64_t key = (uint64_t)g * P.numTiles + tlin;
+        b0 = binStart[key];
+        b1 = binStart[key + 1];
+    }
+    // one atomic per wave on the shared counter; lanes take consecutive slots
+    const bool take = a < nActive && b1 > b0;
+    const uint64_t mask = __ballot(take);
+    if (!mask) return;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(workCount, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (take)
+        work[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] =
+            make_uint4((uint32_t)a, tlin, b0, b1 - b0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1466,23 +1497,26 @@ void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int
     if (nActive == 0) return;
     unsigned blocks = (unsigned)(nActive < persistentBlocks ? nActive : persistentBlocks);
     k_voxelize<<<blocks, kBlock, 0, st>>>(P, G, g, init, work, workCount, list, verts, fnorm, bbox,
-                                          binStart, binTri, occ, hit, rec, recCount, satCounter);
+                                          binStart, binTri, occ, hit, rec, recCount,
+                                          rec_seg_cap(nActive, blocks), satCounter);
 }
 void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nActive,
                         const int32_t* list, const uint32_t* binStart, uint4* work,
                         uint32_t* workCount, uint32_t* recCount) {
     launch_fill_u32(st, workCount, 1, 0u);
-    launch_fill_u32(st, recCount, 1, 0u);
+    launch_fill_u32(st, recCount, kRecSegs, 0u);
     if (nActive == 0) return;
     k_group_tiles<<<grid_for(nActive, kBlock), kBlock, 0, st>>>(P, g, nActive, list, binStart,
                                                                  work, workCount);
 }
 void launch_colour(hipStream_t st, const VoxParams& P, const GroupArgs& G, int init,
-                   const uint4* rec, const uint32_t* recCount, const int32_t* list,
-                   const float* verts, const float* fnorm, uint32_t* col, uint32_t* qn, float* nrm,
-                   unsigned persistentBlocks) {
-    k_colour<<<persistentBlocks, kBlock, 0, st>>>(P, G, init, rec, recCount, list, verts, fnorm,
-                                                  col, qn, nrm);
+                   uint64_t nActive, const uint4* rec, const uint32_t* recCount,
+                   const int32_t* list, const float* verts, const float* fnorm, uint32_t* col,
+                   uint32_t* qn, float* nrm, unsigned persistentBlocks) {
+    const uint64_t vblocks = nActive < persistentBlocks ? nActive : persistentBlocks;
+    k_colour<<<persistentBlocks, kBlock, 0, st>>>(P, G, init, rec, recCount,
+                                                  rec_seg_cap(nActive, vblocks), list, verts,
+                                                  fnorm, col, qn, nrm);
 }
 void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t nActive,
                  const int32_t* list, const int32_t* tileIdx, const uint32_t* binStart,
