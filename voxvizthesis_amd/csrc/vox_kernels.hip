@@ -168,6 +168,38 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int l) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// The active lanes of a wave grouped by key: the group's first lane
+// (leader), a lane's rank in its group (lanes in ascending order) and, in
+// the leader, the group's size. Ballots and shuffles only; one pass per
+// distinct key.
+struct BinGroup {
+    int leader;
+    uint32_t rank, count;
+    bool lead;
+};
+__device__ __forceinline__ BinGroup bin_group(bool act, uint64_t key) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint64_t below = (1ull << lane) - 1ull;
+    BinGroup r{lane, 0u, 0u, false};
+    uint64_t pend = __ballot(act);
+    while (pend) {
+        const int l = __ffsll((unsigned long long)pend) - 1;
+        const uint64_t lk = shfl_u64(key, l);
+        const bool in = act && key == lk;
+        const uint64_t same = __ballot(in);
+        if (in) {
+            r.leader = l;
+            r.rank = (uint32_t)__popcll(same & below);
+        }
+        if (lane == l) {
+            r.count = (uint32_t)__popcll(same);
+            r.lead = true;
+        }
+        pend &= ~same;
+    }
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // setup: ComputeEdgesFaceNormalsAndBounds (VoxelizerKernels.cu 230-347) plus
 // the count of 8^3 tiles each triangle's bounds touch, per (group, tile) bin.
@@ -207,18 +239,13 @@ __global__ void k_setup(VoxParams P, GroupArgs G, int g, const float* __restrict
     uint32_t z1 = mx[2] < G.zEnd ? mx[2] : G.zEnd;
     if (z0 > z1) return;
     TileWalk w(P, g, mn[0] >> 3, mx[0] >> 3, mn[1] >> 3, mx[1] >> 3, z0 >> 3, z1 >> 3);
-    // Neighbouring triangles mostly share tiles: lanes holding the same bin
-    // add once, with the count of the group.
+    // Neighbouring triangles mostly share tiles: each step groups the lanes
+    // holding the same bin (ballots only), then the first lane of every group
+    // adds the group's count, all groups in one atomic instruction.
     bool act = true;
     while (__ballot(act)) {
-        uint64_t pend = __ballot(act);
-        while (pend) {
-            const int l = __ffsll((unsigned long long)pend) - 1;
-            const uint64_t lk = shfl_u64(w.key, l);
-            const uint64_t same = __ballot(act && w.key == lk);
-            if ((int)(threadIdx.x & 63u) == l) atomicAdd(&binCount[lk], (uint32_t)__popcll(same));
-            pend &= ~same;
-        }
+        const BinGroup bg = bin_group(act, w.key);
+        if (act && bg.lead) atomicAdd(&binCount[w.key], bg.count);
         if (act) act = w.next();
     }
 }
@@ -235,24 +262,16 @@ __global__ void k_scatter(VoxParams P, GroupArgs G, int g, const uint32_t* __res
     if (z0 > z1) return;
     TileWalk w(P, g, (bx & 0xffffu) >> 3, (bx >> 16) >> 3, (by & 0xffffu) >> 3, (by >> 16) >> 3,
                z0 >> 3, z1 >> 3);
-    // One atomic per group of lanes holding the same bin; lanes take
-    // consecutive places after the group's base.
-    const int lane = (int)(threadIdx.x & 63u);
-    const uint64_t below = (1ull << lane) - 1ull;
+    // As in k_setup, one returning atomic instruction per step reserves the
+    // places of every group of lanes holding the same bin; the lanes of a
+    // group take consecutive places after its base.
     bool act = true;
     while (__ballot(act)) {
-        uint64_t pend = __ballot(act);
-        while (pend) {
-            const int l = __ffsll((unsigned long long)pend) - 1;
-            const uint64_t lk = shfl_u64(w.key, l);
-            const uint64_t same = __ballot(act && w.key == lk);
-            uint32_t base = 0;
-            if (lane == l) base = atomicAdd(&binFill[lk], (uint32_t)__popcll(same));
-            base = __shfl(base, l);
-            if (act && w.key == lk)
-                binTri[binStart[lk] + base + (uint32_t)__popcll(same & below)] = (uint32_t)t;
-            pend &= ~same;
-        }
+        const BinGroup bg = bin_group(act, w.key);
+        uint32_t r = 0;
+        if (act && bg.lead) r = atomicAdd(&binFill[w.key], bg.count);
+        const uint32_t base = (uint32_t)__shfl((int)r, bg.leader);
+        if (act) binTri[binStart[w.key] + base + bg.rank] = (uint32_t)t;
         if (act) act = w.next();
     }
 }
