@@ -279,7 +279,7 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
     return out
 
 
-def cpu_baseline(sample_n: int):
+def cpu_baseline(sample_n: int, all_core_n: int = 256):
     import oracle
     from voxvizthesis_amd import scenes
     sc = scenes.scene_full_octree(cells=sample_n, N=sample_n, B=16)
@@ -297,6 +297,34 @@ def cpu_baseline(sample_n: int):
         "sample": (f"oracle (scalar C++ port of the reference logic) on the same scene "
                    f"(seeds 2/3, {sample_n + 1}^2 terrain + 2,000 rocks) at {sample_n}^3, "
                    f"16^3 bricks, {dt:.1f} s on one host core"),
+        "seconds": round(dt, 3),
+        "all_core": cpu_all_core(all_core_n),
+    }
+
+
+def cpu_all_core(n: int):
+    """The oracle run concurrently on every host core the process may use (at
+    most 16), one independent copy of a smaller sample per thread: aggregate
+    throughput. The oracle's C++ runs outside the interpreter lock."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+    from voxvizthesis_amd import scenes
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    sc = scenes.scene_full_octree(cells=n, N=n, B=16)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        st = list(ex.map(lambda _: oracle.run(sc, keep_mips=False).status, range(cores)))
+    dt = time.perf_counter() - t0
+    if any(s != 1 for s in st):
+        raise RuntimeError("oracle all-core sample failed")
+    return {
+        "value": round(cores * n ** 3 / dt / 1e9, 6),
+        "unit": "Gvoxels/s",
+        "cores": cores,
+        "sample": (f"{cores} concurrent oracle runs of the same scene at {n}^3 "
+                   f"({n + 1}^2 terrain + 2,000 rocks, 16^3 bricks), {dt:.1f} s wall"),
         "seconds": round(dt, 3),
     }
 
@@ -487,6 +515,9 @@ def main():
         cb = cpu_baseline(args.cpu_sample)
         out["cpu_baseline"] = cb
         out["gpu_vs_cpu"] = round(out["value"] / cb["value"], 1)
+        out["gpu_vs_cpu_all_core"] = round(out["value"] / cb["all_core"]["value"], 1)
+        log(f"[bench] cpu: {cb['value']} Gvoxels/s on one core, "
+            f"{cb['all_core']['value']} on {cb['all_core']['cores']}")
     vz.close()
     print(json.dumps(out))
 

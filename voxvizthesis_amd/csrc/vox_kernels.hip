@@ -982,23 +982,39 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
             const uint64_t st = ((uint64_t)(bz >> 3) * src.TG[1] + (by >> 3)) * src.TG[0] + (bx >> 3);
             const int32_t sa = src.tileIdx[st];
             if (sa >= 0) {
-                const uint32_t xs[2] = {bx & 7u, x1 & 7u}, ys[2] = {by & 7u, y1 & 7u},
-                               zs[2] = {bz & 7u, z1 & 7u};
-                uint32_t vid[8];
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    vid[i] = (zs[(i >> 2) & 1] * 8u + ys[(i >> 1) & 1]) * 8u + xs[i & 1];
+                // bx is even, so (bx, bx + 1) sit together at an 8-byte
+                // aligned place of the tile: one 8-byte load per (z, y) row
+                // and plane. When bx + 1 is past the level's edge the pair's
+                // first voxel stands for both (x1 == bx).
+                const uint32_t ys[2] = {by & 7u, y1 & 7u}, zs[2] = {bz & 7u, z1 & 7u};
+                const bool xdup = x1 == bx;
                 const uint64_t cb = (uint64_t)sa * kTileVox;
                 const uint64_t nb0 = (uint64_t)sa * 3 * kTileVox;
-                f4 avg = unpack4(colSrc[cb + vid[0]], s_inv255);
-                f3 an = {nrmSrc[nb0 + vid[0]], nrmSrc[nb0 + kTileVox + vid[0]],
-                         nrmSrc[nb0 + 2 * kTileVox + vid[0]]};
+                uint2 cp[4];
+                float2 np[3][4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t row = (zs[j >> 1] * 8u + ys[j & 1]) * 8u + (bx & 7u);
+                    cp[j] = *reinterpret_cast<const uint2*>(colSrc + cb + row);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        np[c][j] = *reinterpret_cast<const float2*>(nrmSrc + nb0 + c * kTileVox + row);
+                    if (xdup) {
+                        cp[j].y = cp[j].x;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) np[c][j].y = np[c][j].x;
+                    }
+                }
+                // the sum in the order (z, y, x), x fastest
+                f4 avg = unpack4(cp[0].x, s_inv255);
+                f3 an = {np[0][0].x, np[1][0].x, np[2][0].x};
 #pragma unroll
                 for (int i = 1; i < 8; ++i) {
-                    f4 cur = unpack4(colSrc[cb + vid[i]], s_inv255);
+                    const int j = i >> 1;
+                    f4 cur = unpack4((i & 1) ? cp[j].y : cp[j].x, s_inv255);
                     avg.x += cur.x; avg.y += cur.y; avg.z += cur.z; avg.w += cur.w;
-                    an = add3(an, f3{nrmSrc[nb0 + vid[i]], nrmSrc[nb0 + kTileVox + vid[i]],
-                                     nrmSrc[nb0 + 2 * kTileVox + vid[i]]});
+                    an = add3(an, (i & 1) ? f3{np[0][j].y, np[1][j].y, np[2][j].y}
+                                          : f3{np[0][j].x, np[1][j].x, np[2][j].x});
                 }
                 const float oneOverEight = 1.0f / 8.0f;
                 avg.w *= oneOverEight;
@@ -1086,64 +1102,69 @@ __device__ __forceinline__ void node_span(uint32_t X, uint32_t B, uint32_t b, ui
     if (hi > (int32_t)Rd - 1) hi = (int32_t)Rd - 1;
 }
 
+// One wave per active tile, kBlock / 64 tiles per workgroup: lane l holds the
+// voxels (l & 7, l >> 3, z) for z = 0..7 in registers (eight coalesced
+// 256-byte loads). The wave then walks the nodes whose stored bricks reach
+// the tile (at most 8 for B >= 8): each node's sample is one wave-uniform
+// load, compared with the lane's voxels inside that brick, and one ballot
+// decides the node's flag. No LDS, no barrier.
+constexpr int kNcTiles = kBlock / 64;
 __global__ __launch_bounds__(kBlock) void k_node_compare(
-    LevelMap lm, const int32_t* __restrict__ list, const uint32_t* __restrict__ col, uint32_t Rd,
-    uint32_t Bx, uint32_t By, uint32_t Bz, uint32_t bx, uint32_t by, uint32_t bz, uint32_t xLimit,
-    const uint32_t* __restrict__ cc, uint32_t* __restrict__ flag) {
-    // at most ((8 + B) / B + 1)^3 nodes touch a tile: 216 for B = 2
-    __shared__ uint32_t s_flag[216];
-    const int tid = threadIdx.x;
-    const uint64_t a = blockIdx.x;
-    const uint64_t tlin = (uint64_t)list[a];
+    LevelMap lm, const int32_t* __restrict__ list, uint64_t nActive,
+    const uint32_t* __restrict__ col, uint32_t Rd, uint32_t Bx, uint32_t By, uint32_t Bz,
+    uint32_t bx, uint32_t by, uint32_t bz, uint32_t xLimit, const uint32_t* __restrict__ cc,
+    uint32_t* __restrict__ flag) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint64_t a = (uint64_t)blockIdx.x * kNcTiles + (threadIdx.x >> 6);
+    if (a >= nActive) return;   // whole waves
     uint32_t tx, ty, tz;
-    tile_xyz(tlin, lm.TG, tx, ty, tz);
+    tile_xyz((uint64_t)list[a], lm.TG, tx, ty, tz);
     const uint32_t B3[3] = {Bx, By, Bz}, b3[3] = {bx, by, bz};
     const uint32_t o3[3] = {tx * kTile, ty * kTile, tz * kTile};
-    // nodes whose bricks hold each tile coordinate, per axis
-    __shared__ int32_t s_slo[3][kTile], s_shi[3][kTile];
-    if (tid < 3 * kTile) {
-        const int ax = tid / kTile, l = tid % kTile;
-        int32_t lo, hi;
-        node_span(o3[ax] + (uint32_t)l, B3[ax], b3[ax], Rd, lo, hi);
-        s_slo[ax][l] = lo;
-        s_shi[ax][l] = hi;
-    }
-    if (tid < 216) s_flag[tid] = 0u;
-    __syncthreads();
+    // nodes whose bricks reach the tile, per axis
     int32_t blo[3], bhi[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        blo[ax] = s_slo[ax][0];
-        bhi[ax] = s_shi[ax][kTile - 1];
+        int32_t lo, hi;
+        node_span(o3[ax], B3[ax], b3[ax], Rd, blo[ax], hi);
+        node_span(o3[ax] + kTile - 1, B3[ax], b3[ax], Rd, lo, bhi[ax]);
     }
-    const int32_t ex = bhi[0] - blo[0] + 1, ey = bhi[1] - blo[1] + 1;
-    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
-        const uint32_t v = (uint32_t)(pass * kBlock + tid);
-        const uint32_t lx = v & 7u, ly = (v >> 3) & 7u, lz = v >> 6;
-        const uint32_t X = o3[0] + lx, Y = o3[1] + ly, Z = o3[2] + lz;
-        if (X >= xLimit || X >= lm.dims[0] || Y >= lm.dims[1] || Z >= lm.dims[2]) continue;
-        const int32_t xl = s_slo[0][lx], xh = s_shi[0][lx];
-        const int32_t yl = s_slo[1][ly], yh = s_shi[1][ly];
-        const int32_t zl = s_slo[2][lz], zh = s_shi[2][lz];
-        if (xl > xh || yl > yh || zl > zh) continue;
-        const uint32_t c = col[a * kTileVox + v];
-        for (int32_t nz = zl; nz <= zh; ++nz)
-            for (int32_t ny = yl; ny <= yh; ++ny)
-                for (int32_t nx = xl; nx <= xh; ++nx) {
-                    const uint64_t n = ((uint64_t)nz * Rd + ny) * Rd + nx;
-                    if (c != cc[n]) {
-                        const int32_t li = ((nz - blo[2]) * ey + (ny - blo[1])) * ex + (nx - blo[0]);
-                        s_flag[li] = 1u;
-                    }
-                }
+    if (blo[0] > bhi[0] || blo[1] > bhi[1] || blo[2] > bhi[2]) return;
+    const uint32_t X = o3[0] + ((uint32_t)lane & 7u), Y = o3[1] + ((uint32_t)lane >> 3);
+    const bool colOk = X < xLimit && X < lm.dims[0] && Y < lm.dims[1];
+    // z layers inside the level
+    uint32_t zIn = 0u;
+    uint32_t c[kTile];
+    const uint32_t* tc = col + a * kTileVox + (uint64_t)lane;
+#pragma unroll
+    for (int lz = 0; lz < kTile; ++lz) {
+        c[lz] = tc[lz * 64];
+        if (o3[2] + (uint32_t)lz < lm.dims[2]) zIn |= 1u << lz;
     }
-    __syncthreads();
-    if (tid < 216) {
-        const int32_t ez = bhi[2] - blo[2] + 1;
-        if (tid < ex * ey * ez && s_flag[tid]) {
-            const int32_t nx = blo[0] + tid % ex, ny = blo[1] + (tid / ex) % ey,
-                          nz = blo[2] + tid / (ex * ey);
-            flag[((uint64_t)nz * Rd + ny) * Rd + nx] = 1u;
+    for (int32_t nz = blo[2]; nz <= bhi[2]; ++nz) {
+        // stored brick of node n on an axis: [n*B + b - 1, n*B + b + B]
+        const int32_t z0 = nz * (int32_t)Bz + (int32_t)bz - 1 - (int32_t)o3[2];
+        const int32_t z1 = z0 + (int32_t)Bz + 1;
+        uint32_t zm = 0u;
+#pragma unroll
+        for (int lz = 0; lz < kTile; ++lz)
+            if (lz >= z0 && lz <= z1) zm |= 1u << lz;
+        zm &= zIn;
+        if (!zm) continue;
+        for (int32_t ny = blo[1]; ny <= bhi[1]; ++ny) {
+            const int32_t y0 = ny * (int32_t)By + (int32_t)by - 1;
+            const bool inY = (int32_t)Y >= y0 && (int32_t)Y <= y0 + (int32_t)By + 1;
+            for (int32_t nx = blo[0]; nx <= bhi[0]; ++nx) {
+                const int32_t x0 = nx * (int32_t)Bx + (int32_t)bx - 1;
+                const bool inX = (int32_t)X >= x0 && (int32_t)X <= x0 + (int32_t)Bx + 1;
+                const uint64_t n = ((uint64_t)nz * Rd + ny) * Rd + nx;
+                const uint32_t s = cc[n];
+                uint32_t ne = 0u;
+#pragma unroll
+                for (int lz = 0; lz < kTile; ++lz) ne |= (c[lz] != s ? 1u : 0u) << lz;
+                const bool d = colOk && inX && inY && (ne & zm) != 0u;
+                if (__ballot(d) && lane == 0) flag[n] = 1u;
+            }
         }
     }
 }
@@ -1559,9 +1580,9 @@ void launch_node_type(hipStream_t st, const LevelMap& lm, uint64_t nActive, cons
                                                              border[0], border[1], border[2],
                                                              xLimit, cc, flag);
     if (nActive == 0) return;
-    k_node_compare<<<(unsigned)nActive, kBlock, 0, st>>>(lm, list, col, Rd, B[0], B[1], B[2],
-                                                         border[0], border[1], border[2], xLimit,
-                                                         cc, flag);
+    k_node_compare<<<(unsigned)((nActive + kNcTiles - 1) / kNcTiles), kBlock, 0, st>>>(
+        lm, list, nActive, col, Rd, B[0], B[1], B[2], border[0], border[1], border[2], xLimit, cc,
+        flag);
 }
 void launch_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
                          const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode,
