@@ -62,7 +62,8 @@ struct LevelView {
 struct DepthBufs {
     uint32_t Rd = 0;
     uint64_t nodes = 0, bricks = 0;
-    DevBuf types, cc, flag, excl, bidx, brickNode, bcol, bnrm;
+    DevBuf types, cc, bidx, brickNode, bcol, bnrm;
+    uint64_t off = 0;   // first node of the depth in the all-depth flag array
     DevBuf ccol, cnrm;  // block-compressed bricks (compressed output)
 };
 
@@ -110,6 +111,9 @@ struct vox_ctx {
     // binning
     uint64_t numBins = 0, pairs = 0;
     DevBuf binCount, binStart, binTri, scanTmp;
+    // node flags of every depth, depth-major, and their exclusive scan
+    DevBuf nodeFlag, nodeExcl;
+    uint64_t allNodes = 0;
     // levels
     VoxParams P{};
     std::vector<LevelBufs> lv;
@@ -413,17 +417,21 @@ int vox_allocate_voxel_memory(vox_ctx* h) {
         l.map.tileIdx = l.tileIdx.as<int32_t>();
     }
     h->tree.assign(h->L, DepthBufs());
+    h->allNodes = 0;
     for (int d = 0; d < h->L; ++d) {
         DepthBufs& t = h->tree[d];
         t.Rd = 1u << d;
         t.nodes = (uint64_t)t.Rd * t.Rd * t.Rd;
+        t.off = h->allNodes;
+        h->allNodes += t.nodes;
         if (!t.types.ensure(t.nodes, h->err, "node types") ||
             !t.cc.ensure(t.nodes * 4, h->err, "node colours") ||
-            !t.flag.ensure(t.nodes * 4, h->err, "node flags") ||
-            !t.excl.ensure((t.nodes + 1) * 4, h->err, "node scan") ||
             !t.bidx.ensure(t.nodes * 4, h->err, "brick index"))
             return 0;
     }
+    if (!h->nodeFlag.ensure(h->allNodes * 4, h->err, "node flags") ||
+        !h->nodeExcl.ensure((h->allNodes + 1) * 4, h->err, "node scan"))
+        return 0;
     if (!h->counters.ensure(4096, h->err, "counters")) return 0;
     if (!h->sat.ensure(8, h->err, "sat counter")) return 0;
     h->voxMem = true;
@@ -439,10 +447,11 @@ void vox_deallocate_voxel_memory(vox_ctx* h) {
     }
     h->lv.clear();
     for (auto& t : h->tree) {
-        t.types.release(); t.cc.release(); t.flag.release(); t.excl.release(); t.bidx.release();
+        t.types.release(); t.cc.release(); t.bidx.release();
         t.brickNode.release(); t.bcol.release(); t.bnrm.release();
     }
     h->tree.clear();
+    h->nodeFlag.release(); h->nodeExcl.release();
     h->occ0.release(); h->hit0.release(); h->sat.release(); h->bitmap.release();
     h->work.release(); h->workCount.release(); h->rec.release(); h->recCount.release();
     h->counters.release(); h->binCount.release(); h->binStart.release(); h->binTri.release();
@@ -676,11 +685,8 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         h->run("scan", [&] { scan(h, l.mark.as<uint32_t>(), l.excl.as<uint32_t>(), l.map.numTiles); });
         h->run("compact", [&] {
             launch_compact(st, l.map.numTiles, l.mark.as<uint32_t>(), l.excl.as<uint32_t>(),
-                           l.tileIdx.as<int32_t>(), l.list.as<int32_t>());
+                           l.tileIdx.as<int32_t>(), l.list.as<int32_t>(), ctr + k);
         });
-        if (!h->check(hipMemcpyAsync(ctr + k, l.excl.as<uint32_t>() + l.map.numTiles, 4,
-                                     hipMemcpyDeviceToDevice, st), "tile count"))
-            return 0;
     }
     uint32_t act[VOX_MAX_LEVELS] = {0};
     if (h->replay) {
@@ -791,13 +797,22 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         uint32_t xLimit = type_x_limit(l.map.dims[0], border[0], t.Rd * h->B[0], Ckyz, byz, fyz);
         h->run("node_type", [&] {
             launch_node_type(st, l.map, l.active, l.list, l.col, t.Rd, h->B, border, xLimit,
-                             t.cc.as<uint32_t>(), t.flag.as<uint32_t>());
+                             t.cc.as<uint32_t>(), h->nodeFlag.as<uint32_t>() + t.off);
         });
-        h->run("scan", [&] { scan(h, t.flag.as<uint32_t>(), t.excl.as<uint32_t>(), t.nodes); });
-        if (!h->check(hipMemcpyAsync(ctr + 32 + d, t.excl.as<uint32_t>() + t.nodes, 4,
-                                     hipMemcpyDeviceToDevice, st), "brick count"))
-            return 0;
         for (int a = 0; a < 3; ++a) border[a] = std::max(border[a] >> 1, 1u);
+    }
+    // One scan over the flags of every depth; a depth's brick addresses are
+    // its scan values less the value at its first node.
+    h->run("scan", [&] {
+        scan(h, h->nodeFlag.as<uint32_t>(), h->nodeExcl.as<uint32_t>(), h->allNodes);
+    });
+    {
+        uint64_t offs[VOX_MAX_LEVELS + 1];
+        for (int d = 0; d < L; ++d) offs[d] = h->tree[d].off;
+        offs[L] = h->allNodes;
+        h->run("scan", [&] {
+            launch_range_counts(st, h->nodeExcl.as<uint32_t>(), offs, L, ctr + 32);
+        });
     }
     uint32_t nb[VOX_MAX_LEVELS] = {0};
     if (h->replay) {
@@ -820,9 +835,9 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
             !t.bnrm.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick normals"))
             return 0;
         h->run("brick_assign", [&] {
-            launch_brick_assign(st, t.nodes, t.flag.as<uint32_t>(), t.excl.as<uint32_t>(),
-                                t.bidx.as<uint32_t>(), t.brickNode.as<uint32_t>(),
-                                t.types.as<uint8_t>());
+            launch_brick_assign(st, t.nodes, h->nodeFlag.as<uint32_t>() + t.off,
+                                h->nodeExcl.as<uint32_t>() + t.off, t.bidx.as<uint32_t>(),
+                                t.brickNode.as<uint32_t>(), t.types.as<uint8_t>());
         });
         uint32_t bd[3] = {borders[d][0], borders[d][1], borders[d][2]};
         h->run("brick_gather", [&] {
@@ -1022,15 +1037,15 @@ void buffer_list(vox_ctx* h, std::vector<void*>& out) {
     DevBuf* top[] = {&h->verts, &h->fnorm, &h->bbox, &h->vn, &h->uv, &h->img, &h->grpFill,
                      &h->binCount, &h->binStart, &h->binTri, &h->scanTmp, &h->occ0, &h->hit0,
                      &h->sat, &h->work, &h->workCount, &h->rec, &h->recCount, &h->bitmap,
-                     &h->counters};
+                     &h->counters, &h->nodeFlag, &h->nodeExcl};
     for (DevBuf* b : top) out.push_back(b->p);
     for (auto& l : h->lv) {
         DevBuf* lb[] = {&l.mark, &l.excl, &l.tileIdx, &l.list, &l.col, &l.qn, &l.nrm};
         for (DevBuf* b : lb) out.push_back(b->p);
     }
     for (auto& t : h->tree) {
-        DevBuf* tb[] = {&t.types, &t.cc, &t.flag, &t.excl, &t.bidx, &t.brickNode,
-                        &t.bcol,  &t.bnrm, &t.ccol, &t.cnrm};
+        DevBuf* tb[] = {&t.types, &t.cc, &t.bidx, &t.brickNode, &t.bcol, &t.bnrm, &t.ccol,
+                        &t.cnrm};
         for (DevBuf* b : tb) out.push_back(b->p);
     }
 }

@@ -147,8 +147,9 @@ void launch_scatter(hipStream_t st, const VoxParams& P, const GroupArgs& G, int 
                     uint32_t* binTri);
 void launch_activity0(hipStream_t st, const VoxParams& P, int ngroups, const int32_t* grpFill,
                       const uint32_t* binStart, uint32_t* mark);
+// Also writes excl[n] (the number of marked tiles) to *count.
 void launch_compact(hipStream_t st, uint64_t n, const uint32_t* mark, const uint32_t* excl,
-                    int32_t* tileIdx, int32_t* list);
+                    int32_t* tileIdx, int32_t* list, uint32_t* count);
 void launch_mark_parent(hipStream_t st, const LevelMap& src, const LevelMap& dst,
                         uint32_t* markDst);
 void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g, int init,
@@ -173,6 +174,15 @@ void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64
 void launch_node_type(hipStream_t st, const LevelMap& lm, uint64_t nActive, const int32_t* list,
                       const uint32_t* col, uint32_t Rd, const uint32_t B[3],
                       const uint32_t border[3], uint32_t xLimit, uint32_t* cc, uint32_t* flag);
+// Range boundaries of one scan, at most kMaxLevels ranges (passed by value).
+struct RangeOffs {
+    uint64_t v[kMaxLevels + 1];
+};
+// out[d] = excl[offs[d + 1]] - excl[offs[d]] for d < n (n <= kMaxLevels).
+void launch_range_counts(hipStream_t st, const uint32_t* excl, const uint64_t* offs, int n,
+                         uint32_t* out);
+// excl points at the scan value of the depth's first node: brick addresses
+// are excl[n] - excl[0].
 void launch_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
                          const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode,
                          uint8_t* types);

@@ -302,8 +302,9 @@ __global__ void k_activity0(VoxParams P, int ngroups, const int32_t* __restrict_
 
 __global__ void k_compact(uint64_t n, const uint32_t* __restrict__ mark,
                           const uint32_t* __restrict__ excl, int32_t* __restrict__ tileIdx,
-                          int32_t* __restrict__ list) {
+                          int32_t* __restrict__ list, uint32_t* __restrict__ count) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *count = excl[n];   // the scan's total: active tiles
     if (t >= n) return;
     if (mark[t]) {
         uint32_t e = excl[t];
@@ -1169,6 +1170,14 @@ __global__ __launch_bounds__(kBlock) void k_node_compare(
     }
 }
 
+// out[d] = excl[offs[d + 1]] - excl[offs[d]]: the flags set in each range of
+// an exclusive scan's input.
+__global__ void k_range_counts(const uint32_t* __restrict__ excl, RangeOffs offs, int n,
+                               uint32_t* __restrict__ out) {
+    const int d = (int)threadIdx.x;
+    if (d < n) out[d] = excl[offs.v[d + 1]] - excl[offs.v[d]];
+}
+
 __global__ void k_brick_assign(uint64_t nodes, const uint32_t* __restrict__ flag,
                                const uint32_t* __restrict__ excl, uint32_t* __restrict__ bidx,
                                uint32_t* __restrict__ brickNode, uint8_t* __restrict__ types) {
@@ -1176,8 +1185,9 @@ __global__ void k_brick_assign(uint64_t nodes, const uint32_t* __restrict__ flag
     if (n >= nodes) return;
     types[n] = flag[n] ? 1 : 0;
     if (flag[n]) {
-        bidx[n] = excl[n];
-        brickNode[excl[n]] = (uint32_t)n;
+        const uint32_t b = excl[n] - excl[0];   // excl starts at the depth's first node
+        bidx[n] = b;
+        brickNode[b] = (uint32_t)n;
     } else {
         bidx[n] = 0xffffffffu;
     }
@@ -1520,9 +1530,12 @@ void launch_activity0(hipStream_t st, const VoxParams& P, int ngroups, const int
     k_activity0<<<grid_for(P.numTiles, kBlock), kBlock, 0, st>>>(P, ngroups, grpFill, binStart, mark);
 }
 void launch_compact(hipStream_t st, uint64_t n, const uint32_t* mark, const uint32_t* excl,
-                    int32_t* tileIdx, int32_t* list) {
-    if (n == 0) return;
-    k_compact<<<grid_for(n, kBlock), kBlock, 0, st>>>(n, mark, excl, tileIdx, list);
+                    int32_t* tileIdx, int32_t* list, uint32_t* count) {
+    if (n == 0) {
+        launch_fill_u32(st, count, 1, 0u);
+        return;
+    }
+    k_compact<<<grid_for(n, kBlock), kBlock, 0, st>>>(n, mark, excl, tileIdx, list, count);
 }
 void launch_mark_parent(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint32_t* markDst) {
     if (src.numTiles == 0) return;
@@ -1583,6 +1596,13 @@ void launch_node_type(hipStream_t st, const LevelMap& lm, uint64_t nActive, cons
     k_node_compare<<<(unsigned)((nActive + kNcTiles - 1) / kNcTiles), kBlock, 0, st>>>(
         lm, list, nActive, col, Rd, B[0], B[1], B[2], border[0], border[1], border[2], xLimit, cc,
         flag);
+}
+void launch_range_counts(hipStream_t st, const uint32_t* excl, const uint64_t* offs, int n,
+                         uint32_t* out) {
+    if (n <= 0) return;
+    RangeOffs o{};
+    for (int d = 0; d <= n && d <= kMaxLevels; ++d) o.v[d] = offs[d];
+    k_range_counts<<<1, 64, 0, st>>>(excl, o, n < kMaxLevels ? n : kMaxLevels, out);
 }
 void launch_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
                          const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode,
