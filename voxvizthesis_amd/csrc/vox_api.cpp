@@ -711,7 +711,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
             !l.nrm.ensure((uint64_t)act[k] * kTileVox * 12, h->err, "tile normals"))
             return 0;
     }
-    if (kRecSegs * rec_seg_cap(l0.active, std::min<uint64_t>(l0.active, h->persistentBlocks)) >=
+    if (kRecSegs * rec_seg_cap(l0.active, vox_grid(l0.active, h->persistentBlocks)) >=
         (1ull << 32))
         return h->fail("more hit records than 32-bit indices address; use fewer persistent "
                        "workgroups or a smaller grid"), 0;
@@ -719,8 +719,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         !h->hit0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "hit words") ||
         !h->work.ensure((uint64_t)l0.active * 16, h->err, "tile work list") ||
         !h->workCount.ensure(4, h->err, "tile work count") ||
-        !h->rec.ensure(kRecSegs * rec_seg_cap(l0.active, std::min<uint64_t>(l0.active,
-                                                                          h->persistentBlocks)) *
+        !h->rec.ensure(kRecSegs * rec_seg_cap(l0.active, vox_grid(l0.active, h->persistentBlocks)) *
                            32,
                        h->err, "hit records") ||
         !h->recCount.ensure(4 * kRecSegs, h->err, "hit record counts"))

@@ -46,13 +46,22 @@ constexpr int kBlock = 256;         // threads per workgroup for tile kernels
 constexpr int kFillDepth = 19;      // terrain fill reach (VoxelizerKernels.cu 888)
 // Hit records of the voxelize pass go to kRecSegs segments, workgroup b
 // appending to segment b % kRecSegs with its own counter (no single hot
-// counter).  Workgroup b takes tiles b, b + grid, ...: at most
-// ceil(nActive / grid) tiles of 512 voxels, and a segment serves at most
-// ceil(grid / kRecSegs) workgroups, so this many records always fit.
+// counter).  The voxelize pass takes units of kVoxTiles consecutive tiles of
+// its work list, workgroup b the units b, b + grid, ...: at most
+// ceil(units / grid) units of kVoxTiles tiles of 512 voxels, and a segment
+// serves at most ceil(grid / kRecSegs) workgroups, so this many records
+// always fit.
 constexpr uint32_t kRecSegs = 64;
+constexpr int kVoxTiles = 2;
 inline uint64_t rec_seg_cap(uint64_t nActive, uint64_t grid) {
     if (grid == 0) return 0;
-    return ((nActive + grid - 1) / grid) * ((grid + kRecSegs - 1) / kRecSegs) * 512ull;
+    const uint64_t units = (nActive + kVoxTiles - 1) / kVoxTiles;
+    return ((units + grid - 1) / grid) * kVoxTiles * ((grid + kRecSegs - 1) / kRecSegs) * 512ull;
+}
+// workgroups of the voxelize launch over nActive tiles
+inline uint64_t vox_grid(uint64_t nActive, uint64_t persistent) {
+    const uint64_t units = (nActive + kVoxTiles - 1) / kVoxTiles;
+    return units < persistent ? units : persistent;
 }
 constexpr int kFillTiles = 3;       // ceil(19 / 8)
 

@@ -523,7 +523,7 @@ __device__ __forceinline__ void stage_pair(const float (&v)[3][3], const float (
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_voxelize(
+__global__ __launch_bounds__(kBlock, 4) void k_voxelize(
     VoxParams P, GroupArgs G, int g, int init, const uint4* __restrict__ work,
     const uint32_t* __restrict__ workCount, const int32_t* __restrict__ list,
     const float* __restrict__ verts, const float* __restrict__ fnorm,
@@ -531,8 +531,10 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
     const uint32_t* __restrict__ binTri, uint32_t* __restrict__ occ, uint32_t* __restrict__ hit,
     uint4* __restrict__ rec, uint32_t* __restrict__ recCount, uint64_t segCap,
     unsigned long long* __restrict__ satCounter) {
-    __shared__ uint32_t s_cnt[kTileVox];
-    __shared__ int32_t s_lo0[kTileVox], s_lo1[kTileVox], s_lo2[kTileVox], s_hi[kTileVox];
+    // a work unit is up to kVoxTiles tiles of the group's work list
+    constexpr int kV = kVoxTiles * kTileVox, kW = kVoxTiles * kTileWords;
+    __shared__ uint32_t s_cnt[kV];
+    __shared__ int32_t s_lo0[kV], s_lo1[kV], s_lo2[kV], s_hi[kV];
     __shared__ float s_tri[kStage * kTriStride];
     __shared__ uint32_t s_box[kStage];
     __shared__ uint32_t s_tid[kStage];
@@ -540,29 +542,43 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
     __shared__ uint32_t s_w[kBlock / 64];
 
     __shared__ uint32_t s_magic[65];   // ceil(65536 / d) for d = 1..64
-    __shared__ uint32_t s_hitw[kTileWords], s_hitoff[kTileWords], s_occprev[kTileWords];
+    __shared__ uint32_t s_hitw[kW], s_hitoff[kW], s_occprev[kW];
     __shared__ uint32_t s_recBase;
+    __shared__ uint4 s_org[kVoxTiles];   // per tile of the unit: origin x, y, z, active tile
+    __shared__ uint2 s_bin[kVoxTiles];   // per tile of the unit: first bin entry, entries
     const int tid = threadIdx.x;
     if (tid >= 1 && tid <= 64) s_magic[tid] = (65536u + (uint32_t)tid - 1u) / (uint32_t)tid;
     const uint32_t nwork = *workCount;
+    const uint32_t nunits = (nwork + kVoxTiles - 1) / kVoxTiles;
     uint32_t localTests = 0;
-    // Persistent grid over the tiles that hold triangles of this group.
-    for (uint32_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
-    // one load: (active tile, tile index, first bin entry, entries)
-    const uint4 wk = work[wi];
-    const uint64_t a = wk.x;
-    const uint64_t tlin = wk.y;
-    uint32_t tx, ty, tz;
-    tile_xyz(tlin, P.TG, tx, ty, tz);
-    const uint32_t ox = tx * kTile, oy = ty * kTile, oz = tz * kTile;
-    const uint32_t bStart = wk.z;
-    const uint32_t nb = wk.w;
-    for (int v = tid; v < kTileVox; v += kBlock) {
+    // Persistent grid over units of kVoxTiles tiles that hold triangles of
+    // this group: the fixed cost of a round (staging, scan, barriers, record
+    // reservation) is paid once per unit.
+    for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
+    const uint32_t w0 = ui * kVoxTiles;
+    const uint32_t nt = nwork - w0 < (uint32_t)kVoxTiles ? nwork - w0 : (uint32_t)kVoxTiles;
+    // work entries: (active tile, tile index, first bin entry, entries)
+    uint32_t nb = 0;
+#pragma unroll
+    for (int k = 0; k < kVoxTiles; ++k)
+        if ((uint32_t)k < nt) nb += work[w0 + k].w;
+    if ((uint32_t)tid < nt) {
+        const uint4 wk = work[w0 + tid];
+        uint32_t tx, ty, tz;
+        tile_xyz(wk.y, P.TG, tx, ty, tz);
+        s_org[tid] = make_uint4(tx * kTile, ty * kTile, tz * kTile, wk.x);
+        s_bin[tid] = make_uint2(wk.z, wk.w);
+    }
+    for (int v = tid; v < kV; v += kBlock) {
         s_cnt[v] = 0u;
         s_lo0[v] = s_lo1[v] = s_lo2[v] = 0x7fffffff;
         s_hi[v] = -1;
     }
-    if (tid < kTileWords) s_occprev[tid] = init ? 0u : occ[a * kTileWords + tid];
+    if (tid < kW) {
+        const uint32_t k = (uint32_t)tid / kTileWords;
+        s_occprev[tid] = (init || k >= nt) ? 0u
+                                           : occ[(uint64_t)work[w0 + k].x * kTileWords + tid % kTileWords];
+    }
     __syncthreads();
 
     for (uint32_t c0 = 0; c0 < nb; c0 += kStage) {
@@ -576,7 +592,13 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
         // after the bin entry).
         uint32_t w = 0;
         if ((uint32_t)tid < m) {
-            const uint32_t t = binTri[bStart + c0 + tid];
+            // entry e of the unit's bins, tile 0's first: tile sel, place eb
+            uint32_t sel = 0, eb = c0 + (uint32_t)tid;
+            uint2 bn = s_bin[0];
+            while (eb >= bn.y && sel + 1u < nt) { eb -= bn.y; bn = s_bin[++sel]; }
+            const uint4 og = s_org[sel];
+            const uint32_t ox = og.x, oy = og.y, oz = og.z;
+            const uint32_t t = binTri[bn.x + eb];
             const uint32_t bx = bbox[3 * (size_t)t], by = bbox[3 * (size_t)t + 1],
                            bz = bbox[3 * (size_t)t + 2];
             const float* q = verts + 9 * (size_t)t;
@@ -594,7 +616,7 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
                 w = (uint32_t)((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1));
                 s_box[tid] = (uint32_t)x0 | ((uint32_t)y0 << 3) | ((uint32_t)z0 << 6) |
                              ((uint32_t)(x1 - x0) << 9) | ((uint32_t)(y1 - y0) << 12) |
-                             ((uint32_t)(z1 - z0) << 15);
+                             ((uint32_t)(z1 - z0) << 15) | (sel << 18);
             } else {
                 s_box[tid] = 0u;
             }
@@ -634,16 +656,17 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
         uint32_t item = (uint32_t)tid * per;
         const uint32_t itemEnd = item + per < total ? item + per : total;
         uint32_t j = 0, bx = 0, ex = 1, ey = 1, cx = 0, cy = 0, cz = 0, jEnd = 0;
-        // The current triangle's staged constants, held in registers while
-        // the walk stays on it (LDS is read again only at a triangle change):
-        // face normal, D, plane-bound offsets and error bound, edge tests.
+        uint32_t ox = 0, oy = 0, oz = 0;   // origin of the current triangle's tile
+        // The current triangle's plane-test constants, held in registers
+        // while the walk stays on it (face normal, D, plane-bound offsets and
+        // error bound); the edge-test constants are read from LDS, only for
+        // voxels that pass the plane test.
         float tn0 = 0.0f, tn1 = 0.0f, tn2 = 0.0f, tD = 0.0f, tNeg = 0.0f, tPos = 0.0f, tThr = 0.0f;
-        float te[27];
+        const float* te = s_tri;
         auto load_tri = [&](uint32_t jj) {
             const float* st = &s_tri[jj * kTriStride];
             tn0 = st[0]; tn1 = st[1]; tn2 = st[2]; tD = st[3];
-#pragma unroll
-            for (int k = 0; k < 27; ++k) te[k] = st[4 + k];
+            te = st + 4;
             tNeg = st[31]; tPos = st[32]; tThr = st[33];
         };
         if (item < itemEnd) {
@@ -657,6 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
             bx = s_box[j];
             ex = ((bx >> 9) & 7u) + 1u;
             ey = ((bx >> 12) & 7u) + 1u;
+            { const uint4 og = s_org[bx >> 18]; ox = og.x; oy = og.y; oz = og.z; }
             // r < 512: (r * ceil(65536 / d)) >> 16 == r / d for d <= 64
             const uint32_t exy = ex * ey;
             cz = (r * s_magic[exy]) >> 16;
@@ -673,6 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
                 bx = s_box[j];
                 ex = ((bx >> 9) & 7u) + 1u;
                 ey = ((bx >> 12) & 7u) + 1u;
+                { const uint4 og = s_org[bx >> 18]; ox = og.x; oy = og.y; oz = og.z; }
                 cx = cy = cz = 0;
                 load_tri(j);
             }
@@ -721,7 +746,7 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
                 }
             }
             if (!edgeHit) continue;
-            const uint32_t vi = (lz * 8u + ly) * 8u + lx;
+            const uint32_t vi = ((bx >> 18) << 9) | ((lz * 8u + ly) * 8u + lx);
             const int32_t t = (int32_t)s_tid[j];
             atomicAdd(&s_cnt[vi], 1u);
             // Three lowest indices by an atomic-min cascade: each level keeps
@@ -743,34 +768,39 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
 
     // Hit words by wave ballot, then one record per hit voxel (tile, voxel,
     // count, slots in reference order) for the colour pass.
-    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+    for (int pass = 0; pass < kV / kBlock; ++pass) {
         const uint32_t v = (uint32_t)(pass * kBlock + tid);
         const uint64_t mask = __ballot(s_cnt[v] > 0u);
         if ((tid & 63) == 0) {
-            const uint32_t w0 = v >> 5;  // even word index
-            s_hitw[w0] = (uint32_t)mask;
-            s_hitw[w0 + 1] = (uint32_t)(mask >> 32);
+            const uint32_t hw = v >> 5;  // even word index
+            s_hitw[hw] = (uint32_t)mask;
+            s_hitw[hw + 1] = (uint32_t)(mask >> 32);
         }
     }
     __syncthreads();
-    if (tid < kTileWords) {
+    if (tid < kW) {
         const uint32_t w = s_hitw[tid];
+        const uint32_t k = (uint32_t)tid / kTileWords;
         s_hitoff[tid] = (uint32_t)__popc(w);
-        hit[a * kTileWords + tid] = w;
-        occ[a * kTileWords + tid] |= w;
+        if (k < nt) {
+            const uint64_t o = (uint64_t)s_org[k].w * kTileWords + tid % kTileWords;
+            hit[o] = w;
+            occ[o] |= w;
+        }
     }
     __syncthreads();
     if (tid == 0) {
         uint32_t acc = 0;
-        for (int w = 0; w < kTileWords; ++w) { const uint32_t c = s_hitoff[w]; s_hitoff[w] = acc; acc += c; }
+        for (int w = 0; w < kW; ++w) { const uint32_t c = s_hitoff[w]; s_hitoff[w] = acc; acc += c; }
         const uint32_t seg = blockIdx.x % kRecSegs;
         s_recBase = acc ? (uint32_t)(seg * segCap) + atomicAdd(&recCount[seg], acc) : 0u;
     }
     __syncthreads();
-    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+    for (int pass = 0; pass < kV / kBlock; ++pass) {
         const uint32_t v = (uint32_t)(pass * kBlock + tid);
         const uint32_t w = s_hitw[v >> 5];
         if ((w >> (v & 31u)) & 1u) {
+            const uint32_t k = v >> 9, lv = v & (kTileVox - 1u);
             const uint32_t below = w & ((1u << (v & 31u)) - 1u);
             const uint32_t h = s_hitoff[v >> 5] + (uint32_t)__popc(below);
             const uint32_t cnt = s_cnt[v];
@@ -783,8 +813,9 @@ __global__ __launch_bounds__(kBlock) void k_voxelize(
             const uint32_t ns = k3 + (cnt >= 4u ? 1u : 0u);
             if (cnt >= 4u) sl[3] = s_hi[v];
             const size_t r = (size_t)(s_recBase + h) * 2;
-            const uint32_t X = ox + (v & 7u), Y = oy + ((v >> 3) & 7u), Z = oz + (v >> 6);
-            rec[r] = make_uint4((uint32_t)a, v | (ns << 9) | (wasHit << 12) | (Z << 16), cnt,
+            const uint4 og = s_org[k];
+            const uint32_t X = og.x + (lv & 7u), Y = og.y + ((lv >> 3) & 7u), Z = og.z + (lv >> 6);
+            rec[r] = make_uint4(og.w, lv | (ns << 9) | (wasHit << 12) | (Z << 16), cnt,
                                 (uint32_t)sl[0]);
             rec[r + 1] = make_uint4((uint32_t)sl[1], (uint32_t)sl[2], (uint32_t)sl[3], X | (Y << 16));
         }
@@ -1548,7 +1579,7 @@ void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int
                      unsigned long long* satCounter, uint4* work, uint32_t* workCount,
                      uint4* rec, uint32_t* recCount, unsigned persistentBlocks) {
     if (nActive == 0) return;
-    unsigned blocks = (unsigned)(nActive < persistentBlocks ? nActive : persistentBlocks);
+    const unsigned blocks = (unsigned)vox_grid(nActive, persistentBlocks);
     k_voxelize<<<blocks, kBlock, 0, st>>>(P, G, g, init, work, workCount, list, verts, fnorm, bbox,
                                           binStart, binTri, occ, hit, rec, recCount,
                                           rec_seg_cap(nActive, blocks), satCounter);
@@ -1566,7 +1597,7 @@ void launch_colour(hipStream_t st, const VoxParams& P, const GroupArgs& G, int i
                    uint64_t nActive, const uint4* rec, const uint32_t* recCount,
                    const int32_t* list, const float* verts, const float* fnorm, uint32_t* col,
                    uint32_t* qn, float* nrm, unsigned persistentBlocks) {
-    const uint64_t vblocks = nActive < persistentBlocks ? nActive : persistentBlocks;
+    const uint64_t vblocks = vox_grid(nActive, persistentBlocks);
     k_colour<<<persistentBlocks, kBlock, 0, st>>>(P, G, init, rec, recCount,
                                                   rec_seg_cap(nActive, vblocks), list, verts,
                                                   fnorm, col, qn, nrm);
