@@ -873,11 +873,18 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     const uint64_t words = (nvox + 31) / 32;
     if (!(h->options & VOX_OPT_NO_OCCUPANCY)) {
         if (!h->bitmap.ensure(words * 4, h->err, "occupancy bitmap")) return 0;
-        h->run("fill", [&] { launch_fill_u32(st, h->bitmap.as<uint32_t>(), words, 0u); });
+        bool rows = false;
         h->run("occupancy", [&] {
-            launch_occupancy(st, P, l0.active, l0.list.as<int32_t>(), h->occ0.as<uint32_t>(),
-                             h->bitmap.as<uint32_t>());
+            rows = launch_occupancy_rows(st, P, l0.tileIdx.as<int32_t>(), h->occ0.as<uint32_t>(),
+                                         h->bitmap.as<uint32_t>());
         });
+        if (!rows) {
+            h->run("fill", [&] { launch_fill_u32(st, h->bitmap.as<uint32_t>(), words, 0u); });
+            h->run("occupancy", [&] {
+                launch_occupancy(st, P, l0.active, l0.list.as<int32_t>(), h->occ0.as<uint32_t>(),
+                                 h->bitmap.as<uint32_t>());
+            });
+        }
     }
     h->workReleased = false;
     if (h->replay) {

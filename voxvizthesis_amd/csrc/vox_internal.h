@@ -201,6 +201,10 @@ void launch_brick_gather(hipStream_t st, const LevelMap& lm, const uint32_t* col
                          uint32_t* bcol, uint32_t* bnrm);
 void launch_collapse(hipStream_t st, uint32_t Rd, int childIsLeaf, uint8_t* types,
                      const uint8_t* childTypes);
+// The whole bitmap (no clear needed) when S[0] is a multiple of 32 and a row
+// of tile indices fits in LDS; false (nothing launched) otherwise.
+bool launch_occupancy_rows(hipStream_t st, const VoxParams& P, const int32_t* tileIdx,
+                           const uint32_t* occ, uint32_t* bitmap);
 void launch_occupancy(hipStream_t st, const VoxParams& P, uint64_t nActive, const int32_t* list,
                       const uint32_t* occ, uint32_t* bitmap);
 void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,

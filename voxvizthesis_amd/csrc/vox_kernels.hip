@@ -1328,6 +1328,39 @@ __global__ __launch_bounds__(kBlock) void k_occupancy(VoxParams P, const int32_t
     }
 }
 
+// The whole occupancy bitmap when a voxel row of S[0] bits is a whole number
+// of 32-bit words: one workgroup per (y, z) row of tiles, whose tile indices
+// it keeps in LDS, writes every word of the row's 64 voxel rows once (zeros
+// included), so the bitmap needs no clearing first.
+__global__ __launch_bounds__(kBlock) void k_occupancy_rows(VoxParams P,
+                                                           const int32_t* __restrict__ tileIdx,
+                                                           const uint32_t* __restrict__ occ,
+                                                           uint32_t* __restrict__ bitmap) {
+    extern __shared__ int32_t s_rowIdx[];   // P.TG[0] entries
+    const uint32_t ty = blockIdx.x % P.TG[1], tz = blockIdx.x / P.TG[1];
+    const uint64_t rowBase = ((uint64_t)tz * P.TG[1] + ty) * P.TG[0];
+    for (uint32_t i = threadIdx.x; i < P.TG[0]; i += kBlock) s_rowIdx[i] = tileIdx[rowBase + i];
+    __syncthreads();
+    const uint32_t wpr = P.S[0] >> 5;   // words per voxel row; 4 tiles per word
+    const uint32_t ny = P.S[1] - ty * kTile < (uint32_t)kTile ? P.S[1] - ty * kTile : (uint32_t)kTile;
+    const uint32_t nz = P.S[2] - tz * kTile < (uint32_t)kTile ? P.S[2] - tz * kTile : (uint32_t)kTile;
+    for (uint32_t k = threadIdx.x; k < 64u * wpr; k += kBlock) {
+        const uint32_t r = k / wpr, w = k - r * wpr;   // tile row r = (z & 7) * 8 + (y & 7)
+        const uint32_t ly = r & 7u, lz = r >> 3;
+        if (ly >= ny || lz >= nz) continue;
+        uint32_t word = 0u;
+#pragma unroll
+        for (uint32_t b = 0; b < 4u; ++b) {
+            const int32_t a = s_rowIdx[w * 4u + b];
+            if (a >= 0)
+                word |= ((occ[(uint64_t)a * kTileWords + (r >> 2)] >> ((r & 3u) * 8u)) & 255u)
+                        << (8u * b);
+        }
+        const uint64_t Y = (uint64_t)ty * kTile + ly, Z = (uint64_t)tz * kTile + lz;
+        bitmap[(Z * P.S[1] + Y) * wpr + w] = word;
+    }
+}
+
 __global__ void k_dense_level(LevelMap lm, const uint32_t* __restrict__ col,
                               const float* __restrict__ nrm, uint32_t* __restrict__ outCol,
                               float* __restrict__ outNrm) {
@@ -1659,6 +1692,14 @@ void launch_occupancy(hipStream_t st, const VoxParams& P, uint64_t nActive, cons
                       const uint32_t* occ, uint32_t* bitmap) {
     if (nActive == 0) return;
     k_occupancy<<<(unsigned)nActive, kBlock, 0, st>>>(P, list, occ, bitmap);
+}
+bool launch_occupancy_rows(hipStream_t st, const VoxParams& P, const int32_t* tileIdx,
+                           const uint32_t* occ, uint32_t* bitmap) {
+    if ((P.S[0] & 31u) != 0u || P.TG[0] * 4u > 65536u) return false;
+    const uint64_t rows = (uint64_t)P.TG[1] * P.TG[2];
+    if (rows == 0) return true;
+    k_occupancy_rows<<<(unsigned)rows, kBlock, P.TG[0] * 4u, st>>>(P, tileIdx, occ, bitmap);
+    return true;
 }
 void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
                         const float* nrm, uint32_t* outCol, float* outNrm) {
