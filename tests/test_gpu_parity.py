@@ -47,6 +47,11 @@ def test_mixed_groups_b4():
     _check(scenes.scene_mixed(seed=9, N=64, B=4))
 
 
+def test_big_and_small_triangles_binned():
+    # boxes of 18^3 tiles walk their bins; the small ones are listed as pairs
+    _check(scenes.scene_big_triangles())
+
+
 def test_type_x_split_miss():
     # N=256, B=4: the leaf typing launch split leaves x = N + extra (the +1
     # border of the last node column) unexamined.

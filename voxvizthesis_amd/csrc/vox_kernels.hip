@@ -200,49 +200,148 @@ __device__ __forceinline__ BinGroup bin_group(bool act, uint64_t key) {
     return r;
 }
 
+// i / d for i < 2^22 and a small divisor d, given inv = 1.0f / d: the float
+// quotient is within one of the integer quotient; one correction step.
+__device__ __forceinline__ uint32_t udiv_small(uint32_t i, uint32_t d, float inv) {
+    uint32_t q = (uint32_t)((float)i * inv);
+    const int32_t r = (int32_t)(i - q * d);
+    if (r < 0) --q;
+    else if (r >= (int32_t)d) ++q;
+    return q;
+}
+
+// Binning works on (triangle, tile) pairs: a workgroup lists the pairs of
+// its triangles with a block scan of their tile-box sizes, then every lane
+// takes one pair, so a wave's lanes stay busy whatever the spread of box
+// sizes. A triangle whose box has more than kExpandMax tiles walks its box
+// on its own lane instead (the pair count of a workgroup stays < 2^22).
+constexpr uint32_t kExpandMax = 4096;
+struct PairList {
+    uint32_t off[kBlock + 1];   // exclusive scan of pairs per triangle; off[kBlock] = total
+    uint32_t xy[kBlock];        // tile box start x | y << 16
+    uint32_t z[kBlock];         // tile box start z
+    uint32_t ext[kBlock];       // tile box extent x | y << 16
+    uint32_t tri[kBlock];       // triangle index
+    uint32_t w[kBlock / 64];
+};
+// The bin key of pair p < off[kBlock], and its triangle.
+__device__ __forceinline__ uint64_t pair_key(const PairList& L, const VoxParams& P, uint64_t base,
+                                             uint32_t p, uint32_t& tri) {
+    uint32_t lo = 0, hi = kBlock;   // last triangle j with off[j] <= p (its box holds p)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (L.off[mid] <= p) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t r = p - L.off[lo], e = L.ext[lo];
+    const uint32_t ex = e & 0xffffu, exy = ex * (e >> 16);
+    const uint32_t dz = udiv_small(r, exy, 1.0f / (float)exy);
+    const uint32_t rxy = r - dz * exy;
+    const uint32_t dy = udiv_small(rxy, ex, 1.0f / (float)ex);
+    const uint32_t dx = rxy - dy * ex, xy = L.xy[lo];
+    tri = L.tri[lo];
+    return base + ((uint64_t)(L.z[lo] + dz) * P.TG[1] + ((xy >> 16) + dy)) * P.TG[0] +
+           (xy & 0xffffu) + dx;
+}
+// Tile box of voxel bounds (packed min | max << 16 per axis) within the
+// group's z range: false when empty.
+__device__ __forceinline__ bool tile_box(const GroupArgs& G, uint32_t bx, uint32_t by,
+                                         uint32_t bz, uint32_t (&t0)[3], uint32_t (&t1)[3]) {
+    const uint32_t z0 = (bz & 0xffffu) > G.zStart ? (bz & 0xffffu) : G.zStart;
+    const uint32_t z1 = (bz >> 16) < G.zEnd ? (bz >> 16) : G.zEnd;
+    if (z0 > z1) return false;
+    t0[0] = (bx & 0xffffu) >> 3; t1[0] = (bx >> 16) >> 3;
+    t0[1] = (by & 0xffffu) >> 3; t1[1] = (by >> 16) >> 3;
+    t0[2] = z0 >> 3; t1[2] = z1 >> 3;
+    return true;
+}
+// Fill the workgroup's pair list (barriers inside: every thread calls it).
+// Returns the number of pairs; sets big for a box walked by its own lane.
+__device__ __forceinline__ uint32_t list_pairs(PairList& L, bool has, const uint32_t (&t0)[3],
+                                               const uint32_t (&t1)[3], uint32_t tri, bool& big) {
+    uint32_t w = 0;
+    big = false;
+    if (has) {
+        const uint64_t n = (uint64_t)(t1[0] - t0[0] + 1) * (t1[1] - t0[1] + 1) * (t1[2] - t0[2] + 1);
+        if (n <= kExpandMax) w = (uint32_t)n;
+        else big = true;
+    }
+    uint32_t total;
+    const uint32_t off = block_excl_scan(w, L.w, &total);   // barriers
+    const int tid = threadIdx.x;
+    L.off[tid] = off;
+    L.xy[tid] = t0[0] | (t0[1] << 16);
+    L.z[tid] = t0[2];
+    L.ext[tid] = w ? ((t1[0] - t0[0] + 1) | ((t1[1] - t0[1] + 1) << 16)) : 1u | (1u << 16);
+    L.tri[tid] = tri;
+    if (tid == 0) L.off[kBlock] = total;
+    __syncthreads();
+    return total;
+}
+
 // ---------------------------------------------------------------------------
 // setup: ComputeEdgesFaceNormalsAndBounds (VoxelizerKernels.cu 230-347) plus
 // the count of 8^3 tiles each triangle's bounds touch, per (group, tile) bin.
 // Bounds are packed as min | max << 16 per axis.
-__global__ void k_setup(VoxParams P, GroupArgs G, int g, const float* __restrict__ verts,
-                        float* __restrict__ fnorm, uint32_t* __restrict__ bbox,
-                        uint32_t* __restrict__ binCount) {
-    uint64_t lt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (lt >= G.ntri) return;
-    uint64_t t = G.triOff + lt;
-    const float* q = verts + 9 * t;
-    f3 v[3] = {load3(q), load3(q + 3), load3(q + 6)};
-    f3 e0 = sub3(v[1], v[0]);
-    f3 p0p2 = sub3(v[2], v[0]);
-    f3 n = normalize3(cross3(e0, p0p2));
-    fnorm[3 * t] = n.x;
-    fnorm[3 * t + 1] = n.y;
-    fnorm[3 * t + 2] = n.z;
-    uint32_t mn[3], mx[3];
+__global__ __launch_bounds__(kBlock) void k_setup(VoxParams P, GroupArgs G, int g,
+                                                  const float* __restrict__ verts,
+                                                  float* __restrict__ fnorm,
+                                                  uint32_t* __restrict__ bbox,
+                                                  uint32_t* __restrict__ binCount) {
+    __shared__ PairList L;
+    const uint64_t lt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t t = G.triOff + lt;
+    uint32_t t0[3] = {0, 0, 0}, t1[3] = {0, 0, 0};
+    bool has = false;
+    if (lt < G.ntri) {
+        const float* q = verts + 9 * t;
+        f3 v[3] = {load3(q), load3(q + 3), load3(q + 6)};
+        f3 e0 = sub3(v[1], v[0]);
+        f3 p0p2 = sub3(v[2], v[0]);
+        f3 n = normalize3(cross3(e0, p0p2));
+        fnorm[3 * t] = n.x;
+        fnorm[3 * t + 1] = n.y;
+        fnorm[3 * t + 2] = n.z;
+        uint32_t mn[3], mx[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        f3 diff = {(v[k].x - P.offP[0]) / P.dP[0], (v[k].y - P.offP[1]) / P.dP[1],
-                   (v[k].z - P.offP[2]) / P.dP[2]};
+        for (int k = 0; k < 3; ++k) {
+            f3 diff = {(v[k].x - P.offP[0]) / P.dP[0], (v[k].y - P.offP[1]) / P.dP[1],
+                       (v[k].z - P.offP[2]) / P.dP[2]};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                int64_t iv = trunc_i(cget(diff, a));
+                int64_t hi = (int64_t)P.S[a] - 1;
+                uint32_t u = (uint32_t)(iv < 0 ? 0 : (iv > hi ? hi : iv));
+                if (k == 0) { mn[a] = u; mx[a] = u; }
+                else { mn[a] = u < mn[a] ? u : mn[a]; mx[a] = u > mx[a] ? u : mx[a]; }
+            }
+        }
+        uint32_t pk[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            int64_t iv = trunc_i(cget(diff, a));
-            int64_t hi = (int64_t)P.S[a] - 1;
-            uint32_t u = (uint32_t)(iv < 0 ? 0 : (iv > hi ? hi : iv));
-            if (k == 0) { mn[a] = u; mx[a] = u; }
-            else { mn[a] = u < mn[a] ? u : mn[a]; mx[a] = u > mx[a] ? u : mx[a]; }
+            pk[a] = mn[a] | (mx[a] << 16);
+            bbox[3 * t + a] = pk[a];
         }
+        has = G.overlaps && tile_box(G, pk[0], pk[1], pk[2], t0, t1);
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) bbox[3 * t + a] = mn[a] | (mx[a] << 16);
-    if (!G.overlaps) return;
-    uint32_t z0 = mn[2] > G.zStart ? mn[2] : G.zStart;
-    uint32_t z1 = mx[2] < G.zEnd ? mx[2] : G.zEnd;
-    if (z0 > z1) return;
-    TileWalk w(P, g, mn[0] >> 3, mx[0] >> 3, mn[1] >> 3, mx[1] >> 3, z0 >> 3, z1 >> 3);
-    // Neighbouring triangles mostly share tiles: each step groups the lanes
-    // holding the same bin (ballots only), then the first lane of every group
-    // adds the group's count, all groups in one atomic instruction.
-    bool act = true;
+    if (!G.overlaps) return;   // uniform: the whole grid
+    bool big;
+    const uint32_t total = list_pairs(L, has, t0, t1, (uint32_t)t, big);
+    // Neighbouring triangles mostly share tiles: the lanes holding the same
+    // bin add once, with the count of the group, all groups of a step in one
+    // atomic instruction.
+    const uint64_t base = (uint64_t)g * P.numTiles;
+    for (uint32_t p0 = 0; p0 < total; p0 += kBlock) {
+        const uint32_t p = p0 + threadIdx.x;
+        const bool act = p < total;
+        uint32_t tri = 0;
+        const uint64_t key = act ? pair_key(L, P, base, p, tri) : 0ull;
+        const BinGroup bg = bin_group(act, key);
+        if (act && bg.lead) atomicAdd(&binCount[key], bg.count);
+    }
+    if (!__syncthreads_or(big)) return;
+    TileWalk w(P, g, t0[0], t1[0], t0[1], t1[1], t0[2], t1[2]);
+    bool act = big;
     while (__ballot(act)) {
         const BinGroup bg = bin_group(act, w.key);
         if (act && bg.lead) atomicAdd(&binCount[w.key], bg.count);
@@ -250,28 +349,44 @@ __global__ void k_setup(VoxParams P, GroupArgs G, int g, const float* __restrict
     }
 }
 
-__global__ void k_scatter(VoxParams P, GroupArgs G, int g, const uint32_t* __restrict__ bbox,
-                          const uint32_t* __restrict__ binStart, uint32_t* __restrict__ binFill,
-                          uint32_t* __restrict__ binTri) {
-    uint64_t lt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (lt >= G.ntri || !G.overlaps) return;
-    uint64_t t = G.triOff + lt;
-    uint32_t bx = bbox[3 * t], by = bbox[3 * t + 1], bz = bbox[3 * t + 2];
-    uint32_t z0 = (bz & 0xffffu) > G.zStart ? (bz & 0xffffu) : G.zStart;
-    uint32_t z1 = (bz >> 16) < G.zEnd ? (bz >> 16) : G.zEnd;
-    if (z0 > z1) return;
-    TileWalk w(P, g, (bx & 0xffffu) >> 3, (bx >> 16) >> 3, (by & 0xffffu) >> 3, (by >> 16) >> 3,
-               z0 >> 3, z1 >> 3);
-    // As in k_setup, one returning atomic instruction per step reserves the
-    // places of every group of lanes holding the same bin; the lanes of a
-    // group take consecutive places after its base.
-    bool act = true;
+__global__ __launch_bounds__(kBlock) void k_scatter(VoxParams P, GroupArgs G, int g,
+                                                    const uint32_t* __restrict__ bbox,
+                                                    const uint32_t* __restrict__ binStart,
+                                                    uint32_t* __restrict__ binFill,
+                                                    uint32_t* __restrict__ binTri) {
+    if (!G.overlaps) return;   // uniform: the whole grid
+    __shared__ PairList L;
+    const uint64_t lt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t t = G.triOff + lt;
+    uint32_t t0[3] = {0, 0, 0}, t1[3] = {0, 0, 0};
+    bool has = false;
+    if (lt < G.ntri) has = tile_box(G, bbox[3 * t], bbox[3 * t + 1], bbox[3 * t + 2], t0, t1);
+    bool big;
+    const uint32_t total = list_pairs(L, has, t0, t1, (uint32_t)t, big);
+    // One returning atomic instruction per step reserves the places of every
+    // group of lanes holding the same bin; the lanes of a group take
+    // consecutive places after its base.
+    const uint64_t base = (uint64_t)g * P.numTiles;
+    for (uint32_t p0 = 0; p0 < total; p0 += kBlock) {
+        const uint32_t p = p0 + threadIdx.x;
+        const bool act = p < total;
+        uint32_t tri = 0;
+        const uint64_t key = act ? pair_key(L, P, base, p, tri) : 0ull;
+        const BinGroup bg = bin_group(act, key);
+        uint32_t r = 0;
+        if (act && bg.lead) r = atomicAdd(&binFill[key], bg.count);
+        const uint32_t b0 = (uint32_t)__shfl((int)r, bg.leader);
+        if (act) binTri[binStart[key] + b0 + bg.rank] = tri;
+    }
+    if (!__syncthreads_or(big)) return;
+    TileWalk w(P, g, t0[0], t1[0], t0[1], t1[1], t0[2], t1[2]);
+    bool act = big;
     while (__ballot(act)) {
         const BinGroup bg = bin_group(act, w.key);
         uint32_t r = 0;
         if (act && bg.lead) r = atomicAdd(&binFill[w.key], bg.count);
-        const uint32_t base = (uint32_t)__shfl((int)r, bg.leader);
-        if (act) binTri[binStart[w.key] + base + bg.rank] = (uint32_t)t;
+        const uint32_t b0 = (uint32_t)__shfl((int)r, bg.leader);
+        if (act) binTri[binStart[w.key] + b0 + bg.rank] = (uint32_t)t;
         if (act) act = w.next();
     }
 }
@@ -1227,15 +1342,6 @@ __global__ void k_brick_assign(uint64_t nodes, const uint32_t* __restrict__ flag
 // brick gather: WriteBinaryBrick / WriteUByteBinaryBrick middle-brick case
 // (VoxelBrickWriter.cu 725-818, 509-588): (B+2)^3 voxels, RGBA8 colour and
 // RGB8 normal ((n+1)*0.5*255, truncated).
-// i / d for i < 2^22 and a small divisor d, given inv = 1.0f / d: the float
-// quotient is within one of the integer quotient; one correction step.
-__device__ __forceinline__ uint32_t udiv_small(uint32_t i, uint32_t d, float inv) {
-    uint32_t q = (uint32_t)((float)i * inv);
-    const int32_t r = (int32_t)(i - q * d);
-    if (r < 0) --q;
-    else if (r >= (int32_t)d) ++q;
-    return q;
-}
 
 __global__ __launch_bounds__(kBlock) void k_brick_gather(
     LevelMap lm, const uint32_t* __restrict__ col, const uint32_t* __restrict__ qn,

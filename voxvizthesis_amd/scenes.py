@@ -340,6 +340,20 @@ def scene_mixed(seed: int = 7, N: int = 64, B: int = 8) -> Scene:
                  name=f"mixed_N{N}_B{B}")
 
 
+def scene_big_triangles(seed: int = 13, N: int = 128, B: int = 8) -> Scene:
+    """One untextured group mixing small random triangles with two that span
+    the whole grid (18^3 tiles at N=128, B=8: more (triangle, tile) bins than
+    the binning lists per triangle, so those two walk their boxes instead),
+    each placed among the small ones of a different workgroup of 256."""
+    big = np.array([[[-0.02, -0.02, -0.01], [1.02, 0.01, 1.01], [0.0, 1.02, 1.0]],
+                    [[1.02, 1.02, -0.02], [-0.02, 0.99, 0.5], [0.98, -0.02, 1.02]]],
+                   dtype=np.float32)
+    soup = random_soup(500, seed, N)
+    verts = np.concatenate([soup.verts[:250], big[:1], soup.verts[250:], big[1:]])
+    return Scene(meshes=[Mesh(verts=verts.astype(np.float32), name="big_and_small")],
+                 N=(N, N, N), B=(B, B, B), name=f"big_tris_N{N}_B{B}")
+
+
 def scene_terrain_n(cells: int, N: int, B: int, seed: int = 11) -> Scene:
     """Heightfield of (cells+1)^2 vertices on an N^3 grid of B^3 bricks."""
     return Scene(meshes=[heightfield_terrain(cells, seed)], N=(N, N, N), B=(B, B, B),
