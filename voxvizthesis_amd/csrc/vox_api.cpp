@@ -778,7 +778,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         if (h->lv[k].aliasOf >= 0) continue;
         LevelView s = view(h, srcOf(k));
         LevelView l = view(h, k);
-        h->run("mip", [&] {
+        if (l.active > 0) h->run("mip", [&] {
             launch_mip(st, s.map, l.map, l.active, l.list, s.col, s.nrm, l.col, l.qn, l.nrm);
         });
     }
@@ -839,7 +839,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
                                 t.brickNode.as<uint32_t>(), t.types.as<uint8_t>());
         });
         uint32_t bd[3] = {borders[d][0], borders[d][1], borders[d][2]};
-        h->run("brick_gather", [&] {
+        if (nb[d] > 0) h->run("brick_gather", [&] {
             launch_brick_gather(st, l.map, l.col, l.qn, nb[d],
                                 t.brickNode.as<uint32_t>(), t.Rd, h->B, bd, t.bcol.as<uint32_t>(),
                                 t.bnrm.as<uint32_t>());
