@@ -47,6 +47,16 @@ def test_mixed_groups_b4():
     _check(scenes.scene_mixed(seed=9, N=64, B=4))
 
 
+def test_mixed_groups_rocks_first():
+    # the first group is no terrain: every level-0 voxel no group reaches is zero
+    _check(scenes.scene_mixed(seed=5, order="rocks_first"))
+
+
+def test_mixed_groups_two_terrains():
+    # a second terrain fills over voxels that the first fill and the rocks hold
+    _check(scenes.scene_mixed(seed=6, order="two_terrains"))
+
+
 def test_big_and_small_triangles_binned():
     # boxes of 18^3 tiles walk their bins; the small ones are listed as pairs
     _check(scenes.scene_big_triangles())

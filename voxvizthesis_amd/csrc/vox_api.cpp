@@ -118,6 +118,7 @@ struct vox_ctx {
     VoxParams P{};
     std::vector<LevelBufs> lv;
     DevBuf occ0, hit0, sat, work, workCount, rec, recCount;
+    DevBuf fill0;   // level-0 voxels reached by a terrain fill, 16 words per active tile
     unsigned persistentBlocks = 2048;
     std::vector<DepthBufs> tree;
     DevBuf bitmap;
@@ -452,7 +453,8 @@ void vox_deallocate_voxel_memory(vox_ctx* h) {
     }
     h->tree.clear();
     h->nodeFlag.release(); h->nodeExcl.release();
-    h->occ0.release(); h->hit0.release(); h->sat.release(); h->bitmap.release();
+    h->occ0.release(); h->hit0.release(); h->fill0.release(); h->sat.release();
+    h->bitmap.release();
     h->work.release(); h->workCount.release(); h->rec.release(); h->recCount.release();
     h->counters.release(); h->binCount.release(); h->binStart.release(); h->binTri.release();
     h->scanTmp.release();
@@ -716,6 +718,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         return h->fail("more hit records than 32-bit indices address; use fewer persistent "
                        "workgroups or a smaller grid"), 0;
     if (!h->occ0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "occupancy words") ||
+        !h->fill0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "filled-voxel words") ||
         !h->hit0.ensure((uint64_t)l0.active * kTileWords * 4, h->err, "hit words") ||
         !h->work.ensure((uint64_t)l0.active * 16, h->err, "tile work list") ||
         !h->workCount.ensure(4, h->err, "tile work count") ||
@@ -724,19 +727,15 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
                        h->err, "hit records") ||
         !h->recCount.ensure(4 * kRecSegs, h->err, "hit record counts"))
         return 0;
-    h->run("fill", [&] { launch_fill_u32(st, h->occ0.as<uint32_t>(), l0.active * kTileWords, 0u); });
-    if (firstOverlap >= 0) {
-        const GroupArgs& g0 = h->gargs[firstOverlap];
-        if (!(g0.textured && g0.terrain)) {
-            // tiles without triangles of the first group start as zero voxels
-            h->run("fill", [&] {
-                launch_fill_u32(st, l0.col.as<uint32_t>(), l0.active * kTileVox, 0u);
-                launch_fill_u32(st, l0.qn.as<uint32_t>(), l0.active * kTileVox, 0x007f7f7fu);
-                launch_fill_u32(st, reinterpret_cast<uint32_t*>(l0.nrm.as<float>()),
-                                l0.active * kTileVox * 3, 0u);
-            });
-        }
-    }
+    h->run("fill", [&] {
+        launch_fill_u32(st, h->occ0.as<uint32_t>(), l0.active * kTileWords, 0u);
+        launch_fill_u32(st, h->fill0.as<uint32_t>(), l0.active * kTileWords, 0u);
+    });
+    // Level 0 stores only hit or filled voxels: the others read as zero
+    // wherever level 0 is read (LevelMap::defOcc / defFill), so no tile is
+    // cleared first and the terrain fill writes no zeros.
+    l0.map.defOcc = h->occ0.as<uint32_t>();
+    l0.map.defFill = h->fill0.as<uint32_t>();
 
     // Voxelize, colour and fill, group by group (Voxelizer.cu 2335-2553).
     unsigned long long* satc = h->sat.as<unsigned long long>();
@@ -762,14 +761,14 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
             launch_colour(st, P, ga, g == firstOverlap ? 1 : 0, l0.active, h->rec.as<uint4>(),
                           h->recCount.as<uint32_t>(), l0.list.as<int32_t>(), h->verts.as<float>(),
                           h->fnorm.as<float>(), l0.col.as<uint32_t>(), l0.qn.as<uint32_t>(),
-                          l0.nrm.as<float>(), h->persistentBlocks);
+                          l0.nrm.as<float>(), h->fill0.as<uint32_t>(), h->persistentBlocks);
         });
         if (ga.textured && ga.terrain) {
             h->run("fill_terrain", [&] {
                 launch_fill(st, P, g, g == firstOverlap ? 1 : 0, l0.active, l0.list.as<int32_t>(),
                             l0.tileIdx.as<int32_t>(), h->binStart.as<uint32_t>(),
                             l0.col.as<uint32_t>(), l0.qn.as<uint32_t>(), l0.nrm.as<float>(),
-                            h->hit0.as<uint32_t>());
+                            h->hit0.as<uint32_t>(), h->fill0.as<uint32_t>());
             });
         }
     }
@@ -1043,7 +1042,7 @@ void buffer_list(vox_ctx* h, std::vector<void*>& out) {
     DevBuf* top[] = {&h->verts, &h->fnorm, &h->bbox, &h->vn, &h->uv, &h->img, &h->grpFill,
                      &h->binCount, &h->binStart, &h->binTri, &h->scanTmp, &h->occ0, &h->hit0,
                      &h->sat, &h->work, &h->workCount, &h->rec, &h->recCount, &h->bitmap,
-                     &h->counters, &h->nodeFlag, &h->nodeExcl};
+                     &h->counters, &h->nodeFlag, &h->nodeExcl, &h->fill0};
     for (DevBuf* b : top) out.push_back(b->p);
     for (auto& l : h->lv) {
         DevBuf* lb[] = {&l.mark, &l.excl, &l.tileIdx, &l.list, &l.col, &l.qn, &l.nrm};
@@ -1417,7 +1416,8 @@ int vox_release_work_memory(vox_ctx* h) {
         l.col.release(); l.qn.release(); l.nrm.release();
         l.active = 0;
     }
-    h->occ0.release(); h->hit0.release(); h->work.release(); h->workCount.release();
+    h->occ0.release(); h->hit0.release(); h->fill0.release(); h->work.release();
+    h->workCount.release();
     h->rec.release(); h->recCount.release(); h->binTri.release(); h->binCount.release();
     h->binStart.release(); h->scanTmp.release();
     vox_deallocate_triangle_memory(h);

@@ -92,6 +92,12 @@ struct LevelMap {         // sparse tiles of one mip level
     uint32_t TG[3];       // tile grid
     uint64_t numTiles;
     const int32_t* tileIdx;   // dense map tile -> compact index or -1
+    // Level 0 stores only the voxels that some group hit (defOcc, the
+    // occupancy words) or a terrain fill reached (defFill), 16 words per
+    // active tile; any other voxel of an active tile reads as zero colour,
+    // zero normal.  Both null for the mip levels, which are written in full.
+    const uint32_t* defOcc = nullptr;
+    const uint32_t* defFill = nullptr;
 };
 
 // Brick arrays of one octree, per depth (the voxelizer result's bricks).
@@ -170,13 +176,17 @@ void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int
 void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nActive,
                         const int32_t* list, const uint32_t* binStart, uint4* work,
                         uint32_t* workCount, uint32_t* recCount);
+// fillw: level 0's filled-voxel words (a voxel neither hit before this group
+// nor filled holds no stored value and counts as zero).
 void launch_colour(hipStream_t st, const VoxParams& P, const GroupArgs& G, int init,
                    uint64_t nActive, const uint4* rec, const uint32_t* recCount,
                    const int32_t* list, const float* verts, const float* fnorm, uint32_t* col,
-                   uint32_t* qn, float* nrm, unsigned persistentBlocks);
+                   uint32_t* qn, float* nrm, const uint32_t* fillw, unsigned persistentBlocks);
+// Also records the voxels it fills in fillw (16 words per active tile; set
+// on the first group, OR-ed after).
 void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t nActive,
                  const int32_t* list, const int32_t* tileIdx, const uint32_t* binStart,
-                 uint32_t* col, uint32_t* qn, float* nrm, const uint32_t* hit);
+                 uint32_t* col, uint32_t* qn, float* nrm, const uint32_t* hit, uint32_t* fillw);
 void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
                 const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
                 uint32_t* colDst, uint32_t* qnDst, float* nrmDst);

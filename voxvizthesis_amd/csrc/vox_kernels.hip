@@ -951,7 +951,7 @@ __global__ __launch_bounds__(kBlock) void k_colour(
     VoxParams P, GroupArgs G, int init, const uint4* __restrict__ rec,
     const uint32_t* __restrict__ recCount, uint64_t segCap, const int32_t* __restrict__ list,
     const float* __restrict__ verts, const float* __restrict__ fnorm, uint32_t* __restrict__ col,
-    uint32_t* __restrict__ qn, float* __restrict__ nrm) {
+    uint32_t* __restrict__ qn, float* __restrict__ nrm, const uint32_t* __restrict__ fillw) {
     __shared__ float s_inv255[256];
     __shared__ uint32_t s_pre[kRecSegs + 1];   // record segments: prefix of their counts
     fill_inv255(s_inv255);
@@ -979,10 +979,13 @@ __global__ __launch_bounds__(kBlock) void k_colour(
         const uint32_t cnt = r0.z;
         const int32_t slots[4] = {(int32_t)r0.w, (int32_t)r1.x, (int32_t)r1.y, (int32_t)r1.z};
         const uint64_t o = a * kTileVox + v;
-        uint32_t c = init ? 0u : col[o];
-        f3 nv = init ? f3{0.0f, 0.0f, 0.0f}
-                     : f3{nrm[(a * 3 + 0) * kTileVox + v], nrm[(a * 3 + 1) * kTileVox + v],
-                          nrm[(a * 3 + 2) * kTileVox + v]};
+        // the voxel's value so far: stored when an earlier group hit it or a
+        // terrain fill reached it, zero otherwise
+        const bool prior = !init && (wasHit || ((fillw[a * kTileWords + (v >> 5)] >> (v & 31u)) & 1u));
+        uint32_t c = prior ? col[o] : 0u;
+        f3 nv = prior ? f3{nrm[(a * 3 + 0) * kTileVox + v], nrm[(a * 3 + 1) * kTileVox + v],
+                           nrm[(a * 3 + 2) * kTileVox + v]}
+                      : f3{0.0f, 0.0f, 0.0f};
         const uint32_t X = r1.w & 0xffffu, Y = r1.w >> 16, Z = r0.y >> 16;
         const f3 cpos = {P.offP[0] + (float)X * P.dP[0], P.offP[1] + (float)Y * P.dP[1],
                          P.offP[2] + (float)Z * P.dP[2]};
@@ -1039,7 +1042,8 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init,
                                                  uint32_t* __restrict__ col,
                                                  uint32_t* __restrict__ qn,
                                                  float* __restrict__ nrm,
-                                                 const uint32_t* __restrict__ hit) {
+                                                 const uint32_t* __restrict__ hit,
+                                                 uint32_t* __restrict__ fillw) {
     __shared__ uint32_t s_hit[kFillTiles + 1][kTileWords];
     __shared__ int32_t s_idx[kFillTiles + 1];
     __shared__ uint32_t s_colmask[64];
@@ -1049,7 +1053,6 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init,
     uint32_t tx_, ty_, tz;
     tile_xyz(tlin, P.TG, tx_, ty_, tz);
     const uint64_t slab = (uint64_t)P.TG[0] * P.TG[1];
-    const bool undefinedTile = init != 0;
     if (tid <= kFillTiles) {
         int32_t idx = -1;
         if (tid == 0) idx = (int32_t)a;
@@ -1075,13 +1078,15 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init,
         s_colmask[tid] = m;
     }
     __syncthreads();
+    // Voxels neither hit nor filled are not written: they read as zero
+    // (LevelMap::defOcc / defFill); the filled ones are recorded in fillw.
     for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
         const uint32_t v = (uint32_t)(pass * kBlock + tid);
         const uint32_t c = v & 63u, lz = v >> 6;
         const uint32_t m = s_colmask[c];
-        if ((m >> lz) & 1u) continue;
         const uint32_t above = (m >> (lz + 1u)) & ((1u << kFillDepth) - 1u);
-        if (above) {
+        const bool filled = !((m >> lz) & 1u) && above != 0u;
+        if (filled) {
             const uint32_t zz = lz + (uint32_t)__ffs(above);
             const uint32_t q = zz >> 3, v2 = ((zz & 7u) << 6) | c;
             const uint64_t sa = (uint64_t)s_idx[q];
@@ -1090,11 +1095,17 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init,
 #pragma unroll
             for (int k = 0; k < 3; ++k)
                 nrm[(a * 3 + k) * kTileVox + v] = nrm[(sa * 3 + k) * kTileVox + v2];
-        } else if (undefinedTile) {
-            col[a * kTileVox + v] = 0u;
-            qn[a * kTileVox + v] = kZeroNormalQ;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) nrm[(a * 3 + k) * kTileVox + v] = 0.0f;
+        }
+        const uint64_t fb = __ballot(filled);
+        if ((tid & 63) == 0) {
+            const uint64_t w = a * kTileWords + (v >> 5);   // v >> 5 is even
+            if (init) {
+                fillw[w] = (uint32_t)fb;
+                fillw[w + 1] = (uint32_t)(fb >> 32);
+            } else {
+                fillw[w] |= (uint32_t)fb;
+                fillw[w + 1] |= (uint32_t)(fb >> 32);
+            }
         }
     }
 }
@@ -1114,7 +1125,23 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
     uint32_t tx, ty, tz;
     tile_xyz(tlin, dst.TG, tx, ty, tz);
     __shared__ float s_inv255[256];
+    // stored-voxel words of the (up to) 8 source tiles under this tile, when
+    // the source is level 0; child c = (z & 1) * 4 + (y & 1) * 2 + (x & 1)
+    __shared__ uint32_t s_def[8][kTileWords];
     fill_inv255(s_inv255);
+    if (src.defOcc && tid < 8 * kTileWords) {
+        const uint32_t c = (uint32_t)tid / kTileWords, w = (uint32_t)tid % kTileWords;
+        const uint32_t sx = 2 * tx + (c & 1u), sy = 2 * ty + ((c >> 1) & 1u), sz = 2 * tz + (c >> 2);
+        uint32_t d = 0u;
+        if (sx < src.TG[0] && sy < src.TG[1] && sz < src.TG[2]) {
+            const int32_t ca = src.tileIdx[((uint64_t)sz * src.TG[1] + sy) * src.TG[0] + sx];
+            if (ca >= 0) {
+                const uint64_t o = (uint64_t)ca * kTileWords + w;
+                d = src.defOcc[o] | src.defFill[o];
+            }
+        }
+        s_def[c][w] = d;
+    }
     __syncthreads();
     for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
         const uint32_t v = (uint32_t)(pass * kBlock + tid);
@@ -1142,10 +1169,32 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const uint32_t row = (zs[j >> 1] * 8u + ys[j & 1]) * 8u + (bx & 7u);
-                    cp[j] = *reinterpret_cast<const uint2*>(colSrc + cb + row);
+                    // stored voxels of the pair (level 0 keeps only hit or
+                    // filled voxels; the others read as zero)
+                    uint32_t dm = 3u;
+                    if (src.defOcc) {
+                        const uint32_t ch = (((bz >> 3) & 1u) << 2) | (((by >> 3) & 1u) << 1) |
+                                            ((bx >> 3) & 1u);
+                        dm = (s_def[ch][row >> 5] >> (row & 31u)) & 3u;
+                    }
+                    cp[j] = make_uint2(0u, 0u);
+                    np[0][j] = np[1][j] = np[2][j] = make_float2(0.0f, 0.0f);
+                    if (dm) {
+                        cp[j] = *reinterpret_cast<const uint2*>(colSrc + cb + row);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        np[c][j] = *reinterpret_cast<const float2*>(nrmSrc + nb0 + c * kTileVox + row);
+                        for (int c = 0; c < 3; ++c)
+                            np[c][j] = *reinterpret_cast<const float2*>(nrmSrc + nb0 + c * kTileVox + row);
+                        if (!(dm & 1u)) {
+                            cp[j].x = 0u;
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) np[c][j].x = 0.0f;
+                        }
+                        if (!(dm & 2u)) {
+                            cp[j].y = 0u;
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) np[c][j].y = 0.0f;
+                        }
+                    }
                     if (xdup) {
                         cp[j].y = cp[j].x;
 #pragma unroll
@@ -1195,6 +1244,13 @@ __device__ __forceinline__ int64_t tile_of(const LevelMap& lm, uint32_t X, uint3
 __device__ __forceinline__ uint32_t vox_of(uint32_t X, uint32_t Y, uint32_t Z) {
     return ((Z & 7u) * 8u + (Y & 7u)) * 8u + (X & 7u);
 }
+// Whether voxel v of active tile a holds a stored value (always, except on
+// level 0: hit by some group or reached by a terrain fill).
+__device__ __forceinline__ bool stored(const LevelMap& lm, uint64_t a, uint32_t v) {
+    if (!lm.defOcc) return true;
+    const uint64_t w = a * kTileWords + (v >> 5);
+    return (((lm.defOcc[w] | lm.defFill[w]) >> (v & 31u)) & 1u) != 0u;
+}
 
 // ---------------------------------------------------------------------------
 // Node typing: ComputeOctTreeNodeConstColor + ComputeOctTreeNodeType with the
@@ -1218,7 +1274,12 @@ __global__ void k_node_sample(LevelMap lm, const uint32_t* __restrict__ col, uin
     node_xyz(n, (uint32_t)(__ffs(Rd) - 1), nx, ny, nz);
     const uint32_t sx = nx * Bx + bx, sy = ny * By + by, sz = nz * Bz + bz;
     const int64_t sa = tile_of(lm, sx, sy, sz);
-    const uint32_t sample = sa >= 0 ? col[(uint64_t)sa * kTileVox + vox_of(sx, sy, sz)] : 0u;
+    const uint32_t sv = vox_of(sx, sy, sz);
+    uint32_t sample = 0u;
+    if (sa >= 0) {
+        const uint32_t cv = col[(uint64_t)sa * kTileVox + sv];   // beside the mask load
+        sample = stored(lm, (uint64_t)sa, sv) ? cv : 0u;
+    }
     uint32_t f = 0u;
     if (sample != 0u) {
         const uint32_t r0x = sx - 1, r0y = sy - 1, r0z = sz - 1;
@@ -1283,9 +1344,17 @@ __global__ __launch_bounds__(kBlock) void k_node_compare(
     uint32_t zIn = 0u;
     uint32_t c[kTile];
     const uint32_t* tc = col + a * kTileVox + (uint64_t)lane;
+    // stored-voxel words of the tile (level 0), lane w < 16 holding word w
+    uint32_t dw = 0xffffffffu;
+    if (lm.defOcc && lane < kTileWords)
+        dw = lm.defOcc[a * kTileWords + lane] | lm.defFill[a * kTileWords + lane];
 #pragma unroll
     for (int lz = 0; lz < kTile; ++lz) {
-        c[lz] = tc[lz * 64];
+        bool st = true;
+        if (lm.defOcc)
+            st = (((uint32_t)__shfl((int)dw, lz * 2 + (lane >> 5)) >> (lane & 31)) & 1u) != 0u;
+        const uint32_t cv = tc[lz * 64];   // every voxel of the tile is allocated
+        c[lz] = st ? cv : 0u;
         if (o3[2] + (uint32_t)lz < lm.dims[2]) zIn |= 1u << lz;
     }
     for (int32_t nz = blo[2]; nz <= bhi[2]; ++nz) {
@@ -1348,8 +1417,10 @@ __global__ __launch_bounds__(kBlock) void k_brick_gather(
     const uint32_t* __restrict__ brickNode, uint32_t Rd, uint32_t Bx, uint32_t By, uint32_t Bz,
     uint32_t bx, uint32_t by, uint32_t bz, uint32_t* __restrict__ bcol,
     uint32_t* __restrict__ bnrm) {
-    // tiles under the brick: at most 6 per axis for B <= 38
+    // tiles under the brick: at most 6 per axis for B <= 38; with their
+    // stored-voxel words when the level is level 0
     __shared__ int32_t s_tile[216];
+    __shared__ uint32_t s_def[216 * kTileWords];
     const uint64_t j = blockIdx.x;
     const uint64_t n = brickNode[j];
     uint32_t nx, ny, nz;
@@ -1359,9 +1430,21 @@ __global__ __launch_bounds__(kBlock) void k_brick_gather(
     const uint32_t t0x = r0x >> 3, t0y = r0y >> 3, t0z = r0z >> 3;
     const uint32_t ntx = ((r0x + ex - 1) >> 3) - t0x + 1, nty = ((r0y + ey - 1) >> 3) - t0y + 1,
                    ntz = ((r0z + ez - 1) >> 3) - t0z + 1;
-    for (uint32_t i = threadIdx.x; i < ntx * nty * ntz; i += kBlock) {
-        const uint32_t ix = t0x + i % ntx, iy = t0y + (i / ntx) % nty, iz = t0z + i / (ntx * nty);
-        s_tile[i] = lm.tileIdx[((uint64_t)iz * lm.TG[1] + iy) * lm.TG[0] + ix];
+    const uint32_t ntl = ntx * nty * ntz;
+    for (uint32_t i = threadIdx.x; i < ntl * kTileWords; i += kBlock) {
+        const uint32_t tl = i / kTileWords, w = i % kTileWords;
+        const uint32_t ix = t0x + tl % ntx, iy = t0y + (tl / ntx) % nty, iz = t0z + tl / (ntx * nty);
+        const int32_t ta = lm.tileIdx[((uint64_t)iz * lm.TG[1] + iy) * lm.TG[0] + ix];
+        if (w == 0) s_tile[tl] = ta;
+        uint32_t d = 0xffffffffu;
+        if (lm.defOcc) {
+            d = 0u;
+            if (ta >= 0) {
+                const uint64_t o = (uint64_t)ta * kTileWords + w;
+                d = lm.defOcc[o] | lm.defFill[o];
+            }
+        }
+        s_def[i] = d;
     }
     __syncthreads();
     const uint32_t nvox = ex * ey * ez, exy = ex * ey;
@@ -1372,10 +1455,12 @@ __global__ __launch_bounds__(kBlock) void k_brick_gather(
         const uint32_t y = udiv_small(rxy, ex, invx);
         const uint32_t x = rxy - y * ex;
         const uint32_t X = r0x + x, Y = r0y + y, Z = r0z + z;
-        const int32_t ta = s_tile[(((Z >> 3) - t0z) * nty + ((Y >> 3) - t0y)) * ntx + ((X >> 3) - t0x)];
+        const uint32_t tl = (((Z >> 3) - t0z) * nty + ((Y >> 3) - t0y)) * ntx + ((X >> 3) - t0x);
+        const int32_t ta = s_tile[tl];
         uint32_t c = 0u, q = kZeroNormalQ;
-        if (ta >= 0) {
-            const uint64_t o = (uint64_t)ta * kTileVox + vox_of(X, Y, Z);
+        const uint32_t v = vox_of(X, Y, Z);
+        if (ta >= 0 && ((s_def[tl * kTileWords + (v >> 5)] >> (v & 31u)) & 1u)) {
+            const uint64_t o = (uint64_t)ta * kTileVox + v;
             c = col[o];
             q = qn[o];
         }
@@ -1478,8 +1563,8 @@ __global__ void k_dense_level(LevelMap lm, const uint32_t* __restrict__ col,
     const int64_t ta = tile_of(lm, X, Y, Z);
     uint32_t c = 0u;
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    if (ta >= 0) {
-        const uint32_t v = vox_of(X, Y, Z);
+    const uint32_t v = vox_of(X, Y, Z);
+    if (ta >= 0 && stored(lm, (uint64_t)ta, v)) {
         c = col[(uint64_t)ta * kTileVox + v];
         nx = nrm[((uint64_t)ta * 3 + 0) * kTileVox + v];
         ny = nrm[((uint64_t)ta * 3 + 1) * kTileVox + v];
@@ -1735,18 +1820,18 @@ void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nAct
 void launch_colour(hipStream_t st, const VoxParams& P, const GroupArgs& G, int init,
                    uint64_t nActive, const uint4* rec, const uint32_t* recCount,
                    const int32_t* list, const float* verts, const float* fnorm, uint32_t* col,
-                   uint32_t* qn, float* nrm, unsigned persistentBlocks) {
+                   uint32_t* qn, float* nrm, const uint32_t* fillw, unsigned persistentBlocks) {
     const uint64_t vblocks = vox_grid(nActive, persistentBlocks);
     k_colour<<<persistentBlocks, kBlock, 0, st>>>(P, G, init, rec, recCount,
                                                   rec_seg_cap(nActive, vblocks), list, verts,
-                                                  fnorm, col, qn, nrm);
+                                                  fnorm, col, qn, nrm, fillw);
 }
 void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t nActive,
                  const int32_t* list, const int32_t* tileIdx, const uint32_t* binStart,
-                 uint32_t* col, uint32_t* qn, float* nrm, const uint32_t* hit) {
+                 uint32_t* col, uint32_t* qn, float* nrm, const uint32_t* hit, uint32_t* fillw) {
     if (nActive == 0) return;
     k_fill<<<(unsigned)nActive, kBlock, 0, st>>>(P, g, init, list, tileIdx, binStart, col, qn, nrm,
-                                                 hit);
+                                                 hit, fillw);
 }
 void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
                 const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,

@@ -326,18 +326,28 @@ def scene_two_triangles() -> Scene:
                  p=(0.0, 0.0, 0.0), delta=(1.0, 1.0, 1.0), name="two_triangles")
 
 
-def scene_mixed(seed: int = 7, N: int = 64, B: int = 8) -> Scene:
+def scene_mixed(seed: int = 7, N: int = 64, B: int = 8, order: str = "terrain_first") -> Scene:
     """Several groups in one grid: textured terrain (fill), untextured rocks,
     a textured soup with vertex normals, random UVs and wrap addressing, and
-    an untextured soup with vertex normals."""
+    an untextured soup with vertex normals.  order "rocks_first" puts the rocks
+    before the terrain (the first group is not a terrain, the fill is not the
+    first group's); "two_terrains" replaces the textured soup by a second
+    terrain (another seed) after the rocks, whose fill lands on voxels the
+    first terrain's fill and the rocks already hold."""
     cells = N // 2
     terrain = heightfield_terrain(cells, seed)
     rocks = rocks_on_terrain(terrain_height_fn(seed), 40, seed + 1, rmin=0.01, rmax=0.06)
     soup_t = random_soup(600, seed + 2, N, textured=True, uv_mode="random", vertex_normals=True,
                          translucent=True, addr=(0, 2))
     soup_u = random_soup(400, seed + 3, N, vertex_normals=True)
-    return Scene(meshes=[terrain, rocks, soup_t, soup_u], N=(N, N, N), B=(B, B, B),
-                 name=f"mixed_N{N}_B{B}")
+    if order == "rocks_first":
+        meshes = [rocks, terrain, soup_t, soup_u]
+    elif order == "two_terrains":
+        meshes = [terrain, rocks, heightfield_terrain(cells, seed + 20), soup_u]
+    else:
+        meshes = [terrain, rocks, soup_t, soup_u]
+    suffix = "" if order == "terrain_first" else "_" + order
+    return Scene(meshes=meshes, N=(N, N, N), B=(B, B, B), name=f"mixed_N{N}_B{B}{suffix}")
 
 
 def scene_big_triangles(seed: int = 13, N: int = 128, B: int = 8) -> Scene:
