@@ -522,35 +522,58 @@ __device__ __forceinline__ f2 bary_axis(int axis, f3 c, const f3* v) {
     return bary({c.x, c.y}, {v[0].x, v[0].y}, {v[1].x, v[1].y}, {v[2].x, v[2].y});
 }
 
-// Colour / normal update of one voxel with new hits from group G.
-// slots: up to 4 global triangle indices in reference slot order.
-__device__ void colour_update(const GroupArgs& G, const float* __restrict__ verts,
-                              const float* __restrict__ fnorm, f3 voxCenter, uint32_t count,
-                              const int32_t* slots, int nslots, uint32_t& col, f3& nrm,
-                              const float* __restrict__ inv255) {
+// The triangle attributes the colour pass reads, 27 floats: vertices [0, 9),
+// face normal [9, 12), texture coordinates [12, 18) (textured groups),
+// vertex normals [18, 27) (groups that have them).
+constexpr int kTriAttr = 27;
+__device__ __forceinline__ void load_tri_attr(const GroupArgs& G, const float* __restrict__ verts,
+                                              const float* __restrict__ fnorm, uint32_t ti,
+                                              float (&a)[kTriAttr]) {
+    if (G.textured || G.hasVN) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) a[i] = verts[9 * (size_t)ti + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a[9 + i] = fnorm[3 * (size_t)ti + i];
+    const size_t lt = ti - G.triOff;
+    if (G.textured) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a[12 + i] = G.uv[6 * lt + i];
+    }
+    if (G.hasVN) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) a[18 + i] = G.vn[9 * lt + i];
+    }
+}
+
+// Colour / normal update of one voxel with new hits from group G; get(k, a)
+// gives the attributes of the voxel's k-th slot triangle (slots in reference
+// order).
+template <class Get>
+__device__ void colour_update(const GroupArgs& G, f3 voxCenter, uint32_t count, int nslots,
+                              uint32_t& col, f3& nrm, const float* __restrict__ inv255,
+                              Get&& get) {
     f4 color = unpack4(col, inv255);
     f3 normal = nrm;
     if (G.textured) {
         uint32_t triCount = count < 4u ? count : 4u;
         for (int k = 0; k < nslots; ++k) {
-            uint32_t ti = (uint32_t)slots[k];
-            const float* q = verts + 9 * (size_t)ti;
-            f3 v[3] = {load3(q), load3(q + 3), load3(q + 6)};
-            f3 fn = load3(fnorm + 3 * (size_t)ti);
-            size_t lt = ti - G.triOff;
-            const float* uvp = G.uv + 6 * lt;
+            float a[kTriAttr];
+            get(k, a);
+            f3 v[3] = {{a[0], a[1], a[2]}, {a[3], a[4], a[5]}, {a[6], a[7], a[8]}};
+            f3 fn = {a[9], a[10], a[11]};
             float ax = fabsf(fn.x), ay = fabsf(fn.y), az = fabsf(fn.z);
             int axis = (ax > ay && ax > az) ? 0 : (ay > az ? 1 : 2);
             f2 bc = bary_axis(axis, voxCenter, v);
             f3 addn;
             if (G.hasVN) {
-                const float* nq = G.vn + 9 * lt;
-                addn = interp_n(bc, load3(nq), load3(nq + 3), load3(nq + 6));
+                addn = interp_n(bc, f3{a[18], a[19], a[20]}, f3{a[21], a[22], a[23]},
+                                f3{a[24], a[25], a[26]});
             } else {
                 addn = fn;
             }
             normal = add3(normal, addn);
-            f2 uv1 = {uvp[0], uvp[1]}, uv2 = {uvp[2], uvp[3]}, uv3 = {uvp[4], uvp[5]};
+            f2 uv1 = {a[12], a[13]}, uv2 = {a[14], a[15]}, uv3 = {a[16], a[17]};
             f2 Vv = {(uv2.x - uv1.x) * bc.y, (uv2.y - uv1.y) * bc.y};
             f2 Uu = {(uv3.x - uv1.x) * bc.x, (uv3.y - uv1.y) * bc.x};
             f2 tc = {uv1.x + Uu.x + Vv.x, uv1.y + Uu.y + Vv.y};
@@ -581,16 +604,15 @@ __device__ void colour_update(const GroupArgs& G, const float* __restrict__ vert
     } else {
         uint32_t triCount = count;
         for (int k = 0; k < nslots; ++k) {
-            uint32_t ti = (uint32_t)slots[k];
-            f3 fn = load3(fnorm + 3 * (size_t)ti);
+            float a[kTriAttr];
+            get(k, a);
+            f3 fn = {a[9], a[10], a[11]};
             if (G.hasVN) {
-                const float* q = verts + 9 * (size_t)ti;
-                f3 v[3] = {load3(q), load3(q + 3), load3(q + 6)};
-                size_t lt = ti - G.triOff;
-                const float* nq = G.vn + 9 * lt;
+                f3 v[3] = {{a[0], a[1], a[2]}, {a[3], a[4], a[5]}, {a[6], a[7], a[8]}};
                 int axis = (fn.x > fn.y && fn.x > fn.z) ? 0 : (fn.y > fn.z ? 1 : 2);
                 f2 bc = bary_axis(axis, voxCenter, v);
-                f3 ni = interp_n(bc, load3(nq), load3(nq + 3), load3(nq + 6));
+                f3 ni = interp_n(bc, f3{a[18], a[19], a[20]}, f3{a[21], a[22], a[23]},
+                                 f3{a[24], a[25], a[26]});
                 color.x += ni.x;
                 color.y += ni.y;
                 color.z += ni.z;
@@ -947,6 +969,16 @@ __global__ __launch_bounds__(kBlock, 4) void k_voxelize(
 // Colour pass over the hit records of one group: ComputeColorsAndNormals /
 // ComputeColorsAsNormals (VoxelizerKernels.cu 689-1083) for each voxel with
 // new hits, one thread per voxel (grid-stride).
+// Records are taken kBlock at a time per workgroup; the distinct triangles
+// their slots name are loaded once into LDS (a voxel's slot triangles are
+// mostly shared with its neighbours' records), through a hash table of
+// triangle indices; a slot whose triangle did not get a place there reads
+// global memory. Either way the values are the same.
+constexpr uint32_t kCKeys = 512;          // hash table entries
+constexpr uint32_t kCTris = 224;          // staged triangles per chunk
+constexpr uint32_t kTexLds = 1024;        // texels: a group image this small is read from LDS
+constexpr uint32_t kCStride = kTriAttr + 1;
+constexpr uint32_t kKeyEmpty = 0xffffffffu;
 __global__ __launch_bounds__(kBlock) void k_colour(
     VoxParams P, GroupArgs G, int init, const uint4* __restrict__ rec,
     const uint32_t* __restrict__ recCount, uint64_t segCap, const int32_t* __restrict__ list,
@@ -954,48 +986,145 @@ __global__ __launch_bounds__(kBlock) void k_colour(
     uint32_t* __restrict__ qn, float* __restrict__ nrm, const uint32_t* __restrict__ fillw) {
     __shared__ float s_inv255[256];
     __shared__ uint32_t s_pre[kRecSegs + 1];   // record segments: prefix of their counts
+    __shared__ uint32_t s_key[kCKeys];         // triangle index, or kKeyEmpty
+    __shared__ uint32_t s_idx[kCKeys];         // its staged place (>= kCTris: not staged)
+    __shared__ uint32_t s_nt;
+    __shared__ float s_dat[kCTris * kCStride];
+    __shared__ uint32_t s_tex[kTexLds];
+    const int tid = threadIdx.x;
     fill_inv255(s_inv255);
-    if (threadIdx.x == 0) {
+    // a small image (the usual height ramp) is read from LDS
+    GroupArgs Gl = G;
+    const uint64_t texels = (uint64_t)(G.w > 0 ? G.w : 0) * (uint64_t)(G.h > 0 ? G.h : 0);
+    if (G.textured && G.img && texels <= kTexLds) {
+        for (uint32_t k = tid; k < texels; k += kBlock)
+            s_tex[k] = reinterpret_cast<const uint32_t*>(G.img)[k];
+        Gl.img = reinterpret_cast<const uint8_t*>(s_tex);
+    }
+    for (uint32_t k = tid; k < kCKeys; k += kBlock) s_key[k] = kKeyEmpty;
+    if (tid == 0) {
         uint32_t acc = 0;
         for (uint32_t k = 0; k < kRecSegs; ++k) {
             s_pre[k] = acc;
             acc += recCount[k];
         }
         s_pre[kRecSegs] = acc;
+        s_nt = 0u;
     }
     __syncthreads();
     const uint32_t n = s_pre[kRecSegs];
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        uint32_t lo = 0, hi = kRecSegs;   // segment of the i-th record
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_pre[mid] <= i) lo = mid;
-            else hi = mid;
+    // Staging pays for groups that read vertices (textured, or with vertex
+    // normals); one that reads face normals only goes to global memory.
+    const bool stage = G.textured || G.hasVN;   // uniform
+    // the loop bound is the same for the whole workgroup (barriers inside)
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + (uint32_t)tid;
+        const bool act = i < n;
+        uint4 r0 = make_uint4(0u, 0u, 0u, 0u), r1 = make_uint4(0u, 0u, 0u, 0u);
+        if (act) {
+            uint32_t lo = 0, hi = kRecSegs;   // segment of the i-th record
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+            const uint64_t ri = lo * segCap + (i - s_pre[lo]);
+            r0 = rec[2 * ri];
+            r1 = rec[2 * ri + 1];
         }
-        const uint64_t ri = lo * segCap + (i - s_pre[lo]);
-        const uint4 r0 = rec[2 * ri], r1 = rec[2 * ri + 1];
-        const uint64_t a = r0.x;
-        const uint32_t v = r0.y & 511u, ns = (r0.y >> 9) & 7u, wasHit = (r0.y >> 12) & 1u;
-        const uint32_t cnt = r0.z;
-        const int32_t slots[4] = {(int32_t)r0.w, (int32_t)r1.x, (int32_t)r1.y, (int32_t)r1.z};
-        const uint64_t o = a * kTileVox + v;
-        // the voxel's value so far: stored when an earlier group hit it or a
-        // terrain fill reached it, zero otherwise
-        const bool prior = !init && (wasHit || ((fillw[a * kTileWords + (v >> 5)] >> (v & 31u)) & 1u));
-        uint32_t c = prior ? col[o] : 0u;
-        f3 nv = prior ? f3{nrm[(a * 3 + 0) * kTileVox + v], nrm[(a * 3 + 1) * kTileVox + v],
-                           nrm[(a * 3 + 2) * kTileVox + v]}
-                      : f3{0.0f, 0.0f, 0.0f};
-        const uint32_t X = r1.w & 0xffffu, Y = r1.w >> 16, Z = r0.y >> 16;
-        const f3 cpos = {P.offP[0] + (float)X * P.dP[0], P.offP[1] + (float)Y * P.dP[1],
-                         P.offP[2] + (float)Z * P.dP[2]};
-        const f3 voxCenter = add3(cpos, muls3(f3{P.dP[0], P.dP[1], P.dP[2]}, 0.5f));
-        colour_update(G, verts, fnorm, voxCenter, wasHit + cnt, slots, (int)ns, c, nv, s_inv255);
-        col[o] = c;
-        qn[o] = quant_normal(nv);
-        nrm[(a * 3 + 0) * kTileVox + v] = nv.x;
-        nrm[(a * 3 + 1) * kTileVox + v] = nv.y;
-        nrm[(a * 3 + 2) * kTileVox + v] = nv.z;
+        const uint32_t ns = act ? (r0.y >> 9) & 7u : 0u;
+        const uint32_t slots[4] = {r0.w, r1.x, r1.y, r1.z};
+        // 1. a table entry per distinct slot triangle; the thread that
+        // inserts one stages its attributes
+        uint32_t hpos[4] = {kCKeys, kCKeys, kCKeys, kCKeys};
+        uint32_t own = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (stage && (uint32_t)k < ns) {
+                const uint32_t t = slots[k];
+                uint32_t h = (t * 0x9E3779B1u) >> 23;   // 9 bits
+                for (int probe = 0; probe < 8; ++probe) {
+                    const uint32_t prev = atomicCAS(&s_key[h], kKeyEmpty, t);
+                    if (prev == kKeyEmpty) { own |= 1u << k; hpos[k] = h; break; }
+                    if (prev == t) { hpos[k] = h; break; }
+                    h = (h + 1u) & (kCKeys - 1u);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if ((own >> k) & 1u) {
+                const uint32_t d = atomicAdd(&s_nt, 1u);
+                s_idx[hpos[k]] = d;
+                if (d < kCTris) {
+                    float a[kTriAttr];
+                    load_tri_attr(G, verts, fnorm, slots[k], a);
+                    float* sd = s_dat + d * kCStride;
+#pragma unroll
+                    for (int q = 0; q < 12; ++q) sd[q] = a[q];
+                    if (G.textured) {
+#pragma unroll
+                        for (int q = 12; q < 18; ++q) sd[q] = a[q];
+                    }
+                    if (G.hasVN) {
+#pragma unroll
+                        for (int q = 18; q < kTriAttr; ++q) sd[q] = a[q];
+                    }
+                }
+            }
+        }
+        if (stage) __syncthreads();
+        // 2. the voxel's colour and normals
+        if (act) {
+            const uint64_t a = r0.x;
+            const uint32_t v = r0.y & 511u, wasHit = (r0.y >> 12) & 1u;
+            const uint32_t cnt = r0.z;
+            const uint64_t o = a * kTileVox + v;
+            // the voxel's value so far: stored when an earlier group hit it or a
+            // terrain fill reached it, zero otherwise
+            const bool prior = !init && (wasHit || ((fillw[a * kTileWords + (v >> 5)] >> (v & 31u)) & 1u));
+            uint32_t c = prior ? col[o] : 0u;
+            f3 nv = prior ? f3{nrm[(a * 3 + 0) * kTileVox + v], nrm[(a * 3 + 1) * kTileVox + v],
+                               nrm[(a * 3 + 2) * kTileVox + v]}
+                          : f3{0.0f, 0.0f, 0.0f};
+            const uint32_t X = r1.w & 0xffffu, Y = r1.w >> 16, Z = r0.y >> 16;
+            const f3 cpos = {P.offP[0] + (float)X * P.dP[0], P.offP[1] + (float)Y * P.dP[1],
+                             P.offP[2] + (float)Z * P.dP[2]};
+            const f3 voxCenter = add3(cpos, muls3(f3{P.dP[0], P.dP[1], P.dP[2]}, 0.5f));
+            colour_update(Gl, voxCenter, wasHit + cnt, (int)ns, c, nv, s_inv255,
+                          [&](int k, float (&at)[kTriAttr]) {
+                              const uint32_t hk = hpos[k];
+                              const uint32_t d = hk < kCKeys ? s_idx[hk] : kCTris;
+                              if (d < kCTris) {
+                                  const float* sd = s_dat + d * kCStride;
+#pragma unroll
+                                  for (int q = 0; q < 12; ++q) at[q] = sd[q];
+                                  if (G.textured) {
+#pragma unroll
+                                      for (int q = 12; q < 18; ++q) at[q] = sd[q];
+                                  }
+                                  if (G.hasVN) {
+#pragma unroll
+                                      for (int q = 18; q < kTriAttr; ++q) at[q] = sd[q];
+                                  }
+                              } else {
+                                  load_tri_attr(G, verts, fnorm, slots[k], at);
+                              }
+                          });
+            col[o] = c;
+            qn[o] = quant_normal(nv);
+            nrm[(a * 3 + 0) * kTileVox + v] = nv.x;
+            nrm[(a * 3 + 1) * kTileVox + v] = nv.y;
+            nrm[(a * 3 + 2) * kTileVox + v] = nv.z;
+        }
+        if (!stage) continue;
+        __syncthreads();
+        // 3. empty the table for the next chunk
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if ((own >> k) & 1u) s_key[hpos[k]] = kKeyEmpty;
+        if (tid == 0) s_nt = 0u;
+        __syncthreads();
     }
 }
 
