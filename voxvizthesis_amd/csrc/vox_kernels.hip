@@ -1578,23 +1578,41 @@ __global__ __launch_bounds__(kBlock) void k_brick_gather(
     __syncthreads();
     const uint32_t nvox = ex * ey * ez, exy = ex * ey;
     const float invx = 1.0f / (float)ex, invxy = 1.0f / (float)exy;
-    for (uint32_t i = threadIdx.x; i < nvox; i += kBlock) {
-        const uint32_t z = udiv_small(i, exy, invxy);
-        const uint32_t rxy = i - z * exy;
-        const uint32_t y = udiv_small(rxy, ex, invx);
-        const uint32_t x = rxy - y * ex;
-        const uint32_t X = r0x + x, Y = r0y + y, Z = r0z + z;
-        const uint32_t tl = (((Z >> 3) - t0z) * nty + ((Y >> 3) - t0y)) * ntx + ((X >> 3) - t0x);
-        const int32_t ta = s_tile[tl];
-        uint32_t c = 0u, q = kZeroNormalQ;
-        const uint32_t v = vox_of(X, Y, Z);
-        if (ta >= 0 && ((s_def[tl * kTileWords + (v >> 5)] >> (v & 31u)) & 1u)) {
-            const uint64_t o = (uint64_t)ta * kTileVox + v;
-            c = col[o];
-            q = qn[o];
+    // kGatherBatch voxels per thread at a time: their loads are all issued
+    // before the first store, so they are in flight together.
+    constexpr uint32_t kGatherBatch = 8;
+    for (uint32_t i0 = threadIdx.x; i0 < nvox; i0 += kGatherBatch * kBlock) {
+        uint32_t c[kGatherBatch], q[kGatherBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kGatherBatch; ++u) {
+            const uint32_t i = i0 + u * kBlock;
+            c[u] = 0u;
+            q[u] = kZeroNormalQ;
+            if (i < nvox) {
+                const uint32_t z = udiv_small(i, exy, invxy);
+                const uint32_t rxy = i - z * exy;
+                const uint32_t y = udiv_small(rxy, ex, invx);
+                const uint32_t x = rxy - y * ex;
+                const uint32_t X = r0x + x, Y = r0y + y, Z = r0z + z;
+                const uint32_t tl =
+                    (((Z >> 3) - t0z) * nty + ((Y >> 3) - t0y)) * ntx + ((X >> 3) - t0x);
+                const int32_t ta = s_tile[tl];
+                const uint32_t v = vox_of(X, Y, Z);
+                if (ta >= 0 && ((s_def[tl * kTileWords + (v >> 5)] >> (v & 31u)) & 1u)) {
+                    const uint64_t o = (uint64_t)ta * kTileVox + v;
+                    c[u] = col[o];
+                    q[u] = qn[o];
+                }
+            }
         }
-        bcol[j * nvox + i] = c;
-        bnrm[j * nvox + i] = q;
+#pragma unroll
+        for (uint32_t u = 0; u < kGatherBatch; ++u) {
+            const uint32_t i = i0 + u * kBlock;
+            if (i < nvox) {
+                bcol[j * nvox + i] = c[u];
+                bnrm[j * nvox + i] = q[u];
+            }
+        }
     }
 }
 
