@@ -1209,23 +1209,41 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init,
     __syncthreads();
     // Voxels neither hit nor filled are not written: they read as zero
     // (LevelMap::defOcc / defFill); the filled ones are recorded in fillw.
-    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+    // A filled voxel copies a hit voxel (never itself written here), so the
+    // loads of both passes go out before the first store.
+    constexpr int kPasses = kTileVox / kBlock;
+    bool filled[kPasses];
+    uint32_t cv[kPasses], qv[kPasses];
+    float nv[kPasses][3];
+#pragma unroll
+    for (int pass = 0; pass < kPasses; ++pass) {
         const uint32_t v = (uint32_t)(pass * kBlock + tid);
         const uint32_t c = v & 63u, lz = v >> 6;
         const uint32_t m = s_colmask[c];
         const uint32_t above = (m >> (lz + 1u)) & ((1u << kFillDepth) - 1u);
-        const bool filled = !((m >> lz) & 1u) && above != 0u;
-        if (filled) {
+        filled[pass] = !((m >> lz) & 1u) && above != 0u;
+        cv[pass] = qv[pass] = 0u;
+        nv[pass][0] = nv[pass][1] = nv[pass][2] = 0.0f;
+        if (filled[pass]) {
             const uint32_t zz = lz + (uint32_t)__ffs(above);
             const uint32_t q = zz >> 3, v2 = ((zz & 7u) << 6) | c;
             const uint64_t sa = (uint64_t)s_idx[q];
-            col[a * kTileVox + v] = col[sa * kTileVox + v2];
-            qn[a * kTileVox + v] = qn[sa * kTileVox + v2];
+            cv[pass] = col[sa * kTileVox + v2];
+            qv[pass] = qn[sa * kTileVox + v2];
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
-                nrm[(a * 3 + k) * kTileVox + v] = nrm[(sa * 3 + k) * kTileVox + v2];
+            for (int k = 0; k < 3; ++k) nv[pass][k] = nrm[(sa * 3 + k) * kTileVox + v2];
         }
-        const uint64_t fb = __ballot(filled);
+    }
+#pragma unroll
+    for (int pass = 0; pass < kPasses; ++pass) {
+        const uint32_t v = (uint32_t)(pass * kBlock + tid);
+        if (filled[pass]) {
+            col[a * kTileVox + v] = cv[pass];
+            qn[a * kTileVox + v] = qv[pass];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) nrm[(a * 3 + k) * kTileVox + v] = nv[pass][k];
+        }
+        const uint64_t fb = __ballot(filled[pass]);
         if ((tid & 63) == 0) {
             const uint64_t w = a * kTileWords + (v >> 5);   // v >> 5 is even
             if (init) {
@@ -1272,6 +1290,9 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
         s_def[c][w] = d;
     }
     __syncthreads();
+    // both passes unrolled: the source loads of the second are independent
+    // of the first pass's stores (distinct arrays) and can go out early
+#pragma unroll
     for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
         const uint32_t v = (uint32_t)(pass * kBlock + tid);
         const uint32_t X = tx * kTile + (v & 7u), Y = ty * kTile + ((v >> 3) & 7u),
