@@ -1436,12 +1436,22 @@ __global__ void k_node_sample(LevelMap lm, const uint32_t* __restrict__ col, uin
         uint32_t r1x = sx + Bx;
         if (r1x > xLimit - 1) r1x = xLimit - 1;
         const uint32_t r1y = sy + By, r1z = sz + Bz;
-        for (uint32_t tz = r0z >> 3; tz <= (r1z >> 3) && !f; ++tz)
-            for (uint32_t ty = r0y >> 3; ty <= (r1y >> 3) && !f; ++ty)
-                for (uint32_t tx = r0x >> 3; tx <= (r1x >> 3) && r0x <= r1x; ++tx) {
-                    const uint64_t t = ((uint64_t)tz * lm.TG[1] + ty) * lm.TG[0] + tx;
-                    if (lm.tileIdx[t] < 0) { f = 1u; break; }
+        if (r0x <= r1x) {
+            // every tile of the brick's box, the loads independent of each
+            // other (no early exit), several in flight
+            const uint32_t tx0 = r0x >> 3, tx1 = r1x >> 3, ty0 = r0y >> 3, ty1 = r1y >> 3;
+            const uint32_t nt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1) * ((r1z >> 3) - (r0z >> 3) + 1);
+            uint32_t tx = tx0, ty = ty0, tz = r0z >> 3;
+#pragma unroll 4
+            for (uint32_t k = 0; k < nt; ++k) {
+                const uint64_t t = ((uint64_t)tz * lm.TG[1] + ty) * lm.TG[0] + tx;
+                f |= lm.tileIdx[t] < 0 ? 1u : 0u;
+                if (++tx > tx1) {
+                    tx = tx0;
+                    if (++ty > ty1) { ty = ty0; ++tz; }
                 }
+            }
+        }
     }
     cc[n] = sample;
     flag[n] = f;
