@@ -623,7 +623,12 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
             launch_setup(h->st, P, h->gargs[g], g, h->verts.as<float>(), h->fnorm.as<float>(),
                          h->bbox.as<uint32_t>(), cnt);
     });
-    h->run("scan", [&] { scan(h, cnt, start, h->numBins); });
+    // the scan leaves the counts zeroed for the scatter pass to count again
+    h->run("scan", [&] {
+        if (h->scanTmp.ensure(scan_tmp_words(h->numBins) * 4, h->err, "scan scratch"))
+            launch_scan_clear(h->st, cnt, start, h->numBins, h->scanTmp.as<uint32_t>());
+    });
+    if (!h->scanTmp.p) return 0;
     uint32_t total = 0;
     if (h->replay) {
         total = (uint32_t)h->planPairs;
@@ -636,7 +641,6 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
     }
     h->pairs = total;
     if (!h->binTri.ensure((uint64_t)total * 4, h->err, "bin entries")) return 0;
-    h->run("fill", [&] { launch_fill_u32(h->st, cnt, h->numBins, 0u); });
     h->run("scatter", [&] {
         for (int g = 0; g < G; ++g)
             launch_scatter(h->st, P, h->gargs[g], g, h->bbox.as<uint32_t>(), start, cnt,
@@ -681,8 +685,7 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         if (l.aliasOf >= 0) continue;
         if (k > 0) {
             LevelView s = view(h, srcOf(k));
-            h->run("fill", [&] { launch_fill_u32(st, l.mark.as<uint32_t>(), l.map.numTiles, 0u); });
-            h->run("mark", [&] { launch_mark_parent(st, s.map, l.map, l.mark.as<uint32_t>()); });
+            h->run("mark", [&] { launch_mark_children(st, s.map, l.map, l.mark.as<uint32_t>()); });
         }
         h->run("scan", [&] { scan(h, l.mark.as<uint32_t>(), l.excl.as<uint32_t>(), l.map.numTiles); });
         h->run("compact", [&] {
@@ -727,9 +730,12 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
                        h->err, "hit records") ||
         !h->recCount.ensure(4 * kRecSegs, h->err, "hit record counts"))
         return 0;
+    unsigned long long* satc = h->sat.as<unsigned long long>();
     h->run("fill", [&] {
-        launch_fill_u32(st, h->occ0.as<uint32_t>(), l0.active * kTileWords, 0u);
-        launch_fill_u32(st, h->fill0.as<uint32_t>(), l0.active * kTileWords, 0u);
+        const FillSeg z[3] = {{h->occ0.as<uint32_t>(), l0.active * kTileWords, 0u},
+                              {h->fill0.as<uint32_t>(), l0.active * kTileWords, 0u},
+                              {reinterpret_cast<uint32_t*>(satc), 2, 0u}};
+        launch_fill_segs(st, z, 3);
     });
     // Level 0 stores only hit or filled voxels: the others read as zero
     // wherever level 0 is read (LevelMap::defOcc / defFill), so no tile is
@@ -738,12 +744,9 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     l0.map.defFill = h->fill0.as<uint32_t>();
 
     // Voxelize, colour and fill, group by group (Voxelizer.cu 2335-2553).
-    unsigned long long* satc = h->sat.as<unsigned long long>();
-    h->run("fill", [&] { launch_fill_u32(st, reinterpret_cast<uint32_t*>(satc), 2, 0u); });
     for (int g = 0; g < G; ++g) {
         const GroupArgs& ga = h->gargs[g];
         if (!ga.overlaps) continue;
-        h->run("fill", [&] { launch_fill_u32(st, h->hit0.as<uint32_t>(), l0.active * kTileWords, 0u); });
         h->run("group_tiles", [&] {
             launch_group_tiles(st, P, g, l0.active, l0.list.as<int32_t>(), h->binStart.as<uint32_t>(),
                                h->work.as<uint4>(), h->workCount.as<uint32_t>(),
@@ -822,9 +825,9 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         std::memcpy(h->planNb, nb, sizeof(nb));
     }
     const uint64_t bvox = (uint64_t)(h->B[0] + 2) * (h->B[1] + 2) * (h->B[2] + 2);
-    for (int d = L - 1; d >= 0; --d) {
-        int k = L - 1 - d;
-        LevelView l = view(h, k);
+    DepthOut dout{};
+    dout.L = L;
+    for (int d = 0; d < L; ++d) {
         DepthBufs& t = h->tree[d];
         t.bricks = nb[d];
         h->stats.bricks_total += nb[d];
@@ -832,11 +835,19 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
             !t.bcol.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick colours") ||
             !t.bnrm.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick normals"))
             return 0;
-        h->run("brick_assign", [&] {
-            launch_brick_assign(st, t.nodes, h->nodeFlag.as<uint32_t>() + t.off,
-                                h->nodeExcl.as<uint32_t>() + t.off, t.bidx.as<uint32_t>(),
-                                t.brickNode.as<uint32_t>(), t.types.as<uint8_t>());
-        });
+        dout.bidx[d] = t.bidx.as<uint32_t>();
+        dout.brickNode[d] = t.brickNode.as<uint32_t>();
+        dout.types[d] = t.types.as<uint8_t>();
+        dout.off[d] = t.off;
+    }
+    dout.off[L] = h->allNodes;
+    h->run("brick_assign", [&] {
+        launch_brick_assign_all(st, h->nodeFlag.as<uint32_t>(), h->nodeExcl.as<uint32_t>(), dout);
+    });
+    for (int d = L - 1; d >= 0; --d) {
+        int k = L - 1 - d;
+        LevelView l = view(h, k);
+        DepthBufs& t = h->tree[d];
         uint32_t bd[3] = {borders[d][0], borders[d][1], borders[d][2]};
         if (nb[d] > 0) h->run("brick_gather", [&] {
             launch_brick_gather(st, l.map, l.col, l.qn, nb[d],

@@ -165,8 +165,10 @@ void launch_activity0(hipStream_t st, const VoxParams& P, int ngroups, const int
 // Also writes excl[n] (the number of marked tiles) to *count.
 void launch_compact(hipStream_t st, uint64_t n, const uint32_t* mark, const uint32_t* excl,
                     int32_t* tileIdx, int32_t* list, uint32_t* count);
-void launch_mark_parent(hipStream_t st, const LevelMap& src, const LevelMap& dst,
-                        uint32_t* markDst);
+// mark[t] = 1 when any of the (up to 8) source tiles under level tile t is
+// active, else 0: every mark written, no clearing first.
+void launch_mark_children(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                          uint32_t* markDst);
 void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g, int init,
                      uint64_t nActive, const int32_t* list, const float* verts,
                      const float* fnorm, const uint32_t* bbox, const uint32_t* binStart,
@@ -193,6 +195,18 @@ void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64
 void launch_node_type(hipStream_t st, const LevelMap& lm, uint64_t nActive, const int32_t* list,
                       const uint32_t* col, uint32_t Rd, const uint32_t B[3],
                       const uint32_t border[3], uint32_t xLimit, uint32_t* cc, uint32_t* flag);
+// Brick indices of every depth in one launch over the all-depth node array:
+// node n of depth d (off[d] <= n < off[d + 1]) gets type, brick index
+// (excl[n] - excl[off[d]]) and its brick's node.
+struct DepthOut {
+    uint32_t* bidx[kMaxLevels];
+    uint32_t* brickNode[kMaxLevels];
+    uint8_t* types[kMaxLevels];
+    uint64_t off[kMaxLevels + 1];
+    int L;
+};
+void launch_brick_assign_all(hipStream_t st, const uint32_t* flag, const uint32_t* excl,
+                             const DepthOut& D);
 // Range boundaries of one scan, at most kMaxLevels ranges (passed by value).
 struct RangeOffs {
     uint64_t v[kMaxLevels + 1];
@@ -200,11 +214,6 @@ struct RangeOffs {
 // out[d] = excl[offs[d + 1]] - excl[offs[d]] for d < n (n <= kMaxLevels).
 void launch_range_counts(hipStream_t st, const uint32_t* excl, const uint64_t* offs, int n,
                          uint32_t* out);
-// excl points at the scan value of the depth's first node: brick addresses
-// are excl[n] - excl[0].
-void launch_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
-                         const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode,
-                         uint8_t* types);
 void launch_brick_gather(hipStream_t st, const LevelMap& lm, const uint32_t* col,
                          const uint32_t* qn, uint64_t nbricks, const uint32_t* brickNode,
                          uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
@@ -220,6 +229,13 @@ void launch_occupancy(hipStream_t st, const VoxParams& P, uint64_t nActive, cons
 void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
                         const float* nrm, uint32_t* outCol, float* outNrm);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
+// Up to four (pointer, words, value) fills in one launch.
+struct FillSeg {
+    uint32_t* p;
+    uint64_t n;
+    uint32_t v;
+};
+void launch_fill_segs(hipStream_t st, const FillSeg* segs, int nseg);
 // Copy n16 16-byte words.
 void launch_copy_u128(hipStream_t st, const void* src, void* dst, uint64_t n16);
 // dst row k = src row idx[k] - base, w floats per row.
@@ -255,5 +271,7 @@ void launch_brick_upload(hipStream_t st, const BrickSrc& src, const uint4* req,
 // tmp must hold scan_tmp_words(n) words.
 uint64_t scan_tmp_words(uint64_t n);
 void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp);
+// The same scan, leaving the input zeroed (for counters that count again).
+void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp);
 
 }  // namespace voxmi

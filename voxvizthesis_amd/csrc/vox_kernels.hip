@@ -432,15 +432,20 @@ __global__ void k_compact(uint64_t n, const uint32_t* __restrict__ mark,
 
 // Level-k tile (a,b,c) reads source voxels [16a, 16a+16), i.e. source tiles
 // 2a and 2a+1 on each axis.
-__global__ void k_mark_parent(LevelMap src, LevelMap dst, uint32_t* __restrict__ markDst) {
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= src.numTiles) return;
-    if (src.tileIdx[t] < 0) return;
-    uint32_t tx, ty, tz;
-    tile_xyz(t, src.TG, tx, ty, tz);
-    uint32_t px = tx >> 1, py = ty >> 1, pz = tz >> 1;
-    if (px >= dst.TG[0] || py >= dst.TG[1] || pz >= dst.TG[2]) return;
-    markDst[((uint64_t)pz * dst.TG[1] + py) * dst.TG[0] + px] = 1u;
+__global__ void k_mark_children(LevelMap src, LevelMap dst, uint32_t* __restrict__ markDst) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dst.numTiles) return;
+    uint32_t px, py, pz;
+    tile_xyz(t, dst.TG, px, py, pz);
+    uint32_t m = 0u;
+#pragma unroll
+    for (uint32_t c = 0; c < 8u; ++c) {
+        const uint32_t x = 2 * px + (c & 1u), y = 2 * py + ((c >> 1) & 1u), z = 2 * pz + (c >> 2);
+        if (x < src.TG[0] && y < src.TG[1] && z < src.TG[2] &&
+            src.tileIdx[((uint64_t)z * src.TG[1] + y) * src.TG[0] + x] >= 0)
+            m = 1u;
+    }
+    markDst[t] = m;
 }
 
 // ---------------------------------------------------------------------------
@@ -1192,7 +1197,15 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init,
     if (tid < (kFillTiles + 1) * kTileWords) {
         const int q = tid / kTileWords, wd = tid % kTileWords;
         const int32_t idx = s_idx[q];
-        s_hit[q][wd] = idx >= 0 ? hit[(uint64_t)idx * kTileWords + wd] : 0u;
+        // the voxelize pass wrote hit words for the tiles holding triangles
+        // of this group; every other tile has no hit of it
+        uint32_t w = 0u;
+        if (idx >= 0) {
+            const uint64_t key = (uint64_t)g * P.numTiles + tlin + (uint64_t)q * slab;
+            const uint32_t hw = hit[(uint64_t)idx * kTileWords + wd];   // beside the bin loads
+            w = binStart[key + 1] > binStart[key] ? hw : 0u;
+        }
+        s_hit[q][wd] = w;
     }
     __syncthreads();
     if (tid < 64) {
@@ -1553,20 +1566,24 @@ __global__ void k_range_counts(const uint32_t* __restrict__ excl, RangeOffs offs
     if (d < n) out[d] = excl[offs.v[d + 1]] - excl[offs.v[d]];
 }
 
-__global__ void k_brick_assign(uint64_t nodes, const uint32_t* __restrict__ flag,
-                               const uint32_t* __restrict__ excl, uint32_t* __restrict__ bidx,
-                               uint32_t* __restrict__ brickNode, uint8_t* __restrict__ types) {
-    uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nodes) return;
-    types[n] = flag[n] ? 1 : 0;
-    if (flag[n]) {
-        const uint32_t b = excl[n] - excl[0];   // excl starts at the depth's first node
-        bidx[n] = b;
-        brickNode[b] = (uint32_t)n;
+__global__ void k_brick_assign_all(uint64_t allNodes, const uint32_t* __restrict__ flag,
+                                   const uint32_t* __restrict__ excl, DepthOut D) {
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= allNodes) return;
+    int d = 0;
+    while (d + 1 < D.L && n >= D.off[d + 1]) ++d;
+    const uint64_t ln = n - D.off[d];
+    const uint32_t f = flag[n];
+    D.types[d][ln] = f ? 1 : 0;
+    if (f) {
+        const uint32_t b = excl[n] - excl[D.off[d]];
+        D.bidx[d][ln] = b;
+        D.brickNode[d][b] = (uint32_t)ln;
     } else {
-        bidx[n] = 0xffffffffu;
+        D.bidx[d][ln] = 0xffffffffu;
     }
 }
+
 
 // brick gather: WriteBinaryBrick / WriteUByteBinaryBrick middle-brick case
 // (VoxelBrickWriter.cu 725-818, 509-588): (B+2)^3 voxels, RGBA8 colour and
@@ -1791,7 +1808,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums(uint32_t* __restrict__ sum
 
 __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict__ in, uint64_t n,
                                                       const uint32_t* __restrict__ sums,
-                                                      uint32_t* __restrict__ out) {
+                                                      uint32_t* __restrict__ out,
+                                                      uint32_t* __restrict__ clearIn) {
     __shared__ uint32_t s_w[kBlock / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     uint32_t vals[kScanItems];
@@ -1807,7 +1825,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint64_t i = base + k;
-        if (i < n) out[i] = e;
+        if (i < n) {
+            out[i] = e;
+            if (clearIn) clearIn[i] = 0u;   // the same words as in (read above)
+        }
         e += vals[k];
     }
 }
@@ -1900,6 +1921,21 @@ __global__ void k_copy_u128(const uint4* __restrict__ src, uint4* __restrict__ d
         dst[i] = src[i];
 }
 
+struct FillSegs {
+    FillSeg s[4];
+    int k;
+};
+// Grid-stride over the words of all segments, one after the other.
+__global__ void k_fill_segs(FillSegs F) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
+        uint64_t r = i;
+        int k = 0;
+        while (k < F.k && r >= F.s[k].n) { r -= F.s[k].n; ++k; }
+        if (k >= F.k) return;
+        F.s[k].p[r] = F.s[k].v;
+    }
+}
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -1944,6 +1980,20 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
     if (blocks > 65536) blocks = 65536;
     k_fill_u32<<<(unsigned)blocks, kBlock, 0, st>>>(p, n, v);
 }
+void launch_fill_segs(hipStream_t st, const FillSeg* segs, int nseg) {
+    FillSegs F{};
+    uint64_t total = 0;
+    F.k = 0;
+    for (int i = 0; i < nseg && F.k < 4; ++i) {
+        if (segs[i].n == 0 || !segs[i].p) continue;
+        F.s[F.k++] = segs[i];
+        total += segs[i].n;
+    }
+    if (total == 0) return;
+    uint64_t blocks = (total + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    k_fill_segs<<<(unsigned)blocks, kBlock, 0, st>>>(F);
+}
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1970,9 +2020,10 @@ void launch_compact(hipStream_t st, uint64_t n, const uint32_t* mark, const uint
     }
     k_compact<<<grid_for(n, kBlock), kBlock, 0, st>>>(n, mark, excl, tileIdx, list, count);
 }
-void launch_mark_parent(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint32_t* markDst) {
-    if (src.numTiles == 0) return;
-    k_mark_parent<<<grid_for(src.numTiles, kBlock), kBlock, 0, st>>>(src, dst, markDst);
+void launch_mark_children(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                          uint32_t* markDst) {
+    if (dst.numTiles == 0) return;
+    k_mark_children<<<grid_for(dst.numTiles, kBlock), kBlock, 0, st>>>(src, dst, markDst);
 }
 void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g, int init,
                      uint64_t nActive, const int32_t* list, const float* verts,
@@ -1989,8 +2040,8 @@ void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int
 void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nActive,
                         const int32_t* list, const uint32_t* binStart, uint4* work,
                         uint32_t* workCount, uint32_t* recCount) {
-    launch_fill_u32(st, workCount, 1, 0u);
-    launch_fill_u32(st, recCount, kRecSegs, 0u);
+    const FillSeg z[2] = {{workCount, 1, 0u}, {recCount, kRecSegs, 0u}};
+    launch_fill_segs(st, z, 2);
     if (nActive == 0) return;
     k_group_tiles<<<grid_for(nActive, kBlock), kBlock, 0, st>>>(P, g, nActive, list, binStart,
                                                                  work, workCount);
@@ -2037,11 +2088,12 @@ void launch_range_counts(hipStream_t st, const uint32_t* excl, const uint64_t* o
     for (int d = 0; d <= n && d <= kMaxLevels; ++d) o.v[d] = offs[d];
     k_range_counts<<<1, 64, 0, st>>>(excl, o, n < kMaxLevels ? n : kMaxLevels, out);
 }
-void launch_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
-                         const uint32_t* excl, uint32_t* bidx, uint32_t* brickNode,
-                         uint8_t* types) {
-    k_brick_assign<<<grid_for(nodes, kBlock), kBlock, 0, st>>>(nodes, flag, excl, bidx, brickNode,
-                                                               types);
+void launch_brick_assign_all(hipStream_t st, const uint32_t* flag, const uint32_t* excl,
+                             const DepthOut& D) {
+    if (D.L <= 0) return;
+    const uint64_t all = D.off[D.L];
+    if (all == 0) return;
+    k_brick_assign_all<<<grid_for(all, kBlock), kBlock, 0, st>>>(all, flag, excl, D);
 }
 void launch_brick_gather(hipStream_t st, const LevelMap& lm, const uint32_t* col,
                          const uint32_t* qn, uint64_t nbricks, const uint32_t* brickNode,
@@ -2085,7 +2137,17 @@ void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, 
     }
     k_scan_reduce<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp);
     k_scan_sums<<<1, kBlock, 0, st>>>(tmp, nb, out, n);
-    k_scan_down<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp, out);
+    k_scan_down<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp, out, nullptr);
+}
+void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp) {
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) {
+        launch_fill_u32(st, out, 1, 0u);
+        return;
+    }
+    k_scan_reduce<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp);
+    k_scan_sums<<<1, kBlock, 0, st>>>(tmp, nb, out, n);
+    k_scan_down<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp, out, in);
 }
 
 }  // namespace voxmi
