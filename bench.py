@@ -154,11 +154,12 @@ def serving_bench(vz, scene, frames: int, W: int, H: int, slots, max_up: int, st
 
 
 def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, slots,
-                max_trees: int, max_loading: int):
+                max_trees: int, max_loading: int, serve_group: int):
     """Served paging: a (cells+1)^2 fBm terrain in a cells^3 virtual grid,
     tiled into tile^3 octrees voxelized on demand on the device as the
     seeded flythrough first sees them (the pager), served from one shared
-    brick pool with LRU eviction."""
+    brick pool with LRU eviction; a server frame follows every serve_group
+    trees' ray casts within a frame."""
     import numpy as np
     from voxvizthesis_amd import scenes
     from voxvizthesis_amd.pager import TilePager
@@ -174,7 +175,7 @@ def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, slot
     t0 = time.perf_counter()
     pg = TilePager([terrain], 1.0, tile, B, slots, max_trees=max_trees,
                    max_loading=max_loading, streams=1, max_width=W, max_height=H,
-                   near=1.0, far=10000.0)
+                   near=1.0, far=10000.0, serve_group=serve_group)
     t_setup = time.perf_counter() - t0
     path = scenes.flythrough(seed=5, frames=frames, extent=extent, terrain_seed=5)
     period = 1.0 / 60.0
@@ -205,6 +206,7 @@ def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, slot
                         f"{cells}^3 virtual grid, "
                         f"{tile}^3 tiles of {B}^3 bricks voxelized on demand, {frames}-frame "
                         f"seeded flythrough, {W}x{H} ray cast",
+            "serve_group": serve_group,
             "tiles": len(pg.trees),
             "tiles_voxelized": int(pg.tiles_loaded),
             "pool_slots": int(np.prod(slots)),
@@ -362,6 +364,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--paged-cells", type=int, default=4096)
     ap.add_argument("--no-paged", action="store_true")
+    ap.add_argument("--paged-group", type=int, default=4,
+                    help="trees per server frame within a paged frame (0: one per frame)")
     ap.add_argument("--batch-jobs", type=int, default=32)
     ap.add_argument("--batch-tmax", type=float, default=2.6e5)
     ap.add_argument("--batch-grids", default="64,128,256")
@@ -520,7 +524,7 @@ def main():
     if not args.no_paged:
         log(f"[bench] paged serving over a {args.paged_cells}^3 virtual grid ...")
         pb = paged_bench(args.paged_cells, args.serve_frames, args.serve_width,
-                         args.serve_height, 1024, 16, (56, 56, 56), 48, 4)
+                         args.serve_height, 1024, 16, (56, 56, 56), 48, 4, args.paged_group)
         out["paged"] = pb
         log(f"[bench] paged: {pb['frame_ms']} ms/frame, {pb['tiles_voxelized']} tiles, "
             f"p99 {pb['latency_p99_ms']} ms")

@@ -3,7 +3,8 @@ in HBM, several at once on worker threads while frames are ray cast, give
 every tile the octree it gets alone (voxelized from host arrays with the
 tile's triangles, in input order), bit-exactly; each tree's ray cast over
 its screen rectangle equals the ray-cast oracle's full frame there; every
-slot of the shared pool holds the brick of the node it serves."""
+slot of the shared pool holds the brick of the node it serves, after frames
+whose requests were served group by group within the frame."""
 import numpy as np
 import pytest
 
@@ -49,7 +50,7 @@ def test_pager_tiles_match_alone_and_pool_is_consistent():
     sc = scenes.scene_full_octree(cells=64, N=64, B=8, n_rocks=60)
     W, H = 64, 48
     pg = TilePager(sc.meshes, 1.0 / 128, 64, 8, (8, 8, 8), max_trees=8, max_loading=3,
-                   streams=1, max_width=W, max_height=H, near=1e-3, far=100.0)
+                   streams=1, max_width=W, max_height=H, near=1e-3, far=100.0, serve_group=2)
     try:
         print(f"[pager] {len(pg.trees)} paged trees, grid {pg.gen.dim}", flush=True)
         assert len(pg.trees) >= 4
@@ -60,6 +61,10 @@ def test_pager_tiles_match_alone_and_pool_is_consistent():
             cam = dict(position=pos, look=look, up=up, fov_y_deg=60.0, width=W, height=H)
             if f in (2, 5):
                 pg.drain()       # loads requested so far become trees now
+            if f not in (3, 6, 9):
+                # server frames after every serve_group casts, within the frame
+                pg.frame([cam])
+                continue
             drawn = pg.frame([cam], serve=False)
             if f in (3, 6, 9):
                 pool = pg.pool.textures()[0]

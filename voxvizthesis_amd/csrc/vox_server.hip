@@ -18,6 +18,7 @@
 // attached trees gives each tree's node arrays, node texture and bricks.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -169,25 +170,42 @@ __global__ void k_srv_touch(const uint32_t* __restrict__ ids, const uint64_t* __
     }
 }
 
-__global__ void k_srv_newflags(uint64_t ntex, const uint32_t* __restrict__ pend,
-                               uint32_t* __restrict__ flag) {
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ntex;
-         t += (uint64_t)gridDim.x * blockDim.x)
-        flag[t] = pend[t] == 2u ? 1u : 0u;
+// Tree slots whose request lists a server frame reads, ascending: only their
+// texels can have become pending in it, so the pending pass walks their
+// ranges (n == 0: every texel).  Position v of the walk is texel
+// s[v / cap] * cap + v % cap, ascending in v.
+constexpr uint32_t kMaxPart = 64;
+struct PartSlots {
+    uint32_t n;
+    uint32_t s[kMaxPart];
+};
+__device__ __forceinline__ uint64_t part_texel(const PartSlots& P, uint64_t cap, uint64_t v) {
+    if (P.n == 0) return v;
+    const uint64_t k = v / cap;
+    return (uint64_t)P.s[k] * cap + (v - k * cap);
+}
+
+__global__ void k_srv_newflags(uint64_t nv, PartSlots P, uint64_t cap,
+                               const uint32_t* __restrict__ pend, uint32_t* __restrict__ flag) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nv;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        flag[v] = pend[part_texel(P, cap, v)] == 2u ? 1u : 0u;
 }
 
 // 2. append the new pending texels to the FIFO in ascending order.
 // ctr: [0] head, [1] tail (monotonic), [2] admitted this frame
-__global__ void k_srv_enqueue(uint64_t ntex, const uint32_t* __restrict__ flag,
+__global__ void k_srv_enqueue(uint64_t nv, PartSlots P, uint64_t tcap,
+                              const uint32_t* __restrict__ flag,
                               const uint32_t* __restrict__ excl, uint32_t* __restrict__ pend,
                               const unsigned long long* __restrict__ pendEmit,
                               uint32_t* __restrict__ ring, unsigned long long* __restrict__ ringEmit,
                               uint32_t cap, const uint32_t* __restrict__ ctr) {
     const uint32_t tail = ctr[1];
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ntex;
-         t += (uint64_t)gridDim.x * blockDim.x) {
-        if (!flag[t]) continue;
-        const uint32_t pos = (uint32_t)(((uint64_t)tail + excl[t]) % cap);
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nv;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        if (!flag[v]) continue;
+        const uint64_t t = part_texel(P, tcap, v);
+        const uint32_t pos = (uint32_t)(((uint64_t)tail + excl[v]) % cap);
         ring[pos] = (uint32_t)t;
         ringEmit[pos] = pendEmit[t];
         pend[t] = 1u;
@@ -723,13 +741,26 @@ int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n)
             (uint32_t)(slot[k] * s->cap), frame, tabs, s->info.as<uint32_t>(),
             s->nodeSlot.as<uint32_t>(), s->slotLast.as<uint32_t>(), s->pend.as<uint32_t>(),
             s->pendEmit.as<unsigned long long>(), s->tot.as<unsigned long long>());
-    k_srv_newflags<<<grid_for(nt), kB, 0, st>>>(nt, s->pend.as<uint32_t>(), s->flagT.as<uint32_t>());
-    launch_scan(st, s->flagT.as<uint32_t>(), s->exclT.as<uint32_t>(), nt, s->scanTmpT.as<uint32_t>());
-    k_srv_enqueue<<<grid_for(nt), kB, 0, st>>>(
-        nt, s->flagT.as<uint32_t>(), s->exclT.as<uint32_t>(), s->pend.as<uint32_t>(),
+    // the pending pass over the texels of the trees in this frame's lists
+    PartSlots ps{};
+    {
+        std::vector<uint32_t> u(slot.begin(), slot.end());
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        if (!u.empty() && u.size() <= kMaxPart) {
+            ps.n = (uint32_t)u.size();
+            for (uint32_t k = 0; k < ps.n; ++k) ps.s[k] = u[k];
+        }
+    }
+    const uint64_t nv = ps.n ? (uint64_t)ps.n * s->cap : nt;
+    k_srv_newflags<<<grid_for(nv), kB, 0, st>>>(nv, ps, s->cap, s->pend.as<uint32_t>(),
+                                                s->flagT.as<uint32_t>());
+    launch_scan(st, s->flagT.as<uint32_t>(), s->exclT.as<uint32_t>(), nv, s->scanTmpT.as<uint32_t>());
+    k_srv_enqueue<<<grid_for(nv), kB, 0, st>>>(
+        nv, ps, s->cap, s->flagT.as<uint32_t>(), s->exclT.as<uint32_t>(), s->pend.as<uint32_t>(),
         s->pendEmit.as<unsigned long long>(), s->ring.as<uint32_t>(),
         s->ringEmit.as<unsigned long long>(), (uint32_t)nt, s->ctr.as<uint32_t>());
-    k_srv_tail<<<1, 64, 0, st>>>(s->exclT.as<uint32_t>(), nt, s->ctr.as<uint32_t>());
+    k_srv_tail<<<1, 64, 0, st>>>(s->exclT.as<uint32_t>(), nv, s->ctr.as<uint32_t>());
     launch_fill_u32(st, s->hist.as<uint32_t>(), kKeys, 0u);
     k_srv_keys<<<grid_for(ns), kB, 0, st>>>((uint32_t)ns, frame, s->slotNode.as<uint32_t>(),
                                              s->slotLast.as<uint32_t>(), s->key.as<uint32_t>(),

@@ -17,9 +17,11 @@ allocation, gather and generate on the handle's own stream while frames go
 on (the library releases the interpreter lock in every call).  A finished
 tile becomes an octree on the shared pool (its root brick admitted through
 the LRU like any other) attached to one brick server; every loaded visible
-tree gets a ray cast per camera stream, and one server frame admits all of
-their requests.  Tiles therefore enter as tree slots and memory free up,
-and their voxelization overlaps with the frames' ray casts on the device.
+tree gets a ray cast per camera stream, and a server frame after every few
+of them (serve_group) admits their requests while the next trees are cast,
+so a request waits for its group's casts rather than the whole frame's.
+Tiles therefore enter as tree slots and memory free up, and their
+voxelization overlaps with the frames' ray casts on the device.
 """
 from __future__ import annotations
 
@@ -144,12 +146,16 @@ class TilePager:
     def __init__(self, meshes, voxel_size, N: int, B: int, pool_slots, max_trees: int = 32,
                  max_uploads: int = 65536, max_loading: int = 4, stale_frames: int = 100,
                  streams: int = 1, max_width: int = 1920, max_height: int = 1080,
-                 near: float = 1e-3, far: float = 100.0, options: int = 0):
+                 near: float = 1e-3, far: float = 100.0, options: int = 0,
+                 serve_group: int = 4):
         self._L = _lib.load()
         self.meshes = list(meshes)
         self.N, self.B = int(N), int(B)
         self.gen = PagedOctTreeGenerator(self.meshes, voxel_size, self.N, self.B)
         self.max_loading, self.stale = int(max_loading), int(stale_frames)
+        # trees per server frame within a frame (0: one server frame after all
+        # of the frame's ray casts)
+        self.serve_group = int(serve_group)
         self.streams, self.W, self.H = int(streams), int(max_width), int(max_height)
         self.near, self.far, self.options = float(near), float(far), int(options)
         self.frame_index = 0
@@ -348,8 +354,13 @@ class TilePager:
             tr.requested_frame = f
             self._start_load(tr)
             loading += 1
-        # ray casts of every stream's visible trees, one server frame
-        rcs, out = [], []
+        # ray casts of every stream's visible trees.  With serve, a server
+        # frame follows every serve_group casts: their requests are admitted
+        # while the next trees are cast (which wait for that server frame's
+        # node-texture and pool writes), so a request waits for its group,
+        # not for the whole frame.
+        grouped = serve and self.serve_group > 0
+        rcs, out, group = [], [], []
         for k, (cm, trees) in enumerate(zip(cams, drawn)):
             row, cast = [], []
             for tr in trees:
@@ -363,13 +374,23 @@ class TilePager:
                 rcs.append(rc)
                 cast.append(rc)
                 row.append((tr, rc))
+                if grouped:
+                    group.append(rc)
+                    if len(group) >= self.serve_group:
+                        self.server.frame(group, wait=False)
+                        group = []
             # far to near into the stream's frame (the renderer's blending)
             composite(cast, self.frames[k].ptr, cm["width"], cm["height"], clear=True)
             self._frame_dims[k] = (int(cm["width"]), int(cm["height"]))
             out.append(row)
-        self._cast = rcs
-        if serve:
-            self.serve()
+        if grouped:
+            if group:
+                self.server.frame(group, wait=False)
+            self._cast = []
+        else:
+            self._cast = rcs
+            if serve:
+                self.serve()
         self.frame_index += 1
         return out
 
