@@ -1169,7 +1169,16 @@ __global__ void k_group_tiles(VoxParams P, int g, uint64_t nActive, const int32_
 // this tile and the three above gives the nearest source by one bit scan.
 // For the first group the voxelize / colour passes write hit voxels only, so
 // here every other voxel gets its fill value or zero.
-__global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init,
+// One wave per active tile, kBlock / 64 tiles per workgroup; lane c owns the
+// voxel column c = (x, y) of the tile and walks it top down, so a run of
+// filled voxels under one hit voxel reads that source once. Lane q * 16 + w
+// loads hit word w of the tile q tiles above (q = 0: the tile itself), one
+// load for the 64 words of the four tiles; shuffles give each lane its
+// column's hit bits over 32 layers. Voxels neither hit nor filled are not
+// written (they read as zero through LevelMap::defOcc / defFill); the filled
+// ones are recorded in fillw.
+constexpr int kFillWaves = kBlock / 64;
+__global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init, uint64_t nActive,
                                                  const int32_t* __restrict__ list,
                                                  const int32_t* __restrict__ tileIdx,
                                                  const uint32_t* __restrict__ binStart,
@@ -1178,87 +1187,70 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init,
                                                  float* __restrict__ nrm,
                                                  const uint32_t* __restrict__ hit,
                                                  uint32_t* __restrict__ fillw) {
-    __shared__ uint32_t s_hit[kFillTiles + 1][kTileWords];
-    __shared__ int32_t s_idx[kFillTiles + 1];
-    __shared__ uint32_t s_colmask[64];
-    const int tid = threadIdx.x;
-    const uint64_t a = blockIdx.x;
+    static_assert((kFillTiles + 1) * kTileWords == 64, "one hit word per lane");
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint64_t a = (uint64_t)blockIdx.x * kFillWaves + (threadIdx.x >> 6);
+    if (a >= nActive) return;   // whole waves
     const uint64_t tlin = (uint64_t)list[a];
     uint32_t tx_, ty_, tz;
     tile_xyz(tlin, P.TG, tx_, ty_, tz);
     const uint64_t slab = (uint64_t)P.TG[0] * P.TG[1];
-    if (tid <= kFillTiles) {
-        int32_t idx = -1;
-        if (tid == 0) idx = (int32_t)a;
-        else if (tz + tid < P.TG[2]) idx = tileIdx[tlin + (uint64_t)tid * slab];
-        s_idx[tid] = idx;
+    const uint32_t q = (uint32_t)lane >> 4, wd = (uint32_t)lane & 15u;
+    int32_t idx = -1;
+    if (q == 0) idx = (int32_t)a;
+    else if (tz + q < P.TG[2]) idx = tileIdx[tlin + (uint64_t)q * slab];
+    // the voxelize pass wrote hit words for the tiles holding triangles of
+    // this group; every other tile has no hit of it
+    uint32_t hw = 0u;
+    if (idx >= 0) {
+        const uint64_t key = (uint64_t)g * P.numTiles + tlin + (uint64_t)q * slab;
+        const uint32_t w = hit[(uint64_t)idx * kTileWords + wd];   // beside the bin loads
+        hw = binStart[key + 1] > binStart[key] ? w : 0u;
     }
-    __syncthreads();
-    if (tid < (kFillTiles + 1) * kTileWords) {
-        const int q = tid / kTileWords, wd = tid % kTileWords;
-        const int32_t idx = s_idx[q];
-        // the voxelize pass wrote hit words for the tiles holding triangles
-        // of this group; every other tile has no hit of it
-        uint32_t w = 0u;
-        if (idx >= 0) {
-            const uint64_t key = (uint64_t)g * P.numTiles + tlin + (uint64_t)q * slab;
-            const uint32_t hw = hit[(uint64_t)idx * kTileWords + wd];   // beside the bin loads
-            w = binStart[key + 1] > binStart[key] ? hw : 0u;
+    int32_t sidx[kFillTiles + 1];
+#pragma unroll
+    for (int k = 0; k <= kFillTiles; ++k) sidx[k] = __shfl(idx, k * kTileWords);
+    // hit bits of column c, bit k * 8 + z for layer z of the tile k above
+    const uint32_t c = (uint32_t)lane;
+    uint32_t m = 0u;
+#pragma unroll
+    for (int k = 0; k <= kFillTiles; ++k)
+#pragma unroll
+        for (int lz = 0; lz < kTile; ++lz) {
+            const uint32_t word = (uint32_t)__shfl((int)hw, k * kTileWords + lz * 2 + (int)(c >> 5));
+            m |= ((word >> (c & 31u)) & 1u) << (k * kTile + lz);
         }
-        s_hit[q][wd] = w;
-    }
-    __syncthreads();
-    if (tid < 64) {
-        uint32_t m = 0u;
+    uint32_t lastZ = 0xffffffffu, sc = 0u, sq = 0u;
+    float sn0 = 0.0f, sn1 = 0.0f, sn2 = 0.0f;
 #pragma unroll
-        for (int q = 0; q <= kFillTiles; ++q)
+    for (int lz = kTile - 1; lz >= 0; --lz) {
+        const uint32_t v = ((uint32_t)lz << 6) | c;
+        const uint32_t above = (m >> (lz + 1)) & ((1u << kFillDepth) - 1u);
+        const bool filled = !((m >> lz) & 1u) && above != 0u;
+        if (filled) {
+            const uint32_t zz = (uint32_t)lz + (uint32_t)__ffs(above);
+            if (zz != lastZ) {   // a new source: read it once
+                const uint32_t qq = zz >> 3, v2 = ((zz & 7u) << 6) | c;
+                int32_t s_ = sidx[0];
 #pragma unroll
-            for (int lz = 0; lz < kTile; ++lz) {
-                const uint32_t v2 = ((uint32_t)lz << 6) | (uint32_t)tid;
-                m |= ((s_hit[q][v2 >> 5] >> (v2 & 31u)) & 1u) << (q * kTile + lz);
+                for (int k = 1; k <= kFillTiles; ++k) s_ = qq == (uint32_t)k ? sidx[k] : s_;
+                const uint64_t sa = (uint64_t)s_;
+                sc = col[sa * kTileVox + v2];
+                sq = qn[sa * kTileVox + v2];
+                sn0 = nrm[(sa * 3 + 0) * kTileVox + v2];
+                sn1 = nrm[(sa * 3 + 1) * kTileVox + v2];
+                sn2 = nrm[(sa * 3 + 2) * kTileVox + v2];
+                lastZ = zz;
             }
-        s_colmask[tid] = m;
-    }
-    __syncthreads();
-    // Voxels neither hit nor filled are not written: they read as zero
-    // (LevelMap::defOcc / defFill); the filled ones are recorded in fillw.
-    // A filled voxel copies a hit voxel (never itself written here), so the
-    // loads of both passes go out before the first store.
-    constexpr int kPasses = kTileVox / kBlock;
-    bool filled[kPasses];
-    uint32_t cv[kPasses], qv[kPasses];
-    float nv[kPasses][3];
-#pragma unroll
-    for (int pass = 0; pass < kPasses; ++pass) {
-        const uint32_t v = (uint32_t)(pass * kBlock + tid);
-        const uint32_t c = v & 63u, lz = v >> 6;
-        const uint32_t m = s_colmask[c];
-        const uint32_t above = (m >> (lz + 1u)) & ((1u << kFillDepth) - 1u);
-        filled[pass] = !((m >> lz) & 1u) && above != 0u;
-        cv[pass] = qv[pass] = 0u;
-        nv[pass][0] = nv[pass][1] = nv[pass][2] = 0.0f;
-        if (filled[pass]) {
-            const uint32_t zz = lz + (uint32_t)__ffs(above);
-            const uint32_t q = zz >> 3, v2 = ((zz & 7u) << 6) | c;
-            const uint64_t sa = (uint64_t)s_idx[q];
-            cv[pass] = col[sa * kTileVox + v2];
-            qv[pass] = qn[sa * kTileVox + v2];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) nv[pass][k] = nrm[(sa * 3 + k) * kTileVox + v2];
+            col[a * kTileVox + v] = sc;
+            qn[a * kTileVox + v] = sq;
+            nrm[(a * 3 + 0) * kTileVox + v] = sn0;
+            nrm[(a * 3 + 1) * kTileVox + v] = sn1;
+            nrm[(a * 3 + 2) * kTileVox + v] = sn2;
         }
-    }
-#pragma unroll
-    for (int pass = 0; pass < kPasses; ++pass) {
-        const uint32_t v = (uint32_t)(pass * kBlock + tid);
-        if (filled[pass]) {
-            col[a * kTileVox + v] = cv[pass];
-            qn[a * kTileVox + v] = qv[pass];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) nrm[(a * 3 + k) * kTileVox + v] = nv[pass][k];
-        }
-        const uint64_t fb = __ballot(filled[pass]);
-        if ((tid & 63) == 0) {
-            const uint64_t w = a * kTileWords + (v >> 5);   // v >> 5 is even
+        const uint64_t fb = __ballot(filled);
+        if (lane == 0) {
+            const uint64_t w = a * kTileWords + (uint64_t)lz * 2u;
             if (init) {
                 fillw[w] = (uint32_t)fb;
                 fillw[w + 1] = (uint32_t)(fb >> 32);
@@ -2059,8 +2051,8 @@ void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t n
                  const int32_t* list, const int32_t* tileIdx, const uint32_t* binStart,
                  uint32_t* col, uint32_t* qn, float* nrm, const uint32_t* hit, uint32_t* fillw) {
     if (nActive == 0) return;
-    k_fill<<<(unsigned)nActive, kBlock, 0, st>>>(P, g, init, list, tileIdx, binStart, col, qn, nrm,
-                                                 hit, fillw);
+    k_fill<<<(unsigned)((nActive + kFillWaves - 1) / kFillWaves), kBlock, 0, st>>>(
+        P, g, init, nActive, list, tileIdx, binStart, col, qn, nrm, hit, fillw);
 }
 void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
                 const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
