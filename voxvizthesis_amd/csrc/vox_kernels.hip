@@ -1620,7 +1620,7 @@ __global__ __launch_bounds__(kBlock) void k_brick_gather(
     const float invx = 1.0f / (float)ex, invxy = 1.0f / (float)exy;
     // kGatherBatch voxels per thread at a time: their loads are all issued
     // before the first store, so they are in flight together.
-    constexpr uint32_t kGatherBatch = 8;
+    constexpr uint32_t kGatherBatch = 16;
     for (uint32_t i0 = threadIdx.x; i0 < nvox; i0 += kGatherBatch * kBlock) {
         uint32_t c[kGatherBatch], q[kGatherBatch];
 #pragma unroll
