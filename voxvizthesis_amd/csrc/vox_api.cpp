@@ -624,11 +624,10 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
                          h->bbox.as<uint32_t>(), cnt);
     });
     // the scan leaves the counts zeroed for the scatter pass to count again
+    if (!h->scanTmp.ensure(scan_tmp_words(h->numBins) * 4, h->err, "scan scratch")) return 0;
     h->run("scan", [&] {
-        if (h->scanTmp.ensure(scan_tmp_words(h->numBins) * 4, h->err, "scan scratch"))
-            launch_scan_clear(h->st, cnt, start, h->numBins, h->scanTmp.as<uint32_t>());
+        launch_scan_clear(h->st, cnt, start, h->numBins, h->scanTmp.as<uint32_t>());
     });
-    if (!h->scanTmp.p) return 0;
     uint32_t total = 0;
     if (h->replay) {
         total = (uint32_t)h->planPairs;
