@@ -233,46 +233,84 @@ def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, slot
 def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 5):
     """Batched voxelization: n_jobs independent meshes (half fBm heightfield
     tiles, half random soups, triangle counts log-uniform 2K..tmax, grids from
-    `grids`) captured into one hipGraph.  The autotuner picks each job's brick
-    edge (8/16/32) and persistent launch shape by timing eager runs."""
+    `grids`) captured into hipGraphs.  The autotuner picks each job's brick
+    edge (8/16/32) and persistent launch shape by timing eager runs.  Jobs are
+    admitted by free device memory: while they fit, they join the current
+    graph; when setting one up runs out of memory, the resident jobs are
+    captured into one graph, timed and freed, and the next graph starts with
+    that job (every job keeps its working buffers while its graph is
+    resident).  Times are summed over the graphs."""
     import numpy as np
     from voxvizthesis_amd import scenes
     from voxvizthesis_amd.batch import Batch, autotune, make_job
 
-    jobs_sc = scenes.batch_scenes(seed=4, n_jobs=n_jobs, tmax=tmax, grids=grids)
-    choices, jobs = [], []
     t0 = time.perf_counter()
-    for sc in jobs_sc:
-        if do_autotune:
-            B, pb, _ = autotune(sc, reps=1)
-        else:
-            B, pb = 8, 0
-        choices.append((B, pb))
-        jobs.append(make_job(sc, B, pb))
-    t_tune = time.perf_counter() - t0
-    # eager: every job's setup + generate back to back (host round trips)
-    eager = []
-    for _ in range(reps):
+    jobs_sc = scenes.batch_scenes(seed=4, n_jobs=n_jobs, tmax=tmax, grids=grids)
+    log(f"[bench] batch: {n_jobs} meshes generated in {time.perf_counter() - t0:.1f} s")
+    choices, t_tune = [], 0.0
+    eager_ms = graph_ms = device_ms = check_ms = 0.0
+    nodes, graphs, done = 0, 0, 0
+
+    def oom(e):
+        return "out of memory" in str(e)
+
+    pending = list(jobs_sc)
+    while pending:
+        jobs = []
         t0 = time.perf_counter()
+        while pending:
+            sc = pending[0]
+            try:
+                if do_autotune:
+                    B, pb, _ = autotune(sc, reps=1)
+                else:
+                    B, pb = 8, 0
+                v = make_job(sc, B, pb)
+            except RuntimeError as e:
+                if not oom(e) or not jobs:
+                    raise
+                break
+            jobs.append(v)
+            choices.append((B, pb))
+            pending.pop(0)
+            done += 1
+            if done % 16 == 0:
+                log(f"[bench] batch: {done} of {n_jobs} jobs tuned and set up")
+        t_tune += time.perf_counter() - t0
+        # eager: every job's setup + generate back to back (host round trips)
+        eager = []
+        for _ in range(reps):
+            t1 = time.perf_counter()
+            for v in jobs:
+                v.computeEdgesFaceNormalsAndBounds()
+                v.generateVoxelsAndOctTree(None)
+            eager.append(time.perf_counter() - t1)
+        bt = Batch(jobs)
+        bt.capture()
+        graph, check = [], []
+        for _ in range(reps):
+            t1 = time.perf_counter()
+            if not bt.launch():
+                raise RuntimeError(bt.getErrorMessage())
+            graph.append(time.perf_counter() - t1)
+            check.append(bt.stats().ms_check_last)
+        st = bt.stats()
+        eager_ms += float(np.median(eager)) * 1e3
+        graph_ms += float(np.median(graph)) * 1e3
+        device_ms += st.ms_graph_last
+        check_ms += float(np.median(check))
+        nodes += int(st.graph_nodes)
+        graphs += 1
+        bt.close()
         for v in jobs:
-            v.computeEdgesFaceNormalsAndBounds()
-            v.generateVoxelsAndOctTree(None)
-        eager.append(time.perf_counter() - t0)
-    b = Batch(jobs)
-    b.capture()
-    graph, check = [], []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        if not b.launch():
-            raise RuntimeError(b.getErrorMessage())
-        graph.append(time.perf_counter() - t0)
-        st = b.stats()
-        check.append(st.ms_check_last)
-    st = b.stats()
+            v.close()
+        if pending:
+            log(f"[bench] batch: graph {graphs} of {len(jobs)} jobs done, "
+                f"{len(pending)} jobs left")
     T = sum(sc.ntri for sc in jobs_sc)
     vox = sum(sc.N[0] ** 3 for sc in jobs_sc)
-    g = float(np.median(graph))
-    out = {
+    g = graph_ms * 1e-3
+    return {
         "workload": f"{n_jobs} meshes (fBm tiles + random soups), triangles log-uniform "
                     f"2K-{int(tmax)}, grids {list(grids)}",
         "jobs": n_jobs,
@@ -281,19 +319,16 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
         "autotuned": bool(do_autotune),
         "brick_edges": {str(k): sum(1 for c in choices if c[0] == k) for k in (8, 16, 32)},
         "autotune_s": round(t_tune, 2),
-        "eager_ms": round(float(np.median(eager)) * 1e3, 3),
-        "graph_ms": round(g * 1e3, 3),
-        "graph_device_ms": round(st.ms_graph_last, 3),
-        "checksum_ms": round(float(np.median(check)), 3),
-        "graph_nodes": int(st.graph_nodes),
+        "graphs": graphs,
+        "eager_ms": round(eager_ms, 3),
+        "graph_ms": round(graph_ms, 3),
+        "graph_device_ms": round(device_ms, 3),
+        "checksum_ms": round(check_ms, 3),
+        "graph_nodes": nodes,
         "meshes_per_s": round(n_jobs / g, 1),
         "gvoxels_per_s": round(vox / g / 1e9, 3),
         "mtriangles_per_s": round(T / g / 1e6, 2),
     }
-    b.close()
-    for v in jobs:
-        v.close()
-    return out
 
 
 def cpu_baseline(sample_n: int, all_core_n: int = 256):

@@ -47,3 +47,25 @@ def test_device_array_roundtrip():
         assert (back == a).all()
     finally:
         d.close()
+
+
+def test_failed_allocation_leaves_later_calls_working():
+    # a device allocation that cannot succeed fails cleanly, and the runtime
+    # error it raised is not reported again by the next calls
+    L = _lib.load()
+    assert not L.vox_device_alloc(1 << 60)
+    a = np.arange(64, dtype=np.uint32)
+    d = _lib.DeviceArray(a)
+    try:
+        back = np.empty_like(a)
+        assert L.vox_pool_copy_to_host(back.ctypes.data, d.ptr, a.nbytes)
+        assert (back == a).all()
+    finally:
+        d.close()
+    from voxvizthesis_amd import scenes
+    from voxvizthesis_amd.voxelizer import voxelize_scene
+    vz = voxelize_scene(scenes.scene_two_triangles())
+    try:
+        assert vz.status == 1, vz.getErrorMessage()
+    finally:
+        vz.close()

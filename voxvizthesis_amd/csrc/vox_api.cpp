@@ -1394,7 +1394,10 @@ int vox_wait(vox_ctx* h, int32_t block) {
 
 void* vox_device_alloc(uint64_t bytes) {
     void* p = nullptr;
-    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();   // not left behind as the runtime's last error
+        return nullptr;
+    }
     return p;
 }
 

@@ -25,6 +25,9 @@ struct DevBuf {
         if (e != hipSuccess) {
             err << "hipMalloc failed for " << what << " (" << b << " bytes): "
                 << hipGetErrorString(e) << std::endl;
+            // the runtime keeps the failure as its last error: clear it, or
+            // the next launch check would report it again
+            (void)hipGetLastError();
             p = nullptr;
             return false;
         }
