@@ -180,8 +180,9 @@ int vox_server_frame(vox_server* s, vox_raycaster* rc);
 int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n);
 int vox_server_get_stats(vox_server* s, vox_server_stats* out);
 /* Latencies (device clock ticks, see clock_khz) of the admitted requests
- * since creation, in admission order; copies min(n, cap) values, *n = total
- * (at most 2^22 are kept). */
+ * since creation, in admission order: the first 2^24 are kept; *n = the
+ * number kept, min(*n, cap) of them are copied (out may be NULL to ask for
+ * *n).  vox_server_get_stats counts every upload. */
 int vox_server_latencies(vox_server* s, uint64_t* out, uint64_t cap, uint64_t* n);
 /* Slot of each global node texel (0xffffffff: none) and global node texel
  * of each slot, device pointers (waits for the server's queued work). */

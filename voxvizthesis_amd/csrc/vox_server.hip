@@ -475,7 +475,7 @@ struct vox_server {
     DevBuf ring2, ringEmit2;  // the queue after a detach's compaction
     DevBuf flagT, exclT, scanTmpT, key, hist, sel, flagS, eqExclS, exclS, scanTmpS;
     DevBuf req, reqEmit, nreq, done, log, logCount;
-    uint64_t logCap = 1u << 22;
+    uint64_t logCap = 1u << 24;   // latencies kept (the first ones admitted)
     vox_server_stats stats{};
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::stringstream err;
@@ -839,8 +839,9 @@ int vox_server_latencies(vox_server* s, uint64_t* out, uint64_t cap, uint64_t* n
     if (!s->check(hipMemcpyAsync(&cnt, s->logCount.p, 8, hipMemcpyDeviceToHost, s->st), "count") ||
         !s->check(hipStreamSynchronize(s->st), "synchronize"))
         return 0;
-    *n = cnt;
-    uint64_t m = cnt < s->logCap ? cnt : s->logCap;
+    const uint64_t kept = cnt < s->logCap ? cnt : s->logCap;
+    *n = kept;
+    uint64_t m = kept;
     if (m > cap) m = cap;
     if (m && out && !s->check(hipMemcpy(out, s->log.p, 8 * m, hipMemcpyDeviceToHost), "log"))
         return 0;
