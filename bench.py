@@ -399,6 +399,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--paged-cells", type=int, default=4096)
     ap.add_argument("--no-paged", action="store_true")
+    ap.add_argument("--paged-trees", type=int, default=96,
+                    help="octrees the paged leg keeps loaded (tree slots of the server)")
     ap.add_argument("--paged-group", type=int, default=4,
                     help="trees per server frame within a paged frame (0: one per frame)")
     ap.add_argument("--batch-jobs", type=int, default=32)
@@ -559,7 +561,8 @@ def main():
     if not args.no_paged:
         log(f"[bench] paged serving over a {args.paged_cells}^3 virtual grid ...")
         pb = paged_bench(args.paged_cells, args.serve_frames, args.serve_width,
-                         args.serve_height, 1024, 16, (56, 56, 56), 48, 4, args.paged_group)
+                         args.serve_height, 1024, 16, (56, 56, 56), args.paged_trees, 4,
+                         args.paged_group)
         out["paged"] = pb
         log(f"[bench] paged: {pb['frame_ms']} ms/frame, {pb['tiles_voxelized']} tiles, "
             f"p99 {pb['latency_p99_ms']} ms")
