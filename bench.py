@@ -401,7 +401,7 @@ def main():
     ap.add_argument("--no-paged", action="store_true")
     ap.add_argument("--paged-trees", type=int, default=96,
                     help="octrees the paged leg keeps loaded (tree slots of the server)")
-    ap.add_argument("--paged-group", type=int, default=4,
+    ap.add_argument("--paged-group", type=int, default=3,
                     help="trees per server frame within a paged frame (0: one per frame)")
     ap.add_argument("--batch-jobs", type=int, default=32)
     ap.add_argument("--batch-tmax", type=float, default=2.6e5)

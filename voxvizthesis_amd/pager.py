@@ -147,7 +147,7 @@ class TilePager:
                  max_uploads: int = 65536, max_loading: int = 4, stale_frames: int = 100,
                  streams: int = 1, max_width: int = 1920, max_height: int = 1080,
                  near: float = 1e-3, far: float = 100.0, options: int = 0,
-                 serve_group: int = 4):
+                 serve_group: int = 3):
         self._L = _lib.load()
         self.meshes = list(meshes)
         self.N, self.B = int(N), int(B)
