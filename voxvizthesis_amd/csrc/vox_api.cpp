@@ -194,7 +194,7 @@ inline uint64_t trunc_u(float f) {
 }
 
 bool scan(vox_ctx* h, const uint32_t* in, uint32_t* out, uint64_t n) {
-    if (!h->scanTmp.ensure(scan_tmp_words(n) * 4, h->err, "scan scratch")) return false;
+    if (!ensure_scan_tmp(h->scanTmp, n, h->st, h->err, "scan scratch")) return false;
     launch_scan(h->st, in, out, n, h->scanTmp.as<uint32_t>());
     return true;
 }
@@ -624,7 +624,7 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
                          h->bbox.as<uint32_t>(), cnt);
     });
     // the scan leaves the counts zeroed for the scatter pass to count again
-    if (!h->scanTmp.ensure(scan_tmp_words(h->numBins) * 4, h->err, "scan scratch")) return 0;
+    if (!ensure_scan_tmp(h->scanTmp, h->numBins, h->st, h->err, "scan scratch")) return 0;
     h->run("scan", [&] {
         launch_scan_clear(h->st, cnt, start, h->numBins, h->scanTmp.as<uint32_t>());
     });

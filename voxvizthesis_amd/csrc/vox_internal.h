@@ -271,8 +271,13 @@ void launch_brick_upload(hipStream_t st, const BrickSrc& src, const uint4* req,
                          const uint32_t slots[3], const uint32_t dims[3], uint32_t* col,
                          uint32_t* nrm, uint32_t* tex);
 // exclusive scan of n u32 values; out has n+1 entries, out[n] = total.
-// tmp must hold scan_tmp_words(n) words.
+// tmp must hold scan_tmp_words(n) words, all zero before its first scan
+// (ensure_scan_tmp allocates it so); a scan leaves it ready for the next.
+// Scans that share tmp must not overlap in time; in and out are 16-byte
+// aligned.
 uint64_t scan_tmp_words(uint64_t n);
+bool ensure_scan_tmp(DevBuf& b, uint64_t n, hipStream_t st, std::stringstream& err,
+                     const char* what);
 void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp);
 // The same scan, leaving the input zeroed (for counters that count again).
 void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp);

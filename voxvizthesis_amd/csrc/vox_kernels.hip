@@ -1764,65 +1764,154 @@ __global__ void k_dense_level(LevelMap lm, const uint32_t* __restrict__ col,
 }
 
 // ---------------------------------------------------------------------------
-// Device-wide exclusive scan (reduce, scan of block sums, down-sweep).
-constexpr int kScanItems = 8;
+// Device-wide exclusive scan in one pass (decoupled look-back): each
+// workgroup takes the next tile in launch order from a ticket counter, sums
+// it, publishes the sum, then adds up its predecessors' published values
+// (sums, or inclusive prefixes, which end the walk) and publishes its own
+// inclusive prefix.  A predecessor holds an earlier ticket, so it is running
+// or done and publishes its sum without waiting on anything: every wait ends.
+//
+// Scratch (zeroed once when allocated; each scan leaves it ready for the
+// next): word 0-1 a 64-bit counter (call generation << 32 | next ticket),
+// then one 64-bit status per tile (flag << 32 | value).  Flags carry the
+// call's generation, so statuses of earlier calls read as not ready and
+// nothing is cleared between calls; the holder of the last ticket starts the
+// next generation.
+constexpr int kScanItems = 32;
 constexpr int kScanTile = kBlock * kScanItems;
+// statuses each lane reads per look-back step (a window of 64 * 4 tiles)
+constexpr int kScanLook = 4;
+// bound on status polls (each with a short sleep): a hang guard only, since
+// every awaited status is published without waiting
+constexpr uint32_t kScanSpin = 1u << 22;
 
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in,
-                                                        uint64_t n, uint32_t* __restrict__ sums) {
-    __shared__ uint32_t s_w[kBlock / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-        if (i < n) s += in[i];
-    }
-    uint32_t tot;
-    block_excl_scan(s, s_w, &tot);
-    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+__device__ __forceinline__ uint64_t scan_status_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void scan_status_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_sums(uint32_t* __restrict__ sums, uint64_t nb,
-                                                      uint32_t* __restrict__ out, uint64_t n) {
+__global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ in, uint64_t n,
+                                                 uint32_t nb, uint32_t* __restrict__ out,
+                                                 uint32_t* __restrict__ clearIn,
+                                                 unsigned long long* __restrict__ tmp) {
     __shared__ uint32_t s_w[kBlock / 64];
-    uint32_t carry = 0;
-    for (uint64_t c0 = 0; c0 < nb; c0 += kBlock) {
-        const uint64_t i = c0 + threadIdx.x;
-        const uint32_t v = i < nb ? sums[i] : 0u;
-        uint32_t tot;
-        const uint32_t e = block_excl_scan(v, s_w, &tot);
-        if (i < nb) sums[i] = carry + e;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) out[n] = carry;
-}
+    __shared__ unsigned long long s_tk;
+    __shared__ uint32_t s_prefix;
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (tid == 0) s_tk = atomicAdd(tmp, 1ull);
+    __syncthreads();
+    const unsigned long long tk = s_tk;
+    const uint32_t t = (uint32_t)tk;
+    const uint32_t gen = (uint32_t)(tk >> 32) & 0x3fffffffu;   // flags stay below 2^31 + 1
+    const uint32_t fAgg = 2u * gen + 1u, fPre = 2u * gen + 2u;
+    uint64_t* status = reinterpret_cast<uint64_t*>(tmp + 1);
 
-__global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict__ in, uint64_t n,
-                                                      const uint32_t* __restrict__ sums,
-                                                      uint32_t* __restrict__ out,
-                                                      uint32_t* __restrict__ clearIn) {
-    __shared__ uint32_t s_w[kBlock / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    const uint64_t base = (uint64_t)t * kScanTile + (uint64_t)tid * kScanItems;
+    // a thread's items are contiguous and 128-byte aligned: 16-byte loads
+    // when all are in range
+    const bool full = base + kScanItems <= n;
     uint32_t vals[kScanItems];
-    uint32_t s = 0;
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const uint64_t i = base + k;
-        vals[k] = i < n ? in[i] : 0u;
-        s += vals[k];
-    }
-    uint32_t tot;
-    uint32_t e = block_excl_scan(s, s_w, &tot) + sums[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const uint64_t i = base + k;
-        if (i < n) {
-            out[i] = e;
-            if (clearIn) clearIn[i] = 0u;   // the same words as in (read above)
+        for (int k = 0; k < kScanItems; k += 4) {
+            const uint4 q = *reinterpret_cast<const uint4*>(in + base + k);
+            vals[k] = q.x; vals[k + 1] = q.y; vals[k + 2] = q.z; vals[k + 3] = q.w;
         }
-        e += vals[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const uint64_t i = base + k;
+            vals[k] = i < n ? in[i] : 0u;
+        }
     }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) sum += vals[k];
+    uint32_t tot;
+    const uint32_t e0 = block_excl_scan(sum, s_w, &tot);   // barriers
+    if (tid < 64) {
+        uint32_t excl = 0;
+        if (t == 0) {
+            if (lane == 0) {
+                scan_status_store(status, ((uint64_t)fPre << 32) | tot);
+                // the last ticket starts the next generation at ticket 0
+                if (nb == 1) scan_status_store(reinterpret_cast<uint64_t*>(tmp), (tk + (1ull << 32)) & ~0xffffffffull);
+            }
+        } else {
+            if (lane == 0) scan_status_store(status + t, ((uint64_t)fAgg << 32) | tot);
+            // the wave reads the 64 * kScanLook statuses before tile j0 + 1
+            // at once, lane l those at distances kScanLook * l + q; positions
+            // before tile 0 read as an inclusive prefix of 0
+            int64_t j0 = (int64_t)t - 1;
+            uint32_t polls = 0;
+            while (true) {
+                uint64_t st[kScanLook];
+                while (true) {
+                    bool ready = true;
+#pragma unroll
+                    for (int q = 0; q < kScanLook; ++q) {
+                        const int64_t j = j0 - (int64_t)(kScanLook * lane + q);
+                        st[q] = j >= 0 ? scan_status_load(status + j) : ((uint64_t)fPre << 32);
+                        const uint32_t f = (uint32_t)(st[q] >> 32);
+                        ready = ready && (f == fAgg || f == fPre);
+                    }
+                    if (__ballot(!ready) == 0ull || ++polls >= kScanSpin) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                // the nearest inclusive prefix ends the walk: the lanes
+                // before its lane count in full, its lane up to it
+                int firstPre = kScanLook;
+#pragma unroll
+                for (int q = kScanLook - 1; q >= 0; --q)
+                    if ((uint32_t)(st[q] >> 32) == fPre) firstPre = q;
+                const uint64_t pre = __ballot(firstPre < kScanLook);
+                const int preLane = pre ? __ffsll((unsigned long long)pre) - 1 : 64;
+                const int upto = lane < preLane ? kScanLook : (lane == preLane ? firstPre + 1 : 0);
+                uint32_t v = 0u;
+#pragma unroll
+                for (int q = 0; q < kScanLook; ++q)
+                    if (q < upto) v += (uint32_t)st[q];
+                excl += __shfl(wave_incl_scan(v), 63, 64);
+                if (pre || polls >= kScanSpin) break;
+                j0 -= 64 * kScanLook;
+            }
+            if (lane == 0) {
+                scan_status_store(status + t, ((uint64_t)fPre << 32) | (excl + tot));
+                if (t == nb - 1)
+                    scan_status_store(reinterpret_cast<uint64_t*>(tmp), (tk + (1ull << 32)) & ~0xffffffffull);
+            }
+        }
+        if (lane == 0) s_prefix = excl;
+    }
+    __syncthreads();
+    const uint32_t prefix = s_prefix;
+    uint32_t e = e0 + prefix;
+    // clearIn: the same words as in (read above)
+    if (full) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; k += 4) {
+            uint4 o;
+            o.x = e; e += vals[k];
+            o.y = e; e += vals[k + 1];
+            o.z = e; e += vals[k + 2];
+            o.w = e; e += vals[k + 3];
+            *reinterpret_cast<uint4*>(out + base + k) = o;
+            if (clearIn) *reinterpret_cast<uint4*>(clearIn + base + k) = make_uint4(0u, 0u, 0u, 0u);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const uint64_t i = base + k;
+            if (i < n) {
+                out[i] = e;
+                if (clearIn) clearIn[i] = 0u;
+            }
+            e += vals[k];
+        }
+    }
+    if (t == nb - 1 && tid == 0) out[n] = prefix + tot;
 }
 
 // Order-independent 64-bit checksum of n words: the sum of a mixed
@@ -2120,26 +2209,29 @@ void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
     if (n == 0) return;
     k_dense_level<<<grid_for(n, kBlock), kBlock, 0, st>>>(lm, col, nrm, outCol, outNrm);
 }
-uint64_t scan_tmp_words(uint64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
-void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp) {
+uint64_t scan_tmp_words(uint64_t n) { return 2 * ((n + kScanTile - 1) / kScanTile + 1); }
+bool ensure_scan_tmp(DevBuf& b, uint64_t n, hipStream_t st, std::stringstream& err,
+                     const char* what) {
+    const size_t cap = b.cap;
+    if (!b.ensure(scan_tmp_words(n) * 4, err, what)) return false;
+    if (b.cap != cap) launch_fill_u32(st, b.as<uint32_t>(), b.cap / 4, 0u);
+    return true;
+}
+static void scan_launch(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n,
+                        uint32_t* tmp, uint32_t* clearIn) {
     const uint64_t nb = (n + kScanTile - 1) / kScanTile;
     if (nb == 0) {
         launch_fill_u32(st, out, 1, 0u);
         return;
     }
-    k_scan_reduce<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp);
-    k_scan_sums<<<1, kBlock, 0, st>>>(tmp, nb, out, n);
-    k_scan_down<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp, out, nullptr);
+    k_scan<<<(unsigned)nb, kBlock, 0, st>>>(in, n, (uint32_t)nb, out, clearIn,
+                                             reinterpret_cast<unsigned long long*>(tmp));
+}
+void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp) {
+    scan_launch(st, in, out, n, tmp, nullptr);
 }
 void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp) {
-    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
-    if (nb == 0) {
-        launch_fill_u32(st, out, 1, 0u);
-        return;
-    }
-    k_scan_reduce<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp);
-    k_scan_sums<<<1, kBlock, 0, st>>>(tmp, nb, out, n);
-    k_scan_down<<<(unsigned)nb, kBlock, 0, st>>>(in, n, tmp, out, in);
+    scan_launch(st, in, out, n, tmp, in);
 }
 
 }  // namespace voxmi

@@ -602,12 +602,12 @@ vox_server* vox_server_create_shared(vox_brick_pool* bp, uint32_t max_trees,
         !s->ring2.ensure(4 * nt, s->err, "ring") || !s->ringEmit2.ensure(8 * nt, s->err, "ringEmit") ||
         !s->tot.ensure(32, s->err, "totals") || !s->conflict.ensure(4, s->err, "conflict") ||
         !s->flagT.ensure(4 * nt, s->err, "flagT") || !s->exclT.ensure(4 * (nt + 1), s->err, "exclT") ||
-        !s->scanTmpT.ensure(4 * scan_tmp_words(nt), s->err, "scanTmpT") ||
+        !ensure_scan_tmp(s->scanTmpT, nt, s->st, s->err, "scanTmpT") ||
         !s->key.ensure(4 * ns, s->err, "key") || !s->hist.ensure(4 * kKeys, s->err, "hist") ||
         !s->sel.ensure(16, s->err, "sel") || !s->flagS.ensure(4 * ns, s->err, "flagS") ||
         !s->eqExclS.ensure(4 * (ns + 1), s->err, "eqExclS") ||
         !s->exclS.ensure(4 * (ns + 1), s->err, "exclS") ||
-        !s->scanTmpS.ensure(4 * scan_tmp_words(ns), s->err, "scanTmpS") ||
+        !ensure_scan_tmp(s->scanTmpS, ns, s->st, s->err, "scanTmpS") ||
         !s->req.ensure(16ull * s->maxUp, s->err, "req") ||
         !s->reqEmit.ensure(8ull * s->maxUp, s->err, "reqEmit") || !s->nreq.ensure(4, s->err, "nreq") ||
         !s->done.ensure(8ull * s->maxUp, s->err, "done") ||

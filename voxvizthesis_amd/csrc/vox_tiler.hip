@@ -261,10 +261,10 @@ int vox_tiler_assign(vox_tiler* t, const float* verts, uint64_t ntri, int32_t de
         !t->start.ensure(4 * (n + 1), t->err, "tile offsets") ||
         !t->zmin.ensure(4 * n, t->err, "z min") || !t->zmax.ensure(4 * n, t->err, "z max") ||
         !t->zminf.ensure(4 * n, t->err, "z min") || !t->zmaxf.ensure(4 * n, t->err, "z max") ||
-        !t->scanTmp.ensure(4 * scan_tmp_words(n), t->err, "scan") ||
+        !ensure_scan_tmp(t->scanTmp, n, st, t->err, "scan") ||
         !t->triCnt.ensure(4 * (nt + 1), t->err, "triangle counts") ||
         !t->triOff.ensure(4 * (nt + 1), t->err, "triangle offsets") ||
-        !t->triScan.ensure(4 * scan_tmp_words(nt), t->err, "scan"))
+        !ensure_scan_tmp(t->triScan, nt, st, t->err, "scan"))
         return 0;
     launch_fill_u32(st, t->cnt.as<uint32_t>(), n, 0u);
     k_fill_i32<<<grid_for(n), kB, 0, st>>>(t->zmin.as<int32_t>(), n, 0x7fffffff);
