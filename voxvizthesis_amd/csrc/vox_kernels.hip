@@ -1739,6 +1739,54 @@ __global__ __launch_bounds__(kBlock) void k_occupancy_rows(VoxParams P,
     }
 }
 
+// The same when a voxel row is a whole number of 16-byte runs (S[0] a
+// multiple of 128): a thread takes 16 tiles along x and one occupancy word of
+// each (four voxel rows of the tiles: word g of a tile holds rows 4g .. 4g+3,
+// 8 bits each), loads the 16 words together and writes the four rows' 16
+// bytes each with one store.
+constexpr uint32_t kOccQuadBlock = 128;
+__global__ __launch_bounds__(kOccQuadBlock) void k_occupancy_quads(
+    VoxParams P, const int32_t* __restrict__ tileIdx, const uint32_t* __restrict__ occ,
+    uint32_t* __restrict__ bitmap) {
+    extern __shared__ int32_t s_rowIdx[];   // P.TG[0] entries
+    const uint32_t ty = blockIdx.x % P.TG[1], tz = blockIdx.x / P.TG[1];
+    const uint64_t rowBase = ((uint64_t)tz * P.TG[1] + ty) * P.TG[0];
+    for (uint32_t i = threadIdx.x; i < P.TG[0]; i += kOccQuadBlock)
+        s_rowIdx[i] = tileIdx[rowBase + i];
+    __syncthreads();
+    const uint32_t wpr = P.S[0] >> 5, qpr = wpr >> 2;   // words, 16-byte runs per voxel row
+    const uint32_t ny = P.S[1] - ty * kTile < (uint32_t)kTile ? P.S[1] - ty * kTile : (uint32_t)kTile;
+    const uint32_t nz = P.S[2] - tz * kTile < (uint32_t)kTile ? P.S[2] - tz * kTile : (uint32_t)kTile;
+    for (uint32_t k = threadIdx.x; k < kTileWords * qpr; k += kOccQuadBlock) {
+        const uint32_t g = k / qpr, qx = k - g * qpr;   // occupancy word g, tiles 16qx ..
+        const uint32_t lz = g >> 1, ly0 = (g & 1u) * 4u;
+        if (lz >= nz || ly0 >= ny) continue;
+        uint32_t raw[16];
+#pragma unroll
+        for (uint32_t tt = 0; tt < 16u; ++tt) {
+            const int32_t a = s_rowIdx[qx * 16u + tt];
+            raw[tt] = a >= 0 ? occ[(uint64_t)a * kTileWords + g] : 0u;
+        }
+        const uint64_t Z = (uint64_t)tz * kTile + lz;
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j) {
+            if (ly0 + j >= ny) break;
+            uint32_t wd[4];
+#pragma unroll
+            for (uint32_t b = 0; b < 4u; ++b) {
+                uint32_t x = 0u;
+#pragma unroll
+                for (uint32_t tt = 0; tt < 4u; ++tt)
+                    x |= ((raw[4u * b + tt] >> (8u * j)) & 255u) << (8u * tt);
+                wd[b] = x;
+            }
+            const uint64_t Y = (uint64_t)ty * kTile + ly0 + j;
+            *reinterpret_cast<uint4*>(bitmap + (Z * P.S[1] + Y) * wpr + 4u * qx) =
+                make_uint4(wd[0], wd[1], wd[2], wd[3]);
+        }
+    }
+}
+
 __global__ void k_dense_level(LevelMap lm, const uint32_t* __restrict__ col,
                               const float* __restrict__ nrm, uint32_t* __restrict__ outCol,
                               float* __restrict__ outNrm) {
@@ -2007,20 +2055,30 @@ struct FillSegs {
     int k;
 };
 // Grid-stride over the words of all segments, one after the other.
+// Four words per thread: one 16-byte store where the words are 16-byte
+// aligned and all in range, single-word stores otherwise.
+__device__ __forceinline__ void fill4(uint32_t* p, uint64_t i, uint64_t n, uint32_t v) {
+    if (i + 4 <= n && ((reinterpret_cast<uintptr_t>(p + i) & 15u) == 0u)) {
+        *reinterpret_cast<uint4*>(p + i) = make_uint4(v, v, v, v);
+    } else {
+        for (uint64_t j = i; j < i + 4 && j < n; ++j) p[j] = v;
+    }
+}
+// Segment k is filled in (n_k + 3) / 4 runs of four words.
 __global__ void k_fill_segs(FillSegs F) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
-        uint64_t r = i;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;; q += stride) {
+        uint64_t r = q;
         int k = 0;
-        while (k < F.k && r >= F.s[k].n) { r -= F.s[k].n; ++k; }
+        while (k < F.k && r >= (F.s[k].n + 3) / 4) { r -= (F.s[k].n + 3) / 4; ++k; }
         if (k >= F.k) return;
-        F.s[k].p[r] = F.s[k].v;
+        fill4(F.s[k].p, r * 4, F.s[k].n, F.s[k].v);
     }
 }
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = v;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q * 4 < n; q += stride)
+        fill4(p, q * 4, n, v);
 }
 
 inline unsigned grid_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -2057,7 +2115,7 @@ void launch_copy_u128(hipStream_t st, const void* src, void* dst, uint64_t n16) 
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
     if (n == 0) return;
-    uint64_t blocks = (n + kBlock - 1) / kBlock;
+    uint64_t blocks = ((n + 3) / 4 + kBlock - 1) / kBlock;
     if (blocks > 65536) blocks = 65536;
     k_fill_u32<<<(unsigned)blocks, kBlock, 0, st>>>(p, n, v);
 }
@@ -2068,7 +2126,7 @@ void launch_fill_segs(hipStream_t st, const FillSeg* segs, int nseg) {
     for (int i = 0; i < nseg && F.k < 4; ++i) {
         if (segs[i].n == 0 || !segs[i].p) continue;
         F.s[F.k++] = segs[i];
-        total += segs[i].n;
+        total += (segs[i].n + 3) / 4;
     }
     if (total == 0) return;
     uint64_t blocks = (total + kBlock - 1) / kBlock;
@@ -2200,7 +2258,11 @@ bool launch_occupancy_rows(hipStream_t st, const VoxParams& P, const int32_t* ti
     if ((P.S[0] & 31u) != 0u || P.TG[0] * 4u > 65536u) return false;
     const uint64_t rows = (uint64_t)P.TG[1] * P.TG[2];
     if (rows == 0) return true;
-    k_occupancy_rows<<<(unsigned)rows, kBlock, P.TG[0] * 4u, st>>>(P, tileIdx, occ, bitmap);
+    if ((P.S[0] & 127u) == 0u)
+        k_occupancy_quads<<<(unsigned)rows, kOccQuadBlock, P.TG[0] * 4u, st>>>(P, tileIdx, occ,
+                                                                              bitmap);
+    else
+        k_occupancy_rows<<<(unsigned)rows, kBlock, P.TG[0] * 4u, st>>>(P, tileIdx, occ, bitmap);
     return true;
 }
 void launch_dense_level(hipStream_t st, const LevelMap& lm, const uint32_t* col,
