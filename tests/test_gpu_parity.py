@@ -66,3 +66,10 @@ def test_type_x_split_miss():
     # N=256, B=4: the leaf typing launch split leaves x = N + extra (the +1
     # border of the last node column) unexamined.
     _check(scenes.scene_terrain_n(128, 256, 4), levels=False)
+
+
+def test_brick_sizes_not_powers_of_two():
+    # (B + 2)^3 odd (B = 3, 5): the brick gather writes single words; B = 6
+    # writes 16-byte runs of a region that starts mid-tile
+    for N, B in ((48, 3), (80, 5), (96, 6)):
+        _check(scenes.scene_terrain_n(64, N, B))
