@@ -1856,29 +1856,46 @@ __global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ in
     const uint32_t fAgg = 2u * gen + 1u, fPre = 2u * gen + 2u;
     uint64_t* status = reinterpret_cast<uint64_t*>(tmp + 1);
 
-    const uint64_t base = (uint64_t)t * kScanTile + (uint64_t)tid * kScanItems;
-    // a thread's items are contiguous and 128-byte aligned: 16-byte loads
-    // when all are in range
-    const bool full = base + kScanItems <= n;
-    uint32_t vals[kScanItems];
-    if (full) {
+    // Wave w owns values [w * kScanTile / 4, (w + 1) * kScanTile / 4) of the
+    // tile in kScanItems / 4 rows of 256; lane l holds values 4l .. 4l + 3 of
+    // each row, so every load and store instruction of a wave covers 1 KB.
+    const uint32_t wave = (uint32_t)tid >> 6;
+    const uint64_t tileBase = (uint64_t)t * kScanTile;
+    const uint64_t waveBase = tileBase + (uint64_t)wave * (kScanTile / 4);
+    constexpr int kRows = kScanItems / 4;
+    const bool full = tileBase + kScanTile <= n;
+    uint4 vals[kRows];
 #pragma unroll
-        for (int k = 0; k < kScanItems; k += 4) {
-            const uint4 q = *reinterpret_cast<const uint4*>(in + base + k);
-            vals[k] = q.x; vals[k + 1] = q.y; vals[k + 2] = q.z; vals[k + 3] = q.w;
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < kScanItems; ++k) {
-            const uint64_t i = base + k;
-            vals[k] = i < n ? in[i] : 0u;
+    for (int k = 0; k < kRows; ++k) {
+        const uint64_t i = waveBase + (uint64_t)k * 256u + (uint64_t)lane * 4u;
+        if (full) {
+            vals[k] = *reinterpret_cast<const uint4*>(in + i);
+        } else {
+            vals[k].x = i < n ? in[i] : 0u;
+            vals[k].y = i + 1 < n ? in[i + 1] : 0u;
+            vals[k].z = i + 2 < n ? in[i + 2] : 0u;
+            vals[k].w = i + 3 < n ? in[i + 3] : 0u;
         }
     }
-    uint32_t sum = 0;
+    // exclusive prefix of each lane's run of four within the wave's values
+    uint32_t rowExcl[kRows];
+    uint32_t waveSum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) sum += vals[k];
-    uint32_t tot;
-    const uint32_t e0 = block_excl_scan(sum, s_w, &tot);   // barriers
+    for (int k = 0; k < kRows; ++k) {
+        const uint32_t sum = vals[k].x + vals[k].y + vals[k].z + vals[k].w;
+        const uint32_t inc = wave_incl_scan(sum);
+        rowExcl[k] = waveSum + inc - sum;
+        waveSum += __shfl(inc, 63, 64);
+    }
+    if (lane == 0) s_w[wave] = waveSum;
+    __syncthreads();
+    uint32_t wavePre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t x = s_w[w];
+        wavePre += (uint32_t)w < wave ? x : 0u;
+        tot += x;
+    }
     if (tid < 64) {
         uint32_t excl = 0;
         if (t == 0) {
@@ -1935,28 +1952,27 @@ __global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ in
     }
     __syncthreads();
     const uint32_t prefix = s_prefix;
-    uint32_t e = e0 + prefix;
     // clearIn: the same words as in (read above)
-    if (full) {
 #pragma unroll
-        for (int k = 0; k < kScanItems; k += 4) {
-            uint4 o;
-            o.x = e; e += vals[k];
-            o.y = e; e += vals[k + 1];
-            o.z = e; e += vals[k + 2];
-            o.w = e; e += vals[k + 3];
-            *reinterpret_cast<uint4*>(out + base + k) = o;
-            if (clearIn) *reinterpret_cast<uint4*>(clearIn + base + k) = make_uint4(0u, 0u, 0u, 0u);
-        }
-    } else {
+    for (int k = 0; k < kRows; ++k) {
+        const uint64_t i = waveBase + (uint64_t)k * 256u + (uint64_t)lane * 4u;
+        uint4 o;
+        o.x = prefix + wavePre + rowExcl[k];
+        o.y = o.x + vals[k].x;
+        o.z = o.y + vals[k].y;
+        o.w = o.z + vals[k].z;
+        if (full) {
+            *reinterpret_cast<uint4*>(out + i) = o;
+            if (clearIn) *reinterpret_cast<uint4*>(clearIn + i) = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            const uint32_t ov[4] = {o.x, o.y, o.z, o.w};
 #pragma unroll
-        for (int k = 0; k < kScanItems; ++k) {
-            const uint64_t i = base + k;
-            if (i < n) {
-                out[i] = e;
-                if (clearIn) clearIn[i] = 0u;
+            for (int e = 0; e < 4; ++e) {
+                if (i + e < n) {
+                    out[i + e] = ov[e];
+                    if (clearIn) clearIn[i + e] = 0u;
+                }
             }
-            e += vals[k];
         }
     }
     if (t == nb - 1 && tid == 0) out[n] = prefix + tot;
