@@ -786,21 +786,37 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     // Node typing per depth (computeOctTreeNodes, Voxelizer.cu 931-1290).
     uint32_t border[3] = {h->extra[0], h->extra[1], h->extra[2]};
     std::vector<std::array<uint32_t, 3>> borders(L);
+    // The samples of all depths in one launch, then each depth's comparison.
+    static_assert(kMaxLevels >= VOX_MAX_LEVELS, "one sample entry per depth");
+    SampleDepths S{};
+    S.L = L;
+    for (int a = 0; a < 3; ++a) S.B[a] = h->B[a];
     for (int d = L - 1; d >= 0; --d) {
-        int k = L - 1 - d;
-        LevelView l = view(h, k);
+        LevelView l = view(h, L - 1 - d);
         DepthBufs& t = h->tree[d];
         borders[d] = {border[0], border[1], border[2]};
         const uint32_t Ckyz[2] = {l.map.dims[1], l.map.dims[2]};
         const uint32_t byz[2] = {border[1], border[2]};
         const uint32_t fyz[2] = {t.Rd * h->B[1], t.Rd * h->B[2]};
-        uint32_t xLimit = type_x_limit(l.map.dims[0], border[0], t.Rd * h->B[0], Ckyz, byz, fyz);
-        h->run("node_type", [&] {
-            launch_node_type(st, l.map, l.active, l.list, l.col, t.Rd, h->B, border, xLimit,
-                             t.cc.as<uint32_t>(), h->nodeFlag.as<uint32_t>() + t.off);
-        });
+        S.lm[d] = l.map;
+        S.col[d] = l.col;
+        S.cc[d] = t.cc.as<uint32_t>();
+        S.flag[d] = h->nodeFlag.as<uint32_t>() + t.off;
+        S.Rd[d] = t.Rd;
+        for (int a = 0; a < 3; ++a) S.border[d][a] = border[a];
+        S.xLimit[d] = type_x_limit(l.map.dims[0], border[0], t.Rd * h->B[0], Ckyz, byz, fyz);
+        S.start[d] = t.off;
         for (int a = 0; a < 3; ++a) border[a] = std::max(border[a] >> 1, 1u);
     }
+    S.start[L] = h->allNodes;
+    h->run("node_type", [&] {
+        launch_node_sample_all(st, S);
+        for (int d = L - 1; d >= 0; --d) {
+            LevelView l = view(h, L - 1 - d);
+            launch_node_compare(st, l.map, l.active, l.list, l.col, S.Rd[d], h->B, S.border[d],
+                                S.xLimit[d], S.cc[d], S.flag[d]);
+        }
+    });
     // One scan over the flags of every depth; a depth's brick addresses are
     // its scan values less the value at its first node.
     h->run("scan", [&] {

@@ -195,9 +195,28 @@ void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t n
 void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
                 const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
                 uint32_t* colDst, uint32_t* qnDst, float* nrmDst);
-void launch_node_type(hipStream_t st, const LevelMap& lm, uint64_t nActive, const int32_t* list,
-                      const uint32_t* col, uint32_t Rd, const uint32_t B[3],
-                      const uint32_t border[3], uint32_t xLimit, uint32_t* cc, uint32_t* flag);
+// Node samples of every depth in one launch (node typing, first half):
+// depth d's nodes are the threads start[d] <= n < start[d + 1]; cc and flag
+// of depth d get each node's sample and whether its brick box reaches an
+// inactive tile.
+struct SampleDepths {
+    LevelMap lm[kMaxLevels];
+    const uint32_t* col[kMaxLevels];
+    uint32_t* cc[kMaxLevels];
+    uint32_t* flag[kMaxLevels];
+    uint32_t Rd[kMaxLevels];
+    uint32_t border[kMaxLevels][3];
+    uint32_t xLimit[kMaxLevels];
+    uint64_t start[kMaxLevels + 1];
+    uint32_t B[3];
+    int L;
+};
+void launch_node_sample_all(hipStream_t st, const SampleDepths& S);
+// Node typing, second half: every voxel of the active tiles compared with the
+// samples of the nodes whose bricks hold it.
+void launch_node_compare(hipStream_t st, const LevelMap& lm, uint64_t nActive, const int32_t* list,
+                         const uint32_t* col, uint32_t Rd, const uint32_t B[3],
+                         const uint32_t border[3], uint32_t xLimit, uint32_t* cc, uint32_t* flag);
 // Brick indices of every depth in one launch over the all-depth node array:
 // node n of depth d (off[d] <= n < off[d + 1]) gets type, brick index
 // (excl[n] - excl[off[d]]) and its brick's node.

@@ -1419,12 +1419,11 @@ __device__ __forceinline__ bool stored(const LevelMap& lm, uint64_t a, uint32_t 
 //                (whose voxels are all zero);
 //   node_compare per active tile: every voxel is read once and compared with
 //                the samples of the (at most 8) nodes whose bricks hold it.
-__global__ void k_node_sample(LevelMap lm, const uint32_t* __restrict__ col, uint32_t Rd,
-                              uint32_t Bx, uint32_t By, uint32_t Bz, uint32_t bx, uint32_t by,
-                              uint32_t bz, uint32_t xLimit, uint32_t* __restrict__ cc,
-                              uint32_t* __restrict__ flag) {
-    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (uint64_t)Rd * Rd * Rd) return;
+__device__ __forceinline__ void node_sample_at(const LevelMap& lm, const uint32_t* __restrict__ col,
+                                               uint32_t Rd, uint32_t Bx, uint32_t By, uint32_t Bz,
+                                               uint32_t bx, uint32_t by, uint32_t bz,
+                                               uint32_t xLimit, uint32_t* __restrict__ cc,
+                                               uint32_t* __restrict__ flag, uint64_t n) {
     uint32_t nx, ny, nz;
     node_xyz(n, (uint32_t)(__ffs(Rd) - 1), nx, ny, nz);
     const uint32_t sx = nx * Bx + bx, sy = ny * By + by, sz = nz * Bz + bz;
@@ -1460,6 +1459,17 @@ __global__ void k_node_sample(LevelMap lm, const uint32_t* __restrict__ col, uin
     }
     cc[n] = sample;
     flag[n] = f;
+}
+// The samples of every depth in one launch: thread n is node n - start[d] of
+// depth d for start[d] <= n < start[d + 1].
+__global__ void k_node_sample_all(SampleDepths S) {
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= S.start[S.L]) return;
+    int d = 0;
+    while (d + 1 < S.L && n >= S.start[d + 1]) ++d;
+    node_sample_at(S.lm[d], S.col[d], S.Rd[d], S.B[0], S.B[1], S.B[2], S.border[d][0],
+                   S.border[d][1], S.border[d][2], S.xLimit[d], S.cc[d], S.flag[d],
+                   n - S.start[d]);
 }
 
 // nodes n whose stored brick [n*B + b - 1, n*B + b + B] holds coordinate X
@@ -2224,13 +2234,13 @@ void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64
     k_mip<<<(unsigned)nActive, kBlock, 0, st>>>(src, dst, listDst, colSrc, nrmSrc, colDst, qnDst,
                                                 nrmDst);
 }
-void launch_node_type(hipStream_t st, const LevelMap& lm, uint64_t nActive, const int32_t* list,
-                      const uint32_t* col, uint32_t Rd, const uint32_t B[3],
-                      const uint32_t border[3], uint32_t xLimit, uint32_t* cc, uint32_t* flag) {
-    const uint64_t nodes = (uint64_t)Rd * Rd * Rd;
-    k_node_sample<<<grid_for(nodes, kBlock), kBlock, 0, st>>>(lm, col, Rd, B[0], B[1], B[2],
-                                                             border[0], border[1], border[2],
-                                                             xLimit, cc, flag);
+void launch_node_sample_all(hipStream_t st, const SampleDepths& S) {
+    if (S.L <= 0 || S.start[S.L] == 0) return;
+    k_node_sample_all<<<grid_for(S.start[S.L], kBlock), kBlock, 0, st>>>(S);
+}
+void launch_node_compare(hipStream_t st, const LevelMap& lm, uint64_t nActive, const int32_t* list,
+                         const uint32_t* col, uint32_t Rd, const uint32_t B[3],
+                         const uint32_t border[3], uint32_t xLimit, uint32_t* cc, uint32_t* flag) {
     if (nActive == 0) return;
     k_node_compare<<<(unsigned)((nActive + kNcTiles - 1) / kNcTiles), kBlock, 0, st>>>(
         lm, list, nActive, col, Rd, B[0], B[1], B[2], border[0], border[1], border[2], xLimit, cc,
