@@ -498,7 +498,8 @@ def main():
         pl = kms / launches
         kb = kernel_bytes(name, scene, st, bricks, B, launches)
         ach = kb / (pl * 1e-3) / 1e9
-        tr = tj.get("k_" + name)
+        # the kernel that does the work of every depth in one launch carries "_all"
+        tr = tj.get("k_" + name) or tj.get("k_" + name + "_all")
         return {
             "kernel": name,
             "bound": "hbm",

@@ -859,17 +859,30 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     h->run("brick_assign", [&] {
         launch_brick_assign_all(st, h->nodeFlag.as<uint32_t>(), h->nodeExcl.as<uint32_t>(), dout);
     });
+    // The bricks of all depths in one launch, the deepest (largest) first.
+    GatherDepths GD{};
+    for (int a = 0; a < 3; ++a) GD.B[a] = h->B[a];
+    uint64_t nbAll = 0;
     for (int d = L - 1; d >= 0; --d) {
-        int k = L - 1 - d;
-        LevelView l = view(h, k);
+        if (nb[d] == 0) continue;
+        LevelView l = view(h, L - 1 - d);
         DepthBufs& t = h->tree[d];
-        uint32_t bd[3] = {borders[d][0], borders[d][1], borders[d][2]};
-        if (nb[d] > 0) h->run("brick_gather", [&] {
-            launch_brick_gather(st, l.map, l.col, l.qn, nb[d],
-                                t.brickNode.as<uint32_t>(), t.Rd, h->B, bd, t.bcol.as<uint32_t>(),
-                                t.bnrm.as<uint32_t>());
-        });
+        const int e = GD.n++;
+        GD.lm[e] = l.map;
+        GD.col[e] = l.col;
+        GD.qn[e] = l.qn;
+        GD.brickNode[e] = t.brickNode.as<uint32_t>();
+        GD.bcol[e] = t.bcol.as<uint32_t>();
+        GD.bnrm[e] = t.bnrm.as<uint32_t>();
+        GD.Rd[e] = t.Rd;
+        for (int a = 0; a < 3; ++a) GD.border[e][a] = borders[d][a];
+        GD.start[e] = nbAll;
+        nbAll += nb[d];
     }
+    GD.start[GD.n] = nbAll;
+    if (nbAll >= (1ull << 31))
+        return h->fail("more bricks than one launch's workgroups address"), 0;
+    if (GD.n > 0) h->run("brick_gather", [&] { launch_brick_gather_all(st, GD); });
     // Compressed output: BC3 colours, BC1 normals (WriteCompressedBinaryBrick,
     // VoxelBrickWriter.cu 590-723, with the project's encoder).
     if (h->compressed) {

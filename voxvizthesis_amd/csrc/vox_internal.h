@@ -236,10 +236,22 @@ struct RangeOffs {
 // out[d] = excl[offs[d + 1]] - excl[offs[d]] for d < n (n <= kMaxLevels).
 void launch_range_counts(hipStream_t st, const uint32_t* excl, const uint64_t* offs, int n,
                          uint32_t* out);
-void launch_brick_gather(hipStream_t st, const LevelMap& lm, const uint32_t* col,
-                         const uint32_t* qn, uint64_t nbricks, const uint32_t* brickNode,
-                         uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
-                         uint32_t* bcol, uint32_t* bnrm);
+// The bricks of every depth in one launch: entry e (a depth with bricks)
+// covers workgroups start[e] <= w < start[e + 1], one brick each.
+struct GatherDepths {
+    LevelMap lm[kMaxLevels];
+    const uint32_t* col[kMaxLevels];
+    const uint32_t* qn[kMaxLevels];
+    const uint32_t* brickNode[kMaxLevels];
+    uint32_t* bcol[kMaxLevels];
+    uint32_t* bnrm[kMaxLevels];
+    uint32_t Rd[kMaxLevels];
+    uint32_t border[kMaxLevels][3];
+    uint64_t start[kMaxLevels + 1];
+    uint32_t B[3];
+    int n;
+};
+void launch_brick_gather_all(hipStream_t st, const GatherDepths& G);
 void launch_collapse(hipStream_t st, uint32_t Rd, int childIsLeaf, uint8_t* types,
                      const uint8_t* childTypes);
 // The whole bitmap (no clear needed) when S[0] is a multiple of 32 and a row

@@ -1591,16 +1591,15 @@ __global__ void k_brick_assign_all(uint64_t allNodes, const uint32_t* __restrict
 // (VoxelBrickWriter.cu 725-818, 509-588): (B+2)^3 voxels, RGBA8 colour and
 // RGB8 normal ((n+1)*0.5*255, truncated).
 
-__global__ __launch_bounds__(kBlock) void k_brick_gather(
-    LevelMap lm, const uint32_t* __restrict__ col, const uint32_t* __restrict__ qn,
+__device__ __forceinline__ void brick_gather_at(
+    const LevelMap& lm, const uint32_t* __restrict__ col, const uint32_t* __restrict__ qn,
     const uint32_t* __restrict__ brickNode, uint32_t Rd, uint32_t Bx, uint32_t By, uint32_t Bz,
     uint32_t bx, uint32_t by, uint32_t bz, uint32_t* __restrict__ bcol,
-    uint32_t* __restrict__ bnrm) {
+    uint32_t* __restrict__ bnrm, uint64_t j) {
     // tiles under the brick: at most 6 per axis for B <= 38; with their
     // stored-voxel words when the level is level 0
     __shared__ int32_t s_tile[216];
     __shared__ uint32_t s_def[216 * kTileWords];
-    const uint64_t j = blockIdx.x;
     const uint64_t n = brickNode[j];
     uint32_t nx, ny, nz;
     node_xyz(n, (uint32_t)(__ffs(Rd) - 1), nx, ny, nz);
@@ -1664,6 +1663,17 @@ __global__ __launch_bounds__(kBlock) void k_brick_gather(
             }
         }
     }
+}
+
+// The bricks of every depth in one launch: workgroup w gathers brick
+// w - start[e] of entry e for start[e] <= w < start[e + 1].
+__global__ __launch_bounds__(kBlock) void k_brick_gather_all(GatherDepths G) {
+    const uint64_t w = blockIdx.x;
+    int e = 0;
+    while (e + 1 < G.n && w >= G.start[e + 1]) ++e;
+    brick_gather_at(G.lm[e], G.col[e], G.qn[e], G.brickNode[e], G.Rd[e], G.B[0], G.B[1], G.B[2],
+                    G.border[e][0], G.border[e][1], G.border[e][2], G.bcol[e], G.bnrm[e],
+                    w - G.start[e]);
 }
 
 // CollapseConstantBranches (Voxelizer.cu 701-774), one level at a time,
@@ -2260,14 +2270,9 @@ void launch_brick_assign_all(hipStream_t st, const uint32_t* flag, const uint32_
     if (all == 0) return;
     k_brick_assign_all<<<grid_for(all, kBlock), kBlock, 0, st>>>(all, flag, excl, D);
 }
-void launch_brick_gather(hipStream_t st, const LevelMap& lm, const uint32_t* col,
-                         const uint32_t* qn, uint64_t nbricks, const uint32_t* brickNode,
-                         uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
-                         uint32_t* bcol, uint32_t* bnrm) {
-    if (nbricks == 0) return;
-    k_brick_gather<<<(unsigned)nbricks, kBlock, 0, st>>>(lm, col, qn, brickNode, Rd, B[0], B[1],
-                                                         B[2], border[0], border[1], border[2],
-                                                         bcol, bnrm);
+void launch_brick_gather_all(hipStream_t st, const GatherDepths& G) {
+    if (G.n <= 0 || G.start[G.n] == 0) return;
+    k_brick_gather_all<<<(unsigned)G.start[G.n], kBlock, 0, st>>>(G);
 }
 void launch_collapse(hipStream_t st, uint32_t Rd, int childIsLeaf, uint8_t* types,
                      const uint8_t* childTypes) {
