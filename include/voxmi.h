@@ -235,6 +235,24 @@ int vox_device_free(void* p);
  * reference point that calibrates hardware traffic counters. */
 int vox_device_copy(void* dst, const void* src, uint64_t bytes);
 
+/* Copy bytes from src + offset (device) to host memory dst; returns when
+ * they are on the host.  The counterpart of vox_device_upload.  Not in the
+ * reference. */
+int vox_device_download(void* dst, const void* src, uint64_t offset, uint64_t bytes);
+
+/* Exclusive prefix sum of n uint32 values on the device, modulo 2^32:
+ * out[i] = in[0] + ... + in[i-1] for i <= n (out holds n + 1 values, out[n]
+ * the total).  in and out are device pointers, 16-byte aligned, not
+ * overlapping; clear_input = 1 also sets in[0, n) to zero (counters that
+ * count again).  It is the single-pass decoupled look-back scan that sizes
+ * the voxelizer's (group, tile) bins, tile lists and brick addresses; its
+ * look-back scratch is allocated and freed by the call.  Runs on the null
+ * stream and returns when done: 1 ok, 0 error (vox_scan_get_error, per
+ * calling thread).  Not in the reference, whose triangle batches are fixed
+ * 512-triangle launches (Voxelizer.cu 2359-2400) with no prefix sum. */
+int vox_scan_u32(uint32_t* in, uint32_t* out, uint64_t n, int32_t clear_input);
+const char* vox_scan_get_error(void);
+
 /* Write tree.gvx and voxels_<depth>.gvb of the last vox_generate into
  * out_dir (host-side writer, Voxelizer.cu 2816-3061, VoxelBrickWriter.cu
  * 725-818). */

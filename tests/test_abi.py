@@ -67,6 +67,18 @@ def test_create_validates_dimensions_without_gpu():
         assert lib.vox_get_error(h) == b""
     finally:
         lib.vox_destroy(h)
+    # bricks wider than the gather's LDS tile table (7 or more tiles per
+    # axis) are accepted; (B+2)^3 at or above 2^22 is refused
+    h = lib.vox_create(u3(128, 128, 128), u3(64, 64, 64), None)
+    try:
+        assert lib.vox_get_error(h) == b""
+    finally:
+        lib.vox_destroy(h)
+    h = lib.vox_create(u3(400, 400, 400), u3(200, 200, 200), None)
+    try:
+        assert b"2^22" in lib.vox_get_error(h)
+    finally:
+        lib.vox_destroy(h)
 
 
 def test_cpp_mirror_header_compiles_and_reports(tmp_path):

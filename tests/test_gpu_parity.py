@@ -73,3 +73,25 @@ def test_brick_sizes_not_powers_of_two():
     # writes 16-byte runs of a region that starts mid-tile
     for N, B in ((48, 3), (80, 5), (96, 6)):
         _check(scenes.scene_terrain_n(64, N, B))
+
+
+def test_full_octree_generator_256():
+    # the headline workload's own generator (fBm terrain + 2,000 rocks, 16^3
+    # bricks) at a size the oracle runs in seconds: 256^3, depths 0-4
+    _check(scenes.scene_full_octree(cells=256, N=256, B=16))
+
+
+def test_full_octree_generator_512():
+    # the same at 512^3 (depths 0-5, 526 K triangles), every level compared
+    _check(scenes.scene_full_octree(cells=512, N=512, B=16))
+
+
+def test_reference_default_brick_edge_30():
+    # main.cu's default brick edge of 30 (32^3 stored bricks), N / B = 8
+    _check(scenes.scene_terrain_n(120, 240, 30))
+
+
+def test_brick_wider_than_the_gather_tile_table():
+    # B = 64: a stored brick of 66^3 spans 9 tiles per axis, more than the
+    # brick gather stages in LDS (6), so it reads the tile map directly
+    _check(scenes.scene_terrain_n(64, 128, 64))

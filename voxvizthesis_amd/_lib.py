@@ -28,6 +28,7 @@ EXPORTS = [
     "vox_copy_level", "vox_write_files", "vox_get_compressed", "vox_set_launch",
     "vox_generate_async", "vox_wait", "vox_release_work_memory", "vox_add_group_gather",
     "vox_device_alloc", "vox_device_upload", "vox_device_free", "vox_device_copy",
+    "vox_device_download", "vox_scan_u32", "vox_scan_get_error",
 ]
 # Every symbol include/voxmi_batch.h declares.
 BATCH_EXPORTS = [
@@ -238,6 +239,10 @@ def load():
     L.vox_device_upload.argtypes = [vp, C.c_uint64, vp, C.c_uint64]
     L.vox_device_free.argtypes = [vp]
     L.vox_device_copy.argtypes = [vp, vp, C.c_uint64]
+    L.vox_device_download.argtypes = [vp, vp, C.c_uint64, C.c_uint64]
+    L.vox_scan_u32.argtypes = [vp, vp, C.c_uint64, C.c_int32]
+    L.vox_scan_get_error.restype = C.c_char_p
+    L.vox_scan_get_error.argtypes = []
     # batch
     L.vox_batch_create.restype = vp
     L.vox_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32]

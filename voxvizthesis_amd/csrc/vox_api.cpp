@@ -193,9 +193,14 @@ inline uint64_t trunc_u(float f) {
     return (uint64_t)f;
 }
 
+// Word of h->counters where the scans of a generate report a wait that
+// timed out (zeroed by the setup pass, read at the generate's last sync).
+constexpr uint32_t kScanErrWord = 128;
+
 bool scan(vox_ctx* h, const uint32_t* in, uint32_t* out, uint64_t n) {
     if (!ensure_scan_tmp(h->scanTmp, n, h->st, h->err, "scan scratch")) return false;
-    launch_scan(h->st, in, out, n, h->scanTmp.as<uint32_t>());
+    launch_scan(h->st, in, out, n, h->scanTmp.as<uint32_t>(),
+                h->counters.as<uint32_t>() + kScanErrWord);
     return true;
 }
 
@@ -307,6 +312,11 @@ vox_ctx* vox_create(const uint32_t vox_dim[3], const uint32_t brick_dim[3],
             ok = false;
             break;
         }
+    }
+    if (ok && (uint64_t)(h->B[0] + 2) * (h->B[1] + 2) * (h->B[2] + 2) >= (1ull << 22)) {
+        // brick voxels are indexed with 22-bit quotients in the gather
+        h->err << "bricks of more than 2^22 stored voxels ((B+2)^3) are not supported" << std::endl;
+        ok = false;
     }
     h->L = L;
     h->valid = ok;
@@ -617,7 +627,11 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
         return 0;
     uint32_t* cnt = h->binCount.as<uint32_t>();
     uint32_t* start = h->binStart.as<uint32_t>();
-    h->run("fill", [&] { launch_fill_u32(h->st, cnt, h->numBins, 0u); });
+    h->run("fill", [&] {
+        const FillSeg z[2] = {{cnt, h->numBins, 0u},
+                              {h->counters.as<uint32_t>() + kScanErrWord, 1, 0u}};
+        launch_fill_segs(h->st, z, 2);
+    });
     h->run("setup", [&] {
         for (int g = 0; g < G; ++g)
             launch_setup(h->st, P, h->gargs[g], g, h->verts.as<float>(), h->fnorm.as<float>(),
@@ -626,7 +640,8 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
     // the scan leaves the counts zeroed for the scatter pass to count again
     if (!ensure_scan_tmp(h->scanTmp, h->numBins, h->st, h->err, "scan scratch")) return 0;
     h->run("scan", [&] {
-        launch_scan_clear(h->st, cnt, start, h->numBins, h->scanTmp.as<uint32_t>());
+        launch_scan_clear(h->st, cnt, start, h->numBins, h->scanTmp.as<uint32_t>(),
+                          h->counters.as<uint32_t>() + kScanErrWord);
     });
     uint32_t total = 0;
     if (h->replay) {
@@ -932,9 +947,15 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     }
     (void)hipEventRecord(h->evStop, st);
     unsigned long long sat = 0;
+    uint32_t scanErr = 0;
     if (!h->check(hipMemcpyAsync(&sat, satc, 8, hipMemcpyDeviceToHost, st), "counters") ||
+        !h->check(hipMemcpyAsync(&scanErr, ctr + kScanErrWord, 4, hipMemcpyDeviceToHost, st),
+                  "counters") ||
         !h->check(hipStreamSynchronize(st), "generate"))
         return 0;
+    if (scanErr)
+        return h->fail("a device-wide prefix sum gave up waiting on a predecessor; the result is "
+                       "not valid (run vox_generate again)"), 0;
     h->planned = true;
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, h->evStart, h->evStop);
@@ -1434,6 +1455,15 @@ int vox_device_upload(void* dst, uint64_t offset, const void* src, uint64_t byte
     if (!dst || (bytes && !src)) return 0;
     if (bytes == 0) return 1;
     return hipMemcpy(static_cast<uint8_t*>(dst) + offset, src, bytes, hipMemcpyHostToDevice) ==
+                   hipSuccess
+               ? 1
+               : 0;
+}
+
+int vox_device_download(void* dst, const void* src, uint64_t offset, uint64_t bytes) {
+    if (!src || (bytes && !dst)) return 0;
+    if (bytes == 0) return 1;
+    return hipMemcpy(dst, static_cast<const uint8_t*>(src) + offset, bytes, hipMemcpyDeviceToHost) ==
                    hipSuccess
                ? 1
                : 0;

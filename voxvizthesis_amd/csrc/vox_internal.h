@@ -305,12 +305,16 @@ void launch_brick_upload(hipStream_t st, const BrickSrc& src, const uint4* req,
 // tmp must hold scan_tmp_words(n) words, all zero before its first scan
 // (ensure_scan_tmp allocates it so); a scan leaves it ready for the next.
 // Scans that share tmp must not overlap in time; in and out are 16-byte
-// aligned.
+// aligned.  err (may be null): a device word that gets bit 0 set when a
+// workgroup gave up waiting on a predecessor's published sum (a bounded wait
+// that every schedule ends well within), after which out is not valid.
 uint64_t scan_tmp_words(uint64_t n);
 bool ensure_scan_tmp(DevBuf& b, uint64_t n, hipStream_t st, std::stringstream& err,
                      const char* what);
-void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp);
+void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp,
+                 uint32_t* err = nullptr);
 // The same scan, leaving the input zeroed (for counters that count again).
-void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp);
+void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp,
+                       uint32_t* err = nullptr);
 
 }  // namespace voxmi

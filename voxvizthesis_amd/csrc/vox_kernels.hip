@@ -1596,10 +1596,13 @@ __device__ __forceinline__ void brick_gather_at(
     const uint32_t* __restrict__ brickNode, uint32_t Rd, uint32_t Bx, uint32_t By, uint32_t Bz,
     uint32_t bx, uint32_t by, uint32_t bz, uint32_t* __restrict__ bcol,
     uint32_t* __restrict__ bnrm, uint64_t j) {
-    // tiles under the brick: at most 6 per axis for B <= 38; with their
-    // stored-voxel words when the level is level 0
-    __shared__ int32_t s_tile[216];
-    __shared__ uint32_t s_def[216 * kTileWords];
+    // tiles under the brick, with their stored-voxel words when the level is
+    // level 0: staged in LDS when they fit (at most 6 per axis, which any
+    // B <= 39 meets); a larger brick reads the tile map and the words from
+    // global memory for every voxel
+    constexpr uint32_t kGatherTiles = 216;
+    __shared__ int32_t s_tile[kGatherTiles];
+    __shared__ uint32_t s_def[kGatherTiles * kTileWords];
     const uint64_t n = brickNode[j];
     uint32_t nx, ny, nz;
     node_xyz(n, (uint32_t)(__ffs(Rd) - 1), nx, ny, nz);
@@ -1609,7 +1612,8 @@ __device__ __forceinline__ void brick_gather_at(
     const uint32_t ntx = ((r0x + ex - 1) >> 3) - t0x + 1, nty = ((r0y + ey - 1) >> 3) - t0y + 1,
                    ntz = ((r0z + ez - 1) >> 3) - t0z + 1;
     const uint32_t ntl = ntx * nty * ntz;
-    for (uint32_t i = threadIdx.x; i < ntl * kTileWords; i += kBlock) {
+    const bool staged = ntl <= kGatherTiles;   // uniform over the workgroup
+    for (uint32_t i = threadIdx.x; staged && i < ntl * kTileWords; i += kBlock) {
         const uint32_t tl = i / kTileWords, w = i % kTileWords;
         const uint32_t ix = t0x + tl % ntx, iy = t0y + (tl / ntx) % nty, iz = t0z + tl / (ntx * nty);
         const int32_t ta = lm.tileIdx[((uint64_t)iz * lm.TG[1] + iy) * lm.TG[0] + ix];
@@ -1643,11 +1647,19 @@ __device__ __forceinline__ void brick_gather_at(
                 const uint32_t y = udiv_small(rxy, ex, invx);
                 const uint32_t x = rxy - y * ex;
                 const uint32_t X = r0x + x, Y = r0y + y, Z = r0z + z;
-                const uint32_t tl =
-                    (((Z >> 3) - t0z) * nty + ((Y >> 3) - t0y)) * ntx + ((X >> 3) - t0x);
-                const int32_t ta = s_tile[tl];
                 const uint32_t v = vox_of(X, Y, Z);
-                if (ta >= 0 && ((s_def[tl * kTileWords + (v >> 5)] >> (v & 31u)) & 1u)) {
+                int32_t ta;
+                bool def;
+                if (staged) {
+                    const uint32_t tl =
+                        (((Z >> 3) - t0z) * nty + ((Y >> 3) - t0y)) * ntx + ((X >> 3) - t0x);
+                    ta = s_tile[tl];
+                    def = ((s_def[tl * kTileWords + (v >> 5)] >> (v & 31u)) & 1u) != 0u;
+                } else {
+                    ta = (int32_t)tile_of(lm, X, Y, Z);
+                    def = ta >= 0 && stored(lm, (uint64_t)ta, v);
+                }
+                if (ta >= 0 && def) {
                     const uint64_t o = (uint64_t)ta * kTileVox + v;
                     c[u] = col[o];
                     q[u] = qn[o];
@@ -1863,7 +1875,8 @@ __device__ __forceinline__ void scan_status_store(uint64_t* p, uint64_t v) {
 __global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ in, uint64_t n,
                                                  uint32_t nb, uint32_t* __restrict__ out,
                                                  uint32_t* __restrict__ clearIn,
-                                                 unsigned long long* __restrict__ tmp) {
+                                                 unsigned long long* __restrict__ tmp,
+                                                 uint32_t* __restrict__ err) {
     __shared__ uint32_t s_w[kBlock / 64];
     __shared__ unsigned long long s_tk;
     __shared__ uint32_t s_prefix;
@@ -1959,7 +1972,13 @@ __global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ in
                 for (int q = 0; q < kScanLook; ++q)
                     if (q < upto) v += (uint32_t)st[q];
                 excl += __shfl(wave_incl_scan(v), 63, 64);
-                if (pre || polls >= kScanSpin) break;
+                if (pre) break;
+                if (polls >= kScanSpin) {
+                    // a predecessor never published: the prefix is wrong;
+                    // report it instead of passing it on silently
+                    if (lane == 0 && err) atomicOr(err, 1u);
+                    break;
+                }
                 j0 -= 64 * kScanLook;
             }
             if (lane == 0) {
@@ -2311,20 +2330,22 @@ bool ensure_scan_tmp(DevBuf& b, uint64_t n, hipStream_t st, std::stringstream& e
     return true;
 }
 static void scan_launch(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n,
-                        uint32_t* tmp, uint32_t* clearIn) {
+                        uint32_t* tmp, uint32_t* clearIn, uint32_t* err) {
     const uint64_t nb = (n + kScanTile - 1) / kScanTile;
     if (nb == 0) {
         launch_fill_u32(st, out, 1, 0u);
         return;
     }
     k_scan<<<(unsigned)nb, kBlock, 0, st>>>(in, n, (uint32_t)nb, out, clearIn,
-                                             reinterpret_cast<unsigned long long*>(tmp));
+                                             reinterpret_cast<unsigned long long*>(tmp), err);
 }
-void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp) {
-    scan_launch(st, in, out, n, tmp, nullptr);
+void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp,
+                 uint32_t* err) {
+    scan_launch(st, in, out, n, tmp, nullptr, err);
 }
-void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp) {
-    scan_launch(st, in, out, n, tmp, in);
+void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp,
+                       uint32_t* err) {
+    scan_launch(st, in, out, n, tmp, in, err);
 }
 
 }  // namespace voxmi
