@@ -60,6 +60,12 @@ def lib():
                               C.POINTER(C.c_float), C.POINTER(C.c_float),
                               C.POINTER(RefGroup), C.c_int32, C.c_int32,
                               C.POINTER(C.c_int32)]
+        L.ref_run_chunked.restype = C.c_void_p
+        L.ref_run_chunked.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                      C.POINTER(RefGroup), C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_int32)]
         L.ref_error.restype = C.c_char_p
         L.ref_levels.restype = C.c_int32
         L.ref_levels.argtypes = [C.c_void_p]
@@ -154,7 +160,10 @@ class RefResult:
         self.normal_err = normal_err
 
 
-def run(scene, flags: int = 0, keep_mips: bool = True) -> RefResult:
+def run(scene, flags: int = 0, keep_mips: bool = True, chunk=None) -> RefResult:
+    """Run the oracle on a scene.  chunk (3 ints or None): the reference's
+    voxChunkDim; None or a chunk covering the voxel space is the single-chunk
+    run.  With smaller chunks the levels come back chunk by chunk."""
     L = lib()
     keep = []
     groups = (RefGroup * max(1, len(scene.meshes)))()
@@ -180,7 +189,11 @@ def run(scene, flags: int = 0, keep_mips: bool = True) -> RefResult:
     p = (C.c_float * 3)(*[float(np.float32(x)) for x in scene.p])
     d = (C.c_float * 3)(*[float(np.float32(x)) for x in scene.delta])
     st = C.c_int32(0)
-    h = L.ref_run(N, B, p, d, groups, len(scene.meshes), flags, C.byref(st))
+    if chunk is None:
+        h = L.ref_run(N, B, p, d, groups, len(scene.meshes), flags, C.byref(st))
+    else:
+        ch = (C.c_uint32 * 3)(*[int(c) for c in chunk])
+        h = L.ref_run_chunked(N, B, ch, p, d, groups, len(scene.meshes), flags, C.byref(st))
     if not h:
         raise RuntimeError("oracle failed: " + L.ref_error().decode())
     try:

@@ -59,6 +59,25 @@ ref_result* ref_run(const uint32_t N[3], const uint32_t B[3],
                     int32_t flags, int32_t* status);
 const char* ref_error(void);
 
+/* The same run with the voxel space processed in chunks of chunk[a] voxels
+ * per axis, as generateVoxelsAndOctTree does (Voxelizer.cu 2180-2814): chunks
+ * visited breadth first from the origin, each voxelized, coloured, filled
+ * and mip-mapped on its own (the terrain fill stops at the chunk's z = 0),
+ * node samples clamped into the chunk and node types OR-ed over chunks (the
+ * constant colour is the last chunk's sample), bricks that reach a chunk's
+ * edge assembled piecewise and written once complete (VoxelBrickWriter.cu
+ * 126-205, 1026-1210), and brick indices in that write order.  A chunk at
+ * least N + 4*((N/B)>>1) wide is the single-chunk run (chunk NULL: the same);
+ * a smaller one must be a multiple of 2^(levels-1).  Levels are returned
+ * chunk by chunk: level k is (chunks per axis) * (chunk >> k) voxels per
+ * axis, chunk (i, j, l) at offset (i, j, l) * (chunk >> k).  A NON-CONST node
+ * whose brick was never completed keeps brick index 0xffffffff (the
+ * reference's tree file writes 0 for it, VoxelBrickWriter.h 143-149). */
+ref_result* ref_run_chunked(const uint32_t N[3], const uint32_t B[3], const uint32_t chunk[3],
+                            const float p[3], const float delta[3],
+                            const ref_group* groups, int32_t ngroups,
+                            int32_t flags, int32_t* status);
+
 int32_t         ref_levels(const ref_result* r);
 void            ref_dims(const ref_result* r, uint32_t S[3]);
 const uint32_t* ref_occupancy(const ref_result* r, uint64_t* nwords);
