@@ -17,9 +17,16 @@
  * accumulates in the handle and is read with vox_get_error, as
  * Voxelizer::getErrorMessage (Voxelizer.cu 3176-3187) does.
  *
- * Semantics: the reference run with one chunk covering the whole voxel space
- * S = N + 4*((N/B)>>1) per axis.  A chunk smaller than S is rejected (the
- * multi-chunk seam behaviour is not reproduced).
+ * Semantics: the reference run with the chunk (voxChunkDim) the caller
+ * passes.  A chunk of at least S = N + 4*((N/B)>>1) per axis holds the whole
+ * voxel space in one pass.  A smaller chunk reproduces generateVoxelsAndOctTree's
+ * chunk loop (Voxelizer.cu 2180-2814): chunk origins below N only (so the
+ * last layers of S go unvoxelized when the chunks do not reach them), the
+ * terrain fill stopping at each chunk's z = 0, a mip chain per chunk, node
+ * samples clamped into each chunk with node types OR-ed over chunks and the
+ * last chunk's sample as the constant colour, and bricks numbered in the
+ * order the brick writer completes them (VoxelBrickWriter.cu 820-857,
+ * 1026-1210).  Everything stays on the device either way.
  */
 #ifndef VOXMI_H
 #define VOXMI_H
@@ -109,10 +116,13 @@ typedef struct vox_stats {
 /* cuda::Voxelizer::Voxelizer(voxDim, brickDim, voxChunkDim)
  * (Voxelizer.h 122-124, Voxelizer.cu 125-153).  Always returns a handle
  * (NULL only when out of host memory); invalid dimensions (N/B not a power
- * of two >= 2, unequal ratios, chunk < S) are reported by vox_get_error and
- * make every later call fail.  chunk_dim may be NULL: one chunk covering the
- * voxel space (the reference's default of 256 only covers it for N <= 128
- * with B = 8, or similar). */
+ * of two >= 2, unequal ratios, a chunk below S that is not a multiple of
+ * 2^(levels-1), more than 64 chunks per axis) are reported by vox_get_error
+ * and make every later call fail.  chunk_dim may be NULL: one chunk covering
+ * the voxel space.  A NON-CONST node whose brick was never completed (a
+ * chunk smaller than S that leaves part of its border unvoxelized) has brick
+ * index 0xffffffff in vox_result (written as 0 in tree.gvx, as the reference
+ * does). */
 vox_ctx* vox_create(const uint32_t vox_dim[3], const uint32_t brick_dim[3],
                     const uint32_t chunk_dim[3]);
 /* cuda::Voxelizer::~Voxelizer (Voxelizer.cu 272-277). */
@@ -182,8 +192,13 @@ int vox_get_stats(vox_ctx* h, vox_stats* out);
 /* Copy n bytes from a device pointer of the result to host memory. */
 int vox_copy_to_host(vox_ctx* h, void* dst, const void* src_device, uint64_t nbytes);
 /* Materialise mip level k (0 = full resolution) densely on the host:
- * colors (S_k^3 RGBA8) and normals (3*S_k^3 floats), x fastest.
- * dims receives S_k. */
+ * colors (RGBA8) and normals (3 floats per voxel), x fastest; dims receives
+ * the level's extent.  With one chunk the extent is S_k = S >> k.  With
+ * several chunks each chunk has its own mip chain and the level is laid out
+ * chunk by chunk: (chunks per axis) * (chunk >> k) voxels per axis, chunk i
+ * of an axis at offset i * (chunk >> k) (level 0: the voxel space itself,
+ * cut or zero-padded to that extent).  colors and normals both NULL: only
+ * dims is written. */
 int vox_copy_level(vox_ctx* h, int32_t level, uint32_t* colors, float* normals,
                    uint32_t dims[3]);
 /* Block-compressed bricks of the last vox_generate(..., compressed = 1):

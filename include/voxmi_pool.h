@@ -17,6 +17,7 @@
  *           child block texel x / y / z (0 = no children)
  *   word 1: brick texel x / y / z of a resident brick (slot origin + 1), or
  *           the constant RGBA8 colour (R in the low byte) of a constant node
+ *           (a wide pool, vox_brick_pool_create_wide: the slot index)
  *
  * The brick pool is the reference's 3D texture of (B+2)*slots texels per
  * axis, x fastest, one RGBA8 colour texel and one RGB8 normal texel (in a
@@ -62,6 +63,7 @@ typedef struct vox_brick_pool_view {
     uint32_t* colors;           /* device, dims product RGBA8 texels          */
     uint32_t* normals;          /* device, dims product RGB8 texels           */
     uint32_t  resident;         /* slots holding a brick                      */
+    uint32_t  wide;             /* 1: vox_brick_pool_create_wide layout       */
 } vox_brick_pool_view;
 
 typedef struct vox_octree_view {
@@ -79,6 +81,17 @@ typedef struct vox_octree_view {
  * (GigaVoxelsRenderer.cpp 259-272).  Texel coordinates are 10-bit fields,
  * so (B+2)*slots must stay <= 1024 per axis. */
 vox_brick_pool* vox_brick_pool_create(const uint32_t brick_dim[3], const uint32_t slots[3]);
+/* A brick pool beyond the 10-bit texel pointers of the reference's node
+ * format: nslots slots (up to 2^30) stacked along z, slot s holding texels
+ * [s * (B+2)^3, (s + 1) * (B+2)^3), x fastest.  Word 1 of a node whose brick
+ * is resident holds the slot index s instead of texel coordinates, and the
+ * ray cast samples each brick in its own (B+2)^3 texel frame (normalised
+ * coordinates of the brick, the same trilinear filter).  An extension of the
+ * reference (whose pool is one 3D texture, at most 1024 texels per axis with
+ * 10-bit pointers, i.e. 8.2 GB of 18^3 colour + normal bricks): for pools of
+ * 16 GB and more.  Nodes built on it are not readable by the reference's
+ * shader. */
+vox_brick_pool* vox_brick_pool_create_wide(const uint32_t brick_dim[3], uint64_t nslots);
 void vox_brick_pool_destroy(vox_brick_pool* bp);
 const char* vox_brick_pool_get_error(vox_brick_pool* bp);
 int vox_brick_pool_get_view(vox_brick_pool* bp, vox_brick_pool_view* out);

@@ -26,13 +26,12 @@ enum AddressMode { AddressModeWrap = 0, AddressModeClamp = 1, AddressModeMirror 
 
 class Voxelizer {
 public:
-    // Voxelizer(voxDim, brickDim, voxChunkDim=uvec3(256)) (Voxelizer.h 122-124).
-    // The default chunk here is the whole voxel space: the reference's 256
-    // only covers it for small grids, and a chunk smaller than the voxel space
-    // is rejected (see getErrorMessage()).
-    Voxelizer(const uvec3& voxDim, const uvec3& brickDim, const uvec3* voxChunkDim = nullptr)
-        : _h(vox_create(voxDim.data(), brickDim.data(),
-                        voxChunkDim ? voxChunkDim->data() : nullptr)) {
+    // Voxelizer(voxDim, brickDim, voxChunkDim=uvec3(256)) (Voxelizer.h 122-124),
+    // with the reference's default chunk.  A chunk of at least the voxel
+    // space (N + 4*((N/B)>>1)) processes it in one pass.
+    Voxelizer(const uvec3& voxDim, const uvec3& brickDim,
+              const uvec3& voxChunkDim = uvec3{256u, 256u, 256u})
+        : _h(vox_create(voxDim.data(), brickDim.data(), voxChunkDim.data())) {
         // (voxDim / brickDim) >> 1 per axis (Voxelizer.cu 143-144).
         for (int a = 0; a < 3; ++a)
             _extra[a] = brickDim[a] ? (voxDim[a] / brickDim[a]) >> 1 : 0u;

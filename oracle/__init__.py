@@ -133,6 +133,10 @@ def lib():
         L.ref_raycast.restype = C.c_int32
         L.ref_raycast.argtypes = [C.POINTER(RefRayCamera), C.POINTER(RefRayParams), C.c_int32,
                                   u3p, C.c_void_p, u3p, C.c_void_p, u3p, C.c_void_p, C.c_void_p]
+        L.ref_raycast_pool.restype = C.c_int32
+        L.ref_raycast_pool.argtypes = [C.POINTER(RefRayCamera), C.POINTER(RefRayParams),
+                                       C.c_int32, u3p, C.c_void_p, u3p, C.c_void_p, u3p,
+                                       C.c_int32, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -294,9 +298,11 @@ def pool_load(directory: str, slots=(8, 8, 8), store_const: bool = True) -> RefP
         L.ref_pool_free(h)
 
 
-def raycast(camera: dict, params: dict, levels: int, B, nodes: np.ndarray, pool: np.ndarray):
+def raycast(camera: dict, params: dict, levels: int, B, nodes: np.ndarray, pool: np.ndarray,
+            wide: bool = False):
     """Per-pixel node usage (H, W, 12) and RGBA8 (H, W) of one frame.
-    nodes: (z, y, x, 2) node texture; pool: (z, y, x) brick pool colours."""
+    nodes: (z, y, x, 2) node texture; pool: (z, y, x) brick pool colours
+    (wide: a wide pool, slots stacked along z, word 1 the slot)."""
     L = lib()
     cam = RefRayCamera()
     cam.position[:] = [float(v) for v in camera["position"]]
@@ -320,8 +326,9 @@ def raycast(camera: dict, params: dict, levels: int, B, nodes: np.ndarray, pool:
     W, H = cam.width, cam.height
     usage = np.zeros((H, W, 12), np.uint32)
     rgba = np.zeros((H, W), np.uint32)
-    ok = L.ref_raycast(C.byref(cam), C.byref(prm), levels, b3, nodes.ctypes.data, nd,
-                       pool.ctypes.data, pd, usage.ctypes.data, rgba.ctypes.data)
+    ok = L.ref_raycast_pool(C.byref(cam), C.byref(prm), levels, b3, nodes.ctypes.data, nd,
+                            pool.ctypes.data, pd, 1 if wide else 0, usage.ctypes.data,
+                            rgba.ctypes.data)
     if not ok:
         raise RuntimeError("ray cast oracle: empty volume box")
     return usage, rgba

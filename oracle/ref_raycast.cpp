@@ -59,6 +59,10 @@ struct Ctx {
     const uint32_t* pool;
     uint32_t pdim[3];
     const float* jitter;
+    // wide pool (vox_brick_pool_create_wide): word 1 of a resident node is
+    // its slot; the brick is sampled in its own (B+2)^3 texel frame
+    bool wide = false;
+    uint32_t wideSlots = 0;
 };
 
 uvec2 texelFetch(const Ctx& c, uint32_t x, uint32_t y, uint32_t z) {
@@ -284,13 +288,23 @@ void shade(const Ctx& c, int px, int py, uint32_t usage[12], uint32_t& rgba) {
                         vec3 bp((float)((curBrickPtr.g >> 20u) & 0x3FFu) / c.BrickPoolDimension.x,
                                 (float)((curBrickPtr.g >> 10u) & 0x3FFu) / c.BrickPoolDimension.y,
                                 (float)(curBrickPtr.g & 0x3FFu) / c.BrickPoolDimension.z);
+                        Ctx cb = c;
+                        if (c.wide) {
+                            // the slot's first interior texel in the brick's own frame;
+                            // a word 1 that is no slot (no resident brick on the
+                            // path) reads slot 0
+                            bp = vec3(1.0f / c.BrickPoolDimension.x, 1.0f / c.BrickPoolDimension.y,
+                                      1.0f / c.BrickPoolDimension.z);
+                            const uint32_t slot = curBrickPtr.g < c.wideSlots ? curBrickPtr.g : 0u;
+                            cb.pool = c.pool + (uint64_t)slot * c.pdim[0] * c.pdim[1] * c.pdim[2];
+                        }
                         vec3 curBrickRayPos =
                             bp + ((curBrickRayPosition * c.BrickDimension) / c.BrickPoolDimension);
                         float sampleRatio = std::ldexp(1.0f, c.OctTreeDepthMinusOne - curBrickDepth);
                         vec3 worldRayStep = rayDir;
                         worldRayStep = worldRayStep * c.RayStepSize;
                         worldRayStep = worldRayStep * sampleRatio;
-                        TraverseBrick(c, rayWorldPosition, rayDir, worldRayStep, curBrickRayPos,
+                        TraverseBrick(cb, rayWorldPosition, rayDir, worldRayStep, curBrickRayPos,
                                       brickRayStep, sampleRatio, nodeExtentMin, nodeExtentMax, dst);
                     }
                     if (nodeListTraversalCount >= startPushingNodes &&
@@ -365,11 +379,17 @@ uint64_t splitmix64(uint64_t& s) {
 
 }  // namespace
 
-extern "C" int32_t ref_raycast(const ref_ray_camera* cam, const ref_ray_params* prm, int32_t levels,
-                               const uint32_t B[3], const uint32_t* nodes, const uint32_t ndim[3],
-                               const uint32_t* pool, const uint32_t pdim[3], uint32_t* usage_out,
-                               uint32_t* rgba_out) {
+extern "C" int32_t ref_raycast_pool(const ref_ray_camera* cam, const ref_ray_params* prm,
+                                    int32_t levels, const uint32_t B[3], const uint32_t* nodes,
+                                    const uint32_t ndim[3], const uint32_t* pool,
+                                    const uint32_t pdim[3], int32_t wide, uint32_t* usage_out,
+                                    uint32_t* rgba_out) {
     Ctx c;
+    c.wide = wide != 0;
+    // a wide pool: slots stacked along z, each sampled in its own frame
+    const uint32_t fdim[3] = {wide ? B[0] + 2 : pdim[0], wide ? B[1] + 2 : pdim[1],
+                              wide ? B[2] + 2 : pdim[2]};
+    c.wideSlots = wide ? pdim[2] / (B[2] + 2) : 0u;
     // GigaVoxelsRenderer::drawVolume uniforms (1650-1816)
     float T[3], S[3];
     for (int a = 0; a < 3; ++a) {
@@ -401,11 +421,11 @@ extern "C" int32_t ref_raycast(const ref_ray_camera* cam, const ref_ray_params* 
         const float rayStepRoot = (S[a] * c.RayStepSize) * pow2;
         const float rootVoxel = S[a] / (float)B[a];
         const float ratio = rayStepRoot / rootVoxel;
-        c.BrickStepVector[a] = (1.0f / (float)pdim[a]) * ratio;
+        c.BrickStepVector[a] = (1.0f / (float)fdim[a]) * ratio;
         c.BrickDimension[a] = (float)B[a];
-        c.BrickPoolDimension[a] = (float)pdim[a];
+        c.BrickPoolDimension[a] = (float)fdim[a];
         c.ndim[a] = ndim[a];
-        c.pdim[a] = pdim[a];
+        c.pdim[a] = fdim[a];
     }
     const float bx = (float)B[0], by = (float)B[1], bz = (float)B[2];
     const float minB = bx < ((by < bz) ? by : bz) ? bx : bz;
@@ -444,4 +464,11 @@ extern "C" int32_t ref_raycast(const ref_ray_camera* cam, const ref_ray_params* 
             if (rgba_out) rgba_out[pix] = rgba;
         }
     return 1;
+}
+
+extern "C" int32_t ref_raycast(const ref_ray_camera* cam, const ref_ray_params* prm, int32_t levels,
+                               const uint32_t B[3], const uint32_t* nodes, const uint32_t ndim[3],
+                               const uint32_t* pool, const uint32_t pdim[3], uint32_t* usage_out,
+                               uint32_t* rgba_out) {
+    return ref_raycast_pool(cam, prm, levels, B, nodes, ndim, pool, pdim, 0, usage_out, rgba_out);
 }

@@ -37,6 +37,14 @@ int32_t ref_raycast(const ref_ray_camera* cam, const ref_ray_params* prm, int32_
                     const uint32_t* pool, const uint32_t pdim[3], uint32_t* usage_out,
                     uint32_t* rgba_out);
 
+/* The same over a wide brick pool (vox_brick_pool_create_wide) when wide is
+ * non-zero: pdim = (B+2, B+2, (B+2) * slots), word 1 of a resident node is
+ * the slot, and each brick is sampled in its own (B+2)^3 texel frame. */
+int32_t ref_raycast_pool(const ref_ray_camera* cam, const ref_ray_params* prm, int32_t levels,
+                         const uint32_t B[3], const uint32_t* nodes, const uint32_t ndim[3],
+                         const uint32_t* pool, const uint32_t pdim[3], int32_t wide,
+                         uint32_t* usage_out, uint32_t* rgba_out);
+
 #ifdef __cplusplus
 }
 #endif

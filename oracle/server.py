@@ -29,9 +29,11 @@ RB = 0x80000000
 
 
 class RefServer:
-    def __init__(self, tex, types, bidx, texel, slots, B, max_up):
+    def __init__(self, tex, types, bidx, texel, slots, B, max_up, wide=False):
         """tex: (z, y, x, 2) node texture after the octree is built; types /
-        bidx / texel: per-depth arrays of the tree; slots: pool slots per axis."""
+        bidx / texel: per-depth arrays of the tree; slots: pool slots per axis
+        (wide: a wide pool, slots (1, 1, n), word 1 of a resident node its slot)."""
+        self.wide = wide
         self.nd = (tex.shape[2], tex.shape[1], tex.shape[0])
         self.tex = tex.reshape(-1, 2).copy()
         self.slots = tuple(slots)
@@ -52,7 +54,7 @@ class RefServer:
             x = (((w1 >> 20) & 1023) - 1) // self.e[0]
             y = (((w1 >> 10) & 1023) - 1) // self.e[1]
             z = ((w1 & 1023) - 1) // self.e[2]
-            s = (z * slots[1] + y) * slots[0] + x
+            s = w1 if wide else (z * slots[1] + y) * slots[0] + x
             self.node_slot[int(t)] = s
             self.slot_node[s] = int(t)
         self.fifo = deque()
@@ -130,7 +132,7 @@ class RefServer:
             self.pending.discard(t)
             ox, oy, oz = self._slot_origin(s)
             self.tex[t, 0] |= np.uint32(RB)
-            self.tex[t, 1] = ((ox + 1) << 20) | ((oy + 1) << 10) | (oz + 1)
+            self.tex[t, 1] = s if self.wide else ((ox + 1) << 20) | ((oy + 1) << 10) | (oz + 1)
             done.append((t, s))
         self.admitted.append(done)
         return n
@@ -158,7 +160,8 @@ class RefSharedServer:
             touched, then admission runs once over all trees.
     """
 
-    def __init__(self, slots, B, max_trees, cap, max_up):
+    def __init__(self, slots, B, max_trees, cap, max_up, wide=False):
+        self.wide = wide          # a wide pool: slots (1, 1, n), word 1 the slot
         self.slots = tuple(slots)
         self.e = tuple(b + 2 for b in B)
         self.nslots = slots[0] * slots[1] * slots[2]
@@ -190,7 +193,7 @@ class RefSharedServer:
             sx = (((w1 >> 20) & 1023) - 1) // self.e[0]
             sy = (((w1 >> 10) & 1023) - 1) // self.e[1]
             sz = ((w1 & 1023) - 1) // self.e[2]
-            s = (sz * self.slots[1] + sy) * self.slots[0] + sx
+            s = w1 if self.wide else (sz * self.slots[1] + sy) * self.slots[0] + sx
             assert self.slot_node[s] is None, "slot taken"
             self.node_slot[base + int(x)] = s
             self.slot_node[s] = base + int(x)
@@ -298,7 +301,7 @@ class RefSharedServer:
             ox, oy, oz = (s % sx) * self.e[0], ((s // sx) % sy) * self.e[1], \
                 (s // (sx * sy)) * self.e[2]
             t["tex"][x, 0] |= np.uint32(RB)
-            t["tex"][x, 1] = ((ox + 1) << 20) | ((oy + 1) << 10) | (oz + 1)
+            t["tex"][x, 1] = s if self.wide else ((ox + 1) << 20) | ((oy + 1) << 10) | (oz + 1)
             done.append((g, s))
         self.admitted.append(done)
         return n

@@ -28,8 +28,15 @@ int main() {
         bad |= !vz.getErrorMessage().empty();
     }
     {
-        voxmi::uvec3 chunk{256, 256, 256};
-        voxmi::Voxelizer vz(voxmi::uvec3{1024, 1024, 1024}, voxmi::uvec3{16, 16, 16}, &chunk);
+        // the reference's default chunk (256) with main.cu's 1024^3 / 16^3 grid
+        voxmi::Voxelizer vz(voxmi::uvec3{1024, 1024, 1024}, voxmi::uvec3{16, 16, 16});
+        std::printf("default chunk error '%s'\n", vz.getErrorMessage().c_str());
+        bad |= !vz.getErrorMessage().empty();
+    }
+    {
+        // a chunk that is not a multiple of 2^(levels-1) = 64
+        voxmi::Voxelizer vz(voxmi::uvec3{1024, 1024, 1024}, voxmi::uvec3{16, 16, 16},
+                            voxmi::uvec3{100, 100, 100});
         std::printf("chunk error '%s'\n", vz.getErrorMessage().c_str());
         bad |= vz.getErrorMessage().find("chunk") == std::string::npos;
         bad |= vz.initCuda();      // must refuse after a construction error

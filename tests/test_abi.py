@@ -57,9 +57,16 @@ def test_create_validates_dimensions_without_gpu():
                                   (ctypes.c_float * 3)(1, 1, 1)) == 0
     finally:
         lib.vox_destroy(h)
+    # the reference's default chunk with its 1024^3 / 16^3 grid: several chunks
     h = lib.vox_create(u3(1024, 1024, 1024), u3(16, 16, 16), u3(256, 256, 256))
     try:
-        assert b"chunk dimension" in lib.vox_get_error(h)
+        assert lib.vox_get_error(h) == b""
+    finally:
+        lib.vox_destroy(h)
+    # a chunk below S must be a multiple of 2^(levels-1) (64 here)
+    h = lib.vox_create(u3(1024, 1024, 1024), u3(16, 16, 16), u3(96, 96, 96))
+    try:
+        assert b"multiple of 2^(levels-1)" in lib.vox_get_error(h)
     finally:
         lib.vox_destroy(h)
     h = lib.vox_create(u3(128, 128, 128), u3(8, 8, 8), u3(256, 256, 256))
@@ -92,3 +99,4 @@ def test_cpp_mirror_header_compiles_and_reports(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "extra 8 8 8" in r.stdout
+

@@ -39,10 +39,14 @@ def path(n, W, H, start, end, target):
 
 
 class Rig:
-    def __init__(self, scene, slots, max_up, W, H):
+    def __init__(self, scene, slots, max_up, W, H, wide=False):
+        """wide: a wide pool of prod(slots) slots (word 1 of a resident node
+        is its slot) instead of the reference's pool texture."""
         self.scene = scene
+        self.wide = wide
         self.vz = voxelize_scene(scene, options=VOX_OPT_MIP_CHAIN)
-        self.bp = BrickPool(scene.B, slots)
+        self.bp = (BrickPool(scene.B, wide_slots=int(np.prod(slots))) if wide
+                   else BrickPool(scene.B, slots))
         self.t = OctTree(self.bp, self.vz)
         self.rc = [RayCaster(self.t, W, H)]
         self.srv = BrickServer(self.t, max_up)
@@ -60,7 +64,10 @@ class Rig:
     def ref_server(self, slots, max_up):
         types = [tr["types"] for tr in self.res["tree"]]
         bidx = [tr["bidx"] for tr in self.res["tree"]]
-        return RefServer(self.t.nodes(), types, bidx, self.texel, slots, self.scene.B, max_up)
+        if self.wide:
+            slots = (1, 1, int(np.prod(slots)))
+        return RefServer(self.t.nodes(), types, bidx, self.texel, slots, self.scene.B, max_up,
+                         wide=self.wide)
 
     def brick_of(self, texel):
         d, n = self.info[texel]
@@ -89,20 +96,23 @@ class Rig:
             o.close()
 
 
-def test_serving_flythrough_matches_oracles():
+@pytest.mark.parametrize("wide", [False, True])
+def test_serving_flythrough_matches_oracles(wide):
+    # wide: the same 64 slots as a wide pool (slots stacked along z, word 1
+    # of a resident node its slot, bricks sampled in their own frame)
     W, H, slots, max_up = 64, 48, (4, 4, 4), 16
-    rig = Rig(scenes.scene_small_terrain(), slots, max_up, W, H)
+    rig = Rig(scenes.scene_small_terrain(), slots, max_up, W, H, wide=wide)
     try:
         ref = rig.ref_server(slots, max_up)
         frames = path(6, W, H, (0.5, -0.8, 1.4), (0.45, 0.2, 0.75), (0.5, 0.55, 0.5))
         total = 0
         for cam in frames:
-            print(f"[serve] frame {cam['frame']} pos {cam['position']}", flush=True)
+            print(f"[serve] wide={wide} frame {cam['frame']} pos {cam['position']}", flush=True)
             nodes = rig.t.nodes()
             assert (nodes == ref.texture()).all()
             rig.rc[0].frame(camera(**cam), raycast_params(**rig.prm))
             ru, rcol = oracle.raycast(cam, rig.prm, rig.L, rig.scene.B, nodes,
-                                      rig.bp.textures()[0])
+                                      rig.bp.textures()[0], wide=wide)
             assert (rig.rc[0].usage() == ru).all() and (rig.rc[0].rgba() == rcol).all()
             st = rig.srv.frame(rig.rc[0])
             n = ref.step(sorted(set(ru[ru != 0].tolist())))

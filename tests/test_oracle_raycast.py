@@ -141,3 +141,29 @@ def test_level_of_detail_descends_only_when_close():
                  for k in range(8)}
     assert ids_far == {ROOT_ID}
     assert ids_near and ids_near <= child_ids
+
+
+def test_wide_pool_slot_equals_a_one_slot_reference_pool():
+    """A wide pool (word 1 of a resident node is its slot, each brick sampled
+    in its own frame) reads slot s exactly as the reference's pool texture
+    of a single slot holding that brick reads it: same usage, same colour.
+    A word 1 that names no slot reads slot 0."""
+    rng = np.random.default_rng(3)
+    e = 4   # B = 2
+    bricks = rng.integers(0, 1 << 32, size=(3, e, e, e), dtype=np.uint64).astype(np.uint32)
+    bricks |= np.uint32(0x60000000)   # alpha well above zero
+    wide_pool = bricks.reshape(3 * e, e, e)            # slots stacked along z
+    c = cam((0.45, 0.55, 2.5), (0, 0, -1), w=24, h=16)
+    for s in range(3):
+        ref_u, ref_c = oracle.raycast(c, PRM, 1, (2, 2, 2), one_node_tree(RB, pk(1, 1, 1)), bricks[s])
+        w_u, w_c = oracle.raycast(c, PRM, 1, (2, 2, 2), one_node_tree(RB, s), wide_pool, wide=True)
+        assert (ref_u == w_u).all() and (ref_c == w_c).all()
+        assert ref_c.any()
+    # an out-of-range word 1 reads slot 0
+    ref_u, ref_c = oracle.raycast(c, PRM, 1, (2, 2, 2), one_node_tree(RB, pk(1, 1, 1)), bricks[0])
+    w_u, w_c = oracle.raycast(c, PRM, 1, (2, 2, 2), one_node_tree(RB, 7), wide_pool, wide=True)
+    assert (ref_c == w_c).all()
+    # and the slots differ from each other
+    _, c1 = oracle.raycast(c, PRM, 1, (2, 2, 2), one_node_tree(RB, 1), wide_pool, wide=True)
+    _, c2 = oracle.raycast(c, PRM, 1, (2, 2, 2), one_node_tree(RB, 2), wide_pool, wide=True)
+    assert (c1 != c2).any()

@@ -146,3 +146,23 @@ def test_shared_detach_drops_queued_entries():
     # B's c0 and c1 queued in texel order; one admission, into the empty slot
     assert s.step([(1, [nid(c[1]), nid(c[0])])]) == 1
     assert s.admitted[-1] == [(16 + c[0], 0)] and list(s.fifo) == [16 + c[1]]
+
+
+def test_wide_pool_same_policy_slot_in_word_one():
+    """A wide pool (slots (1, 1, n), word 1 of a resident node its slot)
+    admits the same (texel, slot) pairs as the reference's pool of n slots;
+    only word 1 changes: the slot instead of its texel origin + 1."""
+    tex, types, bidx, texel = tree()
+    wtex = tex.copy()
+    wtex[0, 0, 0, 1] = 0                                       # root's brick: slot 0
+    ref = RefServer(tex, types, bidx, texel, slots=(2, 1, 1), B=(2, 2, 2), max_up=2)
+    wide = RefServer(wtex, types, bidx, texel, slots=(1, 1, 2), B=(2, 2, 2), max_up=2, wide=True)
+    assert wide.node_slot == ref.node_slot == {0: 0}
+    c = [int(t) for t in texel[1]]
+    for ids in ([nid(c[1]), nid(c[0]), nid(c[3])], [nid(c[0]), nid(c[2])], [], [nid(c[6])]):
+        assert wide.step(ids) == ref.step(ids)
+        assert wide.admitted[-1] == ref.admitted[-1]
+        a, b = wide.texture().reshape(-1, 2), ref.texture().reshape(-1, 2)
+        assert (a[:, 0] == b[:, 0]).all()
+        for t, s in wide.node_slot.items():
+            assert a[t, 1] == s and b[t, 1] == pk(1 + 4 * s, 1, 1)

@@ -37,7 +37,7 @@ BATCH_EXPORTS = [
 ]
 # Every symbol include/voxmi_pool.h declares.
 POOL_EXPORTS = [
-    "vox_brick_pool_create", "vox_brick_pool_destroy", "vox_brick_pool_get_error",
+    "vox_brick_pool_create", "vox_brick_pool_create_wide", "vox_brick_pool_destroy", "vox_brick_pool_get_error",
     "vox_brick_pool_get_view", "vox_octree_create", "vox_octree_destroy", "vox_octree_get_error",
     "vox_octree_get_view", "vox_pool_copy_to_host",
 ]
@@ -123,6 +123,7 @@ class VoxBrickPoolView(C.Structure):
     _fields_ = [
         ("brick_dim", C.c_uint32 * 3), ("slots", C.c_uint32 * 3), ("dims", C.c_uint32 * 3),
         ("colors", C.c_void_p), ("normals", C.c_void_p), ("resident", C.c_uint32),
+                ("wide", C.c_uint32),
     ]
 
 
@@ -266,6 +267,8 @@ def load():
     # node pool / brick pool
     L.vox_brick_pool_create.restype = vp
     L.vox_brick_pool_create.argtypes = [u3, u3]
+    L.vox_brick_pool_create_wide.restype = vp
+    L.vox_brick_pool_create_wide.argtypes = [u3, C.c_uint64]
     L.vox_brick_pool_destroy.argtypes = [vp]
     L.vox_brick_pool_destroy.restype = None
     L.vox_brick_pool_get_error.restype = C.c_char_p

@@ -36,8 +36,12 @@ void write_node(std::ofstream& f, const HostTree& t, int d, uint32_t x, uint32_t
     const uint8_t type = t.types[d][i];
     for (int k = 0; k <= d; ++k) f << "    ";
     if (type == 1) {
+        // a NON-CONST node whose brick was never completed (several chunks,
+        // its border in a chunk that is not processed) has no index; the
+        // reference's getBrickIndex answers 0 for it (VoxelBrickWriter.h 143-149)
+        const uint32_t bi = t.bidx[d][i] == 0xffffffffu ? 0u : t.bidx[d][i];
         f << "<Node " << "MipMapX=\"" << xo << "\" " << "MipMapY=\"" << yo << "\" "
-          << "MipMapZ=\"" << zo << "\" " << "Type=\"NON-CONST\" " << "Brick=\"" << t.bidx[d][i]
+          << "MipMapZ=\"" << zo << "\" " << "Type=\"NON-CONST\" " << "Brick=\"" << bi
           << "\" " << "Depth=\"" << d << "\"";
     } else {
         const uint32_t c = t.cc[d][i];

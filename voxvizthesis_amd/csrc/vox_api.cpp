@@ -77,6 +77,14 @@ struct TimedLaunch {
 struct vox_ctx {
     uint32_t N[3] = {0, 0, 0}, B[3] = {0, 0, 0}, chunk[3] = {0, 0, 0};
     uint32_t extra[3] = {0, 0, 0}, S[3] = {0, 0, 0};
+    // Chunk edge per axis as processed (S when the requested chunk is at
+    // least the voxel space: the same run), chunks per axis, and whether
+    // more than one chunk is in play (generateVoxelsAndOctTree's chunk loop,
+    // Voxelizer.cu 2285-2309).
+    uint32_t C[3] = {0, 0, 0}, nc[3] = {1, 1, 1};
+    bool multi = false;
+    DevBuf chunkRank;                       // visit rank per chunk
+    DevBuf keysA, keysB, nodesA, nodesB, sortTmp;   // brick order with several chunks
     int L = 0;
     bool valid = false;
     bool inited = false;
@@ -181,6 +189,16 @@ LevelView view(vox_ctx* h, int k) {
     v.qn = b.qn.as<uint32_t>();
     v.nrm = b.nrm.as<float>();
     return v;
+}
+
+ChunkGeom chunkGeom(const vox_ctx* h) {
+    ChunkGeom g{};
+    for (int a = 0; a < 3; ++a) {
+        g.C[a] = h->C[a];
+        g.nc[a] = h->nc[a];
+    }
+    g.rank = h->chunkRank.as<uint32_t>();
+    return g;
 }
 
 inline float gclampf(float x, float lo, float hi) {
@@ -305,13 +323,33 @@ vox_ctx* vox_create(const uint32_t vox_dim[3], const uint32_t brick_dim[3],
             ok = false;
             break;
         }
-        if (h->chunk[a] < h->S[a]) {
-            h->err << "chunk dimension " << h->chunk[a] << " is smaller than the voxel space "
-                   << h->S[a] << " (N + 4*((N/B)>>1)); multi-chunk seam semantics are not "
-                   << "reproduced, pass a chunk that covers the voxel space" << std::endl;
+    }
+    // Chunks smaller than the voxel space: the chunk must be a multiple of
+    // 2^(levels-1), so that every level's chunk origin (x >> k, Voxelizer.cu
+    // 1273-1276) is a whole number of that level's chunk edges, and at most
+    // 64 chunks per axis.
+    for (int a = 0; a < 3 && ok; ++a) {
+        if (h->chunk[a] >= h->S[a]) {
+            h->C[a] = h->S[a];
+            h->nc[a] = 1;
+            continue;
+        }
+        if (h->chunk[a] == 0 || h->chunk[a] % (1u << (L - 1)) != 0) {
+            h->err << "chunk dimension " << h->chunk[a] << " must be a multiple of 2^(levels-1) = "
+                   << (1u << (L - 1)) << " or at least the voxel space " << h->S[a]
+                   << " (N + 4*((N/B)>>1))" << std::endl;
             ok = false;
             break;
         }
+        h->C[a] = h->chunk[a];
+        // chunk origins are the multiples of the chunk below N (Voxelizer.cu 2300-2302)
+        h->nc[a] = (h->N[a] + h->C[a] - 1) / h->C[a];
+        if (h->nc[a] > (uint32_t)kMaxChunksAxis) {
+            h->err << "more than " << kMaxChunksAxis << " chunks per axis" << std::endl;
+            ok = false;
+            break;
+        }
+        h->multi = true;
     }
     if (ok && (uint64_t)(h->B[0] + 2) * (h->B[1] + 2) * (h->B[2] + 2) >= (1ull << 22)) {
         // brick voxels are indexed with 22-bit quotients in the gather
@@ -409,13 +447,16 @@ int vox_allocate_voxel_memory(vox_ctx* h) {
     for (int a = 0; a < 3; ++a) {
         P.S[a] = h->S[a];
         P.TG[a] = (h->S[a] + kTile - 1) / kTile;
+        P.C[a] = h->C[a];
     }
     P.numTiles = (uint64_t)P.TG[0] * P.TG[1] * P.TG[2];
     h->lv.assign(h->L, LevelBufs());
     for (int k = 0; k < h->L; ++k) {
         LevelBufs& l = h->lv[k];
         for (int a = 0; a < 3; ++a) {
-            l.map.dims[a] = h->S[a] >> k;
+            // level 0 is the voxel space; level k >= 1 holds each chunk's
+            // own mip chain side by side (S >> k with one chunk)
+            l.map.dims[a] = k == 0 ? h->S[a] : h->nc[a] * (h->C[a] >> k);
             l.map.TG[a] = (l.map.dims[a] + kTile - 1) / kTile;
         }
         l.map.numTiles = (uint64_t)l.map.TG[0] * l.map.TG[1] * l.map.TG[2];
@@ -445,6 +486,45 @@ int vox_allocate_voxel_memory(vox_ctx* h) {
         return 0;
     if (!h->counters.ensure(4096, h->err, "counters")) return 0;
     if (!h->sat.ensure(8, h->err, "sat counter")) return 0;
+    if (h->multi) {
+        // Chunk visit order: breadth first from the origin, each chunk
+        // queueing its +x/+y/+z neighbours in x, y, z loop order
+        // (Voxelizer.cu 2285-2309); rank per chunk (i, j, l).
+        const uint32_t* nc = h->nc;
+        std::vector<uint32_t> rank((size_t)nc[0] * nc[1] * nc[2], 0xffffffffu);
+        std::vector<std::array<uint32_t, 3>> q;
+        q.push_back({0, 0, 0});
+        rank[0] = 0;
+        for (size_t head = 0; head < q.size(); ++head) {
+            const auto c = q[head];
+            for (uint32_t dx = 0; dx < 2; ++dx)
+                for (uint32_t dy = 0; dy < 2; ++dy)
+                    for (uint32_t dz = 0; dz < 2; ++dz) {
+                        const uint32_t n[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
+                        if (n[0] >= nc[0] || n[1] >= nc[1] || n[2] >= nc[2]) continue;
+                        const size_t i = ((size_t)n[2] * nc[1] + n[1]) * nc[0] + n[0];
+                        if (rank[i] != 0xffffffffu) continue;
+                        rank[i] = (uint32_t)q.size();
+                        q.push_back({n[0], n[1], n[2]});
+                    }
+        }
+        uint64_t maxNodes = 0;
+        for (int d = 0; d < h->L; ++d) maxNodes = std::max(maxNodes, h->tree[d].nodes);
+        if (!h->chunkRank.ensure(rank.size() * 4, h->err, "chunk ranks") ||
+            !h->check(hipMemcpy(h->chunkRank.p, rank.data(), rank.size() * 4, hipMemcpyHostToDevice),
+                      "chunk ranks") ||
+            !h->keysA.ensure(h->allNodes * 8, h->err, "brick keys") ||
+            !h->keysB.ensure(maxNodes * 8, h->err, "brick keys") ||
+            !h->nodesA.ensure(h->allNodes * 4, h->err, "brick nodes") ||
+            !h->nodesB.ensure(maxNodes * 4, h->err, "brick nodes"))
+            return 0;
+        size_t tmpBytes = 0;
+        if (!h->check(chunk_sort_pairs(h->st, nullptr, &tmpBytes, nullptr, nullptr, nullptr, nullptr,
+                                       (uint32_t)maxNodes, 64),
+                      "sort size") ||
+            !h->sortTmp.ensure(tmpBytes, h->err, "sort scratch"))
+            return 0;
+    }
     h->voxMem = true;
     return 1;
 }
@@ -468,6 +548,8 @@ void vox_deallocate_voxel_memory(vox_ctx* h) {
     h->work.release(); h->workCount.release(); h->rec.release(); h->recCount.release();
     h->counters.release(); h->binCount.release(); h->binStart.release(); h->binTri.release();
     h->scanTmp.release();
+    h->chunkRank.release(); h->keysA.release(); h->keysB.release(); h->nodesA.release();
+    h->nodesB.release(); h->sortTmp.release();
     h->voxMem = false;
     h->haveResult = false;
 }
@@ -611,9 +693,25 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
             mn[a] = (uint32_t)trunc_u(lo);
             mx[a] = (uint32_t)trunc_u(hi);
         }
-        ga.overlaps = (mn[0] <= h->S[0] - 1 && mn[1] <= h->S[1] - 1 && mn[2] <= h->S[2] - 1) ? 1 : 0;
+        // The chunks the bounds overlap (a chunk [iC, iC + C - 1] with i <
+        // nc; one chunk of S voxels when it holds the space): x and y reach
+        // those chunks, z the group's own range inside them.
+        uint32_t lo[3], hi[3];
+        bool any = true;
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t cov = h->nc[a] * h->C[a];
+            if (mn[a] > cov - 1) any = false;
+            const uint32_t i0 = mn[a] / h->C[a], i1 = std::min(mx[a] / h->C[a], h->nc[a] - 1);
+            lo[a] = i0 * h->C[a];
+            hi[a] = std::min(std::min((i1 + 1) * h->C[a], cov), h->S[a]) - 1;
+        }
+        ga.overlaps = any ? 1 : 0;
+        ga.xStart = lo[0];
+        ga.xEnd = hi[0];
+        ga.yStart = lo[1];
+        ga.yEnd = hi[1];
         ga.zStart = mn[2];
-        ga.zEnd = std::min(mx[2], h->S[2] - 1);
+        ga.zEnd = std::min(mx[2], hi[2]);
         fill[g] = (ga.overlaps && ga.textured && ga.terrain) ? 1 : 0;
     }
     // (replay: the flags are already on the device from the eager pass)
@@ -668,6 +766,9 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     if (!h || !h->valid || !idle(h)) return 0;
     h->compressed = compressed != 0;
     if (h->replay && out_dir) return h->fail("a replayed generate writes no files"), 0;
+    if (h->replay && h->multi)
+        return h->fail("a voxelizer with several chunks orders its bricks on the host's schedule "
+                       "and is not captured in a graph"), 0;
     if (!h->prepared && !vox_compute_edges_face_normals_and_bounds(h)) return 0;
     const int G = (int)h->groups.size();
     const int L = h->L;
@@ -695,11 +796,18 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     // dense tile list is sized for every tile of the level.
     for (int k = 0; k < L; ++k) {
         LevelBufs& l = h->lv[k];
-        l.aliasOf = (!chain && k >= 2 && (k % 2) == 0) ? k - 1 : -1;
+        // (each chunk has its own chain: level 2j is not level 2j - 1 then)
+        l.aliasOf = (!chain && !h->multi && k >= 2 && (k % 2) == 0) ? k - 1 : -1;
         if (l.aliasOf >= 0) continue;
         if (k > 0) {
             LevelView s = view(h, srcOf(k));
-            h->run("mark", [&] { launch_mark_children(st, s.map, l.map, l.mark.as<uint32_t>()); });
+            if (h->multi)
+                h->run("mark", [&] {
+                    launch_chunk_mark(st, s.map, l.map, chunkGeom(h), srcOf(k), k,
+                                      l.mark.as<uint32_t>());
+                });
+            else
+                h->run("mark", [&] { launch_mark_children(st, s.map, l.map, l.mark.as<uint32_t>()); });
         }
         h->run("scan", [&] { scan(h, l.mark.as<uint32_t>(), l.excl.as<uint32_t>(), l.map.numTiles); });
         h->run("compact", [&] {
@@ -794,7 +902,11 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         if (h->lv[k].aliasOf >= 0) continue;
         LevelView s = view(h, srcOf(k));
         LevelView l = view(h, k);
-        if (l.active > 0) h->run("mip", [&] {
+        if (l.active > 0 && h->multi) h->run("mip", [&] {
+            launch_chunk_mip(st, s.map, l.map, chunkGeom(h), srcOf(k), k, l.active, l.list, s.col,
+                             s.nrm, l.col, l.qn, l.nrm);
+        });
+        else if (l.active > 0) h->run("mip", [&] {
             launch_mip(st, s.map, l.map, l.active, l.list, s.col, s.nrm, l.col, l.qn, l.nrm);
         });
     }
@@ -824,56 +936,118 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         for (int a = 0; a < 3; ++a) border[a] = std::max(border[a] >> 1, 1u);
     }
     S.start[L] = h->allNodes;
-    h->run("node_type", [&] {
-        launch_node_sample_all(st, S);
-        for (int d = L - 1; d >= 0; --d) {
-            LevelView l = view(h, L - 1 - d);
-            launch_node_compare(st, l.map, l.active, l.list, l.col, S.Rd[d], h->B, S.border[d],
-                                S.xLimit[d], S.cc[d], S.flag[d]);
-        }
-    });
-    // One scan over the flags of every depth; a depth's brick addresses are
-    // its scan values less the value at its first node.
-    h->run("scan", [&] {
-        scan(h, h->nodeFlag.as<uint32_t>(), h->nodeExcl.as<uint32_t>(), h->allNodes);
-    });
-    {
-        uint64_t offs[VOX_MAX_LEVELS + 1];
-        for (int d = 0; d < L; ++d) offs[d] = h->tree[d].off;
-        offs[L] = h->allNodes;
-        h->run("scan", [&] {
-            launch_range_counts(st, h->nodeExcl.as<uint32_t>(), offs, L, ctr + 32);
-        });
-    }
+    const uint64_t bvox = (uint64_t)(h->B[0] + 2) * (h->B[1] + 2) * (h->B[2] + 2);
     uint32_t nb[VOX_MAX_LEVELS] = {0};
-    if (h->replay) {
-        std::memcpy(nb, h->planNb, sizeof(nb));
-    } else {
-        if (!h->check(hipMemcpyAsync(nb, ctr + 32, 4 * L, hipMemcpyDeviceToHost, st), "brick counts") ||
+    if (h->multi) {
+        // Several chunks: per-chunk samples and types OR-ed over chunks, and
+        // bricks numbered in the order the brick writer writes them (the
+        // chunk that completes a brick, inside one chunk before across
+        // chunks, node order), by a key sort per depth.
+        const ChunkGeom cg = chunkGeom(h);
+        uint32_t* bcount = ctr + 64;
+        h->run("fill", [&] {
+            const FillSeg z[1] = {{bcount, (uint64_t)L, 0u}};
+            launch_fill_segs(st, z, 1);
+        });
+        h->run("node_type", [&] {
+            for (int d = L - 1; d >= 0; --d) {
+                LevelView l = view(h, L - 1 - d);
+                DepthBufs& t = h->tree[d];
+                launch_chunk_node_type(st, l.map, l.col, cg, L - 1 - d, t.Rd, h->B, S.border[d],
+                                       t.cc.as<uint32_t>(), S.flag[d]);
+                launch_chunk_brick_keys(st, S.flag[d], cg, L - 1 - d, t.Rd, h->B, S.border[d],
+                                        h->keysA.as<unsigned long long>() + t.off,
+                                        h->nodesA.as<uint32_t>() + t.off, bcount + d);
+            }
+        });
+        if (!h->check(hipMemcpyAsync(nb, bcount, 4 * L, hipMemcpyDeviceToHost, st), "brick counts") ||
             !h->check(hipStreamSynchronize(st), "node typing"))
             return 0;
-        std::memcpy(h->planNb, nb, sizeof(nb));
+        uint32_t rankBits = 0;
+        while ((1ull << rankBits) < (uint64_t)h->nc[0] * h->nc[1] * h->nc[2]) ++rankBits;
+        for (int d = 0; d < L; ++d) {
+            DepthBufs& t = h->tree[d];
+            t.bricks = nb[d];
+            h->stats.bricks_total += nb[d];
+            if (!t.brickNode.ensure((uint64_t)nb[d] * 4, h->err, "brick nodes") ||
+                !t.bcol.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick colours") ||
+                !t.bnrm.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick normals"))
+                return 0;
+            if (nb[d] > 1) {
+                const int bits = (int)(3 * d + 1 + rankBits);
+                size_t tb = 0;
+                if (!h->check(chunk_sort_pairs(st, nullptr, &tb, nullptr, nullptr, nullptr, nullptr,
+                                               nb[d], bits),
+                              "sort size") ||
+                    !h->sortTmp.ensure(tb, h->err, "sort scratch"))
+                    return 0;
+                tb = h->sortTmp.cap;
+                if (!h->check(chunk_sort_pairs(st, h->sortTmp.p, &tb,
+                                               h->keysA.as<unsigned long long>() + t.off,
+                                               h->keysB.as<unsigned long long>(),
+                                               h->nodesA.as<uint32_t>() + t.off,
+                                               h->nodesB.as<uint32_t>(), nb[d], bits),
+                              "brick order"))
+                    return 0;
+            }
+            const uint32_t* sorted =
+                nb[d] > 1 ? h->nodesB.as<uint32_t>() : h->nodesA.as<uint32_t>() + t.off;
+            h->run("brick_assign", [&] {
+                launch_chunk_brick_assign(st, t.nodes, S.flag[d], sorted, nb[d],
+                                          t.types.as<uint8_t>(), t.bidx.as<uint32_t>(),
+                                          t.brickNode.as<uint32_t>());
+            });
+        }
+    } else {
+        h->run("node_type", [&] {
+            launch_node_sample_all(st, S);
+            for (int d = L - 1; d >= 0; --d) {
+                LevelView l = view(h, L - 1 - d);
+                launch_node_compare(st, l.map, l.active, l.list, l.col, S.Rd[d], h->B, S.border[d],
+                                    S.xLimit[d], S.cc[d], S.flag[d]);
+            }
+        });
+        // One scan over the flags of every depth; a depth's brick addresses are
+        // its scan values less the value at its first node.
+        h->run("scan", [&] {
+            scan(h, h->nodeFlag.as<uint32_t>(), h->nodeExcl.as<uint32_t>(), h->allNodes);
+        });
+        {
+            uint64_t offs[VOX_MAX_LEVELS + 1];
+            for (int d = 0; d < L; ++d) offs[d] = h->tree[d].off;
+            offs[L] = h->allNodes;
+            h->run("scan", [&] {
+                launch_range_counts(st, h->nodeExcl.as<uint32_t>(), offs, L, ctr + 32);
+            });
+        }
+        if (h->replay) {
+            std::memcpy(nb, h->planNb, sizeof(nb));
+        } else {
+            if (!h->check(hipMemcpyAsync(nb, ctr + 32, 4 * L, hipMemcpyDeviceToHost, st), "brick counts") ||
+                !h->check(hipStreamSynchronize(st), "node typing"))
+                return 0;
+            std::memcpy(h->planNb, nb, sizeof(nb));
+        }
+        DepthOut dout{};
+        dout.L = L;
+        for (int d = 0; d < L; ++d) {
+            DepthBufs& t = h->tree[d];
+            t.bricks = nb[d];
+            h->stats.bricks_total += nb[d];
+            if (!t.brickNode.ensure((uint64_t)nb[d] * 4, h->err, "brick nodes") ||
+                !t.bcol.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick colours") ||
+                !t.bnrm.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick normals"))
+                return 0;
+            dout.bidx[d] = t.bidx.as<uint32_t>();
+            dout.brickNode[d] = t.brickNode.as<uint32_t>();
+            dout.types[d] = t.types.as<uint8_t>();
+            dout.off[d] = t.off;
+        }
+        dout.off[L] = h->allNodes;
+        h->run("brick_assign", [&] {
+            launch_brick_assign_all(st, h->nodeFlag.as<uint32_t>(), h->nodeExcl.as<uint32_t>(), dout);
+        });
     }
-    const uint64_t bvox = (uint64_t)(h->B[0] + 2) * (h->B[1] + 2) * (h->B[2] + 2);
-    DepthOut dout{};
-    dout.L = L;
-    for (int d = 0; d < L; ++d) {
-        DepthBufs& t = h->tree[d];
-        t.bricks = nb[d];
-        h->stats.bricks_total += nb[d];
-        if (!t.brickNode.ensure((uint64_t)nb[d] * 4, h->err, "brick nodes") ||
-            !t.bcol.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick colours") ||
-            !t.bnrm.ensure((uint64_t)nb[d] * bvox * 4, h->err, "brick normals"))
-            return 0;
-        dout.bidx[d] = t.bidx.as<uint32_t>();
-        dout.brickNode[d] = t.brickNode.as<uint32_t>();
-        dout.types[d] = t.types.as<uint8_t>();
-        dout.off[d] = t.off;
-    }
-    dout.off[L] = h->allNodes;
-    h->run("brick_assign", [&] {
-        launch_brick_assign_all(st, h->nodeFlag.as<uint32_t>(), h->nodeExcl.as<uint32_t>(), dout);
-    });
     // The bricks of all depths in one launch, the deepest (largest) first.
     GatherDepths GD{};
     for (int a = 0; a < 3; ++a) GD.B[a] = h->B[a];
@@ -1052,17 +1226,44 @@ int vox_copy_level(vox_ctx* h, int32_t level, uint32_t* colors, float* normals, 
     if (h->workReleased) return h->fail("vox_copy_level: the level tiles were released"), 0;
     if (level < 0 || level >= h->L) return h->fail("level out of range"), 0;
     LevelView l = view(h, level);
-    for (int a = 0; a < 3; ++a) dims[a] = l.map.dims[a];
+    // with several chunks level 0 is returned in the chunk-by-chunk extent
+    // of the other levels (the chunks' voxels, zero past the voxel space)
+    const bool pad = h->multi && level == 0;
+    for (int a = 0; a < 3; ++a) dims[a] = pad ? h->nc[a] * h->C[a] : l.map.dims[a];
+    if (!colors && !normals) return 1;
+    if (!colors || !normals) return h->fail("vox_copy_level: colors and normals are both needed"), 0;
     const uint64_t n = (uint64_t)l.map.dims[0] * l.map.dims[1] * l.map.dims[2];
     DevBuf c, nr;
     if (!c.ensure(n * 4, h->err, "dense colours") || !nr.ensure(n * 12, h->err, "dense normals"))
         return 0;
     launch_dense_level(h->st, l.map, l.col, l.nrm, c.as<uint32_t>(), nr.as<float>());
-    bool ok = h->check(hipMemcpyAsync(colors, c.p, n * 4, hipMemcpyDeviceToHost, h->st), "copy") &&
-              h->check(hipMemcpyAsync(normals, nr.p, n * 12, hipMemcpyDeviceToHost, h->st), "copy") &&
+    std::vector<uint32_t> hc;
+    std::vector<float> hn;
+    if (pad) {
+        hc.resize(n);
+        hn.resize(3 * n);
+    }
+    bool ok = h->check(hipMemcpyAsync(pad ? hc.data() : colors, c.p, n * 4, hipMemcpyDeviceToHost,
+                                      h->st), "copy") &&
+              h->check(hipMemcpyAsync(pad ? hn.data() : normals, nr.p, n * 12,
+                                      hipMemcpyDeviceToHost, h->st), "copy") &&
               h->check(hipStreamSynchronize(h->st), "copy level");
     c.release();
     nr.release();
+    if (ok && pad) {
+        const uint32_t* sd = l.map.dims;
+        const uint64_t out = (uint64_t)dims[0] * dims[1] * dims[2];
+        std::fill(colors, colors + out, 0u);
+        std::fill(normals, normals + 3 * out, 0.0f);
+        const uint32_t ex = std::min(sd[0], dims[0]);
+        for (uint32_t z = 0; z < std::min(sd[2], dims[2]); ++z)
+            for (uint32_t y = 0; y < std::min(sd[1], dims[1]); ++y) {
+                const uint64_t si = ((uint64_t)z * sd[1] + y) * sd[0];
+                const uint64_t di = ((uint64_t)z * dims[1] + y) * dims[0];
+                std::copy(hc.begin() + si, hc.begin() + si + ex, colors + di);
+                std::copy(hn.begin() + 3 * si, hn.begin() + 3 * (si + ex), normals + 3 * di);
+            }
+    }
     return ok ? 1 : 0;
 }
 

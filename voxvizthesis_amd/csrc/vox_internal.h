@@ -74,12 +74,19 @@ struct VoxParams {
     uint64_t numTiles;    // TG[0]*TG[1]*TG[2]
     float offP[3];        // p - extra*delta
     float dP[3];          // delta
+    // Chunk edge per axis (the reference's voxChunkDim; S when one chunk
+    // holds the voxel space): the colour pass measures voxel centres from
+    // the chunk's origin and the terrain fill stops at the chunk's z = 0.
+    uint32_t C[3];
 };
 
 struct GroupArgs {
     uint32_t triOff;      // first global triangle index of the group
     uint32_t ntri;
+    // voxels the group's triangles may reach: z from the group's bounds, x
+    // and y the chunks its bounds overlap (the whole space with one chunk)
     uint32_t zStart, zEnd;
+    uint32_t xStart, xEnd, yStart, yEnd;
     int32_t overlaps;     // group bbox overlaps the chunk
     int32_t textured;
     int32_t terrain;
@@ -118,6 +125,9 @@ struct OctreeRefs {
     uint32_t B[3];          // brick interior
     uint32_t poolDims[3];   // brick pool texels per axis
     uint32_t poolSlots[3];
+    // wide pool: slots stacked along z, word 1 of a resident node holds the
+    // slot index instead of 10-bit texel coordinates (no 1024-texel limit)
+    uint32_t poolWide;
     uint32_t* poolCol;
     uint32_t* poolNrm;
     hipStream_t st;         // the pool's stream
@@ -148,6 +158,7 @@ bool raycaster_requests(vox_raycaster* rc, RequestRefs& r);
 // A brick pool's storage and its stream.
 struct PoolRefs {
     uint32_t B[3], dims[3], slots[3];
+    uint32_t wide;          // see OctreeRefs::poolWide
     uint32_t* col;
     uint32_t* nrm;
     hipStream_t st;
@@ -254,6 +265,52 @@ struct GatherDepths {
 void launch_brick_gather_all(hipStream_t st, const GatherDepths& G);
 void launch_collapse(hipStream_t st, uint32_t Rd, int childIsLeaf, uint8_t* types,
                      const uint8_t* childTypes);
+
+// ---- several chunks (generateVoxelsAndOctTree's chunk loop, Voxelizer.cu
+// 2180-2814, with chunks smaller than the voxel space) ----
+// Level k is laid out chunk by chunk: chunk i of an axis holds voxels
+// [i * Ck, (i + 1) * Ck), Ck = C >> k, of that chunk's own mip chain.
+constexpr int kMaxChunksAxis = 64;
+struct ChunkGeom {
+    uint32_t C[3];        // chunk edge at level 0
+    uint32_t nc[3];       // chunks per axis (origins below N)
+    // visit rank of chunk (i, j, l) at [(l * nc[1] + j) * nc[0] + i]
+    const uint32_t* rank;
+};
+// mark[t] = 1 when a voxel of level tile t reads an active source tile
+// (source voxels 2x, 2x + 1 of the same chunk, local coordinates).
+// (ks: the source level, kd: the level built from it.)
+void launch_chunk_mark(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                       const ChunkGeom& cg, int ks, int kd, uint32_t* markDst);
+void launch_chunk_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                      const ChunkGeom& cg, int ks, int kd, uint64_t nActive,
+                      const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
+                      uint32_t* colDst, uint32_t* qnDst, float* nrmDst);
+// Node types and constant colours of one depth: flag[n] = 1 when a chunk
+// the node's stored brick reaches holds a typed voxel of the brick unequal
+// to that chunk's sample; cc[n] = the sample of the last chunk in visit
+// order.
+// (k: the level the depth reads.)
+void launch_chunk_node_type(hipStream_t st, const LevelMap& lm, const uint32_t* col,
+                            const ChunkGeom& cg, int k, uint32_t Rd, const uint32_t B[3],
+                            const uint32_t border[3], uint32_t* cc, uint32_t* flag);
+// Brick keys of one depth: for each NON-CONST node whose stored brick lies
+// in processed chunks, key = (rank of the chunk that completes it << (1 +
+// 3d)) | (0 inside one chunk, 1 across chunks) << 3d | node; keys and nodes
+// appended at *count.
+void launch_chunk_brick_keys(hipStream_t st, const uint32_t* flag, const ChunkGeom& cg, int k,
+                             uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
+                             unsigned long long* keys, uint32_t* nodes, uint32_t* count);
+// Sort n (key, node) pairs by key (tmp: temporary storage; *tmpBytes is
+// its size on entry, or the size needed when tmp is null).
+hipError_t chunk_sort_pairs(hipStream_t st, void* tmp, size_t* tmpBytes,
+                            const unsigned long long* keysIn, unsigned long long* keysOut,
+                            const uint32_t* nodesIn, uint32_t* nodesOut, uint32_t n, int keyBits);
+// After the sort: types[node] = flag, bidx[node] = 0xffffffff, then
+// bidx[sorted[j]] = j and brickNode[j] = sorted[j].
+void launch_chunk_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
+                               const uint32_t* sorted, uint32_t nb, uint8_t* types,
+                               uint32_t* bidx, uint32_t* brickNode);
 // The whole bitmap (no clear needed) when S[0] is a multiple of 32 and a row
 // of tile indices fits in LDS; false (nothing launched) otherwise.
 bool launch_occupancy_rows(hipStream_t st, const VoxParams& P, const int32_t* tileIdx,
@@ -299,8 +356,8 @@ void launch_bc_bricks(hipStream_t st, const uint32_t* col, const uint32_t* nrm, 
 // pool slots; *nreq requests, at most maxReq.
 void launch_brick_upload(hipStream_t st, const BrickSrc& src, const uint4* req,
                          const uint32_t* nreq, uint32_t maxReq, const uint32_t e[3],
-                         const uint32_t slots[3], const uint32_t dims[3], uint32_t* col,
-                         uint32_t* nrm, uint32_t* tex);
+                         const uint32_t slots[3], const uint32_t dims[3], uint32_t wide,
+                         uint32_t* col, uint32_t* nrm, uint32_t* tex);
 // exclusive scan of n u32 values; out has n+1 entries, out[n] = total.
 // tmp must hold scan_tmp_words(n) words, all zero before its first scan
 // (ensure_scan_tmp allocates it so); a scan leaves it ready for the next.

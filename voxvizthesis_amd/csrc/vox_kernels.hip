@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "vox_internal.h"
 
 #pragma clang fp contract(off)
@@ -243,16 +245,25 @@ __device__ __forceinline__ uint64_t pair_key(const PairList& L, const VoxParams&
     return base + ((uint64_t)(L.z[lo] + dz) * P.TG[1] + ((xy >> 16) + dy)) * P.TG[0] +
            (xy & 0xffffu) + dx;
 }
-// Tile box of voxel bounds (packed min | max << 16 per axis) within the
-// group's z range: false when empty.
+// Voxel bounds (packed min | max << 16 per axis) clipped to the group's
+// reach (its z range; x and y the chunks it overlaps): false when empty.
+__device__ __forceinline__ bool clip_box(const GroupArgs& G, uint32_t bx, uint32_t by,
+                                         uint32_t bz, uint32_t (&v0)[3], uint32_t (&v1)[3]) {
+    v0[0] = max(bx & 0xffffu, G.xStart); v1[0] = min(bx >> 16, G.xEnd);
+    v0[1] = max(by & 0xffffu, G.yStart); v1[1] = min(by >> 16, G.yEnd);
+    v0[2] = max(bz & 0xffffu, G.zStart); v1[2] = min(bz >> 16, G.zEnd);
+    return v0[0] <= v1[0] && v0[1] <= v1[1] && v0[2] <= v1[2];
+}
+// Tile box of those clipped bounds: false when empty.
 __device__ __forceinline__ bool tile_box(const GroupArgs& G, uint32_t bx, uint32_t by,
                                          uint32_t bz, uint32_t (&t0)[3], uint32_t (&t1)[3]) {
-    const uint32_t z0 = (bz & 0xffffu) > G.zStart ? (bz & 0xffffu) : G.zStart;
-    const uint32_t z1 = (bz >> 16) < G.zEnd ? (bz >> 16) : G.zEnd;
-    if (z0 > z1) return false;
-    t0[0] = (bx & 0xffffu) >> 3; t1[0] = (bx >> 16) >> 3;
-    t0[1] = (by & 0xffffu) >> 3; t1[1] = (by >> 16) >> 3;
-    t0[2] = z0 >> 3; t1[2] = z1 >> 3;
+    uint32_t v0[3], v1[3];
+    if (!clip_box(G, bx, by, bz, v0, v1)) return false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        t0[a] = v0[a] >> 3;
+        t1[a] = v1[a] >> 3;
+    }
     return true;
 }
 // Fill the workgroup's pair list (barriers inside: every thread calls it).
@@ -747,11 +758,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_voxelize(
             const float v[3][3] = {{q[0], q[1], q[2]}, {q[3], q[4], q[5]}, {q[6], q[7], q[8]}};
             const float* fq = fnorm + 3 * (size_t)t;
             const float n[3] = {fq[0], fq[1], fq[2]};
-            int32_t x0 = (int32_t)(bx & 0xffffu) - (int32_t)ox, x1 = (int32_t)(bx >> 16) - (int32_t)ox;
-            int32_t y0 = (int32_t)(by & 0xffffu) - (int32_t)oy, y1 = (int32_t)(by >> 16) - (int32_t)oy;
-            uint32_t zlo = (bz & 0xffffu) > G.zStart ? (bz & 0xffffu) : G.zStart;
-            uint32_t zhi = (bz >> 16) < G.zEnd ? (bz >> 16) : G.zEnd;
-            int32_t z0 = (int32_t)zlo - (int32_t)oz, z1 = (int32_t)zhi - (int32_t)oz;
+            uint32_t c0[3], c1[3];
+            const bool inReach = clip_box(G, bx, by, bz, c0, c1);
+            int32_t x0 = (int32_t)c0[0] - (int32_t)ox, x1 = (int32_t)c1[0] - (int32_t)ox;
+            int32_t y0 = (int32_t)c0[1] - (int32_t)oy, y1 = (int32_t)c1[1] - (int32_t)oy;
+            int32_t z0 = (int32_t)c0[2] - (int32_t)oz, z1 = (int32_t)c1[2] - (int32_t)oz;
+            if (!inReach) x1 = -1;
             x0 = x0 < 0 ? 0 : x0; y0 = y0 < 0 ? 0 : y0; z0 = z0 < 0 ? 0 : z0;
             x1 = x1 > 7 ? 7 : x1; y1 = y1 > 7 ? 7 : y1; z1 = z1 > 7 ? 7 : z1;
             if (x0 <= x1 && y0 <= y1 && z0 <= z1) {
@@ -1093,8 +1105,14 @@ __global__ __launch_bounds__(kBlock) void k_colour(
                                nrm[(a * 3 + 2) * kTileVox + v]}
                           : f3{0.0f, 0.0f, 0.0f};
             const uint32_t X = r1.w & 0xffffu, Y = r1.w >> 16, Z = r0.y >> 16;
-            const f3 cpos = {P.offP[0] + (float)X * P.dP[0], P.offP[1] + (float)Y * P.dP[1],
-                             P.offP[2] + (float)Z * P.dP[2]};
+            // the chunk's origin in float (voxOffsetOrigin, Voxelizer.cu
+            // 2317-2319), then the voxel's place in the chunk (VK:715-719);
+            // with one chunk the origin is offP itself
+            const uint32_t cx = X - X % P.C[0], cy = Y - Y % P.C[1], cz = Z - Z % P.C[2];
+            const f3 org = {P.offP[0] + (float)cx * P.dP[0], P.offP[1] + (float)cy * P.dP[1],
+                            P.offP[2] + (float)cz * P.dP[2]};
+            const f3 cpos = {org.x + (float)(X - cx) * P.dP[0], org.y + (float)(Y - cy) * P.dP[1],
+                             org.z + (float)(Z - cz) * P.dP[2]};
             const f3 voxCenter = add3(cpos, muls3(f3{P.dP[0], P.dP[1], P.dP[2]}, 0.5f));
             colour_update(Gl, voxCenter, wasHit + cnt, (int)ns, c, nv, s_inv255,
                           [&](int k, float (&at)[kTriAttr]) {
@@ -1222,10 +1240,16 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init, u
         }
     uint32_t lastZ = 0xffffffffu, sc = 0u, sq = 0u;
     float sn0 = 0.0f, sn1 = 0.0f, sn2 = 0.0f;
+    // the fill stops at the chunk's z = 0 (the fill of a voxel only reads
+    // voxels above it in the same chunk along z)
+    const uint32_t zt = tz * kTile;
 #pragma unroll
     for (int lz = kTile - 1; lz >= 0; --lz) {
         const uint32_t v = ((uint32_t)lz << 6) | c;
-        const uint32_t above = (m >> (lz + 1)) & ((1u << kFillDepth) - 1u);
+        const uint32_t zg = zt + (uint32_t)lz;
+        const uint32_t zLim = (zg / P.C[2] + 1u) * P.C[2];   // first z of the next chunk
+        const uint32_t reach = min(zLim - zg - 1u, (uint32_t)kFillDepth);
+        const uint32_t above = (m >> (lz + 1)) & ((1u << reach) - 1u);
         const bool filled = !((m >> lz) & 1u) && above != 0u;
         if (filled) {
             const uint32_t zz = (uint32_t)lz + (uint32_t)__ffs(above);
@@ -2136,6 +2160,319 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
         fill4(p, q * 4, n, v);
 }
 
+// ---------------------------------------------------------------------------
+// Several chunks (generateVoxelsAndOctTree with voxChunkDim smaller than the
+// voxel space, Voxelizer.cu 2180-2814).  Level k is laid out chunk by chunk:
+// chunk i of an axis holds voxels [i * Ck, (i + 1) * Ck) of its own mip chain
+// (computeMipMaps per chunk, Voxelizer.cu 491-594), so level k voxel X of
+// chunk i = X / Ck reads source voxels i * Cs + 2 (X - i * Ck) and the next
+// one (clamped into the chunk).  Voxels past the level's extent read as zero.
+__device__ __forceinline__ uint32_t level_col(const LevelMap& lm, const uint32_t* __restrict__ col,
+                                              uint32_t X, uint32_t Y, uint32_t Z) {
+    if (X >= lm.dims[0] || Y >= lm.dims[1] || Z >= lm.dims[2]) return 0u;
+    const int64_t a = tile_of(lm, X, Y, Z);
+    if (a < 0) return 0u;
+    const uint32_t v = vox_of(X, Y, Z);
+    return stored(lm, (uint64_t)a, v) ? col[(uint64_t)a * kTileVox + v] : 0u;
+}
+__device__ __forceinline__ f3 level_nrm(const LevelMap& lm, const float* __restrict__ nrm,
+                                        uint32_t X, uint32_t Y, uint32_t Z) {
+    if (X >= lm.dims[0] || Y >= lm.dims[1] || Z >= lm.dims[2]) return {0.0f, 0.0f, 0.0f};
+    const int64_t a = tile_of(lm, X, Y, Z);
+    if (a < 0) return {0.0f, 0.0f, 0.0f};
+    const uint32_t v = vox_of(X, Y, Z);
+    if (!stored(lm, (uint64_t)a, v)) return {0.0f, 0.0f, 0.0f};
+    const uint64_t b = (uint64_t)a * 3 * kTileVox + v;
+    return {nrm[b], nrm[b + kTileVox], nrm[b + 2 * kTileVox]};
+}
+// Source voxels of level voxel X along one axis.
+__device__ __forceinline__ void chunk_src(uint32_t X, uint32_t Cd, uint32_t Cs, uint32_t& s0,
+                                          uint32_t& s1) {
+    const uint32_t i = X / Cd, x = X - i * Cd;
+    s0 = i * Cs + 2u * x;
+    s1 = i * Cs + min(2u * x + 1u, Cs - 1u);
+}
+
+__global__ void k_chunk_mark(LevelMap src, LevelMap dst, ChunkGeom cg, int ks, int kd,
+                             uint32_t* __restrict__ markDst) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dst.numTiles) return;
+    uint32_t p[3];
+    tile_xyz(t, dst.TG, p[0], p[1], p[2]);
+    // per axis the distinct source tiles the tile's voxels read (at most 8:
+    // a voxel's two sources share a tile)
+    uint32_t st[3][kTile], ns[3] = {0u, 0u, 0u};
+    for (int a = 0; a < 3; ++a) {
+        for (uint32_t q = 0; q < (uint32_t)kTile; ++q) {
+            const uint32_t X = p[a] * kTile + q;
+            if (X >= dst.dims[a]) break;
+            uint32_t s0, s1;
+            chunk_src(X, cg.C[a] >> kd, cg.C[a] >> ks, s0, s1);
+            const uint32_t tt = s0 >> 3;
+            if (tt >= src.TG[a]) continue;
+            if (ns[a] == 0u || st[a][ns[a] - 1u] != tt) st[a][ns[a]++] = tt;
+        }
+    }
+    uint32_t m = 0u;
+    for (uint32_t k = 0; k < ns[2] && !m; ++k)
+        for (uint32_t j = 0; j < ns[1] && !m; ++j)
+            for (uint32_t i = 0; i < ns[0]; ++i)
+                if (src.tileIdx[((uint64_t)st[2][k] * src.TG[1] + st[1][j]) * src.TG[0] + st[0][i]] >= 0) {
+                    m = 1u;
+                    break;
+                }
+    markDst[t] = m;
+}
+
+// ComputeVoxelMipMap8 (VoxelizerKernels.cu 1520-1663) inside each chunk:
+// the same sum order and arithmetic as k_mip, sources addressed per chunk.
+__global__ __launch_bounds__(kBlock) void k_chunk_mip(LevelMap src, LevelMap dst, ChunkGeom cg,
+                                                      int ks, int kd,
+                                                      const int32_t* __restrict__ listDst,
+                                                      const uint32_t* __restrict__ colSrc,
+                                                      const float* __restrict__ nrmSrc,
+                                                      uint32_t* __restrict__ colDst,
+                                                      uint32_t* __restrict__ qnDst,
+                                                      float* __restrict__ nrmDst) {
+    const int tid = threadIdx.x;
+    const uint64_t b = blockIdx.x;
+    const uint64_t tlin = (uint64_t)listDst[b];
+    uint32_t tx, ty, tz;
+    tile_xyz(tlin, dst.TG, tx, ty, tz);
+    __shared__ float s_inv255[256];
+    fill_inv255(s_inv255);
+    __syncthreads();
+    for (int pass = 0; pass < kTileVox / kBlock; ++pass) {
+        const uint32_t v = (uint32_t)(pass * kBlock + tid);
+        const uint32_t X = tx * kTile + (v & 7u), Y = ty * kTile + ((v >> 3) & 7u),
+                       Z = tz * kTile + (v >> 6);
+        uint32_t outC = 0u;
+        f3 outN = {0.0f, 0.0f, 0.0f};
+        if (X < dst.dims[0] && Y < dst.dims[1] && Z < dst.dims[2]) {
+            uint32_t xs[2], ys[2], zs[2];
+            chunk_src(X, cg.C[0] >> kd, cg.C[0] >> ks, xs[0], xs[1]);
+            chunk_src(Y, cg.C[1] >> kd, cg.C[1] >> ks, ys[0], ys[1]);
+            chunk_src(Z, cg.C[2] >> kd, cg.C[2] >> ks, zs[0], zs[1]);
+            // the sum in the order (z, y, x), x fastest
+            f4 avg = {0.0f, 0.0f, 0.0f, 0.0f};
+            f3 an = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t sx = xs[i & 1], sy = ys[(i >> 1) & 1], sz = zs[i >> 2];
+                const f4 cur = unpack4(level_col(src, colSrc, sx, sy, sz), s_inv255);
+                const f3 cn = level_nrm(src, nrmSrc, sx, sy, sz);
+                if (i == 0) {
+                    avg = cur;
+                    an = cn;
+                } else {
+                    avg.x += cur.x; avg.y += cur.y; avg.z += cur.z; avg.w += cur.w;
+                    an = add3(an, cn);
+                }
+            }
+            const float oneOverEight = 1.0f / 8.0f;
+            avg.w *= oneOverEight;
+            avg.x *= oneOverEight;
+            avg.y *= oneOverEight;
+            avg.z *= oneOverEight;
+            an = muls3(an, oneOverEight);
+            float len2 = dot3(an, an);
+            if (len2 > 0.0f) {
+                float l = sqrtf(len2);
+                an = {an.x / l, an.y / l, an.z / l};
+            } else {
+                an = {0.0f, 0.0f, 0.0f};
+            }
+            outC = to_color(avg);
+            outN = an;
+        }
+        colDst[b * kTileVox + v] = outC;
+        qnDst[b * kTileVox + v] = quant_normal(outN);
+        nrmDst[(b * 3 + 0) * kTileVox + v] = outN.x;
+        nrmDst[(b * 3 + 1) * kTileVox + v] = outN.y;
+        nrmDst[(b * 3 + 2) * kTileVox + v] = outN.z;
+    }
+}
+
+// Voxels of a chunk the reference's typing launch reaches along one axis
+// (Voxelizer.cu 1037-1120): tpb = the chunk's extent, cut back past the
+// node grid's end (unsigned arithmetic as there), 8-thread blocks, and on x
+// the split into two launches whose halved block count is rounded up only
+// when odd.
+__device__ __forceinline__ uint32_t chunk_typed_limit(uint32_t Ck, uint32_t o, uint32_t bo,
+                                                      uint32_t full, bool xAxis) {
+    uint32_t tpb = Ck;
+    if (tpb + o - bo > full) tpb = full - (o - bo) + 2u;
+    if (tpb == 0u) return 0u;
+    const uint64_t th = tpb < 8u ? tpb : 8u;
+    uint64_t nb = ((uint64_t)tpb + th - 1u) / th;
+    uint64_t cover = nb * th;
+    if (xAxis && tpb > 128u) {
+        nb >>= 1;
+        if (nb & 1u) nb += 1u;
+        cover = 2ull * nb * th;
+    }
+    return cover < Ck ? (uint32_t)cover : Ck;
+}
+
+// ComputeOctTreeNodeConstColor + ComputeOctTreeNodeType per chunk
+// (VoxelizerKernels.cu 1680-2380, Voxelizer.cu 931-1172), one workgroup per
+// node: for every chunk the node's stored brick reaches, the chunk's sample
+// (the brick's first interior voxel clamped into the chunk) against the
+// brick's voxels in that chunk that the chunk's typing launch reaches; the
+// node is NON-CONST when any differs (types OR-ed over chunks), and its
+// colour is the sample of the last chunk in visit order.
+__global__ __launch_bounds__(kBlock) void k_chunk_node_type(
+    LevelMap lm, const uint32_t* __restrict__ col, ChunkGeom cg, int k, uint32_t Rd, uint32_t B0,
+    uint32_t B1, uint32_t B2, uint32_t bo0, uint32_t bo1, uint32_t bo2, uint32_t* __restrict__ cc,
+    uint32_t* __restrict__ flag) {
+    __shared__ uint32_t s_any, s_diff;
+    const uint32_t Ck[3] = {cg.C[0] >> k, cg.C[1] >> k, cg.C[2] >> k}, Bv[3] = {B0, B1, B2},
+                   bo[3] = {bo0, bo1, bo2};
+    const uint64_t nn = (uint64_t)Rd * Rd * Rd;
+    const uint32_t ld = (uint32_t)(__ffs(Rd) - 1);
+    for (uint64_t n = blockIdx.x; n < nn; n += gridDim.x) {
+        uint32_t nd[3];
+        node_xyz(n, ld, nd[0], nd[1], nd[2]);
+        uint32_t lo[3], hi[3], c0[3], c1[3], sp[3], cov[3];
+        bool inAny = true;
+        for (int a = 0; a < 3; ++a) {
+            sp[a] = nd[a] * Bv[a] + bo[a];
+            lo[a] = sp[a] - 1u;
+            hi[a] = sp[a] + Bv[a];
+            cov[a] = cg.nc[a] * Ck[a];
+            c0[a] = lo[a] / Ck[a];
+            c1[a] = min(hi[a] / Ck[a], cg.nc[a] - 1u);
+            if (c0[a] >= cg.nc[a]) inAny = false;
+        }
+        // the constant colour: the last chunk in visit order
+        if (threadIdx.x == 0) {
+            uint32_t best = 0u, bestRank = 0u;
+            bool have = false;
+            if (inAny)
+                for (uint32_t k = c0[2]; k <= c1[2]; ++k)
+                    for (uint32_t j = c0[1]; j <= c1[1]; ++j)
+                        for (uint32_t i = c0[0]; i <= c1[0]; ++i) {
+                            const uint32_t r = cg.rank[((uint64_t)k * cg.nc[1] + j) * cg.nc[0] + i];
+                            if (!have || r > bestRank) {
+                                have = true;
+                                bestRank = r;
+                                const uint32_t ci[3] = {i, j, k};
+                                uint32_t q[3];
+                                for (int a = 0; a < 3; ++a)
+                                    q[a] = min(max(sp[a], ci[a] * Ck[a]), ci[a] * Ck[a] + Ck[a] - 1u);
+                                best = level_col(lm, col, q[0], q[1], q[2]);
+                            }
+                        }
+            cc[n] = best;
+            s_any = 0u;
+            s_diff = 0u;
+        }
+        __syncthreads();
+        // every voxel of the brick zero (no active tile under it) and so every
+        // sample zero: CONST without reading the voxels
+        if (inAny) {
+            const uint32_t t0[3] = {lo[0] >> 3, lo[1] >> 3, lo[2] >> 3};
+            uint32_t t1[3];
+            for (int a = 0; a < 3; ++a) t1[a] = min(hi[a], cov[a] - 1u) >> 3;
+            const uint32_t ntx = t1[0] - t0[0] + 1u, nty = t1[1] - t0[1] + 1u;
+            const uint32_t ntl = ntx * nty * (t1[2] - t0[2] + 1u);
+            for (uint32_t q = threadIdx.x; q < ntl; q += blockDim.x) {
+                const uint32_t qx = q % ntx, qy = (q / ntx) % nty, qz = q / (ntx * nty);
+                const uint32_t tx = t0[0] + qx, ty = t0[1] + qy, tz = t0[2] + qz;
+                if (tx < lm.TG[0] && ty < lm.TG[1] && tz < lm.TG[2] &&
+                    lm.tileIdx[((uint64_t)tz * lm.TG[1] + ty) * lm.TG[0] + tx] >= 0)
+                    s_any = 1u;
+            }
+        }
+        __syncthreads();
+        if (s_any) {
+            const uint32_t e[3] = {Bv[0] + 2u, Bv[1] + 2u, Bv[2] + 2u};
+            const uint32_t nv = e[0] * e[1] * e[2];
+            const uint32_t full[3] = {Rd * Bv[0], Rd * Bv[1], Rd * Bv[2]};
+            bool diff = false;
+            for (uint32_t q = threadIdx.x; q < nv && !diff; q += blockDim.x) {
+                const uint32_t P3[3] = {lo[0] + q % e[0], lo[1] + (q / e[0]) % e[1],
+                                        lo[2] + q / (e[0] * e[1])};
+                bool typed = true;
+                uint32_t s3[3];
+                for (int a = 0; a < 3; ++a) {
+                    if (P3[a] >= cov[a]) { typed = false; break; }
+                    const uint32_t ci = P3[a] / Ck[a], o = ci * Ck[a];
+                    if (P3[a] - o >= chunk_typed_limit(Ck[a], o, bo[a], full[a], a == 0)) {
+                        typed = false;
+                        break;
+                    }
+                    s3[a] = min(max(sp[a], o), o + Ck[a] - 1u);
+                }
+                if (!typed) continue;
+                const uint32_t vcol = level_col(lm, col, P3[0], P3[1], P3[2]);
+                const uint32_t scol = level_col(lm, col, s3[0], s3[1], s3[2]);
+                diff = vcol != scol;
+            }
+            if (diff) s_diff = 1u;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) flag[n] = s_diff;
+        __syncthreads();
+    }
+}
+
+// Brick keys: a NON-CONST node whose stored brick lies in processed chunks
+// is written by the chunk that completes it (the brick writer's write
+// order, Voxelizer.cu 1173-1262 and 2778, VoxelBrickWriter.cu 1131-1210):
+// inside one chunk it is written while that chunk is typed; across chunks
+// it is assembled and written after the last of them, with the other
+// bricks completed there, in (z, y, x) order.
+__global__ void k_chunk_brick_keys(const uint32_t* __restrict__ flag, ChunkGeom cg, int k,
+                                   uint32_t Rd, uint32_t B0, uint32_t B1, uint32_t B2,
+                                   uint32_t bo0, uint32_t bo1, uint32_t bo2,
+                                   unsigned long long* __restrict__ keys,
+                                   uint32_t* __restrict__ nodes, uint32_t* __restrict__ count) {
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nn = (uint64_t)Rd * Rd * Rd;
+    if (n >= nn || flag[n] != 1u) return;
+    const uint32_t Ck[3] = {cg.C[0] >> k, cg.C[1] >> k, cg.C[2] >> k}, Bv[3] = {B0, B1, B2},
+                   bo[3] = {bo0, bo1, bo2};
+    const uint32_t ld = (uint32_t)(__ffs(Rd) - 1);
+    uint32_t nd[3];
+    node_xyz(n, ld, nd[0], nd[1], nd[2]);
+    uint32_t c0[3], c1[3];
+    bool inside = true;
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t lo = nd[a] * Bv[a] + bo[a] - 1u, hi = nd[a] * Bv[a] + bo[a] + Bv[a];
+        if (hi >= cg.nc[a] * Ck[a]) return;   // part of it never processed: no brick
+        c0[a] = lo / Ck[a];
+        c1[a] = hi / Ck[a];
+        inside = inside && c0[a] == c1[a];
+    }
+    uint32_t last = 0u;
+    for (uint32_t k = c0[2]; k <= c1[2]; ++k)
+        for (uint32_t j = c0[1]; j <= c1[1]; ++j)
+            for (uint32_t i = c0[0]; i <= c1[0]; ++i)
+                last = max(last, cg.rank[((uint64_t)k * cg.nc[1] + j) * cg.nc[0] + i]);
+    const uint32_t sh = 3u * ld;
+    const uint32_t slot = atomicAdd(count, 1u);
+    keys[slot] = ((unsigned long long)last << (sh + 1u)) | ((unsigned long long)(inside ? 0u : 1u) << sh) |
+                 (unsigned long long)n;
+    nodes[slot] = (uint32_t)n;
+}
+
+__global__ void k_chunk_types(uint64_t nn, const uint32_t* __restrict__ flag,
+                              uint8_t* __restrict__ types, uint32_t* __restrict__ bidx) {
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    types[n] = (uint8_t)flag[n];
+    bidx[n] = 0xffffffffu;
+}
+__global__ void k_chunk_bidx(uint32_t nb, const uint32_t* __restrict__ sorted,
+                             uint32_t* __restrict__ bidx, uint32_t* __restrict__ brickNode) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nb) return;
+    const uint32_t n = sorted[j];
+    bidx[n] = j;
+    brickNode[j] = n;
+}
+
 inline unsigned grid_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -2346,6 +2683,43 @@ void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, 
 void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp,
                        uint32_t* err) {
     scan_launch(st, in, out, n, tmp, in, err);
+}
+
+
+void launch_chunk_mark(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                       const ChunkGeom& cg, int ks, int kd, uint32_t* markDst) {
+    if (dst.numTiles == 0) return;
+    k_chunk_mark<<<grid_for(dst.numTiles, kBlock), kBlock, 0, st>>>(src, dst, cg, ks, kd, markDst);
+}
+void launch_chunk_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                      const ChunkGeom& cg, int ks, int kd, uint64_t nActive,
+                      const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
+                      uint32_t* colDst, uint32_t* qnDst, float* nrmDst) {
+    if (nActive == 0) return;
+    k_chunk_mip<<<(unsigned)nActive, kBlock, 0, st>>>(src, dst, cg, ks, kd, listDst, colSrc, nrmSrc,
+                                                      colDst, qnDst, nrmDst);
+}
+void launch_chunk_node_type(hipStream_t st, const LevelMap& lm, const uint32_t* col,
+                            const ChunkGeom& cg, int k, uint32_t Rd, const uint32_t B[3],
+                            const uint32_t border[3], uint32_t* cc, uint32_t* flag) {
+    const uint64_t nn = (uint64_t)Rd * Rd * Rd;
+    const unsigned grid = (unsigned)std::min<uint64_t>(nn, 1u << 20);
+    k_chunk_node_type<<<grid, kBlock, 0, st>>>(lm, col, cg, k, Rd, B[0], B[1], B[2], border[0],
+                                               border[1], border[2], cc, flag);
+}
+void launch_chunk_brick_keys(hipStream_t st, const uint32_t* flag, const ChunkGeom& cg, int k,
+                             uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
+                             unsigned long long* keys, uint32_t* nodes, uint32_t* count) {
+    const uint64_t nn = (uint64_t)Rd * Rd * Rd;
+    k_chunk_brick_keys<<<grid_for(nn, kBlock), kBlock, 0, st>>>(flag, cg, k, Rd, B[0], B[1], B[2],
+                                                                border[0], border[1], border[2],
+                                                                keys, nodes, count);
+}
+void launch_chunk_brick_assign(hipStream_t st, uint64_t nodes, const uint32_t* flag,
+                               const uint32_t* sorted, uint32_t nb, uint8_t* types,
+                               uint32_t* bidx, uint32_t* brickNode) {
+    if (nodes) k_chunk_types<<<grid_for(nodes, kBlock), kBlock, 0, st>>>(nodes, flag, types, bidx);
+    if (nb) k_chunk_bidx<<<grid_for(nb, kBlock), kBlock, 0, st>>>(nb, sorted, bidx, brickNode);
 }
 
 }  // namespace voxmi

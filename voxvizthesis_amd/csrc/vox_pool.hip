@@ -65,6 +65,18 @@ __device__ __forceinline__ void block_origin(uint32_t b, uint32_t bx, uint32_t b
     oz = 2 * bz;
 }
 
+// A NON-CONST node without a brick (several chunks: its brick reaches a
+// chunk that is not processed) reads brick 0, as the reader does with the
+// tree file's Brick="0" (GigaVoxelsReader.cpp 316-321); a depth without any
+// brick makes such a node constant instead (its stored colour).
+__global__ void k_missing_bricks(uint64_t n, uint64_t nb, uint8_t* __restrict__ types,
+                                 uint32_t* __restrict__ bidx) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || types[i] != 1u || bidx[i] != 0xffffffffu) return;
+    if (nb > 0) bidx[i] = 0u;
+    else types[i] = 0u;
+}
+
 // state bit 0: node is in the tree; bit 1: node has a child block (in the tree,
 // above the last depth, not LEAF-CONST: LoadOctTreeNode 326-327).
 __global__ void k_tree_state(uint64_t n, uint32_t sh, int last, const uint8_t* __restrict__ types,
@@ -179,8 +191,8 @@ namespace voxmi {
 __global__ void k_brick_upload(BrickSrc src, const uint4* __restrict__ req,
                                const uint32_t* __restrict__ nreq, uint32_t ex, uint32_t ey,
                                uint32_t ez, uint32_t sx, uint32_t sy, uint32_t dx, uint32_t dy,
-                               uint32_t* __restrict__ col, uint32_t* __restrict__ nrm,
-                               uint32_t* __restrict__ tex) {
+                               uint32_t wide, uint32_t* __restrict__ col,
+                               uint32_t* __restrict__ nrm, uint32_t* __restrict__ tex) {
     const uint32_t count = *nreq;
     const uint32_t nv = ex * ey * ez;
     for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
@@ -201,19 +213,19 @@ __global__ void k_brick_upload(BrickSrc src, const uint4* __restrict__ req,
         }
         if (threadIdx.x == 0 && q.w != kNoTexel) {
             tex[2 * (uint64_t)q.w] |= kResidentBit;
-            tex[2 * (uint64_t)q.w + 1] = pack10(ox + 1, oy + 1, oz + 1);
+            tex[2 * (uint64_t)q.w + 1] = wide ? q.z : pack10(ox + 1, oy + 1, oz + 1);
         }
     }
 }
 
 void launch_brick_upload(hipStream_t st, const BrickSrc& src, const uint4* req,
                          const uint32_t* nreq, uint32_t maxReq, const uint32_t e[3],
-                         const uint32_t slots[3], const uint32_t dims[3], uint32_t* col,
-                         uint32_t* nrm, uint32_t* tex) {
+                         const uint32_t slots[3], const uint32_t dims[3], uint32_t wide,
+                         uint32_t* col, uint32_t* nrm, uint32_t* tex) {
     if (maxReq == 0) return;
     const unsigned grid = maxReq < 4096u ? maxReq : 4096u;
     k_brick_upload<<<grid, kPoolBlock, 0, st>>>(src, req, nreq, e[0], e[1], e[2], slots[0],
-                                                slots[1], dims[0], dims[1], col, nrm, tex);
+                                                slots[1], dims[0], dims[1], wide, col, nrm, tex);
 }
 
 }  // namespace voxmi
@@ -225,6 +237,7 @@ struct vox_brick_pool {
     uint32_t B[3] = {0, 0, 0}, slots[3] = {0, 0, 0}, dims[3] = {0, 0, 0};
     uint32_t nslots = 0, resident = 0;
     bool valid = false;
+    bool wide = false;   // vox_brick_pool_create_wide
     hipStream_t st = nullptr;
     DevBuf col, nrm;
     std::stringstream err;
@@ -338,6 +351,8 @@ bool build_tree(vox_octree* t, const vox_result* r) {
         t->src.col[d] = r->brick_colors[d];
         t->src.nrm[d] = r->brick_normals[d];
         t->bricks[d] = r->bricks[d];
+        k_missing_bricks<<<grid_for(n), kPoolBlock, 0, st>>>(n, r->bricks[d], t->types[d].as<uint8_t>(),
+                                                            t->bidx[d].as<uint32_t>());
     }
     // tree membership, top-down
     for (int d = 0; d < t->L; ++d) {
@@ -411,8 +426,8 @@ bool build_tree(vox_octree* t, const vox_result* r) {
             return false;
         const uint32_t e[3] = {bp->B[0] + 2, bp->B[1] + 2, bp->B[2] + 2};
         launch_brick_upload(st, t->src, t->req.as<uint4>(), t->nreq.as<uint32_t>(), 1, e,
-                            bp->slots, bp->dims, bp->col.as<uint32_t>(), bp->nrm.as<uint32_t>(),
-                            t->tex.as<uint32_t>());
+                            bp->slots, bp->dims, bp->wide ? 1u : 0u, bp->col.as<uint32_t>(),
+                            bp->nrm.as<uint32_t>(), t->tex.as<uint32_t>());
         ++bp->resident;
     }
     return t->check(hipGetLastError(), "brick upload") &&
@@ -432,6 +447,7 @@ bool octree_refs(vox_octree* t, OctreeRefs& r) {
         r.poolDims[a] = t->bp->dims[a];
         r.poolSlots[a] = t->bp->slots[a];
     }
+    r.poolWide = t->bp->wide ? 1u : 0u;
     r.tex = t->tex.as<uint32_t>();
     r.levels = t->L;
     r.poolCol = t->bp->col.as<uint32_t>();
@@ -456,6 +472,7 @@ bool pool_refs(vox_brick_pool* bp, PoolRefs& r) {
         r.dims[a] = bp->dims[a];
         r.slots[a] = bp->slots[a];
     }
+    r.wide = bp->wide ? 1u : 0u;
     r.col = bp->col.as<uint32_t>();
     r.nrm = bp->nrm.as<uint32_t>();
     r.st = bp->st;
@@ -507,6 +524,51 @@ vox_brick_pool* vox_brick_pool_create(const uint32_t brick_dim[3], const uint32_
     return bp;
 }
 
+vox_brick_pool* vox_brick_pool_create_wide(const uint32_t brick_dim[3], uint64_t nslots) {
+    vox_brick_pool* bp = new (std::nothrow) vox_brick_pool;
+    if (!bp) return nullptr;
+    if (!brick_dim) {
+        bp->fail("brick pool: brick_dim is NULL");
+        return bp;
+    }
+    if (nslots == 0 || nslots > (1ull << 30)) {
+        bp->fail("wide brick pool: between 1 and 2^30 slots");
+        return bp;
+    }
+    uint64_t e3 = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (brick_dim[a] == 0) {
+            bp->fail("brick pool: zero brick dimension");
+            return bp;
+        }
+        bp->B[a] = brick_dim[a];
+        e3 *= brick_dim[a] + 2;
+    }
+    // slots stacked along z: slot s holds texels [s * e^3, (s + 1) * e^3)
+    if ((uint64_t)(brick_dim[2] + 2) * nslots >= (1ull << 32)) {
+        bp->fail("wide brick pool: slots * (B+2) above 2^32 texels along z");
+        return bp;
+    }
+    bp->wide = true;
+    bp->slots[0] = bp->slots[1] = 1;
+    bp->slots[2] = (uint32_t)nslots;
+    bp->dims[0] = brick_dim[0] + 2;
+    bp->dims[1] = brick_dim[1] + 2;
+    bp->dims[2] = (brick_dim[2] + 2) * (uint32_t)nslots;
+    bp->nslots = (uint32_t)nslots;
+    const uint64_t total = e3 * nslots;
+    if (!bp->check(hipStreamCreateWithFlags(&bp->st, hipStreamNonBlocking), "hipStreamCreate"))
+        return bp;
+    if (!bp->col.ensure(4 * total, bp->err, "brick pool colors") ||
+        !bp->nrm.ensure(4 * total, bp->err, "brick pool normals"))
+        return bp;
+    launch_fill_u32(bp->st, bp->col.as<uint32_t>(), total, 0u);
+    launch_fill_u32(bp->st, bp->nrm.as<uint32_t>(), total, 0u);
+    if (!bp->check(hipStreamSynchronize(bp->st), "synchronize")) return bp;
+    bp->valid = true;
+    return bp;
+}
+
 void vox_brick_pool_destroy(vox_brick_pool* bp) {
     if (!bp) return;
     if (bp->st) (void)hipStreamSynchronize(bp->st);
@@ -532,6 +594,7 @@ int vox_brick_pool_get_view(vox_brick_pool* bp, vox_brick_pool_view* out) {
     out->colors = bp->col.as<uint32_t>();
     out->normals = bp->nrm.as<uint32_t>();
     out->resident = bp->resident;
+    out->wide = bp->wide ? 1u : 0u;
     return 1;
 }
 

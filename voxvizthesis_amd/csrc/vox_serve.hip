@@ -46,6 +46,10 @@ struct RayUniforms {
     int32_t depthMinusOne, maxPush, W, H, maxTraversals;
     int32_t rx0, ry0, rx1, ry1;  // pixels cast: [rx0, rx1) x [ry0, ry1)
     uint32_t ndim[3], pdim[3];
+    // wide brick pool: word 1 of a resident node is its slot, and each brick
+    // is sampled in its own (B+2)^3 texel frame (poolDim / pdim are then the
+    // brick's texels per axis)
+    uint32_t wide, wideSlots;
 };
 
 struct v3 {
@@ -265,9 +269,19 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
                             }
                         }
                     } else {
-                        const v3 bp = mk((float)((curBrickPtr.y >> 20) & 1023u) / U.poolDim[0],
-                                         (float)((curBrickPtr.y >> 10) & 1023u) / U.poolDim[1],
-                                         (float)(curBrickPtr.y & 1023u) / U.poolDim[2]);
+                        // the brick's first interior texel (slot origin + 1), in the
+                        // pool's normalised coordinates or, in a wide pool, the brick's
+                        const v3 bp = U.wide
+                            ? mk(1.0f / U.poolDim[0], 1.0f / U.poolDim[1], 1.0f / U.poolDim[2])
+                            : mk((float)((curBrickPtr.y >> 20) & 1023u) / U.poolDim[0],
+                                 (float)((curBrickPtr.y >> 10) & 1023u) / U.poolDim[1],
+                                 (float)(curBrickPtr.y & 1023u) / U.poolDim[2]);
+                        // (a path with no resident brick keeps the root's word 1,
+                        // which need not be a slot: slot 0 is read then, where the
+                        // reference reads whatever texel those bits name)
+                        const uint32_t slot = curBrickPtr.y < U.wideSlots ? curBrickPtr.y : 0u;
+                        const uint32_t* bpool =
+                            U.wide ? pool + (uint64_t)slot * U.pdim[0] * U.pdim[1] * U.pdim[2] : pool;
                         v3 cbr = add(bp, divv(mul(curBrickRayPos, ld3(U.brickDim)),
                                               ld3(U.poolDim)));
                         const float ratio = ldexpf(1.0f, U.depthMinusOne - curBrickDepth);
@@ -283,7 +297,7 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
                                 break;
                             }
                             float src[4];
-                            sample_brick(pool, U, u8, cbr, src);
+                            sample_brick(bpool, U, u8, cbr, src);
                             src[3] = 1.0f - (1.0f - src[3]);  // pow(1 - a, 1.0)
                             blend(dst, src);
                             cbr = add(cbr, brickRayStep);
@@ -483,8 +497,8 @@ namespace {
 // Uniforms of one frame (GigaVoxelsRenderer.cpp 1650-1816), in float like
 // the renderer's QVector3D arithmetic.
 bool ray_uniforms(const vox_camera& cam, const vox_raycast_params& prm, int levels,
-                  const uint32_t B[3], const uint32_t poolDims[3], const uint32_t ndim[3],
-                  RayUniforms& U) {
+                  const uint32_t B[3], const uint32_t poolDims[3], uint32_t poolWide,
+                  const uint32_t ndim[3], RayUniforms& U) {
     std::memset(&U, 0, sizeof(U));
     float T[3], S[3];
     for (int a = 0; a < 3; ++a) {
@@ -519,12 +533,16 @@ bool ray_uniforms(const vox_camera& cam, const vox_raycast_params& prm, int leve
         const float rayStepRoot = (S[a] * U.rayStep) * pow2;
         const float rootVoxel = S[a] / (float)B[a];
         const float ratio = rayStepRoot / rootVoxel;
-        U.brickStep[a] = (1.0f / (float)poolDims[a]) * ratio;
+        // a wide pool is sampled one brick at a time: its texels are the frame
+        const uint32_t pd = poolWide ? B[a] + 2u : poolDims[a];
+        U.brickStep[a] = (1.0f / (float)pd) * ratio;
         U.brickDim[a] = (float)B[a];
-        U.poolDim[a] = (float)poolDims[a];
-        U.pdim[a] = poolDims[a];
+        U.poolDim[a] = (float)pd;
+        U.pdim[a] = pd;
         U.ndim[a] = ndim[a];
     }
+    U.wide = poolWide;
+    U.wideSlots = poolWide ? poolDims[2] / (B[2] + 2u) : 0u;
     const float bx = (float)B[0], by = (float)B[1], bz = (float)B[2];
     const float minB = bx < ((by < bz) ? by : bz) ? bx : bz;  // renderer's ternary
     U.rootVoxelHalf = (1.0f / minB) * 0.5f;
@@ -664,7 +682,7 @@ int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycas
     uint32_t* tex = o.tex;
     hipStream_t st = rc->st;
     RayUniforms U;
-    if (!ray_uniforms(*cam, *prm, o.levels, o.B, o.poolDims, o.ndim, U))
+    if (!ray_uniforms(*cam, *prm, o.levels, o.B, o.poolDims, o.poolWide, o.ndim, U))
         return rc->fail("raycast: empty volume box, ray step below 1e-5 or a pixel rectangle "
                         "outside the viewport"), 0;
     settle_timing(rc);  // the previous frame's events are reused below

@@ -78,7 +78,7 @@ __global__ void k_srv_info(uint64_t n, uint32_t d, const uint32_t* __restrict__ 
 // the server already gave to another node is a conflict.
 __global__ void k_srv_adopt(uint64_t ntex, const uint32_t* __restrict__ tex, uint32_t ex,
                             uint32_t ey, uint32_t ez, uint32_t sx, uint32_t sy, uint32_t nslots,
-                            uint32_t base, uint32_t* __restrict__ nodeSlot,
+                            uint32_t wide, uint32_t base, uint32_t* __restrict__ nodeSlot,
                             uint32_t* __restrict__ slotNode, uint32_t* __restrict__ conflict) {
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ntex;
          t += (uint64_t)gridDim.x * blockDim.x) {
@@ -87,7 +87,8 @@ __global__ void k_srv_adopt(uint64_t ntex, const uint32_t* __restrict__ tex, uin
         const uint32_t x = (((w1 >> 20) & 1023u) - 1u) / ex;
         const uint32_t y = (((w1 >> 10) & 1023u) - 1u) / ey;
         const uint32_t z = ((w1 & 1023u) - 1u) / ez;
-        const uint32_t s = (z * sy + y) * sx + x;
+        // a wide pool's word 1 is the slot itself
+        const uint32_t s = wide ? w1 : (z * sy + y) * sx + x;
         if (s >= nslots) continue;
         if (atomicCAS(&slotNode[s], kNone, base + (uint32_t)t) != kNone) {
             atomicOr(conflict, 1u);
@@ -350,8 +351,8 @@ __global__ void k_srv_advance(const uint32_t* __restrict__ sel, uint32_t* __rest
 __global__ void k_srv_upload(const TreeTab* __restrict__ tabs, const uint4* __restrict__ req,
                              const uint32_t* __restrict__ nreq, uint32_t ex, uint32_t ey,
                              uint32_t ez, uint32_t sx, uint32_t sy, uint32_t dx, uint32_t dy,
-                             uint32_t* __restrict__ col, uint32_t* __restrict__ nrm,
-                             unsigned long long* __restrict__ done) {
+                             uint32_t wide, uint32_t* __restrict__ col,
+                             uint32_t* __restrict__ nrm, unsigned long long* __restrict__ done) {
     const uint32_t count = *nreq;
     const uint32_t nv = ex * ey * ez;
     for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
@@ -374,7 +375,7 @@ __global__ void k_srv_upload(const TreeTab* __restrict__ tabs, const uint4* __re
         }
         if (threadIdx.x == 0) {
             T->tex[2 * (uint64_t)q.w] |= kResidentBit;
-            T->tex[2 * (uint64_t)q.w + 1] = pack10(ox + 1, oy + 1, oz + 1);
+            T->tex[2 * (uint64_t)q.w + 1] = wide ? q.z : pack10(ox + 1, oy + 1, oz + 1);
         }
         __syncthreads();  // the whole brick is written
         if (threadIdx.x == 0) done[r] = (unsigned long long)wall_clock64();
@@ -541,7 +542,8 @@ bool attach(vox_server* s, vox_octree* tree, uint32_t* id) {
     }
     k_srv_adopt<<<grid_for(ntex), kB, 0, st>>>(
         ntex, o.tex, o.B[0] + 2, o.B[1] + 2, o.B[2] + 2, o.poolSlots[0], o.poolSlots[1], s->nslots,
-        base, s->nodeSlot.as<uint32_t>(), s->slotNode.as<uint32_t>(), s->conflict.as<uint32_t>());
+        o.poolWide, base, s->nodeSlot.as<uint32_t>(), s->slotNode.as<uint32_t>(),
+        s->conflict.as<uint32_t>());
     k_srv_seed_root<<<1, 64, 0, st>>>(s->tabs.as<TreeTab>(), (uint32_t)i, base,
                                       s->nodeSlot.as<uint32_t>(), s->pend.as<uint32_t>(),
                                       s->pendEmit.as<unsigned long long>());
@@ -788,7 +790,7 @@ int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n)
     const unsigned ugrid = s->maxUp < 4096u ? s->maxUp : 4096u;
     k_srv_upload<<<ugrid, kB, 0, st>>>(tabs, s->req.as<uint4>(), s->nreq.as<uint32_t>(),
                                        P.B[0] + 2, P.B[1] + 2, P.B[2] + 2, P.slots[0], P.slots[1],
-                                       P.dims[0], P.dims[1], P.col, P.nrm,
+                                       P.dims[0], P.dims[1], P.wide, P.col, P.nrm,
                                        s->done.as<unsigned long long>());
     k_srv_latency<<<grid_for(s->maxUp), kB, 0, st>>>(
         s->nreq.as<uint32_t>(), s->reqEmit.as<unsigned long long>(),

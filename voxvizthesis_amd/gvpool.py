@@ -22,11 +22,17 @@ class BrickPool:
     for bricks wider than 14 voxels, else 16^3 (GigaVoxelsRenderer.cpp 259-272).
     """
 
-    def __init__(self, brickDim, slots=None):
+    def __init__(self, brickDim, slots=None, wide_slots=None):
+        """wide_slots: a wide pool of that many slots instead
+        (vox_brick_pool_create_wide: beyond the 10-bit texel pointers, word 1
+        of a resident node holds its slot)."""
         self._L = _lib.load()
         u3 = C.c_uint32 * 3
-        self._h = self._L.vox_brick_pool_create(u3(*brickDim),
-                                                u3(*slots) if slots is not None else None)
+        if wide_slots is not None:
+            self._h = self._L.vox_brick_pool_create_wide(u3(*brickDim), int(wide_slots))
+        else:
+            self._h = self._L.vox_brick_pool_create(u3(*brickDim),
+                                                    u3(*slots) if slots is not None else None)
         if not self._h:
             raise MemoryError("vox_brick_pool_create")
         err = self.getErrorMessage()

@@ -227,7 +227,13 @@ class TilePager:
         blocks = 1 + sum(8 ** d for d in range(L - 1))
         side = int(round((8 * blocks) ** (1.0 / 3.0))) + 2
         self.cap = 8 * blocks + 2 * side * side + 64
-        self.pool = BrickPool((self.B,) * 3, pool_slots)
+        # pool_slots: slots per axis of the reference's pool texture, or one
+        # number for a wide pool of that many slots (beyond 8.2 GB of 18^3
+        # bricks, which 10-bit texel pointers cap)
+        if isinstance(pool_slots, (int, np.integer)):
+            self.pool = BrickPool((self.B,) * 3, wide_slots=int(pool_slots))
+        else:
+            self.pool = BrickPool((self.B,) * 3, pool_slots)
         self.server = BrickServer.shared(self.pool, max_trees, self.cap, max_uploads)
         self.max_trees = int(max_trees)
 

@@ -155,12 +155,13 @@ class Voxelizer:
 
     def level(self, k: int):
         """Dense copy of mip level k: (colors (Z,Y,X) uint32, normals (Z,Y,X,3))."""
-        r = self.raw_result()
-        dims = [int(s) >> k for s in r.S]
+        d = (C.c_uint32 * 3)()
+        if not self._L.vox_copy_level(self._h, k, None, None, d):   # extent only
+            raise RuntimeError(self.getErrorMessage())
+        dims = [int(x) for x in d]
         n = dims[0] * dims[1] * dims[2]
         col = np.empty(n, np.uint32)
         nrm = np.empty(3 * n, np.float32)
-        d = (C.c_uint32 * 3)()
         if not self._L.vox_copy_level(self._h, k, col.ctypes.data, nrm.ctypes.data, d):
             raise RuntimeError(self.getErrorMessage())
         return (col.reshape(dims[2], dims[1], dims[0]),
