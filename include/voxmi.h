@@ -266,6 +266,17 @@ int vox_device_download(void* dst, const void* src, uint64_t offset, uint64_t by
  * calling thread).  Not in the reference, whose triangle batches are fixed
  * 512-triangle launches (Voxelizer.cu 2359-2400) with no prefix sum. */
 int vox_scan_u32(uint32_t* in, uint32_t* out, uint64_t n, int32_t clear_input);
+
+/* vox_scan_u32 without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as vox_scan_u32, for n up to 2^32
+ * values (larger n is rejected with an error before any device work).  The
+ * look-back scratch is a fixed 4 MiB device array inside the library, one
+ * per GPU, held until the process ends; each call clears the part it uses.
+ * Calls on one GPU share that array, so calls that overlap in time on one
+ * GPU (from several host threads) race: serialise them. */
+int vox_scan_u32_noalloc(uint32_t* in, uint32_t* out, uint64_t n, int32_t clear_input);
 const char* vox_scan_get_error(void);
 
 /* Write tree.gvx and voxels_<depth>.gvb of the last vox_generate into

@@ -100,3 +100,10 @@ def test_cpp_mirror_header_compiles_and_reports(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "extra 8 8 8" in r.stdout
 
+
+def test_noalloc_scan_rejects_sizes_beyond_its_scratch_without_gpu():
+    # the size check comes before any device work, so it runs on the CPU;
+    # the pointers are aligned placeholders that are never dereferenced
+    lib = _lib.load()
+    assert lib.vox_scan_u32_noalloc(256, 512, (1 << 32) + 1, 0) == 0
+    assert b"2^32" in lib.vox_scan_get_error()

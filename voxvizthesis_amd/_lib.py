@@ -28,7 +28,7 @@ EXPORTS = [
     "vox_copy_level", "vox_write_files", "vox_get_compressed", "vox_set_launch",
     "vox_generate_async", "vox_wait", "vox_release_work_memory", "vox_add_group_gather",
     "vox_device_alloc", "vox_device_upload", "vox_device_free", "vox_device_copy",
-    "vox_device_download", "vox_scan_u32", "vox_scan_get_error",
+    "vox_device_download", "vox_scan_u32", "vox_scan_u32_noalloc", "vox_scan_get_error",
 ]
 # Every symbol include/voxmi_batch.h declares.
 BATCH_EXPORTS = [
@@ -242,6 +242,7 @@ def load():
     L.vox_device_copy.argtypes = [vp, vp, C.c_uint64]
     L.vox_device_download.argtypes = [vp, vp, C.c_uint64, C.c_uint64]
     L.vox_scan_u32.argtypes = [vp, vp, C.c_uint64, C.c_int32]
+    L.vox_scan_u32_noalloc.argtypes = [vp, vp, C.c_uint64, C.c_int32]
     L.vox_scan_get_error.restype = C.c_char_p
     L.vox_scan_get_error.argtypes = []
     # batch
