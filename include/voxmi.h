@@ -111,6 +111,14 @@ typedef struct vox_stats {
     char     kernel_name[64][32];
     double   kernel_ms[64];           /* summed per kernel name                */
     uint32_t kernel_launches[64];
+    /* With VOX_OPT_TIMING, one voxel count per mip level k >= 1: the stored
+     * source voxels its filter reads (active source tiles under its active
+     * tiles; on level 0 only the voxels some group hit or a fill reached).
+     * The algorithmic bytes of the filter are 16 per such voxel plus 20 per
+     * voxel of its active tiles.  Zero with several chunks. */
+    uint64_t mip_source_voxels[VOX_MAX_LEVELS];
+    uint64_t bins;                    /* (group, tile) bins of the setup pass  */
+    uint64_t scan_values;             /* values the generate's prefix sums read */
 } vox_stats;
 
 /* cuda::Voxelizer::Voxelizer(voxDim, brickDim, voxChunkDim)

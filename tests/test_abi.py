@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols(header=None):
     names = [header] if header else ["voxmi.h", "voxmi_pool.h", "voxmi_serve.h", "voxmi_batch.h",
-                                        "voxmi_tiler.h"]
+                                        "voxmi_tiler.h", "voxmi_terrain.h"]
     out = set()
     for h in names:
         txt = open(os.path.join(ROOT, "include", h)).read()
@@ -29,6 +29,7 @@ def test_header_matches_binding_list():
     assert declared_symbols("voxmi_serve.h") == sorted(_lib.SERVE_EXPORTS)
     assert declared_symbols("voxmi_batch.h") == sorted(_lib.BATCH_EXPORTS)
     assert declared_symbols("voxmi_tiler.h") == sorted(_lib.TILER_EXPORTS)
+    assert declared_symbols("voxmi_terrain.h") == sorted(_lib.TERRAIN_EXPORTS)
 
 
 def test_library_exports_every_declared_symbol():

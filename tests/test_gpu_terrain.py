@@ -1,0 +1,139 @@
+"""Procedural terrain on the device and the served paging built on it (GPU).
+
+The device terrain equals the host scene generator bit for bit: a whole
+64^2-cell grid as triangles, vertex normals and UVs equals
+scenes.heightfield_terrain, and the heights with the ridged term equal the
+float64 host restatement rounded to float32.  The terrain pager (tiles built
+on the device and voxelized on demand, served from a wide brick pool over two
+camera streams) gives every loaded tile the octree the same triangles give
+alone, and every pool slot holds the brick of the node it serves."""
+import numpy as np
+import pytest
+
+from oracle.server import NONE
+from voxvizthesis_amd import scenes
+from voxvizthesis_amd.terrain import DeviceTerrain, TerrainPager, c5_height_fn
+from voxvizthesis_amd.voxelizer import Voxelizer
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_tile_equals_host_generator():
+    print("[terrain] 64^2 cells vs heightfield_terrain", flush=True)
+    t = DeviceTerrain(64, 1.0, 11)
+    try:
+        got = t.mesh(0, 0, 64, 64)
+        ref = scenes.heightfield_terrain(64, 11)
+        assert np.array_equal(got.verts, ref.verts)
+        assert np.array_equal(got.vnormals, ref.vnormals)
+        assert np.array_equal(got.uvs, ref.uvs)
+        # a window equals the same cells of the whole grid
+        w = t.mesh(10, 20, 16, 8)
+        full = ref.verts.reshape(64, 64, 2, 3, 3)[20:28, 10:26].reshape(-1, 3, 3)
+        assert np.array_equal(w.verts, full)
+    finally:
+        t.close()
+
+
+def test_device_heights_with_ridged_term():
+    print("[terrain] heights with the ridged term", flush=True)
+    E = 4096.0
+    t = DeviceTerrain(128, E, 6, 7, 0.05)
+    try:
+        g = np.arange(129, dtype=np.float64) * (E / 128)
+        X, Y = np.meshgrid(g, g, indexing="xy")
+        want = c5_height_fn(6, 7, 0.05, E)(X, Y).astype(np.float32)
+        assert np.array_equal(t.grid(), want)
+        lo, hi = t.range(0, 0, 129, 129)
+        assert lo == want.min() and hi == want.max()
+        z = t.sample(np.array([[0.0, 0.0], [E, E], [1000.0, 2000.0]]))
+        assert z[0] == want[0, 0] and z[1] == want[128, 128]
+        assert z[2] == want[int(np.floor(2000 / 32 + 0.5)), int(np.floor(1000 / 32 + 0.5))]
+    finally:
+        t.close()
+
+
+def snapshot(vz):
+    r = vz.result()
+    out = [r["occupancy"]]
+    for tr in r["tree"]:
+        out += [tr["types"], tr["const"], tr["bidx"], tr["bcol"], tr["bnrm"]]
+    return out
+
+
+def alone(pg, tr):
+    """The tile voxelized alone from host copies of its groups."""
+    x0, y0, cw, ch = pg._win[tr.tile]
+    bufs, _, lo, hi, rocks = pg.tile_groups(tr)
+    for b in bufs:
+        b.close()
+    terr = pg.terrain.mesh(x0, y0, cw, ch)
+    vz = Voxelizer((pg.N,) * 3, (pg.B,) * 3)
+    try:
+        assert vz.initCuda() and vz.allocateVoxelMemory()
+        assert vz.setVoxelizationParams(tr.origin, pg.vs)
+        assert vz.addTriangleGroup(terr.verts, terr.vnormals, terr.uvs, lo, hi, True,
+                                   terr.image)
+        if rocks is not None:
+            assert vz.addMesh(rocks)
+        assert vz.allocateTriangleMemory() and vz.computeEdgesFaceNormalsAndBounds()
+        st = vz.generateVoxelsAndOctTree(None)
+        return st, (snapshot(vz) if st == 1 else None)
+    finally:
+        vz.close()
+
+
+def test_terrain_pager_tiles_equal_alone_and_pool_is_consistent():
+    W, H = 96, 64
+    pg = TerrainPager(256, 64, 8, 512, seed=6, ridged_seed=7, ridged_amp=0.05, rocks=300,
+                      rock_radius=(1.0, 6.0), max_trees=12, max_loading=3, streams=2,
+                      max_width=W, max_height=H, near=1.0, far=2000.0, serve_group=2)
+    try:
+        print(f"[terrain] {len(pg.trees)} paged trees", flush=True)
+        paths = [scenes.flythrough(seed=s, frames=8, extent=256.0, alt=(0.08, 0.2),
+                                   height_fn=pg.height_fn) for s in (5, 9)]
+        for f in range(8):
+            cams = [dict(position=p[f][0], look=p[f][1], up=p[f][2], fov_y_deg=60.0, width=W,
+                         height=H) for p in paths]
+            if f in (2, 5):
+                pg.drain()
+            pg.frame(cams)
+        pg.drain()
+        pg.frame([dict(position=p[-1][0], look=p[-1][1], up=p[-1][2], fov_y_deg=60.0,
+                       width=W, height=H) for p in paths])
+        loaded = [t for t in pg.trees if t.state == "loaded"]
+        print(f"[terrain] loaded {len(loaded)} trees, load ms {np.round(pg.load_ms, 1)}",
+              flush=True)
+        assert len(loaded) >= 2
+        for tr in loaded:
+            st, ref = alone(pg, tr)
+            assert st == 1
+            got = snapshot(tr.vz)
+            assert len(got) == len(ref) and all((a == b).all() for a, b in zip(got, ref)), \
+                f"tile {tr.x},{tr.y},{tr.z}"
+        # every slot of the wide pool holds the brick of the node it serves
+        node_slot, slot_node = pg.server.state()
+        col = pg.pool.textures()[0].reshape(-1)
+        e3 = 10 ** 3
+        by_slot = {pg.server_slot(t): t for t in loaded}
+        n_res = 0
+        for s, gidx in enumerate(slot_node):
+            if gidx == NONE:
+                continue
+            k, x = divmod(int(gidx), pg.cap)
+            tr = by_slot[k]
+            res = tr.vz.result()
+            info = {}
+            for d in range(res["levels"]):
+                tx = tr.octree.node_texel(d)
+                for nn in np.nonzero(tx != NONE)[0]:
+                    info[int(tx[nn])] = (d, int(nn))
+            d, nn = info[x]
+            b = int(res["tree"][d]["bidx"][nn])
+            assert (col[s * e3:(s + 1) * e3] == res["tree"][d]["bcol"][b]).all()
+            n_res += 1
+        assert n_res > 0
+        st = pg.server.stats()
+        print(f"[terrain] uploads {st.uploads} resident {n_res}", flush=True)
+    finally:
+        pg.close()

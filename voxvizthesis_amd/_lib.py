@@ -50,6 +50,11 @@ SERVE_EXPORTS = [
     "vox_server_create_shared", "vox_server_attach", "vox_server_detach",
     "vox_server_frame_multi", "vox_composite",
 ]
+# Every symbol include/voxmi_terrain.h declares.
+TERRAIN_EXPORTS = [
+    "vox_terrain_heights", "vox_terrain_range", "vox_terrain_tile", "vox_terrain_sample",
+    "vox_terrain_get_error",
+]
 # Every symbol include/voxmi_tiler.h declares.
 TILER_EXPORTS = [
     "vox_tiler_create", "vox_tiler_destroy", "vox_tiler_get_error", "vox_tiler_assign",
@@ -116,6 +121,8 @@ class VoxStats(C.Structure):
         ("bricks_total", C.c_uint64), ("ms_total", C.c_double), ("nkernels", C.c_int32),
         ("kernel_name", (C.c_char * 32) * 64), ("kernel_ms", C.c_double * 64),
         ("kernel_launches", C.c_uint32 * 64),
+        ("mip_source_voxels", C.c_uint64 * VOX_MAX_LEVELS), ("bins", C.c_uint64),
+        ("scan_values", C.c_uint64),
     ]
 
 
@@ -123,7 +130,7 @@ class VoxBrickPoolView(C.Structure):
     _fields_ = [
         ("brick_dim", C.c_uint32 * 3), ("slots", C.c_uint32 * 3), ("dims", C.c_uint32 * 3),
         ("colors", C.c_void_p), ("normals", C.c_void_p), ("resident", C.c_uint32),
-                ("wide", C.c_uint32),
+        ("wide", C.c_uint32),
     ]
 
 
@@ -182,6 +189,17 @@ class DeviceArray:
             if not self.L.vox_device_upload(self.ptr, off, piece.ctypes.data, piece.nbytes):
                 self.close()
                 raise RuntimeError("vox_device_upload failed")
+
+    @classmethod
+    def alloc(cls, nbytes: int) -> "DeviceArray":
+        """Uninitialised device memory of nbytes (for a kernel to fill)."""
+        self = cls.__new__(cls)
+        self.L = load()
+        self.nbytes = int(nbytes)
+        self.ptr = self.L.vox_device_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise MemoryError(f"vox_device_alloc({self.nbytes})")
+        return self
 
     def close(self):
         if getattr(self, "ptr", None):
@@ -255,6 +273,17 @@ def load():
     L.vox_batch_capture.argtypes = [vp]
     L.vox_batch_launch.argtypes = [vp]
     L.vox_batch_get_stats.argtypes = [vp, C.POINTER(VoxBatchStats)]
+    # terrain
+    L.vox_terrain_heights.argtypes = [vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64,
+                                      C.c_double]
+    L.vox_terrain_range.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, vp]
+    L.vox_terrain_tile.argtypes = [vp, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_uint32,
+                                   vp, vp, vp]
+    L.vox_terrain_sample.argtypes = [vp, C.c_uint32, C.c_double, vp, C.c_uint32, vp]
+    L.vox_terrain_get_error.restype = C.c_char_p
+    L.vox_terrain_get_error.argtypes = []
     # tiler
     L.vox_tiler_create.restype = vp
     L.vox_tiler_create.argtypes = []

@@ -217,6 +217,7 @@ constexpr uint32_t kScanErrWord = 128;
 
 bool scan(vox_ctx* h, const uint32_t* in, uint32_t* out, uint64_t n) {
     if (!ensure_scan_tmp(h->scanTmp, n, h->st, h->err, "scan scratch")) return false;
+    h->stats.scan_values += n;
     launch_scan(h->st, in, out, n, h->scanTmp.as<uint32_t>(),
                 h->counters.as<uint32_t>() + kScanErrWord);
     return true;
@@ -910,6 +911,20 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
             launch_mip(st, s.map, l.map, l.active, l.list, s.col, s.nrm, l.col, l.qn, l.nrm);
         });
     }
+    // Measurement: the stored source voxels each filter read (timing runs,
+    // one chunk).
+    const bool countMip = (h->options & VOX_OPT_TIMING) && !h->replay && !h->multi;
+    unsigned long long* mipCount = reinterpret_cast<unsigned long long*>(ctr + 256);
+    if (countMip) {
+        const FillSeg z[1] = {{ctr + 256, 2ull * VOX_MAX_LEVELS, 0u}};
+        launch_fill_segs(st, z, 1);
+        for (int k = 1; k < L; ++k) {
+            if (h->lv[k].aliasOf >= 0) continue;
+            LevelView s = view(h, srcOf(k));
+            LevelView l = view(h, k);
+            launch_mip_source_count(st, s.map, l.map, l.active, l.list, mipCount + k);
+        }
+    }
     // Node typing per depth (computeOctTreeNodes, Voxelizer.cu 931-1290).
     uint32_t border[3] = {h->extra[0], h->extra[1], h->extra[2]};
     std::vector<std::array<uint32_t, 3>> borders(L);
@@ -1137,6 +1152,13 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     h->stats.triangles = h->T;
     h->stats.pairs = h->pairs;
     h->stats.sat_tests = sat;
+    h->stats.bins = h->numBins;
+    h->stats.scan_values += h->numBins;   // the setup pass's bin scan
+    if (countMip &&
+        !h->check(hipMemcpy(h->stats.mip_source_voxels, mipCount, 8 * VOX_MAX_LEVELS,
+                            hipMemcpyDeviceToHost),
+                  "mip counts"))
+        return 0;
     if (h->options & VOX_OPT_TIMING) collect_timings(h);
     h->haveResult = true;
     h->prepared = false;  // bins are consumed; a new generate re-runs setup

@@ -203,6 +203,10 @@ void launch_colour(hipStream_t st, const VoxParams& P, const GroupArgs& G, int i
 void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t nActive,
                  const int32_t* list, const int32_t* tileIdx, const uint32_t* binStart,
                  uint32_t* col, uint32_t* qn, float* nrm, const uint32_t* hit, uint32_t* fillw);
+// Stored source voxels the mip filter of dst reads (active source tiles
+// under its active tiles; level 0: hit or filled voxels), added to *out.
+void launch_mip_source_count(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                             uint64_t nActive, const int32_t* listDst, unsigned long long* out);
 void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
                 const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
                 uint32_t* colDst, uint32_t* qnDst, float* nrmDst);

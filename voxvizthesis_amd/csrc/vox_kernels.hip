@@ -1416,6 +1416,33 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
     }
 }
 
+// Measurement only (timing runs): the stored source voxels k_mip reads.
+__global__ void k_mip_source_count(LevelMap src, LevelMap dst, uint64_t nActive,
+                                   const int32_t* __restrict__ listDst,
+                                   unsigned long long* __restrict__ out) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long n = 0;
+    if (b < nActive) {
+        uint32_t tx, ty, tz;
+        tile_xyz((uint64_t)listDst[b], dst.TG, tx, ty, tz);
+        for (uint32_t c = 0; c < 8u; ++c) {
+            const uint32_t sx = 2 * tx + (c & 1u), sy = 2 * ty + ((c >> 1) & 1u), sz = 2 * tz + (c >> 2);
+            if (sx >= src.TG[0] || sy >= src.TG[1] || sz >= src.TG[2]) continue;
+            const int32_t ca = src.tileIdx[((uint64_t)sz * src.TG[1] + sy) * src.TG[0] + sx];
+            if (ca < 0) continue;
+            if (!src.defOcc) {
+                n += kTileVox;
+                continue;
+            }
+            for (int w = 0; w < kTileWords; ++w) {
+                const uint64_t o = (uint64_t)ca * kTileWords + w;
+                n += __popc(src.defOcc[o] | src.defFill[o]);
+            }
+        }
+    }
+    if (n) atomicAdd(out, n);
+}
+
 __device__ __forceinline__ int64_t tile_of(const LevelMap& lm, uint32_t X, uint32_t Y, uint32_t Z) {
     const uint64_t t = ((uint64_t)(Z >> 3) * lm.TG[1] + (Y >> 3)) * lm.TG[0] + (X >> 3);
     return lm.tileIdx[t];
@@ -2685,6 +2712,13 @@ void launch_scan_clear(hipStream_t st, uint32_t* in, uint32_t* out, uint64_t n, 
     scan_launch(st, in, out, n, tmp, in, err);
 }
 
+
+void launch_mip_source_count(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                             uint64_t nActive, const int32_t* listDst, unsigned long long* out) {
+    if (nActive == 0) return;
+    k_mip_source_count<<<grid_for(nActive, kBlock), kBlock, 0, st>>>(src, dst, nActive, listDst,
+                                                                    out);
+}
 
 void launch_chunk_mark(hipStream_t st, const LevelMap& src, const LevelMap& dst,
                        const ChunkGeom& cg, int ks, int kd, uint32_t* markDst) {

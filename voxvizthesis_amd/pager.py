@@ -221,15 +221,18 @@ class TilePager:
                     lo = np.asarray(org, np.float32)
                     hi = (lo + size).astype(np.float32)
                     self.trees.append(PagedTree(x, y, z, t, org, lo, hi, c, r))
-        # shared pool and server; a tree's node texture holds at most
-        # 8 * (1 + internal nodes) texels, plus the texture's rounding
+        self._setup_serving(pool_slots, max_trees, max_uploads)
+
+    def _setup_serving(self, pool_slots, max_trees: int, max_uploads: int):
+        """Shared pool and server.  A tree's node texture holds at most
+        8 * (1 + internal nodes) texels, plus the texture's rounding.
+        pool_slots: slots per axis of the reference's pool texture, or one
+        number for a wide pool of that many slots (beyond 8.2 GB of 18^3
+        bricks, which 10-bit texel pointers cap)."""
         L = int(round(math.log2(self.N // self.B))) + 1
         blocks = 1 + sum(8 ** d for d in range(L - 1))
         side = int(round((8 * blocks) ** (1.0 / 3.0))) + 2
         self.cap = 8 * blocks + 2 * side * side + 64
-        # pool_slots: slots per axis of the reference's pool texture, or one
-        # number for a wide pool of that many slots (beyond 8.2 GB of 18^3
-        # bricks, which 10-bit texel pointers cap)
         if isinstance(pool_slots, (int, np.integer)):
             self.pool = BrickPool((self.B,) * 3, wide_slots=int(pool_slots))
         else:

@@ -92,6 +92,22 @@ def fbm(x: np.ndarray, y: np.ndarray, seed: int, octaves: int = 6, lacunarity: f
     return total / norm
 
 
+def ridged(x: np.ndarray, y: np.ndarray, seed: int, octaves: int = 6, lacunarity: float = 2.0,
+           gain: float = 0.5, base_freq: float = 4.0) -> np.ndarray:
+    """Ridged noise: the normalised sum over octaves of amp * (1 - |noise|)^2
+    (the same octaves as fbm, its own permutation)."""
+    perm = permutation(seed)
+    total = np.zeros_like(x, dtype=np.float64)
+    amp, freq, norm = 1.0, base_freq, 0.0
+    for _ in range(octaves):
+        r = 1.0 - np.abs(gradient_noise(x * freq, y * freq, perm))
+        total += amp * (r * r)
+        norm += amp
+        amp *= gain
+        freq *= lacunarity
+    return total / norm
+
+
 RAMP_STOPS = [
     (0.00, (30, 90, 30)),
     (0.30, (60, 140, 50)),
@@ -371,13 +387,14 @@ def scene_terrain_n(cells: int, N: int, B: int, seed: int = 11) -> Scene:
 
 
 def flythrough(seed: int = 5, frames: int = 300, n_points: int = 12, extent: float = 1.0,
-               terrain_seed: int = 2, alt=(0.05, 0.2)):
+               terrain_seed: int = 2, alt=(0.05, 0.2), height_fn=None):
     """Seeded Catmull-Rom camera path over the terrain: n_points control
     points uniform in the inner 80% of the xy extent at altitude alt above the
-    fBm surface; the camera looks along the path's horizontal heading, tilted
-    down by atan(0.1 / 0.25).  Returns a list of (position, look, up)."""
+    fBm surface (or height_fn's); the camera looks along the path's
+    horizontal heading, tilted down by atan(0.1 / 0.25).  Returns a list of
+    (position, look, up)."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    h = terrain_height_fn(terrain_seed, extent)
+    h = height_fn if height_fn is not None else terrain_height_fn(terrain_seed, extent)
     xy = rng.uniform(0.1 * extent, 0.9 * extent, size=(n_points, 2))
     z = h(xy[:, 0], xy[:, 1]) + rng.uniform(alt[0] * extent, alt[1] * extent, size=n_points)
     P = np.concatenate([xy, z[:, None]], axis=1)
