@@ -13,11 +13,17 @@ ending with the bricks, node arrays and occupancy bitmap in HBM.
 value = N^3 voxels classified per second (Gvoxels/s).  The roofline object
 describes k_voxelize, the kernel with the largest share of device time in the
 rocprof trace of this workload, timed with HIP events on the library's
-stream; "rooflines" gives the next kernels (brick gather, mip filter) the same
-way.  Algorithmic bytes per launch are given by kernel_bytes() below.
-cpu_baseline times the oracle (a scalar C++ port of the reference logic) on
-the same scene at 512^3 on one host core, and concurrent copies of it at
-256^3 on every host core the process may use (at most 16).
+stream; "rooflines" gives the other kernels the same way (brick gather, mip
+filter, and the binning passes: setup, scan, scatter).  Algorithmic bytes per
+launch are given by kernel_bytes() below.  cpu_baseline times the oracle (a
+scalar C++ port of the reference logic) on the same workload, tiled into
+256^3 windows at its own voxel size and spread over every host core the
+process may use (at most 16).
+
+Further legs (each skippable): serving the octree to ray-cast camera streams
+(a pool that holds every brick, and one smaller, so the LRU evicts), paged
+serving over an 8192^3 and a 32768^3 virtual grid with tiles built and
+voxelized on demand, and batched voxelization of 256 meshes of uneven sizes.
 """
 from __future__ import annotations
 
@@ -73,22 +79,32 @@ def kernel_bytes(name, scene, st, bricks, B, launches):
                                    + (24 if m.uvs is not None else 0))
         return (tri_bytes + 16 * st.pairs) / launches
     if name == "mip":
-        # a filtered level reads the voxels of the active source tiles under
-        # its active tiles (RGBA8 colour + fp32 normal, 16 B; at most 8 source
-        # tiles per tile, and an inactive source tile is not read) and writes
-        # colour, packed and fp32 normal (20 B) for each voxel of its active
-        # tiles.  Without the mip-chain option level k >= 2 is filtered from
-        # level k - 2 and the even levels >= 2 alias the level below them.
+        # a filtered level reads the stored voxels of the active source tiles
+        # under its active tiles (RGBA8 colour + fp32 normal, 16 B; level 0
+        # keeps only hit or filled voxels, counted on the device in the timing
+        # step) and writes colour, packed and fp32 normal (20 B) for each voxel
+        # of its active tiles.  Without the mip-chain option the even levels
+        # >= 2 alias the level below them and are not filtered.
         a = st.active_tiles
         tot = 0
         for k in range(1, len(bricks)):
             if k >= 2 and k % 2 == 0:
                 continue
-            src = 0 if k == 1 else k - 2
-            tot += min(a[k] * 8, a[src]) * 512 * 16 + a[k] * 512 * 20
+            tot += st.mip_source_voxels[k] * 16 + a[k] * 512 * 20
         return tot / launches
-    if name == "node_type":
-        return 0.0
+    if name == "setup":
+        # vertices read (36 B / triangle), face normal + packed voxel bounds
+        # written (24 B), each (group, tile) bin count read and written once
+        return (60 * st.triangles + 8 * st.bins) / launches
+    if name == "scan":
+        # every prefix sum reads and writes each value (8 B); the bin scan
+        # also zeroes its input (4 B)
+        return (8 * st.scan_values + 4 * st.bins) / launches
+    if name == "scatter":
+        # packed bounds read (12 B / triangle), one triangle index written per
+        # (triangle, tile) bin entry, bin starts read and fill counts read and
+        # written once
+        return (12 * st.triangles + 4 * st.pairs + 12 * st.bins) / launches
     return 0.0
 
 
@@ -153,79 +169,156 @@ def serving_bench(vz, scene, frames: int, W: int, H: int, slots, max_up: int, st
     return out
 
 
-def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, slots,
-                max_trees: int, max_loading: int, serve_group: int):
+def run_paged(pg, paths, frames: int, W: int, H: int):
+    """Frames at 60 Hz over the camera paths (one per stream): queue the
+    frame, wait for its server step, sleep to the next frame time.  Returns
+    the per-frame wall times, the total and the loaded-tile counts."""
+    import numpy as np
+    period = 1.0 / 60.0
+    wall, loaded_at = [], []
+    t_start = time.perf_counter()
+    for f in range(frames):
+        cams = [dict(position=p[f][0], look=p[f][1], up=p[f][2], fov_y_deg=60.0, width=W,
+                     height=H) for p in paths]
+        a = time.perf_counter()
+        pg.frame(cams)
+        pg.server.stats()
+        wall.append(time.perf_counter() - a)
+        loaded_at.append(pg.tiles_loaded)
+        rest = t_start + (f + 1) * period - time.perf_counter()
+        if rest > 0:
+            time.sleep(rest)
+        if (f + 1) % 100 == 0:
+            log(f"[bench] paged frame {f + 1}: {pg.tiles_loaded} tiles voxelized")
+    return np.asarray(wall), time.perf_counter() - t_start, loaded_at
+
+
+def paged_stats(pg, wall, total, loaded_at, period=1.0 / 60.0):
+    import numpy as np
+    st = pg.server.stats()
+    lat = pg.server.latencies_us() / 1e3
+    loads = np.asarray(pg.load_ms) if pg.load_ms else np.zeros(1)
+    first = next((f for f, n in enumerate(loaded_at) if n > 0), None)
+    return {
+        "tiles": len(pg.trees),
+        "tiles_voxelized": int(pg.tiles_loaded),
+        "frame_period_ms": round(period * 1e3, 3),
+        "frame_ms": round(float(np.mean(wall)) * 1e3, 3),
+        "frame_ms_p99": round(float(np.percentile(wall * 1e3, 99)), 3),
+        "frames_over_period": int((wall > period).sum()),
+        "first_tile_frame": first,
+        # a request that reaches a tile not yet in memory waits for the tile:
+        # from the frame that queued its voxelization to the frame that
+        # attached its octree (its root brick then goes through the LRU
+        # like any other request)
+        "tile_miss_ms_p50": round(float(np.percentile(loads, 50)), 2),
+        "tile_miss_ms_p99": round(float(np.percentile(loads, 99)), 2),
+        "tile_miss_ms_max": round(float(loads.max()), 2),
+        "uploads": int(st.uploads),
+        "evictions": int(st.evictions),
+        "bricks_per_s": round(st.uploads / total, 1),
+        "latency_p50_ms": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
+        "latency_p99_ms": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
+    }
+
+
+def wide_slots(B: int, gb: float) -> int:
+    """Slots of a wide pool of gb GB of (B+2)^3 RGBA8 colour + RGB8 normal
+    bricks."""
+    return int(gb * 1e9 // ((B + 2) ** 3 * 8))
+
+
+def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool_gb: float,
+                max_trees: int, max_loading: int, serve_group: int, f2b_stop=None):
     """Served paging: a (cells+1)^2 fBm terrain in a cells^3 virtual grid,
-    tiled into tile^3 octrees voxelized on demand on the device as the
-    seeded flythrough first sees them (the pager), served from one shared
-    brick pool with LRU eviction; a server frame follows every serve_group
-    trees' ray casts within a frame."""
+    tiled into tile^3 octrees built and voxelized on demand on the device as
+    the seeded flythrough first sees them (the pager), served from one shared
+    brick pool of pool_gb GB (a wide pool beyond 8.2 GB) with LRU eviction;
+    a server frame follows every serve_group trees' ray casts within a
+    frame.  The terrain's heights are the scene generator's fBm (seed 5),
+    built on the device rather than as a host triangle soup."""
     import numpy as np
     from voxvizthesis_amd import scenes
-    from voxvizthesis_amd.pager import TilePager
+    from voxvizthesis_amd.terrain import TerrainPager
 
     # In metres with 1 m voxels, as the reference's tiler assumes (it floors
     # each tree's z origin to a whole unit, main.cu 4258); the camera planes
     # are the reference's 1 m / 10 km.
     extent = float(cells)
+    slots = wide_slots(B, pool_gb) if pool_gb > 8.2 else (56, 56, 56)
     t0 = time.perf_counter()
-    terrain = scenes.heightfield_terrain(cells, 5, extent=extent)
-    t_gen = time.perf_counter() - t0
-    log(f"[bench] paged terrain {cells + 1}^2 generated in {t_gen:.1f} s")
-    t0 = time.perf_counter()
-    pg = TilePager([terrain], 1.0, tile, B, slots, max_trees=max_trees,
-                   max_loading=max_loading, streams=1, max_width=W, max_height=H,
-                   near=1.0, far=10000.0, serve_group=serve_group)
+    pg = TerrainPager(cells, tile, B, slots, seed=5, ridged_amp=0.0, rocks=0,
+                      max_trees=max_trees, max_loading=max_loading, streams=1, max_width=W,
+                      max_height=H, near=1.0, far=10000.0, serve_group=serve_group,
+                      f2b_stop=f2b_stop)
     t_setup = time.perf_counter() - t0
-    path = scenes.flythrough(seed=5, frames=frames, extent=extent, terrain_seed=5)
-    period = 1.0 / 60.0
-    wall, loaded_at = [], []
+    log(f"[bench] paged: {len(pg.trees)} paged trees set up in {t_setup:.1f} s")
+    path = scenes.flythrough(seed=5, frames=frames, extent=extent, height_fn=pg.height_fn)
     try:
-        t_start = time.perf_counter()
-        for f, (pos, look, up) in enumerate(path):
-            # frames at 60 Hz: queue the frame, wait for its server step,
-            # then sleep to the next frame time; tile voxelizations run on
-            # their own streams meanwhile
-            cam = dict(position=pos, look=look, up=up, fov_y_deg=60.0, width=W, height=H)
-            a = time.perf_counter()
-            pg.frame([cam])
-            pg.server.stats()
-            b = time.perf_counter()
-            wall.append(b - a)
-            loaded_at.append(pg.tiles_loaded)
-            rest = t_start + (f + 1) * period - time.perf_counter()
-            if rest > 0:
-                time.sleep(rest)
-        total = time.perf_counter() - t_start
-        st = pg.server.stats()
-        lat = pg.server.latencies_us() / 1e3
-        loads = np.asarray(pg.load_ms) if pg.load_ms else np.zeros(1)
-        first = next((f for f, n in enumerate(loaded_at) if n > 0), None)
-        return {
+        wall, total, loaded_at = run_paged(pg, [path], frames, W, H)
+        out = {
             "workload": f"{cells + 1}^2 fBm terrain (seed 5, {cells} m, 1 m voxels) in a "
                         f"{cells}^3 virtual grid, "
-                        f"{tile}^3 tiles of {B}^3 bricks voxelized on demand, {frames}-frame "
-                        f"seeded flythrough, {W}x{H} ray cast",
+                        f"{tile}^3 tiles of {B}^3 bricks built and voxelized on demand, "
+                        f"{frames}-frame seeded flythrough, {W}x{H} ray cast",
             "serve_group": serve_group,
-            "tiles": len(pg.trees),
-            "tiles_voxelized": int(pg.tiles_loaded),
+            "cast": cast_mode(f2b_stop),
             "pool_slots": int(np.prod(slots)),
             "pool_gb": round(float(np.prod(slots)) * (B + 2) ** 3 * 8 / 1e9, 2),
+            "pool_format": "wide (slot in word 1)" if isinstance(slots, int) else "reference",
             "scene_setup_s": round(t_setup, 2),
             "frames": frames,
-            "frame_period_ms": round(period * 1e3, 3),
-            "frame_ms": round(float(np.mean(wall)) * 1e3, 3),
-            "frame_ms_p99": round(float(np.percentile(np.asarray(wall) * 1e3, 99)), 3),
-            "frames_over_period": int(sum(w > period for w in wall)),
-            "first_tile_frame": first,
-            "tile_load_ms_p50": round(float(np.percentile(loads, 50)), 2),
-            "tile_load_ms_max": round(float(loads.max()), 2),
-            "uploads": int(st.uploads),
-            "evictions": int(st.evictions),
-            "bricks_per_s": round(st.uploads / total, 1),
-            "latency_p50_ms": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
-            "latency_p99_ms": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
         }
+        out.update(paged_stats(pg, wall, total, loaded_at))
+        return out
+    finally:
+        pg.close()
+
+
+def cast_mode(f2b_stop):
+    if f2b_stop is None:
+        return "every tree cast alone, blended far to near (the reference's renderer)"
+    return (f"one front-to-back pass over the visible trees per stream, a pixel skipping "
+            f"the trees behind it once its transmittance is <= {f2b_stop:.5f}")
+
+
+def scale_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool_gb: float,
+                rocks: int, max_trees: int, max_loading: int, serve_group: int,
+                f2b_stop=None):
+    """Served paging at scale: a (cells+1)^2 fBm + ridged terrain built tile
+    by tile on the device (1 m cells), rocks instanced icospheres, tiles
+    voxelized on demand into a pool of pool_gb GB, two camera streams."""
+    import numpy as np
+    from voxvizthesis_amd import scenes
+    from voxvizthesis_amd.terrain import TerrainPager
+
+    slots = wide_slots(B, pool_gb)
+    t0 = time.perf_counter()
+    pg = TerrainPager(cells, tile, B, slots, seed=6, ridged_seed=7, ridged_amp=0.05,
+                      rocks=rocks, rock_seed=8, rock_radius=(2.0, 40.0), max_trees=max_trees,
+                      max_loading=max_loading, streams=2, max_width=W, max_height=H,
+                      near=1.0, far=4.0 * cells, serve_group=serve_group, f2b_stop=f2b_stop)
+    t_setup = time.perf_counter() - t0
+    log(f"[bench] scale: {len(pg.trees)} paged trees set up in {t_setup:.1f} s")
+    paths = [scenes.flythrough(seed=s, frames=frames, extent=float(cells), alt=(0.08, 0.2),
+                               height_fn=pg.height_fn) for s in (6, 16)]
+    try:
+        wall, total, loaded_at = run_paged(pg, paths, frames, W, H)
+        out = {
+            "workload": f"{cells + 1}^2 fBm + ridged terrain (seeds 6/7, {cells} m, 1 m voxels) "
+                        f"in a {cells}^3 virtual grid, {rocks} instanced icosphere rocks, "
+                        f"{tile}^3 tiles of {B}^3 bricks built and voxelized on demand, two "
+                        f"{W}x{H} camera streams over {frames}-frame seeded flythroughs",
+            "pool_slots": int(slots),
+            "pool_gb": round(float(slots) * (B + 2) ** 3 * 8 / 1e9, 2),
+            "pool_format": "wide (slot in word 1)",
+            "cast": cast_mode(f2b_stop),
+            "max_uploads_per_frame": 65536,
+            "scene_setup_s": round(t_setup, 2),
+            "frames": frames,
+        }
+        out.update(paged_stats(pg, wall, total, loaded_at))
+        return out
     finally:
         pg.close()
 
@@ -234,8 +327,10 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
     """Batched voxelization: n_jobs independent meshes (half fBm heightfield
     tiles, half random soups, triangle counts log-uniform 2K..tmax, grids from
     `grids`) captured into hipGraphs.  The autotuner picks each job's brick
-    edge (8/16/32) and persistent launch shape by timing eager runs.  Jobs are
-    admitted by free device memory: while they fit, they join the current
+    edge (8/16/32) and persistent launch shape by timing eager runs.  Every
+    graph voxelizes all of its jobs in one persistent launch over a device
+    work queue (the same graph with each job's own voxelize launches is timed
+    beside it).  Jobs are admitted by free device memory: while they fit, they join the current
     graph; when setting one up runs out of memory, the resident jobs are
     captured into one graph, timed and freed, and the next graph starts with
     that job (every job keeps its working buffers while its graph is
@@ -248,8 +343,8 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
     jobs_sc = scenes.batch_scenes(seed=4, n_jobs=n_jobs, tmax=tmax, grids=grids)
     log(f"[bench] batch: {n_jobs} meshes generated in {time.perf_counter() - t0:.1f} s")
     choices, t_tune = [], 0.0
-    eager_ms = graph_ms = device_ms = check_ms = 0.0
-    nodes, graphs, done = 0, 0, 0
+    eager_ms = graph_ms = device_ms = check_ms = per_job_ms = 0.0
+    nodes, graphs, done, vox_launches = 0, 0, 0, 0
 
     def oom(e):
         return "out of memory" in str(e)
@@ -262,10 +357,10 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
             sc = pending[0]
             try:
                 if do_autotune:
-                    B, pb, _ = autotune(sc, reps=1)
+                    B, pb, _, v = autotune(sc, reps=1, keep=True)
                 else:
                     B, pb = 8, 0
-                v = make_job(sc, B, pb)
+                    v = make_job(sc, B, pb)
             except RuntimeError as e:
                 if not oom(e) or not jobs:
                     raise
@@ -286,20 +381,33 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
                 v.generateVoxelsAndOctTree(None)
             eager.append(time.perf_counter() - t1)
         bt = Batch(jobs)
+        # the graph with every job's own voxelize launches, for comparison
+        bt.set_voxelize(persistent=False)
         bt.capture()
-        graph, check = [], []
+        split = []
+        for _ in range(reps):
+            if not bt.launch():
+                raise RuntimeError(bt.getErrorMessage())
+            split.append(bt.stats().ms_graph_last)
+        per_job_ms += float(np.median(split))
+        # the graph with one persistent voxelize launch over all jobs
+        bt.set_voxelize(persistent=True)
+        bt.capture()
+        graph, check, dev = [], [], []
         for _ in range(reps):
             t1 = time.perf_counter()
             if not bt.launch():
                 raise RuntimeError(bt.getErrorMessage())
             graph.append(time.perf_counter() - t1)
             check.append(bt.stats().ms_check_last)
+            dev.append(bt.stats().ms_graph_last)
         st = bt.stats()
         eager_ms += float(np.median(eager)) * 1e3
         graph_ms += float(np.median(graph)) * 1e3
-        device_ms += st.ms_graph_last
+        device_ms += float(np.median(dev))
         check_ms += float(np.median(check))
         nodes += int(st.graph_nodes)
+        vox_launches += int(st.voxelize_launches)
         graphs += 1
         bt.close()
         for v in jobs:
@@ -323,6 +431,9 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
         "eager_ms": round(eager_ms, 3),
         "graph_ms": round(graph_ms, 3),
         "graph_device_ms": round(device_ms, 3),
+        # the same graph with every job voxelized by launches of its own
+        "graph_device_ms_per_job_voxelize": round(per_job_ms, 3),
+        "voxelize_launches": vox_launches,
         "checksum_ms": round(check_ms, 3),
         "graph_nodes": nodes,
         "meshes_per_s": round(n_jobs / g, 1),
@@ -331,53 +442,74 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
     }
 
 
-def cpu_baseline(sample_n: int, all_core_n: int = 256):
-    import oracle
-    from voxvizthesis_amd import scenes
-    sc = scenes.scene_full_octree(cells=sample_n, N=sample_n, B=16)
-    oracle.build()
-    t0 = time.perf_counter()
-    r = oracle.run(sc, keep_mips=False)
-    dt = time.perf_counter() - t0
-    if r.status != 1:
-        raise RuntimeError("oracle sample failed")
-    return {
-        "value": round(sample_n ** 3 / dt / 1e9, 6),
-        "unit": "Gvoxels/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"oracle (scalar C++ port of the reference logic) on the same scene "
-                   f"(seeds 2/3, {sample_n + 1}^2 terrain + 2,000 rocks) at {sample_n}^3, "
-                   f"16^3 bricks, {dt:.1f} s on one host core"),
-        "seconds": round(dt, 3),
-        "all_core": cpu_all_core(all_core_n),
-    }
-
-
-def cpu_all_core(n: int):
-    """The oracle run concurrently on every host core the process may use (at
-    most 16), one independent copy of a smaller sample per thread: aggregate
-    throughput. The oracle's C++ runs outside the interpreter lock."""
+def cpu_baseline(scene, win: int = 256):
+    """The voxelization workload itself on the host cores: the oracle (the
+    scalar C++ restatement of the reference's voxelizer) over the win^3
+    windows that tile the workload's grid, at its own voxel size, each given
+    the triangles whose bounds meet its voxel space (empty windows skipped),
+    one window per thread at a time on every host core the process may use,
+    at most 16 (the host share of one GPU).  Gvoxels/s = the whole grid over
+    the wall time; the single-core equivalent divides by the summed window
+    times instead."""
     import os
     from concurrent.futures import ThreadPoolExecutor
 
+    import numpy as np
     import oracle
     from voxvizthesis_amd import scenes
+
+    oracle.build()
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
-    sc = scenes.scene_full_octree(cells=n, N=n, B=16)
+    N = scene.N[0]
+    n = N // win
+    d = np.float32(scene.delta[0])
+    extra = (win // scene.B[0]) >> 1
+    bounds = [(m.verts.min(axis=1), m.verts.max(axis=1)) for m in scene.meshes]
+    jobs = []
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                p = (np.asarray(scene.p, np.float32) +
+                     np.array([i, j, k], np.float32) * np.float32(win) * d).astype(np.float32)
+                lo = p - np.float32(extra + 1) * d
+                hi = p + np.float32(win + 3 * extra + 1) * d
+                meshes = []
+                for m, (bmin, bmax) in zip(scene.meshes, bounds):
+                    sel = np.all((bmax >= lo) & (bmin <= hi), axis=1)
+                    if sel.any():
+                        meshes.append(scenes.Mesh(
+                            verts=m.verts[sel],
+                            vnormals=None if m.vnormals is None else m.vnormals[sel],
+                            uvs=None if m.uvs is None else m.uvs[sel], image=m.image,
+                            is_terrain=m.is_terrain, addr=m.addr))
+                if meshes:
+                    jobs.append(scenes.Scene(meshes=meshes, N=(win,) * 3, B=scene.B,
+                                             p=tuple(float(x) for x in p), delta=scene.delta))
+
+    def one(sc):
+        t0 = time.perf_counter()
+        r = oracle.run(sc, keep_mips=False)
+        if r.status == 0:
+            raise RuntimeError("oracle window failed")
+        return time.perf_counter() - t0
+
     t0 = time.perf_counter()
     with ThreadPoolExecutor(cores) as ex:
-        st = list(ex.map(lambda _: oracle.run(sc, keep_mips=False).status, range(cores)))
-    dt = time.perf_counter() - t0
-    if any(s != 1 for s in st):
-        raise RuntimeError("oracle all-core sample failed")
+        times = list(ex.map(one, jobs))
+    wall = time.perf_counter() - t0
     return {
-        "value": round(cores * n ** 3 / dt / 1e9, 6),
+        "value": round(N ** 3 / wall / 1e9, 6),
         "unit": "Gvoxels/s",
         "cores": cores,
-        "sample": (f"{cores} concurrent oracle runs of the same scene at {n}^3 "
-                   f"({n + 1}^2 terrain + 2,000 rocks, 16^3 bricks), {dt:.1f} s wall"),
-        "seconds": round(dt, 3),
+        "kind": "port",
+        "sample": (f"the whole {N}^3 workload ({scene.name}) as {len(jobs)} of its {n ** 3} "
+                   f"windows of {win}^3 voxels (the others hold no triangle), each voxelized "
+                   f"by the oracle (scalar C++ port of the reference logic, {win // scene.B[0]}"
+                   f"-brick windows) on one of {cores} threads; {wall:.1f} s wall, "
+                   f"{sum(times):.1f} s summed"),
+        "seconds": round(wall, 3),
+        "window_seconds_sum": round(sum(times), 3),
+        "single_core_equivalent": round(N ** 3 / sum(times) / 1e9, 6),
     }
 
 
@@ -389,23 +521,37 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain copy kernel (profiled runs)")
-    ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--cpu-window", type=int, default=256,
+                    help="window edge of the host-core baseline over the workload")
     ap.add_argument("--serve-frames", type=int, default=300)
     ap.add_argument("--serve-width", type=int, default=1920)
     ap.add_argument("--serve-height", type=int, default=1080)
     ap.add_argument("--serve2-width", type=int, default=2560)
     ap.add_argument("--serve2-height", type=int, default=1440)
     ap.add_argument("--serve-max-uploads", type=int, default=65536)
+    ap.add_argument("--serve-evict-slots", type=int, default=16,
+                    help="slots per axis of the serving leg whose pool is smaller than the "
+                         "octree's bricks")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--paged-cells", type=int, default=4096)
+    ap.add_argument("--paged-cells", type=int, default=8192)
+    ap.add_argument("--paged-pool-gb", type=float, default=16.0)
     ap.add_argument("--no-paged", action="store_true")
+    ap.add_argument("--scale-cells", type=int, default=32768)
+    ap.add_argument("--scale-pool-gb", type=float, default=128.0)
+    ap.add_argument("--scale-rocks", type=int, default=100000)
+    ap.add_argument("--scale-frames", type=int, default=300)
+    ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--paged-trees", type=int, default=96,
                     help="octrees the paged leg keeps loaded (tree slots of the server)")
+    ap.add_argument("--paged-per-tree", action="store_true",
+                    help="paged legs: cast every tree alone and blend far to near")
+    ap.add_argument("--f2b-stop", type=float, default=1.0 / 255.0,
+                    help="paged legs' front-to-back pass: transmittance at which a pixel stops")
     ap.add_argument("--paged-group", type=int, default=3,
                     help="trees per server frame within a paged frame (0: one per frame)")
-    ap.add_argument("--batch-jobs", type=int, default=32)
-    ap.add_argument("--batch-tmax", type=float, default=2.6e5)
-    ap.add_argument("--batch-grids", default="64,128,256")
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--batch-tmax", type=float, default=2e6)
+    ap.add_argument("--batch-grids", default="64,128,256,512,1024")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
@@ -450,6 +596,10 @@ def main():
         src, dst = L.vox_device_alloc(nb), L.vox_device_alloc(nb)
         if not (src and dst and L.vox_device_copy(dst, src, nb)):
             raise RuntimeError("calibration copy failed")
+        # the same bytes in 256-byte blocks visited in a scattered order: a
+        # second reference point, closer to the gathering kernels
+        if not L.vox_device_copy_scattered(dst, src, nb):
+            raise RuntimeError("scattered calibration copy failed")
         L.vox_device_free(src)
         L.vox_device_free(dst)
         log(f"[bench] calibration copy of {nb} bytes done")
@@ -496,11 +646,12 @@ def main():
     def roofline_of(name):
         kms, launches = kern[name]
         pl = kms / launches
-        kb = kernel_bytes(name, scene, st, bricks, B, launches)
+        # the stats of the timing step: it also counts the mip sources
+        kb = kernel_bytes(name, scene, ts, bricks, B, launches)
         ach = kb / (pl * 1e-3) / 1e9
         # the kernel that does the work of every depth in one launch carries "_all"
         tr = tj.get("k_" + name) or tj.get("k_" + name + "_all")
-        return {
+        out = {
             "kernel": name,
             "bound": "hbm",
             "achieved": round(ach, 2),
@@ -511,6 +662,11 @@ def main():
             "per_launch_ms": round(pl, 4),
             "algorithmic_bytes_per_launch": int(kb),
         }
+        if name == "voxelize":
+            # fraction above is of HBM bandwidth; what bounds this kernel is
+            # VALU issue (SQ counters, profiles/full_octree_traffic.md)
+            out["limiter"] = "VALU issue: ~75 % of the SIMDs' VALU issue rate (SQ counters)"
+        return out
 
     # The roofline names k_voxelize (one launch per triangle group), the
     # voxelization kernel and the largest kernel time per step in the rocprof
@@ -545,7 +701,8 @@ def main():
         "pcie_upload_s": round(t_up, 4),
         "kernels_ms": kern_detail,
         "roofline": roofline_of("voxelize"),
-        "rooflines": [roofline_of(k) for k in ("brick_gather", "mip") if k in kern],
+        "rooflines": [roofline_of(k) for k in ("brick_gather", "mip", "setup", "scan", "scatter")
+                      if k in kern],
     }
     if not args.no_serve:
         log(f"[bench] serving {args.serve_frames} frames at {args.serve_width}x"
@@ -559,14 +716,30 @@ def main():
                             (56, 56, 56), args.serve_max_uploads, streams=2)
         out["serving_two_streams"] = sv2
         log(f"[bench] two streams: {sv2['frame_ms']} ms/frame, p99 {sv2['latency_p99_ms']} ms")
+        # a pool smaller than the octree's bricks, so the LRU evicts
+        e = args.serve_evict_slots
+        sv3 = serving_bench(vz, scene, args.serve_frames, args.serve_width, args.serve_height,
+                            (e, e, e), args.serve_max_uploads)
+        out["serving_evicting"] = sv3
+        log(f"[bench] evicting pool: {sv3['frame_ms']} ms/frame, {sv3['evictions']} evictions, "
+            f"p99 {sv3['latency_p99_ms']} ms")
+    f2b = None if args.paged_per_tree else args.f2b_stop
     if not args.no_paged:
         log(f"[bench] paged serving over a {args.paged_cells}^3 virtual grid ...")
         pb = paged_bench(args.paged_cells, args.serve_frames, args.serve_width,
-                         args.serve_height, 1024, 16, (56, 56, 56), args.paged_trees, 4,
-                         args.paged_group)
+                         args.serve_height, 1024, 16, args.paged_pool_gb, args.paged_trees, 4,
+                         args.paged_group, f2b)
         out["paged"] = pb
         log(f"[bench] paged: {pb['frame_ms']} ms/frame, {pb['tiles_voxelized']} tiles, "
             f"p99 {pb['latency_p99_ms']} ms")
+    if not args.no_scale:
+        log(f"[bench] paged serving at scale over a {args.scale_cells}^3 virtual grid ...")
+        sb = scale_bench(args.scale_cells, args.scale_frames, args.serve2_width,
+                         args.serve2_height, 1024, 16, args.scale_pool_gb, args.scale_rocks,
+                         args.paged_trees, 4, args.paged_group, f2b)
+        out["paged_scale"] = sb
+        log(f"[bench] scale: {sb['frame_ms']} ms/frame, {sb['tiles_voxelized']} tiles, "
+            f"p99 {sb['latency_p99_ms']} ms")
     if not args.no_batch:
         log(f"[bench] batch of {args.batch_jobs} meshes ...")
         bb = batch_bench(args.batch_jobs, args.batch_tmax,
@@ -575,13 +748,13 @@ def main():
         out["batched"] = bb
         log(f"[bench] batch: graph {bb['graph_ms']} ms vs eager {bb['eager_ms']} ms")
     if not args.no_cpu_baseline:
-        log(f"[bench] cpu baseline on a {args.cpu_sample}^3 sample ...")
-        cb = cpu_baseline(args.cpu_sample)
+        log(f"[bench] cpu baseline: the workload in {args.cpu_window}^3 windows ...")
+        cb = cpu_baseline(scene, args.cpu_window)
         out["cpu_baseline"] = cb
         out["gpu_vs_cpu"] = round(out["value"] / cb["value"], 1)
-        out["gpu_vs_cpu_all_core"] = round(out["value"] / cb["all_core"]["value"], 1)
-        log(f"[bench] cpu: {cb['value']} Gvoxels/s on one core, "
-            f"{cb['all_core']['value']} on {cb['all_core']['cores']}")
+        out["gpu_vs_cpu_single_core"] = round(out["value"] / cb["single_core_equivalent"], 1)
+        log(f"[bench] cpu: {cb['value']} Gvoxels/s on {cb['cores']} cores "
+            f"({cb['single_core_equivalent']} single-core equivalent)")
     vz.close()
     print(json.dumps(out))
 

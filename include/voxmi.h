@@ -257,6 +257,12 @@ int vox_device_free(void* p);
  * multiple of 16 bytes); returns when done.  Its known traffic is the
  * reference point that calibrates hardware traffic counters. */
 int vox_device_copy(void* dst, const void* src, uint64_t bytes);
+/* The same copy with its 256-byte blocks visited in a scattered order (block
+ * i of the launch copies block (i * 0x9E3779B1) mod n, n = bytes / 256 a
+ * power of two): every byte still read and written once, but neighbouring
+ * workgroups touch distant lines, as a gathering kernel does.  A second
+ * reference point for the traffic counters.  Not in the reference. */
+int vox_device_copy_scattered(void* dst, const void* src, uint64_t bytes);
 
 /* Copy bytes from src + offset (device) to host memory dst; returns when
  * they are on the host.  The counterpart of vox_device_upload.  Not in the

@@ -6,7 +6,10 @@
  * Voxelizer.cu 2216-2236).  Here a batch of voxelizer handles (meshes or
  * tiles, each with its own grid and brick size) is captured once into a
  * single hipGraph -- triangle setup, binning and the whole generate of every
- * job -- and replayed with one launch.  The capture reuses the sizes an eager
+ * job -- and replayed with one launch.  The voxelization proper of all jobs
+ * is one persistent kernel working through every job's tiles from a device
+ * queue (vox_batch_set_voxelize), so meshes of very different sizes share
+ * the GPU in one launch rather than one small launch each.  The capture reuses the sizes an eager
  * vox_generate of each job measured, so there is no host round trip inside
  * the graph.  Before every launch the jobs' inputs are checksummed on the
  * device and their buffers compared with the captured ones: a launch on
@@ -33,6 +36,9 @@ typedef struct vox_batch_stats {
     double   ms_check_last;     /* input checksums before it                  */
     double   ms_capture;        /* host time of the capture + instantiation   */
     uint64_t graph_nodes;
+    uint32_t voxelize_launches; /* voxelize kernels in the graph               */
+    uint32_t voxelize_blocks;   /* workgroups of each persistent voxelize launch
+                                   (0: one launch per job and group)          */
 } vox_batch_stats;
 
 /* jobs: handles that have each completed one eager vox_generate with their
@@ -48,6 +54,13 @@ int vox_batch_capture(vox_batch* b);
  * Each job's result is then available through vox_get_result. */
 int vox_batch_launch(vox_batch* b);
 int vox_batch_get_stats(vox_batch* b, vox_batch_stats* out);
+/* How the next capture voxelizes.  persistent = 1 (the default): one
+ * persistent launch of `blocks` workgroups (0: 4 per CU) per wave of groups
+ * (wave w holds the w-th group that overlaps its voxel space of every job),
+ * its workgroups taking units of a few tiles of any job from a device queue,
+ * heaviest jobs first; persistent = 0: every job's groups voxelized by
+ * launches of their own inside its branch.  Drops a captured graph. */
+int vox_batch_set_voxelize(vox_batch* b, int32_t persistent, uint32_t blocks);
 
 #ifdef __cplusplus
 }

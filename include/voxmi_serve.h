@@ -101,6 +101,24 @@ const char* vox_raycaster_get_error(vox_raycaster* rc);
 int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycast_params* prm);
 /* The last frame's buffers, counts and timings (waits for the frame). */
 int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out);
+/* One front-to-back frame over the octrees of n ray casters (one camera,
+ * per octree its own parameters: box, pixel rectangle; one jitter seed),
+ * queued on the first caster's stream.  rcs must list the octrees in an
+ * order every ray meets them in (boxes that tile space, sorted outward from
+ * the camera).  Each pixel casts its ray through each octree whose rectangle
+ * holds it exactly as vox_raycast_frame does, and the casts' FragColors are
+ * accumulated as vox_composite's far-to-near blend would accumulate them
+ * (out = sum of T * src * src.a per channel, T the product of (1 - src.a) of
+ * the octrees in front), into frame: width*height RGBA32F pixels on the
+ * device, every pixel written.  A pixel whose T falls to t_stop or below
+ * (0: never) casts no further octree, so their bricks are not requested
+ * through it; with t_stop = 0 the frame equals vox_composite's up to fp32
+ * rounding.  Each caster gets the request list of its octree's casts (read
+ * by vox_server_frame / _multi after this frame); its colour and usage
+ * buffers are not written.  Not in the reference, which rasterizes and
+ * casts every paged octree separately (GigaVoxelsRenderer.cpp 1019-1040). */
+int vox_raycast_frame_multi(vox_raycaster* const* rcs, uint32_t n, const vox_camera* cam,
+                            const vox_raycast_params* prms, float* frame, float t_stop);
 /* Blend the last frames of n ray casters, in the order given (far to near,
  * as the renderer draws its list), into dst: width*height RGBA32F pixels on
  * the device, the renderer's colour target (GigaVoxelsRenderer.cpp 511-515).

@@ -143,3 +143,62 @@ def test_pager_tiles_match_alone_and_pool_is_consistent():
         assert n_res > 0
     finally:
         pg.close()
+
+
+def test_front_to_back_frame_equals_per_tree_casts():
+    """One front-to-back pass over the loaded trees gives the image the
+    per-tree casts' FragColors give when blended front to back (bit-exact,
+    restated in numpy fp32), with every tree's request set equal to its
+    per-tree cast's (no pixel stops) or within it (pixels stop behind
+    opaque trees)."""
+    sc = scenes.scene_full_octree(cells=64, N=64, B=8, n_rocks=60)
+    W, H = 64, 48
+    pg = TilePager(sc.meshes, 1.0 / 128, 64, 8, (8, 8, 8), max_trees=8, max_loading=3,
+                   streams=1, max_width=W, max_height=H, near=1e-3, far=100.0, serve_group=2)
+    try:
+        path = [((0.5 + 0.05 * f, -0.6, 1.1 - 0.02 * f), (0.5, 0.5, 0.45)) for f in range(6)]
+        for f, (pos, target) in enumerate(path):
+            look, up = look_at(pos, target)
+            pg.frame([dict(position=pos, look=look, up=up, fov_y_deg=60.0, width=W, height=H)])
+            pg.drain()
+        checked = 0
+        for pos, target in (((0.5, -0.6, 1.0), (0.5, 0.5, 0.45)),
+                            ((-0.3, 0.2, 0.7), (0.6, 0.6, 0.4))):
+            look, up = look_at(pos, target)
+            cam = dict(position=pos, look=look, up=up, fov_y_deg=60.0, width=W, height=H)
+            for _ in range(2):   # every tree this camera sees loaded first
+                pg.frame([cam])
+                pg.drain()
+            for stop in (0.0, 1.0 / 255.0):
+                print(f"[pager] front to back from {pos}, stop {stop:.4f}", flush=True)
+                pg.f2b_stop = None
+                drawn = pg.frame([cam], serve=False)
+                per = {}
+                for tr, rc in drawn[0]:
+                    r = box_rect(cam, tr.box_min, tr.box_max)
+                    per[id(tr)] = (rc.rgba_f32(), set(rc.requests().tolist()),
+                                   (0, 0, W, H) if r is None else r)
+                pg.f2b_stop = stop
+                drawn2 = pg.frame([cam], serve=False)
+                assert sorted(id(t) for t, _ in drawn2[0]) == sorted(per)
+                img = pg.image(0)
+                acc = np.zeros((H, W, 4), np.float32)
+                T = np.ones((H, W), np.float32)
+                for tr, rc in drawn2[0]:
+                    src, _, (x0, y0, x1, y1) = per[id(tr)]
+                    m = np.zeros((H, W), bool)
+                    m[y0:y1, x0:x1] = True
+                    m &= T > np.float32(stop)
+                    a = src[..., 3]
+                    acc[m] = acc[m] + T[m][:, None] * (src[m] * a[m][:, None])
+                    T[m] = T[m] * (np.float32(1.0) - a[m])
+                assert (img == acc).all()
+                for tr, rc in drawn2[0]:
+                    got, want = set(rc.requests().tolist()), per[id(tr)][1]
+                    assert got == want if stop == 0.0 else got <= want
+                    checked += 1
+                pg.serve()
+        print(f"[pager] checked {checked} tree casts", flush=True)
+        assert checked >= 4
+    finally:
+        pg.close()

@@ -7,8 +7,8 @@ import math
 import numpy as np
 
 from voxvizthesis_amd.gvserve import box_rect
-from voxvizthesis_amd.pager import (frustum_planes, load_order, look_at_matrix, perspective,
-                                    sphere_in_frustum)
+from voxvizthesis_amd.pager import (PagedTree, front_to_back, frustum_planes, load_order,
+                                    look_at_matrix, perspective, sphere_in_frustum)
 
 
 def planes_x(fov=90.0, near=0.1, far=100.0):
@@ -62,3 +62,49 @@ def test_box_rect_centred_and_behind():
     assert box_rect(dict(cam, position=(0.5, 0.5, 0.5)), np.zeros(3), np.ones(3)) is None
     # entirely off to the side: nothing
     assert box_rect(dict(cam, position=(40.0, -2.0, 0.5)), np.zeros(3), np.ones(3)) == ()
+
+
+def _ray_box(o, d, lo, hi):
+    """Entry parameter of the ray o + t d (t >= 0) into the box, or None."""
+    t0, t1 = 0.0, math.inf
+    for a in range(3):
+        if d[a] == 0.0:
+            if o[a] < lo[a] or o[a] > hi[a]:
+                return None
+            continue
+        u, v = (lo[a] - o[a]) / d[a], (hi[a] - o[a]) / d[a]
+        t0, t1 = max(t0, min(u, v)), min(t1, max(u, v))
+    return t0 if t1 > t0 else None
+
+
+def test_front_to_back_order_agrees_with_every_ray():
+    # columns of a 5 x 4 grid of 10 m tiles, each a stack of 1-3 boxes from
+    # its own height; brute force: the boxes a ray enters, by entry time,
+    # come in the same order as in front_to_back's list
+    rng = np.random.Generator(np.random.PCG64(3))
+    trees = []
+    for y in range(4):
+        for x in range(5):
+            oz = float(rng.uniform(-15.0, 15.0))
+            for z in range(int(rng.integers(1, 4))):
+                lo = np.array([10.0 * x, 10.0 * y, oz + 10.0 * z])
+                hi = lo + 10.0
+                trees.append(PagedTree(x, y, z, y * 5 + x, tuple(lo), lo, hi, (lo + hi) / 2,
+                                       8.7))
+    checked = 0
+    for _ in range(300):
+        pos = rng.uniform([-20.0, -20.0, -30.0], [70.0, 60.0, 50.0])
+        order = {id(t): k for k, t in enumerate(front_to_back(trees, pos))}
+        for _ in range(20):
+            # towards a point of the grid, so that most rays cross boxes
+            d = rng.uniform([0.0, 0.0, -15.0], [50.0, 40.0, 45.0]) - pos
+            hits = []
+            for t in trees:
+                e = _ray_box(pos, d, t.box_min, t.box_max)
+                if e is not None:
+                    hits.append((e, order[id(t)]))
+            hits.sort()
+            ranks = [r for _, r in hits]
+            assert ranks == sorted(ranks), (pos, d)
+            checked += len(hits) > 1
+    assert checked > 3000

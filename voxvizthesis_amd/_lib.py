@@ -28,12 +28,13 @@ EXPORTS = [
     "vox_copy_level", "vox_write_files", "vox_get_compressed", "vox_set_launch",
     "vox_generate_async", "vox_wait", "vox_release_work_memory", "vox_add_group_gather",
     "vox_device_alloc", "vox_device_upload", "vox_device_free", "vox_device_copy",
+    "vox_device_copy_scattered",
     "vox_device_download", "vox_scan_u32", "vox_scan_u32_noalloc", "vox_scan_get_error",
 ]
 # Every symbol include/voxmi_batch.h declares.
 BATCH_EXPORTS = [
     "vox_batch_create", "vox_batch_destroy", "vox_batch_get_error", "vox_batch_capture",
-    "vox_batch_launch", "vox_batch_get_stats",
+    "vox_batch_launch", "vox_batch_get_stats", "vox_batch_set_voxelize",
 ]
 # Every symbol include/voxmi_pool.h declares.
 POOL_EXPORTS = [
@@ -48,7 +49,7 @@ SERVE_EXPORTS = [
     "vox_server_create", "vox_server_destroy", "vox_server_get_error", "vox_server_frame",
     "vox_server_get_stats", "vox_server_latencies", "vox_server_state",
     "vox_server_create_shared", "vox_server_attach", "vox_server_detach",
-    "vox_server_frame_multi", "vox_composite",
+    "vox_server_frame_multi", "vox_composite", "vox_raycast_frame_multi",
 ]
 # Every symbol include/voxmi_terrain.h declares.
 TERRAIN_EXPORTS = [
@@ -111,7 +112,8 @@ class VoxTileLists(C.Structure):
 class VoxBatchStats(C.Structure):
     _fields_ = [("jobs", C.c_int32), ("launches", C.c_uint64), ("ms_graph_last", C.c_double),
                 ("ms_check_last", C.c_double), ("ms_capture", C.c_double),
-                ("graph_nodes", C.c_uint64)]
+                ("graph_nodes", C.c_uint64), ("voxelize_launches", C.c_uint32),
+                ("voxelize_blocks", C.c_uint32)]
 
 
 class VoxStats(C.Structure):
@@ -258,6 +260,7 @@ def load():
     L.vox_device_upload.argtypes = [vp, C.c_uint64, vp, C.c_uint64]
     L.vox_device_free.argtypes = [vp]
     L.vox_device_copy.argtypes = [vp, vp, C.c_uint64]
+    L.vox_device_copy_scattered.argtypes = [vp, vp, C.c_uint64]
     L.vox_device_download.argtypes = [vp, vp, C.c_uint64, C.c_uint64]
     L.vox_scan_u32.argtypes = [vp, vp, C.c_uint64, C.c_int32]
     L.vox_scan_u32_noalloc.argtypes = [vp, vp, C.c_uint64, C.c_int32]
@@ -273,6 +276,7 @@ def load():
     L.vox_batch_capture.argtypes = [vp]
     L.vox_batch_launch.argtypes = [vp]
     L.vox_batch_get_stats.argtypes = [vp, C.POINTER(VoxBatchStats)]
+    L.vox_batch_set_voxelize.argtypes = [vp, C.c_int32, C.c_uint32]
     # terrain
     L.vox_terrain_heights.argtypes = [vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64,
                                       C.c_double]
@@ -320,6 +324,8 @@ def load():
     L.vox_raycaster_get_error.restype = C.c_char_p
     L.vox_raycaster_get_error.argtypes = [vp]
     L.vox_raycast_frame.argtypes = [vp, C.POINTER(VoxCamera), C.POINTER(VoxRaycastParams)]
+    L.vox_raycast_frame_multi.argtypes = [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(VoxCamera),
+                                          C.POINTER(VoxRaycastParams), vp, C.c_float]
     L.vox_raycaster_get_frame.argtypes = [vp, C.POINTER(VoxFrameView)]
     # brick server
     L.vox_server_create.restype = vp

@@ -3,8 +3,9 @@ each job's brick edge and persistent launch shape (include/voxmi_batch.h).
 
 A job is a Voxelizer that has run one eager generateVoxelsAndOctTree (which
 sizes its buffers).  `Batch.capture()` records every job's triangle setup and
-generate into a single graph; `Batch.launch()` replays it after checking on
-the device that no job's triangles changed.
+generate into a single graph, with the voxelization of all jobs in one
+persistent launch over a device work queue; `Batch.launch()` replays it after
+checking on the device that no job's triangles changed.
 """
 from __future__ import annotations
 
@@ -31,6 +32,12 @@ class Batch:
 
     def getErrorMessage(self) -> str:
         return self._L.vox_batch_get_error(self._h).decode()
+
+    def set_voxelize(self, persistent: bool = True, blocks: int = 0):
+        """Voxelize every job in one persistent launch per wave of groups
+        (default; blocks 0 = 4 workgroups per CU), or per job and group."""
+        if not self._L.vox_batch_set_voxelize(self._h, 1 if persistent else 0, int(blocks)):
+            raise RuntimeError(self.getErrorMessage())
 
     def capture(self):
         if not self._L.vox_batch_capture(self._h):
@@ -88,23 +95,54 @@ def make_job(scene, B: int, persistent_blocks: int = 0, options: int = 0) -> Vox
     return vz
 
 
-def autotune(scene, edges=(8, 16, 32), blocks_per_cu=(4, 8, 16), cus: int = 256, reps: int = 2):
+def autotune(scene, edges=(8, 16, 32), blocks_per_cu=(4, 8, 16), cus: int = 256, reps: int = 2,
+             keep: bool = False):
     """Time eager generates of the scene for every brick edge and persistent
-    launch shape; return (B, persistent_blocks, ms) of the fastest."""
-    best = None
-    for B in brick_candidates(scene.N[0], edges):
-        for k in blocks_per_cu:
-            pb = k * cus
-            vz = make_job(scene, B, pb)
+    launch shape (one voxelizer per brick edge, its launch shape changed
+    between timings); return (B, persistent_blocks, ms) of the fastest, and
+    with keep=True also that voxelizer, set to the chosen shape (the caller
+    closes it)."""
+    best, best_vz = None, None
+    try:
+        for B in brick_candidates(scene.N[0], edges):
+            vz = make_job(scene, B)
+            won = False
             try:
-                t = math.inf
-                for _ in range(reps):
-                    vz.computeEdgesFaceNormalsAndBounds()
-                    t0 = time.perf_counter()
-                    vz.generateVoxelsAndOctTree(None)
-                    t = min(t, (time.perf_counter() - t0) * 1e3)
-            finally:
+                for k in blocks_per_cu:
+                    pb = k * cus
+                    if not vz._L.vox_set_launch(vz._h, int(pb)):
+                        raise RuntimeError(vz.getErrorMessage())
+                    t = math.inf
+                    for _ in range(reps):
+                        vz.computeEdgesFaceNormalsAndBounds()
+                        t0 = time.perf_counter()
+                        if vz.generateVoxelsAndOctTree(None) == 0:
+                            raise RuntimeError(vz.getErrorMessage())
+                        t = min(t, (time.perf_counter() - t0) * 1e3)
+                    if best is None or t < best[2]:
+                        best = (B, pb, t)
+                        won = True
+            except BaseException:
                 vz.close()
-            if best is None or t < best[2]:
-                best = (B, pb, t)
+                raise
+            if keep and won:
+                if best_vz is not None:
+                    best_vz.close()
+                best_vz = vz
+            else:
+                vz.close()
+    except BaseException:
+        if best_vz is not None:
+            best_vz.close()
+        raise
+    if keep:
+        if best_vz is not None:
+            # the kept voxelizer's last timing may have used another shape
+            best_vz._L.vox_set_launch(best_vz._h, int(best[1]))
+            best_vz.computeEdgesFaceNormalsAndBounds()
+            if best_vz.generateVoxelsAndOctTree(None) == 0:
+                err = best_vz.getErrorMessage()
+                best_vz.close()
+                raise RuntimeError(err)
+        return best[0], best[1], best[2], best_vz
     return best

@@ -763,14 +763,21 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
     return 1;
 }
 
-int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
-    if (!h || !h->valid || !idle(h)) return 0;
-    h->compressed = compressed != 0;
-    if (h->replay && out_dir) return h->fail("a replayed generate writes no files"), 0;
-    if (h->replay && h->multi)
-        return h->fail("a voxelizer with several chunks orders its bricks on the host's schedule "
-                       "and is not captured in a graph"), 0;
-    if (!h->prepared && !vox_compute_edges_face_normals_and_bounds(h)) return 0;
+namespace {
+
+// vox_generate in four parts, so a batch capture can put one persistent
+// voxelize launch of all its jobs between every job's tile maps and its
+// colour pass: the tile maps and buffers (gen_front), per overlapping group
+// its work list (gen_group_tiles), voxelize (gen_voxelize, or its arguments
+// from gen_voxelize_job) and colour + fill (gen_group_tail), then the mips,
+// octree and bricks (gen_back).
+int first_overlap(const vox_ctx* h) {
+    for (size_t g = 0; g < h->gargs.size(); ++g)
+        if (h->gargs[g].overlaps) return (int)g;
+    return -1;
+}
+
+int gen_front(vox_ctx* h) {
     const int G = (int)h->groups.size();
     const int L = h->L;
     VoxParams& P = h->P;
@@ -779,10 +786,6 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
         std::memset(&h->stats, 0, sizeof(h->stats));
         (void)hipEventRecord(h->evStart, st);
     }
-
-    int firstOverlap = -1;
-    for (int g = 0; g < G; ++g)
-        if (h->gargs[g].overlaps) { firstOverlap = g; break; }
 
     // Active tiles: level 0 from the bins, level k from level src(k).
     LevelBufs& l0 = h->lv[0];
@@ -866,38 +869,72 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     l0.map.defOcc = h->occ0.as<uint32_t>();
     l0.map.defFill = h->fill0.as<uint32_t>();
 
-    // Voxelize, colour and fill, group by group (Voxelizer.cu 2335-2553).
-    for (int g = 0; g < G; ++g) {
-        const GroupArgs& ga = h->gargs[g];
-        if (!ga.overlaps) continue;
-        h->run("group_tiles", [&] {
-            launch_group_tiles(st, P, g, l0.active, l0.list.as<int32_t>(), h->binStart.as<uint32_t>(),
-                               h->work.as<uint4>(), h->workCount.as<uint32_t>(),
-                               h->recCount.as<uint32_t>());
+    return 1;
+}
+
+void gen_group_tiles(vox_ctx* h, int g) {
+    LevelBufs& l0 = h->lv[0];
+    h->run("group_tiles", [&] {
+        launch_group_tiles(h->st, h->P, g, l0.active, l0.list.as<int32_t>(),
+                           h->binStart.as<uint32_t>(), h->work.as<uint4>(),
+                           h->workCount.as<uint32_t>(), h->recCount.as<uint32_t>());
+    });
+}
+
+void gen_voxelize(vox_ctx* h, int g) {
+    LevelBufs& l0 = h->lv[0];
+    h->run("voxelize_g" + std::to_string(g), [&] {
+        launch_voxelize(h->st, h->P, h->gargs[g], g, g == first_overlap(h) ? 1 : 0, l0.active,
+                        l0.list.as<int32_t>(), h->verts.as<float>(), h->fnorm.as<float>(),
+                        h->bbox.as<uint32_t>(), h->binStart.as<uint32_t>(),
+                        h->binTri.as<uint32_t>(), h->occ0.as<uint32_t>(), h->hit0.as<uint32_t>(),
+                        h->sat.as<unsigned long long>(), h->work.as<uint4>(),
+                        h->workCount.as<uint32_t>(), h->rec.as<uint4>(),
+                        h->recCount.as<uint32_t>(), h->persistentBlocks);
+    });
+}
+
+VoxUnitJob gen_voxelize_job(vox_ctx* h, int g) {
+    LevelBufs& l0 = h->lv[0];
+    return voxelize_job(h->P, h->gargs[g], g == first_overlap(h) ? 1 : 0, l0.active,
+                        h->verts.as<float>(), h->fnorm.as<float>(), h->bbox.as<uint32_t>(),
+                        h->binTri.as<uint32_t>(), h->occ0.as<uint32_t>(), h->hit0.as<uint32_t>(),
+                        h->work.as<uint4>(), h->workCount.as<uint32_t>(), h->rec.as<uint4>(),
+                        h->recCount.as<uint32_t>(), h->persistentBlocks);
+}
+
+void gen_group_tail(vox_ctx* h, int g) {
+    LevelBufs& l0 = h->lv[0];
+    const GroupArgs& ga = h->gargs[g];
+    const int init = g == first_overlap(h) ? 1 : 0;
+    hipStream_t st = h->st;
+    const VoxParams& P = h->P;
+    h->run("colour_g" + std::to_string(g), [&] {
+        launch_colour(st, P, ga, init, l0.active, h->rec.as<uint4>(), h->recCount.as<uint32_t>(),
+                      l0.list.as<int32_t>(), h->verts.as<float>(), h->fnorm.as<float>(),
+                      l0.col.as<uint32_t>(), l0.qn.as<uint32_t>(), l0.nrm.as<float>(),
+                      h->fill0.as<uint32_t>(), h->persistentBlocks);
+    });
+    if (ga.textured && ga.terrain) {
+        h->run("fill_terrain", [&] {
+            launch_fill(st, P, g, init, l0.active, l0.list.as<int32_t>(),
+                        l0.tileIdx.as<int32_t>(), h->binStart.as<uint32_t>(),
+                        l0.col.as<uint32_t>(), l0.qn.as<uint32_t>(), l0.nrm.as<float>(),
+                        h->hit0.as<uint32_t>(), h->fill0.as<uint32_t>());
         });
-        h->run("voxelize_g" + std::to_string(g), [&] {
-            launch_voxelize(st, P, ga, g, g == firstOverlap ? 1 : 0, l0.active, l0.list.as<int32_t>(),
-                            h->verts.as<float>(), h->fnorm.as<float>(), h->bbox.as<uint32_t>(),
-                            h->binStart.as<uint32_t>(), h->binTri.as<uint32_t>(),
-                            h->occ0.as<uint32_t>(), h->hit0.as<uint32_t>(), satc,
-                            h->work.as<uint4>(), h->workCount.as<uint32_t>(), h->rec.as<uint4>(),
-                            h->recCount.as<uint32_t>(), h->persistentBlocks);
-        });
-        h->run("colour_g" + std::to_string(g), [&] {
-            launch_colour(st, P, ga, g == firstOverlap ? 1 : 0, l0.active, h->rec.as<uint4>(),
-                          h->recCount.as<uint32_t>(), l0.list.as<int32_t>(), h->verts.as<float>(),
-                          h->fnorm.as<float>(), l0.col.as<uint32_t>(), l0.qn.as<uint32_t>(),
-                          l0.nrm.as<float>(), h->fill0.as<uint32_t>(), h->persistentBlocks);
-        });
-        if (ga.textured && ga.terrain) {
-            h->run("fill_terrain", [&] {
-                launch_fill(st, P, g, g == firstOverlap ? 1 : 0, l0.active, l0.list.as<int32_t>(),
-                            l0.tileIdx.as<int32_t>(), h->binStart.as<uint32_t>(),
-                            l0.col.as<uint32_t>(), l0.qn.as<uint32_t>(), l0.nrm.as<float>(),
-                            h->hit0.as<uint32_t>(), h->fill0.as<uint32_t>());
-            });
-        }
     }
+}
+
+int gen_back(vox_ctx* h, const char* out_dir, int binary) {
+    const int L = h->L;
+    VoxParams& P = h->P;
+    hipStream_t st = h->st;
+    LevelBufs& l0 = h->lv[0];
+    const bool chain = (h->options & VOX_OPT_MIP_CHAIN) != 0;
+    auto srcOf = [&](int k) { return chain ? k - 1 : (k == 1 ? 0 : k - 2); };
+    uint32_t* ctr = h->counters.as<uint32_t>();
+    unsigned long long* satc = h->sat.as<unsigned long long>();
+    const int firstOverlap = first_overlap(h);
     // Mip chain (computeMipMaps, Voxelizer.cu 491-594).
     for (int k = 1; k < L; ++k) {
         if (h->lv[k].aliasOf >= 0) continue;
@@ -1172,6 +1209,28 @@ int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
     return firstOverlap >= 0 ? 1 : -1;
 }
 
+
+}  // namespace
+
+int vox_generate(vox_ctx* h, const char* out_dir, int binary, int compressed) {
+    if (!h || !h->valid || !idle(h)) return 0;
+    h->compressed = compressed != 0;
+    if (h->replay && out_dir) return h->fail("a replayed generate writes no files"), 0;
+    if (h->replay && h->multi)
+        return h->fail("a voxelizer with several chunks orders its bricks on the host's schedule "
+                       "and is not captured in a graph"), 0;
+    if (!h->prepared && !vox_compute_edges_face_normals_and_bounds(h)) return 0;
+    if (!gen_front(h)) return 0;
+    // Voxelize, colour and fill, group by group (Voxelizer.cu 2335-2553).
+    for (int g = 0; g < (int)h->gargs.size(); ++g) {
+        if (!h->gargs[g].overlaps) continue;
+        gen_group_tiles(h, g);
+        gen_voxelize(h, g);
+        gen_group_tail(h, g);
+    }
+    return gen_back(h, out_dir, binary);
+}
+
 const char* vox_get_error(vox_ctx* h) {
     if (!h) return "null handle";
     if (!idle(h)) return "the handle is running vox_generate_async; call vox_wait first";
@@ -1410,6 +1469,14 @@ struct vox_batch {
     std::vector<hipEvent_t> joined;
     hipEvent_t fork = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // One persistent voxelize launch per wave of groups (wave w: the w-th
+    // overlapping group of every job that has one) instead of one launch per
+    // job and group: the job table, the per-job unit prefixes and a queue
+    // word per wave, and the events that order the branches around it.
+    bool persistent = true;
+    unsigned voxBlocks = 0;   // 0: 4 workgroups per CU
+    DevBuf jobTable, unitStart, queue;
+    std::vector<hipEvent_t> preVox, postVox;
     vox_batch_stats stats{};
     std::stringstream err;
     std::string errCache;
@@ -1442,6 +1509,15 @@ struct vox_batch {
         return check(hipMemcpyAsync(out.data(), sums.p, 8 * jobs.size(), hipMemcpyDeviceToHost, st),
                      "checksum") &&
                check(hipStreamSynchronize(st), "checksum");
+    }
+    bool ensure_events(std::vector<hipEvent_t>& v, size_t n) {
+        while (v.size() < n) {
+            hipEvent_t e = nullptr;
+            if (!check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"))
+                return false;
+            v.push_back(e);
+        }
+        return true;
     }
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1504,6 +1580,13 @@ void vox_batch_destroy(vox_batch* b) {
     if (b->fork) (void)hipEventDestroy(b->fork);
     for (auto e : b->joined)
         if (e) (void)hipEventDestroy(e);
+    for (auto e : b->preVox)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : b->postVox)
+        if (e) (void)hipEventDestroy(e);
+    b->jobTable.release();
+    b->unitStart.release();
+    b->queue.release();
     for (auto q : b->branch)
         if (q) (void)hipStreamDestroy(q);
     b->sums.release();
@@ -1530,28 +1613,106 @@ int vox_batch_capture(vox_batch* b) {
     for (size_t j = 0; j < b->jobs.size(); ++j) buffer_list(b->jobs[j], b->bufs[j]);
     // Fork-join capture: every job is a branch of its own, forked from and
     // joined back into the batch stream, so the graph holds independent
-    // chains the device can run side by side.
+    // chains the device can run side by side.  With the persistent voxelize
+    // (the default) the chains meet once per wave of groups: every job's work
+    // list of its w-th overlapping group, then one launch that voxelizes all
+    // of them from a device queue, then every job's colour pass.
+    const size_t J = b->jobs.size();
+    std::vector<std::vector<int>> groupsOf(J);
+    size_t waves = 0, entries = 0;
+    for (size_t j = 0; j < J; ++j) {
+        for (size_t g = 0; g < b->jobs[j]->gargs.size(); ++g)
+            if (b->jobs[j]->gargs[g].overlaps) groupsOf[j].push_back((int)g);
+        waves = std::max(waves, groupsOf[j].size());
+        entries += groupsOf[j].size();
+    }
+    // heaviest jobs first in every wave's table: their long units start early
+    std::vector<size_t> order(J);
+    for (size_t j = 0; j < J; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) {
+        return b->jobs[a]->planPairs > b->jobs[c]->planPairs;
+    });
+    unsigned blocks = b->voxBlocks;
+    if (blocks == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        blocks = 4u * (unsigned)cus;
+    }
+    const bool pers = b->persistent && entries > 0;
+    if (pers && (!b->jobTable.ensure(sizeof(VoxUnitJob) * entries, b->err, "batch job table") ||
+                 !b->unitStart.ensure(4 * (entries + waves), b->err, "batch unit offsets") ||
+                 !b->queue.ensure(4 * waves, b->err, "batch queues") ||
+                 !b->ensure_events(b->preVox, J) || !b->ensure_events(b->postVox, waves)))
+        return 0;
+    std::vector<VoxUnitJob> table;
+    table.reserve(entries);
     if (!b->check(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal), "begin capture"))
         return 0;
     bool ok = b->check(hipEventRecord(b->fork, b->st), "fork");
-    for (size_t j = 0; j < b->jobs.size() && ok; ++j) {
+    std::vector<hipStream_t> own(J, nullptr);
+    for (size_t j = 0; j < J; ++j) {
+        own[j] = b->jobs[j]->st;
+        b->jobs[j]->st = b->branch[j];
+        b->jobs[j]->replay = true;
+        b->jobs[j]->prepared = false;
+    }
+    auto job_fail = [&](size_t j) {
+        b->fail("batch: capture of job " + std::to_string(j) + " failed: " + b->jobs[j]->err.str());
+        return false;
+    };
+    for (size_t j = 0; j < J && ok; ++j) {
         vox_ctx* h = b->jobs[j];
-        hipStream_t own = h->st;
         ok = b->check(hipStreamWaitEvent(b->branch[j], b->fork, 0), "fork");
         if (!ok) break;
-        h->st = b->branch[j];
-        h->replay = true;
-        h->prepared = false;
-        ok = vox_compute_edges_face_normals_and_bounds(h) == 1 &&
-             vox_generate(h, nullptr, 1, h->compressed ? 1 : 0) != 0;
-        h->replay = false;
-        h->st = own;
-        if (!ok) {
-            b->fail("batch: capture of job " + std::to_string(j) + " failed: " + h->err.str());
-            break;
+        if (!pers) {
+            ok = (vox_compute_edges_face_normals_and_bounds(h) == 1 &&
+                  vox_generate(h, nullptr, 1, h->compressed ? 1 : 0) != 0) || job_fail(j);
+            continue;
         }
-        ok = b->check(hipEventRecord(b->joined[j], b->branch[j]), "join") &&
+        ok = (vox_compute_edges_face_normals_and_bounds(h) == 1 && gen_front(h) != 0) ||
+             job_fail(j);
+        if (ok) {
+            std::vector<int> now;
+            for (size_t g = 0; g < h->gargs.size(); ++g)
+                if (h->gargs[g].overlaps) now.push_back((int)g);
+            if (now != groupsOf[j]) ok = b->fail("batch: job " + std::to_string(j) +
+                                                 " changed its groups during the capture");
+        }
+    }
+    for (size_t w = 0; w < waves && ok && pers; ++w) {
+        const size_t first = table.size();
+        for (size_t k = 0; k < J && ok; ++k) {
+            const size_t j = order[k];
+            if (groupsOf[j].size() <= w) continue;
+            vox_ctx* h = b->jobs[j];
+            gen_group_tiles(h, groupsOf[j][w]);
+            table.push_back(gen_voxelize_job(h, groupsOf[j][w]));
+            ok = b->check(hipEventRecord(b->preVox[j], b->branch[j]), "join") &&
+                 b->check(hipStreamWaitEvent(b->st, b->preVox[j], 0), "join");
+        }
+        if (!ok) break;
+        const uint32_t n = (uint32_t)(table.size() - first);
+        launch_voxelize_batch(b->st, b->jobTable.as<VoxUnitJob>() + first, n,
+                              b->unitStart.as<uint32_t>() + first + w, b->queue.as<uint32_t>() + w,
+                              blocks);
+        ok = b->check(hipGetLastError(), "batched voxelize") &&
+             b->check(hipEventRecord(b->postVox[w], b->st), "fork");
+        for (size_t j = 0; j < J && ok; ++j) {
+            if (groupsOf[j].size() <= w) continue;
+            ok = b->check(hipStreamWaitEvent(b->branch[j], b->postVox[w], 0), "fork");
+            if (ok) gen_group_tail(b->jobs[j], groupsOf[j][w]);
+        }
+    }
+    for (size_t j = 0; j < J && ok; ++j) {
+        if (pers) ok = gen_back(b->jobs[j], nullptr, 1) != 0 || job_fail(j);
+        ok = ok && b->check(hipEventRecord(b->joined[j], b->branch[j]), "join") &&
              b->check(hipStreamWaitEvent(b->st, b->joined[j], 0), "join");
+    }
+    for (size_t j = 0; j < J; ++j) {
+        b->jobs[j]->replay = false;
+        b->jobs[j]->st = own[j];
     }
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->st, &g);
@@ -1560,6 +1721,15 @@ int vox_batch_capture(vox_batch* b) {
         return 0;
     }
     b->graph = g;
+    if (!table.empty() &&
+        !b->check(hipMemcpy(b->jobTable.p, table.data(), sizeof(VoxUnitJob) * table.size(),
+                            hipMemcpyHostToDevice),
+                  "batch job table")) {
+        b->drop_graph();
+        return 0;
+    }
+    b->stats.voxelize_launches = pers ? (uint32_t)waves : (uint32_t)entries;
+    b->stats.voxelize_blocks = pers ? blocks : 0u;
     // the capture must not have moved any buffer (sizes are the planned ones)
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         std::vector<void*> now;
@@ -1617,6 +1787,14 @@ int vox_batch_launch(vox_batch* b) {
     b->stats.ms_graph_last = c;
     ++b->stats.launches;
     for (vox_ctx* h : b->jobs) h->haveResult = true;
+    return 1;
+}
+
+int vox_batch_set_voxelize(vox_batch* b, int32_t persistent, uint32_t blocks) {
+    if (!b || !b->valid) return 0;
+    b->persistent = persistent != 0;
+    b->voxBlocks = blocks;
+    b->drop_graph();   // takes effect at the next capture
     return 1;
 }
 
@@ -1695,6 +1873,14 @@ int vox_device_download(void* dst, const void* src, uint64_t offset, uint64_t by
 int vox_device_copy(void* dst, const void* src, uint64_t bytes) {
     if (!dst || !src || (bytes & 15u) || ((uintptr_t)dst & 15u) || ((uintptr_t)src & 15u)) return 0;
     launch_copy_u128(nullptr, src, dst, bytes / 16);
+    return hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess ? 1 : 0;
+}
+
+int vox_device_copy_scattered(void* dst, const void* src, uint64_t bytes) {
+    if (!dst || !src || bytes < 256 || (bytes & (bytes - 1)) || ((uintptr_t)dst & 15u) ||
+        ((uintptr_t)src & 15u))
+        return 0;
+    launch_copy_scattered(nullptr, src, dst, bytes / 256);
     return hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess ? 1 : 0;
 }
 

@@ -47,13 +47,14 @@ constexpr int kTileVox = 512;
 constexpr int kTileWords = 16;      // 512 bits per tile
 constexpr int kBlock = 256;         // threads per workgroup for tile kernels
 constexpr int kFillDepth = 19;      // terrain fill reach (VoxelizerKernels.cu 888)
-// Hit records of the voxelize pass go to kRecSegs segments, workgroup b
-// appending to segment b % kRecSegs with its own counter (no single hot
-// counter).  The voxelize pass takes units of kVoxTiles consecutive tiles of
-// its work list, workgroup b the units b, b + grid, ...: at most
-// ceil(units / grid) units of kVoxTiles tiles of 512 voxels, and a segment
-// serves at most ceil(grid / kRecSegs) workgroups, so this many records
-// always fit.
+// Hit records of the voxelize pass go to kRecSegs segments, unit u of the
+// pass (kVoxTiles consecutive tiles of its work list, at most kVoxTiles *
+// 512 records) appending to segment u % kRecSegs with the segment's own
+// counter (no single hot counter).  A segment thus takes at most
+// ceil(units / kRecSegs) units, and the capacity below (sized as if
+// workgroup b of a grid-stride launch of `grid` workgroups filled segment
+// b % kRecSegs) is never smaller: ceil(units / grid) * ceil(grid / kRecSegs)
+// >= ceil(units / kRecSegs), whichever workgroup runs a unit.
 constexpr uint32_t kRecSegs = 64;
 constexpr int kVoxTiles = 2;
 inline uint64_t rec_seg_cap(uint64_t nActive, uint64_t grid) {
@@ -189,6 +190,36 @@ void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int
                      const uint32_t* binTri, uint32_t* occ, uint32_t* hit,
                      unsigned long long* satCounter, uint4* work, uint32_t* workCount,
                      uint4* rec, uint32_t* recCount, unsigned persistentBlocks);
+// One job of a batched voxelize launch: the arguments of one group's
+// k_voxelize launch (its work list and count written on the device by that
+// job's group_tiles pass).
+struct VoxUnitJob {
+    VoxParams P;
+    GroupArgs G;
+    int32_t init;
+    const uint4* work;
+    const uint32_t* workCount;
+    const float* verts;
+    const float* fnorm;
+    const uint32_t* bbox;
+    const uint32_t* binTri;
+    uint32_t* occ;
+    uint32_t* hit;
+    uint4* rec;
+    uint32_t* recCount;
+    uint64_t segCap;
+};
+// The arguments launch_voxelize would pass for one job (nothing launched).
+VoxUnitJob voxelize_job(const VoxParams& P, const GroupArgs& G, int init, uint64_t nActive,
+                        const float* verts, const float* fnorm, const uint32_t* bbox,
+                        const uint32_t* binTri, uint32_t* occ, uint32_t* hit, uint4* work,
+                        uint32_t* workCount, uint4* rec, uint32_t* recCount,
+                        unsigned persistentBlocks);
+// The voxelize pass of njobs jobs (device table) in one persistent launch of
+// `blocks` workgroups over a device queue; unitStart holds njobs + 1 words,
+// queue one.
+void launch_voxelize_batch(hipStream_t st, const VoxUnitJob* jobs, uint32_t njobs,
+                           uint32_t* unitStart, uint32_t* queue, unsigned blocks);
 void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nActive,
                         const int32_t* list, const uint32_t* binStart, uint4* work,
                         uint32_t* workCount, uint32_t* recCount);
@@ -333,6 +364,9 @@ struct FillSeg {
 void launch_fill_segs(hipStream_t st, const FillSeg* segs, int nseg);
 // Copy n16 16-byte words.
 void launch_copy_u128(hipStream_t st, const void* src, void* dst, uint64_t n16);
+// nblocks 256-byte blocks (a power of two), block i of the launch copying
+// block (i * 0x9E3779B1) mod nblocks (odd multiplier: a permutation)
+void launch_copy_scattered(hipStream_t st, const void* src, void* dst, uint64_t nblocks);
 // dst row k = src row idx[k] - base, w floats per row.
 void launch_gather_rows(hipStream_t st, const float* src, const uint32_t* idx, uint64_t n,
                         uint32_t base, uint32_t w, float* dst);

@@ -676,15 +676,16 @@ __device__ __forceinline__ void stage_pair(const float (&v)[3][3], const float (
     }
 }
 
-__global__ __launch_bounds__(kBlock, 4) void k_voxelize(
-    VoxParams P, GroupArgs G, int g, int init, const uint4* __restrict__ work,
-    const uint32_t* __restrict__ workCount, const int32_t* __restrict__ list,
-    const float* __restrict__ verts, const float* __restrict__ fnorm,
-    const uint32_t* __restrict__ bbox, const uint32_t* __restrict__ binStart,
-    const uint32_t* __restrict__ binTri, uint32_t* __restrict__ occ, uint32_t* __restrict__ hit,
-    uint4* __restrict__ rec, uint32_t* __restrict__ recCount, uint64_t segCap,
-    unsigned long long* __restrict__ satCounter) {
-    // a work unit is up to kVoxTiles tiles of the group's work list
+// One unit of the voxelize pass: up to kVoxTiles consecutive tiles (ui *
+// kVoxTiles ...) of a group's work list of nwork tiles.  Returns the SAT
+// tests it ran.  Called by every thread of a workgroup; its LDS is reused by
+// the workgroup's next unit.
+__device__ __forceinline__ uint32_t voxelize_unit(
+    const VoxParams& P, const GroupArgs& G, int init, const uint4* __restrict__ work,
+    uint32_t nwork, uint32_t ui, const float* __restrict__ verts, const float* __restrict__ fnorm,
+    const uint32_t* __restrict__ bbox, const uint32_t* __restrict__ binTri,
+    uint32_t* __restrict__ occ, uint32_t* __restrict__ hit, uint4* __restrict__ rec,
+    uint32_t* __restrict__ recCount, uint64_t segCap) {
     constexpr int kV = kVoxTiles * kTileVox, kW = kVoxTiles * kTileWords;
     __shared__ uint32_t s_cnt[kV];
     __shared__ int32_t s_lo0[kV], s_lo1[kV], s_lo2[kV], s_hi[kV];
@@ -701,13 +702,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_voxelize(
     __shared__ uint2 s_bin[kVoxTiles];   // per tile of the unit: first bin entry, entries
     const int tid = threadIdx.x;
     if (tid >= 1 && tid <= 64) s_magic[tid] = (65536u + (uint32_t)tid - 1u) / (uint32_t)tid;
-    const uint32_t nwork = *workCount;
-    const uint32_t nunits = (nwork + kVoxTiles - 1) / kVoxTiles;
     uint32_t localTests = 0;
-    // Persistent grid over units of kVoxTiles tiles that hold triangles of
-    // this group: the fixed cost of a round (staging, scan, barriers, record
-    // reservation) is paid once per unit.
-    for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
     const uint32_t w0 = ui * kVoxTiles;
     const uint32_t nt = nwork - w0 < (uint32_t)kVoxTiles ? nwork - w0 : (uint32_t)kVoxTiles;
     // work entries: (active tile, tile index, first bin entry, entries)
@@ -946,7 +941,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_voxelize(
     if (tid == 0) {
         uint32_t acc = 0;
         for (int w = 0; w < kW; ++w) { const uint32_t c = s_hitoff[w]; s_hitoff[w] = acc; acc += c; }
-        const uint32_t seg = blockIdx.x % kRecSegs;
+        const uint32_t seg = ui % kRecSegs;
         s_recBase = acc ? (uint32_t)(seg * segCap) + atomicAdd(&recCount[seg], acc) : 0u;
     }
     __syncthreads();
@@ -974,13 +969,89 @@ __global__ __launch_bounds__(kBlock, 4) void k_voxelize(
             rec[r + 1] = make_uint4((uint32_t)sl[1], (uint32_t)sl[2], (uint32_t)sl[3], X | (Y << 16));
         }
     }
-    __syncthreads();   // LDS is reused by the next tile
-    }
+    __syncthreads();   // LDS is reused by the next unit
+    return localTests;
+}
+
+__global__ __launch_bounds__(kBlock, 4) void k_voxelize(
+    VoxParams P, GroupArgs G, int g, int init, const uint4* __restrict__ work,
+    const uint32_t* __restrict__ workCount, const int32_t* __restrict__ list,
+    const float* __restrict__ verts, const float* __restrict__ fnorm,
+    const uint32_t* __restrict__ bbox, const uint32_t* __restrict__ binStart,
+    const uint32_t* __restrict__ binTri, uint32_t* __restrict__ occ, uint32_t* __restrict__ hit,
+    uint4* __restrict__ rec, uint32_t* __restrict__ recCount, uint64_t segCap,
+    unsigned long long* __restrict__ satCounter) {
+    __shared__ uint32_t s_sum[kBlock / 64];
+    const uint32_t nwork = *workCount;
+    const uint32_t nunits = (nwork + kVoxTiles - 1) / kVoxTiles;
+    uint32_t localTests = 0;
+    // Persistent grid over units of kVoxTiles tiles that hold triangles of
+    // this group: the fixed cost of a round (staging, scan, barriers, record
+    // reservation) is paid once per unit.
+    for (uint32_t ui = blockIdx.x; ui < nunits; ui += gridDim.x)
+        localTests += voxelize_unit(P, G, init, work, nwork, ui, verts, fnorm, bbox, binTri, occ,
+                                    hit, rec, recCount, segCap);
     if (satCounter) {
         uint32_t tot;
-        block_excl_scan(localTests, s_w, &tot);
-        if (tid == 0) atomicAdd(satCounter, (unsigned long long)tot);
+        block_excl_scan(localTests, s_sum, &tot);
+        if (threadIdx.x == 0) atomicAdd(satCounter, (unsigned long long)tot);
     }
+}
+
+// The voxelize pass of many voxelizations in one persistent launch: the
+// units of every job's group (job j owns units unitStart[j] ..
+// unitStart[j+1] - 1) are handed out one at a time from a device queue, so
+// workgroups that finish a small job's units go on with any other job's.
+// Jobs are listed heaviest first, so the long units start early.
+__global__ __launch_bounds__(kBlock, 4) void k_voxelize_batch(
+    const VoxUnitJob* __restrict__ jobs, uint32_t njobs, const uint32_t* __restrict__ unitStart,
+    uint32_t* __restrict__ queue) {
+    __shared__ uint32_t s_u, s_j;
+    const uint32_t total = unitStart[njobs];
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t u = atomicAdd(queue, 1u);
+            uint32_t lo = 0, hi = njobs;   // last job whose first unit is <= u
+            if (u < total)
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (unitStart[mid] <= u) lo = mid; else hi = mid;
+                }
+            s_u = u;
+            s_j = lo;
+        }
+        __syncthreads();
+        const uint32_t u = __builtin_amdgcn_readfirstlane(s_u);
+        const uint32_t j = __builtin_amdgcn_readfirstlane(s_j);
+        __syncthreads();   // s_u / s_j are written again only after every thread read them
+        if (u >= total) break;
+        const VoxUnitJob& J = jobs[j];
+        voxelize_unit(J.P, J.G, J.init, J.work, *J.workCount, u - unitStart[j], J.verts, J.fnorm,
+                      J.bbox, J.binTri, J.occ, J.hit, J.rec, J.recCount, J.segCap);
+    }
+}
+
+// Units per job (from the work counts the jobs' group_tiles passes wrote),
+// as a prefix, and the queue reset: one workgroup.
+__global__ __launch_bounds__(kBlock) void k_voxelize_batch_units(const VoxUnitJob* __restrict__ jobs,
+                                                                 uint32_t njobs,
+                                                                 uint32_t* __restrict__ unitStart,
+                                                                 uint32_t* __restrict__ queue) {
+    __shared__ uint32_t s_w[kBlock / 64];
+    __shared__ uint32_t s_carry;
+    if (threadIdx.x == 0) { s_carry = 0; *queue = 0; }
+    __syncthreads();
+    for (uint32_t b = 0; b < njobs; b += kBlock) {
+        const uint32_t j = b + threadIdx.x;
+        const uint32_t n = j < njobs ? (*jobs[j].workCount + kVoxTiles - 1) / kVoxTiles : 0u;
+        uint32_t tot;
+        const uint32_t off = block_excl_scan(n, s_w, &tot);
+        if (j < njobs) unitStart[j] = s_carry + off;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) unitStart[njobs] = s_carry;
 }
 
 // Colour pass over the hit records of one group: ComputeColorsAndNormals /
@@ -2156,6 +2227,19 @@ __global__ void k_copy_u128(const uint4* __restrict__ src, uint4* __restrict__ d
         dst[i] = src[i];
 }
 
+// The same copy over 256-byte blocks in a scattered order: 16 lanes per
+// block, block i of the grid-stride walk copying block (i * odd) mod n.
+__global__ void k_copy_scattered(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                 uint64_t nblocks) {
+    const uint64_t lanes = nblocks * 16;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < lanes;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = ((i >> 4) * 0x9E3779B1ull) & (nblocks - 1);
+        const uint64_t w = b * 16 + (i & 15u);
+        dst[w] = src[w];
+    }
+}
+
 struct FillSegs {
     FillSeg s[4];
     int k;
@@ -2532,6 +2616,12 @@ void launch_copy_u128(hipStream_t st, const void* src, void* dst, uint64_t n16) 
                                          n16);
 }
 
+void launch_copy_scattered(hipStream_t st, const void* src, void* dst, uint64_t nblocks) {
+    if (nblocks == 0) return;
+    k_copy_scattered<<<4096, kBlock, 0, st>>>(static_cast<const uint4*>(src),
+                                              static_cast<uint4*>(dst), nblocks);
+}
+
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
     if (n == 0) return;
     uint64_t blocks = ((n + 3) / 4 + kBlock - 1) / kBlock;
@@ -2594,6 +2684,36 @@ void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int
     k_voxelize<<<blocks, kBlock, 0, st>>>(P, G, g, init, work, workCount, list, verts, fnorm, bbox,
                                           binStart, binTri, occ, hit, rec, recCount,
                                           rec_seg_cap(nActive, blocks), satCounter);
+}
+VoxUnitJob voxelize_job(const VoxParams& P, const GroupArgs& G, int init, uint64_t nActive,
+                        const float* verts, const float* fnorm, const uint32_t* bbox,
+                        const uint32_t* binTri, uint32_t* occ, uint32_t* hit, uint4* work,
+                        uint32_t* workCount, uint4* rec, uint32_t* recCount,
+                        unsigned persistentBlocks) {
+    VoxUnitJob j{};
+    j.P = P;
+    j.G = G;
+    j.init = init;
+    j.work = work;
+    j.workCount = workCount;
+    j.verts = verts;
+    j.fnorm = fnorm;
+    j.bbox = bbox;
+    j.binTri = binTri;
+    j.occ = occ;
+    j.hit = hit;
+    j.rec = rec;
+    j.recCount = recCount;
+    // the job's record segments are sized for its own launch shape; units
+    // go to segment (unit % kRecSegs), which that sizing always holds
+    j.segCap = rec_seg_cap(nActive, vox_grid(nActive, persistentBlocks));
+    return j;
+}
+void launch_voxelize_batch(hipStream_t st, const VoxUnitJob* jobs, uint32_t njobs,
+                           uint32_t* unitStart, uint32_t* queue, unsigned blocks) {
+    if (njobs == 0 || blocks == 0) return;
+    k_voxelize_batch_units<<<1, kBlock, 0, st>>>(jobs, njobs, unitStart, queue);
+    k_voxelize_batch<<<blocks, kBlock, 0, st>>>(jobs, njobs, unitStart, queue);
 }
 void launch_group_tiles(hipStream_t st, const VoxParams& P, int g, uint64_t nActive,
                         const int32_t* list, const uint32_t* binStart, uint4* work,

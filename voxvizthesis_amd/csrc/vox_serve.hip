@@ -154,20 +154,17 @@ __device__ __forceinline__ uint2 fetch_node(const uint32_t* __restrict__ tex, co
     return make_uint2(tex[2 * t], tex[2 * t + 1]);
 }
 
-__global__ void __launch_bounds__(256)
-k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __restrict__ pool,
-          const float* __restrict__ jitter, uint32_t* __restrict__ rgbaOut,
-          float4* __restrict__ rgbaF, uint32_t* __restrict__ usageOut) {
-    __shared__ float u8[256];
-    u8[threadIdx.x] = (float)threadIdx.x / 255.0f;  // blockDim.x == 256
-    __syncthreads();
-    const int i = U.rx0 + (int)blockIdx.x * 16 + (int)(threadIdx.x & 15);
-    const int j = U.ry0 + (int)blockIdx.y * 16 + (int)(threadIdx.x >> 4);
-    if (i >= U.rx1 || j >= U.ry1) return;
-    uint32_t usage[kMaxUsage];
+// One pixel's ray through one octree (GigaVoxels.frag main): its FragColor
+// in dst and its node usage list in usage (both start at zero here).
+// Returns whether the ray met the octree's box.
+__device__ __forceinline__ bool cast_pixel(const RayUniforms& U, const uint32_t* __restrict__ tex,
+                                           const uint32_t* __restrict__ pool,
+                                           const float* __restrict__ jitter,
+                                           const float* __restrict__ u8, int i, int j,
+                                           float (&dst)[4], uint32_t (&usage)[kMaxUsage]) {
 #pragma unroll
     for (int k = 0; k < kMaxUsage; ++k) usage[k] = 0u;
-    float dst[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    dst[0] = dst[1] = dst[2] = dst[3] = 0.0f;
 
     // the pixel's ray, world -> unit volume
     const float sx = (2.0f * ((float)i + 0.5f)) / (float)U.W - 1.0f;
@@ -351,6 +348,22 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
             }
         }
     }
+    return hit;
+}
+
+__global__ void __launch_bounds__(256)
+k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __restrict__ pool,
+          const float* __restrict__ jitter, uint32_t* __restrict__ rgbaOut,
+          float4* __restrict__ rgbaF, uint32_t* __restrict__ usageOut) {
+    __shared__ float u8[256];
+    u8[threadIdx.x] = (float)threadIdx.x / 255.0f;  // blockDim.x == 256
+    __syncthreads();
+    const int i = U.rx0 + (int)blockIdx.x * 16 + (int)(threadIdx.x & 15);
+    const int j = U.ry0 + (int)blockIdx.y * 16 + (int)(threadIdx.x >> 4);
+    if (i >= U.rx1 || j >= U.ry1) return;
+    uint32_t usage[kMaxUsage];
+    float dst[4];
+    cast_pixel(U, tex, pool, jitter, u8, i, j, dst, usage);
     const uint64_t pix = (uint64_t)j * U.W + i;
     if (rgbaOut) {
         uint32_t o = 0;
@@ -366,6 +379,95 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
     u4[0] = make_uint4(usage[0], usage[1], usage[2], usage[3]);
     u4[1] = make_uint4(usage[4], usage[5], usage[6], usage[7]);
     u4[2] = make_uint4(usage[8], usage[9], usage[10], usage[11]);
+}
+
+// One octree of a front-to-back frame over several octrees: its uniforms
+// and buffers, and the request list its rays append to.
+struct TreeCast {
+    RayUniforms U;
+    const uint32_t* tex;
+    const uint32_t* pool;
+    uint32_t* stamp;              // node texel -> tag of the frame that listed it
+    uint32_t* list;
+    unsigned long long* times;
+    uint32_t* counts;             // appended (capped), distinct
+    uint32_t tag, cap;
+};
+
+// Append a wave's node ids of one usage slot to the octree's request list:
+// lanes holding the same id elect one leader (ballot loop), leaders claim
+// the node texel's frame stamp, the first claimant appends the id.
+__device__ __forceinline__ void append_requests(const TreeCast& tc, uint32_t id) {
+    uint64_t pending = __ballot(id != 0u);
+    bool leader = false;
+    while (pending) {
+        const int l = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lid = __shfl(id, l);
+        const uint64_t same = __ballot(id == lid);
+        if ((int)(threadIdx.x & 63) == l) leader = true;
+        pending &= ~same;
+    }
+    if (!leader) return;
+    const uint32_t x = ((id >> 20) & 1023u) - 1u;
+    const uint32_t y = ((id >> 10) & 1023u) - 1u;
+    const uint32_t z = (id & 1023u) - 1u;
+    const uint32_t* nd = tc.U.ndim;
+    if (x >= nd[0] || y >= nd[1] || z >= nd[2]) return;
+    const uint64_t t = ((uint64_t)z * nd[1] + y) * nd[0] + x;
+    if (tc.stamp[t] != tc.tag && atomicExch(&tc.stamp[t], tc.tag) != tc.tag) {
+        const uint32_t k = atomicAdd(&tc.counts[1], 1u);
+        if (k < tc.cap) {
+            tc.list[k] = id;
+            tc.times[k] = (unsigned long long)wall_clock64();  // emitted
+            atomicAdd(&tc.counts[0], 1u);
+        }
+    }
+}
+
+// Front to back over n octrees listed in an order every ray meets them in
+// (boxes that tile space; the caller's order): each pixel casts its ray
+// through every octree whose pixel rectangle holds it, exactly as one
+// octree's frame does (cast_pixel), and accumulates the casts' FragColors
+// as the renderer's far-to-near blend would (glBlendFunc(GL_SRC_ALPHA,
+// GL_ONE_MINUS_SRC_ALPHA) per channel, alpha included): out = sum over
+// octrees of T * src * src.a, T the product of (1 - src.a) of the octrees
+// before.  A pixel stops once T <= tStop (0: never), a wave once all its
+// pixels have; the octrees it skips list no requests for it.
+__global__ void __launch_bounds__(256)
+k_raycast_multi(const TreeCast* __restrict__ trees, uint32_t n, const float* __restrict__ jitter,
+                int32_t W, int32_t H, float tStop, float4* __restrict__ frame) {
+    __shared__ float u8[256];
+    u8[threadIdx.x] = (float)threadIdx.x / 255.0f;  // blockDim.x == 256
+    __syncthreads();
+    const int i = (int)blockIdx.x * 16 + (int)(threadIdx.x & 15);
+    const int j = (int)blockIdx.y * 16 + (int)(threadIdx.x >> 4);
+    const bool in = i < W && j < H;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float T = 1.0f;
+    for (uint32_t k = 0; k < n; ++k) {
+        const bool want = in && T > tStop;
+        if (!__any(want)) break;
+        const TreeCast& tc = trees[k];
+        const RayUniforms& U = tc.U;
+        uint32_t usage[kMaxUsage];
+        float dst[4];
+        bool act = want && i >= U.rx0 && i < U.rx1 && j >= U.ry0 && j < U.ry1;
+        if (act) {
+            act = cast_pixel(U, tc.tex, tc.pool, jitter, u8, i, j, dst, usage);
+        } else {
+#pragma unroll
+            for (int q = 0; q < kMaxUsage; ++q) usage[q] = 0u;
+        }
+        if (act) {
+            const float a = dst[3];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[c] = acc[c] + T * (dst[c] * a);
+            T = T * (1.0f - a);
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxUsage; ++q) append_requests(tc, usage[q]);
+    }
+    if (in) frame[(uint64_t)j * W + i] = make_float4(acc[0], acc[1], acc[2], acc[3]);
 }
 
 // Distinct node ids of the frame's pixel rectangle: lanes of a wave
@@ -457,6 +559,13 @@ struct vox_raycaster {
     uint32_t ndim[3] = {0, 0, 0};
     hipStream_t st = nullptr;  // the ray caster's own stream
     DevBuf rgba, rgbaF, usage, req, reqTime, counts, stamp, jitter;
+    DevBuf multiTable;          // octree table of front-to-back frames queued here
+    // its pinned host staging, two buffers in turn, each reused only after
+    // the copy that read it last has run
+    void* hostTable[2] = {nullptr, nullptr};
+    size_t hostCap[2] = {0, 0};
+    hipEvent_t tableCopied[2] = {nullptr, nullptr};
+    int tableNext = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     hipEvent_t done = nullptr;  // the last frame's request list is complete
     hipEvent_t read = nullptr;  // a composite has read the last frame
@@ -646,6 +755,11 @@ void vox_raycaster_destroy(vox_raycaster* rc) {
     rc->counts.release();
     rc->stamp.release();
     rc->jitter.release();
+    rc->multiTable.release();
+    for (int b = 0; b < 2; ++b) {
+        if (rc->hostTable[b]) (void)hipHostFree(rc->hostTable[b]);
+        if (rc->tableCopied[b]) (void)hipEventDestroy(rc->tableCopied[b]);
+    }
     if (rc->st) (void)hipStreamDestroy(rc->st);
     delete rc;
 }
@@ -729,6 +843,108 @@ int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycas
         return 0;
     rc->timingPending = true;
     rc->countsPending = true;
+    return 1;
+}
+
+int vox_raycast_frame_multi(vox_raycaster* const* rcs, uint32_t n, const vox_camera* cam,
+                            const vox_raycast_params* prms, float* frame, float t_stop) {
+    if (!rcs || n == 0 || !cam || !prms || !frame || !(t_stop >= 0.0f)) return 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!rcs[k] || !rcs[k]->valid) return 0;
+        for (uint32_t m = 0; m < k; ++m)
+            if (rcs[m] == rcs[k]) return rcs[k]->fail("raycast multi: a ray caster appears twice"), 0;
+    }
+    vox_raycaster* r0 = rcs[0];
+    if (cam->width <= 0 || cam->height <= 0) return r0->fail("raycast multi: empty viewport"), 0;
+    hipStream_t st = r0->st;
+    std::vector<TreeCast> table(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        vox_raycaster* rc = rcs[k];
+        if (cam->width > rc->maxW || cam->height > rc->maxH)
+            return rc->fail("raycast multi: viewport exceeds the ray caster's frame size"), 0;
+        if (prms[k].jitter_seed != prms[0].jitter_seed)
+            return rc->fail("raycast multi: the octrees' jitter seeds differ"), 0;
+        OctreeRefs o;
+        if (!octree_refs(rc->tree, o)) return rc->fail("raycast multi: octree unavailable"), 0;
+        TreeCast& tc = table[k];
+        if (!ray_uniforms(*cam, prms[k], o.levels, o.B, o.poolDims, o.poolWide, o.ndim, tc.U))
+            return rc->fail("raycast multi: empty volume box, ray step below 1e-5 or a pixel "
+                            "rectangle outside the viewport"), 0;
+        settle_timing(rc);
+        // this octree's node texture and pool as their last writer left them,
+        // and its own stream's earlier work
+        if (!rc->check(hipStreamWaitEvent(st, o.ready, 0), "wait for the node texture") ||
+            (rc != r0 && !rc->check(hipStreamWaitEvent(st, rc->done, 0), "wait for a frame")))
+            return 0;
+        if (rc->readPending && !rc->check(hipStreamWaitEvent(st, rc->read, 0), "wait for a composite"))
+            return 0;
+        rc->readPending = false;
+        const uint64_t ntex = (uint64_t)rc->ndim[0] * rc->ndim[1] * rc->ndim[2];
+        if (++rc->tag == 0) {  // stamp wrap: clear
+            launch_fill_u32(st, rc->stamp.as<uint32_t>(), ntex, 0u);
+            rc->tag = 1;
+        }
+        launch_fill_u32(st, rc->counts.as<uint32_t>(), 2, 0u);
+        tc.tex = o.tex;
+        tc.pool = o.poolCol;
+        tc.stamp = rc->stamp.as<uint32_t>();
+        tc.list = rc->req.as<uint32_t>();
+        tc.times = rc->reqTime.as<unsigned long long>();
+        tc.counts = rc->counts.as<uint32_t>();
+        tc.tag = rc->tag;
+        tc.cap = rc->maxReq;
+        rc->W = cam->width;
+        rc->H = cam->height;
+        rc->rect[0] = tc.U.rx0;
+        rc->rect[1] = tc.U.ry0;
+        rc->rect[2] = tc.U.rx1;
+        rc->rect[3] = tc.U.ry1;
+    }
+    if (!r0->haveJitter || r0->jitterSeed != prms[0].jitter_seed) {
+        k_jitter<<<4, 256, 0, st>>>(prms[0].jitter_seed, r0->jitter.as<float>());
+        r0->jitterSeed = prms[0].jitter_seed;
+        r0->haveJitter = true;
+    }
+    // the table is copied in stream order, after the previous frame's kernel
+    // has read the device copy, from a pinned staging buffer whose last copy
+    // has run
+    const size_t tbytes = sizeof(TreeCast) * n;
+    const int b = r0->tableNext;
+    r0->tableNext ^= 1;
+    if (!r0->tableCopied[b] &&
+        !r0->check(hipEventCreateWithFlags(&r0->tableCopied[b], hipEventDisableTiming),
+                   "hipEventCreate"))
+        return 0;
+    if (!r0->check(hipEventSynchronize(r0->tableCopied[b]), "octree table staging")) return 0;
+    if (r0->hostCap[b] < tbytes) {
+        if (r0->hostTable[b]) (void)hipHostFree(r0->hostTable[b]);
+        r0->hostTable[b] = nullptr;
+        r0->hostCap[b] = 0;
+        if (!r0->check(hipHostMalloc(&r0->hostTable[b], tbytes, hipHostMallocDefault),
+                       "pinned octree table"))
+            return 0;
+        r0->hostCap[b] = tbytes;
+    }
+    std::memcpy(r0->hostTable[b], table.data(), tbytes);
+    if (!r0->multiTable.ensure(tbytes, r0->err, "octree table") ||
+        !r0->check(hipMemcpyAsync(r0->multiTable.p, r0->hostTable[b], tbytes,
+                                  hipMemcpyHostToDevice, st),
+                   "octree table") ||
+        !r0->check(hipEventRecord(r0->tableCopied[b], st), "event record"))
+        return 0;
+    (void)hipEventRecord(r0->ev[0], st);
+    dim3 grid((unsigned)(cam->width + 15) / 16, (unsigned)(cam->height + 15) / 16);
+    k_raycast_multi<<<grid, 256, 0, st>>>(r0->multiTable.as<TreeCast>(), n, r0->jitter.as<float>(),
+                                          cam->width, cam->height, t_stop,
+                                          reinterpret_cast<float4*>(frame));
+    (void)hipEventRecord(r0->ev[1], st);
+    (void)hipEventRecord(r0->ev[2], st);
+    if (!r0->check(hipGetLastError(), "raycast multi kernel")) return 0;
+    r0->timingPending = true;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!rcs[k]->check(hipEventRecord(rcs[k]->done, st), "event record")) return 0;
+        rcs[k]->countsPending = true;
+    }
     return 1;
 }
 

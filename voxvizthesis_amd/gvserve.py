@@ -155,6 +155,21 @@ def composite(raycasters, dst_ptr: int, width: int, height: int, clear: bool = T
         raise RuntimeError("vox_composite failed")
 
 
+def raycast_front_to_back(raycasters, cam: _lib.VoxCamera, prms, dst_ptr: int,
+                          t_stop: float = 0.0):
+    """vox_raycast_frame_multi: one front-to-back frame over the casters'
+    octrees (listed in an order every ray meets them in), one parameter set
+    per caster, into the RGBA32F device buffer dst_ptr (every pixel
+    written); each caster gets its octree's request list."""
+    L = _lib.load()
+    n = len(raycasters)
+    arr = (C.c_void_p * max(n, 1))(*[r._h for r in raycasters])
+    parr = (_lib.VoxRaycastParams * max(n, 1))(*prms)
+    if not L.vox_raycast_frame_multi(arr, n, C.byref(cam), parr, dst_ptr, float(t_stop)):
+        raise RuntimeError("vox_raycast_frame_multi failed: " +
+                           (raycasters[0].getErrorMessage() if n else "no octrees"))
+
+
 class BrickServer:
     def __init__(self, octree, max_uploads_per_frame=0, _handle=None):
         self._L = _lib.load()

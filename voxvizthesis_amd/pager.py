@@ -33,7 +33,8 @@ import numpy as np
 
 from . import _lib
 from .gvpool import BrickPool, OctTree
-from .gvserve import BrickServer, RayCaster, box_rect, camera, composite, raycast_params
+from .gvserve import (BrickServer, RayCaster, box_rect, camera, composite, raycast_front_to_back,
+                      raycast_params)
 from .tiler import PagedOctTreeGenerator
 from .voxelizer import Voxelizer
 
@@ -106,6 +107,25 @@ def sphere_in_frustum(planes: np.ndarray, center, radius: float):
     return True, d
 
 
+def front_to_back(trees, pos):
+    """The trees in an order every ray from pos meets them in.  Tiles sit on
+    a regular grid of columns in x and y, each column a contiguous stack of
+    boxes in z: along a ray the column indices move away from the camera's
+    column one step at a time (so the L1 column distance grows), and inside a
+    column the boxes are met in order of their distance from the camera's
+    height.  Sorting by (column distance, height distance) therefore orders
+    every ray's boxes front to back."""
+    def key(tr):
+        sx = float(tr.box_max[0] - tr.box_min[0])
+        sy = float(tr.box_max[1] - tr.box_min[1])
+        dx = math.floor((float(pos[0]) - float(tr.box_min[0])) / sx)
+        dy = math.floor((float(pos[1]) - float(tr.box_min[1])) / sy)
+        z = float(pos[2])
+        dz = max(float(tr.box_min[2]) - z, z - float(tr.box_max[2]), 0.0)
+        return (abs(dx) + abs(dy), dz)
+    return sorted(trees, key=key)
+
+
 def load_order(requests):
     """SortByDistAndFrameIndex (GigaVoxelsDatabasePager.cpp 220-230): most
     recently accessed first, then nearest.  requests: [(tree, last access,
@@ -147,9 +167,14 @@ class TilePager:
                  max_uploads: int = 65536, max_loading: int = 4, stale_frames: int = 100,
                  streams: int = 1, max_width: int = 1920, max_height: int = 1080,
                  near: float = 1e-3, far: float = 100.0, options: int = 0,
-                 serve_group: int = 3):
+                 serve_group: int = 3, f2b_stop=None):
         self._L = _lib.load()
         self.meshes = list(meshes)
+        # f2b_stop: None casts and blends every tree separately, far to near
+        # (the reference's renderer); a number casts each stream's trees in
+        # one front-to-back pass, a pixel skipping the trees behind it once
+        # its transmittance is at most f2b_stop (0: none skipped)
+        self.f2b_stop = f2b_stop
         self.N, self.B = int(N), int(B)
         self.gen = PagedOctTreeGenerator(self.meshes, voxel_size, self.N, self.B)
         self.max_loading, self.stale = int(max_loading), int(stale_frames)
@@ -363,6 +388,8 @@ class TilePager:
             tr.requested_frame = f
             self._start_load(tr)
             loading += 1
+        if self.f2b_stop is not None:
+            return self._frame_f2b(cams, drawn, prm_extra, serve)
         # ray casts of every stream's visible trees.  With serve, a server
         # frame follows every serve_group casts: their requests are admitted
         # while the next trees are cast (which wait for that server frame's
@@ -400,6 +427,38 @@ class TilePager:
             self._cast = rcs
             if serve:
                 self.serve()
+        self.frame_index += 1
+        return out
+
+    def _frame_f2b(self, cams, drawn, prm_extra, serve: bool):
+        """Per stream one front-to-back cast over its visible trees (nearest
+        first), straight into the stream's frame; then one server frame for
+        every stream's request lists."""
+        f = self.frame_index
+        rcs, out = [], []
+        for k, (cm, trees) in enumerate(zip(cams, drawn)):
+            row, cast, prms = [], [], []
+            for tr in front_to_back(trees, cm["position"]):
+                rect = box_rect(cm, tr.box_min, tr.box_max)
+                if rect == ():
+                    continue                              # nothing on screen
+                rc = tr.rcs[k]
+                prms.append(raycast_params(tr.box_min, tr.box_max, rect=rect,
+                                           **(prm_extra or {})))
+                cast.append(rc)
+                row.append((tr, rc))
+            cam = camera(cm["position"], cm["look"], cm["up"], cm["fov_y_deg"], cm["width"],
+                         cm["height"], f)
+            if cast:
+                raycast_front_to_back(cast, cam, prms, self.frames[k].ptr, self.f2b_stop)
+            else:
+                composite([], self.frames[k].ptr, cm["width"], cm["height"], clear=True)
+            rcs += cast
+            self._frame_dims[k] = (int(cm["width"]), int(cm["height"]))
+            out.append(row)
+        self._cast = rcs
+        if serve:
+            self.serve()
         self.frame_index += 1
         return out
 

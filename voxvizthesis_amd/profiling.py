@@ -7,7 +7,9 @@
 `traffic` turns the FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch) into
 HBM bytes per launch.  With --known-copy BYTES (a run of bench.py
 --copy-calibration) each counter is scaled by the ratio it shows on the plain
-copy kernel k_copy_u128, which reads and writes exactly BYTES.  Otherwise
+copy kernel k_copy_u128, which reads and writes exactly BYTES; the ratios the
+same copy shows with its 256-byte blocks in a scattered order
+(k_copy_scattered) are reported beside them.  Otherwise
 the WRITE_SIZE scale comes from the largest k_fill_u32 dispatch, which
 writes exactly the S^3 occupancy bitmap (--known-write), and applies to
 FETCH_SIZE too unless --known-fetch names the bytes the largest
@@ -64,6 +66,12 @@ def traffic(fetch_csv, write_csv, known_fetch, known_write, steps=1, known_copy=
         fw = known_write / max(w["k_fill_u32"])
         ff = known_fetch / max(f["k_scan_reduce"]) if known_fetch else fw
     res = {"fetch_scale": ff, "write_scale": fw, "kernels": {}}
+    if known_copy and f.get("k_copy_scattered") and w.get("k_copy_scattered"):
+        # the second reference point: the same bytes copied in 256-byte
+        # blocks in a scattered order; its scales bound how far the plain
+        # copy's scales carry over to kernels that gather
+        res["scattered_fetch_scale"] = known_copy / max(f["k_copy_scattered"])
+        res["scattered_write_scale"] = known_copy / max(w["k_copy_scattered"])
     for k in sorted(set(f) | set(w)):
         fs = f.get(k, [])
         ws = w.get(k, [])
@@ -99,6 +107,9 @@ def main():
         for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["bytes_per_launch"]):
             print(f"{k:20s} launches={v['launches']:3d} bytes/launch={v['bytes_per_launch']:.4g}")
         print(f"scales: fetch {res['fetch_scale']:.4f} write {res['write_scale']:.4f}")
+        if "scattered_fetch_scale" in res:
+            print(f"scattered copy: fetch {res['scattered_fetch_scale']:.4f} "
+                  f"write {res['scattered_write_scale']:.4f}")
 
 
 if __name__ == "__main__":

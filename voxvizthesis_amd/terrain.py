@@ -153,8 +153,8 @@ class TerrainPager(TilePager):
     range, each built on the device and voxelized when first seen.
 
     pool_slots as for TilePager (one number: a wide pool of that many
-    slots).  The frame loop, culling, load order, unloading and serving are
-    TilePager's."""
+    slots).  The frame loop, culling, load order, unloading and serving
+    (f2b_stop included) are TilePager's."""
 
     def __init__(self, cells: int, N: int, B: int, pool_slots, seed: int = 6,
                  ridged_seed: int = 7, ridged_amp: float = 0.05, rocks: int = 100000,
@@ -162,7 +162,7 @@ class TerrainPager(TilePager):
                  max_uploads: int = 65536, max_loading: int = 4, stale_frames: int = 100,
                  streams: int = 2, max_width: int = 2560, max_height: int = 1440,
                  near: float = 1.0, far: float = 40000.0, options: int = 0,
-                 serve_group: int = 3):
+                 serve_group: int = 3, f2b_stop=None):
         from concurrent.futures import ThreadPoolExecutor
         self._L = _lib.load()
         self.meshes = []
@@ -170,6 +170,7 @@ class TerrainPager(TilePager):
         self.vs = np.full(3, 1.0, np.float32)
         self.max_loading, self.stale = int(max_loading), int(stale_frames)
         self.serve_group = int(serve_group)
+        self.f2b_stop = f2b_stop
         self.streams, self.W, self.H = int(streams), int(max_width), int(max_height)
         self.near, self.far, self.options = float(near), float(far), int(options)
         self.frame_index = 0
