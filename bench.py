@@ -277,9 +277,12 @@ def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool
 
 def cast_mode(f2b_stop):
     if f2b_stop is None:
-        return "every tree cast alone, blended far to near (the reference's renderer)"
-    return (f"one front-to-back pass over the visible trees per stream, a pixel skipping "
-            f"the trees behind it once its transmittance is <= {f2b_stop:.5f}")
+        return ("every tree cast alone over its screen rectangle, blended far to near (the "
+                "reference's renderer); a server frame after every serve_group casts")
+    return ("the visible trees cast front to back in passes of serve_group, serve_group, then "
+            "doubling numbers of trees, each resuming the frame where the nearer trees left "
+            f"it, a pixel skipping the trees behind it once its transmittance is <= "
+            f"{f2b_stop:.5f}; a server frame after every pass")
 
 
 def scale_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool_gb: float,
@@ -327,10 +330,10 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
     """Batched voxelization: n_jobs independent meshes (half fBm heightfield
     tiles, half random soups, triangle counts log-uniform 2K..tmax, grids from
     `grids`) captured into hipGraphs.  The autotuner picks each job's brick
-    edge (8/16/32) and persistent launch shape by timing eager runs.  Every
-    graph voxelizes all of its jobs in one persistent launch over a device
-    work queue (the same graph with each job's own voxelize launches is timed
-    beside it).  Jobs are admitted by free device memory: while they fit, they join the current
+    edge (8/16/32) and persistent launch shape by timing eager runs, and each
+    graph's voxelize launch structure by timing replays: every job's own
+    launches, or one persistent launch over all jobs from a device work
+    queue.  Jobs are admitted by free device memory: while they fit, they join the current
     graph; when setting one up runs out of memory, the resident jobs are
     captured into one graph, timed and freed, and the next graph starts with
     that job (every job keeps its working buffers while its graph is
@@ -343,8 +346,10 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
     jobs_sc = scenes.batch_scenes(seed=4, n_jobs=n_jobs, tmax=tmax, grids=grids)
     log(f"[bench] batch: {n_jobs} meshes generated in {time.perf_counter() - t0:.1f} s")
     choices, t_tune = [], 0.0
-    eager_ms = graph_ms = device_ms = check_ms = per_job_ms = 0.0
+    eager_ms = graph_ms = device_ms = check_ms = 0.0
     nodes, graphs, done, vox_launches = 0, 0, 0, 0
+    vox_modes = (0, 1)
+    mode_ms, chosen = {}, []
 
     def oom(e):
         return "out of memory" in str(e)
@@ -381,17 +386,23 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
                 v.generateVoxelsAndOctTree(None)
             eager.append(time.perf_counter() - t1)
         bt = Batch(jobs)
-        # the graph with every job's own voxelize launches, for comparison
-        bt.set_voxelize(persistent=False)
-        bt.capture()
-        split = []
-        for _ in range(reps):
-            if not bt.launch():
-                raise RuntimeError(bt.getErrorMessage())
-            split.append(bt.stats().ms_graph_last)
-        per_job_ms += float(np.median(split))
-        # the graph with one persistent voxelize launch over all jobs
-        bt.set_voxelize(persistent=True)
+        # the voxelize launch structure is tuned per graph: a launch per job,
+        # or one persistent launch over all jobs; the faster (device time) is
+        # captured again and timed below
+        tried = {}
+        for K in vox_modes:
+            bt.set_voxelize(persistent=K)
+            bt.capture()
+            dv = []
+            for _ in range(reps):
+                if not bt.launch():
+                    raise RuntimeError(bt.getErrorMessage())
+                dv.append(bt.stats().ms_graph_last)
+            tried[K] = float(np.median(dv))
+            mode_ms[K] = mode_ms.get(K, 0.0) + tried[K]
+        best = min(tried, key=tried.get)
+        chosen.append(best)
+        bt.set_voxelize(persistent=best)
         bt.capture()
         graph, check, dev = [], [], []
         for _ in range(reps):
@@ -431,8 +442,12 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
         "eager_ms": round(eager_ms, 3),
         "graph_ms": round(graph_ms, 3),
         "graph_device_ms": round(device_ms, 3),
-        # the same graph with every job voxelized by launches of its own
-        "graph_device_ms_per_job_voxelize": round(per_job_ms, 3),
+        # device time summed over the graphs for each voxelize launch
+        # structure tried (0: a launch per job; 1: one persistent launch over
+        # all jobs), and the structure chosen per graph
+        "graph_device_ms_by_voxelize_launches": {str(k): round(v, 3)
+                                                  for k, v in mode_ms.items()},
+        "voxelize_mode_per_graph": chosen,
         "voxelize_launches": vox_launches,
         "checksum_ms": round(check_ms, 3),
         "graph_nodes": nodes,
@@ -543,10 +558,12 @@ def main():
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--paged-trees", type=int, default=96,
                     help="octrees the paged leg keeps loaded (tree slots of the server)")
-    ap.add_argument("--paged-per-tree", action="store_true",
-                    help="paged legs: cast every tree alone and blend far to near")
     ap.add_argument("--f2b-stop", type=float, default=1.0 / 255.0,
-                    help="paged legs' front-to-back pass: transmittance at which a pixel stops")
+                    help="paged_front_to_back leg: transmittance at which a pixel stops")
+    ap.add_argument("--no-paged-f2b", action="store_true",
+                    help="skip the C4 leg cast front to back in passes")
+    ap.add_argument("--no-paged-per-tree", action="store_true",
+                    help="skip the C4 leg that casts every tree alone")
     ap.add_argument("--paged-group", type=int, default=3,
                     help="trees per server frame within a paged frame (0: one per frame)")
     ap.add_argument("--batch-jobs", type=int, default=256)
@@ -723,20 +740,26 @@ def main():
         out["serving_evicting"] = sv3
         log(f"[bench] evicting pool: {sv3['frame_ms']} ms/frame, {sv3['evictions']} evictions, "
             f"p99 {sv3['latency_p99_ms']} ms")
-    f2b = None if args.paged_per_tree else args.f2b_stop
-    if not args.no_paged:
+    if not args.no_paged and not args.no_paged_per_tree:
         log(f"[bench] paged serving over a {args.paged_cells}^3 virtual grid ...")
         pb = paged_bench(args.paged_cells, args.serve_frames, args.serve_width,
                          args.serve_height, 1024, 16, args.paged_pool_gb, args.paged_trees, 4,
-                         args.paged_group, f2b)
+                         args.paged_group)
         out["paged"] = pb
         log(f"[bench] paged: {pb['frame_ms']} ms/frame, {pb['tiles_voxelized']} tiles, "
             f"p99 {pb['latency_p99_ms']} ms")
+    if not args.no_paged and not args.no_paged_f2b:
+        log("[bench] the same, cast front to back in passes ...")
+        pf = paged_bench(args.paged_cells, args.serve_frames, args.serve_width,
+                         args.serve_height, 1024, 16, args.paged_pool_gb, args.paged_trees, 4,
+                         args.paged_group, args.f2b_stop)
+        out["paged_front_to_back"] = pf
+        log(f"[bench] front to back: {pf['frame_ms']} ms/frame, p99 {pf['latency_p99_ms']} ms")
     if not args.no_scale:
         log(f"[bench] paged serving at scale over a {args.scale_cells}^3 virtual grid ...")
         sb = scale_bench(args.scale_cells, args.scale_frames, args.serve2_width,
                          args.serve2_height, 1024, 16, args.scale_pool_gb, args.scale_rocks,
-                         args.paged_trees, 4, args.paged_group, f2b)
+                         args.paged_trees, 4, args.paged_group)
         out["paged_scale"] = sb
         log(f"[bench] scale: {sb['frame_ms']} ms/frame, {sb['tiles_voxelized']} tiles, "
             f"p99 {sb['latency_p99_ms']} ms")

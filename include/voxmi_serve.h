@@ -111,14 +111,23 @@ int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out);
  * (out = sum of T * src * src.a per channel, T the product of (1 - src.a) of
  * the octrees in front), into frame: width*height RGBA32F pixels on the
  * device, every pixel written.  A pixel whose T falls to t_stop or below
- * (0: never) casts no further octree, so their bricks are not requested
+ * (t_stop 0: never, even at T = 0) casts no further octree, so their bricks are not requested
  * through it; with t_stop = 0 the frame equals vox_composite's up to fp32
  * rounding.  Each caster gets the request list of its octree's casts (read
  * by vox_server_frame / _multi after this frame); its colour and usage
- * buffers are not written.  Not in the reference, which rasterizes and
- * casts every paged octree separately (GigaVoxelsRenderer.cpp 1019-1040). */
+ * buffers are not written.  A frame may be cast in several passes over
+ * consecutive runs of that order (a server frame between them serves the
+ * nearer octrees' requests first): transmittance (width*height floats on the
+ * device, or NULL) receives each pixel's T, and resume = 1 continues from
+ * frame and transmittance as the previous pass left them; a pass writes only
+ * the union of its octrees' pixel rectangles, and resume = 0 first clears
+ * the whole frame (and sets transmittance to 1).  Whatever wrote frame last
+ * (the previous pass, or the last frame) must be ordered before the call on
+ * the device, as a server frame between them orders it.  Not in the reference, which rasterizes and casts every
+ * paged octree separately (GigaVoxelsRenderer.cpp 1019-1040). */
 int vox_raycast_frame_multi(vox_raycaster* const* rcs, uint32_t n, const vox_camera* cam,
-                            const vox_raycast_params* prms, float* frame, float t_stop);
+                            const vox_raycast_params* prms, float* frame, float* transmittance,
+                            float t_stop, int32_t resume);
 /* Blend the last frames of n ray casters, in the order given (far to near,
  * as the renderer draws its list), into dst: width*height RGBA32F pixels on
  * the device, the renderer's colour target (GigaVoxelsRenderer.cpp 511-515).

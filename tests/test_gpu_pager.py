@@ -10,8 +10,8 @@ import pytest
 
 import oracle
 from oracle.server import NONE
-from voxvizthesis_amd import scenes
-from voxvizthesis_amd.gvserve import box_rect
+from voxvizthesis_amd import _lib, scenes
+from voxvizthesis_amd.gvserve import box_rect, camera, raycast_front_to_back, raycast_params
 from voxvizthesis_amd.pager import TilePager
 from voxvizthesis_amd.tiler import PagedOctTreeGenerator, TileAssignment
 from voxvizthesis_amd.voxelizer import Voxelizer
@@ -150,7 +150,8 @@ def test_front_to_back_frame_equals_per_tree_casts():
     per-tree casts' FragColors give when blended front to back (bit-exact,
     restated in numpy fp32), with every tree's request set equal to its
     per-tree cast's (no pixel stops) or within it (pixels stop behind
-    opaque trees)."""
+    opaque trees); two passes, the second resuming the first, give the same
+    image and request sets as one."""
     sc = scenes.scene_full_octree(cells=64, N=64, B=8, n_rocks=60)
     W, H = 64, 48
     pg = TilePager(sc.meshes, 1.0 / 128, 64, 8, (8, 8, 8), max_trees=8, max_loading=3,
@@ -179,6 +180,9 @@ def test_front_to_back_frame_equals_per_tree_casts():
                     per[id(tr)] = (rc.rgba_f32(), set(rc.requests().tolist()),
                                    (0, 0, W, H) if r is None else r)
                 pg.f2b_stop = stop
+                # the same frame number: which node ids a pixel lists
+                # depends on its parity (the usage-list push bound)
+                pg.frame_index -= 1
                 drawn2 = pg.frame([cam], serve=False)
                 assert sorted(id(t) for t, _ in drawn2[0]) == sorted(per)
                 img = pg.image(0)
@@ -188,15 +192,47 @@ def test_front_to_back_frame_equals_per_tree_casts():
                     src, _, (x0, y0, x1, y1) = per[id(tr)]
                     m = np.zeros((H, W), bool)
                     m[y0:y1, x0:x1] = True
-                    m &= T > np.float32(stop)
+                    if stop > 0.0:
+                        m &= T > np.float32(stop)
                     a = src[..., 3]
                     acc[m] = acc[m] + T[m][:, None] * (src[m] * a[m][:, None])
                     T[m] = T[m] * (np.float32(1.0) - a[m])
                 assert (img == acc).all()
+                got_all = {}
                 for tr, rc in drawn2[0]:
                     got, want = set(rc.requests().tolist()), per[id(tr)][1]
                     assert got == want if stop == 0.0 else got <= want
+                    got_all[id(tr)] = got
                     checked += 1
+                # the same frame in two passes, the second resuming where the
+                # first left every pixel: the same image and request lists
+                trs = [tr for tr, _ in drawn2[0]]
+                rcs2 = [rc for _, rc in drawn2[0]]
+                if len(rcs2) >= 2:
+                    vcam = camera(pos, look, up, 60.0, W, H, pg.frame_index - 1)
+                    prms = [raycast_params(t.box_min, t.box_max,
+                                           rect=box_rect(cam, t.box_min, t.box_max))
+                            for t in trs]
+                    frame = _lib.DeviceArray.alloc(16 * W * H)
+                    trans = _lib.DeviceArray.alloc(4 * W * H)
+                    try:
+                        h = len(rcs2) // 2
+                        raycast_front_to_back(rcs2[:h], vcam, prms[:h], frame.ptr, stop,
+                                              trans.ptr, resume=False)
+                        # (reading the lists waits for the first pass)
+                        parts = {id(t): set(rc.requests().tolist())
+                                 for t, rc in zip(trs[:h], rcs2[:h])}
+                        raycast_front_to_back(rcs2[h:], vcam, prms[h:], frame.ptr, stop,
+                                              trans.ptr, resume=True)
+                        parts.update({id(t): set(rc.requests().tolist())
+                                      for t, rc in zip(trs[h:], rcs2[h:])})
+                        two = np.empty((H, W, 4), np.float32)
+                        assert pg._L.vox_pool_copy_to_host(two.ctypes.data, frame.ptr, two.nbytes)
+                        assert (two == img).all()
+                        assert parts == got_all
+                    finally:
+                        frame.close()
+                        trans.close()
                 pg.serve()
         print(f"[pager] checked {checked} tree casts", flush=True)
         assert checked >= 4

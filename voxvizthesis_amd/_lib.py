@@ -325,7 +325,8 @@ def load():
     L.vox_raycaster_get_error.argtypes = [vp]
     L.vox_raycast_frame.argtypes = [vp, C.POINTER(VoxCamera), C.POINTER(VoxRaycastParams)]
     L.vox_raycast_frame_multi.argtypes = [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(VoxCamera),
-                                          C.POINTER(VoxRaycastParams), vp, C.c_float]
+                                          C.POINTER(VoxRaycastParams), vp, vp, C.c_float,
+                                          C.c_int32]
     L.vox_raycaster_get_frame.argtypes = [vp, C.POINTER(VoxFrameView)]
     # brick server
     L.vox_server_create.restype = vp

@@ -431,21 +431,35 @@ __device__ __forceinline__ void append_requests(const TreeCast& tc, uint32_t id)
 // as the renderer's far-to-near blend would (glBlendFunc(GL_SRC_ALPHA,
 // GL_ONE_MINUS_SRC_ALPHA) per channel, alpha included): out = sum over
 // octrees of T * src * src.a, T the product of (1 - src.a) of the octrees
-// before.  A pixel stops once T <= tStop (0: never), a wave once all its
-// pixels have; the octrees it skips list no requests for it.
+// before.  A pixel stops once T <= tStop (tStop 0: never, not even at
+// T = 0), a wave once all its pixels have; the octrees it skips list no
+// requests for it.  With resume the pass continues a frame: acc from frame,
+// T from trans (a pass over the octrees in front wrote both); trans, when
+// given, receives each pixel's T for the next pass.
 __global__ void __launch_bounds__(256)
 k_raycast_multi(const TreeCast* __restrict__ trees, uint32_t n, const float* __restrict__ jitter,
-                int32_t W, int32_t H, float tStop, float4* __restrict__ frame) {
+                int32_t W, int32_t x0, int32_t y0, int32_t x1, int32_t y1, float tStop,
+                float4* __restrict__ frame, float* __restrict__ trans, int resume) {
     __shared__ float u8[256];
     u8[threadIdx.x] = (float)threadIdx.x / 255.0f;  // blockDim.x == 256
     __syncthreads();
-    const int i = (int)blockIdx.x * 16 + (int)(threadIdx.x & 15);
-    const int j = (int)blockIdx.y * 16 + (int)(threadIdx.x >> 4);
-    const bool in = i < W && j < H;
+    // pixels of the union of the octrees' rectangles, [x0, x1) x [y0, y1)
+    const int i = x0 + (int)blockIdx.x * 16 + (int)(threadIdx.x & 15);
+    const int j = y0 + (int)blockIdx.y * 16 + (int)(threadIdx.x >> 4);
+    const bool in = i < x1 && j < y1;
+    const uint64_t pix = (uint64_t)j * W + i;
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float T = 1.0f;
+    if (resume && in) {   // the frame so far, from the octrees in front
+        const float4 f = frame[pix];
+        acc[0] = f.x;
+        acc[1] = f.y;
+        acc[2] = f.z;
+        acc[3] = f.w;
+        T = trans[pix];
+    }
     for (uint32_t k = 0; k < n; ++k) {
-        const bool want = in && T > tStop;
+        const bool want = in && (tStop <= 0.0f || T > tStop);
         if (!__any(want)) break;
         const TreeCast& tc = trees[k];
         const RayUniforms& U = tc.U;
@@ -467,8 +481,12 @@ k_raycast_multi(const TreeCast* __restrict__ trees, uint32_t n, const float* __r
 #pragma unroll
         for (int q = 0; q < kMaxUsage; ++q) append_requests(tc, usage[q]);
     }
-    if (in) frame[(uint64_t)j * W + i] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    if (in) {
+        frame[pix] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        if (trans) trans[pix] = T;
+    }
 }
+
 
 // Distinct node ids of the frame's pixel rectangle: lanes of a wave
 // holding the same id elect one leader (ballot loop), leaders claim the node
@@ -847,8 +865,12 @@ int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycas
 }
 
 int vox_raycast_frame_multi(vox_raycaster* const* rcs, uint32_t n, const vox_camera* cam,
-                            const vox_raycast_params* prms, float* frame, float t_stop) {
+                            const vox_raycast_params* prms, float* frame, float* transmittance,
+                            float t_stop, int32_t resume) {
     if (!rcs || n == 0 || !cam || !prms || !frame || !(t_stop >= 0.0f)) return 0;
+    if (resume && !transmittance)
+        return rcs[0] ? rcs[0]->fail("raycast multi: resuming needs the transmittance buffer"),
+               0 : 0;
     for (uint32_t k = 0; k < n; ++k) {
         if (!rcs[k] || !rcs[k]->valid) return 0;
         for (uint32_t m = 0; m < k; ++m)
@@ -905,6 +927,22 @@ int vox_raycast_frame_multi(vox_raycaster* const* rcs, uint32_t n, const vox_cam
         r0->jitterSeed = prms[0].jitter_seed;
         r0->haveJitter = true;
     }
+    // a new frame starts cleared (colour 0, transmittance 1); a pass then
+    // covers the union of its octrees' rectangles only
+    if (!resume) {
+        launch_fill_u32(st, reinterpret_cast<uint32_t*>(frame), 4ull * cam->width * cam->height,
+                        0u);
+        if (transmittance)
+            launch_fill_u32(st, reinterpret_cast<uint32_t*>(transmittance),
+                            (uint64_t)cam->width * cam->height, 0x3f800000u);  // 1.0f
+    }
+    int32_t ux0 = cam->width, uy0 = cam->height, ux1 = 0, uy1 = 0;
+    for (const TreeCast& tc : table) {
+        ux0 = std::min(ux0, tc.U.rx0);
+        uy0 = std::min(uy0, tc.U.ry0);
+        ux1 = std::max(ux1, tc.U.rx1);
+        uy1 = std::max(uy1, tc.U.ry1);
+    }
     // the table is copied in stream order, after the previous frame's kernel
     // has read the device copy, from a pinned staging buffer whose last copy
     // has run
@@ -933,10 +971,11 @@ int vox_raycast_frame_multi(vox_raycaster* const* rcs, uint32_t n, const vox_cam
         !r0->check(hipEventRecord(r0->tableCopied[b], st), "event record"))
         return 0;
     (void)hipEventRecord(r0->ev[0], st);
-    dim3 grid((unsigned)(cam->width + 15) / 16, (unsigned)(cam->height + 15) / 16);
+    dim3 grid((unsigned)(ux1 - ux0 + 15) / 16, (unsigned)(uy1 - uy0 + 15) / 16);
     k_raycast_multi<<<grid, 256, 0, st>>>(r0->multiTable.as<TreeCast>(), n, r0->jitter.as<float>(),
-                                          cam->width, cam->height, t_stop,
-                                          reinterpret_cast<float4*>(frame));
+                                          cam->width, ux0, uy0, ux1, uy1, t_stop,
+                                          reinterpret_cast<float4*>(frame), transmittance,
+                                          resume ? 1 : 0);
     (void)hipEventRecord(r0->ev[1], st);
     (void)hipEventRecord(r0->ev[2], st);
     if (!r0->check(hipGetLastError(), "raycast multi kernel")) return 0;
@@ -983,7 +1022,10 @@ int vox_raycaster_get_frame(vox_raycaster* rc, vox_frame_view* out) {
     if (!rc || !rc->valid || !out) return 0;
     if (rc->countsPending) {
         uint32_t cnt[2] = {0, 0};
-        if (!rc->check(hipMemcpyAsync(cnt, rc->counts.p, 8, hipMemcpyDeviceToHost, rc->st),
+        // the frame may have been cast on another caster's stream (a
+        // front-to-back pass); its done event marks the end either way
+        if (!rc->check(hipStreamWaitEvent(rc->st, rc->done, 0), "wait for the frame") ||
+            !rc->check(hipMemcpyAsync(cnt, rc->counts.p, 8, hipMemcpyDeviceToHost, rc->st),
                        "counts") ||
             !rc->check(hipStreamSynchronize(rc->st), "synchronize"))
             return 0;

@@ -156,16 +156,19 @@ def composite(raycasters, dst_ptr: int, width: int, height: int, clear: bool = T
 
 
 def raycast_front_to_back(raycasters, cam: _lib.VoxCamera, prms, dst_ptr: int,
-                          t_stop: float = 0.0):
-    """vox_raycast_frame_multi: one front-to-back frame over the casters'
+                          t_stop: float = 0.0, trans_ptr=None, resume: bool = False):
+    """vox_raycast_frame_multi: one front-to-back pass over the casters'
     octrees (listed in an order every ray meets them in), one parameter set
     per caster, into the RGBA32F device buffer dst_ptr (every pixel
-    written); each caster gets its octree's request list."""
+    written); each caster gets its octree's request list.  trans_ptr
+    (width*height floats) receives each pixel's transmittance; resume
+    continues a frame from dst_ptr and trans_ptr."""
     L = _lib.load()
     n = len(raycasters)
     arr = (C.c_void_p * max(n, 1))(*[r._h for r in raycasters])
     parr = (_lib.VoxRaycastParams * max(n, 1))(*prms)
-    if not L.vox_raycast_frame_multi(arr, n, C.byref(cam), parr, dst_ptr, float(t_stop)):
+    if not L.vox_raycast_frame_multi(arr, n, C.byref(cam), parr, dst_ptr, trans_ptr,
+                                     float(t_stop), 1 if resume else 0):
         raise RuntimeError("vox_raycast_frame_multi failed: " +
                            (raycasters[0].getErrorMessage() if n else "no octrees"))
 

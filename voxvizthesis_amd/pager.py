@@ -431,10 +431,19 @@ class TilePager:
         return out
 
     def _frame_f2b(self, cams, drawn, prm_extra, serve: bool):
-        """Per stream one front-to-back cast over its visible trees (nearest
-        first), straight into the stream's frame; then one server frame for
-        every stream's request lists."""
+        """Per stream the visible trees in front-to-back order, cast in
+        passes straight into the stream's frame, each pass resuming where the
+        nearer trees left every pixel; with serve, a server frame follows
+        every pass, so the nearest trees' requests are served first and a
+        request waits for its pass, not the frame.  The passes hold
+        serve_group trees, then serve_group again, then twice as many as the
+        pass before (the far trees are small on screen and mostly hidden);
+        one pass with serve_group 0."""
         f = self.frame_index
+        if getattr(self, "_trans", None) is None:
+            self._trans = [_lib.DeviceArray.alloc(4 * self.W * self.H)
+                           for _ in range(self.streams)]
+        grouped = serve and self.serve_group > 0
         rcs, out = [], []
         for k, (cm, trees) in enumerate(zip(cams, drawn)):
             row, cast, prms = [], [], []
@@ -442,22 +451,30 @@ class TilePager:
                 rect = box_rect(cm, tr.box_min, tr.box_max)
                 if rect == ():
                     continue                              # nothing on screen
-                rc = tr.rcs[k]
                 prms.append(raycast_params(tr.box_min, tr.box_max, rect=rect,
                                            **(prm_extra or {})))
-                cast.append(rc)
-                row.append((tr, rc))
+                cast.append(tr.rcs[k])
+                row.append((tr, tr.rcs[k]))
             cam = camera(cm["position"], cm["look"], cm["up"], cm["fov_y_deg"], cm["width"],
                          cm["height"], f)
-            if cast:
-                raycast_front_to_back(cast, cam, prms, self.frames[k].ptr, self.f2b_stop)
-            else:
+            if not cast:
                 composite([], self.frames[k].ptr, cm["width"], cm["height"], clear=True)
-            rcs += cast
+            a, step = 0, (self.serve_group if grouped else len(cast))
+            while a < len(cast):
+                part = cast[a:a + step]
+                raycast_front_to_back(part, cam, prms[a:a + step], self.frames[k].ptr,
+                                      self.f2b_stop, self._trans[k].ptr, resume=a > 0)
+                if grouped:
+                    self.server.frame(part, wait=False)
+                else:
+                    rcs += part
+                if a > 0:
+                    step *= 2
+                a += len(part)
             self._frame_dims[k] = (int(cm["width"]), int(cm["height"]))
             out.append(row)
         self._cast = rcs
-        if serve:
+        if serve and not grouped:
             self.serve()
         self.frame_index += 1
         return out
@@ -508,6 +525,9 @@ class TilePager:
         for fr in getattr(self, "frames", []):
             fr.close()
         self.frames = []
+        for t in getattr(self, "_trans", None) or []:
+            t.close()
+        self._trans = None
         for d in getattr(self, "_dev", []):
             for a in d.values():
                 if a is not None:
