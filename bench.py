@@ -332,8 +332,8 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
     `grids`) captured into hipGraphs.  The autotuner picks each job's brick
     edge (8/16/32) and persistent launch shape by timing eager runs, and each
     graph's voxelize launch structure by timing replays: every job's own
-    launches, or one persistent launch over all jobs from a device work
-    queue.  Jobs are admitted by free device memory: while they fit, they join the current
+    launches, or one persistent launch from a device work queue over all
+    jobs or over the small ones.  Jobs are admitted by free device memory: while they fit, they join the current
     graph; when setting one up runs out of memory, the resident jobs are
     captured into one graph, timed and freed, and the next graph starts with
     that job (every job keeps its working buffers while its graph is
@@ -348,7 +348,7 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
     choices, t_tune = [], 0.0
     eager_ms = graph_ms = device_ms = check_ms = 0.0
     nodes, graphs, done, vox_launches = 0, 0, 0, 0
-    vox_modes = (0, 1)
+    vox_modes = (0, -1, 4096, 32768)
     mode_ms, chosen = {}, []
 
     def oom(e):
@@ -386,9 +386,10 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
                 v.generateVoxelsAndOctTree(None)
             eager.append(time.perf_counter() - t1)
         bt = Batch(jobs)
-        # the voxelize launch structure is tuned per graph: a launch per job,
-        # or one persistent launch over all jobs; the faster (device time) is
-        # captured again and timed below
+        # the voxelize launch structure is tuned per graph: a launch per job
+        # (0), one persistent launch over all jobs (-1), or over the jobs of
+        # at most n active level-0 tiles (n) with the others launching their
+        # own; the fastest (device time) is captured again and timed below
         tried = {}
         for K in vox_modes:
             bt.set_voxelize(persistent=K)
@@ -443,8 +444,9 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
         "graph_ms": round(graph_ms, 3),
         "graph_device_ms": round(device_ms, 3),
         # device time summed over the graphs for each voxelize launch
-        # structure tried (0: a launch per job; 1: one persistent launch over
-        # all jobs), and the structure chosen per graph
+        # structure tried (0: a launch per job; -1: one persistent launch over
+        # all jobs; n: over the jobs of at most n level-0 tiles), and the
+        # structure chosen per graph
         "graph_device_ms_by_voxelize_launches": {str(k): round(v, 3)
                                                   for k, v in mode_ms.items()},
         "voxelize_mode_per_graph": chosen,

@@ -54,12 +54,17 @@ int vox_batch_capture(vox_batch* b);
  * Each job's result is then available through vox_get_result. */
 int vox_batch_launch(vox_batch* b);
 int vox_batch_get_stats(vox_batch* b, vox_batch_stats* out);
-/* How the next capture voxelizes.  persistent = 1 (the default): one
- * persistent launch of `blocks` workgroups (0: 4 per CU) per wave of groups
- * (wave w holds the w-th group that overlaps its voxel space of every job),
- * its workgroups taking units of a few tiles of any job from a device queue,
- * heaviest jobs first; persistent = 0: every job's groups voxelized by
- * launches of their own inside its branch.  Drops a captured graph. */
+/* How the next capture voxelizes.  The jobs chosen by `persistent` are
+ * voxelized together: one persistent launch of `blocks` workgroups (0: 4 per
+ * CU) per wave of groups (wave w holds the w-th group that overlaps its voxel
+ * space of each of those jobs), its workgroups taking units of a few tiles
+ * of any of them from a device queue, heaviest jobs first; the launch waits
+ * for those jobs' tile maps and their colour passes wait for it.  The other
+ * jobs voxelize their groups by launches of their own inside their branches.
+ * persistent < 0 (the default): every job; 0: none; n > 0: the jobs whose
+ * last generate found at most n active 8^3 tiles in level 0 (small jobs,
+ * whose own launches would leave the GPU mostly idle).  Drops a captured
+ * graph. */
 int vox_batch_set_voxelize(vox_batch* b, int32_t persistent, uint32_t blocks);
 
 #ifdef __cplusplus

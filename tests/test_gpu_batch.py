@@ -2,8 +2,9 @@
 captured graph reproduces each job's eager result bit-exactly, with eager
 work on other handles between the replays; the batched-mesh generator's jobs
 (terrain tiles and random soups of uneven sizes, brick edges 8 and 16, all
-voxelized by one persistent launch) equal the oracle, and equal the graph
-that voxelizes each job with launches of its own; a job whose triangles
+voxelized by one persistent launch) equal the oracle, and equal the graphs
+that voxelize each job with launches of its own, or only the small jobs in
+one persistent launch; a job whose triangles
 changed is refused before the graph runs."""
 import ctypes as C
 import dataclasses
@@ -117,6 +118,18 @@ def test_generator_jobs_in_one_persistent_launch_equal_oracle():
         assert b.launch(), b.getErrorMessage()
         for i, v in enumerate(jobs):
             assert same(snapshot(v), pers[i]), f"job {i}: per-job launches differ"
+        # the small jobs in the persistent launch, the others on their own
+        small = sorted(v.stats().active_tiles[0] for v in jobs)[len(jobs) // 2]
+        b.set_voxelize(persistent=int(small))
+        b.capture()
+        n_small = sum(v.stats().active_tiles[0] <= small for v in jobs)
+        assert 0 < n_small < len(jobs)
+        assert b.stats().voxelize_launches == 1 + len(jobs) - n_small
+        for v in jobs:
+            scribble(v)
+        assert b.launch(), b.getErrorMessage()
+        for i, v in enumerate(jobs):
+            assert same(snapshot(v), pers[i]), f"job {i}: mixed launches differ"
     finally:
         if b is not None:
             b.close()

@@ -33,10 +33,14 @@ class Batch:
     def getErrorMessage(self) -> str:
         return self._L.vox_batch_get_error(self._h).decode()
 
-    def set_voxelize(self, persistent: bool = True, blocks: int = 0):
-        """Voxelize every job in one persistent launch per wave of groups
-        (default; blocks 0 = 4 workgroups per CU), or per job and group."""
-        if not self._L.vox_batch_set_voxelize(self._h, 1 if persistent else 0, int(blocks)):
+    def set_voxelize(self, persistent=-1, blocks: int = 0):
+        """Which jobs share one persistent voxelize launch per wave of groups
+        (blocks 0 = 4 workgroups per CU): -1 / True every job, 0 / False
+        none (a launch per job and group), n > 0 the jobs with at most n
+        active level-0 tiles."""
+        if persistent is True:
+            persistent = -1
+        if not self._L.vox_batch_set_voxelize(self._h, int(persistent), int(blocks)):
             raise RuntimeError(self.getErrorMessage())
 
     def capture(self):

@@ -1473,7 +1473,9 @@ struct vox_batch {
     // overlapping group of every job that has one) instead of one launch per
     // job and group: the job table, the per-job unit prefixes and a queue
     // word per wave, and the events that order the branches around it.
-    bool persistent = true;
+    // which jobs: 0 none, < 0 all, > 0 those with at most that many level-0
+    // tiles (the others keep launches of their own)
+    int32_t persistent = -1;
     unsigned voxBlocks = 0;   // 0: 4 workgroups per CU
     DevBuf jobTable, unitStart, queue;
     std::vector<hipEvent_t> preVox, postVox;
@@ -1619,8 +1621,12 @@ int vox_batch_capture(vox_batch* b) {
     // of them from a device queue, then every job's colour pass.
     const size_t J = b->jobs.size();
     std::vector<std::vector<int>> groupsOf(J);
+    std::vector<char> inPers(J, 0);
     size_t waves = 0, entries = 0;
     for (size_t j = 0; j < J; ++j) {
+        inPers[j] = b->persistent < 0 ||
+                    (b->persistent > 0 && b->jobs[j]->planAct[0] <= (uint32_t)b->persistent);
+        if (!inPers[j]) continue;
         for (size_t g = 0; g < b->jobs[j]->gargs.size(); ++g)
             if (b->jobs[j]->gargs[g].overlaps) groupsOf[j].push_back((int)g);
         waves = std::max(waves, groupsOf[j].size());
@@ -1640,7 +1646,7 @@ int vox_batch_capture(vox_batch* b) {
             cus = 256;
         blocks = 4u * (unsigned)cus;
     }
-    const bool pers = b->persistent && entries > 0;
+    const bool pers = entries > 0;
     if (pers && (!b->jobTable.ensure(sizeof(VoxUnitJob) * entries, b->err, "batch job table") ||
                  !b->unitStart.ensure(4 * (entries + waves), b->err, "batch unit offsets") ||
                  !b->queue.ensure(4 * waves, b->err, "batch queues") ||
@@ -1666,7 +1672,7 @@ int vox_batch_capture(vox_batch* b) {
         vox_ctx* h = b->jobs[j];
         ok = b->check(hipStreamWaitEvent(b->branch[j], b->fork, 0), "fork");
         if (!ok) break;
-        if (!pers) {
+        if (!pers || !inPers[j]) {
             ok = (vox_compute_edges_face_normals_and_bounds(h) == 1 &&
                   vox_generate(h, nullptr, 1, h->compressed ? 1 : 0) != 0) || job_fail(j);
             continue;
@@ -1706,7 +1712,7 @@ int vox_batch_capture(vox_batch* b) {
         }
     }
     for (size_t j = 0; j < J && ok; ++j) {
-        if (pers) ok = gen_back(b->jobs[j], nullptr, 1) != 0 || job_fail(j);
+        if (pers && inPers[j]) ok = gen_back(b->jobs[j], nullptr, 1) != 0 || job_fail(j);
         ok = ok && b->check(hipEventRecord(b->joined[j], b->branch[j]), "join") &&
              b->check(hipStreamWaitEvent(b->st, b->joined[j], 0), "join");
     }
@@ -1728,7 +1734,13 @@ int vox_batch_capture(vox_batch* b) {
         b->drop_graph();
         return 0;
     }
-    b->stats.voxelize_launches = pers ? (uint32_t)waves : (uint32_t)entries;
+    {
+        uint64_t own = 0;   // launches of the jobs outside the persistent launch
+        for (size_t j = 0; j < J; ++j)
+            if (!inPers[j] || !pers)
+                for (const GroupArgs& ga : b->jobs[j]->gargs) own += ga.overlaps ? 1 : 0;
+        b->stats.voxelize_launches = (uint32_t)(own + (pers ? waves : 0));
+    }
     b->stats.voxelize_blocks = pers ? blocks : 0u;
     // the capture must not have moved any buffer (sizes are the planned ones)
     for (size_t j = 0; j < b->jobs.size(); ++j) {
@@ -1792,7 +1804,7 @@ int vox_batch_launch(vox_batch* b) {
 
 int vox_batch_set_voxelize(vox_batch* b, int32_t persistent, uint32_t blocks) {
     if (!b || !b->valid) return 0;
-    b->persistent = persistent != 0;
+    b->persistent = persistent;
     b->voxBlocks = blocks;
     b->drop_graph();   // takes effect at the next capture
     return 1;
