@@ -94,17 +94,20 @@ def kernel_bytes(name, scene, st, bricks, B, launches):
         return tot / launches
     if name == "setup":
         # vertices read (36 B / triangle), face normal + packed voxel bounds
-        # written (24 B), each (group, tile) bin count read and written once
-        return (60 * st.triangles + 8 * st.bins) / launches
+        # written (24 B), and the count of every (group, tile) bin that a
+        # triangle reaches written (4 B; at most one per (triangle, tile)
+        # pair, the bound used here).  Empty bins are never touched.
+        return (60 * st.triangles + 4 * st.pairs) / launches
     if name == "scan":
         # every prefix sum reads and writes each value (8 B); the bin scan
         # also zeroes its input (4 B)
         return (8 * st.scan_values + 4 * st.bins) / launches
     if name == "scatter":
-        # packed bounds read (12 B / triangle), one triangle index written per
-        # (triangle, tile) bin entry, bin starts read and fill counts read and
-        # written once
-        return (12 * st.triangles + 4 * st.pairs + 12 * st.bins) / launches
+        # packed bounds read (12 B / triangle) and one triangle index written
+        # per (triangle, tile) bin entry (4 B); the bins' start and fill words
+        # it also reads and updates are not counted (a few per tile, mostly
+        # in L2)
+        return (12 * st.triangles + 4 * st.pairs) / launches
     return 0.0
 
 
@@ -199,6 +202,7 @@ def paged_stats(pg, wall, total, loaded_at, period=1.0 / 60.0):
     lat = pg.server.latencies_us() / 1e3
     loads = np.asarray(pg.load_ms) if pg.load_ms else np.zeros(1)
     first = next((f for f, n in enumerate(loaded_at) if n > 0), None)
+    phases = getattr(pg, "load_phases", [])
     return {
         "tiles": len(pg.trees),
         "tiles_voxelized": int(pg.tiles_loaded),
@@ -214,6 +218,11 @@ def paged_stats(pg, wall, total, loaded_at, period=1.0 / 60.0):
         "tile_miss_ms_p50": round(float(np.percentile(loads, 50)), 2),
         "tile_miss_ms_p99": round(float(np.percentile(loads, 99)), 2),
         "tile_miss_ms_max": round(float(loads.max()), 2),
+        # host time of a tile load's phases on its loader thread (median):
+        # voxelizer creation and voxel memory, the tile's triangles built,
+        # triangle memory + setup, generate, release of the work memory
+        "tile_load_phase_ms_p50": {k: round(float(np.median([p[k] for p in phases])), 2)
+                                   for k in (phases[0] if phases else {})},
         "uploads": int(st.uploads),
         "evictions": int(st.evictions),
         "bricks_per_s": round(st.uploads / total, 1),

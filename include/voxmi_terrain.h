@@ -8,7 +8,8 @@
  * of this project's synthetic workloads (fBm of 2-D gradient noise, a
  * 256-texel height ramp), so a tile built here equals the host generator's
  * mesh on the same grid.  Every call runs on the null stream and returns when
- * its work is done: 1 ok, 0 error (vox_terrain_get_error, per thread).
+ * its work is done, waiting for that work only (the library's other streams
+ * are non-blocking): 1 ok, 0 error (vox_terrain_get_error, per thread).
  */
 #ifndef VOXMI_TERRAIN_H
 #define VOXMI_TERRAIN_H

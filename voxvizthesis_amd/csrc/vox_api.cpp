@@ -18,6 +18,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <thread>
@@ -28,6 +29,37 @@
 #include "vox_internal.h"
 
 using namespace voxmi;
+
+namespace voxmi {
+
+namespace {
+std::mutex g_blockMu;
+std::multimap<size_t, void*> g_blocks;   // cached device blocks by size
+}  // namespace
+
+void* block_take(size_t bytes, size_t* cap) {
+    std::lock_guard<std::mutex> lk(g_blockMu);
+    auto it = g_blocks.lower_bound(bytes);
+    if (it == g_blocks.end() || it->first > bytes + bytes / 4) return nullptr;
+    void* p = it->second;
+    *cap = it->first;
+    g_blocks.erase(it);
+    return p;
+}
+
+void block_give(void* p, size_t cap) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_blockMu);
+    g_blocks.emplace(cap, p);
+}
+
+void block_flush() {
+    std::lock_guard<std::mutex> lk(g_blockMu);
+    for (auto& b : g_blocks) (void)hipFree(b.second);
+    g_blocks.clear();
+}
+
+}  // namespace voxmi
 
 namespace {
 
@@ -1905,14 +1937,18 @@ int vox_device_free(void* p) {
 int vox_release_work_memory(vox_ctx* h) {
     if (!h || !h->valid || !idle(h)) return 0;
     if (h->st && !h->check(hipStreamSynchronize(h->st), "synchronize")) return 0;
+    // only this handle's stream uses these buffers, and it is idle now: they
+    // go to the block cache (no hipFree, which would wait for the device)
     for (auto& l : h->lv) {
-        l.col.release(); l.qn.release(); l.nrm.release();
+        l.col.recycle(); l.qn.recycle(); l.nrm.recycle();
         l.active = 0;
     }
-    h->occ0.release(); h->hit0.release(); h->fill0.release(); h->work.release();
-    h->workCount.release();
-    h->rec.release(); h->recCount.release(); h->binTri.release(); h->binCount.release();
-    h->binStart.release(); h->scanTmp.release();
+    h->occ0.recycle(); h->hit0.recycle(); h->fill0.recycle(); h->work.recycle();
+    h->workCount.recycle();
+    h->rec.recycle(); h->recCount.recycle(); h->binTri.recycle(); h->binCount.recycle();
+    h->binStart.recycle(); h->scanTmp.recycle();
+    h->verts.recycle(); h->fnorm.recycle(); h->bbox.recycle(); h->vn.recycle();
+    h->gbounds.recycle(); h->uv.recycle(); h->img.recycle(); h->grpFill.recycle();
     vox_deallocate_triangle_memory(h);
     h->planned = false;
     h->workReleased = true;

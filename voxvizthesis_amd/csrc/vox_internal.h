@@ -13,6 +13,15 @@
 
 namespace voxmi {
 
+// Device blocks handed back by recycle(): kept in a process-wide cache and
+// handed out again by ensure() instead of a new hipMalloc, so that buffers a
+// handle releases do not cost a hipFree (which waits for the whole device).
+// block_take returns a cached block of at least `bytes` and at most a quarter
+// more (its size in *cap), or null; block_flush frees every cached block.
+void* block_take(size_t bytes, size_t* cap);
+void block_give(void* p, size_t cap);
+void block_flush();
+
 // Grow-only device buffer; allocation failures are reported into err.
 struct DevBuf {
     void* p = nullptr;
@@ -21,7 +30,18 @@ struct DevBuf {
         if (bytes <= cap && p) return true;
         if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
         size_t b = std::max<size_t>(bytes, 256);
+        size_t got = 0;
+        if ((p = block_take(b, &got)) != nullptr) {
+            cap = got;
+            return true;
+        }
         hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            // cached blocks may hold the memory: free them and try again
+            (void)hipGetLastError();
+            block_flush();
+            e = hipMalloc(&p, b);
+        }
         if (e != hipSuccess) {
             err << "hipMalloc failed for " << what << " (" << b << " bytes): "
                 << hipGetErrorString(e) << std::endl;
@@ -36,6 +56,14 @@ struct DevBuf {
     }
     void release() {
         if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    // Give the block to the cache instead of freeing it.  Only when no
+    // queued device work can still use it (its stream synchronised, and no
+    // other stream reads it).
+    void recycle() {
+        if (p) block_give(p, cap);
         p = nullptr;
         cap = 0;
     }

@@ -243,17 +243,21 @@ int vox_terrain_range(const float* heights, uint32_t n1, uint32_t x0, uint32_t y
     if (!heights || !out_minmax || w == 0 || h == 0 || (uint64_t)x0 + w > n1 ||
         (uint64_t)y0 + h > n1)
         return fail("vox_terrain_range: window outside the grid"), 0;
-    int32_t* d = nullptr;
-    if (!check(hipMalloc(&d, 8), "vox_terrain_range")) return 0;
+    // the result words: one pair per calling thread, allocated on its first
+    // call and kept (no hipFree, which would wait for the whole device)
+    thread_local int32_t* d = nullptr;
+    if (!d && !check(hipMalloc(&d, 8), "vox_terrain_range")) return 0;
     const int32_t init[2] = {INT32_MAX, INT32_MIN};
     int32_t r[2] = {0, 0};
-    bool ok = check(hipMemcpy(d, init, 8, hipMemcpyHostToDevice), "vox_terrain_range");
+    // on the null stream, which the library's non-blocking streams do not
+    // wait for and which waits for none of them
+    bool ok = check(hipMemcpyAsync(d, init, 8, hipMemcpyHostToDevice, nullptr), "vox_terrain_range");
     if (ok) {
         k_range<<<h < 4096u ? h : 4096u, 256>>>(heights, n1, x0, y0, w, h, d);
         ok = check(hipGetLastError(), "vox_terrain_range") &&
-             check(hipMemcpy(r, d, 8, hipMemcpyDeviceToHost), "vox_terrain_range");
+             check(hipMemcpyAsync(r, d, 8, hipMemcpyDeviceToHost, nullptr), "vox_terrain_range") &&
+             check(hipStreamSynchronize(nullptr), "vox_terrain_range");
     }
-    (void)hipFree(d);
     if (!ok) return 0;
     out_minmax[0] = from_ordered(r[0]);
     out_minmax[1] = from_ordered(r[1]);
@@ -273,8 +277,9 @@ int vox_terrain_tile(const float* heights, uint32_t n1, double extent, uint32_t 
     const uint64_t n = (uint64_t)cw * ch;
     k_tile<<<blocks_for(n), 256>>>(heights, n1, step, dxf, x0, y0, cw, ch, (double)hmin,
                                    (double)hmax, tex_width, verts, vnormals, uvs);
+    // waits for this launch only (the null stream), not for the device
     if (!check(hipGetLastError(), "vox_terrain_tile") ||
-        !check(hipDeviceSynchronize(), "vox_terrain_tile"))
+        !check(hipStreamSynchronize(nullptr), "vox_terrain_tile"))
         return 0;
     return 1;
 }

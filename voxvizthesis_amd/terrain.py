@@ -64,17 +64,24 @@ class DeviceTerrain:
             raise RuntimeError(self._err())
         return float(mm[0]), float(mm[1])
 
-    def tile(self, x0: int, y0: int, cw: int, ch: int):
+    def tile(self, x0: int, y0: int, cw: int, ch: int, out=None):
         """Device arrays (verts, vnormals, uvs) of the cells [x0, x0+cw) x
-        [y0, y0+ch): 2 * cw * ch triangles."""
+        [y0, y0+ch): 2 * cw * ch triangles; into `out` (three arrays large
+        enough) when given, else new arrays."""
         nt = 2 * cw * ch
-        bufs = [_lib.DeviceArray.alloc(4 * 9 * nt), _lib.DeviceArray.alloc(4 * 9 * nt),
-                _lib.DeviceArray.alloc(4 * 6 * nt)]
+        need = (4 * 9 * nt, 4 * 9 * nt, 4 * 6 * nt)
+        if out is not None:
+            if any(b.nbytes < n for b, n in zip(out, need)):
+                raise ValueError("tile buffers too small")
+            bufs = out
+        else:
+            bufs = [_lib.DeviceArray.alloc(n) for n in need]
         if not self._L.vox_terrain_tile(self.heights.ptr, self.n1, self.extent, x0, y0, cw, ch,
                                         self.lo, self.hi, self.tex_width, bufs[0].ptr,
                                         bufs[1].ptr, bufs[2].ptr):
-            for b in bufs:
-                b.close()
+            if out is None:
+                for b in bufs:
+                    b.close()
             raise RuntimeError(self._err())
         return bufs
 
@@ -177,6 +184,11 @@ class TerrainPager(TilePager):
         self._cast = []
         self._loader = ThreadPoolExecutor(max_workers=self.max_loading)
         self.load_ms = []
+        self.load_phases = []     # per load: host ms of its phases
+        import threading
+        self._tls = threading.local()
+        self._tls_lock = threading.Lock()
+        self._all_tile_bufs = []
         self.tiles_loaded = 0
         self.terrain = DeviceTerrain(cells, float(cells), seed, ridged_seed, ridged_amp)
         self.height_fn = c5_height_fn(seed, ridged_seed, ridged_amp, float(cells))
@@ -247,11 +259,11 @@ class TerrainPager(TilePager):
             lo, hi = hi, f32(hi + sz)
         return out
 
-    def tile_groups(self, tr: PagedTree):
-        """The tile's terrain window (device arrays + group bounds) and its
-        rocks (host mesh or None)."""
+    def tile_groups(self, tr: PagedTree, out=None):
+        """The tile's terrain window (device arrays, in `out` when given +
+        group bounds) and its rocks (host mesh or None)."""
         x0, y0, cw, ch = self._win[tr.tile]
-        bufs = self.terrain.tile(x0, y0, cw, ch)
+        bufs = self.terrain.tile(x0, y0, cw, ch, out)
         zlo, zhi = self.terrain.range(x0, y0, cw + 1, ch + 1)
         step = self.terrain.extent / self.terrain.cells
         lo = (f32(x0 * step), f32(y0 * step), f32(zlo))
@@ -261,13 +273,29 @@ class TerrainPager(TilePager):
         sel = self.rocks.select((lo[0], lo[1], z0), (hi[0], hi[1], z1))
         return bufs, 2 * cw * ch, lo, hi, self.rocks.mesh(sel)
 
+    def _tile_buffers(self):
+        """This loader thread's tile arrays, sized for the largest window and
+        kept for the pager's life (no allocation or release per tile)."""
+        bufs = getattr(self._tls, "bufs", None)
+        if bufs is None:
+            w = self.N + 4 * self.extra
+            nt = 2 * w * w
+            bufs = [_lib.DeviceArray.alloc(n) for n in (36 * nt, 36 * nt, 24 * nt)]
+            self._tls.bufs = bufs
+            with self._tls_lock:
+                self._all_tile_bufs += bufs
+        return bufs
+
     def _load(self, tr: PagedTree):
+        import time
+        t = [time.perf_counter()]
         vz = Voxelizer((self.N,) * 3, (self.B,) * 3, None, options=self.options)
-        bufs = []
         try:
             ok = vz.initCuda() and vz.allocateVoxelMemory() and \
                 vz.setVoxelizationParams(tr.origin, self.vs)
-            bufs, nt, lo, hi, rocks = self.tile_groups(tr)
+            t.append(time.perf_counter())
+            bufs, nt, lo, hi, rocks = self.tile_groups(tr, self._tile_buffers())
+            t.append(time.perf_counter())
             grp = _lib.VoxGroup()
             grp.verts, grp.vnormals, grp.uvs = bufs[0].ptr, bufs[1].ptr, bufs[2].ptr
             grp.ntri = nt
@@ -281,16 +309,25 @@ class TerrainPager(TilePager):
             if rocks is not None:
                 ok = ok and vz.addMesh(rocks)
             ok = ok and vz.allocateTriangleMemory() and vz.computeEdgesFaceNormalsAndBounds()
+            t.append(time.perf_counter())
             st = vz.generateVoxelsAndOctTree(None) if ok else 0
+            t.append(time.perf_counter())
             if st == 1:
                 vz.releaseWorkMemory()
+            t.append(time.perf_counter())
+            self.load_phases.append(dict(zip(
+                ("create", "tile", "setup", "generate", "release"),
+                [(b - a) * 1e3 for a, b in zip(t, t[1:])])))
             return vz, st
-        finally:
-            for b in bufs:
-                b.close()
+        except BaseException:
+            vz.close()
+            raise
 
     def close(self):
         super().close()
+        for b in getattr(self, "_all_tile_bufs", []):
+            b.close()
+        self._all_tile_bufs = []
         if getattr(self, "terrain", None) is not None:
             self.terrain.close()
             self.terrain = None
