@@ -357,7 +357,8 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
     choices, t_tune = [], 0.0
     eager_ms = graph_ms = device_ms = check_ms = 0.0
     nodes, graphs, done, vox_launches = 0, 0, 0, 0
-    vox_modes = (0, -1, 4096, 32768)
+    # (jobs in the persistent launch, its workgroups; 0 = 4 per CU)
+    vox_modes = ((0, 0), (-1, 0), (4096, 0))
     mode_ms, chosen = {}, []
 
     def oom(e):
@@ -401,7 +402,7 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
         # own; the fastest (device time) is captured again and timed below
         tried = {}
         for K in vox_modes:
-            bt.set_voxelize(persistent=K)
+            bt.set_voxelize(persistent=K[0], blocks=K[1])
             bt.capture()
             dv = []
             for _ in range(reps):
@@ -412,7 +413,7 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
             mode_ms[K] = mode_ms.get(K, 0.0) + tried[K]
         best = min(tried, key=tried.get)
         chosen.append(best)
-        bt.set_voxelize(persistent=best)
+        bt.set_voxelize(persistent=best[0], blocks=best[1])
         bt.capture()
         graph, check, dev = [], [], []
         for _ in range(reps):
@@ -453,9 +454,10 @@ def batch_bench(n_jobs: int, tmax: float, grids, do_autotune: bool, reps: int = 
         "graph_ms": round(graph_ms, 3),
         "graph_device_ms": round(device_ms, 3),
         # device time summed over the graphs for each voxelize launch
-        # structure tried (0: a launch per job; -1: one persistent launch over
-        # all jobs; n: over the jobs of at most n level-0 tiles), and the
-        # structure chosen per graph
+        # structure tried, (jobs, workgroups): jobs 0 a launch per job, -1 one
+        # persistent launch over all jobs, n over the jobs of at most n
+        # level-0 tiles; workgroups of the persistent launch, 0 = 4 per CU;
+        # and the structure chosen per graph
         "graph_device_ms_by_voxelize_launches": {str(k): round(v, 3)
                                                   for k, v in mode_ms.items()},
         "voxelize_mode_per_graph": chosen,

@@ -8,9 +8,15 @@
 // 725-818, 509-588): int32 compressed flag, then per brick in index order
 // u64 dims, u32 borders, int32 0, u32 size, RGBA8, int32 0, u32 size, RGB8;
 // compressed (WriteCompressedBinaryBrick, 590-723): the same with flags 1
-// and BC3 colour / BC1 normal blocks.
+// and BC3 colour / BC1 normal blocks.  Text bricks, voxels_<depth>.gvt
+// (StartTextBricksFile / WriteTextBrick / EndTextBricksFile, 45-68, 207-325,
+// 96-103): XML-like rows of the brick's (B+2)^3 window of its mip level,
+// colours as their four bytes written as characters (the reference streams
+// unsigned chars), float normals in default iostream formatting.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -68,15 +74,54 @@ void write_node(std::ofstream& f, const HostTree& t, int d, uint32_t x, uint32_t
     }
 }
 
+// One brick of a text bricks file: the brick's window of (B+2)^3 voxels,
+// x0..: its first voxel in the level (dense colours / normals of extent S),
+// written at (wx, wy, wz) with the row coordinates counted from there.
+void write_text_brick(std::ofstream& f, const uint32_t* col, const float* nrm, const uint32_t S[3],
+                      const uint32_t x0[3], const uint64_t w0[3], const uint32_t e[3]) {
+    f << " <Brick " << "X=\"" << w0[0] << "\" " << "Y=\"" << w0[1] << "\" "
+      << "Z=\"" << w0[2] << "\" " << "DimX=\"" << e[0] << "\" " << "DimY=\"" << e[1] << "\" "
+      << "DimZ=\"" << e[2] << "\" " << "BorderX=\"" << 1 << "\" " << "BorderY=\"" << 1 << "\" "
+      << "BorderZ=\"" << 1 << "\" " << "/>" << std::endl;
+    f << "  <BrickColors>" << std::endl;
+    for (uint32_t z = 0; z < e[2]; ++z)
+        for (uint32_t y = 0; y < e[1]; ++y) {
+            f << "    <Row Y=\"" << w0[1] + y << "\" Z=\"" << w0[2] + z << "\">" << std::endl;
+            const uint64_t row = ((uint64_t)(x0[2] + z) * S[1] + (x0[1] + y)) * S[0] + x0[0];
+            for (uint32_t x = 0; x < e[0]; ++x) {
+                const uint32_t c = col[row + x];
+                f << "(" << (char)(c & 255u) << "," << (char)((c >> 8) & 255u) << ","
+                  << (char)((c >> 16) & 255u) << "," << (char)(c >> 24) << ") ";
+            }
+            f << std::endl << "    </Row>" << std::endl;
+        }
+    f << "  </BrickColors>" << std::endl;
+    f << "  <BrickGradients>" << std::endl;
+    for (uint32_t z = 0; z < e[2]; ++z)
+        for (uint32_t y = 0; y < e[1]; ++y) {
+            f << "    <Row YZ=\"" << w0[1] + y << " " << w0[2] + z << "\">" << std::endl;
+            const uint64_t row = ((uint64_t)(x0[2] + z) * S[1] + (x0[1] + y)) * S[0] + x0[0];
+            for (uint32_t x = 0; x < e[0]; ++x) {
+                const float* n = nrm + 3 * (row + x);
+                f << "(" << n[0] << "," << n[1] << "," << n[2] << ") ";
+            }
+            f << std::endl << "    </Row>" << std::endl;
+        }
+    f << "  </BrickGradients>" << std::endl;
+    f << " </Brick>" << std::endl;
+}
+
 }  // namespace
 
 // Declared in voxmi.h; defined here with access to the result through the
 // public C ABI only.
 extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3],
-                                    const float delta[3], const uint32_t N[3]);
+                                    const float delta[3], const uint32_t N[3], int binary,
+                                    const uint32_t extra[3]);
 
 extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3],
-                                    const float delta[3], const uint32_t N[3]) {
+                                    const float delta[3], const uint32_t N[3], int binary,
+                                    const uint32_t extra[3]) {
     vox_result r;
     if (!vox_get_result(h, &r)) return 0;
     vox_compressed cz;
@@ -89,6 +134,12 @@ extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float
     t.cc.resize(t.L);
     t.bidx.resize(t.L);
     const uint64_t bvox = (uint64_t)(t.B[0] + 2) * (t.B[1] + 2) * (t.B[2] + 2);
+    // brick border of each depth's level: extra at level 0, then halved, at
+    // least 1 (Voxelizer.cu 1270-1272)
+    std::vector<std::array<uint32_t, 3>> bord(t.L);
+    for (int k = 0; k < t.L; ++k)
+        for (int a = 0; a < 3; ++a)
+            bord[k][a] = k == 0 ? extra[a] : std::max(bord[k - 1][a] >> 1, 1u);
     for (int d = 0; d < t.L; ++d) {
         const uint64_t n = r.nodes[d];
         t.types[d].resize(n);
@@ -98,6 +149,46 @@ extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float
             !vox_copy_to_host(h, t.cc[d].data(), r.const_colors[d], n * 4) ||
             !vox_copy_to_host(h, t.bidx[d].data(), r.brick_index[d], n * 4))
             return 0;
+        if (!binary) {
+            // voxels_<depth>.gvt from the dense level (L-1-d) on the host
+            std::ofstream f(std::string(out_dir) + "/voxels_" + std::to_string(d) + ".gvt",
+                            std::fstream::out);
+            if (!f.is_open()) return 0;
+            f << "<Bricks>" << std::endl;
+            const uint64_t nb = r.bricks[d];
+            if (nb > 0) {
+                const int k = t.L - 1 - d;
+                uint32_t S[3];
+                if (!vox_copy_level(h, k, nullptr, nullptr, S)) return 0;
+                const uint64_t nv = (uint64_t)S[0] * S[1] * S[2];
+                std::vector<uint32_t> col(nv);
+                std::vector<float> nrm(3 * nv);
+                std::vector<uint32_t> bn(nb);
+                if (!vox_copy_level(h, k, col.data(), nrm.data(), S) ||
+                    !vox_copy_to_host(h, bn.data(), r.brick_node[d], nb * 4))
+                    return 0;
+                const uint32_t Rd = 1u << d;
+                const uint32_t e[3] = {t.B[0] + 2, t.B[1] + 2, t.B[2] + 2};
+                for (uint64_t b = 0; b < nb; ++b) {
+                    const uint32_t nx = bn[b] % Rd, ny = (bn[b] / Rd) % Rd, nz = bn[b] / (Rd * Rd);
+                    const uint32_t m[3] = {nx * t.B[0] + bord[k][0], ny * t.B[1] + bord[k][1],
+                                           nz * t.B[2] + bord[k][2]};
+                    const uint32_t x0[3] = {m[0] - 1, m[1] - 1, m[2] - 1};
+                    // a brick reaching the chunk's far side was stored as a
+                    // partial brick and is written from its own buffer, whose
+                    // window starts at 0 (VoxelBrickWriter.cu 1179-1190);
+                    // the others from the level at their offset
+                    bool partial = false;
+                    for (int a = 0; a < 3; ++a) partial = partial || m[a] + t.B[a] >= S[a];
+                    const uint64_t w0[3] = {partial ? 0u : x0[0], partial ? 0u : x0[1],
+                                            partial ? 0u : x0[2]};
+                    write_text_brick(f, col.data(), nrm.data(), S, x0, w0, e);
+                }
+            }
+            f << "</Bricks>" << std::endl;
+            if (f.fail()) return 0;
+            continue;
+        }
         // voxels_<depth>.gvb
         std::string fn = std::string(out_dir) + "/voxels_" + std::to_string(d) + ".gvb";
         std::ofstream f(fn, std::ios_base::out | std::ios_base::binary);
@@ -176,7 +267,8 @@ extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float
        << "VolumeXSize=\"" << N[0] << "\" " << "VolumeYSize=\"" << N[1] << "\" "
        << "VolumeZSize=\"" << N[2] << "\" " << "BrickXSize=\"" << t.B[0] << "\" "
        << "BrickYSize=\"" << t.B[1] << "\" " << "BrickZSize=\"" << t.B[2] << "\" "
-       << "Binary=\"YES\" " << "Compressed=\"" << (compressed ? "YES" : "NO") << "\" >"
+       << "Binary=\"" << (binary ? "YES" : "NO") << "\" " << "Compressed=\""
+       << (compressed ? "YES" : "NO") << "\" >"
        << std::endl;
     write_node(tf, t, 0, 0, 0, 0, 0, 0, 0);
     tf << "</GigaVoxelsOctTree>" << std::endl;
