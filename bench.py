@@ -577,7 +577,7 @@ def main():
                     help="skip the C4 leg cast front to back in passes")
     ap.add_argument("--no-paged-per-tree", action="store_true",
                     help="skip the C4 leg that casts every tree alone")
-    ap.add_argument("--paged-group", type=int, default=3,
+    ap.add_argument("--paged-group", type=int, default=2,
                     help="trees per server frame within a paged frame (0: one per frame)")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-tmax", type=float, default=2e6)

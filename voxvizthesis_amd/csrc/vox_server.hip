@@ -467,7 +467,6 @@ struct vox_server {
     vox_brick_pool* bp = nullptr;
     PoolRefs pool{};
     bool valid = false;
-    bool timingPending = false;
     uint32_t maxUp = 0, nslots = 0, frame = 0, maxTrees = 0;
     uint64_t cap = 0, ntex = 0;  // texels per tree, all trees
     hipStream_t st = nullptr;
@@ -478,7 +477,12 @@ struct vox_server {
     DevBuf req, reqEmit, nreq, done, log, logCount;
     uint64_t logCap = 1u << 24;   // latencies kept (the first ones admitted)
     vox_server_stats stats{};
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    // HIP-event pairs timing the server frames, used in turn: a frame waits
+    // on the host only for the frame kTimingRing frames back, so the host can
+    // queue frames ahead of the device
+    static constexpr int kTimingRing = 8;
+    hipEvent_t ev[kTimingRing][2] = {};
+    int evNext = 0, evPending = 0;   // next pair; pairs recorded, not settled
     std::stringstream err;
     std::string errCache;
     bool fail(const std::string& m) {
@@ -495,15 +499,19 @@ struct vox_server {
             if (trees[i] == t) return (int)i;
         return -1;
     }
-    // Time of the frame whose events were recorded last (waits for it).
-    void settle_timing() {
-        if (!timingPending) return;
-        (void)hipEventSynchronize(ev[1]);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
-        stats.ms_update_last = ms;
-        stats.ms_update_total += ms;
-        timingPending = false;
+    // Add the times of recorded frames, oldest first: all of them (waiting
+    // for each) with all = true, else only down to `keep` frames still
+    // pending, waiting only for those it must settle.
+    void settle_timing(bool all = true, int keep = 0) {
+        while (evPending > (all ? 0 : keep)) {
+            const int i = (evNext - evPending + kTimingRing) % kTimingRing;
+            (void)hipEventSynchronize(ev[i][1]);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev[i][0], ev[i][1]);
+            stats.ms_update_last = ms;
+            stats.ms_update_total += ms;
+            --evPending;
+        }
     }
 };
 
@@ -628,8 +636,9 @@ vox_server* vox_server_create_shared(vox_brick_pool* bp, uint32_t max_trees,
     launch_fill_u32(st, s->logCount.as<uint32_t>(), 2, 0u);
     // the pool's own stream may still be clearing it
     (void)hipStreamSynchronize(s->pool.st);
-    for (auto& e : s->ev)
-        if (!s->check(hipEventCreate(&e), "hipEventCreate")) return s;
+    for (auto& pr : s->ev)
+        for (auto& e : pr)
+            if (!s->check(hipEventCreate(&e), "hipEventCreate")) return s;
     int khz = 0;
     int dev = 0;
     (void)hipGetDevice(&dev);
@@ -660,8 +669,9 @@ vox_server* vox_server_create(vox_octree* tree, uint32_t max_uploads_per_frame) 
 void vox_server_destroy(vox_server* s) {
     if (!s) return;
     if (s->st) (void)hipStreamSynchronize(s->st);
-    for (auto e : s->ev)
-        if (e) (void)hipEventDestroy(e);
+    for (auto& pr : s->ev)
+        for (auto e : pr)
+            if (e) (void)hipEventDestroy(e);
     DevBuf* bufs[] = {&s->tabs, &s->info, &s->nodeSlot, &s->slotNode, &s->slotLast, &s->pend,
                       &s->pendEmit, &s->ring, &s->ringEmit, &s->ctr, &s->tot, &s->conflict,
                       &s->ring2, &s->ringEmit2,
@@ -729,14 +739,15 @@ int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n)
         slot[k] = (uint32_t)i;
     }
     hipStream_t st = s->st;
-    s->settle_timing();  // the events are reused below
+    s->settle_timing(false, vox_server::kTimingRing - 1);  // frees the pair used below
     const uint32_t frame = ++s->frame;
     const uint64_t nt = s->ntex, ns = s->nslots;
     TreeTab* tabs = s->tabs.as<TreeTab>();
     for (uint32_t k = 0; k < n; ++k)
         if (!s->check(hipStreamWaitEvent(st, lists[k].done, 0), "wait for a request list"))
             return 0;
-    (void)hipEventRecord(s->ev[0], st);
+    const int evi = s->evNext;
+    (void)hipEventRecord(s->ev[evi][0], st);
     for (uint32_t k = 0; k < n; ++k)
         k_srv_touch<<<grid_for(lists[k].maxIds), kB, 0, st>>>(
             lists[k].ids, lists[k].times, lists[k].count, lists[k].maxIds, slot[k],
@@ -797,12 +808,13 @@ int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n)
         s->done.as<unsigned long long>(), s->log.as<unsigned long long>(), s->logCap,
         s->logCount.as<unsigned long long>(), s->tot.as<unsigned long long>());
     k_srv_logcount<<<1, 64, 0, st>>>(s->nreq.as<uint32_t>(), s->logCount.as<unsigned long long>());
-    (void)hipEventRecord(s->ev[1], st);
+    (void)hipEventRecord(s->ev[evi][1], st);
+    s->evNext = (evi + 1) % vox_server::kTimingRing;
+    ++s->evPending;
     if (!s->check(hipGetLastError(), "server kernels")) return 0;
     // ray casts of the next frame wait for the node textures and pool
     for (vox_octree* t : s->trees)
         if (t && !octree_record_written(t, st)) return s->fail("server: event record failed"), 0;
-    s->timingPending = true;
     s->stats.frames = frame;
     return 1;
 }
