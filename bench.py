@@ -231,10 +231,10 @@ def paged_stats(pg, wall, total, loaded_at, period=1.0 / 60.0):
     }
 
 
-def wide_slots(B: int, gb: float) -> int:
-    """Slots of a wide pool of gb GB of (B+2)^3 RGBA8 colour + RGB8 normal
-    bricks."""
-    return int(gb * 1e9 // ((B + 2) ** 3 * 8))
+def wide_slots(B: int, gb: float, normals: bool = False) -> int:
+    """Slots of a wide pool of gb GB of (B+2)^3 RGBA8 colour bricks (+ RGB8
+    normals when `normals`)."""
+    return int(gb * 1e9 // ((B + 2) ** 3 * (8 if normals else 4)))
 
 
 def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool_gb: float,
@@ -242,7 +242,8 @@ def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool
     """Served paging: a (cells+1)^2 fBm terrain in a cells^3 virtual grid,
     tiled into tile^3 octrees built and voxelized on demand on the device as
     the seeded flythrough first sees them (the pager), served from one shared
-    brick pool of pool_gb GB (a wide pool beyond 8.2 GB) with LRU eviction;
+    brick pool of pool_gb GB of RGBA8 colour bricks (a wide pool) with LRU
+    eviction;
     a server frame follows every serve_group trees' ray casts within a
     frame.  The terrain's heights are the scene generator's fBm (seed 5),
     built on the device rather than as a host triangle soup."""
@@ -254,12 +255,12 @@ def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool
     # each tree's z origin to a whole unit, main.cu 4258); the camera planes
     # are the reference's 1 m / 10 km.
     extent = float(cells)
-    slots = wide_slots(B, pool_gb) if pool_gb > 8.2 else (56, 56, 56)
+    slots = wide_slots(B, pool_gb)
     t0 = time.perf_counter()
     pg = TerrainPager(cells, tile, B, slots, seed=5, ridged_amp=0.0, rocks=0,
                       max_trees=max_trees, max_loading=max_loading, streams=1, max_width=W,
                       max_height=H, near=1.0, far=10000.0, serve_group=serve_group,
-                      f2b_stop=f2b_stop)
+                      f2b_stop=f2b_stop, pool_normals=False)
     t_setup = time.perf_counter() - t0
     log(f"[bench] paged: {len(pg.trees)} paged trees set up in {t_setup:.1f} s")
     path = scenes.flythrough(seed=5, frames=frames, extent=extent, height_fn=pg.height_fn)
@@ -273,8 +274,8 @@ def paged_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool
             "serve_group": serve_group,
             "cast": cast_mode(f2b_stop),
             "pool_slots": int(np.prod(slots)),
-            "pool_gb": round(float(np.prod(slots)) * (B + 2) ** 3 * 8 / 1e9, 2),
-            "pool_format": "wide (slot in word 1)" if isinstance(slots, int) else "reference",
+            "pool_gb": round(float(np.prod(slots)) * (B + 2) ** 3 * 4 / 1e9, 2),
+            "pool_format": "wide (slot in word 1), RGBA8 colour only",
             "scene_setup_s": round(t_setup, 2),
             "frames": frames,
         }
@@ -309,7 +310,8 @@ def scale_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool
     pg = TerrainPager(cells, tile, B, slots, seed=6, ridged_seed=7, ridged_amp=0.05,
                       rocks=rocks, rock_seed=8, rock_radius=(2.0, 40.0), max_trees=max_trees,
                       max_loading=max_loading, streams=2, max_width=W, max_height=H,
-                      near=1.0, far=4.0 * cells, serve_group=serve_group, f2b_stop=f2b_stop)
+                      near=1.0, far=4.0 * cells, serve_group=serve_group, f2b_stop=f2b_stop,
+                      pool_normals=False)
     t_setup = time.perf_counter() - t0
     log(f"[bench] scale: {len(pg.trees)} paged trees set up in {t_setup:.1f} s")
     paths = [scenes.flythrough(seed=s, frames=frames, extent=float(cells), alt=(0.08, 0.2),
@@ -322,8 +324,8 @@ def scale_bench(cells: int, frames: int, W: int, H: int, tile: int, B: int, pool
                         f"{tile}^3 tiles of {B}^3 bricks built and voxelized on demand, two "
                         f"{W}x{H} camera streams over {frames}-frame seeded flythroughs",
             "pool_slots": int(slots),
-            "pool_gb": round(float(slots) * (B + 2) ** 3 * 8 / 1e9, 2),
-            "pool_format": "wide (slot in word 1)",
+            "pool_gb": round(float(slots) * (B + 2) ** 3 * 4 / 1e9, 2),
+            "pool_format": "wide (slot in word 1), RGBA8 colour only",
             "cast": cast_mode(f2b_stop),
             "max_uploads_per_frame": 65536,
             "scene_setup_s": round(t_setup, 2),

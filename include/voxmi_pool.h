@@ -92,6 +92,12 @@ vox_brick_pool* vox_brick_pool_create(const uint32_t brick_dim[3], const uint32_
  * 16 GB and more.  Nodes built on it are not readable by the reference's
  * shader. */
 vox_brick_pool* vox_brick_pool_create_wide(const uint32_t brick_dim[3], uint64_t nslots);
+/* A wide pool of RGBA8 colour bricks only (vox_brick_pool_view.normals is
+ * NULL; uploads skip the normals): twice the slots in the same memory, for
+ * a ray cast that samples colour only, as this library's does.  The paged
+ * workloads' pools are sized this way (16 GB of 18^3 RGBA8 bricks hold
+ * 686 K slots). */
+vox_brick_pool* vox_brick_pool_create_colour(const uint32_t brick_dim[3], uint64_t nslots);
 void vox_brick_pool_destroy(vox_brick_pool* bp);
 const char* vox_brick_pool_get_error(vox_brick_pool* bp);
 int vox_brick_pool_get_view(vox_brick_pool* bp, vox_brick_pool_view* out);

@@ -6,8 +6,9 @@ scenes.heightfield_terrain, and the heights with the ridged term equal the
 float64 host restatement rounded to float32.  The terrain pager (tiles built
 on the device and voxelized on demand, served from a wide brick pool over two
 camera streams cast front to back in passes of two trees, a server frame
-after each) gives every loaded tile the octree the same triangles give alone,
-and every pool slot holds the brick of the node it serves."""
+after each; a colour-only pool) gives every loaded tile the octree the same
+triangles give alone, and every pool slot holds the brick of the node it
+serves."""
 import numpy as np
 import pytest
 
@@ -89,7 +90,7 @@ def test_terrain_pager_tiles_equal_alone_and_pool_is_consistent():
     pg = TerrainPager(256, 64, 8, 512, seed=6, ridged_seed=7, ridged_amp=0.05, rocks=300,
                       rock_radius=(1.0, 6.0), max_trees=12, max_loading=3, streams=2,
                       max_width=W, max_height=H, near=1.0, far=2000.0, serve_group=2,
-                      f2b_stop=1.0 / 255.0)
+                      f2b_stop=1.0 / 255.0, pool_normals=False)
     try:
         print(f"[terrain] {len(pg.trees)} paged trees", flush=True)
         paths = [scenes.flythrough(seed=s, frames=8, extent=256.0, alt=(0.08, 0.2),

@@ -38,7 +38,8 @@ BATCH_EXPORTS = [
 ]
 # Every symbol include/voxmi_pool.h declares.
 POOL_EXPORTS = [
-    "vox_brick_pool_create", "vox_brick_pool_create_wide", "vox_brick_pool_destroy", "vox_brick_pool_get_error",
+    "vox_brick_pool_create", "vox_brick_pool_create_wide", "vox_brick_pool_create_colour",
+    "vox_brick_pool_destroy", "vox_brick_pool_get_error",
     "vox_brick_pool_get_view", "vox_octree_create", "vox_octree_destroy", "vox_octree_get_error",
     "vox_octree_get_view", "vox_pool_copy_to_host",
 ]
@@ -303,6 +304,8 @@ def load():
     L.vox_brick_pool_create.argtypes = [u3, u3]
     L.vox_brick_pool_create_wide.restype = vp
     L.vox_brick_pool_create_wide.argtypes = [u3, C.c_uint64]
+    L.vox_brick_pool_create_colour.restype = vp
+    L.vox_brick_pool_create_colour.argtypes = [u3, C.c_uint64]
     L.vox_brick_pool_destroy.argtypes = [vp]
     L.vox_brick_pool_destroy.restype = None
     L.vox_brick_pool_get_error.restype = C.c_char_p

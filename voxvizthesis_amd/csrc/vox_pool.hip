@@ -209,7 +209,7 @@ __global__ void k_brick_upload(BrickSrc src, const uint4* __restrict__ req,
             const uint32_t z = yz / ey;
             const uint64_t t = ((uint64_t)(oz + z) * dy + (oy + y)) * dx + (ox + x);
             col[t] = sc[v];
-            nrm[t] = sn[v];
+            if (nrm) nrm[t] = sn[v];   // a colour-only pool keeps no normals
         }
         if (threadIdx.x == 0 && q.w != kNoTexel) {
             tex[2 * (uint64_t)q.w] |= kResidentBit;
@@ -524,7 +524,9 @@ vox_brick_pool* vox_brick_pool_create(const uint32_t brick_dim[3], const uint32_
     return bp;
 }
 
-vox_brick_pool* vox_brick_pool_create_wide(const uint32_t brick_dim[3], uint64_t nslots) {
+namespace {
+
+vox_brick_pool* create_wide(const uint32_t brick_dim[3], uint64_t nslots, bool normals) {
     vox_brick_pool* bp = new (std::nothrow) vox_brick_pool;
     if (!bp) return nullptr;
     if (!brick_dim) {
@@ -560,13 +562,23 @@ vox_brick_pool* vox_brick_pool_create_wide(const uint32_t brick_dim[3], uint64_t
     if (!bp->check(hipStreamCreateWithFlags(&bp->st, hipStreamNonBlocking), "hipStreamCreate"))
         return bp;
     if (!bp->col.ensure(4 * total, bp->err, "brick pool colors") ||
-        !bp->nrm.ensure(4 * total, bp->err, "brick pool normals"))
+        (normals && !bp->nrm.ensure(4 * total, bp->err, "brick pool normals")))
         return bp;
     launch_fill_u32(bp->st, bp->col.as<uint32_t>(), total, 0u);
-    launch_fill_u32(bp->st, bp->nrm.as<uint32_t>(), total, 0u);
+    if (normals) launch_fill_u32(bp->st, bp->nrm.as<uint32_t>(), total, 0u);
     if (!bp->check(hipStreamSynchronize(bp->st), "synchronize")) return bp;
     bp->valid = true;
     return bp;
+}
+
+}  // namespace
+
+vox_brick_pool* vox_brick_pool_create_wide(const uint32_t brick_dim[3], uint64_t nslots) {
+    return create_wide(brick_dim, nslots, true);
+}
+
+vox_brick_pool* vox_brick_pool_create_colour(const uint32_t brick_dim[3], uint64_t nslots) {
+    return create_wide(brick_dim, nslots, false);
 }
 
 void vox_brick_pool_destroy(vox_brick_pool* bp) {

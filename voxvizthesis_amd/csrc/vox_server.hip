@@ -371,7 +371,7 @@ __global__ void k_srv_upload(const TreeTab* __restrict__ tabs, const uint4* __re
             const uint32_t z = yz / ey;
             const uint64_t t = ((uint64_t)(oz + z) * dy + (oy + y)) * dx + (ox + x);
             col[t] = sc[v];
-            nrm[t] = sn[v];
+            if (nrm) nrm[t] = sn[v];   // a colour-only pool keeps no normals
         }
         if (threadIdx.x == 0) {
             T->tex[2 * (uint64_t)q.w] |= kResidentBit;

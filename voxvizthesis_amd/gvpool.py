@@ -22,13 +22,16 @@ class BrickPool:
     for bricks wider than 14 voxels, else 16^3 (GigaVoxelsRenderer.cpp 259-272).
     """
 
-    def __init__(self, brickDim, slots=None, wide_slots=None):
+    def __init__(self, brickDim, slots=None, wide_slots=None, normals: bool = True):
         """wide_slots: a wide pool of that many slots instead
         (vox_brick_pool_create_wide: beyond the 10-bit texel pointers, word 1
-        of a resident node holds its slot)."""
+        of a resident node holds its slot); with normals=False a colour-only
+        wide pool (vox_brick_pool_create_colour)."""
         self._L = _lib.load()
         u3 = C.c_uint32 * 3
-        if wide_slots is not None:
+        if wide_slots is not None and not normals:
+            self._h = self._L.vox_brick_pool_create_colour(u3(*brickDim), int(wide_slots))
+        elif wide_slots is not None:
             self._h = self._L.vox_brick_pool_create_wide(u3(*brickDim), int(wide_slots))
         else:
             self._h = self._L.vox_brick_pool_create(u3(*brickDim),
@@ -53,11 +56,15 @@ class BrickPool:
         return tuple(self.view().dims)
 
     def textures(self):
-        """(colors, normals) as host arrays of shape (z, y, x), uint32."""
+        """(colors, normals) as host arrays of shape (z, y, x), uint32
+        (normals None for a colour-only pool)."""
         v = self.view()
         n = v.dims[0] * v.dims[1] * v.dims[2]
         out = []
         for ptr in (v.colors, v.normals):
+            if not ptr:
+                out.append(None)
+                continue
             a = np.empty(n, np.uint32)
             if not self._L.vox_pool_copy_to_host(a.ctypes.data, ptr, a.nbytes):
                 raise RuntimeError("copy failed")

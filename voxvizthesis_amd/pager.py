@@ -167,7 +167,7 @@ class TilePager:
                  max_uploads: int = 65536, max_loading: int = 4, stale_frames: int = 100,
                  streams: int = 1, max_width: int = 1920, max_height: int = 1080,
                  near: float = 1e-3, far: float = 100.0, options: int = 0,
-                 serve_group: int = 3, f2b_stop=None):
+                 serve_group: int = 3, f2b_stop=None, pool_normals: bool = True):
         self._L = _lib.load()
         self.meshes = list(meshes)
         # f2b_stop: None casts and blends every tree separately, far to near
@@ -175,6 +175,9 @@ class TilePager:
         # one front-to-back pass, a pixel skipping the trees behind it once
         # its transmittance is at most f2b_stop (0: none skipped)
         self.f2b_stop = f2b_stop
+        # pool_normals False: a colour-only wide pool (the ray cast samples
+        # colour only), twice the slots in the same memory
+        self.pool_normals = bool(pool_normals)
         self.N, self.B = int(N), int(B)
         self.gen = PagedOctTreeGenerator(self.meshes, voxel_size, self.N, self.B)
         self.max_loading, self.stale = int(max_loading), int(stale_frames)
@@ -259,7 +262,8 @@ class TilePager:
         side = int(round((8 * blocks) ** (1.0 / 3.0))) + 2
         self.cap = 8 * blocks + 2 * side * side + 64
         if isinstance(pool_slots, (int, np.integer)):
-            self.pool = BrickPool((self.B,) * 3, wide_slots=int(pool_slots))
+            self.pool = BrickPool((self.B,) * 3, wide_slots=int(pool_slots),
+                                  normals=getattr(self, "pool_normals", True))
         else:
             self.pool = BrickPool((self.B,) * 3, pool_slots)
         self.server = BrickServer.shared(self.pool, max_trees, self.cap, max_uploads)

@@ -169,7 +169,7 @@ class TerrainPager(TilePager):
                  max_uploads: int = 65536, max_loading: int = 4, stale_frames: int = 100,
                  streams: int = 2, max_width: int = 2560, max_height: int = 1440,
                  near: float = 1.0, far: float = 40000.0, options: int = 0,
-                 serve_group: int = 3, f2b_stop=None):
+                 serve_group: int = 3, f2b_stop=None, pool_normals: bool = True):
         from concurrent.futures import ThreadPoolExecutor
         self._L = _lib.load()
         self.meshes = []
@@ -178,6 +178,7 @@ class TerrainPager(TilePager):
         self.max_loading, self.stale = int(max_loading), int(stale_frames)
         self.serve_group = int(serve_group)
         self.f2b_stop = f2b_stop
+        self.pool_normals = bool(pool_normals)
         self.streams, self.W, self.H = int(streams), int(max_width), int(max_height)
         self.near, self.far, self.options = float(near), float(far), int(options)
         self.frame_index = 0
