@@ -105,12 +105,21 @@ __device__ __forceinline__ void unpack_rgba(uint32_t c, float o[4]) {
 
 // texture(BrickSampler, s): trilinear, clamp to edge.  unorm8 channel c
 // reads u8[c] = c / 255 (a table of the correctly rounded quotients); the
-// pool has at most 2^30 texels, so texel indices fit 32 bits.
-__device__ void sample_brick(const uint32_t* __restrict__ col, const RayUniforms& U,
-                             const float* __restrict__ u8, v3 s, float r[4]) {
+// pool has at most 2^30 texels, so texel indices fit 32 bits.  The eight
+// texels are loaded before any is used (one memory round trip per sample),
+// then weighted in corner order 0..7.
+// A node texture or brick pool in device memory.  Typed so, loads through
+// it are global loads also where the pointer itself was read from memory (an
+// octree table), which the compiler would otherwise load with flat ones.
+typedef const __attribute__((address_space(1))) uint32_t* gu32;
+
+__device__ __forceinline__ void sample_brick(gu32 __restrict__ col,
+                                             const RayUniforms& U,
+                                             const float* __restrict__ u8, v3 s, float r[4]) {
     float c[3] = {s.x, s.y, s.z};
     int i0[3];
     float w[3];
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         float u = c[a];
         if (!(u == u)) u = 0.0f;
@@ -119,22 +128,29 @@ __device__ void sample_brick(const uint32_t* __restrict__ col, const RayUniforms
         w[a] = rintf((b - f) * 256.0f) / 256.0f;
         i0[a] = (int)fmaxf(fminf(f, 1.0e9f), -1.0e9f);
     }
-    r[0] = r[1] = r[2] = r[3] = 0.0f;
+    uint32_t t[8];
+#pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
         int ix = i0[0] + dx, iy = i0[1] + dy, iz = i0[2] + dz;
         ix = ix < 0 ? 0 : (ix >= (int)U.pdim[0] ? (int)U.pdim[0] - 1 : ix);
         iy = iy < 0 ? 0 : (iy >= (int)U.pdim[1] ? (int)U.pdim[1] - 1 : iy);
         iz = iz < 0 ? 0 : (iz >= (int)U.pdim[2] ? (int)U.pdim[2] - 1 : iz);
+        t[k] = col[((uint32_t)iz * U.pdim[1] + (uint32_t)iy) * U.pdim[0] + (uint32_t)ix];
+    }
+    r[0] = r[1] = r[2] = r[3] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
         const float wx = dx ? w[0] : 1.0f - w[0];
         const float wy = dy ? w[1] : 1.0f - w[1];
         const float wz = dz ? w[2] : 1.0f - w[2];
         const float wk = (wx * wy) * wz;
-        const uint32_t c = col[((uint32_t)iz * U.pdim[1] + (uint32_t)iy) * U.pdim[0] + (uint32_t)ix];
-        r[0] = r[0] + wk * u8[c & 255u];
-        r[1] = r[1] + wk * u8[(c >> 8) & 255u];
-        r[2] = r[2] + wk * u8[(c >> 16) & 255u];
-        r[3] = r[3] + wk * u8[c >> 24];
+        const uint32_t v = t[k];
+        r[0] = r[0] + wk * u8[v & 255u];
+        r[1] = r[1] + wk * u8[(v >> 8) & 255u];
+        r[2] = r[2] + wk * u8[(v >> 16) & 255u];
+        r[3] = r[3] + wk * u8[v >> 24];
     }
 }
 
@@ -146,7 +162,7 @@ __device__ __forceinline__ void blend(float dst[4], const float src[4]) {
     dst[3] = k * src[3] + dst[3];
 }
 
-__device__ __forceinline__ uint2 fetch_node(const uint32_t* __restrict__ tex, const RayUniforms& U,
+__device__ __forceinline__ uint2 fetch_node(gu32 __restrict__ tex, const RayUniforms& U,
                                             uint32_t x, uint32_t y, uint32_t z) {
     // texelFetch outside the texture reads 0 (never reached with valid pointers)
     if (x >= U.ndim[0] || y >= U.ndim[1] || z >= U.ndim[2]) return make_uint2(0u, 0u);
@@ -157,8 +173,8 @@ __device__ __forceinline__ uint2 fetch_node(const uint32_t* __restrict__ tex, co
 // One pixel's ray through one octree (GigaVoxels.frag main): its FragColor
 // in dst and its node usage list in usage (both start at zero here).
 // Returns whether the ray met the octree's box.
-__device__ __forceinline__ bool cast_pixel(const RayUniforms& U, const uint32_t* __restrict__ tex,
-                                           const uint32_t* __restrict__ pool,
+__device__ __forceinline__ bool cast_pixel(const RayUniforms& U, gu32 __restrict__ tex,
+                                           gu32 __restrict__ pool,
                                            const float* __restrict__ jitter,
                                            const float* __restrict__ u8, int i, int j,
                                            float (&dst)[4], uint32_t (&usage)[kMaxUsage]) {
@@ -277,7 +293,7 @@ __device__ __forceinline__ bool cast_pixel(const RayUniforms& U, const uint32_t*
                         // which need not be a slot: slot 0 is read then, where the
                         // reference reads whatever texel those bits name)
                         const uint32_t slot = curBrickPtr.y < U.wideSlots ? curBrickPtr.y : 0u;
-                        const uint32_t* bpool =
+                        gu32 bpool =
                             U.wide ? pool + (uint64_t)slot * U.pdim[0] * U.pdim[1] * U.pdim[2] : pool;
                         v3 cbr = add(bp, divv(mul(curBrickRayPos, ld3(U.brickDim)),
                                               ld3(U.poolDim)));
@@ -363,7 +379,7 @@ k_raycast(RayUniforms U, const uint32_t* __restrict__ tex, const uint32_t* __res
     if (i >= U.rx1 || j >= U.ry1) return;
     uint32_t usage[kMaxUsage];
     float dst[4];
-    cast_pixel(U, tex, pool, jitter, u8, i, j, dst, usage);
+    cast_pixel(U, (gu32)tex, (gu32)pool, jitter, u8, i, j, dst, usage);
     const uint64_t pix = (uint64_t)j * U.W + i;
     if (rgbaOut) {
         uint32_t o = 0;
@@ -467,7 +483,7 @@ k_raycast_multi(const TreeCast* __restrict__ trees, uint32_t n, const float* __r
         float dst[4];
         bool act = want && i >= U.rx0 && i < U.rx1 && j >= U.ry0 && j < U.ry1;
         if (act) {
-            act = cast_pixel(U, tc.tex, tc.pool, jitter, u8, i, j, dst, usage);
+            act = cast_pixel(U, (gu32)tc.tex, (gu32)tc.pool, jitter, u8, i, j, dst, usage);
         } else {
 #pragma unroll
             for (int q = 0; q < kMaxUsage; ++q) usage[q] = 0u;
