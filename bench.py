@@ -571,6 +571,8 @@ def main():
     ap.add_argument("--scale-rocks", type=int, default=100000)
     ap.add_argument("--scale-frames", type=int, default=300)
     ap.add_argument("--no-scale", action="store_true")
+    ap.add_argument("--scale-f2b", action="store_true",
+                    help="cast the scale leg's trees front to back in passes")
     ap.add_argument("--paged-trees", type=int, default=96,
                     help="octrees the paged leg keeps loaded (tree slots of the server)")
     ap.add_argument("--f2b-stop", type=float, default=1.0 / 255.0,
@@ -774,7 +776,8 @@ def main():
         log(f"[bench] paged serving at scale over a {args.scale_cells}^3 virtual grid ...")
         sb = scale_bench(args.scale_cells, args.scale_frames, args.serve2_width,
                          args.serve2_height, 1024, 16, args.scale_pool_gb, args.scale_rocks,
-                         args.paged_trees, 4, args.paged_group)
+                         args.paged_trees, 4, args.paged_group,
+                         args.f2b_stop if args.scale_f2b else None)
         out["paged_scale"] = sb
         log(f"[bench] scale: {sb['frame_ms']} ms/frame, {sb['tiles_voxelized']} tiles, "
             f"p99 {sb['latency_p99_ms']} ms")
