@@ -143,6 +143,10 @@ struct vox_ctx {
     std::string errCache;
     std::vector<HostGroup> groups;
     hipStream_t st = nullptr;
+    // work of a generate that needs none of the work queued beside it on st
+    // (occupancy bitmap, collapse), joined back into st before it returns
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
 
     // triangle memory
     uint64_t T = 0;
@@ -406,6 +410,9 @@ void vox_destroy(vox_ctx* h) {
     for (auto e : h->eventPool) (void)hipEventDestroy(e);
     if (h->evStart) (void)hipEventDestroy(h->evStart);
     if (h->evStop) (void)hipEventDestroy(h->evStop);
+    if (h->evFork) (void)hipEventDestroy(h->evFork);
+    if (h->evJoin) (void)hipEventDestroy(h->evJoin);
+    if (h->side) (void)hipStreamDestroy(h->side);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
 }
@@ -461,7 +468,11 @@ int vox_init(vox_ctx* h) {
     if (!h || !h->valid || !idle(h)) return 0;
     if (h->inited) return 1;
     if (!h->check(hipSetDevice(0), "hipSetDevice")) return 0;
-    if (!h->check(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking), "hipStreamCreate")) return 0;
+    if (!h->check(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking), "hipStreamCreate") ||
+        !h->check(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking), "hipStreamCreate") ||
+        !h->check(hipEventCreateWithFlags(&h->evFork, hipEventDisableTiming), "hipEventCreate") ||
+        !h->check(hipEventCreateWithFlags(&h->evJoin, hipEventDisableTiming), "hipEventCreate"))
+        return 0;
     (void)hipEventCreate(&h->evStart);
     (void)hipEventCreate(&h->evStop);
     int cus = 0;
@@ -971,6 +982,51 @@ int gen_back(vox_ctx* h, const char* out_dir, int binary) {
     uint32_t* ctr = h->counters.as<uint32_t>();
     unsigned long long* satc = h->sat.as<unsigned long long>();
     const int firstOverlap = first_overlap(h);
+    // The occupancy bitmap needs level 0 only, and the collapse the node
+    // types only: both go to the side stream, where they fill the wait for
+    // the brick counts and run beside the brick gather, and are joined back
+    // into st before the counters are read (on every return).  A timing
+    // run, whose spans are events on st, and a captured generate keep
+    // everything on st.
+    const bool overlap = h->side && !h->replay && !(h->options & VOX_OPT_TIMING);
+    hipStream_t aux = overlap ? h->side : st;
+    struct SideJoin {
+        vox_ctx* h;
+        bool forked = false;
+        void fork() {
+            (void)hipEventRecord(h->evFork, h->st);
+            (void)hipStreamWaitEvent(h->side, h->evFork, 0);
+            forked = true;
+        }
+        void join() {
+            if (!forked) return;
+            (void)hipEventRecord(h->evJoin, h->side);
+            (void)hipStreamWaitEvent(h->st, h->evJoin, 0);
+            forked = false;
+        }
+        ~SideJoin() { join(); }
+    } side{h};
+    // Dense occupancy bitmap (launched once the node types are queued).
+    const uint64_t nvox = (uint64_t)h->S[0] * h->S[1] * h->S[2];
+    const uint64_t words = (nvox + 31) / 32;
+    const bool wantOcc = !(h->options & VOX_OPT_NO_OCCUPANCY);
+    if (wantOcc && !h->bitmap.ensure(words * 4, h->err, "occupancy bitmap")) return 0;
+    auto occupancy = [&] {
+        if (!wantOcc) return;
+        if (overlap) side.fork();
+        bool rows = false;
+        h->run("occupancy", [&] {
+            rows = launch_occupancy_rows(aux, P, l0.tileIdx.as<int32_t>(), h->occ0.as<uint32_t>(),
+                                         h->bitmap.as<uint32_t>());
+        });
+        if (!rows) {
+            h->run("fill", [&] { launch_fill_u32(aux, h->bitmap.as<uint32_t>(), words, 0u); });
+            h->run("occupancy", [&] {
+                launch_occupancy(aux, P, l0.active, l0.list.as<int32_t>(), h->occ0.as<uint32_t>(),
+                                 h->bitmap.as<uint32_t>());
+            });
+        }
+    };
     // Mip chain (computeMipMaps, Voxelizer.cu 491-594).
     for (int k = 1; k < L; ++k) {
         if (h->lv[k].aliasOf >= 0) continue;
@@ -1048,6 +1104,7 @@ int gen_back(vox_ctx* h, const char* out_dir, int binary) {
                                         h->nodesA.as<uint32_t>() + t.off, bcount + d);
             }
         });
+        occupancy();
         if (!h->check(hipMemcpyAsync(nb, bcount, 4 * L, hipMemcpyDeviceToHost, st), "brick counts") ||
             !h->check(hipStreamSynchronize(st), "node typing"))
             return 0;
@@ -1108,6 +1165,7 @@ int gen_back(vox_ctx* h, const char* out_dir, int binary) {
                 launch_range_counts(st, h->nodeExcl.as<uint32_t>(), offs, L, ctr + 32);
             });
         }
+        occupancy();
         if (h->replay) {
             std::memcpy(nb, h->planNb, sizeof(nb));
         } else {
@@ -1134,6 +1192,14 @@ int gen_back(vox_ctx* h, const char* out_dir, int binary) {
         dout.off[L] = h->allNodes;
         h->run("brick_assign", [&] {
             launch_brick_assign_all(st, h->nodeFlag.as<uint32_t>(), h->nodeExcl.as<uint32_t>(), dout);
+        });
+    }
+    // CollapseConstantBranches (Voxelizer.cu 701-774): the types are final.
+    if (overlap) side.fork();
+    for (int d = L - 2; d >= 0; --d) {
+        h->run("collapse", [&] {
+            launch_collapse(aux, h->tree[d].Rd, d + 1 == L - 1 ? 1 : 0,
+                            h->tree[d].types.as<uint8_t>(), h->tree[d + 1].types.as<uint8_t>());
         });
     }
     // The bricks of all depths in one launch, the deepest (largest) first.
@@ -1176,31 +1242,7 @@ int gen_back(vox_ctx* h, const char* out_dir, int binary) {
             });
         }
     }
-    // CollapseConstantBranches (Voxelizer.cu 701-774).
-    for (int d = L - 2; d >= 0; --d) {
-        h->run("collapse", [&] {
-            launch_collapse(st, h->tree[d].Rd, d + 1 == L - 1 ? 1 : 0, h->tree[d].types.as<uint8_t>(),
-                            h->tree[d + 1].types.as<uint8_t>());
-        });
-    }
-    // Dense occupancy bitmap.
-    const uint64_t nvox = (uint64_t)h->S[0] * h->S[1] * h->S[2];
-    const uint64_t words = (nvox + 31) / 32;
-    if (!(h->options & VOX_OPT_NO_OCCUPANCY)) {
-        if (!h->bitmap.ensure(words * 4, h->err, "occupancy bitmap")) return 0;
-        bool rows = false;
-        h->run("occupancy", [&] {
-            rows = launch_occupancy_rows(st, P, l0.tileIdx.as<int32_t>(), h->occ0.as<uint32_t>(),
-                                         h->bitmap.as<uint32_t>());
-        });
-        if (!rows) {
-            h->run("fill", [&] { launch_fill_u32(st, h->bitmap.as<uint32_t>(), words, 0u); });
-            h->run("occupancy", [&] {
-                launch_occupancy(st, P, l0.active, l0.list.as<int32_t>(), h->occ0.as<uint32_t>(),
-                                 h->bitmap.as<uint32_t>());
-            });
-        }
-    }
+    side.join();
     h->workReleased = false;
     if (h->replay) {
         h->haveResult = true;
@@ -1287,6 +1329,11 @@ int vox_reset(vox_ctx* h) {
     if (h->evStart) (void)hipEventDestroy(h->evStart);
     if (h->evStop) (void)hipEventDestroy(h->evStop);
     h->evStart = h->evStop = nullptr;
+    if (h->evFork) (void)hipEventDestroy(h->evFork);
+    if (h->evJoin) (void)hipEventDestroy(h->evJoin);
+    h->evFork = h->evJoin = nullptr;
+    if (h->side) (void)hipStreamDestroy(h->side);
+    h->side = nullptr;
     if (h->st) (void)hipStreamDestroy(h->st);
     h->st = nullptr;
     h->inited = false;
