@@ -6,9 +6,10 @@ import math
 
 import numpy as np
 
-from voxvizthesis_amd.gvserve import box_rect
+from voxvizthesis_amd.gvserve import box_rect, box_rects
 from voxvizthesis_amd.pager import (PagedTree, front_to_back, frustum_planes, load_order,
-                                    look_at_matrix, perspective, sphere_in_frustum)
+                                    look_at_matrix, perspective, sphere_in_frustum,
+                                    spheres_in_frustum)
 
 
 def planes_x(fov=90.0, near=0.1, far=100.0):
@@ -108,3 +109,65 @@ def test_front_to_back_order_agrees_with_every_ray():
             assert ranks == sorted(ranks), (pos, d)
             checked += len(hits) > 1
     assert checked > 3000
+
+
+def test_spheres_in_frustum_equals_one_at_a_time():
+    rng = np.random.default_rng(3)
+    for trial in range(4):
+        pos = rng.uniform(-50, 50, 3)
+        look = rng.normal(size=3)
+        up = np.cross(look, rng.normal(size=3))
+        planes = frustum_planes(perspective(rng.uniform(20, 90), rng.uniform(0.5, 2.0), 0.5, 400.0),
+                                look_at_matrix(pos, look, up))
+        centers = rng.uniform(-300, 300, (500, 3)).astype(np.float32)
+        radii = rng.uniform(0.1, 60.0, 500)
+        inside, dist = spheres_in_frustum(planes, centers, radii)
+        for k in range(500):
+            ok, d = sphere_in_frustum(planes, centers[k], float(radii[k]))
+            assert bool(inside[k]) == ok and float(dist[k]) == d, k
+        assert inside.any() and not inside.all()
+
+
+def _rect_one(cam, lo, hi):
+    # one box, corner by corner (float64, dots summed x, y, z)
+    pos = np.asarray(cam["position"], np.float64)
+    look = np.asarray(cam["look"], np.float64)
+    up = np.asarray(cam["up"], np.float64)
+    left = np.cross(up, look)
+    W, H = int(cam["width"]), int(cam["height"])
+    tan_y = math.tan(math.radians(float(cam["fov_y_deg"])) * 0.5)
+    tan_x = tan_y * W / H
+    xs, ys = [], []
+    for c in range(8):
+        p = [(hi if c & 1 else lo)[0] - pos[0], (hi if c & 2 else lo)[1] - pos[1],
+             (hi if c & 4 else lo)[2] - pos[2]]
+        d = lambda v: (p[0] * v[0] + p[1] * v[1]) + p[2] * v[2]
+        zc = d(look)
+        if zc <= 1e-9:
+            return None
+        xs.append(((-d(left) / zc) / tan_x + 1.0) * 0.5 * W)
+        ys.append(((d(up) / zc) / tan_y + 1.0) * 0.5 * H)
+    x0, y0 = max(math.floor(min(xs)) - 1, 0), max(math.floor(min(ys)) - 1, 0)
+    x1, y1 = min(math.ceil(max(xs)) + 1, W), min(math.ceil(max(ys)) + 1, H)
+    return () if (x1 <= x0 or y1 <= y0) else (x0, y0, x1, y1)
+
+
+def test_box_rects_equal_one_box_at_a_time():
+    rng = np.random.default_rng(5)
+    kinds = set()
+    for trial in range(4):
+        look = rng.normal(size=3)
+        look /= np.linalg.norm(look)
+        up = np.cross(look, rng.normal(size=3))
+        up /= np.linalg.norm(up)
+        cam = dict(position=rng.uniform(-5, 5, 3), look=look, up=up,
+                   fov_y_deg=float(rng.uniform(30, 90)), width=640, height=360)
+        lo = rng.uniform(-40, 40, (300, 3))
+        hi = lo + rng.uniform(0.5, 8.0, (300, 3))
+        got = box_rects(cam, lo, hi)
+        for k in range(300):
+            want = _rect_one(cam, lo[k], hi[k])
+            assert got[k] == want, (trial, k, got[k], want)
+            assert box_rect(cam, lo[k], hi[k]) == want
+            kinds.add(type(want).__name__ if want != () else "empty")
+    assert kinds == {"NoneType", "tuple", "empty"}

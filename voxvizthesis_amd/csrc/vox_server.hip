@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <new>
 #include <sstream>
 #include <string>
@@ -403,6 +404,17 @@ __global__ void k_srv_logcount(const uint32_t* __restrict__ nreq,
     if (threadIdx.x == 0 && blockIdx.x == 0) *logCount += *nreq;
 }
 
+// Slots holding a brick, added into *count (one atomic per wave).
+__global__ void k_srv_count_resident(uint32_t nslots, const uint32_t* __restrict__ slotNode,
+                                     uint32_t* __restrict__ count) {
+    uint32_t c = 0;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nslots;
+         s += gridDim.x * blockDim.x)
+        c += slotNode[s] != kNone ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o);
+    if ((threadIdx.x & 63u) == 0u && c) atomicAdd(count, c);
+}
+
 // Detach: the tree's slots become empty (taken first) and its nodes lose
 // their resident bits ...
 __global__ void k_srv_detach_slots(uint32_t nslots, uint32_t lo, uint32_t hi,
@@ -475,14 +487,19 @@ struct vox_server {
     DevBuf ring2, ringEmit2;  // the queue after a detach's compaction
     DevBuf flagT, exclT, scanTmpT, key, hist, sel, flagS, eqExclS, exclS, scanTmpS;
     DevBuf req, reqEmit, nreq, done, log, logCount;
+    DevBuf resident;          // slot count of the last stats call
     uint64_t logCap = 1u << 24;   // latencies kept (the first ones admitted)
     vox_server_stats stats{};
-    // HIP-event pairs timing the server frames, used in turn: a frame waits
-    // on the host only for the frame kTimingRing frames back, so the host can
-    // queue frames ahead of the device
-    static constexpr int kTimingRing = 8;
-    hipEvent_t ev[kTimingRing][2] = {};
-    int evNext = 0, evPending = 0;   // next pair; pairs recorded, not settled
+    // HIP-event pairs timing the server frames: the pairs of queued frames
+    // (oldest first) and the free ones.  A frame settles the finished pairs
+    // without waiting and takes a free pair, or makes one, so the host can
+    // queue many frames ahead of the device (a pager queues a server frame
+    // after every few ray casts); only past kMaxQueued pairs does it wait
+    // for the oldest.
+    struct EvPair { hipEvent_t a = nullptr, b = nullptr; };
+    static constexpr size_t kMaxQueued = 512;
+    std::deque<EvPair> evQueued;
+    std::vector<EvPair> evFree;
     std::stringstream err;
     std::string errCache;
     bool fail(const std::string& m) {
@@ -499,19 +516,55 @@ struct vox_server {
             if (trees[i] == t) return (int)i;
         return -1;
     }
-    // Add the times of recorded frames, oldest first: all of them (waiting
-    // for each) with all = true, else only down to `keep` frames still
-    // pending, waiting only for those it must settle.
-    void settle_timing(bool all = true, int keep = 0) {
-        while (evPending > (all ? 0 : keep)) {
-            const int i = (evNext - evPending + kTimingRing) % kTimingRing;
-            (void)hipEventSynchronize(ev[i][1]);
+    // Add the times of queued frames, oldest first: all of them, waiting
+    // for each, with all = true; else the finished ones, without waiting,
+    // and then, while more than `keep` are queued, the oldest, waiting.
+    void settle_timing(bool all = true, size_t keep = kMaxQueued) {
+        while (!evQueued.empty()) {
+            const EvPair pr = evQueued.front();
+            if (all || evQueued.size() > keep) {
+                (void)hipEventSynchronize(pr.b);
+            } else if (hipEventQuery(pr.b) != hipSuccess) {
+                (void)hipGetLastError();   // not ready is no error of the next launch
+                break;
+            }
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, ev[i][0], ev[i][1]);
+            (void)hipEventElapsedTime(&ms, pr.a, pr.b);
             stats.ms_update_last = ms;
             stats.ms_update_total += ms;
-            --evPending;
+            evQueued.pop_front();
+            evFree.push_back(pr);
         }
+    }
+    // A pair for the next frame (settles what it can first).
+    bool take_pair(EvPair& out) {
+        settle_timing(false, kMaxQueued - 1);
+        if (evFree.empty()) {
+            EvPair pr;
+            if (!check(hipEventCreate(&pr.a), "hipEventCreate")) return false;
+            if (!check(hipEventCreate(&pr.b), "hipEventCreate")) {
+                (void)hipEventDestroy(pr.a);
+                return false;
+            }
+            evFree.push_back(pr);
+        }
+        out = evFree.back();
+        evFree.pop_back();
+        return true;
+    }
+    void destroy_events() {
+        for (auto* v : {&evFree}) {
+            for (auto& pr : *v) {
+                (void)hipEventDestroy(pr.a);
+                (void)hipEventDestroy(pr.b);
+            }
+            v->clear();
+        }
+        for (auto& pr : evQueued) {
+            (void)hipEventDestroy(pr.a);
+            (void)hipEventDestroy(pr.b);
+        }
+        evQueued.clear();
     }
 };
 
@@ -622,7 +675,8 @@ vox_server* vox_server_create_shared(vox_brick_pool* bp, uint32_t max_trees,
         !s->reqEmit.ensure(8ull * s->maxUp, s->err, "reqEmit") || !s->nreq.ensure(4, s->err, "nreq") ||
         !s->done.ensure(8ull * s->maxUp, s->err, "done") ||
         !s->log.ensure(8 * s->logCap, s->err, "latency log") ||
-        !s->logCount.ensure(8, s->err, "latency count"))
+        !s->logCount.ensure(8, s->err, "latency count") ||
+        !s->resident.ensure(4, s->err, "resident count"))
         return s;
     hipStream_t st = s->st;
     launch_fill_u32(st, s->tabs.as<uint32_t>(), sizeof(TreeTab) / 4 * max_trees, 0u);
@@ -636,9 +690,6 @@ vox_server* vox_server_create_shared(vox_brick_pool* bp, uint32_t max_trees,
     launch_fill_u32(st, s->logCount.as<uint32_t>(), 2, 0u);
     // the pool's own stream may still be clearing it
     (void)hipStreamSynchronize(s->pool.st);
-    for (auto& pr : s->ev)
-        for (auto& e : pr)
-            if (!s->check(hipEventCreate(&e), "hipEventCreate")) return s;
     int khz = 0;
     int dev = 0;
     (void)hipGetDevice(&dev);
@@ -669,15 +720,13 @@ vox_server* vox_server_create(vox_octree* tree, uint32_t max_uploads_per_frame) 
 void vox_server_destroy(vox_server* s) {
     if (!s) return;
     if (s->st) (void)hipStreamSynchronize(s->st);
-    for (auto& pr : s->ev)
-        for (auto e : pr)
-            if (e) (void)hipEventDestroy(e);
+    s->destroy_events();
     DevBuf* bufs[] = {&s->tabs, &s->info, &s->nodeSlot, &s->slotNode, &s->slotLast, &s->pend,
                       &s->pendEmit, &s->ring, &s->ringEmit, &s->ctr, &s->tot, &s->conflict,
                       &s->ring2, &s->ringEmit2,
                       &s->flagT, &s->exclT, &s->scanTmpT, &s->key, &s->hist, &s->sel, &s->flagS,
                       &s->eqExclS, &s->exclS, &s->scanTmpS, &s->req, &s->reqEmit, &s->nreq,
-                      &s->done, &s->log, &s->logCount};
+                      &s->done, &s->log, &s->logCount, &s->resident};
     for (DevBuf* b : bufs) b->release();
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
@@ -739,15 +788,15 @@ int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n)
         slot[k] = (uint32_t)i;
     }
     hipStream_t st = s->st;
-    s->settle_timing(false, vox_server::kTimingRing - 1);  // frees the pair used below
-    const uint32_t frame = ++s->frame;
-    const uint64_t nt = s->ntex, ns = s->nslots;
-    TreeTab* tabs = s->tabs.as<TreeTab>();
     for (uint32_t k = 0; k < n; ++k)
         if (!s->check(hipStreamWaitEvent(st, lists[k].done, 0), "wait for a request list"))
             return 0;
-    const int evi = s->evNext;
-    (void)hipEventRecord(s->ev[evi][0], st);
+    vox_server::EvPair evp;
+    if (!s->take_pair(evp)) return 0;
+    const uint32_t frame = ++s->frame;
+    const uint64_t nt = s->ntex, ns = s->nslots;
+    TreeTab* tabs = s->tabs.as<TreeTab>();
+    (void)hipEventRecord(evp.a, st);
     for (uint32_t k = 0; k < n; ++k)
         k_srv_touch<<<grid_for(lists[k].maxIds), kB, 0, st>>>(
             lists[k].ids, lists[k].times, lists[k].count, lists[k].maxIds, slot[k],
@@ -808,9 +857,8 @@ int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n)
         s->done.as<unsigned long long>(), s->log.as<unsigned long long>(), s->logCap,
         s->logCount.as<unsigned long long>(), s->tot.as<unsigned long long>());
     k_srv_logcount<<<1, 64, 0, st>>>(s->nreq.as<uint32_t>(), s->logCount.as<unsigned long long>());
-    (void)hipEventRecord(s->ev[evi][1], st);
-    s->evNext = (evi + 1) % vox_server::kTimingRing;
-    ++s->evPending;
+    (void)hipEventRecord(evp.b, st);
+    s->evQueued.push_back(evp);
     if (!s->check(hipGetLastError(), "server kernels")) return 0;
     // ray casts of the next frame wait for the node textures and pool
     for (vox_octree* t : s->trees)
@@ -827,11 +875,16 @@ int vox_server_get_stats(vox_server* s, vox_server_stats* out) {
     if (!s || !s->valid || !out) return 0;
     uint32_t ctr[4] = {0, 0, 0, 0};
     unsigned long long tot[4] = {0, 0, 0, 0};
-    std::vector<uint32_t> sn(s->nslots);
-    if (!s->check(hipMemcpyAsync(ctr, s->ctr.p, 16, hipMemcpyDeviceToHost, s->st), "counters") ||
+    uint32_t resident = 0;
+    launch_fill_u32(s->st, s->resident.as<uint32_t>(), 1, 0u);
+    k_srv_count_resident<<<grid_for(s->nslots < 4096ull * kB ? s->nslots : 4096ull * kB), kB, 0,
+                           s->st>>>((uint32_t)s->nslots, s->slotNode.as<uint32_t>(),
+                                    s->resident.as<uint32_t>());
+    if (!s->check(hipGetLastError(), "resident count") ||
+        !s->check(hipMemcpyAsync(ctr, s->ctr.p, 16, hipMemcpyDeviceToHost, s->st), "counters") ||
         !s->check(hipMemcpyAsync(tot, s->tot.p, 32, hipMemcpyDeviceToHost, s->st), "totals") ||
-        !s->check(hipMemcpyAsync(sn.data(), s->slotNode.p, 4ull * s->nslots, hipMemcpyDeviceToHost,
-                                 s->st), "slots") ||
+        !s->check(hipMemcpyAsync(&resident, s->resident.p, 4, hipMemcpyDeviceToHost, s->st),
+                  "resident count") ||
         !s->check(hipStreamSynchronize(s->st), "synchronize"))
         return 0;
     s->settle_timing();
@@ -840,9 +893,7 @@ int vox_server_get_stats(vox_server* s, vox_server_stats* out) {
     s->stats.requests = tot[2];
     s->stats.pending = ctr[1] - ctr[0];
     s->stats.last_uploads = ctr[2];
-    uint32_t r = 0;
-    for (uint32_t v : sn) r += v != kNone;
-    s->stats.resident = r;
+    s->stats.resident = resident;
     *out = s->stats;
     return 1;
 }

@@ -44,11 +44,15 @@ def raycast_params(vol_min, vol_max, ray_step=0.0, lod_scalar=0.0, jitter_seed=0
     return p
 
 
-def box_rect(cam: dict, box_min, box_max):
-    """Pixel rectangle holding the projection of a world box for a camera
-    dict (position, look, up, fov_y_deg, width, height), with a one-pixel
-    margin; None when a corner lies at or behind the camera plane (then the
-    whole viewport is cast), () when the projection misses the viewport."""
+def box_rects(cam: dict, box_min, box_max) -> list:
+    """Pixel rectangles holding the projections of m world boxes (box_min,
+    box_max: (m, 3)) for a camera dict (position, look, up, fov_y_deg,
+    width, height), each with a one-pixel margin; None for a box with a
+    corner at or behind the camera plane (then the whole viewport is cast),
+    () for one whose projection misses the viewport.  Float64 throughout,
+    dot products summed x, y, z in that order."""
+    lo = np.asarray(box_min, np.float64).reshape(-1, 3)
+    hi = np.asarray(box_max, np.float64).reshape(-1, 3)
     pos = np.asarray(cam["position"], np.float64)
     look = np.asarray(cam["look"], np.float64)
     up = np.asarray(cam["up"], np.float64)
@@ -56,24 +60,38 @@ def box_rect(cam: dict, box_min, box_max):
     W, H = int(cam["width"]), int(cam["height"])
     tan_y = np.tan(np.radians(float(cam["fov_y_deg"])) * 0.5)
     tan_x = tan_y * W / H
-    xs, ys = [], []
-    for c in range(8):
-        p = np.array([(box_max if c & 1 else box_min)[0], (box_max if c & 2 else box_min)[1],
-                      (box_max if c & 4 else box_min)[2]], np.float64) - pos
-        zc = p.dot(look)
-        if zc <= 1e-9:
-            return None
-        sx = (-p.dot(left) / zc) / tan_x   # right = -left
-        sy = (p.dot(up) / zc) / tan_y
-        xs.append((sx + 1.0) * 0.5 * W)
-        ys.append((sy + 1.0) * 0.5 * H)
-    x0 = max(int(np.floor(min(xs))) - 1, 0)
-    y0 = max(int(np.floor(min(ys))) - 1, 0)
-    x1 = min(int(np.ceil(max(xs))) + 1, W)
-    y1 = min(int(np.ceil(max(ys))) + 1, H)
-    if x1 <= x0 or y1 <= y0:
-        return ()
-    return (x0, y0, x1, y1)
+    c = np.arange(8)
+    sel = np.stack([(c & 1) != 0, (c & 2) != 0, (c & 4) != 0], axis=1)       # (8, 3)
+    corners = np.where(sel[None], hi[:, None, :], lo[:, None, :]) - pos      # (m, 8, 3)
+
+    def dot(v):
+        return (corners[..., 0] * v[0] + corners[..., 1] * v[1]) + corners[..., 2] * v[2]
+
+    zc = dot(look)
+    behind = (zc <= 1e-9).any(axis=1)
+    zs = np.where(zc <= 1e-9, 1.0, zc)
+    sx = (-dot(left) / zs) / tan_x                                            # right = -left
+    sy = (dot(up) / zs) / tan_y
+    xs = (sx + 1.0) * 0.5 * W
+    ys = (sy + 1.0) * 0.5 * H
+    x0 = np.maximum(np.floor(xs.min(axis=1)).astype(np.int64) - 1, 0)
+    y0 = np.maximum(np.floor(ys.min(axis=1)).astype(np.int64) - 1, 0)
+    x1 = np.minimum(np.ceil(xs.max(axis=1)).astype(np.int64) + 1, W)
+    y1 = np.minimum(np.ceil(ys.max(axis=1)).astype(np.int64) + 1, H)
+    out = []
+    for k in range(lo.shape[0]):
+        if behind[k]:
+            out.append(None)
+        elif x1[k] <= x0[k] or y1[k] <= y0[k]:
+            out.append(())
+        else:
+            out.append((int(x0[k]), int(y0[k]), int(x1[k]), int(y1[k])))
+    return out
+
+
+def box_rect(cam: dict, box_min, box_max):
+    """box_rects for one box."""
+    return box_rects(cam, box_min, box_max)[0]
 
 
 class RayCaster:

@@ -33,7 +33,7 @@ import numpy as np
 
 from . import _lib
 from .gvpool import BrickPool, OctTree
-from .gvserve import (BrickServer, RayCaster, box_rect, camera, composite, raycast_front_to_back,
+from .gvserve import (BrickServer, RayCaster, box_rects, camera, composite, raycast_front_to_back,
                       raycast_params)
 from .tiler import PagedOctTreeGenerator
 from .voxelizer import Voxelizer
@@ -105,6 +105,23 @@ def sphere_in_frustum(planes: np.ndarray, center, radius: float):
         if d <= -radius:
             return False, d
     return True, d
+
+
+def spheres_in_frustum(planes: np.ndarray, centers, radii):
+    """sphere_in_frustum for n spheres at once (centers (n, 3), radii (n,)),
+    in the same float32 arithmetic: per sphere whether it is visible and its
+    distance to the near plane (visible) or to the first plane it lies
+    behind."""
+    c = np.asarray(centers, np.float32).reshape(-1, 3)
+    r = np.asarray(radii, np.float64).reshape(-1)
+    pl = np.asarray(planes).astype(np.float32)
+    d = ((pl[None, :, 0] * c[:, None, 0] + pl[None, :, 1] * c[:, None, 1])
+         + pl[None, :, 2] * c[:, None, 2]) + pl[None, :, 3]                 # (n, 6) float32
+    behind = d.astype(np.float64) <= -r[:, None]
+    inside = ~behind.any(axis=1)
+    first = np.where(inside, 5, behind.argmax(axis=1))
+    dist = d[np.arange(c.shape[0]), first].astype(np.float64)
+    return inside, dist
 
 
 def front_to_back(trees, pos):
@@ -361,16 +378,19 @@ class TilePager:
         for tr in self.trees:
             if tr.state == "loading" and tr._job.done():
                 self._finish_load(tr)
-        # cull per stream (CullVisitor)
+        # cull per stream (CullVisitor), every tree's sphere at once
+        if getattr(self, "_spheres", None) is None or len(self._spheres[1]) != len(self.trees):
+            self._spheres = (np.array([tr.center for tr in self.trees], np.float32).reshape(-1, 3),
+                             np.array([tr.radius for tr in self.trees], np.float64))
         drawn, requests = [], {}
         for cm in cams:
             proj = perspective(cm["fov_y_deg"], cm["width"] / cm["height"], self.near, self.far)
             planes = frustum_planes(proj, look_at_matrix(cm["position"], cm["look"], cm["up"]))
+            inside, dists = spheres_in_frustum(planes, *self._spheres)
             vis = []
-            for i, tr in enumerate(self.trees):
-                inside, dist = sphere_in_frustum(planes, tr.center, tr.radius)
-                if not inside:
-                    continue
+            for i in np.flatnonzero(inside).tolist():
+                tr = self.trees[i]
+                dist = float(dists[i])
                 tr.last_access = f
                 if tr.state == "loaded":
                     vis.append((dist - tr.radius, i))
@@ -403,14 +423,16 @@ class TilePager:
         rcs, out, group = [], [], []
         for k, (cm, trees) in enumerate(zip(cams, drawn)):
             row, cast = [], []
-            for tr in trees:
-                rect = box_rect(cm, tr.box_min, tr.box_max)
+            cam = camera(cm["position"], cm["look"], cm["up"], cm["fov_y_deg"], cm["width"],
+                         cm["height"], f)
+            rects = box_rects(cm, [tr.box_min for tr in trees], [tr.box_max for tr in trees]) \
+                if trees else []
+            for tr, rect in zip(trees, rects):
                 if rect == ():
                     continue                              # nothing on screen
                 rc = tr.rcs[k]
                 prm = raycast_params(tr.box_min, tr.box_max, rect=rect, **(prm_extra or {}))
-                rc.frame(camera(cm["position"], cm["look"], cm["up"], cm["fov_y_deg"],
-                                cm["width"], cm["height"], f), prm, wait=False)
+                rc.frame(cam, prm, wait=False)
                 rcs.append(rc)
                 cast.append(rc)
                 row.append((tr, rc))
@@ -451,8 +473,10 @@ class TilePager:
         rcs, out = [], []
         for k, (cm, trees) in enumerate(zip(cams, drawn)):
             row, cast, prms = [], [], []
-            for tr in front_to_back(trees, cm["position"]):
-                rect = box_rect(cm, tr.box_min, tr.box_max)
+            order = front_to_back(trees, cm["position"])
+            rects = box_rects(cm, [tr.box_min for tr in order], [tr.box_max for tr in order]) \
+                if order else []
+            for tr, rect in zip(order, rects):
                 if rect == ():
                     continue                              # nothing on screen
                 prms.append(raycast_params(tr.box_min, tr.box_max, rect=rect,
