@@ -398,10 +398,11 @@ class TilePager:
                     requests[i] = min(requests.get(i, dist), dist)
             vis.sort(reverse=True)                       # far to near
             drawn.append([self.trees[i] for _, i in vis])
-        # unload trees unseen for stale_frames frames
-        for tr in self.trees:
-            if tr.state == "loaded" and f - tr.last_access > self.stale:
-                self._unload(tr)
+        # trees unseen for stale_frames frames are unloaded once this frame's
+        # casts are queued (none of them is drawn; freeing their device
+        # memory waits for the device, which then has the frame to run)
+        stale = [tr for tr in self.trees
+                 if tr.state == "loaded" and f - tr.last_access > self.stale]
         # start loads, most recent then nearest, as load slots free up
         loading = sum(t.state == "loading" for t in self.trees)
         order = load_order([(i, self.trees[i].last_access, d) for i, d in requests.items()])
@@ -413,7 +414,10 @@ class TilePager:
             self._start_load(tr)
             loading += 1
         if self.f2b_stop is not None:
-            return self._frame_f2b(cams, drawn, prm_extra, serve)
+            out = self._frame_f2b(cams, drawn, prm_extra, serve)
+            for tr in stale:
+                self._unload(tr)
+            return out
         # ray casts of every stream's visible trees.  With serve, a server
         # frame follows every serve_group casts: their requests are admitted
         # while the next trees are cast (which wait for that server frame's
@@ -453,6 +457,8 @@ class TilePager:
             self._cast = rcs
             if serve:
                 self.serve()
+        for tr in stale:
+            self._unload(tr)
         self.frame_index += 1
         return out
 
