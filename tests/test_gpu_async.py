@@ -87,3 +87,29 @@ def test_busy_handle_refuses_calls():
         assert int(np.unpackbits(r["occupancy"].view(np.uint8)).sum()) > 0
     finally:
         vz.close()
+
+
+def test_timing_run_equals_overlapped_run():
+    """A generate queues its occupancy bitmap and collapse on a second stream
+    beside the mip chain and the brick gather; a timing run keeps them on the
+    handle's stream.  Both give the same result, run after run on one
+    handle, including the occupancy bitmap and the collapsed types."""
+    from voxvizthesis_amd._lib import VOX_OPT_TIMING
+    for sc, B in ((scenes.scene_small_terrain(), 8), (scenes.scene_mixed(seed=9, N=64, B=4), 4)):
+        print(f"[async] timing vs overlapped {sc.name}", flush=True)
+        vz = upload(sc, B)
+        try:
+            assert vz.generateVoxelsAndOctTree(None) == 1, vz.getErrorMessage()
+            ref = snapshot(vz)
+            vz._L.vox_set_options(vz._h, VOX_OPT_TIMING)
+            assert vz.computeEdgesFaceNormalsAndBounds()
+            assert vz.generateVoxelsAndOctTree(None) == 1, vz.getErrorMessage()
+            assert same(snapshot(vz), ref)
+            vz._L.vox_set_options(vz._h, 0)
+            for _ in range(2):
+                assert vz.computeEdgesFaceNormalsAndBounds()
+                assert vz.generateVoxelsAndOctTree(None) == 1, vz.getErrorMessage()
+                assert same(snapshot(vz), ref)
+            assert any(t.any() for t in ref[1::5])   # some node typed
+        finally:
+            vz.close()
