@@ -240,14 +240,17 @@ __global__ void k_srv_keys(uint32_t nslots, uint32_t frame, const uint32_t* __re
             }
             key[s] = k;
         }
+        // the wave's most common keys (runs of equal ages) by ballot, at
+        // most four rounds; the rest one LDS atomic per lane
         uint64_t pend = __ballot(valid);
-        while (pend) {
+        for (int round = 0; round < 4 && pend; ++round) {
             const int l = __ffsll((unsigned long long)pend) - 1;
             const uint32_t lk = __shfl(k, l);
-            const uint64_t same = __ballot(valid && k == lk);
+            const uint64_t same = __ballot(valid && k == lk) & pend;
             if (lane == l) atomicAdd(&h[lk], (uint32_t)__popcll(same));
             pend &= ~same;
         }
+        if ((pend >> lane) & 1ull) atomicAdd(&h[k], 1u);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kKeys; i += blockDim.x)
@@ -824,7 +827,10 @@ int vox_server_frame_multi(vox_server* s, vox_raycaster* const* rcs, uint32_t n)
         s->ringEmit.as<unsigned long long>(), (uint32_t)nt, s->ctr.as<uint32_t>());
     k_srv_tail<<<1, 64, 0, st>>>(s->exclT.as<uint32_t>(), nv, s->ctr.as<uint32_t>());
     launch_fill_u32(st, s->hist.as<uint32_t>(), kKeys, 0u);
-    k_srv_keys<<<grid_for(ns), kB, 0, st>>>((uint32_t)ns, frame, s->slotNode.as<uint32_t>(),
+    // a bounded grid: each workgroup adds its histogram to the global one
+    // (257 atomics), so fewer workgroups mean fewer same-address atomics
+    const unsigned kgrid = grid_for(ns) < 1024u ? grid_for(ns) : 1024u;
+    k_srv_keys<<<kgrid, kB, 0, st>>>((uint32_t)ns, frame, s->slotNode.as<uint32_t>(),
                                              s->slotLast.as<uint32_t>(), s->key.as<uint32_t>(),
                                              s->hist.as<uint32_t>());
     k_srv_threshold<<<1, 64, 0, st>>>(s->hist.as<uint32_t>(), s->ctr.as<uint32_t>(), s->maxUp,
