@@ -520,9 +520,12 @@ k_requests(const uint32_t* __restrict__ usage, uint32_t W, int32_t rx0, int32_t 
         const uint64_t e = base + threadIdx.x;
         uint32_t id = 0u;
         if (e < n) {
-            const uint64_t p = e / kMaxUsage;
-            const uint32_t k = (uint32_t)(e - p * kMaxUsage);
-            const uint64_t px = (uint64_t)rx0 + p % rw, py = (uint64_t)ry0 + p / rw;
+            // n < 2^32 (checked at launch): 32-bit index arithmetic
+            const uint32_t e32 = (uint32_t)e;
+            const uint32_t p = e32 / (uint32_t)kMaxUsage;
+            const uint32_t k = e32 - p * (uint32_t)kMaxUsage;
+            const uint32_t prow = p / rw;
+            const uint64_t px = (uint64_t)rx0 + (p - prow * rw), py = (uint64_t)ry0 + prow;
             id = usage[(py * W + px) * kMaxUsage + k];
         }
         // wave-level deduplication
@@ -850,6 +853,8 @@ int vox_raycast_frame(vox_raycaster* rc, const vox_camera* cam, const vox_raycas
     rc->H = cam->height;
     const uint32_t rw = (uint32_t)(U.rx1 - U.rx0), rh = (uint32_t)(U.ry1 - U.ry0);
     const uint64_t npix = (uint64_t)rw * rh;
+    if (npix * kMaxUsage >= (1ull << 32))
+        return rc->fail("raycast: 2^32 usage entries or more"), 0;
     if (++rc->tag == 0) {  // stamp wrap: clear
         launch_fill_u32(st, rc->stamp.as<uint32_t>(), (uint64_t)nd[0] * nd[1] * nd[2], 0u);
         rc->tag = 1;
