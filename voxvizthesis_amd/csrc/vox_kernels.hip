@@ -2426,25 +2426,43 @@ __device__ __forceinline__ uint32_t chunk_typed_limit(uint32_t Ck, uint32_t o, u
 }
 
 // ComputeOctTreeNodeConstColor + ComputeOctTreeNodeType per chunk
-// (VoxelizerKernels.cu 1680-2380, Voxelizer.cu 931-1172), one workgroup per
+// (VoxelizerKernels.cu 1680-2380, Voxelizer.cu 931-1172), one wave per
 // node: for every chunk the node's stored brick reaches, the chunk's sample
 // (the brick's first interior voxel clamped into the chunk) against the
 // brick's voxels in that chunk that the chunk's typing launch reaches; the
 // node is NON-CONST when any differs (types OR-ed over chunks), and its
 // colour is the sample of the last chunk in visit order.
+//
+// A brick window with no active tile under it reads zero everywhere (every
+// sample lies in the window, so the colour is zero too): CONST, colour 0,
+// without further reads.  Otherwise the window's voxel offsets u along
+// each axis map to a chunk slot j (the chunk holding lo + u, counted from
+// the window's first) and to whether that chunk's typing launch reaches
+// them (kChunkTypedBit); the chunks' samples are read once per node (slot
+// triple -> colour) and the wave stops at the first voxel that differs.
+// Windows wider than kChunkAxis or with more than kChunkSamples slot
+// triples read every voxel's sample directly.
+constexpr uint32_t kChunkAxis = 66, kChunkSamples = 64, kChunkTypedBit = 1u << 31;
+constexpr int kChunkWaves = kBlock / 64;
 __global__ __launch_bounds__(kBlock) void k_chunk_node_type(
     LevelMap lm, const uint32_t* __restrict__ col, ChunkGeom cg, int k, uint32_t Rd, uint32_t B0,
     uint32_t B1, uint32_t B2, uint32_t bo0, uint32_t bo1, uint32_t bo2, uint32_t* __restrict__ cc,
     uint32_t* __restrict__ flag) {
-    __shared__ uint32_t s_any, s_diff;
+    __shared__ uint32_t s_ax[kChunkWaves][3][kChunkAxis];
+    __shared__ uint32_t s_sco[kChunkWaves][3][kChunkSamples];
+    __shared__ uint32_t s_samp[kChunkWaves][kChunkSamples];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t Ck[3] = {cg.C[0] >> k, cg.C[1] >> k, cg.C[2] >> k}, Bv[3] = {B0, B1, B2},
                    bo[3] = {bo0, bo1, bo2};
+    const uint32_t e[3] = {Bv[0] + 2u, Bv[1] + 2u, Bv[2] + 2u};
+    const uint32_t full[3] = {Rd * Bv[0], Rd * Bv[1], Rd * Bv[2]};
     const uint64_t nn = (uint64_t)Rd * Rd * Rd;
     const uint32_t ld = (uint32_t)(__ffs(Rd) - 1);
-    for (uint64_t n = blockIdx.x; n < nn; n += gridDim.x) {
+    for (uint64_t n = (uint64_t)blockIdx.x * kChunkWaves + wv; n < nn;
+         n += (uint64_t)gridDim.x * kChunkWaves) {
         uint32_t nd[3];
         node_xyz(n, ld, nd[0], nd[1], nd[2]);
-        uint32_t lo[3], hi[3], c0[3], c1[3], sp[3], cov[3];
+        uint32_t lo[3], hi[3], c0[3], c1[3], sp[3], cov[3], m[3];
         bool inAny = true;
         for (int a = 0; a < 3; ++a) {
             sp[a] = nd[a] * Bv[a] + bo[a];
@@ -2453,78 +2471,172 @@ __global__ __launch_bounds__(kBlock) void k_chunk_node_type(
             cov[a] = cg.nc[a] * Ck[a];
             c0[a] = lo[a] / Ck[a];
             c1[a] = min(hi[a] / Ck[a], cg.nc[a] - 1u);
+            m[a] = (lo[a] + e[a] - 1u) / Ck[a] - c0[a] + 1u;
             if (c0[a] >= cg.nc[a]) inAny = false;
         }
-        // the constant colour: the last chunk in visit order
-        if (threadIdx.x == 0) {
-            uint32_t best = 0u, bestRank = 0u;
-            bool have = false;
-            if (inAny)
-                for (uint32_t k = c0[2]; k <= c1[2]; ++k)
-                    for (uint32_t j = c0[1]; j <= c1[1]; ++j)
-                        for (uint32_t i = c0[0]; i <= c1[0]; ++i) {
-                            const uint32_t r = cg.rank[((uint64_t)k * cg.nc[1] + j) * cg.nc[0] + i];
-                            if (!have || r > bestRank) {
-                                have = true;
-                                bestRank = r;
-                                const uint32_t ci[3] = {i, j, k};
-                                uint32_t q[3];
-                                for (int a = 0; a < 3; ++a)
-                                    q[a] = min(max(sp[a], ci[a] * Ck[a]), ci[a] * Ck[a] + Ck[a] - 1u);
-                                best = level_col(lm, col, q[0], q[1], q[2]);
-                            }
-                        }
-            cc[n] = best;
-            s_any = 0u;
-            s_diff = 0u;
-        }
-        __syncthreads();
-        // every voxel of the brick zero (no active tile under it) and so every
-        // sample zero: CONST without reading the voxels
+        // any active tile under the window
+        bool any = false;
         if (inAny) {
             const uint32_t t0[3] = {lo[0] >> 3, lo[1] >> 3, lo[2] >> 3};
             uint32_t t1[3];
             for (int a = 0; a < 3; ++a) t1[a] = min(hi[a], cov[a] - 1u) >> 3;
             const uint32_t ntx = t1[0] - t0[0] + 1u, nty = t1[1] - t0[1] + 1u;
             const uint32_t ntl = ntx * nty * (t1[2] - t0[2] + 1u);
-            for (uint32_t q = threadIdx.x; q < ntl; q += blockDim.x) {
+            bool mine = false;
+            for (uint32_t q = lane; q < ntl; q += 64u) {
                 const uint32_t qx = q % ntx, qy = (q / ntx) % nty, qz = q / (ntx * nty);
                 const uint32_t tx = t0[0] + qx, ty = t0[1] + qy, tz = t0[2] + qz;
                 if (tx < lm.TG[0] && ty < lm.TG[1] && tz < lm.TG[2] &&
                     lm.tileIdx[((uint64_t)tz * lm.TG[1] + ty) * lm.TG[0] + tx] >= 0)
-                    s_any = 1u;
+                    mine = true;
             }
+            any = __ballot(mine) != 0ull;
         }
-        __syncthreads();
-        if (s_any) {
-            const uint32_t e[3] = {Bv[0] + 2u, Bv[1] + 2u, Bv[2] + 2u};
-            const uint32_t nv = e[0] * e[1] * e[2];
-            const uint32_t full[3] = {Rd * Bv[0], Rd * Bv[1], Rd * Bv[2]};
-            bool diff = false;
-            for (uint32_t q = threadIdx.x; q < nv && !diff; q += blockDim.x) {
-                const uint32_t P3[3] = {lo[0] + q % e[0], lo[1] + (q / e[0]) % e[1],
-                                        lo[2] + q / (e[0] * e[1])};
-                bool typed = true;
-                uint32_t s3[3];
-                for (int a = 0; a < 3; ++a) {
-                    if (P3[a] >= cov[a]) { typed = false; break; }
-                    const uint32_t ci = P3[a] / Ck[a], o = ci * Ck[a];
-                    if (P3[a] - o >= chunk_typed_limit(Ck[a], o, bo[a], full[a], a == 0)) {
-                        typed = false;
-                        break;
+        if (!any) {
+            if (lane == 0) {
+                cc[n] = 0u;
+                flag[n] = 0u;
+            }
+            continue;
+        }
+        const bool tables = e[0] <= kChunkAxis && e[1] <= kChunkAxis && e[2] <= kChunkAxis &&
+                            m[0] * m[1] * m[2] <= kChunkSamples;   // uniform
+        bool diff = false;
+        if (tables) {
+            uint32_t (&ax)[3][kChunkAxis] = s_ax[wv];
+            uint32_t (&sco)[3][kChunkSamples] = s_sco[wv];
+            uint32_t* samp = s_samp[wv];
+            for (uint32_t t = lane; t < e[0] + e[1] + e[2]; t += 64u) {
+                const int a = t < e[0] ? 0 : (t < e[0] + e[1] ? 1 : 2);
+                const uint32_t u = t - (a == 0 ? 0u : (a == 1 ? e[0] : e[0] + e[1]));
+                const uint32_t P = lo[a] + u;
+                const uint32_t ci = P / Ck[a], o = ci * Ck[a];
+                const bool typed = P < cov[a] &&
+                                   P - o < chunk_typed_limit(Ck[a], o, bo[a], full[a], a == 0);
+                ax[a][u] = (typed ? kChunkTypedBit : 0u) | (ci - c0[a]);
+            }
+            for (uint32_t t = lane; t < m[0] + m[1] + m[2]; t += 64u) {
+                const int a = t < m[0] ? 0 : (t < m[0] + m[1] ? 1 : 2);
+                const uint32_t j = t - (a == 0 ? 0u : (a == 1 ? m[0] : m[0] + m[1]));
+                const uint32_t o = (c0[a] + j) * Ck[a];
+                sco[a][j] = min(max(sp[a], o), o + Ck[a] - 1u);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            // slot triples: the sample and, for the chunks the node's colour
+            // is taken from (c0..c1), the visit rank
+            const uint32_t mt = m[0] * m[1] * m[2];
+            uint32_t myRank = 0u, myCol = 0u;
+            bool ranked = false;
+            for (uint32_t t = lane; t < mt; t += 64u) {
+                const uint32_t j0 = t % m[0], j1 = (t / m[0]) % m[1], j2 = t / (m[0] * m[1]);
+                const uint32_t c = level_col(lm, col, sco[0][j0], sco[1][j1], sco[2][j2]);
+                samp[t] = c;
+                if (c0[0] + j0 <= c1[0] && c0[1] + j1 <= c1[1] && c0[2] + j2 <= c1[2]) {
+                    const uint32_t r = cg.rank[((uint64_t)(c0[2] + j2) * cg.nc[1] + (c0[1] + j1)) *
+                                                   cg.nc[0] + (c0[0] + j0)];
+                    if (!ranked || r > myRank) {
+                        ranked = true;
+                        myRank = r;
+                        myCol = c;
                     }
-                    s3[a] = min(max(sp[a], o), o + Ck[a] - 1u);
                 }
-                if (!typed) continue;
-                const uint32_t vcol = level_col(lm, col, P3[0], P3[1], P3[2]);
-                const uint32_t scol = level_col(lm, col, s3[0], s3[1], s3[2]);
-                diff = vcol != scol;
             }
-            if (diff) s_diff = 1u;
+            // the last chunk in visit order (ranks are distinct)
+            uint32_t best = ranked ? myRank : 0u, bestCol = myCol;
+            bool haveAny = ranked;
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint32_t r2 = (uint32_t)__shfl_xor((int)best, off);
+                const uint32_t c2 = (uint32_t)__shfl_xor((int)bestCol, off);
+                const bool h2 = __shfl_xor((int)haveAny, off) != 0;
+                if (h2 && (!haveAny || r2 > best)) {
+                    best = r2;
+                    bestCol = c2;
+                    haveAny = true;
+                }
+            }
+            if (lane == 0) cc[n] = haveAny ? bestCol : 0u;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            // voxel q = (w * e1 + v) * e0 + u; q / e0 and q / (e0 e1) by
+            // multiply-high with ceil(2^32 / d), exact for q * d < 2^32
+            // (windows of at most 66^3 voxels)
+            const uint32_t e01 = e[0] * e[1], nv = e01 * e[2];
+            const uint32_t inv0 = (uint32_t)((0x100000000ull + e[0] - 1u) / e[0]);
+            const uint32_t inv01 = (uint32_t)((0x100000000ull + e01 - 1u) / e01);
+            for (uint32_t b = 0; b < nv; b += 64u) {
+                const uint32_t q = b + lane;
+                bool d = false;
+                if (q < nv) {
+                    const uint32_t w = __umulhi(q, inv01), r = q - w * e01;
+                    const uint32_t v = __umulhi(r, inv0), u = r - v * e[0];
+                    const uint32_t ax0 = ax[0][u], ay = ax[1][v], az = ax[2][w];
+                    if (ax0 & ay & az & kChunkTypedBit) {
+                        const uint32_t j0 = ax0 & 0xffffu, j1 = ay & 0xffffu, j2 = az & 0xffffu;
+                        const uint32_t scol = samp[(j2 * m[1] + j1) * m[0] + j0];
+                        d = level_col(lm, col, lo[0] + u, lo[1] + v, lo[2] + w) != scol;
+                    }
+                }
+                if (__ballot(d)) {
+                    diff = true;
+                    break;
+                }
+            }
+            // the tables are rewritten by this wave's next node only after
+            // every lane has left this one
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            // the constant colour: the last chunk in visit order
+            if (lane == 0) {
+                uint32_t best = 0u, bestRank = 0u;
+                bool have = false;
+                for (uint32_t kk = c0[2]; kk <= c1[2]; ++kk)
+                    for (uint32_t j = c0[1]; j <= c1[1]; ++j)
+                        for (uint32_t i = c0[0]; i <= c1[0]; ++i) {
+                            const uint32_t r = cg.rank[((uint64_t)kk * cg.nc[1] + j) * cg.nc[0] + i];
+                            if (!have || r > bestRank) {
+                                have = true;
+                                bestRank = r;
+                                const uint32_t ci[3] = {i, j, kk};
+                                uint32_t q[3];
+                                for (int a = 0; a < 3; ++a)
+                                    q[a] = min(max(sp[a], ci[a] * Ck[a]), ci[a] * Ck[a] + Ck[a] - 1u);
+                                best = level_col(lm, col, q[0], q[1], q[2]);
+                            }
+                        }
+                cc[n] = best;
+            }
+            const uint32_t nv = e[0] * e[1] * e[2];
+            for (uint32_t b = 0; b < nv; b += 64u) {
+                const uint32_t q = b + lane;
+                bool d = false;
+                if (q < nv) {
+                    const uint32_t P3[3] = {lo[0] + q % e[0], lo[1] + (q / e[0]) % e[1],
+                                            lo[2] + q / (e[0] * e[1])};
+                    bool typed = true;
+                    uint32_t s3[3];
+                    for (int a = 0; a < 3; ++a) {
+                        if (P3[a] >= cov[a]) { typed = false; break; }
+                        const uint32_t ci = P3[a] / Ck[a], o = ci * Ck[a];
+                        if (P3[a] - o >= chunk_typed_limit(Ck[a], o, bo[a], full[a], a == 0)) {
+                            typed = false;
+                            break;
+                        }
+                        s3[a] = min(max(sp[a], o), o + Ck[a] - 1u);
+                    }
+                    if (typed)
+                        d = level_col(lm, col, P3[0], P3[1], P3[2]) !=
+                            level_col(lm, col, s3[0], s3[1], s3[2]);
+                }
+                if (__ballot(d)) {
+                    diff = true;
+                    break;
+                }
+            }
         }
-        __syncthreads();
-        if (threadIdx.x == 0) flag[n] = s_diff;
-        __syncthreads();
+        if (lane == 0) flag[n] = diff ? 1u : 0u;
     }
 }
 
@@ -2857,7 +2969,7 @@ void launch_chunk_node_type(hipStream_t st, const LevelMap& lm, const uint32_t* 
                             const ChunkGeom& cg, int k, uint32_t Rd, const uint32_t B[3],
                             const uint32_t border[3], uint32_t* cc, uint32_t* flag) {
     const uint64_t nn = (uint64_t)Rd * Rd * Rd;
-    const unsigned grid = (unsigned)std::min<uint64_t>(nn, 1u << 20);
+    const unsigned grid = (unsigned)std::min<uint64_t>((nn + kChunkWaves - 1) / kChunkWaves, 1u << 20);
     k_chunk_node_type<<<grid, kBlock, 0, st>>>(lm, col, cg, k, Rd, B[0], B[1], B[2], border[0],
                                                border[1], border[2], cc, flag);
 }
