@@ -2286,15 +2286,19 @@ __device__ __forceinline__ uint32_t level_col(const LevelMap& lm, const uint32_t
     const uint32_t v = vox_of(X, Y, Z);
     return stored(lm, (uint64_t)a, v) ? col[(uint64_t)a * kTileVox + v] : 0u;
 }
-__device__ __forceinline__ f3 level_nrm(const LevelMap& lm, const float* __restrict__ nrm,
-                                        uint32_t X, uint32_t Y, uint32_t Z) {
-    if (X >= lm.dims[0] || Y >= lm.dims[1] || Z >= lm.dims[2]) return {0.0f, 0.0f, 0.0f};
+// level_col and level_nrm of one voxel with one tile lookup.
+__device__ __forceinline__ uint32_t level_cn(const LevelMap& lm, const uint32_t* __restrict__ col,
+                                             const float* __restrict__ nrm, uint32_t X, uint32_t Y,
+                                             uint32_t Z, f3& n) {
+    n = {0.0f, 0.0f, 0.0f};
+    if (X >= lm.dims[0] || Y >= lm.dims[1] || Z >= lm.dims[2]) return 0u;
     const int64_t a = tile_of(lm, X, Y, Z);
-    if (a < 0) return {0.0f, 0.0f, 0.0f};
+    if (a < 0) return 0u;
     const uint32_t v = vox_of(X, Y, Z);
-    if (!stored(lm, (uint64_t)a, v)) return {0.0f, 0.0f, 0.0f};
+    if (!stored(lm, (uint64_t)a, v)) return 0u;
     const uint64_t b = (uint64_t)a * 3 * kTileVox + v;
-    return {nrm[b], nrm[b + kTileVox], nrm[b + 2 * kTileVox]};
+    n = {nrm[b], nrm[b + kTileVox], nrm[b + 2 * kTileVox]};
+    return col[(uint64_t)a * kTileVox + v];
 }
 // Source voxels of level voxel X along one axis.
 __device__ __forceinline__ void chunk_src(uint32_t X, uint32_t Cd, uint32_t Cs, uint32_t& s0,
@@ -2367,11 +2371,17 @@ __global__ __launch_bounds__(kBlock) void k_chunk_mip(LevelMap src, LevelMap dst
             // the sum in the order (z, y, x), x fastest
             f4 avg = {0.0f, 0.0f, 0.0f, 0.0f};
             f3 an = {0.0f, 0.0f, 0.0f};
+            // the eight source voxels' values loaded before any is summed
+            uint32_t cs[8];
+            f3 ns[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                cs[i] = level_cn(src, colSrc, nrmSrc, xs[i & 1], ys[(i >> 1) & 1], zs[i >> 2],
+                                 ns[i]);
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const uint32_t sx = xs[i & 1], sy = ys[(i >> 1) & 1], sz = zs[i >> 2];
-                const f4 cur = unpack4(level_col(src, colSrc, sx, sy, sz), s_inv255);
-                const f3 cn = level_nrm(src, nrmSrc, sx, sy, sz);
+                const f4 cur = unpack4(cs[i], s_inv255);
+                const f3 cn = ns[i];
                 if (i == 0) {
                     avg = cur;
                     an = cn;
