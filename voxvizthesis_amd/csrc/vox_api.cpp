@@ -1094,11 +1094,9 @@ int gen_back(vox_ctx* h, const char* out_dir, int binary) {
             launch_fill_segs(st, z, 1);
         });
         h->run("node_type", [&] {
+            launch_chunk_node_type(st, S, cg);
             for (int d = L - 1; d >= 0; --d) {
-                LevelView l = view(h, L - 1 - d);
                 DepthBufs& t = h->tree[d];
-                launch_chunk_node_type(st, l.map, l.col, cg, L - 1 - d, t.Rd, h->B, S.border[d],
-                                       t.cc.as<uint32_t>(), S.flag[d]);
                 launch_chunk_brick_keys(st, S.flag[d], cg, L - 1 - d, t.Rd, h->B, S.border[d],
                                         h->keysA.as<unsigned long long>() + t.off,
                                         h->nodesA.as<uint32_t>() + t.off, bcount + d);

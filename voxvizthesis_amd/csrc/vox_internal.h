@@ -349,14 +349,12 @@ void launch_chunk_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst,
                       const ChunkGeom& cg, int ks, int kd, uint64_t nActive,
                       const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
                       uint32_t* colDst, uint32_t* qnDst, float* nrmDst);
-// Node types and constant colours of one depth: flag[n] = 1 when a chunk
-// the node's stored brick reaches holds a typed voxel of the brick unequal
-// to that chunk's sample; cc[n] = the sample of the last chunk in visit
-// order.
-// (k: the level the depth reads.)
-void launch_chunk_node_type(hipStream_t st, const LevelMap& lm, const uint32_t* col,
-                            const ChunkGeom& cg, int k, uint32_t Rd, const uint32_t B[3],
-                            const uint32_t border[3], uint32_t* cc, uint32_t* flag);
+// Node types and constant colours of every depth d of S (level L - 1 - d,
+// S.lm / S.col; S.cc, S.flag; nodes numbered from S.start[d]): flag[n] = 1
+// when a chunk the node's stored brick reaches holds a typed voxel of the
+// brick unequal to that chunk's sample; cc[n] = the sample of the last chunk
+// in visit order.
+void launch_chunk_node_type(hipStream_t st, const SampleDepths& S, const ChunkGeom& cg);
 // Brick keys of one depth: for each NON-CONST node whose stored brick lies
 // in processed chunks, key = (rank of the chunk that completes it << (1 +
 // 3d)) | (0 inside one chunk, 1 across chunks) << 3d | node; keys and nodes

@@ -2451,25 +2451,37 @@ __device__ __forceinline__ uint32_t chunk_typed_limit(uint32_t Ck, uint32_t o, u
 // them (kChunkTypedBit); the chunks' samples are read once per node (slot
 // triple -> colour) and the wave stops at the first voxel that differs.
 // Windows wider than kChunkAxis or with more than kChunkSamples slot
-// triples read every voxel's sample directly.
+// triples read every voxel's sample directly.  The nodes of every depth in
+// one launch, the coarse depths (few nodes, each a long walk) first, so
+// their waves run beside the many waves of the finer depths.
 constexpr uint32_t kChunkAxis = 66, kChunkSamples = 64, kChunkTypedBit = 1u << 31;
 constexpr int kChunkWaves = kBlock / 64;
-__global__ __launch_bounds__(kBlock) void k_chunk_node_type(
-    LevelMap lm, const uint32_t* __restrict__ col, ChunkGeom cg, int k, uint32_t Rd, uint32_t B0,
-    uint32_t B1, uint32_t B2, uint32_t bo0, uint32_t bo1, uint32_t bo2, uint32_t* __restrict__ cc,
-    uint32_t* __restrict__ flag) {
+__global__ __launch_bounds__(kBlock) void k_chunk_node_type(SampleDepths S, ChunkGeom cg) {
     __shared__ uint32_t s_ax[kChunkWaves][3][kChunkAxis];
     __shared__ uint32_t s_sco[kChunkWaves][3][kChunkSamples];
     __shared__ uint32_t s_samp[kChunkWaves][kChunkSamples];
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const uint32_t Ck[3] = {cg.C[0] >> k, cg.C[1] >> k, cg.C[2] >> k}, Bv[3] = {B0, B1, B2},
-                   bo[3] = {bo0, bo1, bo2};
+    const uint32_t lane = threadIdx.x & 63u;
+    // the wave's index, and below its node and depth, in scalar registers
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t Bv[3] = {S.B[0], S.B[1], S.B[2]};
     const uint32_t e[3] = {Bv[0] + 2u, Bv[1] + 2u, Bv[2] + 2u};
-    const uint32_t full[3] = {Rd * Bv[0], Rd * Bv[1], Rd * Bv[2]};
-    const uint64_t nn = (uint64_t)Rd * Rd * Rd;
-    const uint32_t ld = (uint32_t)(__ffs(Rd) - 1);
-    for (uint64_t n = (uint64_t)blockIdx.x * kChunkWaves + wv; n < nn;
-         n += (uint64_t)gridDim.x * kChunkWaves) {
+    const uint64_t total = S.start[S.L];
+    for (uint64_t g = (uint64_t)blockIdx.x * kChunkWaves + wv; g < total;
+         g += (uint64_t)gridDim.x * kChunkWaves) {
+        int dd = 0;   // the node's depth (uniform over the wave)
+        while (g >= S.start[dd + 1]) ++dd;
+        dd = __builtin_amdgcn_readfirstlane(dd);
+        const uint64_t n = g - S.start[dd];
+        const int k = S.L - 1 - dd;
+        const uint32_t Rd = S.Rd[dd];
+        const LevelMap& lm = S.lm[dd];
+        const uint32_t* __restrict__ col = S.col[dd];
+        uint32_t* __restrict__ cc = S.cc[dd];
+        uint32_t* __restrict__ flag = S.flag[dd];
+        const uint32_t Ck[3] = {cg.C[0] >> k, cg.C[1] >> k, cg.C[2] >> k},
+                       bo[3] = {S.border[dd][0], S.border[dd][1], S.border[dd][2]};
+        const uint32_t full[3] = {Rd * Bv[0], Rd * Bv[1], Rd * Bv[2]};
+        const uint32_t ld = (uint32_t)(__ffs(Rd) - 1);
         uint32_t nd[3];
         node_xyz(n, ld, nd[0], nd[1], nd[2]);
         uint32_t lo[3], hi[3], c0[3], c1[3], sp[3], cov[3], m[3];
@@ -2975,13 +2987,11 @@ void launch_chunk_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst,
     k_chunk_mip<<<(unsigned)nActive, kBlock, 0, st>>>(src, dst, cg, ks, kd, listDst, colSrc, nrmSrc,
                                                       colDst, qnDst, nrmDst);
 }
-void launch_chunk_node_type(hipStream_t st, const LevelMap& lm, const uint32_t* col,
-                            const ChunkGeom& cg, int k, uint32_t Rd, const uint32_t B[3],
-                            const uint32_t border[3], uint32_t* cc, uint32_t* flag) {
-    const uint64_t nn = (uint64_t)Rd * Rd * Rd;
+void launch_chunk_node_type(hipStream_t st, const SampleDepths& S, const ChunkGeom& cg) {
+    const uint64_t nn = S.start[S.L];
+    if (nn == 0) return;
     const unsigned grid = (unsigned)std::min<uint64_t>((nn + kChunkWaves - 1) / kChunkWaves, 1u << 20);
-    k_chunk_node_type<<<grid, kBlock, 0, st>>>(lm, col, cg, k, Rd, B[0], B[1], B[2], border[0],
-                                               border[1], border[2], cc, flag);
+    k_chunk_node_type<<<grid, kBlock, 0, st>>>(S, cg);
 }
 void launch_chunk_brick_keys(hipStream_t st, const uint32_t* flag, const ChunkGeom& cg, int k,
                              uint32_t Rd, const uint32_t B[3], const uint32_t border[3],
