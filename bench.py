@@ -111,6 +111,50 @@ def kernel_bytes(name, scene, st, bricks, B, launches):
     return 0.0
 
 
+def chunked_bench(scene, chunk: int, steps: int, warmup: int):
+    """The headline workload with voxChunkDim = chunk^3, the reference's
+    default (Voxelizer.h 124; its caller keeps it, main.cu 4169-4170): per
+    chunk mips and node samples, types OR-ed over chunks, bricks across
+    chunks assembled, bricks numbered in the writer's order.  Same timing as
+    the headline (setup + generate per step, inputs resident)."""
+    from voxvizthesis_amd.voxelizer import Voxelizer
+    vz = Voxelizer(scene.N, scene.B, (chunk,) * 3)
+    try:
+        ok = vz.initCuda() and vz.allocateVoxelMemory() and \
+            vz.setVoxelizationParams(scene.p, scene.delta)
+        for m in scene.meshes:
+            ok = ok and vz.addMesh(m)
+        ok = ok and vz.allocateTriangleMemory()
+        if not ok:
+            raise RuntimeError(vz.getErrorMessage())
+
+        def step():
+            if not vz.computeEdgesFaceNormalsAndBounds() or \
+                    vz.generateVoxelsAndOctTree(None) != 1:
+                raise RuntimeError(vz.getErrorMessage())
+
+        for _ in range(warmup):
+            step()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        res = vz.result(with_bricks=False)
+        bricks = [t["nb"] for t in res["tree"]]
+        n = scene.N[0] * scene.N[1] * scene.N[2]
+        return {
+            "workload": f"the headline workload in {chunk}^3 chunks "
+                        f"({(scene.N[0] // chunk) ** 3} chunks)",
+            "chunk": chunk,
+            "ms_per_step": round(ms, 4),
+            "gvoxels_per_s": round(n / (ms * 1e-3) / 1e9, 2),
+            "bricks": bricks,
+            "bricks_per_s": round(sum(bricks) / (ms * 1e-3), 1),
+        }
+    finally:
+        vz.close()
+
+
 def serving_bench(vz, scene, frames: int, W: int, H: int, slots, max_up: int, streams: int = 1):
     """Paged flythrough over the voxelized octree: per frame each camera
     stream's W x H sparse-octree ray cast emits brick requests and one brick
@@ -588,6 +632,9 @@ def main():
     ap.add_argument("--batch-grids", default="64,128,256,512,1024")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--chunk", type=int, default=256,
+                    help="also time the workload in chunks of this edge (the reference's "
+                         "default voxChunkDim); 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the voxelizer "
@@ -738,6 +785,10 @@ def main():
         "rooflines": [roofline_of(k) for k in ("brick_gather", "mip", "setup", "scan", "scatter")
                       if k in kern],
     }
+    if args.chunk > 0:
+        log(f"[bench] the same workload in {args.chunk}^3 chunks ...")
+        out["chunked"] = chunked_bench(scene, args.chunk, args.steps, args.warmup)
+        log(f"[bench] chunked: {out['chunked']['ms_per_step']} ms/step")
     if not args.no_serve:
         log(f"[bench] serving {args.serve_frames} frames at {args.serve_width}x"
             f"{args.serve_height} ...")
