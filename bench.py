@@ -111,6 +111,34 @@ def kernel_bytes(name, scene, st, bricks, B, launches):
     return 0.0
 
 
+def copy_reference(nbytes: int, reps: int):
+    """GB/s of a plain device copy of nbytes (read + write counted), timed
+    on the host around each synchronised copy; the median of reps."""
+    import numpy as np
+    from voxvizthesis_amd import _lib
+    L = _lib.load()
+    src, dst = L.vox_device_alloc(nbytes), L.vox_device_alloc(nbytes)
+    try:
+        if not (src and dst):
+            raise RuntimeError("copy reference: device allocation failed")
+        ts = []
+        for i in range(reps + 1):
+            t0 = time.perf_counter()
+            if not L.vox_device_copy(dst, src, nbytes):
+                raise RuntimeError("copy reference failed")
+            if i:
+                ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        gbs = 2 * nbytes / t / 1e9
+        return {"bytes_per_copy": 2 * nbytes, "ms": round(t * 1e3, 4), "gbs": round(gbs, 1),
+                "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    finally:
+        if src:
+            L.vox_device_free(src)
+        if dst:
+            L.vox_device_free(dst)
+
+
 def chunked_bench(scene, chunk: int, steps: int, warmup: int):
     """The headline workload with voxChunkDim = chunk^3, the reference's
     default (Voxelizer.h 124; its caller keeps it, main.cu 4169-4170): per
@@ -785,6 +813,10 @@ def main():
         "rooflines": [roofline_of(k) for k in ("brick_gather", "mip", "setup", "scan", "scatter")
                       if k in kern],
     }
+    # an attainable-bandwidth reference beside the 8 TB/s peak: a plain
+    # 16-byte-per-lane device copy of 4 GiB (read + write), median of 10
+    out["hbm_copy"] = copy_reference(4 << 30, 10)
+    log(f"[bench] plain copy: {out['hbm_copy']['gbs']} GB/s")
     if args.chunk > 0:
         log(f"[bench] the same workload in {args.chunk}^3 chunks ...")
         out["chunked"] = chunked_bench(scene, args.chunk, args.steps, args.warmup)

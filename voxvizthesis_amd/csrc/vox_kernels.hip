@@ -2222,9 +2222,17 @@ __global__ void k_tri_bounds(const float* __restrict__ v, uint64_t n, int32_t* _
 
 // Plain copy of n 16-byte words (the reference point of traffic counters).
 __global__ void k_copy_u128(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    // four coalesced 16-byte loads in flight per lane, then their stores
+    const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * T < n; i += 4 * T) {
+        const uint4 a = src[i], b = src[i + T], c = src[i + 2 * T], d = src[i + 3 * T];
+        dst[i] = a;
+        dst[i + T] = b;
+        dst[i + 2 * T] = c;
+        dst[i + 3 * T] = d;
+    }
+    for (; i < n; i += T) dst[i] = src[i];
 }
 
 // The same copy over 256-byte blocks in a scattered order: 16 lanes per
@@ -2746,8 +2754,11 @@ void launch_tri_bounds(hipStream_t st, const float* v, uint64_t n, int32_t* out)
 
 void launch_copy_u128(hipStream_t st, const void* src, void* dst, uint64_t n16) {
     if (n16 == 0) return;
-    k_copy_u128<<<4096, kBlock, 0, st>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst),
-                                         n16);
+    // about one pass of four 16-byte words per lane (a grid-stride walk of
+    // few steps moved the most bytes per second)
+    const uint64_t blocks = std::min<uint64_t>((n16 + 4ull * kBlock - 1) / (4ull * kBlock), 262144);
+    k_copy_u128<<<(unsigned)blocks, kBlock, 0, st>>>(static_cast<const uint4*>(src),
+                                                   static_cast<uint4*>(dst), n16);
 }
 
 void launch_copy_scattered(hipStream_t st, const void* src, void* dst, uint64_t nblocks) {
