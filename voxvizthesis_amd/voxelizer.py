@@ -16,6 +16,12 @@ ADDR_WRAP, ADDR_CLAMP, ADDR_MIRROR, ADDR_BORDER = 0, 1, 2, 3
 
 
 class Voxelizer:
+    """voxChunkDim: the chunk edge per axis (Voxelizer.h 124).  None here
+    means one chunk that holds the whole voxel space; the reference's own
+    default is (256, 256, 256), which its caller keeps (main.cu 4169-4170):
+    pass it to get the reference's chunked results for N above 256 (the C++
+    mirror, include/voxmi_voxelizer.hpp, defaults to it)."""
+
     def __init__(self, voxDim, brickDim, voxChunkDim=None, options: int = 0):
         self._L = _lib.load()
         n = (C.c_uint32 * 3)(*[int(x) for x in voxDim])
