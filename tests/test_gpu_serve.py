@@ -173,3 +173,33 @@ def test_two_streams_same_alone_and_mixed():
     assert len(mixed) > len(alone)
     for t in common:
         assert alone[t] == mixed[t]
+
+
+def test_queued_frames_equal_waited_frames():
+    """Forty frames queued without a host wait (more server frames queued
+    than any fixed set of timing events would hold) end in the same node
+    texture, pool and counts as the same frames each waited for, and the
+    server's update times of every frame are counted."""
+    W, H = 48, 32
+    cams = path(40, W, H, (0.5, -0.8, 1.4), (0.9, 0.2, 0.8), (0.5, 0.55, 0.5))
+    sc = scenes.scene_small_terrain()
+
+    def run(wait):
+        print(f"[serve] 40 frames, wait={wait}", flush=True)
+        rig = Rig(sc, (4, 4, 4), 16, W, H)     # a small pool: evictions too
+        try:
+            for c in cams:
+                rig.rc[0].frame(camera(**c), raycast_params(**rig.prm), wait=wait)
+                rig.srv.frame(rig.rc[0], wait=wait)
+            st = rig.srv.stats()
+            col, _ = rig.bp.textures()
+            return (rig.t.nodes().copy(), col.copy(), int(st.uploads), int(st.evictions),
+                    int(st.resident), int(st.frames), float(st.ms_update_total))
+        finally:
+            rig.close()
+
+    a, b = run(False), run(True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert a[2:6] == b[2:6] and a[5] == 40 and a[2] > 0
+    assert a[3] > 0 or a[2] <= 64
+    assert a[6] > 0.0 and b[6] > 0.0
