@@ -117,11 +117,11 @@ void write_text_brick(std::ofstream& f, const uint32_t* col, const float* nrm, c
 // public C ABI only.
 extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3],
                                     const float delta[3], const uint32_t N[3], int binary,
-                                    const uint32_t extra[3]);
+                                    const uint32_t extra[3], const uint32_t chunk[3], int multi);
 
 extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3],
                                     const float delta[3], const uint32_t N[3], int binary,
-                                    const uint32_t extra[3]) {
+                                    const uint32_t extra[3], const uint32_t chunk[3], int multi) {
     vox_result r;
     if (!vox_get_result(h, &r)) return 0;
     vox_compressed cz;
@@ -177,11 +177,28 @@ extern "C" int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float
                     // a brick reaching the chunk's far side was stored as a
                     // partial brick and is written from its own buffer, whose
                     // window starts at 0 (VoxelBrickWriter.cu 1179-1190);
-                    // the others from the level at their offset
+                    // the others from their chunk's level at their offset in
+                    // it (Voxelizer.cu 1128-1130, 1182-1253: the node's place
+                    // less the chunk's origin at this level).  With several
+                    // chunks a brick is partial exactly when its window
+                    // [x0, x0 + B + 1] leaves one chunk of C >> k voxels; the
+                    // window's voxels are the level's at those places either
+                    // way (the chunks' levels laid side by side).
                     bool partial = false;
-                    for (int a = 0; a < 3; ++a) partial = partial || m[a] + t.B[a] >= S[a];
-                    const uint64_t w0[3] = {partial ? 0u : x0[0], partial ? 0u : x0[1],
-                                            partial ? 0u : x0[2]};
+                    uint32_t origin[3] = {0u, 0u, 0u};
+                    for (int a = 0; a < 3; ++a) {
+                        if (multi) {
+                            const uint32_t Ck = chunk[a] >> k;
+                            const uint32_t c0 = x0[a] / Ck, c1 = (x0[a] + t.B[a] + 1u) / Ck;
+                            partial = partial || c0 != c1;
+                            origin[a] = c0 * Ck;
+                        } else {
+                            partial = partial || m[a] + t.B[a] >= S[a];
+                        }
+                    }
+                    const uint64_t w0[3] = {partial ? 0u : x0[0] - origin[0],
+                                            partial ? 0u : x0[1] - origin[1],
+                                            partial ? 0u : x0[2] - origin[2]};
                     write_text_brick(f, col.data(), nrm.data(), S, x0, w0, e);
                 }
             }

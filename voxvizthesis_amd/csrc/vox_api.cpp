@@ -808,7 +808,8 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
 
 // the file writers (gvformat.cpp)
 int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3], const float delta[3],
-                         const uint32_t N[3], int binary, const uint32_t extra[3]);
+                         const uint32_t N[3], int binary, const uint32_t extra[3],
+                         const uint32_t chunk[3], int multi);
 
 namespace {
 
@@ -1276,9 +1277,8 @@ int gen_back(vox_ctx* h, const char* out_dir, int binary) {
     h->haveResult = true;
     h->prepared = false;  // bins are consumed; a new generate re-runs setup
     if (out_dir) {
-        if (!binary && h->multi)
-            return h->fail("text (.gvt) bricks are written for one chunk only"), 0;
-        if (!vox_write_files_impl(h, out_dir, h->p, h->dP, h->N, binary ? 1 : 0, h->extra))
+        if (!vox_write_files_impl(h, out_dir, h->p, h->dP, h->N, binary ? 1 : 0, h->extra, h->C,
+                                  h->multi ? 1 : 0))
             return h->fail(std::string("failed to write GigaVoxels files to ") + out_dir), 0;
     }
     return firstOverlap >= 0 ? 1 : -1;
@@ -1429,7 +1429,8 @@ int vox_copy_level(vox_ctx* h, int32_t level, uint32_t* colors, float* normals, 
 }
 
 int vox_write_files_impl(vox_ctx* h, const char* out_dir, const float p[3], const float delta[3],
-                         const uint32_t N[3], int binary, const uint32_t extra[3]);
+                         const uint32_t N[3], int binary, const uint32_t extra[3],
+                         const uint32_t chunk[3], int multi);
 
 int vox_get_compressed(vox_ctx* h, vox_compressed* out) {
     if (!h || !h->valid || !out || !idle(h)) return 0;
@@ -1448,7 +1449,7 @@ int vox_get_compressed(vox_ctx* h, vox_compressed* out) {
 
 int vox_write_files(vox_ctx* h, const char* out_dir) {
     if (!h || !h->valid || !out_dir || !idle(h)) return 0;
-    if (!vox_write_files_impl(h, out_dir, h->p, h->dP, h->N, 1, h->extra))
+    if (!vox_write_files_impl(h, out_dir, h->p, h->dP, h->N, 1, h->extra, h->C, h->multi ? 1 : 0))
         return h->fail(std::string("failed to write GigaVoxels files to ") + out_dir), 0;
     return 1;
 }
