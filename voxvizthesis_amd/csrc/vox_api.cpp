@@ -775,9 +775,8 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
         launch_fill_segs(h->st, z, 2);
     });
     h->run("setup", [&] {
-        for (int g = 0; g < G; ++g)
-            launch_setup(h->st, P, h->gargs[g], g, h->verts.as<float>(), h->fnorm.as<float>(),
-                         h->bbox.as<uint32_t>(), cnt);
+        launch_setup(h->st, P, h->gargs, h->verts.as<float>(), h->fnorm.as<float>(),
+                     h->bbox.as<uint32_t>(), cnt);
     });
     // the scan leaves the counts zeroed for the scatter pass to count again
     if (!ensure_scan_tmp(h->scanTmp, h->numBins, h->st, h->err, "scan scratch")) return 0;
@@ -798,9 +797,8 @@ int vox_compute_edges_face_normals_and_bounds(vox_ctx* h) {
     h->pairs = total;
     if (!h->binTri.ensure((uint64_t)total * 4, h->err, "bin entries")) return 0;
     h->run("scatter", [&] {
-        for (int g = 0; g < G; ++g)
-            launch_scatter(h->st, P, h->gargs[g], g, h->bbox.as<uint32_t>(), start, cnt,
-                           h->binTri.as<uint32_t>());
+        launch_scatter(h->st, P, h->gargs, h->bbox.as<uint32_t>(), start, cnt,
+                       h->binTri.as<uint32_t>());
     });
     h->prepared = true;
     return 1;

@@ -198,9 +198,20 @@ bool pool_refs(vox_brick_pool* bp, PoolRefs& r);
 bool octree_record_written(vox_octree* t, hipStream_t st);
 
 // ---- launchers (all asynchronous on `st`) ----
-void launch_setup(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
+// Up to kGroupSet groups in one launch: group i (index g[i] of the handle)
+// takes the workgroups start[i] .. start[i + 1] - 1.
+constexpr int kGroupSet = 8;
+struct GroupSet {
+    GroupArgs G[kGroupSet];
+    uint32_t start[kGroupSet + 1];
+    int32_t g[kGroupSet];
+    int n;
+};
+// Setup (bounds, face normal, bin counts) and scatter of every group, a
+// launch per kGroupSet groups.
+void launch_setup(hipStream_t st, const VoxParams& P, const std::vector<GroupArgs>& groups,
                   const float* verts, float* fnorm, uint32_t* bbox, uint32_t* binCount);
-void launch_scatter(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
+void launch_scatter(hipStream_t st, const VoxParams& P, const std::vector<GroupArgs>& groups,
                     const uint32_t* bbox, const uint32_t* binStart, uint32_t* binFill,
                     uint32_t* binTri);
 void launch_activity0(hipStream_t st, const VoxParams& P, int ngroups, const int32_t* grpFill,

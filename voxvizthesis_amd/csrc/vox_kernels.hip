@@ -294,13 +294,23 @@ __device__ __forceinline__ uint32_t list_pairs(PairList& L, bool has, const uint
 // setup: ComputeEdgesFaceNormalsAndBounds (VoxelizerKernels.cu 230-347) plus
 // the count of 8^3 tiles each triangle's bounds touch, per (group, tile) bin.
 // Bounds are packed as min | max << 16 per axis.
-__global__ __launch_bounds__(kBlock) void k_setup(VoxParams P, GroupArgs G, int g,
+// The workgroup's group of a launch over several groups (scalar registers).
+__device__ __forceinline__ int set_group(const GroupSet& S) {
+    int gi = 0;
+    while (gi + 1 < S.n && blockIdx.x >= S.start[gi + 1]) ++gi;
+    return __builtin_amdgcn_readfirstlane(gi);
+}
+
+__global__ __launch_bounds__(kBlock) void k_setup(VoxParams P, GroupSet S,
                                                   const float* __restrict__ verts,
                                                   float* __restrict__ fnorm,
                                                   uint32_t* __restrict__ bbox,
                                                   uint32_t* __restrict__ binCount) {
     __shared__ PairList L;
-    const uint64_t lt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int gi = set_group(S);
+    const GroupArgs& G = S.G[gi];
+    const int g = S.g[gi];
+    const uint64_t lt = (uint64_t)(blockIdx.x - S.start[gi]) * blockDim.x + threadIdx.x;
     const uint64_t t = G.triOff + lt;
     uint32_t t0[3] = {0, 0, 0}, t1[3] = {0, 0, 0};
     bool has = false;
@@ -335,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(VoxParams P, GroupArgs G, int 
         }
         has = G.overlaps && tile_box(G, pk[0], pk[1], pk[2], t0, t1);
     }
-    if (!G.overlaps) return;   // uniform: the whole grid
+    if (!G.overlaps) return;   // uniform: the whole workgroup
     bool big;
     const uint32_t total = list_pairs(L, has, t0, t1, (uint32_t)t, big);
     // Neighbouring triangles mostly share tiles: the lanes holding the same
@@ -360,14 +370,17 @@ __global__ __launch_bounds__(kBlock) void k_setup(VoxParams P, GroupArgs G, int 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(VoxParams P, GroupArgs G, int g,
+__global__ __launch_bounds__(kBlock) void k_scatter(VoxParams P, GroupSet S,
                                                     const uint32_t* __restrict__ bbox,
                                                     const uint32_t* __restrict__ binStart,
                                                     uint32_t* __restrict__ binFill,
                                                     uint32_t* __restrict__ binTri) {
-    if (!G.overlaps) return;   // uniform: the whole grid
+    const int gi = set_group(S);
+    const GroupArgs& G = S.G[gi];
+    const int g = S.g[gi];
+    if (!G.overlaps) return;   // uniform: the whole workgroup
     __shared__ PairList L;
-    const uint64_t lt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t lt = (uint64_t)(blockIdx.x - S.start[gi]) * blockDim.x + threadIdx.x;
     const uint64_t t = G.triOff + lt;
     uint32_t t0[3] = {0, 0, 0}, t1[3] = {0, 0, 0};
     bool has = false;
@@ -2790,16 +2803,47 @@ void launch_fill_segs(hipStream_t st, const FillSeg* segs, int nseg) {
 
 // ---------------------------------------------------------------------------
 // launchers
-void launch_setup(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
-                  const float* verts, float* fnorm, uint32_t* bbox, uint32_t* binCount) {
-    if (G.ntri == 0) return;
-    k_setup<<<grid_for(G.ntri, kBlock), kBlock, 0, st>>>(P, G, g, verts, fnorm, bbox, binCount);
+namespace {
+// Launches over the groups g0 .. g1 - 1 that `use` admits, kGroupSet at a
+// time, each group's triangles in its own range of workgroups.
+template <class Use, class Launch>
+void launch_group_sets(const std::vector<GroupArgs>& groups, Use use, Launch launch) {
+    GroupSet S{};
+    uint32_t blocks = 0;
+    auto flush = [&] {
+        if (S.n == 0) return;
+        S.start[S.n] = blocks;
+        launch(S, blocks);
+        S = GroupSet{};
+        blocks = 0;
+    };
+    for (int g = 0; g < (int)groups.size(); ++g) {
+        if (!use(groups[g])) continue;
+        S.G[S.n] = groups[g];
+        S.g[S.n] = g;
+        S.start[S.n] = blocks;
+        blocks += grid_for(groups[g].ntri, kBlock);
+        if (++S.n == kGroupSet) flush();
+    }
+    flush();
 }
-void launch_scatter(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g,
+}  // namespace
+
+void launch_setup(hipStream_t st, const VoxParams& P, const std::vector<GroupArgs>& groups,
+                  const float* verts, float* fnorm, uint32_t* bbox, uint32_t* binCount) {
+    launch_group_sets(groups, [](const GroupArgs& G) { return G.ntri > 0; },
+                      [&](const GroupSet& S, uint32_t blocks) {
+                          k_setup<<<blocks, kBlock, 0, st>>>(P, S, verts, fnorm, bbox, binCount);
+                      });
+}
+void launch_scatter(hipStream_t st, const VoxParams& P, const std::vector<GroupArgs>& groups,
                     const uint32_t* bbox, const uint32_t* binStart, uint32_t* binFill,
                     uint32_t* binTri) {
-    if (G.ntri == 0 || !G.overlaps) return;
-    k_scatter<<<grid_for(G.ntri, kBlock), kBlock, 0, st>>>(P, G, g, bbox, binStart, binFill, binTri);
+    launch_group_sets(groups, [](const GroupArgs& G) { return G.ntri > 0 && G.overlaps; },
+                      [&](const GroupSet& S, uint32_t blocks) {
+                          k_scatter<<<blocks, kBlock, 0, st>>>(P, S, bbox, binStart, binFill,
+                                                               binTri);
+                      });
 }
 void launch_activity0(hipStream_t st, const VoxParams& P, int ngroups, const int32_t* grpFill,
                       const uint32_t* binStart, uint32_t* mark) {
