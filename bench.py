@@ -814,9 +814,12 @@ def main():
                       if k in kern],
     }
     # an attainable-bandwidth reference beside the 8 TB/s peak: a plain
-    # 16-byte-per-lane device copy of 4 GiB (read + write), median of 10
-    out["hbm_copy"] = copy_reference(4 << 30, 10)
-    log(f"[bench] plain copy: {out['hbm_copy']['gbs']} GB/s")
+    # 16-byte-per-lane device copy of 4 GiB (read + write), median of 10;
+    # left out of counter runs, whose one calibration copy must be the only
+    # launch of the copy kernel
+    if not args.copy_calibration:
+        out["hbm_copy"] = copy_reference(4 << 30, 10)
+        log(f"[bench] plain copy: {out['hbm_copy']['gbs']} GB/s")
     if args.chunk > 0:
         log(f"[bench] the same workload in {args.chunk}^3 chunks ...")
         out["chunked"] = chunked_bench(scene, args.chunk, args.steps, args.warmup)
