@@ -52,6 +52,11 @@ def test_mixed_small_chunks_mip_chain():
     _check(scenes.scene_mixed(seed=3, N=32, B=4), 8, chain=True)
 
 
+def test_more_groups_than_one_binning_launch_holds():
+    # two launches each of setup and scatter in every chunk
+    _check(scenes.scene_many_groups(seed=300), 32)
+
+
 @pytest.mark.parametrize("chunk", [32, 48])
 def test_mixed_chunks_uncovered_and_wide(chunk):
     # 32: the chunks reach 64 of the 80 voxels per axis; 48: they reach 96

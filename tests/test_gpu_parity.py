@@ -57,6 +57,13 @@ def test_mixed_groups_two_terrains():
     _check(scenes.scene_mixed(seed=6, order="two_terrains"))
 
 
+def test_more_groups_than_one_binning_launch_holds():
+    # setup and scatter take at most 8 groups per launch: 11 groups with
+    # triangles (10 inside the grid) need two launches of each, and the
+    # terrain's fill runs in the second
+    _check(scenes.scene_many_groups())
+
+
 def test_big_and_small_triangles_binned():
     # boxes of 18^3 tiles walk their bins; the small ones are listed as pairs
     _check(scenes.scene_big_triangles())

@@ -366,6 +366,21 @@ def scene_mixed(seed: int = 7, N: int = 64, B: int = 8, order: str = "terrain_fi
     return Scene(meshes=meshes, N=(N, N, N), B=(B, B, B), name=f"mixed_N{N}_B{B}{suffix}")
 
 
+def scene_many_groups(seed: int = 100, N: int = 64, B: int = 8) -> Scene:
+    """Twelve groups in one grid: nine random soups (every third textured
+    with random UVs, every second with vertex normals), a soup wholly
+    outside the grid (no bins), an empty group and a textured terrain as the
+    eleventh group, whose fill lands on what ten groups before it hold."""
+    meshes = [random_soup(120 + 40 * i, seed + i, N, textured=(i % 3 == 0), uv_mode="random",
+                          vertex_normals=(i % 2 == 1)) for i in range(9)]
+    far = random_soup(50, seed + 100, N)
+    far.verts = (far.verts + np.float32(3.0)).astype(np.float32)
+    meshes.insert(3, far)
+    meshes.insert(6, Mesh(verts=np.zeros((0, 3, 3), np.float32), name="empty"))
+    meshes.insert(10, heightfield_terrain(N // 2, seed + 17))
+    return Scene(meshes=meshes, N=(N, N, N), B=(B, B, B), name=f"many_groups_N{N}_B{B}")
+
+
 def scene_big_triangles(seed: int = 13, N: int = 128, B: int = 8) -> Scene:
     """One untextured group mixing small random triangles with two that span
     the whole grid (18^3 tiles at N=128, B=8: more (triangle, tile) bins than
