@@ -79,6 +79,12 @@ def test_terrain_256_b8(chunk):
     _check(scenes.scene_terrain_n(128, 256, 8), chunk)
 
 
+def test_terrain_256_b8_mip_chain():
+    # levels 1-3 filtered per chunk through the single-chunk filter with
+    # chunk offsets of zero (each level reads the one below)
+    _check(scenes.scene_terrain_n(128, 256, 8), 64, chain=True)
+
+
 @pytest.mark.parametrize("chunk", [256, 320])
 def test_full_octree_generator_512_chunks(chunk):
     ref = _check(scenes.scene_full_octree(cells=512, N=512, B=16), chunk)

@@ -1031,7 +1031,13 @@ int gen_back(vox_ctx* h, const char* out_dir, int binary) {
         if (h->lv[k].aliasOf >= 0) continue;
         LevelView s = view(h, srcOf(k));
         LevelView l = view(h, k);
-        if (l.active > 0 && h->multi) h->run("mip", [&] {
+        MipShift shift;
+        if (l.active > 0 && h->multi && mip_shift_for(s.map, l.map, chunkGeom(h), srcOf(k), k, shift))
+            h->run("mip", [&] {
+                launch_mip_shifted(st, s.map, l.map, shift, l.active, l.list, s.col, s.nrm, l.col,
+                                   l.qn, l.nrm);
+            });
+        else if (l.active > 0 && h->multi) h->run("mip", [&] {
             launch_chunk_mip(st, s.map, l.map, chunkGeom(h), srcOf(k), k, l.active, l.list, s.col,
                              s.nrm, l.col, l.qn, l.nrm);
         });

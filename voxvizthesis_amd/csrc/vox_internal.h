@@ -356,6 +356,20 @@ struct ChunkGeom {
 // (ks: the source level, kd: the level built from it.)
 void launch_chunk_mark(hipStream_t st, const LevelMap& src, const LevelMap& dst,
                        const ChunkGeom& cg, int ks, int kd, uint32_t* markDst);
+// The chunked mip filter as the single-chunk one with a per-chunk source
+// offset (MipShift), where the chunk geometry allows it: every axis's chunk
+// extent at the level a multiple of the tile edge, the per-chunk offset a
+// multiple of two tiles and every source inside the source level.
+struct MipShift {
+    uint32_t cd[3];    // chunk extent at the level (0: one chunk, no shift)
+    uint32_t off[3];   // source offset per chunk: Cs - 2 Cd
+};
+bool mip_shift_for(const LevelMap& src, const LevelMap& dst, const ChunkGeom& cg, int ks, int kd,
+                   MipShift& out);
+void launch_mip_shifted(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                        const MipShift& shift, uint64_t nActive, const int32_t* listDst,
+                        const uint32_t* colSrc, const float* nrmSrc, uint32_t* colDst,
+                        uint32_t* qnDst, float* nrmDst);
 void launch_chunk_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst,
                       const ChunkGeom& cg, int ks, int kd, uint64_t nActive,
                       const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,

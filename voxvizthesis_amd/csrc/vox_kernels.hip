@@ -1372,7 +1372,14 @@ __global__ __launch_bounds__(kBlock) void k_fill(VoxParams P, int g, int init, u
 
 // ---------------------------------------------------------------------------
 // mip: ComputeVoxelMipMap8 (VoxelizerKernels.cu 1520-1663).
-__global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
+// With several chunks, voxel X of chunk i (X = i * Cd + x) reads source
+// voxels i * Cs + 2x and i * Cs + 2x + 1 (Cd, Cs: the chunk's extent at the
+// two levels), i.e. 2X + i * (Cs - 2 Cd).  Where Cd is a multiple of the
+// tile edge, i * (Cs - 2 Cd) a multiple of two tiles and every source below
+// the source level's end, a tile's sources are the 2x2x2 tiles at 2t
+// shifted by that offset, with the same parities, and the filter is this
+// one; shift.cd = 0 on an axis means one chunk (no shift).
+__global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst, MipShift shift,
                                                 const int32_t* __restrict__ listDst,
                                                 const uint32_t* __restrict__ colSrc,
                                                 const float* __restrict__ nrmSrc,
@@ -1384,6 +1391,10 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
     const uint64_t tlin = (uint64_t)listDst[b];
     uint32_t tx, ty, tz;
     tile_xyz(tlin, dst.TG, tx, ty, tz);
+    // source offsets of this tile's chunk (multiples of 16 voxels)
+    const uint32_t ox = shift.cd[0] ? (tx * kTile) / shift.cd[0] * shift.off[0] : 0u;
+    const uint32_t oy = shift.cd[1] ? (ty * kTile) / shift.cd[1] * shift.off[1] : 0u;
+    const uint32_t oz = shift.cd[2] ? (tz * kTile) / shift.cd[2] * shift.off[2] : 0u;
     __shared__ float s_inv255[256];
     // stored-voxel words of the (up to) 8 source tiles under this tile, when
     // the source is level 0; child c = (z & 1) * 4 + (y & 1) * 2 + (x & 1)
@@ -1391,7 +1402,8 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
     fill_inv255(s_inv255);
     if (src.defOcc && tid < 8 * kTileWords) {
         const uint32_t c = (uint32_t)tid / kTileWords, w = (uint32_t)tid % kTileWords;
-        const uint32_t sx = 2 * tx + (c & 1u), sy = 2 * ty + ((c >> 1) & 1u), sz = 2 * tz + (c >> 2);
+        const uint32_t sx = 2 * tx + (ox >> 3) + (c & 1u), sy = 2 * ty + (oy >> 3) + ((c >> 1) & 1u),
+                       sz = 2 * tz + (oz >> 3) + (c >> 2);
         uint32_t d = 0u;
         if (sx < src.TG[0] && sy < src.TG[1] && sz < src.TG[2]) {
             const int32_t ca = src.tileIdx[((uint64_t)sz * src.TG[1] + sy) * src.TG[0] + sx];
@@ -1413,7 +1425,7 @@ __global__ __launch_bounds__(kBlock) void k_mip(LevelMap src, LevelMap dst,
         uint32_t outC = 0u;
         f3 outN = {0.0f, 0.0f, 0.0f};
         if (X < dst.dims[0] && Y < dst.dims[1] && Z < dst.dims[2]) {
-            const uint32_t bx = 2 * X, by = 2 * Y, bz = 2 * Z;
+            const uint32_t bx = 2 * X + ox, by = 2 * Y + oy, bz = 2 * Z + oz;
             const uint32_t x1 = min(bx + 1, src.dims[0] - 1), y1 = min(by + 1, src.dims[1] - 1),
                            z1 = min(bz + 1, src.dims[2] - 1);
             const uint64_t st = ((uint64_t)(bz >> 3) * src.TG[1] + (by >> 3)) * src.TG[0] + (bx >> 3);
@@ -2932,9 +2944,33 @@ void launch_fill(hipStream_t st, const VoxParams& P, int g, int init, uint64_t n
 void launch_mip(hipStream_t st, const LevelMap& src, const LevelMap& dst, uint64_t nActive,
                 const int32_t* listDst, const uint32_t* colSrc, const float* nrmSrc,
                 uint32_t* colDst, uint32_t* qnDst, float* nrmDst) {
+    launch_mip_shifted(st, src, dst, MipShift{}, nActive, listDst, colSrc, nrmSrc, colDst, qnDst,
+                       nrmDst);
+}
+void launch_mip_shifted(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                        const MipShift& shift, uint64_t nActive, const int32_t* listDst,
+                        const uint32_t* colSrc, const float* nrmSrc, uint32_t* colDst,
+                        uint32_t* qnDst, float* nrmDst) {
     if (nActive == 0) return;
-    k_mip<<<(unsigned)nActive, kBlock, 0, st>>>(src, dst, listDst, colSrc, nrmSrc, colDst, qnDst,
-                                                nrmDst);
+    k_mip<<<(unsigned)nActive, kBlock, 0, st>>>(src, dst, shift, listDst, colSrc, nrmSrc, colDst,
+                                                qnDst, nrmDst);
+}
+bool mip_shift_for(const LevelMap& src, const LevelMap& dst, const ChunkGeom& cg, int ks, int kd,
+                   MipShift& out) {
+    out = MipShift{};
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t Cd = cg.C[a] >> kd, Cs = cg.C[a] >> ks;
+        if (Cd == 0 || Cd % kTile != 0 || Cs < 2 * Cd) return false;
+        const uint32_t off = Cs - 2 * Cd;
+        if (off % (2 * kTile) != 0) return false;
+        if (dst.dims[a] == 0) return false;
+        // the last voxel's second source (the largest) below the source's end
+        const uint64_t X = dst.dims[a] - 1, i = X / Cd, x = X - i * Cd;
+        if (i * Cs + 2 * x + 1 >= src.dims[a]) return false;
+        out.cd[a] = Cd;
+        out.off[a] = off;
+    }
+    return true;
 }
 void launch_node_sample_all(hipStream_t st, const SampleDepths& S) {
     if (S.L <= 0 || S.start[S.L] == 0) return;
