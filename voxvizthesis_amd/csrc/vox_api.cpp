@@ -851,7 +851,12 @@ int gen_front(vox_ctx* h) {
         if (l.aliasOf >= 0) continue;
         if (k > 0) {
             LevelView s = view(h, srcOf(k));
-            if (h->multi)
+            MipShift shift;
+            if (h->multi && mip_shift_for(s.map, l.map, chunkGeom(h), srcOf(k), k, shift))
+                h->run("mark", [&] {
+                    launch_mark_children_shifted(st, s.map, l.map, shift, l.mark.as<uint32_t>());
+                });
+            else if (h->multi)
                 h->run("mark", [&] {
                     launch_chunk_mark(st, s.map, l.map, chunkGeom(h), srcOf(k), k,
                                       l.mark.as<uint32_t>());

@@ -366,6 +366,8 @@ struct MipShift {
 };
 bool mip_shift_for(const LevelMap& src, const LevelMap& dst, const ChunkGeom& cg, int ks, int kd,
                    MipShift& out);
+void launch_mark_children_shifted(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                                  const MipShift& shift, uint32_t* markDst);
 void launch_mip_shifted(hipStream_t st, const LevelMap& src, const LevelMap& dst,
                         const MipShift& shift, uint64_t nActive, const int32_t* listDst,
                         const uint32_t* colSrc, const float* nrmSrc, uint32_t* colDst,

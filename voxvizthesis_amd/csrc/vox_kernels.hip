@@ -455,16 +455,22 @@ __global__ void k_compact(uint64_t n, const uint32_t* __restrict__ mark,
 }
 
 // Level-k tile (a,b,c) reads source voxels [16a, 16a+16), i.e. source tiles
-// 2a and 2a+1 on each axis.
-__global__ void k_mark_children(LevelMap src, LevelMap dst, uint32_t* __restrict__ markDst) {
+// 2a and 2a+1 on each axis, shifted by the tile's chunk offset (MipShift,
+// whole tile pairs) with several chunks.
+__global__ void k_mark_children(LevelMap src, LevelMap dst, MipShift shift,
+                                uint32_t* __restrict__ markDst) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= dst.numTiles) return;
     uint32_t px, py, pz;
     tile_xyz(t, dst.TG, px, py, pz);
+    const uint32_t ox = shift.cd[0] ? (px * kTile) / shift.cd[0] * shift.off[0] / kTile : 0u;
+    const uint32_t oy = shift.cd[1] ? (py * kTile) / shift.cd[1] * shift.off[1] / kTile : 0u;
+    const uint32_t oz = shift.cd[2] ? (pz * kTile) / shift.cd[2] * shift.off[2] / kTile : 0u;
     uint32_t m = 0u;
 #pragma unroll
     for (uint32_t c = 0; c < 8u; ++c) {
-        const uint32_t x = 2 * px + (c & 1u), y = 2 * py + ((c >> 1) & 1u), z = 2 * pz + (c >> 2);
+        const uint32_t x = 2 * px + ox + (c & 1u), y = 2 * py + oy + ((c >> 1) & 1u),
+                       z = 2 * pz + oz + (c >> 2);
         if (x < src.TG[0] && y < src.TG[1] && z < src.TG[2] &&
             src.tileIdx[((uint64_t)z * src.TG[1] + y) * src.TG[0] + x] >= 0)
             m = 1u;
@@ -2871,8 +2877,12 @@ void launch_compact(hipStream_t st, uint64_t n, const uint32_t* mark, const uint
 }
 void launch_mark_children(hipStream_t st, const LevelMap& src, const LevelMap& dst,
                           uint32_t* markDst) {
+    launch_mark_children_shifted(st, src, dst, MipShift{}, markDst);
+}
+void launch_mark_children_shifted(hipStream_t st, const LevelMap& src, const LevelMap& dst,
+                                  const MipShift& shift, uint32_t* markDst) {
     if (dst.numTiles == 0) return;
-    k_mark_children<<<grid_for(dst.numTiles, kBlock), kBlock, 0, st>>>(src, dst, markDst);
+    k_mark_children<<<grid_for(dst.numTiles, kBlock), kBlock, 0, st>>>(src, dst, shift, markDst);
 }
 void launch_voxelize(hipStream_t st, const VoxParams& P, const GroupArgs& G, int g, int init,
                      uint64_t nActive, const int32_t* list, const float* verts,
