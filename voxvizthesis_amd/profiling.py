@@ -1,6 +1,8 @@
 """Summaries of rocprofv3 output for the voxelizer workloads.
 
     python -m voxvizthesis_amd.profiling stats <kt_kernel_stats.csv>
+    python -m voxvizthesis_amd.profiling trace <kt_kernel_trace.csv> \
+        [--after-last KERNEL] [--per-step KERNEL]
     python -m voxvizthesis_amd.profiling traffic <fetch.csv> <write.csv> \
         --known-fetch BYTES --known-write BYTES [--json out.json]
 
@@ -21,9 +23,14 @@ import argparse
 import collections
 import csv
 import json
+import re
 
 
 def kname(s: str) -> str:
+    # rocPRIM's sort kernels: the part of the template that names the pass
+    m = re.search(r"wrapped_(\w+?)_config", s)
+    if s.startswith("void rocprim") and m:
+        return "rocprim " + m.group(1)
     for prefix in ("voxmi::(anonymous namespace)::", "(anonymous namespace)::"):
         if s.startswith(prefix):
             s = s[len(prefix):]
@@ -82,12 +89,44 @@ def traffic(fetch_csv, write_csv, known_fetch, known_write, steps=1, known_copy=
     return res
 
 
+def trace_stats(path: str, after_last: str = "", per_step: str = "") -> str:
+    """Markdown table of a kernel-trace CSV: the dispatches after the last
+    dispatch of after_last (a kernel that ends one phase of a run), per
+    kernel calls, average and, with per_step (a kernel dispatched once per
+    step), the time per step and its share."""
+    rows = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), kname(r["Kernel_Name"]))
+            for r in csv.DictReader(open(path))]
+    rows.sort()
+    if after_last:
+        last = max((i for i, r in enumerate(rows) if r[2] == after_last), default=-1)
+        rows = rows[last + 1:]
+    tot = collections.defaultdict(float)
+    calls = collections.Counter()
+    for s0, e0, n in rows:
+        tot[n] += (e0 - s0) / 1e3
+        calls[n] += 1
+    steps = calls[per_step] if per_step else 0
+    allus = sum(tot.values())
+    out = ["| kernel | calls | avg us | us per step | share % |", "|---|---|---|---|---|"]
+    for n in sorted(tot, key=lambda k: -tot[k]):
+        ps = f"{tot[n] / steps:.1f}" if steps else ""
+        out.append(f"| {n} | {calls[n]} | {tot[n] / calls[n]:.1f} | {ps} | "
+                   f"{100.0 * tot[n] / allus:.1f} |")
+    if steps:
+        out.append(f"| total ({steps} steps) | | | {allus / steps:.1f} | 100.0 |")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     a = sub.add_parser("stats")
     a.add_argument("csv")
     a.add_argument("--prefix", default="")
+    t = sub.add_parser("trace")
+    t.add_argument("csv")
+    t.add_argument("--after-last", default="")
+    t.add_argument("--per-step", default="")
     b = sub.add_parser("traffic")
     b.add_argument("fetch")
     b.add_argument("write")
@@ -98,6 +137,8 @@ def main():
     args = ap.parse_args()
     if args.cmd == "stats":
         print(stats(args.csv, args.prefix))
+    elif args.cmd == "trace":
+        print(trace_stats(args.csv, args.after_last, args.per_step))
     else:
         res = traffic(args.fetch, args.write, args.known_fetch, args.known_write,
                       known_copy=args.known_copy)
